@@ -1,0 +1,130 @@
+"""Benchmark: D3Q19 TRT (Λ = 3/16) triply periodic Taylor–Green vortex, 256³, FP32 storage.
+
+One "step" = one single-pass stream–collide launch over the whole 256³ lattice (populations
+resident in HBM).  Prints ONE JSON line with MLUPS, the HBM roofline of the step kernel
+(algorithmic 152 B per cell update in FP32 = 19 reads + 19 writes of 4 B) and a CPU
+baseline (the FP32 oracle restatement, OpenMP on the host cores, bounded sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "krakenn.jl_amd"))
+
+HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E, published
+BYTES_PER_POP = {"f64": 8, "f32": 4, "f32s": 4, "f16s": 2}
+
+
+def cpu_baseline(N, nu, storage, budget_s=12.0, max_steps=30):
+    """Oracle D3Q19 TRT pull step (dsl/bricks_3d.jl composition) on the host, timed."""
+    import numpy as np
+
+    sys.path.insert(0, ROOT)
+    from oracle import pyoracle as po
+
+    dt = np.float64 if storage == "f64" else np.float32
+    import kraken_mi355x as kmi
+
+    rho, ux, uy, uz = kmi.taylor_green_3d_fields(N, 0.05)
+    f = po.equilibrium(rho, ux, uy, uz, dtype=dt)
+    o = np.empty_like(f, order="F")
+    sp, sm = kmi.trt_rates(nu)
+    po.call("fused_trt_3d", o, f, N, N, N, 1, 1, 1, sp, sm, dtype=dt)  # warm-up (page-in)
+    steps, t0 = 0, time.perf_counter()
+    while steps < max_steps:
+        po.call("fused_trt_3d", o, f, N, N, N, 1, 1, 1, sp, sm, dtype=dt)
+        f, o = o, f
+        steps += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    el = time.perf_counter() - t0
+    return {
+        "value": round(N ** 3 * steps / el / 1e6, 2),
+        "unit": "MLUPS",
+        "cores": po.threads(),
+        "kind": "port",
+        "sample": f"D3Q19 TRT {storage.upper()} {N}^3 periodic Taylor-Green, {steps} steps "
+                  f"({el:.1f} s) of the C++ oracle fused pull+TRT, OpenMP",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=64)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--storage", default="f32", choices=sorted(BYTES_PER_POP))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--variants", action="store_true", help="also time f64 and f16s storage")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import kraken_mi355x as kmi
+
+    N, nu, U0 = args.n, 0.01, 0.05
+    fields = kmi.taylor_green_3d_fields(N, U0)
+
+    def run(storage, steps, warmup):
+        desc = kmi.taylor_green_3d_desc(N, nu, storage)
+        with kmi.Lattice(desc) as lat:
+            lat.init_equilibrium(*fields)
+            lat.step(warmup)
+            lat.backend.synchronize()
+            t0 = time.perf_counter()
+            ms = lat.step_timed(steps)
+            lat.backend.synchronize()
+            wall = time.perf_counter() - t0
+            mass = lat.reduce("mass")
+        return ms, wall, mass
+
+    ms, wall, mass = run(args.storage, args.steps, args.warmup)
+    cells = N ** 3
+    mlups = cells * args.steps / wall / 1e6
+    per_launch_s = ms / 1e3 / args.steps
+    alg_bytes = cells * 19 * 2 * BYTES_PER_POP[args.storage]
+    achieved = alg_bytes / per_launch_s / 1e9
+    out = {
+        "metric": "MLUPS (D3Q19 TRT 256^3 Taylor-Green, FP32 storage) and % of HBM roofline",
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.storage in ("f32", "f32s", "f16s") else "f64",
+        "data": "synthetic (analytic Taylor-Green initial field, no dataset)",
+        "config": {"workload": f"D3Q19 TRT Lambda=3/16 triply periodic Taylor-Green {N}^3, "
+                               f"U0={U0}, nu={nu}, storage={args.storage}",
+                   "cells": cells, "storage": args.storage, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "bytes_per_launch": alg_bytes, "avg_launch_ms": round(per_launch_s * 1e3, 5)},
+        "mass_after": mass,
+    }
+    if args.variants:
+        var = {}
+        for st in ("f64", "f16s"):
+            if st == args.storage:
+                continue
+            m2, w2, _ = run(st, max(args.steps // 2, 64), args.warmup)
+            var[st] = {"MLUPS": round(cells * max(args.steps // 2, 64) / w2 / 1e6, 1),
+                       "frac": round(cells * 38 * BYTES_PER_POP[st] / (m2 / 1e3 / max(args.steps // 2, 64)) / 1e9
+                                     / HBM_PEAK_GBPS, 4)}
+        out["variants"] = var
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(N, nu, args.storage)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

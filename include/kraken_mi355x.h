@@ -1,0 +1,219 @@
+/*
+ * kraken_mi355x.h — C ABI of the MI355X lattice-Boltzmann stream–collide library.
+ *
+ * This is the drop-in boundary for Kraken.jl's fused stream–collide hot path.  A host
+ * (Julia `ccall`, Python `ctypes`, C) hands over host arrays in the reference's own
+ * layout — column-major `f[i, j, (k,) q]`, x fastest, one contiguous plane per
+ * velocity `q`, q 1-based in Julia and 0-based here — and the library keeps the
+ * populations resident in HBM between calls.
+ *
+ * Reference interfaces replaced (paths relative to the Kraken.jl source tree):
+ *   - backend value / dispatch:   src/simulation_runner.jl:110-160, 235-443
+ *                                  (`_backend_from_krk_symbol`, `run_simulation`)
+ *   - state allocation / init:     src/drivers/basic.jl:51-101 (`initialize_2d/3d`)
+ *   - fused single-pass step:      src/kernels/fused_bgk_2d.jl:67, fused_trt_2d.jl:217,
+ *                                  fused_thermal_2d.jl:251, dsl/bricks_3d.jl:64-229
+ *   - unfused step chains:         src/drivers/basic.jl:121-136 (`run_cavity_2d`),
+ *                                  src/drivers/thermal.jl:100-133 (`run_rayleigh_benard_2d`),
+ *                                  src/kernels/collide_mrt_2d.jl:118 (`collide_mrt_2d!`)
+ *   - moments:                     src/kernels/macroscopic.jl:5-80,
+ *                                  src/kernels/thermal_2d.jl:113-138
+ *   - residual / Nusselt:          src/drivers/gnf_periodic_steady.jl:186-196,
+ *                                  src/drivers/thermal.jl:225-234
+ *
+ * Conventions
+ *   - Every call returns a kmi_status (0 = KMI_OK).  On failure, kmi_last_error(ctx)
+ *     (or kmi_last_error(NULL) for calls that have no context yet) names the cause.
+ *     Nothing ever falls back to a CPU path: a missing device is an error.
+ *   - The library owns device memory; the caller owns every host buffer it passes.
+ *   - One context = one device + one HIP stream.  Calls on one context are serialised
+ *     by the caller; distinct contexts are independent.
+ */
+#ifndef KRAKEN_MI355X_H
+#define KRAKEN_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KMI_ABI_VERSION 1
+
+typedef enum {
+  KMI_OK = 0,
+  KMI_ERR_ARG = 1,         /* invalid argument (shape, enum, NULL pointer) */
+  KMI_ERR_DEVICE = 2,      /* HIP runtime error or no usable device */
+  KMI_ERR_STATE = 3,       /* call not valid in the object's current state */
+  KMI_ERR_NOMEM = 4,       /* device allocation failed */
+  KMI_ERR_UNSUPPORTED = 5  /* combination the reference does not define */
+} kmi_status;
+
+typedef enum { KMI_D2Q9 = 0, KMI_D3Q19 = 1 } kmi_stencil;
+
+/* Population storage in HBM.  Arithmetic is FP64 for F64 and FP32 otherwise.
+ * F32S / F16S store the deviation f_q - w_q (thermal: g_q - w_q T_ref); F16S scales
+ * it by 2^15 before rounding to IEEE binary16. */
+typedef enum {
+  KMI_STORE_F64 = 0,
+  KMI_STORE_F32 = 1,
+  KMI_STORE_F32S = 2,
+  KMI_STORE_F16S = 3
+} kmi_storage;
+
+typedef enum { KMI_BGK = 0, KMI_TRT = 1, KMI_MRT = 2 } kmi_collision;
+
+/* Face treatment of the flow populations.
+ *   PERIODIC        — wrap; both faces of the axis must be periodic.
+ *   WALL            — halfway bounce-back folded into the pull (collide_stream_2d.jl:75-101).
+ *   MOVING_WALL     — halfway bounce-back with the Ladd momentum term
+ *                     f_q = f*_opp(q) + 6 w_q rho_w (c_q . u_w)   (stream_periodic_3d.jl:106-166).
+ *   ZOUHE_VELOCITY  — bounce-back pull, then the Zou–He velocity closure of the
+ *                     reference on that face (boundary_2d.jl:13-44, 135-185).
+ *   ZOUHE_PRESSURE  — bounce-back pull, then the Zou–He pressure closure
+ *                     (boundary_2d.jl:206-299). */
+typedef enum {
+  KMI_FACE_PERIODIC = 0,
+  KMI_FACE_WALL = 1,
+  KMI_FACE_MOVING_WALL = 2,
+  KMI_FACE_ZOUHE_VELOCITY = 3,
+  KMI_FACE_ZOUHE_PRESSURE = 4
+} kmi_face_kind;
+
+/* Faces: x-, x+, y-, y+, z-, z+ (the reference's west/east/south/north/bottom/top). */
+typedef enum {
+  KMI_WEST = 0,
+  KMI_EAST = 1,
+  KMI_SOUTH = 2,
+  KMI_NORTH = 3,
+  KMI_BOTTOM = 4,
+  KMI_TOP = 5
+} kmi_face_id;
+
+/* Thermal (g) populations on a non-periodic face: adiabatic = bounce-back,
+ * Dirichlet = halfway anti-bounce-back g_q = -g*_opp(q) + 2 w_q T_w with the
+ * reference's corner rule (thermal_2d.jl:140-172, 371-417). */
+typedef enum { KMI_THERMAL_ADIABATIC = 0, KMI_THERMAL_DIRICHLET = 1 } kmi_thermal_face;
+
+/* Velocity seen by the thermal collision.
+ *   SPLIT — the reference's unfused Rayleigh–Bénard chain: u from the post-stream
+ *           populations without the Guo half-force shift (thermal.jl:123-128).
+ *   FUSED — the reference's fused natural-convection kernel: u with the F/2 shift
+ *           (fused_thermal_2d.jl:61-72, 197-200). */
+typedef enum { KMI_COUPLING_SPLIT = 0, KMI_COUPLING_FUSED = 1 } kmi_thermal_coupling;
+
+typedef struct {
+  int32_t kind;     /* kmi_face_kind */
+  int32_t thermal;  /* kmi_thermal_face (used when desc.thermal != 0) */
+  double u[3];      /* wall velocity (MOVING_WALL) or imposed velocity (ZOUHE_VELOCITY) */
+  double rho;       /* wall density (MOVING_WALL) or imposed density (ZOUHE_PRESSURE) */
+  double T;         /* wall temperature (thermal DIRICHLET) */
+} kmi_face;
+
+typedef struct {
+  int32_t stencil;    /* kmi_stencil */
+  int32_t storage;    /* kmi_storage */
+  int32_t collision;  /* kmi_collision; MRT is D2Q9 only */
+  int32_t thermal;    /* 0 / 1: D2Q9 double-distribution temperature field g */
+  int64_t nx, ny, nz; /* nz = 1 for D2Q9 */
+  double omega;             /* BGK rate  1/(3 nu + 1/2)                     (basic.jl:30) */
+  double s_plus, s_minus;   /* TRT rates, see kmi_trt_rates              (fused_trt_2d.jl:201) */
+  double s_e, s_eps, s_q, s_nu; /* MRT rates                            (collide_mrt_2d.jl:118) */
+  double omega_T;           /* thermal BGK rate 1/(3 alpha + 1/2)           (thermal.jl:65) */
+  double beta_g;            /* Boussinesq  F_y = beta_g (T - T_ref)         (thermal_2d.jl:435) */
+  double T_ref;             /* buoyancy reference temperature (also the g storage shift) */
+  int32_t coupling;         /* kmi_thermal_coupling */
+  int32_t n_bc_order;       /* 0: default order W,E,S,N,B,T; else the first n entries of */
+  int32_t bc_order[6];      /*   bc_order give the order Zou–He closures are applied in   */
+  kmi_face faces[6];        /* indexed by kmi_face_id; z faces ignored for D2Q9           */
+} kmi_desc;
+
+typedef struct kmi_ctx kmi_ctx;
+typedef struct kmi_lattice kmi_lattice;
+
+/* Population fields of a lattice. */
+typedef enum { KMI_FIELD_F = 0, KMI_FIELD_G = 1 } kmi_field;
+/* Element type of a host buffer. */
+typedef enum { KMI_HOST_F64 = 0, KMI_HOST_F32 = 1 } kmi_host_dtype;
+
+/* Reductions, all accumulated in FP64 with compensated (Neumaier) sums.
+ *   MASS            out[0] = sum_cells rho
+ *   MOMENTUM        out[0..2] = sum_cells rho u
+ *   KINETIC_ENERGY  out[0] = sum_cells 1/2 rho |u|^2
+ *   THERMAL         out[0] = sum_cells T
+ *   NUSSELT         out[0] = hot-wall Nusselt number; requires two opposite Dirichlet
+ *                   faces, the hot one named by `arg` (a kmi_face_id) — estimator of
+ *                   thermal.jl:225-234 (second-order one-sided gradient, rows 2..N-1;
+ *                   all rows when the wall-parallel axis is periodic)
+ *   MAX_SPEED       out[0] = max_cells |u|  */
+typedef enum {
+  KMI_RED_MASS = 0,
+  KMI_RED_MOMENTUM = 1,
+  KMI_RED_KINETIC_ENERGY = 2,
+  KMI_RED_THERMAL = 3,
+  KMI_RED_NUSSELT = 4,
+  KMI_RED_MAX_SPEED = 5
+} kmi_reduction;
+
+/* ---- library / context ---------------------------------------------------------- */
+int kmi_abi_version(void);
+/* Number of visible HIP devices (0 is a valid answer; no error). */
+int kmi_device_count(int* n);
+/* Human-readable description of the last error on `ctx` (NULL: the calling thread's
+ * last context-free error).  Never NULL; valid until the next call on the same ctx. */
+const char* kmi_last_error(const kmi_ctx* ctx);
+int kmi_ctx_create(int device, kmi_ctx** out);
+int kmi_ctx_destroy(kmi_ctx* ctx);
+int kmi_ctx_synchronize(kmi_ctx* ctx);
+
+/* TRT rates from viscosity and magic parameter: s_plus = 1/(3 nu + 1/2),
+ * s_minus = 1/(Lambda/(3 nu) + 1/2)   (src/kernels/fused_trt_2d.jl:201-205). */
+void kmi_trt_rates(double nu, double Lambda, double* s_plus, double* s_minus);
+/* Fill `desc` with the defaults of a D2Q9/D3Q19 lattice: all faces WALL, BGK from nu. */
+void kmi_desc_init(kmi_desc* desc, int32_t stencil, int64_t nx, int64_t ny, int64_t nz,
+                   double nu);
+/* Check a descriptor without a device: KMI_OK, or the status kmi_lattice_create would
+ * return, with the reason in kmi_last_error(NULL). */
+int kmi_desc_validate(const kmi_desc* desc);
+/* sizeof(kmi_desc) as compiled into the library (host bindings check their layout). */
+size_t kmi_desc_sizeof(void);
+
+/* ---- single lattice (configs: cavity, Taylor–Green, Rayleigh–Bénard) ------------- */
+/* Allocates f (and g) on the device and sets them to the rest equilibrium f_q = w_q
+ * (g_q = w_q T_ref), as initialize_2d/3d (basic.jl:51-101). */
+int kmi_lattice_create(kmi_ctx* ctx, const kmi_desc* desc, kmi_lattice** out);
+int kmi_lattice_destroy(kmi_lattice* lat);
+/* Solid mask, one byte per cell in [i, j, k] order (nonzero = solid); NULL clears it. */
+int kmi_set_mask(kmi_lattice* lat, const uint8_t* is_solid);
+/* Whole population field in the reference layout [i, j, (k,) q], nx*ny*nz*Q values. */
+int kmi_upload(kmi_lattice* lat, int32_t field, const void* host, int32_t host_dtype);
+int kmi_download(kmi_lattice* lat, int32_t field, void* host, int32_t host_dtype);
+/* f_q = feq_q(rho, u) per cell (equilibrium of basic.jl / d2q9.jl:60-67); any of
+ * ux/uy/uz may be NULL (zero), rho may be NULL (one).  Host arrays of nx*ny*nz. */
+int kmi_init_equilibrium(kmi_lattice* lat, const double* rho, const double* ux,
+                         const double* uy, const double* uz);
+/* g_q = w_q T per cell (thermal.jl:74-97). */
+int kmi_init_temperature(kmi_lattice* lat, const double* T);
+/* Advance `nsteps` single-pass steps (stream + boundaries + collide in one launch each). */
+int kmi_step(kmi_lattice* lat, int64_t nsteps);
+/* As kmi_step, and also the relative one-step velocity change of the last step,
+ * r_u = ||u^k - u^(k-1)||_2 / ||u^k||_2 over fluid cells (gnf_periodic_steady.jl:186-196). */
+int kmi_step_residual(kmi_lattice* lat, int64_t nsteps, double* r_u);
+/* Device-timed kmi_step: elapsed milliseconds between HIP events on the lattice stream. */
+int kmi_step_timed(kmi_lattice* lat, int64_t nsteps, double* ms);
+/* Moments of the stored (post-collision) populations, as compute_macroscopic_2d/3d and
+ * compute_temperature_2d (macroscopic.jl:5-80, thermal_2d.jl:113-138).  Any pointer may
+ * be NULL.  Host arrays of nx*ny*nz doubles. */
+int kmi_moments(kmi_lattice* lat, double* rho, double* ux, double* uy, double* uz,
+                double* T);
+int kmi_reduce(kmi_lattice* lat, int32_t kind, int32_t arg, double* out);
+/* Steps taken since creation (or since the last upload/init). */
+int kmi_lattice_steps(const kmi_lattice* lat, int64_t* steps);
+/* Bytes of device memory held by the lattice. */
+int kmi_lattice_device_bytes(const kmi_lattice* lat, int64_t* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KRAKEN_MI355X_H */

@@ -1,0 +1,753 @@
+// kmi_api.hip — host implementation of the kraken_mi355x C ABI.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "kmi_common.h"
+
+namespace kmi {
+hipError_t launch_step(int q, int st, int coll, bool th, bool bnd, const StepArgs& a, dim3 g,
+                       dim3 b, hipStream_t s);
+hipError_t launch_to_storage(int Q, int st, const double* src, void* dst, long long ncells,
+                             long long plane, double shift, hipStream_t s);
+hipError_t launch_from_storage(int Q, int st, const void* src, double* dst, long long ncells,
+                               long long plane, double shift, hipStream_t s);
+hipError_t launch_init_eq(int Q, int st, void* dst, const double* rho, const double* ux,
+                          const double* uy, const double* uz, long long ncells, long long plane,
+                          hipStream_t s);
+hipError_t launch_init_temp(int st, void* dst, const double* T, long long ncells, long long plane,
+                            double T0, hipStream_t s);
+hipError_t launch_moments(int Q, int st, const void* f, const void* g, double* rho, double* ux,
+                          double* uy, double* uz, double* T, long long ncells, long long plane,
+                          double T0, hipStream_t s);
+int reduce_groups(long long ncells);
+hipError_t launch_reduce_cells(int Q, int st, const void* f, const void* g, const uint8_t* mask,
+                               double* uprev, long long ncells, long long plane, double T0,
+                               int kind, double* partials, double* out, hipStream_t s);
+hipError_t launch_nusselt(int st, const void* g, int nx, int ny, long long plane, double T0,
+                          int face, double Th, double dT, int lo, int hi, double* partials,
+                          double* out, hipStream_t s);
+}  // namespace kmi
+
+using namespace kmi;
+
+struct kmi_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  double* partials = nullptr;  // 2048 groups × 3 values
+  double* out = nullptr;       // 8 values
+};
+
+static constexpr int kGraphSteps = 32;  // steps per captured graph (even: parity returns)
+
+struct kmi_lattice {
+  kmi_ctx* ctx = nullptr;
+  kmi_desc desc{};
+  int Q = 9;
+  long long ncells = 0, plane = 0;
+  size_t esize = 8;
+  void* f[2] = {nullptr, nullptr};
+  void* g[2] = {nullptr, nullptr};
+  uint8_t* mask = nullptr;
+  double* uprev = nullptr;
+  int cur = 0;
+  int64_t steps = 0;
+  StepArgs args{};
+  bool bnd = true;
+  dim3 grid, block;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+static thread_local std::string g_err = "no error";
+
+static int fail(kmi_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  else g_err = msg;
+  return code;
+}
+
+static int hip_fail(kmi_ctx* ctx, hipError_t e, const char* what) {
+  return fail(ctx, KMI_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define KMI_HIP(ctx, call)                                \
+  do {                                                    \
+    hipError_t _e = (call);                               \
+    if (_e != hipSuccess) return hip_fail(ctx, _e, #call); \
+  } while (0)
+
+static const char* kFaceName[6] = {"west", "east", "south", "north", "bottom", "top"};
+
+extern "C" {
+
+int kmi_abi_version(void) { return KMI_ABI_VERSION; }
+
+int kmi_device_count(int* n) {
+  if (!n) return fail(nullptr, KMI_ERR_ARG, "kmi_device_count: n is NULL");
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+  *n = c;
+  return KMI_OK;
+}
+
+const char* kmi_last_error(const kmi_ctx* ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
+
+int kmi_ctx_create(int device, kmi_ctx** out) {
+  if (!out) return fail(nullptr, KMI_ERR_ARG, "kmi_ctx_create: out is NULL");
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+    return fail(nullptr, KMI_ERR_DEVICE,
+                "kmi_ctx_create: no HIP device is visible; the MI355X backend is unavailable");
+  if (device < 0 || device >= n)
+    return fail(nullptr, KMI_ERR_ARG, "kmi_ctx_create: device " + std::to_string(device) +
+                                          " out of range (" + std::to_string(n) + " visible)");
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) return hip_fail(nullptr, e, "hipSetDevice");
+  kmi_ctx* c = new (std::nothrow) kmi_ctx();
+  if (!c) return fail(nullptr, KMI_ERR_NOMEM, "kmi_ctx_create: out of host memory");
+  c->device = device;
+  if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipMalloc(&c->partials, 2048 * 3 * sizeof(double))) != hipSuccess ||
+      (e = hipMalloc(&c->out, 8 * sizeof(double))) != hipSuccess) {
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->partials) (void)hipFree(c->partials);
+    delete c;
+    return hip_fail(nullptr, e, "kmi_ctx_create");
+  }
+  c->err = "no error";
+  *out = c;
+  return KMI_OK;
+}
+
+int kmi_ctx_destroy(kmi_ctx* ctx) {
+  if (!ctx) return KMI_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipFree(ctx->partials);
+  (void)hipFree(ctx->out);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return KMI_OK;
+}
+
+int kmi_ctx_synchronize(kmi_ctx* ctx) {
+  if (!ctx) return fail(nullptr, KMI_ERR_ARG, "kmi_ctx_synchronize: ctx is NULL");
+  KMI_HIP(ctx, hipSetDevice(ctx->device));
+  KMI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return KMI_OK;
+}
+
+void kmi_trt_rates(double nu, double Lambda, double* s_plus, double* s_minus) {
+  // fused_trt_2d.jl:201-205
+  if (s_plus) *s_plus = 1.0 / (3 * nu + 0.5);
+  if (s_minus) *s_minus = 1.0 / (Lambda / (3 * nu) + 0.5);
+}
+
+void kmi_desc_init(kmi_desc* d, int32_t stencil, int64_t nx, int64_t ny, int64_t nz, double nu) {
+  if (!d) return;
+  std::memset(d, 0, sizeof(*d));
+  d->stencil = stencil;
+  d->storage = KMI_STORE_F64;
+  d->collision = KMI_BGK;
+  d->nx = nx;
+  d->ny = ny;
+  d->nz = stencil == KMI_D2Q9 ? 1 : nz;
+  d->omega = 1.0 / (3.0 * nu + 0.5);  // omega(config), basic.jl:30
+  kmi_trt_rates(nu, 3.0 / 16.0, &d->s_plus, &d->s_minus);
+  d->s_e = 1.4;  // collide_mrt_2d! defaults, collide_mrt_2d.jl:118
+  d->s_eps = 1.4;
+  d->s_q = 1.2;
+  d->s_nu = 1.0 / (3.0 * nu + 0.5);
+  d->omega_T = d->omega;
+  d->coupling = KMI_COUPLING_SPLIT;
+  for (int f = 0; f < 6; ++f) {
+    d->faces[f].kind = KMI_FACE_WALL;
+    d->faces[f].thermal = KMI_THERMAL_ADIABATIC;
+    d->faces[f].rho = 1.0;
+  }
+}
+
+}  // extern "C"
+
+// ---- validation -----------------------------------------------------------------------
+static int validate(const kmi_desc& d, std::string& why, int& code) {
+  code = KMI_ERR_ARG;
+  if (d.stencil != KMI_D2Q9 && d.stencil != KMI_D3Q19) return why = "unknown stencil", 1;
+  if (d.storage < KMI_STORE_F64 || d.storage > KMI_STORE_F16S) return why = "unknown storage", 1;
+  if (d.collision < KMI_BGK || d.collision > KMI_MRT) return why = "unknown collision", 1;
+  const int64_t nz = d.stencil == KMI_D2Q9 ? 1 : d.nz;
+  if (d.nx < 1 || d.ny < 1 || nz < 1) return why = "grid sizes must be positive", 1;
+  if (d.ny > 65535 || nz > 65535) return why = "ny and nz must be at most 65535", 1;
+  if (double(d.nx) * double(d.ny) * double(nz) * (d.stencil == KMI_D2Q9 ? 9 : 19) >= 2147483647.0)
+    return code = KMI_ERR_UNSUPPORTED, why = "more than 2^31 populations per field", 1;
+  const int nf = d.stencil == KMI_D2Q9 ? 4 : 6;
+  for (int f = 0; f < nf; ++f)
+    if (d.faces[f].kind < KMI_FACE_PERIODIC || d.faces[f].kind > KMI_FACE_ZOUHE_PRESSURE)
+      return why = std::string("unknown face kind on ") + kFaceName[f], 1;
+  for (int a = 0; a < nf / 2; ++a) {
+    const bool p0 = d.faces[2 * a].kind == KMI_FACE_PERIODIC;
+    const bool p1 = d.faces[2 * a + 1].kind == KMI_FACE_PERIODIC;
+    if (p0 != p1)
+      return why = std::string("periodic face ") + kFaceName[p0 ? 2 * a : 2 * a + 1] +
+                   " needs a periodic opposite face",
+             1;
+  }
+  if (d.collision == KMI_MRT && d.stencil != KMI_D2Q9)
+    return code = KMI_ERR_UNSUPPORTED, why = "MRT is defined for D2Q9 only", 1;
+  if (d.thermal && d.stencil != KMI_D2Q9)
+    return code = KMI_ERR_UNSUPPORTED, why = "the thermal DDF is D2Q9 only", 1;
+  if (d.thermal && d.collision != KMI_BGK)
+    return code = KMI_ERR_UNSUPPORTED,
+           why = "the thermal DDF uses BGK + Guo for the flow (thermal_2d.jl:421)", 1;
+  for (int f = 0; f < nf; ++f) {
+    const kmi_face& F = d.faces[f];
+    if (F.kind == KMI_FACE_ZOUHE_VELOCITY || F.kind == KMI_FACE_ZOUHE_PRESSURE) {
+      if (d.stencil != KMI_D2Q9)
+        return code = KMI_ERR_UNSUPPORTED, why = "Zou-He faces are implemented for D2Q9", 1;
+      if (F.kind == KMI_FACE_ZOUHE_VELOCITY) {
+        if (f == KMI_EAST)
+          return code = KMI_ERR_UNSUPPORTED,
+                 why = "Zou-He velocity on the east face is not defined by the reference "
+                       "(handlers.jl:228-237 support north, south, west)",
+                 1;
+        if (F.u[1] != 0.0 || F.u[2] != 0.0)
+          return code = KMI_ERR_UNSUPPORTED,
+                 why = std::string("Zou-He velocity on ") + kFaceName[f] +
+                       " takes an x velocity only (boundary_2d.jl:13-185)",
+                 1;
+      } else {
+        if (f != KMI_EAST && f != KMI_WEST)
+          return code = KMI_ERR_UNSUPPORTED,
+                 why = std::string("Pressure boundary condition on face ") + kFaceName[f] +
+                       " is not supported (supported: west, east; handlers.jl:256-269)",
+                 1;
+        if (!(F.rho > 0.0)) return why = "Zou-He pressure density must be positive", 1;
+      }
+    }
+    if (d.thermal && F.thermal == KMI_THERMAL_DIRICHLET && F.kind == KMI_FACE_PERIODIC)
+      return why = std::string("Dirichlet temperature on periodic face ") + kFaceName[f], 1;
+  }
+  if (d.n_bc_order < 0 || d.n_bc_order > 6) return why = "n_bc_order must be in 0..6", 1;
+  for (int n = 0; n < d.n_bc_order; ++n)
+    if (d.bc_order[n] < 0 || d.bc_order[n] >= nf) return why = "bc_order names an unknown face", 1;
+  return 0;
+}
+
+static void build_args(kmi_lattice* L) {
+  const kmi_desc& d = L->desc;
+  StepArgs& a = L->args;
+  std::memset(&a, 0, sizeof(a));
+  a.nx = int(d.nx);
+  a.ny = int(d.ny);
+  a.nz = int(d.stencil == KMI_D2Q9 ? 1 : d.nz);
+  a.plane = L->plane;
+  a.per[0] = d.faces[KMI_WEST].kind == KMI_FACE_PERIODIC;
+  a.per[1] = d.faces[KMI_SOUTH].kind == KMI_FACE_PERIODIC;
+  a.per[2] = L->Q == 19 && d.faces[KMI_BOTTOM].kind == KMI_FACE_PERIODIC;
+  const int nf = L->Q == 9 ? 4 : 6;
+  a.any_moving = 0;
+  for (int f = 0; f < nf; ++f) a.any_moving |= d.faces[f].kind == KMI_FACE_MOVING_WALL;
+  int order[6] = {0, 1, 2, 3, 4, 5};
+  int n = nf;
+  if (d.n_bc_order > 0) {
+    n = d.n_bc_order;
+    for (int i = 0; i < n; ++i) order[i] = d.bc_order[i];
+  }
+  a.n_zh = 0;
+  for (int i = 0; i < n; ++i) {
+    const int k = d.faces[order[i]].kind;
+    if (k == KMI_FACE_ZOUHE_VELOCITY || k == KMI_FACE_ZOUHE_PRESSURE) a.zh_face[a.n_zh++] = order[i];
+  }
+  a.omega = d.omega;
+  a.s_plus = d.s_plus;
+  a.s_minus = d.s_minus;
+  a.s_e = d.s_e;
+  a.s_eps = d.s_eps;
+  a.s_q = d.s_q;
+  a.s_nu = d.s_nu;
+  a.omega_T = d.omega_T;
+  a.beta_g = d.beta_g;
+  a.T_ref = d.T_ref;
+  a.coupling = d.coupling;
+  bool dir[4];
+  for (int f = 0; f < 4; ++f)
+    dir[f] = d.thermal && d.faces[f].kind != KMI_FACE_PERIODIC &&
+             d.faces[f].thermal == KMI_THERMAL_DIRICHLET;
+  // Corner link temperature, in the storage arithmetic (_corner_temp, thermal_2d.jl:299-300).
+  auto corner = [&](int fa, int fb) -> double {
+    if (!dir[fb]) return d.faces[fa].T;
+    if (d.storage == KMI_STORE_F32) return double((float(d.faces[fa].T) + float(d.faces[fb].T)) / 2.0f);
+    return (d.faces[fa].T + d.faces[fb].T) / 2.0;
+  };
+  for (int f = 0; f < 6; ++f) {
+    FaceArgs& F = a.face[f];
+    F.kind = d.faces[f].kind;
+    F.dirichlet = f < 4 ? dir[f] : 0;
+    for (int c = 0; c < 3; ++c) F.u[c] = d.faces[f].u[c];
+    F.rho = d.faces[f].rho;
+    F.T = d.faces[f].T;
+    F.Tc[0] = F.Tc[1] = d.faces[f].T;
+    F.skip[0] = F.skip[1] = 0;
+    if (f < 4 && dir[f]) {
+      if (f == KMI_WEST || f == KMI_EAST) {
+        F.Tc[0] = corner(f, KMI_SOUTH);
+        F.Tc[1] = corner(f, KMI_NORTH);
+      } else {
+        F.skip[0] = dir[KMI_WEST];
+        F.skip[1] = dir[KMI_EAST];
+      }
+    }
+  }
+  bool any_nonperiodic = false;
+  for (int f = 0; f < nf; ++f) any_nonperiodic |= d.faces[f].kind != KMI_FACE_PERIODIC;
+  L->bnd = any_nonperiodic || L->mask != nullptr;
+  // Launch shape: one x-row segment of up to 256 cells per work-group; small grids use
+  // 64-wide groups so that the launch still spreads over the 256 CUs.
+  unsigned bx = unsigned(std::min<int64_t>(256, ((d.nx + 63) / 64) * 64));
+  const long long groups = ((d.nx + bx - 1) / bx) * d.ny * a.nz;
+  if (groups < 2048 && bx > 64) bx = 64;
+  L->block = dim3(bx, 1, 1);
+  L->grid = dim3(unsigned((d.nx + bx - 1) / bx), unsigned(d.ny), unsigned(a.nz));
+}
+
+static int launch_one(kmi_lattice* L, hipStream_t s) {
+  StepArgs a = L->args;
+  a.fi = L->f[L->cur];
+  a.fo = L->f[L->cur ^ 1];
+  a.gi = L->g[L->cur];
+  a.go = L->g[L->cur ^ 1];
+  a.mask = L->mask;
+  hipError_t e = launch_step(L->Q, L->desc.storage, L->desc.collision, L->desc.thermal != 0,
+                             L->bnd, a, L->grid, L->block, s);
+  if (e != hipSuccess) return hip_fail(L->ctx, e, "step kernel launch");
+  L->cur ^= 1;
+  return KMI_OK;
+}
+
+static void drop_graphs(kmi_lattice* L) {
+  for (int p = 0; p < 2; ++p)
+    if (L->graph[p]) {
+      (void)hipGraphExecDestroy(L->graph[p]);
+      L->graph[p] = nullptr;
+    }
+}
+
+// kGraphSteps launches captured once per buffer parity and replayed.
+static int run_steps(kmi_lattice* L, int64_t n) {
+  kmi_ctx* c = L->ctx;
+  hipStream_t s = c->stream;
+  while (n >= kGraphSteps) {
+    const int p = L->cur;
+    if (!L->graph[p]) {
+      hipGraph_t gph = nullptr;
+      KMI_HIP(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      for (int k = 0; k < kGraphSteps; ++k) {
+        int rc = launch_one(L, s);
+        if (rc != KMI_OK) {
+          (void)hipStreamEndCapture(s, &gph);
+          if (gph) (void)hipGraphDestroy(gph);
+          return rc;
+        }
+      }
+      KMI_HIP(c, hipStreamEndCapture(s, &gph));
+      hipError_t e = hipGraphInstantiate(&L->graph[p], gph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(gph);
+      if (e != hipSuccess) return hip_fail(c, e, "hipGraphInstantiate");
+      // capture only recorded the launches; parity is back where it started
+    }
+    KMI_HIP(c, hipGraphLaunch(L->graph[p], s));
+    n -= kGraphSteps;
+    L->steps += kGraphSteps;
+  }
+  for (; n > 0; --n) {
+    int rc = launch_one(L, s);
+    if (rc != KMI_OK) return rc;
+    L->steps += 1;
+  }
+  return KMI_OK;
+}
+
+static double storage_shift_f() { return 1.0; }
+
+extern "C" {
+
+int kmi_desc_validate(const kmi_desc* desc) {
+  if (!desc) return fail(nullptr, KMI_ERR_ARG, "kmi_desc_validate: desc is NULL");
+  std::string why;
+  int code;
+  if (validate(*desc, why, code)) return fail(nullptr, code, "kmi_desc_validate: " + why);
+  return KMI_OK;
+}
+
+size_t kmi_desc_sizeof(void) { return sizeof(kmi_desc); }
+
+int kmi_lattice_create(kmi_ctx* ctx, const kmi_desc* desc, kmi_lattice** out) {
+  if (!ctx) return fail(nullptr, KMI_ERR_ARG, "kmi_lattice_create: ctx is NULL");
+  if (!desc || !out) return fail(ctx, KMI_ERR_ARG, "kmi_lattice_create: NULL argument");
+  *out = nullptr;
+  std::string why;
+  int code;
+  if (validate(*desc, why, code)) return fail(ctx, code, "kmi_lattice_create: " + why);
+  KMI_HIP(ctx, hipSetDevice(ctx->device));
+  kmi_lattice* L = new (std::nothrow) kmi_lattice();
+  if (!L) return fail(ctx, KMI_ERR_NOMEM, "kmi_lattice_create: out of host memory");
+  L->ctx = ctx;
+  L->desc = *desc;
+  if (L->desc.stencil == KMI_D2Q9) L->desc.nz = 1;
+  L->Q = desc->stencil == KMI_D2Q9 ? 9 : 19;
+  L->ncells = (long long)desc->nx * desc->ny * L->desc.nz;
+  L->plane = L->ncells;
+  L->esize = storage_bytes(desc->storage);
+  const size_t bytes = size_t(L->plane) * L->Q * L->esize;
+  hipError_t e = hipSuccess;
+  for (int p = 0; p < 2 && e == hipSuccess; ++p) e = hipMalloc(&L->f[p], bytes);
+  for (int p = 0; p < 2 && e == hipSuccess && desc->thermal; ++p) e = hipMalloc(&L->g[p], bytes);
+  if (e == hipSuccess) e = hipEventCreate(&L->ev[0]);
+  if (e == hipSuccess) e = hipEventCreate(&L->ev[1]);
+  if (e != hipSuccess) {
+    kmi_lattice_destroy(L);
+    return fail(ctx, e == hipErrorOutOfMemory ? KMI_ERR_NOMEM : KMI_ERR_DEVICE,
+                std::string("kmi_lattice_create: ") + hipGetErrorString(e));
+  }
+  build_args(L);
+  // rest equilibrium f = w_q, g = w_q T_ref (initialize_2d/3d, basic.jl:64-70)
+  int rc = kmi_init_equilibrium(L, nullptr, nullptr, nullptr, nullptr);
+  if (rc == KMI_OK && desc->thermal) rc = kmi_init_temperature(L, nullptr);
+  if (rc != KMI_OK) {
+    kmi_lattice_destroy(L);
+    return rc;
+  }
+  *out = L;
+  return KMI_OK;
+}
+
+int kmi_lattice_destroy(kmi_lattice* L) {
+  if (!L) return KMI_OK;
+  (void)hipSetDevice(L->ctx->device);
+  (void)hipStreamSynchronize(L->ctx->stream);
+  drop_graphs(L);
+  for (int p = 0; p < 2; ++p) {
+    if (L->f[p]) (void)hipFree(L->f[p]);
+    if (L->g[p]) (void)hipFree(L->g[p]);
+    if (L->ev[p]) (void)hipEventDestroy(L->ev[p]);
+  }
+  if (L->mask) (void)hipFree(L->mask);
+  if (L->uprev) (void)hipFree(L->uprev);
+  delete L;
+  return KMI_OK;
+}
+
+int kmi_set_mask(kmi_lattice* L, const uint8_t* is_solid) {
+  if (!L) return fail(nullptr, KMI_ERR_ARG, "kmi_set_mask: lattice is NULL");
+  kmi_ctx* c = L->ctx;
+  KMI_HIP(c, hipSetDevice(c->device));
+  KMI_HIP(c, hipStreamSynchronize(c->stream));
+  drop_graphs(L);
+  if (!is_solid) {
+    if (L->mask) KMI_HIP(c, hipFree(L->mask));
+    L->mask = nullptr;
+  } else {
+    if (!L->mask) KMI_HIP(c, hipMalloc(&L->mask, size_t(L->ncells)));
+    KMI_HIP(c, hipMemcpy(L->mask, is_solid, size_t(L->ncells), hipMemcpyHostToDevice));
+  }
+  build_args(L);
+  return KMI_OK;
+}
+
+static int field_ptrs(kmi_lattice* L, int field, void**& buf, double& shift) {
+  if (field == KMI_FIELD_F) {
+    buf = L->f;
+    shift = storage_shift_f();
+    return KMI_OK;
+  }
+  if (field == KMI_FIELD_G && L->desc.thermal) {
+    buf = L->g;
+    shift = L->desc.T_ref;
+    return KMI_OK;
+  }
+  return fail(L->ctx, KMI_ERR_ARG, "unknown field or lattice without thermal populations");
+}
+
+int kmi_upload(kmi_lattice* L, int32_t field, const void* host, int32_t host_dtype) {
+  if (!L) return fail(nullptr, KMI_ERR_ARG, "kmi_upload: lattice is NULL");
+  kmi_ctx* c = L->ctx;
+  if (!host) return fail(c, KMI_ERR_ARG, "kmi_upload: host buffer is NULL");
+  if (host_dtype != KMI_HOST_F64 && host_dtype != KMI_HOST_F32)
+    return fail(c, KMI_ERR_ARG, "kmi_upload: unknown host dtype");
+  void** buf;
+  double shift;
+  if (int rc = field_ptrs(L, field, buf, shift)) return rc;
+  KMI_HIP(c, hipSetDevice(c->device));
+  const size_t n = size_t(L->ncells) * L->Q;
+  std::vector<double> tmp;
+  const double* src = static_cast<const double*>(host);
+  if (host_dtype == KMI_HOST_F32) {
+    tmp.resize(n);
+    const float* h = static_cast<const float*>(host);
+    for (size_t i = 0; i < n; ++i) tmp[i] = h[i];
+    src = tmp.data();
+  }
+  double* stage = nullptr;
+  KMI_HIP(c, hipMalloc(&stage, n * sizeof(double)));
+  hipError_t e = hipMemcpyAsync(stage, src, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess)
+    e = launch_to_storage(L->Q, L->desc.storage, stage, buf[L->cur], L->ncells, L->plane, shift,
+                          c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(stage);
+  if (e != hipSuccess) return hip_fail(c, e, "kmi_upload");
+  if (field == KMI_FIELD_F) L->steps = 0;
+  return KMI_OK;
+}
+
+int kmi_download(kmi_lattice* L, int32_t field, void* host, int32_t host_dtype) {
+  if (!L) return fail(nullptr, KMI_ERR_ARG, "kmi_download: lattice is NULL");
+  kmi_ctx* c = L->ctx;
+  if (!host) return fail(c, KMI_ERR_ARG, "kmi_download: host buffer is NULL");
+  if (host_dtype != KMI_HOST_F64 && host_dtype != KMI_HOST_F32)
+    return fail(c, KMI_ERR_ARG, "kmi_download: unknown host dtype");
+  void** buf;
+  double shift;
+  if (int rc = field_ptrs(L, field, buf, shift)) return rc;
+  KMI_HIP(c, hipSetDevice(c->device));
+  const size_t n = size_t(L->ncells) * L->Q;
+  double* stage = nullptr;
+  KMI_HIP(c, hipMalloc(&stage, n * sizeof(double)));
+  std::vector<double> tmp;
+  double* dst = static_cast<double*>(host);
+  if (host_dtype == KMI_HOST_F32) {
+    tmp.resize(n);
+    dst = tmp.data();
+  }
+  hipError_t e = launch_from_storage(L->Q, L->desc.storage, buf[L->cur], stage, L->ncells,
+                                     L->plane, shift, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dst, stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(stage);
+  if (e != hipSuccess) return hip_fail(c, e, "kmi_download");
+  if (host_dtype == KMI_HOST_F32) {
+    float* h = static_cast<float*>(host);
+    for (size_t i = 0; i < n; ++i) h[i] = float(tmp[i]);
+  }
+  return KMI_OK;
+}
+
+// Copies up to `count` host arrays of ncells doubles to one device block (NULL stays NULL).
+static int stage_fields(kmi_lattice* L, const double* const* host, int count, double** dev,
+                        double*& block) {
+  kmi_ctx* c = L->ctx;
+  int used = 0;
+  for (int k = 0; k < count; ++k) used += host[k] != nullptr;
+  block = nullptr;
+  for (int k = 0; k < count; ++k) dev[k] = nullptr;
+  if (used == 0) return KMI_OK;
+  KMI_HIP(c, hipMalloc(&block, size_t(used) * L->ncells * sizeof(double)));
+  int slot = 0;
+  for (int k = 0; k < count; ++k) {
+    if (!host[k]) continue;
+    dev[k] = block + size_t(slot++) * L->ncells;
+    hipError_t e = hipMemcpyAsync(dev[k], host[k], size_t(L->ncells) * sizeof(double),
+                                  hipMemcpyHostToDevice, c->stream);
+    if (e != hipSuccess) {
+      (void)hipFree(block);
+      block = nullptr;
+      return hip_fail(c, e, "staging host field");
+    }
+  }
+  return KMI_OK;
+}
+
+int kmi_init_equilibrium(kmi_lattice* L, const double* rho, const double* ux, const double* uy,
+                         const double* uz) {
+  if (!L) return fail(nullptr, KMI_ERR_ARG, "kmi_init_equilibrium: lattice is NULL");
+  kmi_ctx* c = L->ctx;
+  KMI_HIP(c, hipSetDevice(c->device));
+  const double* host[4] = {rho, ux, uy, L->Q == 19 ? uz : nullptr};
+  double* dev[4];
+  double* block;
+  if (int rc = stage_fields(L, host, 4, dev, block)) return rc;
+  hipError_t e = launch_init_eq(L->Q, L->desc.storage, L->f[L->cur], dev[0], dev[1], dev[2],
+                                dev[3], L->ncells, L->plane, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (block) (void)hipFree(block);
+  if (e != hipSuccess) return hip_fail(c, e, "kmi_init_equilibrium");
+  L->steps = 0;
+  return KMI_OK;
+}
+
+int kmi_init_temperature(kmi_lattice* L, const double* T) {
+  if (!L) return fail(nullptr, KMI_ERR_ARG, "kmi_init_temperature: lattice is NULL");
+  kmi_ctx* c = L->ctx;
+  if (!L->desc.thermal) return fail(c, KMI_ERR_STATE, "kmi_init_temperature: lattice is not thermal");
+  KMI_HIP(c, hipSetDevice(c->device));
+  const double* host[1] = {T};
+  double* dev[1];
+  double* block;
+  if (int rc = stage_fields(L, host, 1, dev, block)) return rc;
+  hipError_t e = launch_init_temp(L->desc.storage, L->g[L->cur], dev[0], L->ncells, L->plane,
+                                  L->desc.T_ref, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (block) (void)hipFree(block);
+  if (e != hipSuccess) return hip_fail(c, e, "kmi_init_temperature");
+  return KMI_OK;
+}
+
+int kmi_step(kmi_lattice* L, int64_t nsteps) {
+  if (!L) return fail(nullptr, KMI_ERR_ARG, "kmi_step: lattice is NULL");
+  if (nsteps < 0) return fail(L->ctx, KMI_ERR_ARG, "kmi_step: nsteps must be >= 0");
+  KMI_HIP(L->ctx, hipSetDevice(L->ctx->device));
+  if (int rc = run_steps(L, nsteps)) return rc;
+  KMI_HIP(L->ctx, hipStreamSynchronize(L->ctx->stream));
+  return KMI_OK;
+}
+
+int kmi_step_timed(kmi_lattice* L, int64_t nsteps, double* ms) {
+  if (!L) return fail(nullptr, KMI_ERR_ARG, "kmi_step_timed: lattice is NULL");
+  kmi_ctx* c = L->ctx;
+  if (nsteps < 0 || !ms) return fail(c, KMI_ERR_ARG, "kmi_step_timed: bad arguments");
+  KMI_HIP(c, hipSetDevice(c->device));
+  KMI_HIP(c, hipEventRecord(L->ev[0], c->stream));
+  if (int rc = run_steps(L, nsteps)) return rc;
+  KMI_HIP(c, hipEventRecord(L->ev[1], c->stream));
+  KMI_HIP(c, hipEventSynchronize(L->ev[1]));
+  float t = 0.0f;
+  KMI_HIP(c, hipEventElapsedTime(&t, L->ev[0], L->ev[1]));
+  *ms = t;
+  return KMI_OK;
+}
+
+static int reduce_cells(kmi_lattice* L, int kind, double* out3) {
+  kmi_ctx* c = L->ctx;
+  hipError_t e = launch_reduce_cells(L->Q, L->desc.storage, L->f[L->cur],
+                                     L->desc.thermal ? L->g[L->cur] : nullptr, L->mask, L->uprev,
+                                     L->ncells, L->plane, L->desc.T_ref, kind, c->partials, c->out,
+                                     c->stream);
+  if (e != hipSuccess) return hip_fail(c, e, "reduction kernel");
+  if (kind == 101) return KMI_OK;
+  KMI_HIP(c, hipMemcpyAsync(out3, c->out, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  KMI_HIP(c, hipStreamSynchronize(c->stream));
+  return KMI_OK;
+}
+
+int kmi_step_residual(kmi_lattice* L, int64_t nsteps, double* r_u) {
+  if (!L) return fail(nullptr, KMI_ERR_ARG, "kmi_step_residual: lattice is NULL");
+  kmi_ctx* c = L->ctx;
+  if (nsteps < 1 || !r_u) return fail(c, KMI_ERR_ARG, "kmi_step_residual: nsteps >= 1 and r_u required");
+  KMI_HIP(c, hipSetDevice(c->device));
+  if (!L->uprev) KMI_HIP(c, hipMalloc(&L->uprev, size_t(3) * L->ncells * sizeof(double)));
+  if (int rc = run_steps(L, nsteps - 1)) return rc;
+  double tmp[3];
+  if (int rc = reduce_cells(L, 101, tmp)) return rc;  // u^(k-1) → uprev
+  if (int rc = run_steps(L, 1)) return rc;
+  if (int rc = reduce_cells(L, 100, tmp)) return rc;
+  *r_u = std::sqrt(tmp[0] / tmp[1]);  // gnf_periodic_steady.jl:192
+  return KMI_OK;
+}
+
+int kmi_moments(kmi_lattice* L, double* rho, double* ux, double* uy, double* uz, double* T) {
+  if (!L) return fail(nullptr, KMI_ERR_ARG, "kmi_moments: lattice is NULL");
+  kmi_ctx* c = L->ctx;
+  if (T && !L->desc.thermal) return fail(c, KMI_ERR_STATE, "kmi_moments: lattice is not thermal");
+  KMI_HIP(c, hipSetDevice(c->device));
+  double* host[5] = {rho, ux, uy, L->Q == 19 ? uz : nullptr, T};
+  int used = 0;
+  for (double* h : host) used += h != nullptr;
+  if (uz && L->Q == 9)
+    for (long long i = 0; i < L->ncells; ++i) uz[i] = 0.0;
+  if (used == 0) return KMI_OK;
+  double* block = nullptr;
+  KMI_HIP(c, hipMalloc(&block, size_t(used) * L->ncells * sizeof(double)));
+  double* dev[5];
+  int slot = 0;
+  for (int k = 0; k < 5; ++k) dev[k] = host[k] ? block + size_t(slot++) * L->ncells : nullptr;
+  hipError_t e = launch_moments(L->Q, L->desc.storage, L->f[L->cur],
+                                L->desc.thermal ? L->g[L->cur] : nullptr, dev[0], dev[1], dev[2],
+                                dev[3], dev[4], L->ncells, L->plane, L->desc.T_ref, c->stream);
+  for (int k = 0; k < 5 && e == hipSuccess; ++k)
+    if (host[k])
+      e = hipMemcpyAsync(host[k], dev[k], size_t(L->ncells) * sizeof(double),
+                         hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(block);
+  if (e != hipSuccess) return hip_fail(c, e, "kmi_moments");
+  return KMI_OK;
+}
+
+int kmi_reduce(kmi_lattice* L, int32_t kind, int32_t arg, double* out) {
+  if (!L) return fail(nullptr, KMI_ERR_ARG, "kmi_reduce: lattice is NULL");
+  kmi_ctx* c = L->ctx;
+  if (!out) return fail(c, KMI_ERR_ARG, "kmi_reduce: out is NULL");
+  KMI_HIP(c, hipSetDevice(c->device));
+  double r[3] = {0, 0, 0};
+  switch (kind) {
+    case KMI_RED_MASS:
+    case KMI_RED_KINETIC_ENERGY:
+    case KMI_RED_MAX_SPEED:
+      if (int rc = reduce_cells(L, kind, r)) return rc;
+      out[0] = r[0];
+      return KMI_OK;
+    case KMI_RED_MOMENTUM:
+      if (int rc = reduce_cells(L, kind, r)) return rc;
+      out[0] = r[0];
+      out[1] = r[1];
+      out[2] = r[2];
+      return KMI_OK;
+    case KMI_RED_THERMAL:
+      if (!L->desc.thermal) return fail(c, KMI_ERR_STATE, "kmi_reduce: lattice is not thermal");
+      if (int rc = reduce_cells(L, kind, r)) return rc;
+      out[0] = r[0];
+      return KMI_OK;
+    case KMI_RED_NUSSELT: {
+      const kmi_desc& d = L->desc;
+      if (!d.thermal) return fail(c, KMI_ERR_STATE, "kmi_reduce: lattice is not thermal");
+      if (arg < 0 || arg > 3) return fail(c, KMI_ERR_ARG, "kmi_reduce: Nusselt face must be W, E, S or N");
+      const int opp = arg ^ 1;
+      auto dirichlet = [&](int f) {
+        return d.faces[f].kind != KMI_FACE_PERIODIC && d.faces[f].thermal == KMI_THERMAL_DIRICHLET;
+      };
+      if (!dirichlet(arg) || !dirichlet(opp))
+        return fail(c, KMI_ERR_STATE, "kmi_reduce: Nusselt needs Dirichlet temperatures on the named face and its opposite");
+      const bool xface = arg == KMI_WEST || arg == KMI_EAST;
+      const int npar = int(xface ? d.ny : d.nx);
+      const bool per = d.faces[xface ? KMI_SOUTH : KMI_WEST].kind == KMI_FACE_PERIODIC;
+      const int lo = per ? 0 : 1, hi = per ? npar : npar - 1;
+      if (hi - lo < 1 || (xface ? d.nx : d.ny) < 2)
+        return fail(c, KMI_ERR_STATE, "kmi_reduce: grid too small for the Nusselt estimator");
+      const double Th = d.faces[arg].T, dT = d.faces[arg].T - d.faces[opp].T;
+      hipError_t e = launch_nusselt(d.storage, L->g[L->cur], int(d.nx), int(d.ny), L->plane,
+                                    d.T_ref, arg, Th, dT, lo, hi, c->partials, c->out, c->stream);
+      if (e != hipSuccess) return hip_fail(c, e, "Nusselt kernel");
+      KMI_HIP(c, hipMemcpyAsync(r, c->out, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      KMI_HIP(c, hipStreamSynchronize(c->stream));
+      out[0] = r[0] / double(hi - lo);
+      return KMI_OK;
+    }
+    default:
+      return fail(c, KMI_ERR_ARG, "kmi_reduce: unknown reduction");
+  }
+}
+
+int kmi_lattice_steps(const kmi_lattice* L, int64_t* steps) {
+  if (!L || !steps) return fail(nullptr, KMI_ERR_ARG, "kmi_lattice_steps: NULL argument");
+  *steps = L->steps;
+  return KMI_OK;
+}
+
+int kmi_lattice_device_bytes(const kmi_lattice* L, int64_t* bytes) {
+  if (!L || !bytes) return fail(nullptr, KMI_ERR_ARG, "kmi_lattice_device_bytes: NULL argument");
+  int64_t b = int64_t(L->plane) * L->Q * int64_t(L->esize) * (L->desc.thermal ? 4 : 2);
+  if (L->mask) b += L->ncells;
+  if (L->uprev) b += 3 * L->ncells * int64_t(sizeof(double));
+  *bytes = b;
+  return KMI_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,358 @@
+// kmi_aux.hip — layout conversion, initial states, moments and FP64 reductions.
+//
+// Reductions are deterministic: each work-group folds its cells in FP64 with a
+// wave64 butterfly (__shfl_xor) and an LDS pass across waves, writes one partial per
+// group, and a single work-group combines the partials in fixed order with Neumaier
+// compensation.  No atomics, so repeated calls return identical bits.
+#include "kmi_common.h"
+
+namespace kmi {
+
+constexpr int kRedBlock = 256;
+constexpr int kRedMaxVals = 4;
+
+// --- conversion host layout (double, [cell, q]) <-> storage ------------------------------
+// shift_q: w_q for f, w_q T_ref for g (shifted storage only).
+template <int Q, int ST>
+__global__ void k_to_storage(const double* __restrict__ src, void* dst_, long long ncells,
+                             long long plane, double shift_scale) {
+  using S = Store<ST>;
+  using E = typename S::elem;
+  using R = typename S::real;
+  E* dst = static_cast<E*>(dst_);
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  sfor<Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    double v = src[c + q * ncells];
+    if constexpr (S::shifted) v = v - Lat<Q>::w[q] * shift_scale;
+    S::st(dst, c + q * plane, R(v));
+  });
+}
+
+template <int Q, int ST>
+__global__ void k_from_storage(const void* src_, double* __restrict__ dst, long long ncells,
+                               long long plane, double shift_scale) {
+  using S = Store<ST>;
+  using E = typename S::elem;
+  const E* src = static_cast<const E*>(src_);
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  sfor<Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    double v = double(S::ld(src, c + q * plane));
+    if constexpr (S::shifted) v = v + Lat<Q>::w[q] * shift_scale;
+    dst[c + q * ncells] = v;
+  });
+}
+
+// --- initial states -------------------------------------------------------------------
+// f_q = feq_q(ρ, u), written as lattice/d2q9.jl:60-67 / d3q19.jl:67-75 (cu = Σ c_a u_a).
+template <int Q, int ST>
+__global__ void k_init_eq(void* dst_, const double* __restrict__ rho, const double* __restrict__ ux,
+                          const double* __restrict__ uy, const double* __restrict__ uz,
+                          long long ncells, long long plane) {
+  using S = Store<ST>;
+  using E = typename S::elem;
+  using R = typename S::real;
+  E* dst = static_cast<E*>(dst_);
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  const double r = rho ? rho[c] : 1.0;
+  const double u = ux ? ux[c] : 0.0, v = uy ? uy[c] : 0.0, w3 = uz ? uz[c] : 0.0;
+  sfor<Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr double w = Lat<Q>::w[q];
+    if constexpr (!S::shifted) {
+      const R rr = R(r), a = R(u), b = R(v), d = R(w3);
+      R cu = R(Lat<Q>::cx[q]) * a + R(Lat<Q>::cy[q]) * b;
+      if constexpr (Q == 19) cu = cu + R(Lat<Q>::cz[q]) * d;
+      R usq = a * a + b * b;
+      if constexpr (Q == 19) usq = usq + d * d;
+      S::st(dst, c + q * plane,
+            R(w) * rr * (R(1) + R(3) * cu + R(4.5) * cu * cu - R(1.5) * usq));
+    } else {
+      double cu = Lat<Q>::cx[q] * u + Lat<Q>::cy[q] * v + Lat<Q>::cz[q] * w3;
+      double usq = u * u + v * v + w3 * w3;
+      double x = 3.0 * cu + 4.5 * cu * cu - 1.5 * usq;
+      S::st(dst, c + q * plane, R(w * ((r - 1.0) + r * x)));
+    }
+  });
+}
+
+// g_q = w_q T (thermal.jl:94-96).
+template <int ST>
+__global__ void k_init_temp(void* dst_, const double* __restrict__ T, long long ncells,
+                            long long plane, double T0) {
+  using S = Store<ST>;
+  using E = typename S::elem;
+  using R = typename S::real;
+  E* dst = static_cast<E*>(dst_);
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  const double t = T ? T[c] : T0;
+  sfor<9>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    if constexpr (!S::shifted) S::st(dst, c + q * plane, R(Lat<9>::w[q]) * R(t));
+    else S::st(dst, c + q * plane, R(Lat<9>::w[q] * (t - T0)));
+  });
+}
+
+// --- per-cell moments ------------------------------------------------------------------
+// Plain storage: compute_macroscopic_2d/3d and compute_temperature_2d in the storage type
+// (macroscopic.jl:5-61, thermal_2d.jl:113-119).  Shifted: FP64 from the deviations.
+template <int Q, int ST>
+KMI_DEV void cell_moments(const void* f_, const void* g_, long long c, long long plane, double T0,
+                          double& rho, double& ux, double& uy, double& uz, double& T) {
+  using S = Store<ST>;
+  using E = typename S::elem;
+  using R = typename S::real;
+  const E* f = static_cast<const E*>(f_);
+  if constexpr (!S::shifted) {
+    R fp[Q];
+    sfor<Q>([&](auto qc) { fp[decltype(qc)::value] = S::ld(f, c + decltype(qc)::value * plane); });
+    const R r = zeroth_moment<Q>(fp);
+    const R inv = R(1) / r;
+    rho = r;
+    ux = first_moment<Q, 0>(fp) * inv;
+    uy = first_moment<Q, 1>(fp) * inv;
+    if constexpr (Q == 19) uz = first_moment<Q, 2>(fp) * inv;
+    else uz = 0.0;
+  } else {
+    double fp[Q];
+    sfor<Q>([&](auto qc) {
+      fp[decltype(qc)::value] = double(S::ld(f, c + decltype(qc)::value * plane));
+    });
+    const double r = 1.0 + zeroth_moment<Q>(fp);
+    rho = r;
+    ux = first_moment<Q, 0>(fp) / r;
+    uy = first_moment<Q, 1>(fp) / r;
+    if constexpr (Q == 19) uz = first_moment<Q, 2>(fp) / r;
+    else uz = 0.0;
+  }
+  T = 0.0;
+  if constexpr (Q == 9) {
+    if (g_) {
+      const E* g = static_cast<const E*>(g_);
+      if constexpr (!S::shifted) {
+        R gp[9];
+        sfor<9>([&](auto qc) { gp[decltype(qc)::value] = S::ld(g, c + decltype(qc)::value * plane); });
+        T = zeroth_moment<9>(gp);
+      } else {
+        double gp[9];
+        sfor<9>([&](auto qc) {
+          gp[decltype(qc)::value] = double(S::ld(g, c + decltype(qc)::value * plane));
+        });
+        T = T0 + zeroth_moment<9>(gp);
+      }
+    }
+  }
+}
+
+template <int Q, int ST>
+__global__ void k_moments(const void* f, const void* g, double* rho, double* ux, double* uy,
+                          double* uz, double* T, long long ncells, long long plane, double T0) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  double r, u, v, w, t;
+  cell_moments<Q, ST>(f, g, c, plane, T0, r, u, v, w, t);
+  if (rho) rho[c] = r;
+  if (ux) ux[c] = u;
+  if (uy) uy[c] = v;
+  if (uz) uz[c] = w;
+  if (T) T[c] = t;
+}
+
+// --- block reduction -------------------------------------------------------------------
+template <int NV>
+KMI_DEV void block_reduce(double (&v)[NV], bool is_max, double* partials) {
+  __shared__ double lds[kRedBlock / 64][kRedMaxVals];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int n = 0; n < NV; ++n) {
+    double x = v[n];
+    for (int off = 32; off > 0; off >>= 1) {
+      const double y = __shfl_xor(x, off, 64);
+      x = is_max ? fmax(x, y) : x + y;
+    }
+    if (lane == 0) lds[wave][n] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) / 64;
+#pragma unroll
+    for (int n = 0; n < NV; ++n) {
+      double x = lds[0][n];
+      for (int w = 1; w < nw; ++w) x = is_max ? fmax(x, lds[w][n]) : x + lds[w][n];
+      partials[blockIdx.x * NV + n] = x;
+    }
+  }
+}
+
+// kind: 0 mass, 1 momentum, 2 kinetic energy, 3 thermal, 5 max speed,
+//       100 residual (Σ fluid |u − u_prev|², Σ fluid |u|²), 101 snapshot u → u_prev.
+template <int Q, int ST>
+__global__ void __launch_bounds__(kRedBlock)
+    k_reduce_cells(const void* f, const void* g, const uint8_t* mask, double* uprev,
+                   long long ncells, long long plane, double T0, int kind, double* partials) {
+  double acc[3] = {0.0, 0.0, 0.0};
+  const bool is_max = kind == KMI_RED_MAX_SPEED;
+  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < ncells;
+       c += (long long)gridDim.x * blockDim.x) {
+    double r, u, v, w, t;
+    cell_moments<Q, ST>(f, g, c, plane, T0, r, u, v, w, t);
+    switch (kind) {
+      case KMI_RED_MASS: acc[0] += r; break;
+      case KMI_RED_MOMENTUM: acc[0] += r * u; acc[1] += r * v; acc[2] += r * w; break;
+      case KMI_RED_KINETIC_ENERGY: acc[0] += 0.5 * r * (u * u + v * v + w * w); break;
+      case KMI_RED_THERMAL: acc[0] += t; break;
+      case KMI_RED_MAX_SPEED: acc[0] = fmax(acc[0], sqrt(u * u + v * v + w * w)); break;
+      case 100:
+        if (!(mask && mask[c])) {
+          const double du = u - uprev[c], dv = v - uprev[c + ncells], dw = w - uprev[c + 2 * ncells];
+          acc[0] += du * du + dv * dv + dw * dw;
+          acc[1] += u * u + v * v + w * w;
+        }
+        break;
+      case 101:
+        uprev[c] = u;
+        uprev[c + ncells] = v;
+        uprev[c + 2 * ncells] = w;
+        break;
+      default: break;
+    }
+  }
+  if (kind == 101) return;
+  block_reduce<3>(acc, is_max, partials);
+}
+
+// Hot-wall Nusselt number per wall-parallel index p (thermal.jl:225-234):
+// Nu_p = −H (−8 T_w + 9 T_1 − T_2) / (3 dx) / ΔT, with T_1, T_2 the first two nodes.
+template <int ST>
+__global__ void __launch_bounds__(kRedBlock)
+    k_nusselt(const void* g_, int nx, int ny, long long plane, double T0, int face, double Th,
+              double dT, int lo, int hi, double* partials) {
+  using S = Store<ST>;
+  using E = typename S::elem;
+  const E* g = static_cast<const E*>(g_);
+  double acc[1] = {0.0};
+  const bool xface = face == KMI_WEST || face == KMI_EAST;
+  const double H = xface ? nx : ny;
+  for (int p = lo + blockIdx.x * blockDim.x + threadIdx.x; p < hi; p += gridDim.x * blockDim.x) {
+    long long c1, c2;
+    if (face == KMI_WEST) { c1 = (long long)p * nx; c2 = c1 + 1; }
+    else if (face == KMI_EAST) { c1 = (long long)p * nx + nx - 1; c2 = c1 - 1; }
+    else if (face == KMI_SOUTH) { c1 = p; c2 = p + nx; }
+    else { c1 = (long long)(ny - 1) * nx + p; c2 = c1 - nx; }
+    double T1 = 0.0, T2 = 0.0;
+    if constexpr (!S::shifted) {
+      using R = typename S::real;
+      R a = S::ld(g, c1), b = S::ld(g, c2);
+      for (int q = 1; q < 9; ++q) { a = a + S::ld(g, c1 + q * plane); b = b + S::ld(g, c2 + q * plane); }
+      T1 = a;
+      T2 = b;
+    } else {
+      double a = 0.0, b = 0.0;
+      for (int q = 0; q < 9; ++q) { a += double(S::ld(g, c1 + q * plane)); b += double(S::ld(g, c2 + q * plane)); }
+      T1 = T0 + a;
+      T2 = T0 + b;
+    }
+    acc[0] += -H * (-8 * Th + 9 * T1 - T2) / 3.0 / dT;
+  }
+  block_reduce<1>(acc, false, partials);
+}
+
+// Combine `nb` partials of NV values each, in order, with Neumaier compensation.
+__global__ void k_finalize(const double* __restrict__ partials, int nb, int nv, int is_max,
+                           double* __restrict__ out) {
+  const int n = threadIdx.x;
+  if (n >= nv) return;
+  double s = is_max ? partials[n] : 0.0, comp = 0.0;
+  for (int b = is_max ? 1 : 0; b < nb; ++b) {
+    const double x = partials[b * nv + n];
+    if (is_max) { s = fmax(s, x); continue; }
+    const double t = s + x;
+    comp += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
+    s = t;
+  }
+  out[n] = s + comp;
+}
+
+// ---- host-side launchers ----------------------------------------------------------------
+#define KMI_ST_SWITCH(st, CALL)                                \
+  switch (st) {                                                \
+    case KMI_STORE_F64: { constexpr int ST = KMI_STORE_F64; CALL; } break;   \
+    case KMI_STORE_F32: { constexpr int ST = KMI_STORE_F32; CALL; } break;   \
+    case KMI_STORE_F32S: { constexpr int ST = KMI_STORE_F32S; CALL; } break; \
+    default: { constexpr int ST = KMI_STORE_F16S; CALL; } break;             \
+  }
+
+static inline dim3 grid1(long long n) { return dim3(unsigned((n + 255) / 256)); }
+
+hipError_t launch_to_storage(int Q, int st, const double* src, void* dst, long long ncells,
+                             long long plane, double shift, hipStream_t s) {
+  if (Q == 9) { KMI_ST_SWITCH(st, hipLaunchKernelGGL((k_to_storage<9, ST>), grid1(ncells), dim3(256), 0, s, src, dst, ncells, plane, shift)); }
+  else { KMI_ST_SWITCH(st, hipLaunchKernelGGL((k_to_storage<19, ST>), grid1(ncells), dim3(256), 0, s, src, dst, ncells, plane, shift)); }
+  return hipGetLastError();
+}
+
+hipError_t launch_from_storage(int Q, int st, const void* src, double* dst, long long ncells,
+                               long long plane, double shift, hipStream_t s) {
+  if (Q == 9) { KMI_ST_SWITCH(st, hipLaunchKernelGGL((k_from_storage<9, ST>), grid1(ncells), dim3(256), 0, s, src, dst, ncells, plane, shift)); }
+  else { KMI_ST_SWITCH(st, hipLaunchKernelGGL((k_from_storage<19, ST>), grid1(ncells), dim3(256), 0, s, src, dst, ncells, plane, shift)); }
+  return hipGetLastError();
+}
+
+hipError_t launch_init_eq(int Q, int st, void* dst, const double* rho, const double* ux,
+                          const double* uy, const double* uz, long long ncells, long long plane,
+                          hipStream_t s) {
+  if (Q == 9) { KMI_ST_SWITCH(st, hipLaunchKernelGGL((k_init_eq<9, ST>), grid1(ncells), dim3(256), 0, s, dst, rho, ux, uy, uz, ncells, plane)); }
+  else { KMI_ST_SWITCH(st, hipLaunchKernelGGL((k_init_eq<19, ST>), grid1(ncells), dim3(256), 0, s, dst, rho, ux, uy, uz, ncells, plane)); }
+  return hipGetLastError();
+}
+
+hipError_t launch_init_temp(int st, void* dst, const double* T, long long ncells, long long plane,
+                            double T0, hipStream_t s) {
+  KMI_ST_SWITCH(st, hipLaunchKernelGGL((k_init_temp<ST>), grid1(ncells), dim3(256), 0, s, dst, T, ncells, plane, T0));
+  return hipGetLastError();
+}
+
+hipError_t launch_moments(int Q, int st, const void* f, const void* g, double* rho, double* ux,
+                          double* uy, double* uz, double* T, long long ncells, long long plane,
+                          double T0, hipStream_t s) {
+  if (Q == 9) { KMI_ST_SWITCH(st, hipLaunchKernelGGL((k_moments<9, ST>), grid1(ncells), dim3(256), 0, s, f, g, rho, ux, uy, uz, T, ncells, plane, T0)); }
+  else { KMI_ST_SWITCH(st, hipLaunchKernelGGL((k_moments<19, ST>), grid1(ncells), dim3(256), 0, s, f, g, rho, ux, uy, uz, T, ncells, plane, T0)); }
+  return hipGetLastError();
+}
+
+// Number of partial slots a reduction launch needs (3 values per group).
+int reduce_groups(long long ncells) {
+  long long g = (ncells + kRedBlock - 1) / kRedBlock;
+  return int(g < 2048 ? (g < 1 ? 1 : g) : 2048);
+}
+
+hipError_t launch_reduce_cells(int Q, int st, const void* f, const void* g, const uint8_t* mask,
+                               double* uprev, long long ncells, long long plane, double T0,
+                               int kind, double* partials, double* out, hipStream_t s) {
+  const int nb = reduce_groups(ncells);
+  if (Q == 9) { KMI_ST_SWITCH(st, hipLaunchKernelGGL((k_reduce_cells<9, ST>), dim3(nb), dim3(kRedBlock), 0, s, f, g, mask, uprev, ncells, plane, T0, kind, partials)); }
+  else { KMI_ST_SWITCH(st, hipLaunchKernelGGL((k_reduce_cells<19, ST>), dim3(nb), dim3(kRedBlock), 0, s, f, g, mask, uprev, ncells, plane, T0, kind, partials)); }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || kind == 101) return e;
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, s, partials, nb, 3,
+                     kind == KMI_RED_MAX_SPEED ? 1 : 0, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_nusselt(int st, const void* g, int nx, int ny, long long plane, double T0,
+                          int face, double Th, double dT, int lo, int hi, double* partials,
+                          double* out, hipStream_t s) {
+  const int nb = (hi - lo + kRedBlock - 1) / kRedBlock;
+  KMI_ST_SWITCH(st, hipLaunchKernelGGL((k_nusselt<ST>), dim3(nb), dim3(kRedBlock), 0, s, g, nx, ny, plane, T0, face, Th, dT, lo, hi, partials));
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, s, partials, nb, 1, 0, out);
+  return hipGetLastError();
+}
+
+}  // namespace kmi

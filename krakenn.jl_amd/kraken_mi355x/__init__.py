@@ -1,0 +1,39 @@
+"""kraken_mi355x — MI355X-native stream–collide backend for Kraken.jl's hot path.
+
+The solver runs in hand-written CDNA4 HIP kernels (libkraken_mi355x.so, C ABI in
+include/kraken_mi355x.h).  This package is the host-side mirror of the reference's
+driver API; it never computes populations itself.
+"""
+from . import _lib
+from ._lib import KrakenError, load
+from .drivers import (
+    D2Q9,
+    D3Q19,
+    LBMConfig,
+    cavity_desc,
+    lbm_config,
+    natural_convection_desc,
+    natural_convection_initial_temperature,
+    omega,
+    rayleigh_benard_desc,
+    rb_initial_temperature,
+    reynolds,
+    run_cavity_2d,
+    run_natural_convection_2d,
+    run_rayleigh_benard_2d,
+    run_taylor_green_3d,
+    run_to_residual,
+    taylor_green_3d_desc,
+    taylor_green_3d_fields,
+    trt_rates,
+)
+from .lattice import ArgumentError, Lattice, MI355XBackend, make_desc
+
+__all__ = [
+    "ArgumentError", "D2Q9", "D3Q19", "KrakenError", "LBMConfig", "Lattice", "MI355XBackend",
+    "cavity_desc", "lbm_config", "load", "make_desc", "natural_convection_desc",
+    "natural_convection_initial_temperature", "omega", "rayleigh_benard_desc",
+    "rb_initial_temperature", "reynolds", "run_cavity_2d", "run_natural_convection_2d",
+    "run_rayleigh_benard_2d", "run_taylor_green_3d", "run_to_residual", "taylor_green_3d_desc",
+    "taylor_green_3d_fields", "trt_rates",
+]

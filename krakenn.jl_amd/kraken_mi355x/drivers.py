@@ -1,0 +1,235 @@
+"""Host drivers mirroring Kraken.jl's hot-path API on the MI355X backend.
+
+Names, arguments and return values follow the reference (src/drivers/basic.jl,
+src/drivers/thermal.jl, src/kernels/fused_trt_2d.jl); every time step runs in the
+single-pass HIP kernel of libkraken_mi355x.so.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+from .lattice import ArgumentError, Lattice, MI355XBackend, make_desc
+
+W9 = np.array([4 / 9, 1 / 9, 1 / 9, 1 / 9, 1 / 9, 1 / 36, 1 / 36, 1 / 36, 1 / 36])
+
+
+class D2Q9:
+    """Lattice marker (src/lattice/d2q9.jl:17)."""
+
+
+class D3Q19:
+    """Lattice marker (src/lattice/d3q19.jl:11)."""
+
+
+@dataclass
+class LBMConfig:
+    """LBMConfig of src/drivers/basic.jl:6-23."""
+
+    lattice: object
+    Nx: int
+    Ny: int
+    Nz: int = 1
+    nu: float = 0.1
+    u_lid: float = 0.1
+    max_steps: int = 10000
+    output_interval: int = 1000
+
+
+def lbm_config(lattice, *, Nx, Ny, Nz=1, nu=None, ν=None, u_lid=0.1, max_steps=10000,
+               output_interval=1000) -> LBMConfig:
+    nu = ν if nu is None else nu
+    if nu is None:
+        raise ArgumentError("LBMConfig needs a viscosity nu")
+    return LBMConfig(lattice, Nx, Ny, 1 if isinstance(lattice, D2Q9) else Nz, float(nu), float(u_lid),
+                     int(max_steps), int(output_interval))
+
+
+def omega(config: LBMConfig) -> float:
+    """ω = 1/(3ν + 1/2) (basic.jl:30)."""
+    return 1.0 / (3.0 * config.nu + 0.5)
+
+
+def reynolds(config: LBMConfig) -> float:
+    """Re = u_lid·N/ν with N = Ny in 2D, Nz in 3D (basic.jl:37-40)."""
+    n = config.Ny if isinstance(config.lattice, D2Q9) else config.Nz
+    return config.u_lid * n / config.nu
+
+
+def trt_rates(nu: float, Lambda: float = 3.0 / 16.0):
+    """(s_plus, s_minus) (fused_trt_2d.jl:201-205)."""
+    return L.trt_rates(nu, Lambda)
+
+
+def _storage(T) -> str:
+    if isinstance(T, str):
+        return T
+    if T in (np.float64, float):
+        return "f64"
+    if T == np.float32:
+        return "f32"
+    raise ArgumentError(f"unsupported element type {T!r}")
+
+
+def _backend(backend) -> MI355XBackend:
+    if backend is None or backend in ("mi355x", ":mi355x"):
+        return MI355XBackend()
+    if isinstance(backend, MI355XBackend):
+        return backend
+    raise ArgumentError(f"unknown backend {backend!r}; this package provides MI355XBackend() only")
+
+
+def cavity_desc(N, nu, u_lid, *, storage="f64", collision="bgk", lid="zouhe", Ny=None):
+    """Lid-driven cavity: halfway-BB walls, lid on the north face (basic.jl:121-136).
+    lid='zouhe' is the reference's Zou–He lid (boundary_2d.jl:13-44); lid='ladd' is the
+    moving-wall bounce-back of stream_periodic_3d.jl:97-166 restated on D2Q9."""
+    Ny = N if Ny is None else Ny
+    top = {"kind": "zouhe_velocity" if lid == "zouhe" else "moving_wall", "u": (u_lid, 0.0, 0.0)}
+    return make_desc("d2q9", N, Ny, 1, nu, storage=storage, collision=collision,
+                     faces={"west": {"kind": "wall"}, "east": {"kind": "wall"},
+                            "south": {"kind": "wall"}, "north": top})
+
+
+def run_cavity_2d(config: LBMConfig, *, backend=None, T=np.float64, collision="bgk", lid="zouhe"):
+    """run_cavity_2d (basic.jl:112-144): rest start, `max_steps` steps, final moments."""
+    if not isinstance(config.lattice, D2Q9):
+        raise ArgumentError("run_cavity_2d needs a D2Q9 config")
+    d = cavity_desc(config.Nx, config.nu, config.u_lid, storage=_storage(T), collision=collision,
+                    lid=lid, Ny=config.Ny)
+    with Lattice(d, _backend(backend)) as lat:
+        lat.step(config.max_steps)
+        m = lat.moments()
+    return {"ρ": m["rho"], "ux": m["ux"], "uy": m["uy"], "config": config}
+
+
+def taylor_green_3d_fields(N, U0):
+    """u = U0 sin x cos y cos z, v = −U0 cos x sin y cos z, w = 0, ρ = 1, x = 2π(i−1)/N
+    (node convention of basic.jl:259-262)."""
+    x = 2 * np.pi * np.arange(N) / N
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    ux = U0 * np.sin(X) * np.cos(Y) * np.cos(Z)
+    uy = -U0 * np.cos(X) * np.sin(Y) * np.cos(Z)
+    uz = np.zeros_like(ux)
+    return np.ones_like(ux), ux, uy, uz
+
+
+def taylor_green_3d_desc(N, nu, storage="f32", Lambda=3.0 / 16.0):
+    """Triply periodic D3Q19 TRT (PullHalfwayBB_3D with periodic wrap + CollideTRTDirect_3D,
+    dsl/bricks_3d.jl:64-229)."""
+    per = {f: {"kind": "periodic"} for f in ("west", "east", "south", "north", "bottom", "top")}
+    return make_desc("d3q19", N, N, N, nu, storage=storage, collision="trt", faces=per, Lambda=Lambda)
+
+
+def run_taylor_green_3d(*, N=256, nu=0.01, U0=0.05, max_steps=1000, storage="f32", backend=None):
+    d = taylor_green_3d_desc(N, nu, storage)
+    with Lattice(d, _backend(backend)) as lat:
+        lat.init_equilibrium(*taylor_green_3d_fields(N, U0))
+        lat.step(max_steps)
+        m = lat.moments()
+    m["max_steps"] = max_steps
+    return m
+
+
+def rb_initial_temperature(Nx, Ny, T_hot, T_cold, seed="sine", noise=1e-3, rng_seed=12345):
+    """Linear conduction profile between halfway walls plus a seed (thermal.jl:74-97).
+    seed='sine': the reference's 0.01ΔT sin(2πi/Nx) sin(πj/Ny); seed='noise': uniform
+    noise of amplitude `noise`·ΔT from numpy's PCG64 with `rng_seed`."""
+    dT = T_hot - T_cold
+    i = np.arange(1, Nx + 1)[:, None]
+    j = np.arange(1, Ny + 1)[None, :]
+    T = T_hot - dT * (j - 0.5) / Ny + 0.0 * i
+    if seed == "sine":
+        T = T + (0.01 * dT) * np.sin(2 * np.pi * i / Nx) * np.sin(np.pi * j / Ny)
+    elif seed == "noise":
+        rng = np.random.default_rng(rng_seed)
+        T = T + noise * dT * rng.uniform(-1.0, 1.0, size=(Nx, Ny))
+    return np.asfortranarray(T)
+
+
+def rayleigh_benard_desc(Nx, Ny, Ra, Pr, T_hot=1.0, T_cold=0.0, nu=None, alpha=None, storage="f64"):
+    """Rayleigh–Bénard (thermal.jl:31-141): periodic x, hot south / cold north halfway
+    anti-bounce-back, Boussinesq F_y = β_g (T − T_ref), unfused-chain velocity coupling."""
+    dT = T_hot - T_cold
+    if dT == 0:
+        raise ArgumentError("run_rayleigh_benard_2d: T_hot and T_cold are equal")
+    H = Ny
+    nu = 0.05 if nu is None else float(nu)
+    a = nu / Pr if alpha is None else float(alpha)
+    beta_g = Ra * nu * a / (dT * H ** 3)
+    T_ref = (T_hot + T_cold) / 2
+    faces = {"west": {"kind": "periodic"}, "east": {"kind": "periodic"},
+             "south": {"kind": "wall", "thermal": "dirichlet", "T": T_hot},
+             "north": {"kind": "wall", "thermal": "dirichlet", "T": T_cold}}
+    d = make_desc("d2q9", Nx, Ny, 1, nu, storage=storage, collision="bgk", faces=faces, thermal=True,
+                  omega_T=1.0 / (3.0 * a + 0.5), beta_g=beta_g, T_ref=T_ref, coupling="split")
+    return d, {"nu": nu, "alpha": a, "beta_g": beta_g, "T_ref": T_ref}
+
+
+def run_rayleigh_benard_2d(*, Nx=128, Ny=32, Ra=2000.0, Pr=1.0, T_hot=1.0, T_cold=0.0,
+                           max_steps=20000, nu=None, alpha=None, storage="f64", seed="sine",
+                           backend=None):
+    d, p = rayleigh_benard_desc(Nx, Ny, Ra, Pr, T_hot, T_cold, nu, alpha, storage)
+    with Lattice(d, _backend(backend)) as lat:
+        lat.init_temperature(rb_initial_temperature(Nx, Ny, T_hot, T_cold, seed))
+        lat.step(max_steps)
+        m = lat.moments(temperature=True)
+        Nu = lat.reduce("nusselt", "south")
+    return {"ρ": m["rho"], "ux": m["ux"], "uy": m["uy"], "Temp": m["Temp"], "Nu": Nu, "Ra": Ra,
+            "Pr": p["nu"] / p["alpha"], "ν": p["nu"], "α": p["alpha"], "T_hot": T_hot, "T_cold": T_cold}
+
+
+def natural_convection_desc(N, Ra, Pr, T_hot=1.0, T_cold=0.0, storage="f64"):
+    """Differentially heated cavity (thermal.jl:161-238, fused_thermal_2d.jl:178-207):
+    hot west / cold east anti-bounce-back, adiabatic south/north, fused-kernel coupling."""
+    dT = T_hot - T_cold
+    nu = 0.05
+    a = nu / Pr
+    beta_g = Ra * nu * a / (dT * float(N) ** 3)
+    faces = {"west": {"kind": "wall", "thermal": "dirichlet", "T": T_hot},
+             "east": {"kind": "wall", "thermal": "dirichlet", "T": T_cold},
+             "south": {"kind": "wall"}, "north": {"kind": "wall"}}
+    return make_desc("d2q9", N, N, 1, nu, storage=storage, collision="bgk", faces=faces, thermal=True,
+                     omega_T=1.0 / (3.0 * a + 0.5), beta_g=beta_g, T_ref=(T_hot + T_cold) / 2,
+                     coupling="fused")
+
+
+def natural_convection_initial_temperature(N, T_hot=1.0, T_cold=0.0):
+    """Horizontal linear profile + sine seed (thermal.jl:196-205)."""
+    dT = T_hot - T_cold
+    i = np.arange(1, N + 1)[:, None]
+    j = np.arange(1, N + 1)[None, :]
+    T = T_hot - dT * (i - 0.5) / N + (0.01 * dT) * np.sin(2 * np.pi * i / N) * np.sin(np.pi * j / N)
+    return np.asfortranarray(T)
+
+
+def run_natural_convection_2d(*, N=128, Ra=1e4, Pr=0.71, T_hot=1.0, T_cold=0.0, max_steps=50000,
+                              storage="f64", backend=None):
+    d = natural_convection_desc(N, Ra, Pr, T_hot, T_cold, storage)
+    with Lattice(d, _backend(backend)) as lat:
+        lat.init_temperature(natural_convection_initial_temperature(N, T_hot, T_cold))
+        lat.step(max_steps)
+        m = lat.moments(temperature=True)
+        Nu = lat.reduce("nusselt", "west")
+    return {"ρ": m["rho"], "ux": m["ux"], "uy": m["uy"], "Temp": m["Temp"], "Nu": Nu, "Ra": Ra, "Pr": Pr}
+
+
+def run_to_residual(lat: Lattice, tol=1e-7, check_every=100, max_steps=10_000_000, max_wall=math.inf):
+    """Step until r_u = ‖u^k − u^(k−1)‖₂/‖u^k‖₂ < tol, checked every `check_every` steps
+    (the stop rule of gnf_periodic_steady.jl:114-227).  Returns (steps, r_u, converged)."""
+    t0 = time.perf_counter()
+    steps, r = 0, math.inf
+    while steps < max_steps:
+        n = min(check_every, max_steps - steps)
+        r = lat.step_residual(n)
+        steps += n
+        if not math.isfinite(r):
+            return steps, r, False
+        if r < tol:
+            return steps, r, True
+        if time.perf_counter() - t0 > max_wall:
+            break
+    return steps, r, False

@@ -1,0 +1,218 @@
+"""Backend value and device-resident lattice, over the C ABI.
+
+`MI355XBackend` plays the role of the KernelAbstractions backend object that Kraken's
+drivers take as `backend=` (src/drivers/basic.jl:51-52, src/simulation_runner.jl:110-160):
+asking for it where no device is visible raises `ArgumentError`, never a silent CPU run
+(the reference's contract, test/test_simulation_runner.jl:327-400).
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, byref, c_double, c_int, c_int64, c_uint8, c_void_p
+
+import numpy as np
+
+from . import _lib as L
+
+
+class ArgumentError(ValueError):
+    """Invalid argument or unavailable backend (Julia's ArgumentError)."""
+
+
+def _dptr(a):
+    return None if a is None else a.ctypes.data_as(POINTER(c_double))
+
+
+class MI355XBackend:
+    """One HIP device + stream (a kmi_ctx).  Shared by every lattice built on it."""
+
+    _cache: dict = {}
+
+    def __new__(cls, device: int = 0):
+        if device in cls._cache:
+            return cls._cache[device]
+        self = super().__new__(cls)
+        lib = L.load()
+        ctx = c_void_p()
+        rc = lib.kmi_ctx_create(device, byref(ctx))
+        if rc != L.KMI_OK:
+            msg = lib.kmi_last_error(None).decode()
+            raise ArgumentError(f"Backend mi355x (device {device}) is unavailable: {msg}")
+        self.device = device
+        self.ctx = ctx
+        cls._cache[device] = self
+        return self
+
+    def __repr__(self):
+        return f"MI355XBackend(device={self.device})"
+
+    def synchronize(self):
+        L.check(L.load().kmi_ctx_synchronize(self.ctx), self.ctx)
+
+    @staticmethod
+    def device_count() -> int:
+        n = c_int()
+        L.check(L.load().kmi_device_count(byref(n)))
+        return n.value
+
+
+def make_desc(stencil, nx, ny, nz=1, nu=0.1, storage="f64", collision="bgk", faces=None,
+              thermal=False, omega=None, s_plus=None, s_minus=None, Lambda=3.0 / 16.0,
+              s_e=1.4, s_eps=1.4, s_q=1.2, omega_T=None, beta_g=0.0, T_ref=0.0,
+              coupling="split", bc_order=None) -> L.KmiDesc:
+    """Build a kmi_desc.  `faces` maps face name -> dict(kind=..., u=..., rho=..., T=...,
+    thermal='dirichlet'|'adiabatic')."""
+    st = {"d2q9": L.D2Q9, "d3q19": L.D3Q19}.get(str(stencil).lower(), stencil)
+    d = L.desc_init(st, nx, ny, nz, nu)
+    d.storage = L.STORAGE_NAMES[storage] if isinstance(storage, str) else storage
+    d.collision = L.COLLISION_NAMES[collision] if isinstance(collision, str) else collision
+    d.thermal = 1 if thermal else 0
+    if omega is not None:
+        d.omega = omega
+    sp, sm = L.trt_rates(nu, Lambda)
+    d.s_plus = sp if s_plus is None else s_plus
+    d.s_minus = sm if s_minus is None else s_minus
+    d.s_e, d.s_eps, d.s_q = s_e, s_eps, s_q
+    d.s_nu = 1.0 / (3.0 * nu + 0.5)
+    d.omega_T = d.omega if omega_T is None else omega_T
+    d.beta_g = beta_g
+    d.T_ref = T_ref
+    d.coupling = L.COUPLING_FUSED if coupling == "fused" else L.COUPLING_SPLIT
+    kinds = {"periodic": L.FACE_PERIODIC, "wall": L.FACE_WALL, "moving_wall": L.FACE_MOVING_WALL,
+             "zouhe_velocity": L.FACE_ZOUHE_VELOCITY, "zouhe_pressure": L.FACE_ZOUHE_PRESSURE}
+    for name, spec in (faces or {}).items():
+        f = d.faces[L.FACE_NAMES[name]]
+        k = spec.get("kind", "wall")
+        f.kind = kinds[k] if isinstance(k, str) else k
+        u = list(spec.get("u", (0.0, 0.0, 0.0))) + [0.0, 0.0, 0.0]
+        for c in range(3):
+            f.u[c] = float(u[c])
+        f.rho = float(spec.get("rho", 1.0))
+        f.T = float(spec.get("T", 0.0))
+        f.thermal = L.THERMAL_DIRICHLET if spec.get("thermal") == "dirichlet" else L.THERMAL_ADIABATIC
+    if bc_order:
+        d.n_bc_order = len(bc_order)
+        for n, name in enumerate(bc_order):
+            d.bc_order[n] = L.FACE_NAMES[name] if isinstance(name, str) else name
+    return d
+
+
+class Lattice:
+    """A device-resident lattice (kmi_lattice).  Host arrays use the reference layout
+    f[i, j, (k,) q]: numpy arrays of shape (nx, ny, Q) / (nx, ny, nz, Q) in Fortran order."""
+
+    def __init__(self, desc: L.KmiDesc, backend: MI355XBackend | None = None):
+        self.backend = backend or MI355XBackend()
+        self.desc = desc
+        self.Q = 9 if desc.stencil == L.D2Q9 else 19
+        self.shape = (desc.nx, desc.ny) if self.Q == 9 else (desc.nx, desc.ny, desc.nz)
+        self.ncells = int(np.prod(self.shape))
+        lib = L.load()
+        h = c_void_p()
+        L.check(lib.kmi_lattice_create(self.backend.ctx, byref(desc), byref(h)), self.backend.ctx)
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.load().kmi_lattice_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _chk(self, rc):
+        L.check(rc, self.backend.ctx)
+
+    # --- populations ---
+    def upload(self, f, field="f"):
+        fld = L.FIELD_G if field == "g" else L.FIELD_F
+        a = np.asarray(f)
+        dt = L.HOST_F32 if a.dtype == np.float32 else L.HOST_F64
+        a = np.asfortranarray(a, dtype=np.float32 if dt == L.HOST_F32 else np.float64)
+        if a.size != self.ncells * self.Q:
+            raise ArgumentError(f"population array has {a.size} values, expected {self.ncells * self.Q}")
+        self._chk(L.load().kmi_upload(self._h, fld, a.ctypes.data_as(c_void_p), dt))
+
+    def download(self, field="f", dtype=np.float64):
+        fld = L.FIELD_G if field == "g" else L.FIELD_F
+        out = np.empty(self.shape + (self.Q,), dtype=dtype, order="F")
+        dt = L.HOST_F32 if dtype == np.float32 else L.HOST_F64
+        self._chk(L.load().kmi_download(self._h, fld, out.ctypes.data_as(c_void_p), dt))
+        return out
+
+    def set_mask(self, is_solid):
+        if is_solid is None:
+            self._chk(L.load().kmi_set_mask(self._h, None))
+            return
+        m = np.asfortranarray(np.asarray(is_solid, dtype=np.uint8))
+        if m.size != self.ncells:
+            raise ArgumentError("mask size does not match the lattice")
+        self._chk(L.load().kmi_set_mask(self._h, m.ctypes.data_as(POINTER(c_uint8))))
+
+    def init_equilibrium(self, rho=None, ux=None, uy=None, uz=None):
+        arrs = [None if a is None else np.asfortranarray(np.broadcast_to(a, self.shape), dtype=np.float64)
+                for a in (rho, ux, uy, uz)]
+        self._chk(L.load().kmi_init_equilibrium(self._h, *[_dptr(a) for a in arrs]))
+
+    def init_temperature(self, T=None):
+        a = None if T is None else np.asfortranarray(np.broadcast_to(T, self.shape), dtype=np.float64)
+        self._chk(L.load().kmi_init_temperature(self._h, _dptr(a)))
+
+    # --- time stepping ---
+    def step(self, n=1):
+        self._chk(L.load().kmi_step(self._h, int(n)))
+
+    def step_timed(self, n):
+        ms = c_double()
+        self._chk(L.load().kmi_step_timed(self._h, int(n), byref(ms)))
+        return ms.value
+
+    def step_residual(self, n):
+        r = c_double()
+        self._chk(L.load().kmi_step_residual(self._h, int(n), byref(r)))
+        return r.value
+
+    @property
+    def steps(self):
+        s = c_int64()
+        self._chk(L.load().kmi_lattice_steps(self._h, byref(s)))
+        return s.value
+
+    @property
+    def device_bytes(self):
+        b = c_int64()
+        self._chk(L.load().kmi_lattice_device_bytes(self._h, byref(b)))
+        return b.value
+
+    # --- observables ---
+    def moments(self, temperature=False):
+        shp = self.shape
+        rho = np.empty(shp, order="F")
+        ux = np.empty(shp, order="F")
+        uy = np.empty(shp, order="F")
+        uz = np.empty(shp, order="F") if self.Q == 19 else None
+        T = np.empty(shp, order="F") if temperature else None
+        self._chk(L.load().kmi_moments(self._h, *[_dptr(a) for a in (rho, ux, uy, uz, T)]))
+        out = {"rho": rho, "ux": ux, "uy": uy}
+        if uz is not None:
+            out["uz"] = uz
+        if T is not None:
+            out["Temp"] = T
+        return out
+
+    def reduce(self, kind, arg=0):
+        out = (c_double * 3)()
+        k = {"mass": L.RED_MASS, "momentum": L.RED_MOMENTUM, "kinetic_energy": L.RED_KINETIC_ENERGY,
+             "thermal": L.RED_THERMAL, "nusselt": L.RED_NUSSELT, "max_speed": L.RED_MAX_SPEED}.get(kind, kind)
+        a = L.FACE_NAMES[arg] if isinstance(arg, str) else arg
+        self._chk(L.load().kmi_reduce(self._h, k, a, out))
+        return tuple(out) if k == L.RED_MOMENTUM else out[0]

@@ -1,0 +1,1126 @@
+// kraken_oracle.cpp — CPU restatement of Kraken.jl's stream–collide hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the oracle the MI355X library is checked
+// against: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+// load it, and only as the checker or the timed CPU baseline.  The product library
+// never links, loads or falls back to it.
+//
+// Every function restates one reference kernel (paths relative to the Kraken.jl
+// tree) operation for operation, in the same floating-point evaluation order, so
+// that a build with -ffp-contract=off reproduces Julia's IEEE results (Julia does not
+// contract a*b+c without muladd/@fastmath).  Arrays use the reference's column-major
+// layout f[i, j, (k,) q] with 0-based i, j, k, q here.
+//
+// Pinning (see tests/test_oracle_*.py): lattice identities, equilibrium moments,
+// delta-streaming, bounce-back / Ladd / Zou–He closed forms, BGK/TRT/MRT relaxation
+// of the (anti)symmetric and moment-space modes, TRT≡BGK at Λ=(1/s−½)², rest-state
+// fixed point, exact Couette, Poiseuille and Taylor–Green decay, Ghia Re=100 and the
+// reference's Ra=1e3 Nusselt number.  Equality of each generic descriptor-driven
+// step with the restated reference kernel chain is checked bit for bit.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../include/kraken_mi355x.h"
+
+namespace ko {
+
+// ----------------------------------------------------------------------------------
+// Lattice constants — src/lattice/d2q9.jl:17-29, src/lattice/d3q19.jl:13-47
+// (0-based here: q_here = q_reference - 1).
+// ----------------------------------------------------------------------------------
+static const double W9[9] = {4.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0,
+                             1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
+static const int CX9[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+static const int CY9[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+static const int OPP9[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
+
+static const double W19[19] = {1.0 / 3.0,  1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0,
+                               1.0 / 18.0, 1.0 / 18.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
+                               1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
+                               1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
+static const int CX19[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+static const int CY19[19] = {0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1};
+static const int CZ19[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1};
+static const int OPP19[19] = {0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15};
+
+// 6 w_q as written in the reference's Ladd term (stream_periodic_3d.jl:126-127:
+// T(1/6) for the w = 1/36 edge populations); axis populations use 6/18 and 6/9.
+static double six_w(double w) {
+  if (w == 1.0 / 36.0) return 1.0 / 6.0;
+  if (w == 1.0 / 18.0) return 1.0 / 3.0;
+  if (w == 1.0 / 9.0) return 2.0 / 3.0;
+  return 2.0;  // rest population never crosses a face
+}
+
+// feq as written in src/kernels/equilibrium_helpers.jl:8-50 and
+// equilibrium_helpers_3d.jl:26-100: w ρ (1 + 3cu + 4.5cu·cu − 1.5usq), left to right.
+template <class T>
+static inline T feq(double w, T rho, T cu, T usq) {
+  return T(w) * rho * (T(1) + T(3) * cu + T(4.5) * cu * cu - T(1.5) * usq);
+}
+
+// c·u in the reference's written form: first non-zero component, then ± the others
+// (e.g. `-ux + uy`, `ux - uz`).
+template <class T>
+static inline T cdotu(int cx, int cy, int cz, T ux, T uy, T uz) {
+  T acc = T(0);
+  bool first = true;
+  auto add = [&](int c, T u) {
+    if (c == 0) return;
+    if (first) {
+      acc = (c > 0) ? u : -u;
+      first = false;
+    } else {
+      acc = (c > 0) ? acc + u : acc - u;
+    }
+  };
+  add(cx, ux);
+  add(cy, uy);
+  add(cz, uz);
+  return acc;
+}
+
+static inline size_t I2(int64_t i, int64_t j, int64_t nx) { return size_t(i + nx * j); }
+static inline size_t I3(int64_t i, int64_t j, int64_t k, int64_t nx, int64_t ny) {
+  return size_t(i + nx * (j + ny * k));
+}
+
+// ----------------------------------------------------------------------------------
+// D2Q9 moments — equilibrium_helpers.jl:54-61 / macroscopic.jl:5-18
+// ----------------------------------------------------------------------------------
+template <class T>
+static inline void moments9(const T* f, T& rho, T& ux, T& uy) {
+  rho = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8];
+  T inv = T(1) / rho;
+  ux = (f[1] - f[3] + f[5] - f[6] - f[7] + f[8]) * inv;
+  uy = (f[2] - f[4] + f[5] + f[6] - f[7] - f[8]) * inv;
+}
+
+// D3Q19 moments — equilibrium_helpers_3d.jl:102-112 / macroscopic.jl:42-61
+template <class T>
+static inline void moments19(const T* f, T& rho, T& ux, T& uy, T& uz) {
+  rho = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8] + f[9] + f[10] +
+        f[11] + f[12] + f[13] + f[14] + f[15] + f[16] + f[17] + f[18];
+  T inv = T(1) / rho;
+  ux = (f[1] - f[2] + f[7] - f[8] + f[9] - f[10] + f[11] - f[12] + f[13] - f[14]) * inv;
+  uy = (f[3] - f[4] + f[7] + f[8] - f[9] - f[10] + f[15] - f[16] + f[17] - f[18]) * inv;
+  uz = (f[5] - f[6] + f[11] + f[12] - f[13] - f[14] + f[15] + f[16] - f[17] - f[18]) * inv;
+}
+
+// ----------------------------------------------------------------------------------
+// Per-cell collisions (operate on a register copy fp[Q], write fo[Q])
+// ----------------------------------------------------------------------------------
+// BGK D2Q9 — collide_stream_2d.jl:105-131 (and fused_bgk_2d.jl:38-56)
+template <class T>
+static void bgk9(const T* fp, T* fo, T omega, T& rho, T& ux, T& uy) {
+  moments9(fp, rho, ux, uy);
+  T usq = ux * ux + uy * uy;
+  for (int q = 0; q < 9; ++q) {
+    T cu = cdotu<T>(CX9[q], CY9[q], 0, ux, uy, T(0));
+    T fe = (q == 0) ? T(W9[0]) * rho * (T(1) - T(1.5) * usq) : feq<T>(W9[q], rho, cu, usq);
+    fo[q] = fp[q] - omega * (fp[q] - fe);
+  }
+}
+
+// TRT D2Q9 — fused_trt_2d.jl:137-170: f* = f − a(f−feq) − b(f̄−feq̄)
+template <class T>
+static void trt9(const T* fp, T* fo, T sp, T sm, T& rho, T& ux, T& uy) {
+  moments9(fp, rho, ux, uy);
+  T usq = ux * ux + uy * uy;
+  T fe[9];
+  fe[0] = T(W9[0]) * rho * (T(1) - T(1.5) * usq);
+  for (int q = 1; q < 9; ++q)
+    fe[q] = feq<T>(W9[q], rho, cdotu<T>(CX9[q], CY9[q], 0, ux, uy, T(0)), usq);
+  T a = (sp + sm) * T(0.5);
+  T b = (sp - sm) * T(0.5);
+  fo[0] = fp[0] - sp * (fp[0] - fe[0]);
+  for (int q = 1; q < 9; ++q) {
+    int o = OPP9[q];
+    fo[q] = fp[q] - a * (fp[q] - fe[q]) - b * (fp[o] - fe[o]);
+  }
+}
+
+// MRT D2Q9 (Lallemand–Luo) — collide_mrt_2d.jl:56-107, hand-coded M and M^-1
+template <class T>
+static void mrt9(const T* fp, T* fo, T s_e, T s_eps, T s_q, T s_nu) {
+  T f1 = fp[0], f2 = fp[1], f3 = fp[2], f4 = fp[3], f5 = fp[4], f6 = fp[5], f7 = fp[6],
+    f8 = fp[7], f9 = fp[8];
+  T rho = f1 + f2 + f3 + f4 + f5 + f6 + f7 + f8 + f9;
+  T e = -T(4) * f1 - f2 - f3 - f4 - f5 + T(2) * (f6 + f7 + f8 + f9);
+  T eps = T(4) * f1 - T(2) * (f2 + f3 + f4 + f5) + f6 + f7 + f8 + f9;
+  T jx = f2 - f4 + f6 - f7 - f8 + f9;
+  T qx = -T(2) * f2 + T(2) * f4 + f6 - f7 - f8 + f9;
+  T jy = f3 - f5 + f6 + f7 - f8 - f9;
+  T qy = -T(2) * f3 + T(2) * f5 + f6 + f7 - f8 - f9;
+  T pxx = f2 - f3 + f4 - f5;
+  T pxy = f6 - f7 + f8 - f9;
+  T inv = T(1) / rho;
+  T ux = jx * inv;
+  T uy = jy * inv;
+  T usq = ux * ux + uy * uy;
+  T e_eq = -T(2) * rho + T(3) * rho * usq;
+  T eps_eq = rho - T(3) * rho * usq;
+  T qx_eq = -rho * ux;
+  T qy_eq = -rho * uy;
+  T pxx_eq = rho * (ux * ux - uy * uy);
+  T pxy_eq = rho * ux * uy;
+  T es = e - s_e * (e - e_eq);
+  T ep = eps - s_eps * (eps - eps_eq);
+  T qxs = qx - s_q * (qx - qx_eq);
+  T qys = qy - s_q * (qy - qy_eq);
+  T ps = pxx - s_nu * (pxx - pxx_eq);
+  T pxys = pxy - s_nu * (pxy - pxy_eq);
+  T r = rho, jxs = jx, jys = jy;
+  fo[0] = T(1.0 / 9) * r - T(1.0 / 9) * es + T(1.0 / 9) * ep;
+  fo[1] = T(1.0 / 9) * r - T(1.0 / 36) * es - T(1.0 / 18) * ep + T(1.0 / 6) * jxs -
+          T(1.0 / 6) * qxs + T(1.0 / 4) * ps;
+  fo[2] = T(1.0 / 9) * r - T(1.0 / 36) * es - T(1.0 / 18) * ep + T(1.0 / 6) * jys -
+          T(1.0 / 6) * qys - T(1.0 / 4) * ps;
+  fo[3] = T(1.0 / 9) * r - T(1.0 / 36) * es - T(1.0 / 18) * ep - T(1.0 / 6) * jxs +
+          T(1.0 / 6) * qxs + T(1.0 / 4) * ps;
+  fo[4] = T(1.0 / 9) * r - T(1.0 / 36) * es - T(1.0 / 18) * ep - T(1.0 / 6) * jys +
+          T(1.0 / 6) * qys - T(1.0 / 4) * ps;
+  fo[5] = T(1.0 / 9) * r + T(1.0 / 18) * es + T(1.0 / 36) * ep + T(1.0 / 6) * jxs +
+          T(1.0 / 12) * qxs + T(1.0 / 6) * jys + T(1.0 / 12) * qys + T(1.0 / 4) * pxys;
+  fo[6] = T(1.0 / 9) * r + T(1.0 / 18) * es + T(1.0 / 36) * ep - T(1.0 / 6) * jxs -
+          T(1.0 / 12) * qxs + T(1.0 / 6) * jys + T(1.0 / 12) * qys - T(1.0 / 4) * pxys;
+  fo[7] = T(1.0 / 9) * r + T(1.0 / 18) * es + T(1.0 / 36) * ep - T(1.0 / 6) * jxs -
+          T(1.0 / 12) * qxs - T(1.0 / 6) * jys - T(1.0 / 12) * qys + T(1.0 / 4) * pxys;
+  fo[8] = T(1.0 / 9) * r + T(1.0 / 18) * es + T(1.0 / 36) * ep + T(1.0 / 6) * jxs +
+          T(1.0 / 12) * qxs - T(1.0 / 6) * jys - T(1.0 / 12) * qys - T(1.0 / 4) * pxys;
+}
+
+// BGK D3Q19 — collide_stream_3d.jl:63-95 (same order as feq_3d)
+template <class T>
+static void bgk19(const T* fp, T* fo, T omega, T& rho, T& ux, T& uy, T& uz) {
+  moments19(fp, rho, ux, uy, uz);
+  T usq = ux * ux + uy * uy + uz * uz;
+  for (int q = 0; q < 19; ++q) {
+    T fe = (q == 0) ? T(W19[0]) * rho * (T(1) - T(1.5) * usq)
+                    : feq<T>(W19[q], rho, cdotu<T>(CX19[q], CY19[q], CZ19[q], ux, uy, uz), usq);
+    fo[q] = fp[q] - omega * (fp[q] - fe);
+  }
+}
+
+// TRT D3Q19 — dsl/bricks_3d.jl:168-229 (Moments_3D + CollideTRTDirect_3D)
+template <class T>
+static void trt19(const T* fp, T* fo, T sp, T sm, T& rho, T& ux, T& uy, T& uz) {
+  moments19(fp, rho, ux, uy, uz);
+  T usq = ux * ux + uy * uy + uz * uz;
+  T fe[19];
+  fe[0] = T(W19[0]) * rho * (T(1) - T(1.5) * usq);
+  for (int q = 1; q < 19; ++q)
+    fe[q] = feq<T>(W19[q], rho, cdotu<T>(CX19[q], CY19[q], CZ19[q], ux, uy, uz), usq);
+  T a = (sp + sm) * T(0.5);
+  T b = (sp - sm) * T(0.5);
+  fo[0] = fp[0] - sp * (fp[0] - fe[0]);
+  for (int q = 1; q < 19; ++q) {
+    int o = OPP19[q];
+    fo[q] = fp[q] - a * (fp[q] - fe[q]) - b * (fp[o] - fe[o]);
+  }
+}
+
+// Thermal BGK on g — thermal_2d.jl:16-60 / fused_thermal_2d.jl:77-112:
+// geq = w T (1 + 3 c·u), no u² term.
+template <class T>
+static void thermal9(const T* gp, T* go, T ux, T uy, T omT) {
+  T Temp = gp[0] + gp[1] + gp[2] + gp[3] + gp[4] + gp[5] + gp[6] + gp[7] + gp[8];
+  T t3 = T(3);
+  T geq;
+  geq = T(4.0 / 9.0) * Temp;
+  go[0] = gp[0] - omT * (gp[0] - geq);
+  geq = T(1.0 / 9.0) * Temp * (T(1) + t3 * ux);
+  go[1] = gp[1] - omT * (gp[1] - geq);
+  geq = T(1.0 / 9.0) * Temp * (T(1) + t3 * uy);
+  go[2] = gp[2] - omT * (gp[2] - geq);
+  geq = T(1.0 / 9.0) * Temp * (T(1) - t3 * ux);
+  go[3] = gp[3] - omT * (gp[3] - geq);
+  geq = T(1.0 / 9.0) * Temp * (T(1) - t3 * uy);
+  go[4] = gp[4] - omT * (gp[4] - geq);
+  geq = T(1.0 / 36.0) * Temp * (T(1) + t3 * (ux + uy));
+  go[5] = gp[5] - omT * (gp[5] - geq);
+  geq = T(1.0 / 36.0) * Temp * (T(1) + t3 * (-ux + uy));
+  go[6] = gp[6] - omT * (gp[6] - geq);
+  geq = T(1.0 / 36.0) * Temp * (T(1) + t3 * (-ux - uy));
+  go[7] = gp[7] - omT * (gp[7] - geq);
+  geq = T(1.0 / 36.0) * Temp * (T(1) + t3 * (ux - uy));
+  go[8] = gp[8] - omT * (gp[8] - geq);
+}
+
+// BGK + Guo Boussinesq forcing — thermal_2d.jl:421-480 / fused_thermal_2d.jl:116-157.
+// ρ, ux, uy are the force-shifted moments computed by the caller.
+template <class T>
+static void guo9(const T* fp, T* fo, T rho, T ux, T uy, T fy, T om) {
+  T fx = T(0);
+  T usq = ux * ux + uy * uy;
+  T guo = T(1) - om / T(2);
+  T t3 = T(3);
+  T Sq;
+  Sq = T(4.0 / 9.0) * ((-ux) * fx + (-uy) * fy) * t3;
+  fo[0] = fp[0] - om * (fp[0] - T(4.0 / 9.0) * rho * (T(1) - T(1.5) * usq)) + guo * Sq;
+  Sq = T(1.0 / 9.0) * ((T(1) - ux) * fx + (-uy) * fy) * t3 + T(1.0 / 9.0) * ux * fx * T(9);
+  fo[1] = fp[1] - om * (fp[1] - feq<T>(1.0 / 9.0, rho, ux, usq)) + guo * Sq;
+  Sq = T(1.0 / 9.0) * ((-ux) * fx + (T(1) - uy) * fy) * t3 + T(1.0 / 9.0) * uy * fy * T(9);
+  fo[2] = fp[2] - om * (fp[2] - feq<T>(1.0 / 9.0, rho, uy, usq)) + guo * Sq;
+  Sq = T(1.0 / 9.0) * ((-T(1) - ux) * fx + (-uy) * fy) * t3 + T(1.0 / 9.0) * ux * fx * T(9);
+  fo[3] = fp[3] - om * (fp[3] - feq<T>(1.0 / 9.0, rho, -ux, usq)) + guo * Sq;
+  Sq = T(1.0 / 9.0) * ((-ux) * fx + (-T(1) - uy) * fy) * t3 + T(1.0 / 9.0) * uy * fy * T(9);
+  fo[4] = fp[4] - om * (fp[4] - feq<T>(1.0 / 9.0, rho, -uy, usq)) + guo * Sq;
+  Sq = T(1.0 / 36.0) * ((T(1) - ux) * fx + (T(1) - uy) * fy) * t3 +
+       T(1.0 / 36.0) * (ux + uy) * (fx + fy) * T(9);
+  fo[5] = fp[5] - om * (fp[5] - feq<T>(1.0 / 36.0, rho, ux + uy, usq)) + guo * Sq;
+  Sq = T(1.0 / 36.0) * ((-T(1) - ux) * fx + (T(1) - uy) * fy) * t3 +
+       T(1.0 / 36.0) * (-ux + uy) * (-fx + fy) * T(9);
+  fo[6] = fp[6] - om * (fp[6] - feq<T>(1.0 / 36.0, rho, -ux + uy, usq)) + guo * Sq;
+  Sq = T(1.0 / 36.0) * ((-T(1) - ux) * fx + (-T(1) - uy) * fy) * t3 +
+       T(1.0 / 36.0) * (-ux - uy) * (-fx - fy) * T(9);
+  fo[7] = fp[7] - om * (fp[7] - feq<T>(1.0 / 36.0, rho, -ux - uy, usq)) + guo * Sq;
+  Sq = T(1.0 / 36.0) * ((T(1) - ux) * fx + (-T(1) - uy) * fy) * t3 +
+       T(1.0 / 36.0) * (ux - uy) * (fx - fy) * T(9);
+  fo[8] = fp[8] - om * (fp[8] - feq<T>(1.0 / 36.0, rho, ux - uy, usq)) + guo * Sq;
+}
+
+// ----------------------------------------------------------------------------------
+// Reference kernels restated one for one (whole-array operations)
+// ----------------------------------------------------------------------------------
+
+// stream_2d_kernel! — collide_stream_2d.jl:75-101: pull, halfway BB at all four edges.
+template <class T>
+static void stream_2d(T* fo, const T* fi, int64_t nx, int64_t ny) {
+  const size_t P = size_t(nx * ny);
+#pragma omp parallel for
+  for (int64_t j = 0; j < ny; ++j)
+    for (int64_t i = 0; i < nx; ++i) {
+      size_t c = I2(i, j, nx);
+      for (int q = 0; q < 9; ++q) {
+        int64_t si = i - CX9[q], sj = j - CY9[q];
+        bool ok = si >= 0 && si < nx && sj >= 0 && sj < ny;
+        fo[c + q * P] = ok ? fi[I2(si, sj, nx) + q * P] : fi[c + OPP9[q] * P];
+      }
+    }
+}
+
+// stream_fully_periodic_2d_kernel! — stream_periodic_2d.jl:79-98
+template <class T>
+static void stream_fully_periodic_2d(T* fo, const T* fi, int64_t nx, int64_t ny) {
+  const size_t P = size_t(nx * ny);
+#pragma omp parallel for
+  for (int64_t j = 0; j < ny; ++j)
+    for (int64_t i = 0; i < nx; ++i)
+      for (int q = 0; q < 9; ++q) {
+        int64_t si = (i - CX9[q] + nx) % nx, sj = (j - CY9[q] + ny) % ny;
+        fo[I2(i, j, nx) + q * P] = fi[I2(si, sj, nx) + q * P];
+      }
+}
+
+// stream_periodic_x_wall_y_2d_kernel! — stream_periodic_2d.jl:5-55
+template <class T>
+static void stream_periodic_x_wall_y_2d(T* fo, const T* fi, int64_t nx, int64_t ny) {
+  const size_t P = size_t(nx * ny);
+#pragma omp parallel for
+  for (int64_t j = 0; j < ny; ++j)
+    for (int64_t i = 0; i < nx; ++i) {
+      size_t c = I2(i, j, nx);
+      for (int q = 0; q < 9; ++q) {
+        int64_t si = (i - CX9[q] + nx) % nx, sj = j - CY9[q];
+        bool ok = sj >= 0 && sj < ny;
+        fo[c + q * P] = ok ? fi[I2(si, sj, nx) + q * P] : fi[c + OPP9[q] * P];
+      }
+    }
+}
+
+// collide_2d_kernel! — collide_stream_2d.jl:105-131 (solid: swap pairs in place)
+template <class T>
+static void collide_2d(T* f, const uint8_t* solid, int64_t nx, int64_t ny, T omega) {
+  const size_t P = size_t(nx * ny);
+#pragma omp parallel for
+  for (int64_t j = 0; j < ny; ++j)
+    for (int64_t i = 0; i < nx; ++i) {
+      size_t c = I2(i, j, nx);
+      T fp[9], fo[9], r, ux, uy;
+      for (int q = 0; q < 9; ++q) fp[q] = f[c + q * P];
+      if (solid && solid[c]) {
+        for (int q = 0; q < 9; ++q) fo[q] = fp[OPP9[q]];
+      } else {
+        bgk9(fp, fo, omega, r, ux, uy);
+      }
+      for (int q = 0; q < 9; ++q) f[c + q * P] = fo[q];
+    }
+}
+
+// In-place TRT collide with the arithmetic of fused_trt_step_kernel! (fused_trt_2d.jl:125-171);
+// the fused kernel equals stream_2d followed by this (checked bit for bit in the tests).
+template <class T>
+static void collide_trt_2d(T* f, const uint8_t* solid, int64_t nx, int64_t ny, T sp, T sm) {
+  const size_t P = size_t(nx * ny);
+#pragma omp parallel for
+  for (int64_t j = 0; j < ny; ++j)
+    for (int64_t i = 0; i < nx; ++i) {
+      size_t c = I2(i, j, nx);
+      T fp[9], fo[9], r, ux, uy;
+      for (int q = 0; q < 9; ++q) fp[q] = f[c + q * P];
+      if (solid && solid[c]) {
+        for (int q = 0; q < 9; ++q) fo[q] = fp[OPP9[q]];
+      } else {
+        trt9(fp, fo, sp, sm, r, ux, uy);
+      }
+      for (int q = 0; q < 9; ++q) f[c + q * P] = fo[q];
+    }
+}
+
+// collide_mrt_2d_kernel! — collide_mrt_2d.jl:43-109
+template <class T>
+static void collide_mrt_2d(T* f, const uint8_t* solid, int64_t nx, int64_t ny, T s_e, T s_eps,
+                           T s_q, T s_nu) {
+  const size_t P = size_t(nx * ny);
+#pragma omp parallel for
+  for (int64_t j = 0; j < ny; ++j)
+    for (int64_t i = 0; i < nx; ++i) {
+      size_t c = I2(i, j, nx);
+      T fp[9], fo[9];
+      for (int q = 0; q < 9; ++q) fp[q] = f[c + q * P];
+      if (solid && solid[c]) {
+        for (int q = 0; q < 9; ++q) fo[q] = fp[OPP9[q]];
+      } else {
+        mrt9(fp, fo, s_e, s_eps, s_q, s_nu);
+      }
+      for (int q = 0; q < 9; ++q) f[c + q * P] = fo[q];
+    }
+}
+
+// fused_bgk_step_kernel! — fused_bgk_2d.jl:8-59 / fused_trt_step_kernel! — fused_trt_2d.jl:101-173
+template <class T>
+static void fused_2d(T* fo, const T* fi, T* rho, T* ux, T* uy, const uint8_t* solid, int64_t nx,
+                     int64_t ny, bool trt, T a1, T a2) {
+  const size_t P = size_t(nx * ny);
+#pragma omp parallel for
+  for (int64_t j = 0; j < ny; ++j)
+    for (int64_t i = 0; i < nx; ++i) {
+      size_t c = I2(i, j, nx);
+      T fp[9], o[9], r, u, v;
+      for (int q = 0; q < 9; ++q) {
+        int64_t si = i - CX9[q], sj = j - CY9[q];
+        bool ok = si >= 0 && si < nx && sj >= 0 && sj < ny;
+        fp[q] = ok ? fi[I2(si, sj, nx) + q * P] : fi[c + OPP9[q] * P];
+      }
+      if (solid && solid[c]) {
+        for (int q = 0; q < 9; ++q) o[q] = fp[OPP9[q]];
+        r = T(1);
+        u = T(0);
+        v = T(0);
+      } else if (trt) {
+        trt9(fp, o, a1, a2, r, u, v);
+      } else {
+        bgk9(fp, o, a1, r, u, v);
+      }
+      for (int q = 0; q < 9; ++q) fo[c + q * P] = o[q];
+      if (rho) rho[c] = r;
+      if (ux) ux[c] = u;
+      if (uy) uy[c] = v;
+    }
+}
+
+// compute_macroscopic_2d_kernel! — macroscopic.jl:5-18
+template <class T>
+static void macroscopic_2d(T* rho, T* ux, T* uy, const T* f, int64_t nx, int64_t ny) {
+  const size_t P = size_t(nx * ny);
+#pragma omp parallel for
+  for (int64_t j = 0; j < ny; ++j)
+    for (int64_t i = 0; i < nx; ++i) {
+      size_t c = I2(i, j, nx);
+      T fp[9], r, u, v;
+      for (int q = 0; q < 9; ++q) fp[q] = f[c + q * P];
+      moments9(fp, r, u, v);
+      if (rho) rho[c] = r;
+      if (ux) ux[c] = u;
+      if (uy) uy[c] = v;
+    }
+}
+
+// zou_he_velocity_north_2d_kernel! — boundary_2d.jl:13-44 (uy_wall = 0)
+template <class T>
+static void zou_he_north(T* f, T u, int64_t nx, int64_t ny) {
+  const size_t P = size_t(nx * ny);
+  for (int64_t i = 0; i < nx; ++i) {
+    size_t c = I2(i, ny - 1, nx);
+    T f1 = f[c], f2 = f[c + P], f3 = f[c + 2 * P], f4 = f[c + 3 * P], f6 = f[c + 5 * P],
+      f7 = f[c + 6 * P];
+    T rw = (f1 + f2 + f4 + T(2) * (f3 + f6 + f7)) / (T(1) + u * T(0));
+    f[c + 4 * P] = f3;
+    f[c + 8 * P] = f7 - T(0.5) * (f2 - f4) + T(0.5) * rw * u;
+    f[c + 7 * P] = f6 + T(0.5) * (f2 - f4) - T(0.5) * rw * u;
+  }
+}
+
+// zou_he_velocity_south_2d_kernel! — boundary_2d.jl:135-150
+template <class T>
+static void zou_he_south(T* f, T u, int64_t nx, int64_t ny) {
+  const size_t P = size_t(nx * ny);
+  for (int64_t i = 0; i < nx; ++i) {
+    size_t c = I2(i, 0, nx);
+    T f1 = f[c], f2 = f[c + P], f4 = f[c + 3 * P], f5 = f[c + 4 * P], f8 = f[c + 7 * P],
+      f9 = f[c + 8 * P];
+    T rw = f1 + f2 + f4 + T(2) * (f5 + f8 + f9);
+    f[c + 2 * P] = f5;
+    f[c + 5 * P] = f8 - T(0.5) * (f2 - f4) + T(0.5) * rw * u;
+    f[c + 6 * P] = f9 + T(0.5) * (f2 - f4) - T(0.5) * rw * u;
+  }
+}
+
+// zou_he_velocity_west_2d_kernel! — boundary_2d.jl:171-185
+template <class T>
+static void zou_he_west(T* f, T u, int64_t nx, int64_t ny) {
+  const size_t P = size_t(nx * ny);
+  for (int64_t j = 0; j < ny; ++j) {
+    size_t c = I2(0, j, nx);
+    T f1 = f[c], f3 = f[c + 2 * P], f4 = f[c + 3 * P], f5 = f[c + 4 * P], f7 = f[c + 6 * P],
+      f8 = f[c + 7 * P];
+    T rw = (f1 + f3 + f5 + T(2) * (f4 + f7 + f8)) / (T(1) - u);
+    f[c + P] = f4 + T(2.0 / 3.0) * rw * u;
+    f[c + 5 * P] = f8 - T(0.5) * (f3 - f5) + T(1.0 / 6.0) * rw * u;
+    f[c + 8 * P] = f7 + T(0.5) * (f3 - f5) + T(1.0 / 6.0) * rw * u;
+  }
+}
+
+// zou_he_pressure_east_2d_kernel! — boundary_2d.jl:206-223
+template <class T>
+static void zou_he_pressure_east(T* f, T rho_out, int64_t nx, int64_t ny) {
+  const size_t P = size_t(nx * ny);
+  for (int64_t j = 0; j < ny; ++j) {
+    size_t c = I2(nx - 1, j, nx);
+    T f1 = f[c], f2 = f[c + P], f3 = f[c + 2 * P], f5 = f[c + 4 * P], f6 = f[c + 5 * P],
+      f9 = f[c + 8 * P];
+    T ux = -T(1) + (f1 + f3 + f5 + T(2) * (f2 + f6 + f9)) / rho_out;
+    f[c + 3 * P] = f2 - T(2.0 / 3.0) * rho_out * ux;
+    f[c + 6 * P] = f9 - T(0.5) * (f3 - f5) - T(1.0 / 6.0) * rho_out * ux;
+    f[c + 7 * P] = f6 + T(0.5) * (f3 - f5) - T(1.0 / 6.0) * rho_out * ux;
+  }
+}
+
+// zou_he_pressure_west_2d_kernel! — boundary_2d.jl:264-281
+template <class T>
+static void zou_he_pressure_west(T* f, T rho_in, int64_t nx, int64_t ny) {
+  const size_t P = size_t(nx * ny);
+  for (int64_t j = 0; j < ny; ++j) {
+    size_t c = I2(0, j, nx);
+    T f1 = f[c], f3 = f[c + 2 * P], f4 = f[c + 3 * P], f5 = f[c + 4 * P], f7 = f[c + 6 * P],
+      f8 = f[c + 7 * P];
+    T ux = T(1) - (f1 + f3 + f5 + T(2) * (f4 + f7 + f8)) / rho_in;
+    f[c + P] = f4 + T(2.0 / 3.0) * rho_in * ux;
+    f[c + 5 * P] = f8 - T(0.5) * (f3 - f5) + T(1.0 / 6.0) * rho_in * ux;
+    f[c + 8 * P] = f7 + T(0.5) * (f3 - f5) + T(1.0 / 6.0) * rho_in * ux;
+  }
+}
+
+// stream_3d_kernel! — collide_stream_3d.jl:5-43: pull, halfway BB at all six faces.
+// stream_periodic_xz_wall_y_3d_kernel! — stream_periodic_3d.jl:24-75 (px = pz = true);
+// PullHalfwayBB_3D{true} — dsl/bricks_3d.jl:106-136 (pz = true only).
+template <class T>
+static void stream_3d_flags(T* fo, const T* fi, int64_t nx, int64_t ny, int64_t nz, bool px,
+                            bool py, bool pz) {
+  const size_t P = size_t(nx * ny * nz);
+#pragma omp parallel for
+  for (int64_t k = 0; k < nz; ++k)
+    for (int64_t j = 0; j < ny; ++j)
+      for (int64_t i = 0; i < nx; ++i) {
+        size_t c = I3(i, j, k, nx, ny);
+        for (int q = 0; q < 19; ++q) {
+          int64_t si = i - CX19[q], sj = j - CY19[q], sk = k - CZ19[q];
+          bool ok = true;
+          if (px) si = (si + nx) % nx; else ok = ok && si >= 0 && si < nx;
+          if (py) sj = (sj + ny) % ny; else ok = ok && sj >= 0 && sj < ny;
+          if (pz) sk = (sk + nz) % nz; else ok = ok && sk >= 0 && sk < nz;
+          fo[c + q * P] = ok ? fi[I3(si, sj, sk, nx, ny) + q * P] : fi[c + OPP19[q] * P];
+        }
+      }
+}
+
+// stream_periodic_xz_movingwall_y_3d_kernel! — stream_periodic_3d.jl:106-166
+template <class T>
+static void stream_periodic_xz_movingwall_y_3d(T* fo, const T* fi, T Ub, T Ut, T rho_w,
+                                               int64_t nx, int64_t ny, int64_t nz) {
+  const size_t P = size_t(nx * ny * nz);
+  T cbot = T(1.0 / 6.0) * rho_w * Ub;
+  T ctop = T(1.0 / 6.0) * rho_w * Ut;
+#pragma omp parallel for
+  for (int64_t k = 0; k < nz; ++k)
+    for (int64_t j = 0; j < ny; ++j)
+      for (int64_t i = 0; i < nx; ++i) {
+        size_t c = I3(i, j, k, nx, ny);
+        for (int q = 0; q < 19; ++q) {
+          int64_t si = (i - CX19[q] + nx) % nx, sj = j - CY19[q], sk = (k - CZ19[q] + nz) % nz;
+          bool ok = sj >= 0 && sj < ny;
+          T v = ok ? fi[I3(si, sj, sk, nx, ny) + q * P] : fi[c + OPP19[q] * P];
+          if (!ok) {
+            if (q == 7) v = v + cbot;       // q8  (+x,+y) at j = 1
+            else if (q == 8) v = v - cbot;  // q9  (-x,+y) at j = 1
+            else if (q == 9) v = v + ctop;  // q10 (+x,-y) at j = Ny
+            else if (q == 10) v = v - ctop; // q11 (-x,-y) at j = Ny
+          }
+          fo[c + q * P] = v;
+        }
+      }
+}
+
+// collide_3d_kernel! — collide_stream_3d.jl:47-97
+template <class T>
+static void collide_3d(T* f, const uint8_t* solid, int64_t nx, int64_t ny, int64_t nz, T omega) {
+  const size_t P = size_t(nx * ny * nz);
+#pragma omp parallel for
+  for (int64_t k = 0; k < nz; ++k)
+    for (int64_t j = 0; j < ny; ++j)
+      for (int64_t i = 0; i < nx; ++i) {
+        size_t c = I3(i, j, k, nx, ny);
+        T fp[19], fo[19], r, u, v, w;
+        for (int q = 0; q < 19; ++q) fp[q] = f[c + q * P];
+        if (solid && solid[c]) {
+          for (int q = 0; q < 19; ++q) fo[q] = fp[OPP19[q]];
+        } else {
+          bgk19(fp, fo, omega, r, u, v, w);
+        }
+        for (int q = 0; q < 19; ++q) f[c + q * P] = fo[q];
+      }
+}
+
+// In-place D3Q19 TRT collide with the arithmetic of CollideTRTDirect_3D (bricks_3d.jl:178-229);
+// solid cells swap pairs like collide_3d_kernel!.
+template <class T>
+static void collide_trt_3d(T* f, const uint8_t* solid, int64_t nx, int64_t ny, int64_t nz, T sp,
+                           T sm) {
+  const size_t P = size_t(nx * ny * nz);
+#pragma omp parallel for
+  for (int64_t k = 0; k < nz; ++k)
+    for (int64_t j = 0; j < ny; ++j)
+      for (int64_t i = 0; i < nx; ++i) {
+        size_t c = I3(i, j, k, nx, ny);
+        T fp[19], fo[19], r, u, v, w;
+        for (int q = 0; q < 19; ++q) fp[q] = f[c + q * P];
+        if (solid && solid[c]) {
+          for (int q = 0; q < 19; ++q) fo[q] = fp[OPP19[q]];
+        } else {
+          trt19(fp, fo, sp, sm, r, u, v, w);
+        }
+        for (int q = 0; q < 19; ++q) f[c + q * P] = fo[q];
+      }
+}
+
+// Single pass PullHalfwayBB_3D (per-axis periodic wrap as stream_periodic_3d.jl:29-33)
+// + Moments_3D + CollideTRTDirect_3D + WriteMoments_3D — the DSL composition of
+// dsl/bricks_3d.jl:64-229, 476-483.  Used as the CPU baseline of the D3Q19 TRT case.
+template <class T>
+static void fused_trt_3d(T* fo, const T* fi, int64_t nx, int64_t ny, int64_t nz, bool px, bool py,
+                         bool pz, T sp, T sm) {
+  const size_t P = size_t(nx * ny * nz);
+#pragma omp parallel for
+  for (int64_t k = 0; k < nz; ++k)
+    for (int64_t j = 0; j < ny; ++j)
+      for (int64_t i = 0; i < nx; ++i) {
+        size_t c = I3(i, j, k, nx, ny);
+        T fp[19], o[19], r, u, v, w;
+        for (int q = 0; q < 19; ++q) {
+          int64_t si = i - CX19[q], sj = j - CY19[q], sk = k - CZ19[q];
+          bool ok = true;
+          if (px) si = (si + nx) % nx; else ok = ok && si >= 0 && si < nx;
+          if (py) sj = (sj + ny) % ny; else ok = ok && sj >= 0 && sj < ny;
+          if (pz) sk = (sk + nz) % nz; else ok = ok && sk >= 0 && sk < nz;
+          fp[q] = ok ? fi[I3(si, sj, sk, nx, ny) + q * P] : fi[c + OPP19[q] * P];
+        }
+        trt19(fp, o, sp, sm, r, u, v, w);
+        for (int q = 0; q < 19; ++q) fo[c + q * P] = o[q];
+      }
+}
+
+// compute_macroscopic_3d_kernel! — macroscopic.jl:42-61
+template <class T>
+static void macroscopic_3d(T* rho, T* ux, T* uy, T* uz, const T* f, int64_t nx, int64_t ny,
+                           int64_t nz) {
+  const size_t P = size_t(nx * ny * nz);
+#pragma omp parallel for
+  for (int64_t c = 0; c < int64_t(P); ++c) {
+    T fp[19], r, u, v, w;
+    for (int q = 0; q < 19; ++q) fp[q] = f[c + q * P];
+    moments19(fp, r, u, v, w);
+    if (rho) rho[c] = r;
+    if (ux) ux[c] = u;
+    if (uy) uy[c] = v;
+    if (uz) uz[c] = w;
+  }
+}
+
+// --- thermal kernels ---------------------------------------------------------------
+
+// collide_thermal_2d_kernel! — thermal_2d.jl:16-60 (u arrays from compute_macroscopic_2d!)
+template <class T>
+static void collide_thermal_2d(T* g, const T* ux, const T* uy, int64_t nx, int64_t ny, T omT) {
+  const size_t P = size_t(nx * ny);
+#pragma omp parallel for
+  for (int64_t c = 0; c < int64_t(P); ++c) {
+    T gp[9], go[9];
+    for (int q = 0; q < 9; ++q) gp[q] = g[c + q * P];
+    thermal9(gp, go, ux[c], uy[c], omT);
+    for (int q = 0; q < 9; ++q) g[c + q * P] = go[q];
+  }
+}
+
+// compute_temperature_2d_kernel! — thermal_2d.jl:113-119
+template <class T>
+static void compute_temperature_2d(T* Temp, const T* g, int64_t nx, int64_t ny) {
+  const size_t P = size_t(nx * ny);
+  for (int64_t c = 0; c < int64_t(P); ++c)
+    Temp[c] = g[c] + g[c + P] + g[c + 2 * P] + g[c + 3 * P] + g[c + 4 * P] + g[c + 5 * P] +
+              g[c + 6 * P] + g[c + 7 * P] + g[c + 8 * P];
+}
+
+// _abb_slot — thermal_2d.jl:175-178
+template <class T>
+static inline T abb_slot(T gq, T w2, T Tface, bool on_corner, T Tcorner, bool skip) {
+  T Tlink = on_corner ? Tcorner : Tface;
+  return (on_corner && skip) ? gq : -gq + w2 * Tlink;
+}
+
+// apply_fixed_temp_{south,north,west,east}_2d_kernel! — thermal_2d.jl:182-290, with corner
+// temperatures Tc[0..1] and skip flags sk[0..1] as passed by _apply_fixed_temp_faces_2d!.
+template <class T>
+static void fixed_temp_face(T* g, int face, T Tw, const T* Tc, const bool* sk, int64_t nx,
+                            int64_t ny) {
+  const size_t P = size_t(nx * ny);
+  T wa = T(2) * T(1.0 / 9.0);
+  T we = T(2) * T(1.0 / 36.0);
+  if (face == KMI_SOUTH || face == KMI_NORTH) {
+    int64_t j = (face == KMI_SOUTH) ? 0 : ny - 1;
+    // south: 3 / 6 (corner i=1) / 7 (corner i=Nx); north: 5 / 9 (i=1) / 8 (i=Nx)
+    int qa = (face == KMI_SOUTH) ? 2 : 4, q1 = (face == KMI_SOUTH) ? 5 : 8,
+        q2 = (face == KMI_SOUTH) ? 6 : 7;
+    for (int64_t i = 0; i < nx; ++i) {
+      size_t c = I2(i, j, nx);
+      g[c + qa * P] = -g[c + qa * P] + wa * Tw;
+      g[c + q1 * P] = abb_slot(g[c + q1 * P], we, Tw, i == 0, Tc[0], sk[0]);
+      g[c + q2 * P] = abb_slot(g[c + q2 * P], we, Tw, i == nx - 1, Tc[1], sk[1]);
+    }
+  } else {
+    int64_t i = (face == KMI_WEST) ? 0 : nx - 1;
+    // west: 2 / 6 (corner j=1) / 9 (corner j=Ny); east: 4 / 7 (j=1) / 8 (j=Ny)
+    int qa = (face == KMI_WEST) ? 1 : 3, q1 = (face == KMI_WEST) ? 5 : 6,
+        q2 = (face == KMI_WEST) ? 8 : 7;
+    for (int64_t j = 0; j < ny; ++j) {
+      size_t c = I2(i, j, nx);
+      g[c + qa * P] = -g[c + qa * P] + wa * Tw;
+      g[c + q1 * P] = abb_slot(g[c + q1 * P], we, Tw, j == 0, Tc[0], sk[0]);
+      g[c + q2 * P] = abb_slot(g[c + q2 * P], we, Tw, j == ny - 1, Tc[1], sk[1]);
+    }
+  }
+}
+
+// apply_fixed_temp_2d! / _apply_fixed_temp_faces_2d! — thermal_2d.jl:371-417:
+// corner links take the mean temperature and belong to the x-face.
+template <class T>
+static void apply_fixed_temp_2d(T* g, const bool* dir, const double* Tf, int64_t nx, int64_t ny) {
+  auto corner = [&](int a, int b) -> T {
+    return dir[b] ? (T(Tf[a]) + T(Tf[b])) / T(2) : T(Tf[a]);
+  };
+  const int order[4] = {KMI_WEST, KMI_EAST, KMI_SOUTH, KMI_NORTH};
+  for (int f : order) {
+    if (!dir[f]) continue;
+    T Tc[2];
+    bool sk[2];
+    if (f == KMI_WEST || f == KMI_EAST) {
+      Tc[0] = corner(f, KMI_SOUTH);
+      Tc[1] = corner(f, KMI_NORTH);
+      sk[0] = sk[1] = false;
+    } else {
+      Tc[0] = Tc[1] = T(Tf[f]);
+      sk[0] = dir[KMI_WEST];
+      sk[1] = dir[KMI_EAST];
+    }
+    fixed_temp_face(g, f, T(Tf[f]), Tc, sk, nx, ny);
+  }
+}
+
+// collide_boussinesq_2d_kernel! — thermal_2d.jl:421-480
+template <class T>
+static void collide_boussinesq_2d(T* f, const T* Temp, const uint8_t* solid, int64_t nx,
+                                  int64_t ny, T om, T beta_g, T T_ref) {
+  const size_t P = size_t(nx * ny);
+#pragma omp parallel for
+  for (int64_t c = 0; c < int64_t(P); ++c) {
+    T fp[9], fo[9];
+    for (int q = 0; q < 9; ++q) fp[q] = f[c + q * P];
+    if (solid && solid[c]) {
+      for (int q = 0; q < 9; ++q) fo[q] = fp[OPP9[q]];
+    } else {
+      T fy = beta_g * (Temp[c] - T_ref);
+      T fx = T(0);
+      T rho = fp[0] + fp[1] + fp[2] + fp[3] + fp[4] + fp[5] + fp[6] + fp[7] + fp[8];
+      T inv = T(1) / rho;
+      T ux = ((fp[1] - fp[3] + fp[5] - fp[6] - fp[7] + fp[8]) + fx / T(2)) * inv;
+      T uy = ((fp[2] - fp[4] + fp[5] + fp[6] - fp[7] - fp[8]) + fy / T(2)) * inv;
+      guo9(fp, fo, rho, ux, uy, fy, om);
+    }
+    for (int q = 0; q < 9; ++q) f[c + q * P] = fo[q];
+  }
+}
+
+// fused_natconv_step_kernel! — fused_thermal_2d.jl:178-207 (Dirichlet west/east, adiabatic
+// south/north, no solids).
+template <class T>
+static void fused_natconv_step(T* fo, const T* fi, T* go, const T* gi, T* Temp, int64_t nx,
+                               int64_t ny, T om_f, T om_T, T beta_g, T T_ref, T T_hot,
+                               T T_cold) {
+  const size_t P = size_t(nx * ny);
+#pragma omp parallel for
+  for (int64_t j = 0; j < ny; ++j)
+    for (int64_t i = 0; i < nx; ++i) {
+      size_t c = I2(i, j, nx);
+      T fp[9], gp[9], f2[9], g2[9];
+      for (int q = 0; q < 9; ++q) {
+        int64_t si = i - CX9[q], sj = j - CY9[q];
+        bool ok = si >= 0 && si < nx && sj >= 0 && sj < ny;
+        fp[q] = ok ? fi[I2(si, sj, nx) + q * P] : fi[c + OPP9[q] * P];
+        gp[q] = ok ? gi[I2(si, sj, nx) + q * P] : gi[c + OPP9[q] * P];
+      }
+      // apply_thermal_bc_cavity — fused_thermal_2d.jl:37-57
+      if (i == 0) {
+        gp[1] = -gp[1] + T(2) * T(1.0 / 9.0) * T_hot;
+        gp[5] = -gp[5] + T(2) * T(1.0 / 36.0) * T_hot;
+        gp[8] = -gp[8] + T(2) * T(1.0 / 36.0) * T_hot;
+      }
+      if (i == nx - 1) {
+        gp[3] = -gp[3] + T(2) * T(1.0 / 9.0) * T_cold;
+        gp[6] = -gp[6] + T(2) * T(1.0 / 36.0) * T_cold;
+        gp[7] = -gp[7] + T(2) * T(1.0 / 36.0) * T_cold;
+      }
+      T Tl = gp[0] + gp[1] + gp[2] + gp[3] + gp[4] + gp[5] + gp[6] + gp[7] + gp[8];
+      if (Temp) Temp[c] = Tl;
+      // macroscopic_boussinesq — fused_thermal_2d.jl:61-72
+      T fy = beta_g * (Tl - T_ref);
+      T rho = fp[0] + fp[1] + fp[2] + fp[3] + fp[4] + fp[5] + fp[6] + fp[7] + fp[8];
+      T inv = T(1) / rho;
+      T ux = (fp[1] - fp[3] + fp[5] - fp[6] - fp[7] + fp[8]) * inv;
+      T uy = ((fp[2] - fp[4] + fp[5] + fp[6] - fp[7] - fp[8]) + fy / T(2)) * inv;
+      thermal9(gp, g2, ux, uy, om_T);
+      guo9(fp, f2, rho, ux, uy, fy, om_f);
+      for (int q = 0; q < 9; ++q) {
+        go[c + q * P] = g2[q];
+        fo[c + q * P] = f2[q];
+      }
+    }
+}
+
+// ----------------------------------------------------------------------------------
+// Generic descriptor-driven step: the semantics the MI355X kernels implement.
+// Each piece is one of the restated reference kernels above, chained in the order of
+// the reference drivers (basic.jl:121-136, thermal.jl:100-133, test_mrt.jl:57-63).
+// ----------------------------------------------------------------------------------
+struct Axis {
+  int64_t n;
+  bool periodic;
+};
+
+// Pull with per-face rules: periodic faces wrap; a source beyond any other face is
+// replaced by the node's own opposite population (halfway bounce-back; the reference's
+// `ifelse(inside, pull, f_in[opp])`), plus the Ladd term 6 w_q ρ_w (c_q·u_w) of every
+// crossed MOVING_WALL face (stream_periodic_3d.jl:97-166).  Zou–He faces pull with
+// bounce-back as in stream_2d! before the closure overwrites the unknowns.
+template <class T>
+static void pull_generic(const kmi_desc& d, T* fo, const T* fi) {
+  const int Q = d.stencil == KMI_D2Q9 ? 9 : 19;
+  const int64_t nx = d.nx, ny = d.ny, nz = d.stencil == KMI_D2Q9 ? 1 : d.nz;
+  const size_t P = size_t(nx * ny * nz);
+  const int* cx = Q == 9 ? CX9 : CX19;
+  const int* cy = Q == 9 ? CY9 : CY19;
+  const int* cz = Q == 9 ? nullptr : CZ19;
+  const int* opp = Q == 9 ? OPP9 : OPP19;
+  const double* w = Q == 9 ? W9 : W19;
+  const bool per[3] = {d.faces[KMI_WEST].kind == KMI_FACE_PERIODIC,
+                       d.faces[KMI_SOUTH].kind == KMI_FACE_PERIODIC,
+                       Q == 19 && d.faces[KMI_BOTTOM].kind == KMI_FACE_PERIODIC};
+  const int64_t n[3] = {nx, ny, nz};
+#pragma omp parallel for
+  for (int64_t k = 0; k < nz; ++k)
+    for (int64_t j = 0; j < ny; ++j)
+      for (int64_t i = 0; i < nx; ++i) {
+        const int64_t x[3] = {i, j, k};
+        size_t c = I3(i, j, k, nx, ny);
+        for (int q = 0; q < Q; ++q) {
+          const int cq[3] = {cx[q], cy[q], cz ? cz[q] : 0};
+          int64_t s[3];
+          bool ok = true;
+          int crossed[3] = {-1, -1, -1};
+          for (int a = 0; a < 3; ++a) {
+            s[a] = x[a] - cq[a];
+            if (s[a] < 0 || s[a] >= n[a]) {
+              if (per[a]) {
+                s[a] = (s[a] + n[a]) % n[a];
+              } else {
+                ok = false;
+                crossed[a] = 2 * a + (s[a] < 0 ? 0 : 1);
+              }
+            }
+          }
+          T v;
+          if (ok) {
+            v = fi[I3(s[0], s[1], s[2], nx, ny) + q * P];
+          } else {
+            v = fi[c + opp[q] * P];
+            for (int a = 0; a < 3; ++a) {
+              if (crossed[a] < 0) continue;
+              const kmi_face& F = d.faces[crossed[a]];
+              if (F.kind != KMI_FACE_MOVING_WALL) continue;
+              T cu = cdotu<T>(cq[0], cq[1], cq[2], T(F.u[0]), T(F.u[1]), T(F.u[2]));
+              v = v + T(six_w(w[q])) * T(F.rho) * cu;
+            }
+          }
+          fo[c + q * P] = v;
+        }
+      }
+}
+
+template <class T>
+static void zou_he_faces(const kmi_desc& d, T* f) {
+  int order[6] = {0, 1, 2, 3, 4, 5};
+  int n = 6;
+  if (d.n_bc_order > 0) {
+    n = d.n_bc_order;
+    for (int i = 0; i < n; ++i) order[i] = d.bc_order[i];
+  }
+  for (int oi = 0; oi < n; ++oi) {
+    int fid = order[oi];
+    const kmi_face& F = d.faces[fid];
+    if (F.kind == KMI_FACE_ZOUHE_VELOCITY) {
+      if (fid == KMI_NORTH) zou_he_north(f, T(F.u[0]), d.nx, d.ny);
+      else if (fid == KMI_SOUTH) zou_he_south(f, T(F.u[0]), d.nx, d.ny);
+      else if (fid == KMI_WEST) zou_he_west(f, T(F.u[0]), d.nx, d.ny);
+    } else if (F.kind == KMI_FACE_ZOUHE_PRESSURE) {
+      if (fid == KMI_EAST) zou_he_pressure_east(f, T(F.rho), d.nx, d.ny);
+      else if (fid == KMI_WEST) zou_he_pressure_west(f, T(F.rho), d.nx, d.ny);
+    }
+  }
+}
+
+template <class T>
+static void generic_step(const kmi_desc& d, const uint8_t* solid, T*& f, T*& f2, T*& g, T*& g2,
+                         std::vector<T>& scratch) {
+  const int64_t nx = d.nx, ny = d.ny, nz = d.stencil == KMI_D2Q9 ? 1 : d.nz;
+  const size_t P = size_t(nx * ny * nz);
+  pull_generic(d, f2, f);
+  if (d.stencil == KMI_D2Q9) zou_he_faces(d, f2);
+  if (d.thermal) {
+    pull_generic(d, g2, g);
+    bool dir[4];
+    double Tf[4];
+    for (int a = 0; a < 4; ++a) {
+      dir[a] = d.faces[a].kind != KMI_FACE_PERIODIC && d.faces[a].thermal == KMI_THERMAL_DIRICHLET;
+      Tf[a] = d.faces[a].T;
+    }
+    apply_fixed_temp_2d(g2, dir, Tf, nx, ny);
+    scratch.resize(4 * P);
+    T* Temp = scratch.data();
+    T* rho = Temp + P;
+    T* ux = rho + P;
+    T* uy = ux + P;
+    compute_temperature_2d(Temp, g2, nx, ny);
+    if (d.coupling == KMI_COUPLING_SPLIT) {
+      // thermal.jl:123-125: u from compute_macroscopic_2d! (no force shift)
+      macroscopic_2d(rho, ux, uy, f2, nx, ny);
+    } else {
+      // fused_thermal_2d.jl:61-72: u with the F/2 shift in y
+      for (size_t c = 0; c < P; ++c) {
+        T fp[9];
+        for (int q = 0; q < 9; ++q) fp[q] = f2[c + q * P];
+        T fy = T(d.beta_g) * (Temp[c] - T(d.T_ref));
+        T r = fp[0] + fp[1] + fp[2] + fp[3] + fp[4] + fp[5] + fp[6] + fp[7] + fp[8];
+        T inv = T(1) / r;
+        ux[c] = (fp[1] - fp[3] + fp[5] - fp[6] - fp[7] + fp[8]) * inv;
+        uy[c] = ((fp[2] - fp[4] + fp[5] + fp[6] - fp[7] - fp[8]) + fy / T(2)) * inv;
+      }
+    }
+    collide_thermal_2d(g2, ux, uy, nx, ny, T(d.omega_T));
+    collide_boussinesq_2d(f2, Temp, solid, nx, ny, T(d.omega), T(d.beta_g), T(d.T_ref));
+    std::swap(g, g2);
+  } else if (d.stencil == KMI_D2Q9) {
+    if (d.collision == KMI_BGK) collide_2d(f2, solid, nx, ny, T(d.omega));
+    else if (d.collision == KMI_TRT) collide_trt_2d(f2, solid, nx, ny, T(d.s_plus), T(d.s_minus));
+    else collide_mrt_2d(f2, solid, nx, ny, T(d.s_e), T(d.s_eps), T(d.s_q), T(d.s_nu));
+  } else {
+    if (d.collision == KMI_BGK) collide_3d(f2, solid, nx, ny, nz, T(d.omega));
+    else collide_trt_3d(f2, solid, nx, ny, nz, T(d.s_plus), T(d.s_minus));
+  }
+  std::swap(f, f2);
+}
+
+// Neumaier-compensated sum.
+struct KSum {
+  double s = 0.0, c = 0.0;
+  void add(double x) {
+    double t = s + x;
+    if (std::fabs(s) >= std::fabs(x)) c += (s - t) + x;
+    else c += (x - t) + s;
+    s = t;
+  }
+  double val() const { return s + c; }
+};
+
+}  // namespace ko
+
+// ==================================================================================
+// Exported C entry points (double: _d, float: _f).  Array arguments follow the
+// reference layout; `solid` may be NULL.
+// ==================================================================================
+extern "C" {
+
+#define KO_BOTH(X) X(d, double) X(f, float)
+
+#define KO_STREAM(S, T)                                                                  \
+  void ko_stream_2d_##S(T* fo, const T* fi, int64_t nx, int64_t ny) {                   \
+    ko::stream_2d(fo, fi, nx, ny);                                                       \
+  }                                                                                      \
+  void ko_stream_fully_periodic_2d_##S(T* fo, const T* fi, int64_t nx, int64_t ny) {    \
+    ko::stream_fully_periodic_2d(fo, fi, nx, ny);                                        \
+  }                                                                                      \
+  void ko_stream_periodic_x_wall_y_2d_##S(T* fo, const T* fi, int64_t nx, int64_t ny) { \
+    ko::stream_periodic_x_wall_y_2d(fo, fi, nx, ny);                                     \
+  }                                                                                      \
+  void ko_collide_2d_##S(T* f, const uint8_t* s, int64_t nx, int64_t ny, double om) {   \
+    ko::collide_2d(f, s, nx, ny, T(om));                                                 \
+  }                                                                                      \
+  void ko_collide_trt_2d_##S(T* f, const uint8_t* s, int64_t nx, int64_t ny, double sp, \
+                             double sm) {                                                \
+    ko::collide_trt_2d(f, s, nx, ny, T(sp), T(sm));                                      \
+  }                                                                                      \
+  void ko_collide_mrt_2d_##S(T* f, const uint8_t* s, int64_t nx, int64_t ny, double se, \
+                             double sp2, double sq, double sn) {                         \
+    ko::collide_mrt_2d(f, s, nx, ny, T(se), T(sp2), T(sq), T(sn));                       \
+  }                                                                                      \
+  void ko_fused_bgk_step_2d_##S(T* fo, const T* fi, T* r, T* u, T* v, const uint8_t* s, \
+                                int64_t nx, int64_t ny, double om) {                     \
+    ko::fused_2d(fo, fi, r, u, v, s, nx, ny, false, T(om), T(0));                        \
+  }                                                                                      \
+  void ko_fused_trt_step_2d_##S(T* fo, const T* fi, T* r, T* u, T* v, const uint8_t* s, \
+                                int64_t nx, int64_t ny, double sp, double sm) {          \
+    ko::fused_2d(fo, fi, r, u, v, s, nx, ny, true, T(sp), T(sm));                        \
+  }                                                                                      \
+  void ko_macroscopic_2d_##S(T* r, T* u, T* v, const T* f, int64_t nx, int64_t ny) {    \
+    ko::macroscopic_2d(r, u, v, f, nx, ny);                                              \
+  }                                                                                      \
+  void ko_zou_he_north_2d_##S(T* f, double u, int64_t nx, int64_t ny) {                 \
+    ko::zou_he_north(f, T(u), nx, ny);                                                   \
+  }                                                                                      \
+  void ko_zou_he_south_2d_##S(T* f, double u, int64_t nx, int64_t ny) {                 \
+    ko::zou_he_south(f, T(u), nx, ny);                                                   \
+  }                                                                                      \
+  void ko_zou_he_west_2d_##S(T* f, double u, int64_t nx, int64_t ny) {                  \
+    ko::zou_he_west(f, T(u), nx, ny);                                                    \
+  }                                                                                      \
+  void ko_zou_he_pressure_east_2d_##S(T* f, double r, int64_t nx, int64_t ny) {         \
+    ko::zou_he_pressure_east(f, T(r), nx, ny);                                           \
+  }                                                                                      \
+  void ko_zou_he_pressure_west_2d_##S(T* f, double r, int64_t nx, int64_t ny) {         \
+    ko::zou_he_pressure_west(f, T(r), nx, ny);                                           \
+  }                                                                                      \
+  void ko_stream_3d_##S(T* fo, const T* fi, int64_t nx, int64_t ny, int64_t nz, int px,  \
+                        int py, int pz) {                                                \
+    ko::stream_3d_flags(fo, fi, nx, ny, nz, px != 0, py != 0, pz != 0);                  \
+  }                                                                                      \
+  void ko_stream_periodic_xz_movingwall_y_3d_##S(T* fo, const T* fi, double ub,         \
+                                                 double ut, double rw, int64_t nx,       \
+                                                 int64_t ny, int64_t nz) {               \
+    ko::stream_periodic_xz_movingwall_y_3d(fo, fi, T(ub), T(ut), T(rw), nx, ny, nz);     \
+  }                                                                                      \
+  void ko_collide_3d_##S(T* f, const uint8_t* s, int64_t nx, int64_t ny, int64_t nz,    \
+                         double om) {                                                    \
+    ko::collide_3d(f, s, nx, ny, nz, T(om));                                             \
+  }                                                                                      \
+  void ko_collide_trt_3d_##S(T* f, const uint8_t* s, int64_t nx, int64_t ny, int64_t nz,\
+                             double sp, double sm) {                                     \
+    ko::collide_trt_3d(f, s, nx, ny, nz, T(sp), T(sm));                                  \
+  }                                                                                      \
+  void ko_fused_trt_3d_##S(T* fo, const T* fi, int64_t nx, int64_t ny, int64_t nz,      \
+                           int px, int py, int pz, double sp, double sm) {               \
+    ko::fused_trt_3d(fo, fi, nx, ny, nz, px != 0, py != 0, pz != 0, T(sp), T(sm));       \
+  }                                                                                      \
+  void ko_macroscopic_3d_##S(T* r, T* u, T* v, T* w, const T* f, int64_t nx, int64_t ny,\
+                             int64_t nz) {                                               \
+    ko::macroscopic_3d(r, u, v, w, f, nx, ny, nz);                                       \
+  }                                                                                      \
+  void ko_collide_thermal_2d_##S(T* g, const T* u, const T* v, int64_t nx, int64_t ny,  \
+                                 double om) {                                            \
+    ko::collide_thermal_2d(g, u, v, nx, ny, T(om));                                      \
+  }                                                                                      \
+  void ko_compute_temperature_2d_##S(T* t, const T* g, int64_t nx, int64_t ny) {        \
+    ko::compute_temperature_2d(t, g, nx, ny);                                            \
+  }                                                                                      \
+  void ko_apply_fixed_temp_2d_##S(T* g, const int* dir, const double* Tf, int64_t nx,   \
+                                  int64_t ny) {                                          \
+    bool b[4] = {dir[0] != 0, dir[1] != 0, dir[2] != 0, dir[3] != 0};                    \
+    ko::apply_fixed_temp_2d(g, b, Tf, nx, ny);                                           \
+  }                                                                                      \
+  void ko_collide_boussinesq_2d_##S(T* f, const T* t, const uint8_t* s, int64_t nx,     \
+                                    int64_t ny, double om, double bg, double tr) {       \
+    ko::collide_boussinesq_2d(f, t, s, nx, ny, T(om), T(bg), T(tr));                     \
+  }                                                                                      \
+  void ko_fused_natconv_step_2d_##S(T* fo, const T* fi, T* go, const T* gi, T* t,       \
+                                    int64_t nx, int64_t ny, double of, double ot,        \
+                                    double bg, double tr, double th, double tc) {        \
+    ko::fused_natconv_step(fo, fi, go, gi, t, nx, ny, T(of), T(ot), T(bg), T(tr), T(th), \
+                           T(tc));                                                       \
+  }                                                                                      \
+  void ko_pull_generic_##S(const kmi_desc* d, T* fo, const T* fi) {                     \
+    ko::pull_generic(*d, fo, fi);                                                        \
+  }                                                                                      \
+  /* Advance `nsteps` steps of the generic chain; f (and g) are updated in place. */     \
+  void ko_run_##S(const kmi_desc* d, const uint8_t* solid, T* f, T* g, int64_t nsteps) { \
+    const int Q = d->stencil == KMI_D2Q9 ? 9 : 19;                                       \
+    const size_t n = size_t(d->nx * d->ny * (d->stencil == KMI_D2Q9 ? 1 : d->nz)) * Q;   \
+    std::vector<T> b1(n), b2(d->thermal ? n : 0), scratch;                               \
+    T *fa = f, *fb = b1.data(), *ga = g, *gb = d->thermal ? b2.data() : nullptr;         \
+    for (int64_t s = 0; s < nsteps; ++s) ko::generic_step(*d, solid, fa, fb, ga, gb, scratch); \
+    if (fa != f) std::memcpy(f, fa, n * sizeof(T));                                      \
+    if (d->thermal && ga != g) std::memcpy(g, ga, n * sizeof(T));                        \
+  }
+
+KO_BOTH(KO_STREAM)
+
+/* Hot-wall Nusselt number — thermal.jl:225-234 (west wall, rows 2..Ny-1), generalised to
+ * the hot face `face`; `periodic_parallel` != 0 averages over every wall-parallel index. */
+double ko_nusselt_d(const double* Temp, int64_t nx, int64_t ny, int face, double T_hot,
+                    double T_cold, int periodic_parallel) {
+  double dT = T_hot - T_cold;
+  bool xface = face == KMI_WEST || face == KMI_EAST;
+  int64_t npar = xface ? ny : nx;
+  double H = double(xface ? nx : ny);
+  double dx = 1.0;
+  int64_t lo = periodic_parallel ? 0 : 1, hi = periodic_parallel ? npar : npar - 1;
+  double sum = 0.0;
+  for (int64_t p = lo; p < hi; ++p) {
+    double t1, t2;
+    if (face == KMI_WEST) { t1 = Temp[ko::I2(0, p, nx)]; t2 = Temp[ko::I2(1, p, nx)]; }
+    else if (face == KMI_EAST) { t1 = Temp[ko::I2(nx - 1, p, nx)]; t2 = Temp[ko::I2(nx - 2, p, nx)]; }
+    else if (face == KMI_SOUTH) { t1 = Temp[ko::I2(p, 0, nx)]; t2 = Temp[ko::I2(p, 1, nx)]; }
+    else { t1 = Temp[ko::I2(p, ny - 1, nx)]; t2 = Temp[ko::I2(p, ny - 2, nx)]; }
+    double nu = -H * (-8 * T_hot + 9 * t1 - t2) / (3 * dx) / dT;
+    sum += nu;
+  }
+  return sum / double(hi - lo);
+}
+
+/* Compensated sum of an array (for mass / energy checks). */
+double ko_ksum_d(const double* x, int64_t n) {
+  ko::KSum s;
+  for (int64_t i = 0; i < n; ++i) s.add(x[i]);
+  return s.val();
+}
+
+int ko_omp_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+}  // extern "C"

@@ -1,0 +1,231 @@
+"""GPU parity: the HIP single-pass kernel against the CPU oracle on the same inputs.
+
+FP64 and FP32 storage must be bit-identical to the oracle's restatement of the reference
+chain in the same precision.  Shifted storage (F32S, F16S) is checked against the FP64
+oracle with the tolerances written in each test.
+"""
+import math
+import zlib
+
+import numpy as np
+import pytest
+
+from test_oracle_lattice import rng_pops
+
+pytestmark = pytest.mark.gpu
+
+
+def _mask_blob(shape, centre, r):
+    idx = np.indices(shape)
+    d2 = sum((idx[a] - centre[a]) ** 2 for a in range(len(shape)))
+    return np.asfortranarray((d2 <= r * r).astype(np.uint8))
+
+
+def _case(kmi, name, storage):
+    """(desc, shape, mask, thermal, steps) of a parity case."""
+    per = lambda *fs: {f: {"kind": "periodic"} for f in fs}
+    if name == "d2q9_cavity_zh_bgk_mask":
+        return kmi.cavity_desc(24, 0.05, 0.1, storage=storage, Ny=20), (24, 20), _mask_blob((24, 20), (10, 8), 2), False, 45
+    if name == "d2q9_cavity_ladd_trt":
+        return kmi.cavity_desc(20, 0.03, 0.1, storage=storage, Ny=18, collision="trt", lid="ladd"), (20, 18), None, False, 40
+    if name == "d2q9_cavity_mrt":
+        return kmi.cavity_desc(22, 0.01, 0.1, storage=storage, Ny=16, collision="mrt"), (22, 16), None, False, 40
+    if name == "d2q9_periodic_bgk":
+        return kmi.make_desc("d2q9", 32, 24, 1, 0.02, storage=storage, faces=per("west", "east", "south", "north")), (32, 24), None, False, 40
+    if name == "d2q9_periodic_trt":
+        return kmi.make_desc("d2q9", 16, 16, 1, 0.02, storage=storage, collision="trt",
+                             faces=per("west", "east", "south", "north")), (16, 16), None, False, 33
+    if name == "d2q9_couette_zh":
+        f = per("west", "east")
+        f.update(south={"kind": "zouhe_velocity", "u": (0.05, 0, 0)}, north={"kind": "zouhe_velocity", "u": (0, 0, 0)})
+        return kmi.make_desc("d2q9", 8, 16, 1, 0.1, storage=storage, faces=f, bc_order=["south", "north"]), (8, 16), None, False, 40
+    if name == "d2q9_channel_inlet_outlet":
+        f = {"west": {"kind": "zouhe_velocity", "u": (0.04, 0, 0)}, "east": {"kind": "zouhe_pressure", "rho": 1.0},
+             "south": {"kind": "wall"}, "north": {"kind": "wall"}}
+        return kmi.make_desc("d2q9", 40, 16, 1, 0.04, storage=storage, faces=f), (40, 16), _mask_blob((40, 16), (12, 8), 3), False, 50
+    if name == "d2q9_rb_split":
+        d, _ = kmi.rayleigh_benard_desc(32, 16, Ra=1e4, Pr=0.71, storage=storage)
+        return d, (32, 16), None, True, 40
+    if name == "d2q9_natconv_fused":
+        return kmi.natural_convection_desc(24, 1e4, 0.71, storage=storage), (24, 24), None, True, 40
+    if name == "d2q9_thermal_corners_mask":
+        f = {"west": {"kind": "wall", "thermal": "dirichlet", "T": 0.9}, "south": {"kind": "wall", "thermal": "dirichlet", "T": 0.3},
+             "east": {"kind": "wall"}, "north": {"kind": "moving_wall", "u": (0.03, 0, 0)}}
+        d = kmi.make_desc("d2q9", 16, 12, 1, 0.05, storage=storage, faces=f, thermal=True, omega_T=1.5,
+                          beta_g=1e-4, T_ref=0.5)
+        return d, (16, 12), _mask_blob((16, 12), (8, 6), 1), True, 35
+    if name == "d3q19_periodic_trt":
+        return kmi.make_desc("d3q19", 16, 12, 10, 0.01, storage=storage, collision="trt",
+                             faces=per("west", "east", "south", "north", "bottom", "top")), (16, 12, 10), None, False, 40
+    if name == "d3q19_cavity_bgk_ladd_mask":
+        d = kmi.make_desc("d3q19", 12, 10, 8, 0.05, storage=storage,
+                          faces={"top": {"kind": "moving_wall", "u": (0.05, 0.02, 0)}})
+        return d, (12, 10, 8), _mask_blob((12, 10, 8), (6, 5, 4), 2), False, 35
+    if name == "d3q19_couette_trt_ladd":
+        f = per("west", "east", "bottom", "top")
+        f.update(south={"kind": "moving_wall", "u": (0.03, 0, 0)}, north={"kind": "moving_wall", "u": (-0.02, 0, 0)})
+        return kmi.make_desc("d3q19", 8, 12, 6, 0.05, storage=storage, collision="trt", faces=f), (8, 12, 6), None, False, 40
+    if name == "d3q19_pz_trt_walls":
+        return kmi.make_desc("d3q19", 10, 8, 6, 0.02, storage=storage, collision="trt",
+                             faces=per("bottom", "top")), (10, 8, 6), None, False, 33
+    raise KeyError(name)
+
+
+CASES = ["d2q9_cavity_zh_bgk_mask", "d2q9_cavity_ladd_trt", "d2q9_cavity_mrt", "d2q9_periodic_bgk",
+         "d2q9_periodic_trt", "d2q9_couette_zh", "d2q9_channel_inlet_outlet", "d2q9_rb_split",
+         "d2q9_natconv_fused", "d2q9_thermal_corners_mask", "d3q19_periodic_trt",
+         "d3q19_cavity_bgk_ladd_mask", "d3q19_couette_trt_ladd", "d3q19_pz_trt_walls"]
+
+
+def _init(shape, thermal, seed):
+    Q = 9 if len(shape) == 2 else 19
+    f = rng_pops(shape, Q, seed=seed, amp=0.02)
+    g = None
+    if thermal:
+        r = np.random.default_rng(seed + 100)
+        T = 0.5 + 0.3 * r.uniform(-1, 1, size=shape)
+        g = np.asfortranarray(np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)[None, None, :] * T[:, :, None])
+        g = g * (1 + 0.01 * r.uniform(-1, 1, size=g.shape))
+        g = np.asfortranarray(g)
+    return f, g
+
+
+def _run_gpu(kmi, backend, desc, f, g, mask, steps, dtype):
+    with kmi.Lattice(desc, backend) as lat:
+        if mask is not None:
+            lat.set_mask(mask)
+        lat.upload(f.astype(dtype, order="F"))
+        if g is not None:
+            lat.upload(g.astype(dtype, order="F"), field="g")
+        lat.step(steps)
+        fo = lat.download(dtype=dtype)
+        go = lat.download("g", dtype=dtype) if g is not None else None
+    return fo, go
+
+
+@pytest.mark.parametrize("storage,dtype", [("f64", np.float64), ("f32", np.float32)])
+@pytest.mark.parametrize("case", CASES)
+def test_bitwise_parity(kmi, po, backend, case, storage, dtype):
+    desc, shape, mask, thermal, steps = _case(kmi, case, storage)
+    print(f"[parity] {case} {storage} shape={shape} steps={steps}", flush=True)
+    f, g = _init(shape, thermal, seed=zlib.crc32(case.encode()) % 1000)
+    fg, gg = _run_gpu(kmi, backend, desc, f, g, mask, steps, dtype)
+    fr = f.astype(dtype, order="F")
+    gr = None if g is None else g.astype(dtype, order="F")
+    po.run(desc, fr, gr, steps, mask=mask)
+    assert np.all(np.isfinite(fr))
+    assert np.array_equal(fg, fr), f"max |Δf| = {np.max(np.abs(fg.astype(float) - fr)):.3e}"
+    if thermal:
+        assert np.array_equal(gg, gr), f"max |Δg| = {np.max(np.abs(gg.astype(float) - gr)):.3e}"
+
+
+# F32S: FP32 arithmetic on deviations; F16S: additionally binary16 storage of 2^15·(f − w).
+SHIFTED_TOL = {"f32s": 2e-6, "f16s": 3e-4}
+
+
+@pytest.mark.parametrize("storage", ["f32s", "f16s"])
+@pytest.mark.parametrize("case", CASES)
+def test_shifted_storage_against_fp64(kmi, po, backend, case, storage):
+    desc, shape, mask, thermal, steps = _case(kmi, case, storage)
+    print(f"[shifted] {case} {storage}", flush=True)
+    f, g = _init(shape, thermal, seed=7 + zlib.crc32(case.encode()) % 1000)
+    fg, gg = _run_gpu(kmi, backend, desc, f, g, mask, steps, np.float64)
+    d64, *_ = _case(kmi, case, "f64")
+    fr, gr = f.copy(order="F"), (None if g is None else g.copy(order="F"))
+    po.run(d64, fr, gr, steps, mask=mask)
+    tol = SHIFTED_TOL[storage]
+    err = np.max(np.abs(fg - fr))
+    assert err < tol, f"{storage}: max |Δf| = {err:.3e} (tol {tol})"
+    if thermal:
+        errg = np.max(np.abs(gg - gr))
+        assert errg < 4 * tol, f"{storage}: max |Δg| = {errg:.3e}"
+
+
+@pytest.mark.parametrize("storage,dtype", [("f64", np.float64), ("f32", np.float32)])
+def test_moments_match_macroscopic(kmi, po, backend, storage, dtype):
+    for case in ("d2q9_natconv_fused", "d3q19_couette_trt_ladd"):
+        desc, shape, mask, thermal, steps = _case(kmi, case, storage)
+        f, g = _init(shape, thermal, seed=3)
+        with kmi.Lattice(desc, backend) as lat:
+            lat.upload(f.astype(dtype, order="F"))
+            if thermal:
+                lat.upload(g.astype(dtype, order="F"), "g")
+            lat.step(5)
+            m = lat.moments(temperature=thermal)
+            fd = lat.download(dtype=dtype)
+            gd = lat.download("g", dtype=dtype) if thermal else None
+        mr = po.moments(fd)
+        for k in mr:
+            assert np.array_equal(m[k], mr[k].astype(np.float64)), k
+        if thermal:
+            assert np.array_equal(m["Temp"], po.temperature(gd).astype(np.float64))
+
+
+def test_reductions(kmi, po, backend):
+    desc, shape, mask, thermal, steps = _case(kmi, "d2q9_natconv_fused", "f64")
+    f, g = _init(shape, thermal, seed=4)
+    with kmi.Lattice(desc, backend) as lat:
+        lat.upload(f)
+        lat.upload(g, "g")
+        lat.step(30)
+        m = lat.moments(temperature=True)
+        mass = lat.reduce("mass")
+        mom = lat.reduce("momentum")
+        ke = lat.reduce("kinetic_energy")
+        th = lat.reduce("thermal")
+        vmax = lat.reduce("max_speed")
+        nu = lat.reduce("nusselt", "west")
+        mass2 = lat.reduce("mass")
+    rho, ux, uy = m["rho"], m["ux"], m["uy"]
+    assert mass == mass2  # deterministic
+    assert abs(mass - math.fsum(rho.ravel())) <= 1e-14 * mass
+    assert abs(mom[0] - math.fsum((rho * ux).ravel())) <= 1e-15 * mass
+    assert abs(mom[1] - math.fsum((rho * uy).ravel())) <= 1e-15 * mass
+    assert abs(ke - math.fsum((0.5 * rho * (ux ** 2 + uy ** 2)).ravel())) <= 1e-13 * abs(ke)
+    assert abs(th - math.fsum(m["Temp"].ravel())) <= 1e-14 * abs(th)
+    assert vmax == np.max(np.sqrt(ux ** 2 + uy ** 2))
+    nref = po.nusselt(m["Temp"], 0, 1.0, 0.0, False)
+    assert abs(nu - nref) <= 1e-12 * abs(nref)
+
+
+def test_residual_matches_definition(kmi, po, backend):
+    """r_u = ‖u^k − u^(k−1)‖₂ / ‖u^k‖₂ over fluid cells (gnf_periodic_steady.jl:186-196)."""
+    desc, shape, mask, thermal, steps = _case(kmi, "d2q9_cavity_zh_bgk_mask", "f64")
+    with kmi.Lattice(desc, backend) as lat:
+        lat.set_mask(mask)
+        lat.step(40)
+        f0 = lat.download()
+        r = lat.step_residual(3)
+    fr = f0.copy(order="F")
+    po.run(desc, fr, None, 2, mask=mask)
+    u0 = po.moments(fr)
+    po.run(desc, fr, None, 1, mask=mask)
+    u1 = po.moments(fr)
+    fl = mask == 0
+    du2 = ((u1["ux"] - u0["ux"]) ** 2 + (u1["uy"] - u0["uy"]) ** 2)[fl].sum()
+    u2 = (u1["ux"] ** 2 + u1["uy"] ** 2)[fl].sum()
+    assert abs(r - math.sqrt(du2 / u2)) <= 1e-10 * r
+
+
+def test_graph_replay_equals_eager_steps(kmi, backend):
+    """64 steps through the captured graph (2 replays) equal 64 single launches."""
+    desc, shape, mask, thermal, steps = _case(kmi, "d2q9_rb_split", "f32")
+    f, g = _init(shape, thermal, seed=5)
+    out = []
+    for chunks in ([64], [1] * 64, [32, 7, 25]):
+        with kmi.Lattice(desc, backend) as lat:
+            lat.upload(f)
+            lat.upload(g, "g")
+            for n in chunks:
+                lat.step(n)
+            assert lat.steps == 64
+            out.append((lat.download(), lat.download("g")))
+    for fo, go in out[1:]:
+        assert np.array_equal(fo, out[0][0]) and np.array_equal(go, out[0][1])
+
+
+def test_invalid_descriptor_raises(kmi, backend):
+    d = kmi.make_desc("d3q19", 8, 8, 8, collision="mrt")
+    with pytest.raises(kmi.KrakenError) as e:
+        kmi.Lattice(d, backend)
+    assert e.value.status == 5 and "MRT" in str(e.value)
