@@ -59,6 +59,7 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--storage", default="f32", choices=sorted(BYTES_PER_POP))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--variants", action="store_true", help="also time f64 and f16s storage")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -75,6 +76,8 @@ def main():
         desc = kmi.taylor_green_3d_desc(N, nu, storage)
         with kmi.Lattice(desc) as lat:
             lat.init_equilibrium(*fields)
+            if not args.no_autotune:
+                lat.autotune(64)  # restores the populations afterwards
             lat.step(warmup)
             lat.backend.synchronize()
             t0 = time.perf_counter()
@@ -82,9 +85,11 @@ def main():
             lat.backend.synchronize()
             wall = time.perf_counter() - t0
             mass = lat.reduce("mass")
-        return ms, wall, mass
+            launch = lat.launch
+        return ms, wall, mass, launch
 
-    ms, wall, mass = run(args.storage, args.steps, args.warmup)
+    ms, wall, mass, launch = run(args.storage, args.steps, args.warmup)
+    copy_gbps = kmi.MI355XBackend().copy_bandwidth(2 * 1024 ** 3, 20)
     cells = N ** 3
     mlups = cells * args.steps / wall / 1e6
     per_launch_s = ms / 1e3 / args.steps
@@ -108,7 +113,10 @@ def main():
                    "cells": cells, "storage": args.storage, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
-                     "bytes_per_launch": alg_bytes, "avg_launch_ms": round(per_launch_s * 1e3, 5)},
+                     "bytes_per_launch": alg_bytes, "avg_launch_ms": round(per_launch_s * 1e3, 5),
+                     "copy_measured": round(copy_gbps, 1),
+                     "frac_of_copy": round(achieved / copy_gbps, 4)},
+        "launch": launch,
         "mass_after": mass,
     }
     if args.variants:
@@ -116,7 +124,7 @@ def main():
         for st in ("f64", "f16s"):
             if st == args.storage:
                 continue
-            m2, w2, _ = run(st, max(args.steps // 2, 64), args.warmup)
+            m2, w2, _, _ = run(st, max(args.steps // 2, 64), args.warmup)
             var[st] = {"MLUPS": round(cells * max(args.steps // 2, 64) / w2 / 1e6, 1),
                        "frac": round(cells * 38 * BYTES_PER_POP[st] / (m2 / 1e3 / max(args.steps // 2, 64)) / 1e9
                                      / HBM_PEAK_GBPS, 4)}

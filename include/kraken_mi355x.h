@@ -213,6 +213,19 @@ int kmi_lattice_steps(const kmi_lattice* lat, int64_t* steps);
 /* Bytes of device memory held by the lattice. */
 int kmi_lattice_device_bytes(const kmi_lattice* lat, int64_t* bytes);
 
+/* ---- launch tuning (no reference counterpart: KA picks its own workgroup size;
+ *      src/kernels/persistent_bgk_2d.jl:17-56 exposes a `workgroupsize` knob) --------- */
+/* Work-group of bx × by cells (bx*by a multiple of 64, at most 256) and non-temporal
+ * population stores on/off.  Results do not depend on these settings. */
+int kmi_lattice_set_launch(kmi_lattice* lat, int32_t bx, int32_t by, int32_t nontemporal);
+int kmi_lattice_get_launch(const kmi_lattice* lat, int32_t* bx, int32_t* by, int32_t* nontemporal);
+/* Time every candidate shape × store policy on this lattice for ~steps_per_trial steps
+ * each, keep the fastest, and restore the populations, parity and step count. */
+int kmi_lattice_autotune(kmi_lattice* lat, int64_t steps_per_trial, double* best_ms_per_step);
+/* Device copy bandwidth (float4 loads, non-temporal stores) over `bytes`, GB/s counting
+ * read + write, averaged over `reps` copies: the practical HBM ceiling of a 1:1 kernel. */
+int kmi_measure_copy_bandwidth(kmi_ctx* ctx, int64_t bytes, int32_t reps, double* gbps);
+
 #ifdef __cplusplus
 }
 #endif

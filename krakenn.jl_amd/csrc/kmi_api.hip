@@ -10,8 +10,9 @@
 #include "kmi_common.h"
 
 namespace kmi {
-hipError_t launch_step(int q, int st, int coll, bool th, bool bnd, const StepArgs& a, dim3 g,
-                       dim3 b, hipStream_t s);
+hipError_t launch_step(int q, int st, int coll, bool th, bool bnd, bool nts, const StepArgs& a,
+                       dim3 g, dim3 b, hipStream_t s);
+hipError_t launch_copy_probe(const void* src, void* dst, long long bytes, hipStream_t s);
 hipError_t launch_to_storage(int Q, int st, const double* src, void* dst, long long ncells,
                              long long plane, double shift, hipStream_t s);
 hipError_t launch_from_storage(int Q, int st, const void* src, double* dst, long long ncells,
@@ -59,6 +60,7 @@ struct kmi_lattice {
   int64_t steps = 0;
   StepArgs args{};
   bool bnd = true;
+  int bx = 0, by = 1, nts = 0;  // launch shape and store policy (0: not chosen yet)
   dim3 grid, block;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   hipEvent_t ev[2] = {nullptr, nullptr};
@@ -308,13 +310,27 @@ static void build_args(kmi_lattice* L) {
   bool any_nonperiodic = false;
   for (int f = 0; f < nf; ++f) any_nonperiodic |= d.faces[f].kind != KMI_FACE_PERIODIC;
   L->bnd = any_nonperiodic || L->mask != nullptr;
-  // Launch shape: one x-row segment of up to 256 cells per work-group; small grids use
-  // 64-wide groups so that the launch still spreads over the 256 CUs.
-  unsigned bx = unsigned(std::min<int64_t>(256, ((d.nx + 63) / 64) * 64));
-  const long long groups = ((d.nx + bx - 1) / bx) * d.ny * a.nz;
+}
+
+static void set_geometry(kmi_lattice* L, int bx, int by, int nts) {
+  L->bx = bx;
+  L->by = by;
+  L->nts = nts;
+  L->block = dim3(unsigned(bx), unsigned(by), 1);
+  L->grid = dim3(unsigned((L->desc.nx + bx - 1) / bx), unsigned((L->desc.ny + by - 1) / by),
+                 unsigned(L->args.nz));
+}
+
+// Default launch: one x-row segment of up to 256 cells per work-group (64-wide groups on
+// small grids so the launch still spreads over the 256 CUs); non-temporal stores once a
+// population field outgrows the 256 MB last-level cache.
+static void default_geometry(kmi_lattice* L) {
+  const kmi_desc& d = L->desc;
+  int bx = int(std::min<int64_t>(256, ((d.nx + 63) / 64) * 64));
+  const long long groups = ((d.nx + bx - 1) / bx) * d.ny * L->args.nz;
   if (groups < 2048 && bx > 64) bx = 64;
-  L->block = dim3(bx, 1, 1);
-  L->grid = dim3(unsigned((d.nx + bx - 1) / bx), unsigned(d.ny), unsigned(a.nz));
+  const double field_bytes = double(L->plane) * L->Q * L->esize;
+  set_geometry(L, bx, 1, field_bytes > 512e6 ? 1 : 0);
 }
 
 static int launch_one(kmi_lattice* L, hipStream_t s) {
@@ -325,7 +341,7 @@ static int launch_one(kmi_lattice* L, hipStream_t s) {
   a.go = L->g[L->cur ^ 1];
   a.mask = L->mask;
   hipError_t e = launch_step(L->Q, L->desc.storage, L->desc.collision, L->desc.thermal != 0,
-                             L->bnd, a, L->grid, L->block, s);
+                             L->bnd, L->nts != 0, a, L->grid, L->block, s);
   if (e != hipSuccess) return hip_fail(L->ctx, e, "step kernel launch");
   L->cur ^= 1;
   return KMI_OK;
@@ -417,6 +433,7 @@ int kmi_lattice_create(kmi_ctx* ctx, const kmi_desc* desc, kmi_lattice** out) {
                 std::string("kmi_lattice_create: ") + hipGetErrorString(e));
   }
   build_args(L);
+  default_geometry(L);
   // rest equilibrium f = w_q, g = w_q T_ref (initialize_2d/3d, basic.jl:64-70)
   int rc = kmi_init_equilibrium(L, nullptr, nullptr, nullptr, nullptr);
   if (rc == KMI_OK && desc->thermal) rc = kmi_init_temperature(L, nullptr);
@@ -738,6 +755,126 @@ int kmi_reduce(kmi_lattice* L, int32_t kind, int32_t arg, double* out) {
 int kmi_lattice_steps(const kmi_lattice* L, int64_t* steps) {
   if (!L || !steps) return fail(nullptr, KMI_ERR_ARG, "kmi_lattice_steps: NULL argument");
   *steps = L->steps;
+  return KMI_OK;
+}
+
+int kmi_lattice_set_launch(kmi_lattice* L, int32_t bx, int32_t by, int32_t nontemporal) {
+  if (!L) return fail(nullptr, KMI_ERR_ARG, "kmi_lattice_set_launch: lattice is NULL");
+  if (bx < 1 || by < 1 || bx * by > 256 || (bx * by) % 64 != 0 || (L->desc.ny + by - 1) / by > 65535)
+    return fail(L->ctx, KMI_ERR_ARG,
+                "kmi_lattice_set_launch: need bx*by a multiple of 64 and at most 256");
+  KMI_HIP(L->ctx, hipSetDevice(L->ctx->device));
+  KMI_HIP(L->ctx, hipStreamSynchronize(L->ctx->stream));
+  drop_graphs(L);
+  set_geometry(L, bx, by, nontemporal ? 1 : 0);
+  return KMI_OK;
+}
+
+int kmi_lattice_get_launch(const kmi_lattice* L, int32_t* bx, int32_t* by, int32_t* nontemporal) {
+  if (!L || !bx || !by || !nontemporal)
+    return fail(nullptr, KMI_ERR_ARG, "kmi_lattice_get_launch: NULL argument");
+  *bx = L->bx;
+  *by = L->by;
+  *nontemporal = L->nts;
+  return KMI_OK;
+}
+
+// Times each candidate launch shape × store policy on the lattice itself (graph-replayed
+// steps between HIP events, best of two) and keeps the fastest.  The populations, buffer
+// parity and step counter are restored afterwards, so tuning leaves the run unchanged.
+int kmi_lattice_autotune(kmi_lattice* L, int64_t steps_per_trial, double* best_ms_per_step) {
+  if (!L) return fail(nullptr, KMI_ERR_ARG, "kmi_lattice_autotune: lattice is NULL");
+  kmi_ctx* c = L->ctx;
+  KMI_HIP(c, hipSetDevice(c->device));
+  KMI_HIP(c, hipStreamSynchronize(c->stream));
+  const int64_t trial = std::max<int64_t>(kGraphSteps, (steps_per_trial + kGraphSteps - 1) /
+                                                           kGraphSteps * kGraphSteps);
+  const size_t bytes = size_t(L->plane) * L->Q * L->esize;
+  void* save_f = nullptr;
+  void* save_g = nullptr;
+  KMI_HIP(c, hipMalloc(&save_f, bytes));
+  hipError_t e = hipMemcpyAsync(save_f, L->f[L->cur], bytes, hipMemcpyDeviceToDevice, c->stream);
+  if (e == hipSuccess && L->desc.thermal) {
+    e = hipMalloc(&save_g, bytes);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(save_g, L->g[L->cur], bytes, hipMemcpyDeviceToDevice, c->stream);
+  }
+  const int cur0 = L->cur;
+  const int64_t steps0 = L->steps;
+  const int bx0 = L->bx, by0 = L->by, nts0 = L->nts;
+  const int shapes[][2] = {{256, 1}, {128, 1}, {128, 2}, {64, 1}, {64, 2}, {64, 4}, {32, 2},
+                           {32, 4}, {32, 8}, {16, 4}, {16, 8}, {16, 16}};
+  double best = 1e300;
+  int bb[3] = {bx0, by0, nts0};
+  for (const auto& sh : shapes) {
+    if (e != hipSuccess) break;
+    const int bx = sh[0], by = sh[1];
+    if (bx > ((L->desc.nx + 63) / 64) * 64 && bx > 64) continue;  // mostly idle lanes
+    if (by > 1 && by > L->desc.ny) continue;
+    if ((L->desc.ny + by - 1) / by > 65535) continue;
+    for (int nts = 0; nts < 2 && e == hipSuccess; ++nts) {
+      drop_graphs(L);
+      set_geometry(L, bx, by, nts);
+      if (run_steps(L, kGraphSteps) != KMI_OK) { e = hipErrorUnknown; break; }
+      double t = 1e300;
+      for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
+        e = hipEventRecord(L->ev[0], c->stream);
+        if (e == hipSuccess && run_steps(L, trial) != KMI_OK) e = hipErrorUnknown;
+        if (e == hipSuccess) e = hipEventRecord(L->ev[1], c->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(L->ev[1]);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, L->ev[0], L->ev[1]);
+        t = std::min(t, double(ms) / double(trial));
+      }
+      if (e == hipSuccess && t < best) {
+        best = t;
+        bb[0] = bx;
+        bb[1] = by;
+        bb[2] = nts;
+      }
+    }
+  }
+  drop_graphs(L);
+  set_geometry(L, bb[0], bb[1], bb[2]);
+  hipError_t e2 = hipMemcpyAsync(L->f[cur0], save_f, bytes, hipMemcpyDeviceToDevice, c->stream);
+  if (e2 == hipSuccess && save_g)
+    e2 = hipMemcpyAsync(L->g[cur0], save_g, bytes, hipMemcpyDeviceToDevice, c->stream);
+  if (e2 == hipSuccess) e2 = hipStreamSynchronize(c->stream);
+  (void)hipFree(save_f);
+  if (save_g) (void)hipFree(save_g);
+  L->cur = cur0;
+  L->steps = steps0;
+  if (e != hipSuccess) return hip_fail(c, e, "kmi_lattice_autotune");
+  if (e2 != hipSuccess) return hip_fail(c, e2, "kmi_lattice_autotune (restore)");
+  if (best_ms_per_step) *best_ms_per_step = best;
+  return KMI_OK;
+}
+
+int kmi_measure_copy_bandwidth(kmi_ctx* ctx, int64_t bytes, int32_t reps, double* gbps) {
+  if (!ctx || !gbps || bytes < 1024 || reps < 1)
+    return fail(ctx, KMI_ERR_ARG, "kmi_measure_copy_bandwidth: bad arguments");
+  KMI_HIP(ctx, hipSetDevice(ctx->device));
+  bytes = bytes / 256 * 256;
+  void *a = nullptr, *b = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipMalloc(&a, size_t(bytes));
+  if (e == hipSuccess) e = hipMalloc(&b, size_t(bytes));
+  if (e == hipSuccess) e = hipMemsetAsync(a, 0, size_t(bytes), ctx->stream);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = launch_copy_probe(a, b, bytes, ctx->stream);  // warm-up
+  if (e == hipSuccess) e = hipEventRecord(e0, ctx->stream);
+  for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_copy_probe(a, b, bytes, ctx->stream);
+  if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  if (e != hipSuccess) return hip_fail(ctx, e, "kmi_measure_copy_bandwidth");
+  *gbps = 2.0 * double(bytes) * reps / (double(ms) * 1e-3) / 1e9;
   return KMI_OK;
 }
 

@@ -278,6 +278,23 @@ __global__ void k_finalize(const double* __restrict__ partials, int nb, int nv, 
   out[n] = s + comp;
 }
 
+// Plain device copy (float4 loads, non-temporal float4 stores): the practical 1:1
+// read/write HBM ceiling the stream–collide kernel is compared against.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(256) k_copy_probe(const f32x4* __restrict__ s,
+                                                    f32x4* __restrict__ d, long long n4) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
+       i += (long long)gridDim.x * blockDim.x)
+    __builtin_nontemporal_store(s[i], d + i);
+}
+
+hipError_t launch_copy_probe(const void* src, void* dst, long long bytes, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy_probe, dim3(1024), dim3(256), 0, s, static_cast<const f32x4*>(src),
+                     static_cast<f32x4*>(dst), bytes / 16);
+  return hipGetLastError();
+}
+
 // ---- host-side launchers ----------------------------------------------------------------
 #define KMI_ST_SWITCH(st, CALL)                                \
   switch (st) {                                                \

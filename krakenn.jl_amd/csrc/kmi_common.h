@@ -117,13 +117,22 @@ KMI_DEV R zeroth_moment(const R (&f)[Q]) {
 template <int S>
 struct Store;
 
+// st<true>: non-temporal store (streams past the caches; pays off when the populations
+// do not fit in the last-level cache).
+template <class E>
+KMI_DEV void store_elem(E* p, size_t i, E v, bool nt) {
+  if (nt) __builtin_nontemporal_store(v, p + i);
+  else p[i] = v;
+}
+
 template <>
 struct Store<KMI_STORE_F64> {
   using elem = double;
   using real = double;
   static constexpr bool shifted = false;
   KMI_DEV static real ld(const elem* p, size_t i) { return p[i]; }
-  KMI_DEV static void st(elem* p, size_t i, real v) { p[i] = v; }
+  template <bool NT = false>
+  KMI_DEV static void st(elem* p, size_t i, real v) { store_elem(p, i, v, NT); }
 };
 
 template <>
@@ -132,7 +141,8 @@ struct Store<KMI_STORE_F32> {
   using real = float;
   static constexpr bool shifted = false;
   KMI_DEV static real ld(const elem* p, size_t i) { return p[i]; }
-  KMI_DEV static void st(elem* p, size_t i, real v) { p[i] = v; }
+  template <bool NT = false>
+  KMI_DEV static void st(elem* p, size_t i, real v) { store_elem(p, i, v, NT); }
 };
 
 template <>
@@ -141,7 +151,8 @@ struct Store<KMI_STORE_F32S> {
   using real = float;
   static constexpr bool shifted = true;
   KMI_DEV static real ld(const elem* p, size_t i) { return p[i]; }
-  KMI_DEV static void st(elem* p, size_t i, real v) { p[i] = v; }
+  template <bool NT = false>
+  KMI_DEV static void st(elem* p, size_t i, real v) { store_elem(p, i, v, NT); }
 };
 
 constexpr float kF16Scale = 32768.0f;
@@ -153,7 +164,10 @@ struct Store<KMI_STORE_F16S> {
   using real = float;
   static constexpr bool shifted = true;
   KMI_DEV static real ld(const elem* p, size_t i) { return float(p[i]) * kF16InvScale; }
-  KMI_DEV static void st(elem* p, size_t i, real v) { p[i] = _Float16(v * kF16Scale); }
+  template <bool NT = false>
+  KMI_DEV static void st(elem* p, size_t i, real v) {
+    store_elem(p, i, _Float16(v * kF16Scale), NT);
+  }
 };
 
 inline size_t storage_bytes(int s) {

@@ -327,15 +327,16 @@ KMI_DEV void thermal_faces(R (&g)[9], const StepArgs& a, int i, int j) {
 
 // ---- the step kernel ----------------------------------------------------------------------
 // BND = false: every face periodic, no mask, no closures (the Taylor–Green fast path).
-template <int Q, int ST, int COLL, bool TH, bool BND>
+// NTS: non-temporal population stores.  Work-group = bx × by cells of one z-plane.
+template <int Q, int ST, int COLL, bool TH, bool BND, bool NTS>
 __global__ void __launch_bounds__(256) k_step(const StepArgs a) {
   using S = Store<ST>;
   using E = typename S::elem;
   using R = typename S::real;
   constexpr bool SH = S::shifted;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.nx) return;
-  const int j = blockIdx.y;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= a.nx || j >= a.ny) return;
   const int k = (Q == 19) ? int(blockIdx.z) : 0;
   const int nx = a.nx, ny = a.ny;
   const long long P = a.plane;
@@ -459,7 +460,7 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs a) {
     }
     sfor<9>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
-      S::st(go, self + q * P, go_r[q]);
+      S::template st<NTS>(go, self + q * P, go_r[q]);
     });
   } else {
     if (solid) {
@@ -476,47 +477,48 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs a) {
   }
   sfor<Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    S::st(fo, self + q * P, fo_r[q]);
+    S::template st<NTS>(fo, self + q * P, fo_r[q]);
   });
 }
 
 // ---- dispatch ---------------------------------------------------------------------------
 template <int Q, int ST, int COLL, bool TH, bool BND>
-static hipError_t launch(const StepArgs& a, dim3 g, dim3 b, hipStream_t s) {
-  hipLaunchKernelGGL((k_step<Q, ST, COLL, TH, BND>), g, b, 0, s, a);
+static hipError_t launch(bool nts, const StepArgs& a, dim3 g, dim3 b, hipStream_t s) {
+  if (nts) hipLaunchKernelGGL((k_step<Q, ST, COLL, TH, BND, true>), g, b, 0, s, a);
+  else hipLaunchKernelGGL((k_step<Q, ST, COLL, TH, BND, false>), g, b, 0, s, a);
   return hipGetLastError();
 }
 
 template <int Q, int ST>
-static hipError_t launch_st(int coll, bool th, bool bnd, const StepArgs& a, dim3 g, dim3 b,
-                            hipStream_t s) {
+static hipError_t launch_st(int coll, bool th, bool bnd, bool nts, const StepArgs& a, dim3 g,
+                            dim3 b, hipStream_t s) {
   if constexpr (Q == 9) {
-    if (th) return bnd ? launch<9, ST, KMI_BGK, true, true>(a, g, b, s)
-                       : launch<9, ST, KMI_BGK, true, false>(a, g, b, s);
-    if (coll == KMI_MRT) return bnd ? launch<9, ST, KMI_MRT, false, true>(a, g, b, s)
-                                    : launch<9, ST, KMI_MRT, false, false>(a, g, b, s);
+    if (th) return bnd ? launch<9, ST, KMI_BGK, true, true>(nts, a, g, b, s)
+                       : launch<9, ST, KMI_BGK, true, false>(nts, a, g, b, s);
+    if (coll == KMI_MRT) return bnd ? launch<9, ST, KMI_MRT, false, true>(nts, a, g, b, s)
+                                    : launch<9, ST, KMI_MRT, false, false>(nts, a, g, b, s);
   }
-  if (coll == KMI_TRT) return bnd ? launch<Q, ST, KMI_TRT, false, true>(a, g, b, s)
-                                  : launch<Q, ST, KMI_TRT, false, false>(a, g, b, s);
-  return bnd ? launch<Q, ST, KMI_BGK, false, true>(a, g, b, s)
-             : launch<Q, ST, KMI_BGK, false, false>(a, g, b, s);
+  if (coll == KMI_TRT) return bnd ? launch<Q, ST, KMI_TRT, false, true>(nts, a, g, b, s)
+                                  : launch<Q, ST, KMI_TRT, false, false>(nts, a, g, b, s);
+  return bnd ? launch<Q, ST, KMI_BGK, false, true>(nts, a, g, b, s)
+             : launch<Q, ST, KMI_BGK, false, false>(nts, a, g, b, s);
 }
 
 template <int Q>
-static hipError_t launch_q(int st, int coll, bool th, bool bnd, const StepArgs& a, dim3 g,
-                           dim3 b, hipStream_t s) {
+static hipError_t launch_q(int st, int coll, bool th, bool bnd, bool nts, const StepArgs& a,
+                           dim3 g, dim3 b, hipStream_t s) {
   switch (st) {
-    case KMI_STORE_F64: return launch_st<Q, KMI_STORE_F64>(coll, th, bnd, a, g, b, s);
-    case KMI_STORE_F32: return launch_st<Q, KMI_STORE_F32>(coll, th, bnd, a, g, b, s);
-    case KMI_STORE_F32S: return launch_st<Q, KMI_STORE_F32S>(coll, th, bnd, a, g, b, s);
-    default: return launch_st<Q, KMI_STORE_F16S>(coll, th, bnd, a, g, b, s);
+    case KMI_STORE_F64: return launch_st<Q, KMI_STORE_F64>(coll, th, bnd, nts, a, g, b, s);
+    case KMI_STORE_F32: return launch_st<Q, KMI_STORE_F32>(coll, th, bnd, nts, a, g, b, s);
+    case KMI_STORE_F32S: return launch_st<Q, KMI_STORE_F32S>(coll, th, bnd, nts, a, g, b, s);
+    default: return launch_st<Q, KMI_STORE_F16S>(coll, th, bnd, nts, a, g, b, s);
   }
 }
 
-hipError_t launch_step(int q, int st, int coll, bool th, bool bnd, const StepArgs& a, dim3 g,
-                       dim3 b, hipStream_t s) {
-  return q == 9 ? launch_q<9>(st, coll, th, bnd, a, g, b, s)
-                : launch_q<19>(st, coll, false, bnd, a, g, b, s);
+hipError_t launch_step(int q, int st, int coll, bool th, bool bnd, bool nts, const StepArgs& a,
+                       dim3 g, dim3 b, hipStream_t s) {
+  return q == 9 ? launch_q<9>(st, coll, th, bnd, nts, a, g, b, s)
+                : launch_q<19>(st, coll, false, bnd, nts, a, g, b, s);
 }
 
 }  // namespace kmi

@@ -92,6 +92,10 @@ SIGNATURES = {
     "kmi_reduce": (c_int, [c_void_p, c_int32, c_int32, POINTER(c_double)]),
     "kmi_lattice_steps": (c_int, [c_void_p, POINTER(c_int64)]),
     "kmi_lattice_device_bytes": (c_int, [c_void_p, POINTER(c_int64)]),
+    "kmi_lattice_set_launch": (c_int, [c_void_p, c_int32, c_int32, c_int32]),
+    "kmi_lattice_get_launch": (c_int, [c_void_p, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)]),
+    "kmi_lattice_autotune": (c_int, [c_void_p, c_int64, POINTER(c_double)]),
+    "kmi_measure_copy_bandwidth": (c_int, [c_void_p, c_int64, c_int32, POINTER(c_double)]),
 }
 
 _lib = None

@@ -49,6 +49,12 @@ class MI355XBackend:
     def synchronize(self):
         L.check(L.load().kmi_ctx_synchronize(self.ctx), self.ctx)
 
+    def copy_bandwidth(self, nbytes=2 * 1024 ** 3, reps=20):
+        """Measured device copy bandwidth in GB/s (read + write counted)."""
+        g = c_double()
+        L.check(L.load().kmi_measure_copy_bandwidth(self.ctx, int(nbytes), int(reps), byref(g)), self.ctx)
+        return g.value
+
     @staticmethod
     def device_count() -> int:
         n = c_int()
@@ -186,6 +192,22 @@ class Lattice:
         s = c_int64()
         self._chk(L.load().kmi_lattice_steps(self._h, byref(s)))
         return s.value
+
+    # --- launch tuning (results never depend on it) ---
+    def set_launch(self, bx, by=1, nontemporal=False):
+        self._chk(L.load().kmi_lattice_set_launch(self._h, int(bx), int(by), int(bool(nontemporal))))
+
+    @property
+    def launch(self):
+        bx, by, nt = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        self._chk(L.load().kmi_lattice_get_launch(self._h, byref(bx), byref(by), byref(nt)))
+        return {"bx": bx.value, "by": by.value, "nontemporal": bool(nt.value)}
+
+    def autotune(self, steps_per_trial=64):
+        """Pick the fastest launch shape / store policy on this lattice; state is restored."""
+        ms = c_double()
+        self._chk(L.load().kmi_lattice_autotune(self._h, int(steps_per_trial), byref(ms)))
+        return ms.value
 
     @property
     def device_bytes(self):

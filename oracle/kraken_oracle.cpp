@@ -825,9 +825,10 @@ struct Axis {
 // replaced by the node's own opposite population (halfway bounce-back; the reference's
 // `ifelse(inside, pull, f_in[opp])`), plus the Ladd term 6 w_q ρ_w (c_q·u_w) of every
 // crossed MOVING_WALL face (stream_periodic_3d.jl:97-166).  Zou–He faces pull with
-// bounce-back as in stream_2d! before the closure overwrites the unknowns.
+// bounce-back as in stream_2d! before the closure overwrites the unknowns.  The Ladd
+// term is a momentum correction: thermal populations (ladd = false) bounce back plainly.
 template <class T>
-static void pull_generic(const kmi_desc& d, T* fo, const T* fi) {
+static void pull_generic(const kmi_desc& d, T* fo, const T* fi, bool ladd = true) {
   const int Q = d.stencil == KMI_D2Q9 ? 9 : 19;
   const int64_t nx = d.nx, ny = d.ny, nz = d.stencil == KMI_D2Q9 ? 1 : d.nz;
   const size_t P = size_t(nx * ny * nz);
@@ -870,7 +871,7 @@ static void pull_generic(const kmi_desc& d, T* fo, const T* fi) {
             for (int a = 0; a < 3; ++a) {
               if (crossed[a] < 0) continue;
               const kmi_face& F = d.faces[crossed[a]];
-              if (F.kind != KMI_FACE_MOVING_WALL) continue;
+              if (!ladd || F.kind != KMI_FACE_MOVING_WALL) continue;
               T cu = cdotu<T>(cq[0], cq[1], cq[2], T(F.u[0]), T(F.u[1]), T(F.u[2]));
               v = v + T(six_w(w[q])) * T(F.rho) * cu;
             }
@@ -910,7 +911,7 @@ static void generic_step(const kmi_desc& d, const uint8_t* solid, T*& f, T*& f2,
   pull_generic(d, f2, f);
   if (d.stencil == KMI_D2Q9) zou_he_faces(d, f2);
   if (d.thermal) {
-    pull_generic(d, g2, g);
+    pull_generic(d, g2, g, false);
     bool dir[4];
     double Tf[4];
     for (int a = 0; a < 4; ++a) {

@@ -224,6 +224,32 @@ def test_graph_replay_equals_eager_steps(kmi, backend):
         assert np.array_equal(fo, out[0][0]) and np.array_equal(go, out[0][1])
 
 
+def test_autotune_restores_state_and_launch_shape_is_invisible(kmi, backend):
+    desc, shape, mask, thermal, steps = _case(kmi, "d2q9_rb_split", "f32")
+    f, g = _init(shape, thermal, seed=6)
+    outs = []
+    for mode in ("default", "tuned", (64, 1, 0), (16, 4, 1), (32, 8, 0), (128, 2, 1)):
+        with kmi.Lattice(desc, backend) as lat:
+            lat.upload(f)
+            lat.upload(g, "g")
+            lat.step(10)
+            before = (lat.download(), lat.download("g"))
+            if mode == "tuned":
+                lat.autotune(32)
+                assert lat.steps == 10
+                assert np.array_equal(lat.download(), before[0]) and np.array_equal(lat.download("g"), before[1])
+            elif mode != "default":
+                lat.set_launch(*mode)
+                assert lat.launch == {"bx": mode[0], "by": mode[1], "nontemporal": bool(mode[2])}
+            lat.step(50)
+            outs.append((lat.download(), lat.download("g")))
+    for fo, go in outs[1:]:
+        assert np.array_equal(fo, outs[0][0]) and np.array_equal(go, outs[0][1])
+    with kmi.Lattice(desc, backend) as lat:
+        with pytest.raises(kmi.KrakenError):
+            lat.set_launch(48, 1)
+
+
 def test_invalid_descriptor_raises(kmi, backend):
     d = kmi.make_desc("d3q19", 8, 8, 8, collision="mrt")
     with pytest.raises(kmi.KrakenError) as e:
