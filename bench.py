@@ -90,6 +90,15 @@ def main():
 
     ms, wall, mass, launch = run(args.storage, args.steps, args.warmup)
     copy_gbps = kmi.MI355XBackend().copy_bandwidth(2 * 1024 ** 3, 20)
+    # HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated on a
+    # copy kernel of known size), recorded for this workload in profiles/traffic.json.
+    traffic = None
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as fh:
+            rec = json.load(fh).get(f"d3q19_trt_{args.storage}_{N}")
+        traffic = rec["bytes_per_launch"] if rec else None
+    except (OSError, ValueError, KeyError):
+        traffic = None
     cells = N ** 3
     mlups = cells * args.steps / wall / 1e6
     per_launch_s = ms / 1e3 / args.steps
@@ -112,7 +121,7 @@ def main():
                                f"U0={U0}, nu={nu}, storage={args.storage}",
                    "cells": cells, "storage": args.storage, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
                      "bytes_per_launch": alg_bytes, "avg_launch_ms": round(per_launch_s * 1e3, 5),
                      "copy_measured": round(copy_gbps, 1),
                      "frac_of_copy": round(achieved / copy_gbps, 4)},
