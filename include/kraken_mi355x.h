@@ -226,6 +226,62 @@ int kmi_lattice_autotune(kmi_lattice* lat, int64_t steps_per_trial, double* best
  * read + write, averaged over `reps` copies: the practical HBM ceiling of a 1:1 kernel. */
 int kmi_measure_copy_bandwidth(kmi_ctx* ctx, int64_t bytes, int32_t reps, double* gbps);
 
+/* ---- batched lid-driven cavities (no reference counterpart: the reference runs one
+ *      cavity per driver call, e.g. test/test_mrt.jl:57-63 and basic.jl:112-144) -------
+ * An ensemble of square D2Q9 cavities of uneven side N_c and viscosity nu_c advanced
+ * together: every step is ONE launch over a packed list of tiles (tw × th cells of one
+ * cavity) walked by a persistent grid; tiles are ordered so that neighbouring tiles
+ * share an XCD.  Each cavity: halfway bounce-back walls, lid on the north face
+ * (Zou–He as run_cavity_2d, or Ladd), BGK / TRT / MRT; per-cavity residual r_u every
+ * `check_every` steps; converged or diverged cavities are retired from the tile list. */
+typedef struct kmi_batch kmi_batch;
+
+typedef struct {
+  int32_t storage;    /* kmi_storage */
+  int32_t collision;  /* kmi_collision (rates from each cavity's nu; MRT s_e/s_eps/s_q below) */
+  int32_t lid;        /* KMI_FACE_ZOUHE_VELOCITY or KMI_FACE_MOVING_WALL */
+  int32_t pad;
+  double u_lid;
+  double s_e, s_eps, s_q; /* MRT (collide_mrt_2d.jl:118 defaults 1.4, 1.4, 1.2) */
+  double Lambda;          /* TRT magic parameter (3/16) */
+} kmi_batch_desc;
+
+typedef enum {
+  KMI_CAV_RUNNING = 0,
+  KMI_CAV_CONVERGED = 1,
+  KMI_CAV_DIVERGED = 2,   /* non-finite residual: retired without touching the others */
+  KMI_CAV_MAX_STEPS = 3
+} kmi_cavity_status;
+
+int kmi_batch_create(kmi_ctx* ctx, const kmi_batch_desc* desc, int32_t ncav, const int32_t* n,
+                     const double* nu, kmi_batch** out);
+int kmi_batch_destroy(kmi_batch* b);
+/* Per-cavity state in the reference layout f[i, j, q] (n_c × n_c × 9). */
+int kmi_batch_upload(kmi_batch* b, int32_t cavity, const void* host, int32_t host_dtype);
+int kmi_batch_download(kmi_batch* b, int32_t cavity, void* host, int32_t host_dtype);
+int kmi_batch_init_equilibrium(kmi_batch* b, int32_t cavity, const double* rho, const double* ux,
+                               const double* uy);
+int kmi_batch_moments(kmi_batch* b, int32_t cavity, double* rho, double* ux, double* uy);
+/* Mark every cavity running again with zero steps (populations untouched). */
+int kmi_batch_reset_status(kmi_batch* b);
+/* Advance all running cavities `nsteps` steps (no residual checks). */
+int kmi_batch_step(kmi_batch* b, int64_t nsteps);
+/* Step until every cavity is converged (r_u < tol), diverged or at max_steps, checking
+ * every `check_every` (>= 2) steps; elapsed device milliseconds in *ms (may be NULL). */
+int kmi_batch_run(kmi_batch* b, double tol, int64_t check_every, int64_t max_steps, double* ms);
+/* Arrays of ncav entries (any may be NULL). */
+int kmi_batch_status(const kmi_batch* b, int32_t* status, int64_t* steps, double* residual);
+/* Tiles of tw × th cells (tw*th = 256, tw in {32,64,128,256}), `groups` persistent
+ * work-groups (0: one per tile slot), non-temporal stores on/off. */
+int kmi_batch_set_launch(kmi_batch* b, int32_t tw, int32_t th, int32_t groups, int32_t nontemporal);
+int kmi_batch_get_launch(const kmi_batch* b, int32_t* tw, int32_t* th, int32_t* groups,
+                         int32_t* nontemporal);
+/* Time every candidate tile shape × group count × store policy and keep the fastest;
+ * populations, parity, status and step counts are restored. */
+int kmi_batch_autotune(kmi_batch* b, int64_t steps_per_trial, double* best_ms_per_step);
+/* Cells of the running cavities (the work of one step). */
+int kmi_batch_active_cells(const kmi_batch* b, int64_t* cells);
+
 #ifdef __cplusplus
 }
 #endif

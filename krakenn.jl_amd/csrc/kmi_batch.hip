@@ -1,0 +1,759 @@
+// kmi_batch.hip — batched square lid-driven D2Q9 cavities of uneven size in one launch.
+//
+// Populations of every cavity live in one packed allocation (cavity c: 9 planes of
+// n_c² cells at element offset foff_c, the reference layout per cavity).  A step is ONE
+// launch of k_batch_step over a list of tile slots: each slot names (cavity, x0, y0) of a
+// tw × th tile; a persistent grid of `groups` work-groups walks the slots with stride
+// `groups`.  Slots are permuted so that slot s runs on XCD s mod 8 and each XCD owns one
+// contiguous run of the cavity-major tile order (neighbouring tiles share an L2).
+//
+// The per-cell arithmetic is the single-lattice kernel's (kmi_cell.h), so a cavity of the
+// batch is bit-identical to the same cavity run alone (tested).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "kmi_cell.h"
+#include "kmi_internal.h"
+
+namespace kmi {
+
+struct CavRec {
+  long long foff;  // element offset of the cavity's populations
+  long long coff;  // cell offset (auxiliary per-cell arrays)
+  int n;
+  int pad;
+  double om;  // BGK ω / MRT s_ν / TRT s⁺
+  double sm;  // TRT s⁻
+};
+
+struct BatchArgs {
+  const void* fi;
+  void* fo;
+  const CavRec* cav;
+  const int* slots;  // 3 ints per slot: cavity (-1: empty), x0, y0
+  int nslots;
+  int tw;
+  int lid;  // KMI_FACE_ZOUHE_VELOCITY or KMI_FACE_MOVING_WALL
+  int pad;
+  double u_lid;
+  double s_e, s_eps, s_q;
+  double* uprev;           // 2 × total cells (ux, uy)
+  long long total_cells;
+  double* partials;        // 2 per slot
+};
+
+template <int ST, int COLL, bool NTS>
+__global__ void __launch_bounds__(256) k_batch_step(const BatchArgs a) {
+  using S = Store<ST>;
+  using E = typename S::elem;
+  using R = typename S::real;
+  constexpr bool SH = S::shifted;
+  const int lx = threadIdx.x % a.tw, ly = threadIdx.x / a.tw;
+  for (int t = blockIdx.x; t < a.nslots; t += gridDim.x) {
+    const int c = a.slots[3 * t];
+    if (c < 0) continue;
+    const CavRec cv = a.cav[c];
+    const int n = cv.n;
+    const int i = a.slots[3 * t + 1] + lx, j = a.slots[3 * t + 2] + ly;
+    if (i >= n || j >= n) continue;
+    const long long P = (long long)n * n;
+    const E* __restrict__ src = static_cast<const E*>(a.fi) + cv.foff;
+    E* __restrict__ dst = static_cast<E*>(a.fo) + cv.foff;
+    const int self = i + n * j;
+    R fp[9], fo[9];
+    // pull with halfway bounce-back on all four faces (stream_2d_kernel!,
+    // collide_stream_2d.jl:75-101); Ladd term on the north lid when it is a moving wall
+    sfor<9>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      constexpr int cx = Lat<9>::cx[q], cy = Lat<9>::cy[q];
+      constexpr int o = Lat<9>::opp[q];
+      const bool ok = (cx > 0 ? i > 0 : (cx < 0 ? i < n - 1 : true)) &&
+                      (cy > 0 ? j > 0 : (cy < 0 ? j < n - 1 : true));
+      const long long addr = ok ? (long long)((i - cx) + n * (j - cy)) + q * P : (long long)self + o * P;
+      R v = S::ld(src, addr);
+      if constexpr (cy < 0) {
+        if (!ok && a.lid == KMI_FACE_MOVING_WALL && j == n - 1)
+          v = v + R(six_w(Lat<9>::w[q])) * R(1.0) * cdotu<cx, cy, 0>(R(a.u_lid), R(0.0), R(0.0));
+      }
+      fp[q] = v;
+    });
+    if (a.lid == KMI_FACE_ZOUHE_VELOCITY && j == n - 1) zh_north<SH>(fp, R(a.u_lid));
+    if constexpr (COLL == KMI_BGK) collide_bgk<9, SH>(fp, fo, R(cv.om));
+    else if constexpr (COLL == KMI_TRT) collide_trt<9, SH>(fp, fo, R(cv.om), R(cv.sm));
+    else collide_mrt<SH>(fp, fo, R(a.s_e), R(a.s_eps), R(a.s_q), R(cv.om));
+    sfor<9>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      S::template st<NTS>(dst, self + q * P, fo[q]);
+    });
+  }
+}
+
+// Velocity of a cell from the stored populations, FP64 (compute_macroscopic_2d order).
+template <int ST>
+KMI_DEV void cell_u(const void* f, const CavRec& cv, int self, double& ux, double& uy) {
+  using S = Store<ST>;
+  using E = typename S::elem;
+  const E* p = static_cast<const E*>(f) + cv.foff;
+  const long long P = (long long)cv.n * cv.n;
+  double fp[9];
+  sfor<9>([&](auto qc) { fp[decltype(qc)::value] = double(S::ld(p, self + decltype(qc)::value * P)); });
+  const double r = S::shifted ? 1.0 + zeroth_moment<9>(fp) : zeroth_moment<9>(fp);
+  ux = first_moment<9, 0>(fp) / r;
+  uy = first_moment<9, 1>(fp) / r;
+}
+
+// mode 0: u → uprev;  mode 1: per-slot partials of Σ|u − uprev|² and Σ|u|².
+template <int ST>
+__global__ void __launch_bounds__(256) k_batch_resid(const BatchArgs a, const void* f, int mode) {
+  __shared__ double lds[4][2];
+  const int lx = threadIdx.x % a.tw, ly = threadIdx.x / a.tw;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int t = blockIdx.x; t < a.nslots; t += gridDim.x) {
+    const int c = a.slots[3 * t];  // uniform over the work-group
+    if (c < 0) continue;
+    const CavRec cv = a.cav[c];
+    const int i = a.slots[3 * t + 1] + lx, j = a.slots[3 * t + 2] + ly;
+    double d2 = 0.0, u2 = 0.0;
+    if (i < cv.n && j < cv.n) {
+      const int self = i + cv.n * j;
+      double ux, uy;
+      cell_u<ST>(f, cv, self, ux, uy);
+      const long long k = cv.coff + self;
+      if (mode == 0) {
+        a.uprev[k] = ux;
+        a.uprev[a.total_cells + k] = uy;
+      } else {
+        const double dx = ux - a.uprev[k], dy = uy - a.uprev[a.total_cells + k];
+        d2 = dx * dx + dy * dy;
+        u2 = ux * ux + uy * uy;
+      }
+    }
+    if (mode == 0) continue;
+    for (int off = 32; off > 0; off >>= 1) {
+      d2 += __shfl_xor(d2, off, 64);
+      u2 += __shfl_xor(u2, off, 64);
+    }
+    __syncthreads();  // previous slot's LDS reads are done
+    if (lane == 0) {
+      lds[wave][0] = d2;
+      lds[wave][1] = u2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double s0 = 0.0, s1 = 0.0;
+      for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) {
+        s0 += lds[w][0];
+        s1 += lds[w][1];
+      }
+      a.partials[2 * t] = s0;
+      a.partials[2 * t + 1] = s1;
+    }
+  }
+}
+
+// One thread per cavity: Neumaier sum of its slots' partials (fixed order) → r_u.
+__global__ void k_batch_final(const double* __restrict__ partials, const int* __restrict__ cav_slots,
+                              const int* __restrict__ cav_range, const int* __restrict__ active,
+                              int ncav, double* __restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncav) return;
+  if (!active[c]) return;
+  double s[2] = {0.0, 0.0}, comp[2] = {0.0, 0.0};
+  for (int k = cav_range[c]; k < cav_range[c + 1]; ++k) {
+    const int t = cav_slots[k];
+    for (int v = 0; v < 2; ++v) {
+      const double x = partials[2 * t + v];
+      const double y = s[v] + x;
+      comp[v] += fabs(s[v]) >= fabs(x) ? (s[v] - y) + x : (x - y) + s[v];
+      s[v] = y;
+    }
+  }
+  const double d2 = s[0] + comp[0], u2 = s[1] + comp[1];
+  out[3 * c] = d2;
+  out[3 * c + 1] = u2;
+  out[3 * c + 2] = sqrt(d2 / u2);  // gnf_periodic_steady.jl:192
+}
+
+#define KMI_BATCH_ST(st, CALL)                                               \
+  switch (st) {                                                              \
+    case KMI_STORE_F64: { constexpr int ST = KMI_STORE_F64; CALL; } break;   \
+    case KMI_STORE_F32: { constexpr int ST = KMI_STORE_F32; CALL; } break;   \
+    case KMI_STORE_F32S: { constexpr int ST = KMI_STORE_F32S; CALL; } break; \
+    default: { constexpr int ST = KMI_STORE_F16S; CALL; } break;             \
+  }
+
+static hipError_t launch_batch_step(int st, int coll, bool nts, const BatchArgs& a, int groups,
+                                    hipStream_t s) {
+  const dim3 g(static_cast<unsigned>(groups)), b(256);
+#define KMI_BATCH_COLL(ST)                                                                        \
+  if (coll == KMI_BGK) {                                                                         \
+    if (nts) hipLaunchKernelGGL((k_batch_step<ST, KMI_BGK, true>), g, b, 0, s, a);               \
+    else hipLaunchKernelGGL((k_batch_step<ST, KMI_BGK, false>), g, b, 0, s, a);                  \
+  } else if (coll == KMI_TRT) {                                                                  \
+    if (nts) hipLaunchKernelGGL((k_batch_step<ST, KMI_TRT, true>), g, b, 0, s, a);               \
+    else hipLaunchKernelGGL((k_batch_step<ST, KMI_TRT, false>), g, b, 0, s, a);                  \
+  } else {                                                                                       \
+    if (nts) hipLaunchKernelGGL((k_batch_step<ST, KMI_MRT, true>), g, b, 0, s, a);               \
+    else hipLaunchKernelGGL((k_batch_step<ST, KMI_MRT, false>), g, b, 0, s, a);                  \
+  }
+  KMI_BATCH_ST(st, KMI_BATCH_COLL(ST));
+#undef KMI_BATCH_COLL
+  return hipGetLastError();
+}
+
+static hipError_t launch_batch_resid(int st, const BatchArgs& a, const void* f, int mode,
+                                     int groups, hipStream_t s) {
+  KMI_BATCH_ST(st, hipLaunchKernelGGL((k_batch_resid<ST>), dim3(unsigned(groups)), dim3(256), 0, s, a, f, mode));
+  return hipGetLastError();
+}
+
+}  // namespace kmi
+
+using namespace kmi;
+
+struct kmi_batch {
+  kmi_ctx* ctx = nullptr;
+  kmi_batch_desc desc{};
+  int ncav = 0;
+  std::vector<int> n;
+  std::vector<double> nu;
+  std::vector<CavRec> cav;
+  CavRec* cav_d = nullptr;
+  size_t esize = 4;
+  long long total_pop = 0, total_cells = 0;
+  void* f[2] = {nullptr, nullptr};
+  int cur = 0;
+  // tiles
+  int tw = 64, th = 4, groups = 0, nts = 0;
+  bool tiles_dirty = true;
+  int nslots = 0;
+  int* slots_d = nullptr;
+  size_t slots_cap = 0;
+  int* cav_slots_d = nullptr;
+  size_t cav_slots_cap = 0;
+  int* cav_range_d = nullptr;
+  int* active_d = nullptr;
+  double* uprev = nullptr;
+  double* partials = nullptr;
+  size_t partials_cap = 0;
+  double* resid_d = nullptr;  // 3 per cavity
+  std::vector<int> status;
+  std::vector<int64_t> steps;
+  std::vector<double> resid;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  int64_t graph_check = 0;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+static void batch_drop_graphs(kmi_batch* B) {
+  for (auto& g : B->graph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
+static int batch_groups(const kmi_batch* B) {
+  int g = B->groups > 0 ? std::min(B->groups, B->nslots) : B->nslots;
+  g = ((g + 7) / 8) * 8;  // multiple of 8: slot s keeps running on XCD s mod 8
+  return std::max(g, 8);
+}
+
+// Cavity-major tile order, then slot permutation so XCD x owns run x of that order.
+static int batch_build_tiles(kmi_batch* B) {
+  kmi_ctx* c = B->ctx;
+  std::vector<int> tiles;  // (cavity, x0, y0)
+  std::vector<int> first(B->ncav + 1, 0);
+  for (int cv = 0; cv < B->ncav; ++cv) {
+    first[cv] = int(tiles.size() / 3);
+    if (B->status[cv] != KMI_CAV_RUNNING) continue;
+    const int n = B->n[cv];
+    for (int y = 0; y < n; y += B->th)
+      for (int x = 0; x < n; x += B->tw) {
+        tiles.push_back(cv);
+        tiles.push_back(x);
+        tiles.push_back(y);
+      }
+  }
+  const int T = int(tiles.size() / 3);
+  first[B->ncav] = T;
+  const int chunk = (T + 7) / 8;
+  B->nslots = std::max(8 * chunk, 8);
+  std::vector<int> slots(size_t(3) * B->nslots, -1);
+  std::vector<int> slot_of(T, -1);
+  for (int s = 0; s < B->nslots; ++s) {
+    const int idx = (s % 8) * chunk + s / 8;
+    if (idx < T) {
+      for (int k = 0; k < 3; ++k) slots[3 * s + k] = tiles[3 * idx + k];
+      slot_of[idx] = s;
+    }
+  }
+  // per-cavity slot lists in tile order (for the deterministic residual sums)
+  std::vector<int> cav_slots(std::max(T, 1));
+  std::vector<int> range(B->ncav + 1, 0), active(B->ncav, 0);
+  for (int cv = 0; cv < B->ncav; ++cv) {
+    range[cv] = first[cv];
+    active[cv] = B->status[cv] == KMI_CAV_RUNNING;
+  }
+  range[B->ncav] = T;
+  for (int k = 0; k < T; ++k) cav_slots[k] = slot_of[k];
+  auto grow = [&](void** p, size_t& cap, size_t need) -> hipError_t {
+    if (need <= cap) return hipSuccess;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    cap = 0;
+    hipError_t e = hipMalloc(p, need);
+    if (e == hipSuccess) cap = need;
+    return e;
+  };
+  KMI_HIP(c, grow((void**)&B->slots_d, B->slots_cap, slots.size() * sizeof(int)));
+  KMI_HIP(c, grow((void**)&B->cav_slots_d, B->cav_slots_cap, cav_slots.size() * sizeof(int)));
+  KMI_HIP(c, grow((void**)&B->partials, B->partials_cap, size_t(2) * B->nslots * sizeof(double)));
+  KMI_HIP(c, hipMemcpy(B->slots_d, slots.data(), slots.size() * sizeof(int), hipMemcpyHostToDevice));
+  KMI_HIP(c, hipMemcpy(B->cav_slots_d, cav_slots.data(), cav_slots.size() * sizeof(int), hipMemcpyHostToDevice));
+  KMI_HIP(c, hipMemcpy(B->cav_range_d, range.data(), range.size() * sizeof(int), hipMemcpyHostToDevice));
+  KMI_HIP(c, hipMemcpy(B->active_d, active.data(), active.size() * sizeof(int), hipMemcpyHostToDevice));
+  B->tiles_dirty = false;
+  batch_drop_graphs(B);
+  return KMI_OK;
+}
+
+static BatchArgs batch_args(const kmi_batch* B) {
+  BatchArgs a{};
+  a.fi = B->f[B->cur];
+  a.fo = B->f[B->cur ^ 1];
+  a.cav = B->cav_d;
+  a.slots = B->slots_d;
+  a.nslots = B->nslots;
+  a.tw = B->tw;
+  a.lid = B->desc.lid;
+  a.u_lid = B->desc.u_lid;
+  a.s_e = B->desc.s_e;
+  a.s_eps = B->desc.s_eps;
+  a.s_q = B->desc.s_q;
+  a.uprev = B->uprev;
+  a.total_cells = B->total_cells;
+  a.partials = B->partials;
+  return a;
+}
+
+static int batch_launch_step(kmi_batch* B, hipStream_t s) {
+  BatchArgs a = batch_args(B);
+  hipError_t e = launch_batch_step(B->desc.storage, B->desc.collision, B->nts != 0, a,
+                                   batch_groups(B), s);
+  if (e != hipSuccess) return hip_fail(B->ctx, e, "batch step launch");
+  B->cur ^= 1;
+  return KMI_OK;
+}
+
+static int batch_ensure_tiles(kmi_batch* B) {
+  return B->tiles_dirty ? batch_build_tiles(B) : KMI_OK;
+}
+
+static int batch_active(const kmi_batch* B) {
+  int k = 0;
+  for (int s : B->status) k += s == KMI_CAV_RUNNING;
+  return k;
+}
+
+// One check block: (check − 1) steps, snapshot u, 1 step, residual partials, per-cavity sums.
+static int batch_record_block(kmi_batch* B, int64_t check, hipStream_t s) {
+  for (int64_t k = 0; k + 1 < check; ++k)
+    if (int rc = batch_launch_step(B, s)) return rc;
+  BatchArgs a = batch_args(B);
+  hipError_t e = launch_batch_resid(B->desc.storage, a, B->f[B->cur], 0, batch_groups(B), s);
+  if (e != hipSuccess) return hip_fail(B->ctx, e, "batch snapshot");
+  if (int rc = batch_launch_step(B, s)) return rc;
+  a = batch_args(B);
+  e = launch_batch_resid(B->desc.storage, a, B->f[B->cur], 1, batch_groups(B), s);
+  if (e != hipSuccess) return hip_fail(B->ctx, e, "batch residual");
+  hipLaunchKernelGGL(k_batch_final, dim3(unsigned((B->ncav + 63) / 64)), dim3(64), 0, s,
+                     B->partials, B->cav_slots_d, B->cav_range_d, B->active_d, B->ncav, B->resid_d);
+  e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(B->ctx, e, "batch residual sum");
+  return KMI_OK;
+}
+
+extern "C" {
+
+int kmi_batch_create(kmi_ctx* ctx, const kmi_batch_desc* desc, int32_t ncav, const int32_t* n,
+                     const double* nu, kmi_batch** out) {
+  if (!ctx) return fail(nullptr, KMI_ERR_ARG, "kmi_batch_create: ctx is NULL");
+  if (!desc || !n || !nu || !out || ncav < 1)
+    return fail(ctx, KMI_ERR_ARG, "kmi_batch_create: bad arguments");
+  *out = nullptr;
+  if (desc->storage < KMI_STORE_F64 || desc->storage > KMI_STORE_F16S)
+    return fail(ctx, KMI_ERR_ARG, "kmi_batch_create: unknown storage");
+  if (desc->collision < KMI_BGK || desc->collision > KMI_MRT)
+    return fail(ctx, KMI_ERR_ARG, "kmi_batch_create: unknown collision");
+  if (desc->lid != KMI_FACE_ZOUHE_VELOCITY && desc->lid != KMI_FACE_MOVING_WALL)
+    return fail(ctx, KMI_ERR_ARG, "kmi_batch_create: lid must be Zou-He velocity or moving wall");
+  long long cells = 0;
+  for (int c = 0; c < ncav; ++c) {
+    if (n[c] < 2 || n[c] > 16384) return fail(ctx, KMI_ERR_ARG, "kmi_batch_create: cavity side out of range");
+    if (!(nu[c] > 0)) return fail(ctx, KMI_ERR_ARG, "kmi_batch_create: viscosity must be positive");
+    cells += (long long)n[c] * n[c];
+  }
+  if (double(cells) * 9 >= 2147483647.0)
+    return fail(ctx, KMI_ERR_UNSUPPORTED, "kmi_batch_create: more than 2^31 populations");
+  KMI_HIP(ctx, hipSetDevice(ctx->device));
+  kmi_batch* B = new (std::nothrow) kmi_batch();
+  if (!B) return fail(ctx, KMI_ERR_NOMEM, "kmi_batch_create: out of host memory");
+  B->ctx = ctx;
+  B->desc = *desc;
+  B->ncav = ncav;
+  B->n.assign(n, n + ncav);
+  B->nu.assign(nu, nu + ncav);
+  B->esize = storage_bytes(desc->storage);
+  long long off = 0, coff = 0;
+  for (int c = 0; c < ncav; ++c) {
+    CavRec r{};
+    r.foff = off;
+    r.coff = coff;
+    r.n = n[c];
+    r.om = 1.0 / (3.0 * nu[c] + 0.5);  // ω = s_ν = s⁺ = 1/(3ν + 1/2)
+    r.sm = 1.0 / (desc->Lambda / (3.0 * nu[c]) + 0.5);
+    B->cav.push_back(r);
+    off += 9LL * n[c] * n[c];
+    coff += (long long)n[c] * n[c];
+  }
+  B->total_pop = off;
+  B->total_cells = coff;
+  B->status.assign(ncav, KMI_CAV_RUNNING);
+  B->steps.assign(ncav, 0);
+  B->resid.assign(ncav, INFINITY);
+  hipError_t e = hipMalloc(&B->f[0], size_t(off) * B->esize);
+  if (e == hipSuccess) e = hipMalloc(&B->f[1], size_t(off) * B->esize);
+  if (e == hipSuccess) e = hipMalloc(&B->cav_d, sizeof(CavRec) * ncav);
+  if (e == hipSuccess) e = hipMalloc(&B->cav_range_d, sizeof(int) * (ncav + 1));
+  if (e == hipSuccess) e = hipMalloc(&B->active_d, sizeof(int) * ncav);
+  if (e == hipSuccess) e = hipMalloc(&B->resid_d, sizeof(double) * 3 * ncav);
+  if (e == hipSuccess) e = hipMalloc(&B->uprev, sizeof(double) * 2 * coff);
+  if (e == hipSuccess) e = hipMemcpy(B->cav_d, B->cav.data(), sizeof(CavRec) * ncav, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipEventCreate(&B->ev[0]);
+  if (e == hipSuccess) e = hipEventCreate(&B->ev[1]);
+  if (e != hipSuccess) {
+    kmi_batch_destroy(B);
+    return hip_fail(ctx, e, "kmi_batch_create");
+  }
+  // rest state f = w_q for every cavity (initialize_2d, basic.jl:64-70)
+  for (int c = 0; c < ncav; ++c) {
+    if (int rc = kmi_batch_init_equilibrium(B, c, nullptr, nullptr, nullptr)) {
+      kmi_batch_destroy(B);
+      return rc;
+    }
+  }
+  if (int rc = batch_build_tiles(B)) {
+    kmi_batch_destroy(B);
+    return rc;
+  }
+  *out = B;
+  return KMI_OK;
+}
+
+int kmi_batch_destroy(kmi_batch* B) {
+  if (!B) return KMI_OK;
+  (void)hipSetDevice(B->ctx->device);
+  (void)hipStreamSynchronize(B->ctx->stream);
+  batch_drop_graphs(B);
+  void* ptrs[] = {B->f[0], B->f[1], B->cav_d, B->slots_d, B->cav_slots_d, B->cav_range_d,
+                  B->active_d, B->uprev, B->partials, B->resid_d};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  for (auto ev : B->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  delete B;
+  return KMI_OK;
+}
+
+static int batch_check_cav(kmi_batch* B, int32_t c) {
+  if (!B) return fail(nullptr, KMI_ERR_ARG, "batch is NULL");
+  if (c < 0 || c >= B->ncav) return fail(B->ctx, KMI_ERR_ARG, "cavity index out of range");
+  return KMI_OK;
+}
+
+int kmi_batch_upload(kmi_batch* B, int32_t c, const void* host, int32_t dt) {
+  if (int rc = batch_check_cav(B, c)) return rc;
+  kmi_ctx* x = B->ctx;
+  if (!host || (dt != KMI_HOST_F64 && dt != KMI_HOST_F32)) return fail(x, KMI_ERR_ARG, "kmi_batch_upload: bad buffer");
+  KMI_HIP(x, hipSetDevice(x->device));
+  const long long nc = (long long)B->n[c] * B->n[c];
+  std::vector<double> tmp;
+  const double* src = static_cast<const double*>(host);
+  if (dt == KMI_HOST_F32) {
+    tmp.resize(size_t(9 * nc));
+    for (long long i = 0; i < 9 * nc; ++i) tmp[i] = static_cast<const float*>(host)[i];
+    src = tmp.data();
+  }
+  double* stage = nullptr;
+  KMI_HIP(x, hipMalloc(&stage, size_t(9 * nc) * sizeof(double)));
+  hipError_t e = hipMemcpyAsync(stage, src, size_t(9 * nc) * sizeof(double), hipMemcpyHostToDevice, x->stream);
+  char* base = static_cast<char*>(B->f[B->cur]) + B->cav[c].foff * B->esize;
+  if (e == hipSuccess) e = launch_to_storage(9, B->desc.storage, stage, base, nc, nc, 1.0, x->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(x->stream);
+  (void)hipFree(stage);
+  if (e != hipSuccess) return hip_fail(x, e, "kmi_batch_upload");
+  return KMI_OK;
+}
+
+int kmi_batch_download(kmi_batch* B, int32_t c, void* host, int32_t dt) {
+  if (int rc = batch_check_cav(B, c)) return rc;
+  kmi_ctx* x = B->ctx;
+  if (!host || (dt != KMI_HOST_F64 && dt != KMI_HOST_F32)) return fail(x, KMI_ERR_ARG, "kmi_batch_download: bad buffer");
+  KMI_HIP(x, hipSetDevice(x->device));
+  const long long nc = (long long)B->n[c] * B->n[c];
+  double* stage = nullptr;
+  KMI_HIP(x, hipMalloc(&stage, size_t(9 * nc) * sizeof(double)));
+  std::vector<double> tmp;
+  double* dst = static_cast<double*>(host);
+  if (dt == KMI_HOST_F32) {
+    tmp.resize(size_t(9 * nc));
+    dst = tmp.data();
+  }
+  const char* base = static_cast<const char*>(B->f[B->cur]) + B->cav[c].foff * B->esize;
+  hipError_t e = launch_from_storage(9, B->desc.storage, base, stage, nc, nc, 1.0, x->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dst, stage, size_t(9 * nc) * sizeof(double), hipMemcpyDeviceToHost, x->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(x->stream);
+  (void)hipFree(stage);
+  if (e != hipSuccess) return hip_fail(x, e, "kmi_batch_download");
+  if (dt == KMI_HOST_F32)
+    for (long long i = 0; i < 9 * nc; ++i) static_cast<float*>(host)[i] = float(tmp[i]);
+  return KMI_OK;
+}
+
+int kmi_batch_init_equilibrium(kmi_batch* B, int32_t c, const double* rho, const double* ux,
+                               const double* uy) {
+  if (int rc = batch_check_cav(B, c)) return rc;
+  kmi_ctx* x = B->ctx;
+  KMI_HIP(x, hipSetDevice(x->device));
+  const long long nc = (long long)B->n[c] * B->n[c];
+  const double* host[3] = {rho, ux, uy};
+  double* dev[3] = {nullptr, nullptr, nullptr};
+  double* block = nullptr;
+  int used = 0;
+  for (auto h : host) used += h != nullptr;
+  if (used) KMI_HIP(x, hipMalloc(&block, size_t(used) * nc * sizeof(double)));
+  hipError_t e = hipSuccess;
+  for (int k = 0, slot = 0; k < 3 && e == hipSuccess; ++k) {
+    if (!host[k]) continue;
+    dev[k] = block + size_t(slot++) * nc;
+    e = hipMemcpyAsync(dev[k], host[k], size_t(nc) * sizeof(double), hipMemcpyHostToDevice, x->stream);
+  }
+  char* base = static_cast<char*>(B->f[B->cur]) + B->cav[c].foff * B->esize;
+  if (e == hipSuccess) e = launch_init_eq(9, B->desc.storage, base, dev[0], dev[1], dev[2], nullptr, nc, nc, x->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(x->stream);
+  if (block) (void)hipFree(block);
+  if (e != hipSuccess) return hip_fail(x, e, "kmi_batch_init_equilibrium");
+  return KMI_OK;
+}
+
+int kmi_batch_moments(kmi_batch* B, int32_t c, double* rho, double* ux, double* uy) {
+  if (int rc = batch_check_cav(B, c)) return rc;
+  kmi_ctx* x = B->ctx;
+  KMI_HIP(x, hipSetDevice(x->device));
+  const long long nc = (long long)B->n[c] * B->n[c];
+  double* block = nullptr;
+  KMI_HIP(x, hipMalloc(&block, size_t(3) * nc * sizeof(double)));
+  const char* base = static_cast<const char*>(B->f[B->cur]) + B->cav[c].foff * B->esize;
+  hipError_t e = launch_moments(9, B->desc.storage, base, nullptr, block, block + nc, block + 2 * nc,
+                                nullptr, nullptr, nc, nc, 0.0, x->stream);
+  double* host[3] = {rho, ux, uy};
+  for (int k = 0; k < 3 && e == hipSuccess; ++k)
+    if (host[k]) e = hipMemcpyAsync(host[k], block + k * nc, size_t(nc) * sizeof(double), hipMemcpyDeviceToHost, x->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(x->stream);
+  (void)hipFree(block);
+  if (e != hipSuccess) return hip_fail(x, e, "kmi_batch_moments");
+  return KMI_OK;
+}
+
+int kmi_batch_reset_status(kmi_batch* B) {
+  if (!B) return fail(nullptr, KMI_ERR_ARG, "batch is NULL");
+  std::fill(B->status.begin(), B->status.end(), int(KMI_CAV_RUNNING));
+  std::fill(B->steps.begin(), B->steps.end(), 0);
+  std::fill(B->resid.begin(), B->resid.end(), INFINITY);
+  B->tiles_dirty = true;
+  return KMI_OK;
+}
+
+int kmi_batch_step(kmi_batch* B, int64_t nsteps) {
+  if (!B) return fail(nullptr, KMI_ERR_ARG, "batch is NULL");
+  if (nsteps < 0) return fail(B->ctx, KMI_ERR_ARG, "kmi_batch_step: nsteps must be >= 0");
+  kmi_ctx* x = B->ctx;
+  KMI_HIP(x, hipSetDevice(x->device));
+  if (int rc = batch_ensure_tiles(B)) return rc;
+  if (batch_active(B) == 0) return KMI_OK;
+  for (int64_t k = 0; k < nsteps; ++k)
+    if (int rc = batch_launch_step(B, x->stream)) return rc;
+  for (int c = 0; c < B->ncav; ++c)
+    if (B->status[c] == KMI_CAV_RUNNING) B->steps[c] += nsteps;
+  KMI_HIP(x, hipStreamSynchronize(x->stream));
+  return KMI_OK;
+}
+
+int kmi_batch_run(kmi_batch* B, double tol, int64_t check, int64_t max_steps, double* ms) {
+  if (!B) return fail(nullptr, KMI_ERR_ARG, "batch is NULL");
+  kmi_ctx* x = B->ctx;
+  if (check < 2 || max_steps < 1) return fail(x, KMI_ERR_ARG, "kmi_batch_run: check_every >= 2 and max_steps >= 1");
+  KMI_HIP(x, hipSetDevice(x->device));
+  hipStream_t s = x->stream;
+  if (B->graph_check != check) {
+    batch_drop_graphs(B);
+    B->graph_check = check;
+  }
+  KMI_HIP(x, hipEventRecord(B->ev[0], s));
+  std::vector<double> r(size_t(3) * B->ncav);
+  while (batch_active(B) > 0) {
+    if (int rc = batch_ensure_tiles(B)) return rc;
+    const int p = B->cur;
+    if (!B->graph[p]) {
+      hipGraph_t gph = nullptr;
+      KMI_HIP(x, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      int rc = batch_record_block(B, check, s);
+      hipError_t e = hipStreamEndCapture(s, &gph);
+      B->cur = p;  // capture recorded the launches only
+      if (rc != KMI_OK) {
+        if (gph) (void)hipGraphDestroy(gph);
+        return rc;
+      }
+      if (e != hipSuccess) return hip_fail(x, e, "batch graph capture");
+      e = hipGraphInstantiate(&B->graph[p], gph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(gph);
+      if (e != hipSuccess) return hip_fail(x, e, "batch graph instantiate");
+    }
+    KMI_HIP(x, hipGraphLaunch(B->graph[p], s));
+    B->cur = int((p + check) & 1);
+    KMI_HIP(x, hipMemcpyAsync(r.data(), B->resid_d, r.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    KMI_HIP(x, hipStreamSynchronize(s));
+    for (int c = 0; c < B->ncav; ++c) {
+      if (B->status[c] != KMI_CAV_RUNNING) continue;
+      B->steps[c] += check;
+      const double ru = r[3 * c + 2];
+      B->resid[c] = ru;
+      int st = KMI_CAV_RUNNING;
+      if (!std::isfinite(ru) && !(r[3 * c] == 0.0 && r[3 * c + 1] == 0.0)) st = KMI_CAV_DIVERGED;
+      else if (ru < tol || (r[3 * c] == 0.0 && r[3 * c + 1] == 0.0)) st = KMI_CAV_CONVERGED;
+      else if (B->steps[c] >= max_steps) st = KMI_CAV_MAX_STEPS;
+      if (st != KMI_CAV_RUNNING) {
+        B->status[c] = st;
+        B->tiles_dirty = true;
+      }
+    }
+  }
+  KMI_HIP(x, hipEventRecord(B->ev[1], s));
+  KMI_HIP(x, hipEventSynchronize(B->ev[1]));
+  float t = 0.f;
+  KMI_HIP(x, hipEventElapsedTime(&t, B->ev[0], B->ev[1]));
+  if (ms) *ms = t;
+  return KMI_OK;
+}
+
+int kmi_batch_status(const kmi_batch* B, int32_t* status, int64_t* steps, double* residual) {
+  if (!B) return fail(nullptr, KMI_ERR_ARG, "batch is NULL");
+  for (int c = 0; c < B->ncav; ++c) {
+    if (status) status[c] = B->status[c];
+    if (steps) steps[c] = B->steps[c];
+    if (residual) residual[c] = B->resid[c];
+  }
+  return KMI_OK;
+}
+
+int kmi_batch_set_launch(kmi_batch* B, int32_t tw, int32_t th, int32_t groups, int32_t nts) {
+  if (!B) return fail(nullptr, KMI_ERR_ARG, "batch is NULL");
+  if (!((tw == 32 || tw == 64 || tw == 128 || tw == 256) && tw * th == 256) || groups < 0)
+    return fail(B->ctx, KMI_ERR_ARG, "kmi_batch_set_launch: tiles of tw x th = 256 cells, tw in {32,64,128,256}");
+  KMI_HIP(B->ctx, hipSetDevice(B->ctx->device));
+  KMI_HIP(B->ctx, hipStreamSynchronize(B->ctx->stream));
+  B->tw = tw;
+  B->th = th;
+  B->groups = groups;
+  B->nts = nts ? 1 : 0;
+  B->tiles_dirty = true;
+  batch_drop_graphs(B);
+  return KMI_OK;
+}
+
+int kmi_batch_get_launch(const kmi_batch* B, int32_t* tw, int32_t* th, int32_t* groups, int32_t* nts) {
+  if (!B || !tw || !th || !groups || !nts) return fail(nullptr, KMI_ERR_ARG, "kmi_batch_get_launch: NULL argument");
+  *tw = B->tw;
+  *th = B->th;
+  *groups = B->groups;
+  *nts = B->nts;
+  return KMI_OK;
+}
+
+int kmi_batch_autotune(kmi_batch* B, int64_t steps_per_trial, double* best_ms) {
+  if (!B) return fail(nullptr, KMI_ERR_ARG, "batch is NULL");
+  kmi_ctx* x = B->ctx;
+  KMI_HIP(x, hipSetDevice(x->device));
+  KMI_HIP(x, hipStreamSynchronize(x->stream));
+  if (batch_active(B) == 0) return fail(x, KMI_ERR_STATE, "kmi_batch_autotune: no running cavity");
+  const int64_t trial = std::max<int64_t>(2, steps_per_trial);
+  const size_t bytes = size_t(B->total_pop) * B->esize;
+  void* save = nullptr;
+  KMI_HIP(x, hipMalloc(&save, bytes));
+  hipError_t e = hipMemcpyAsync(save, B->f[B->cur], bytes, hipMemcpyDeviceToDevice, x->stream);
+  const int cur0 = B->cur;
+  const int shapes[][2] = {{256, 1}, {128, 2}, {64, 4}, {32, 8}};
+  const int group_opts[] = {0, 8192, 4096, 2048, 1024};
+  double best = 1e300;
+  int bb[4] = {B->tw, B->th, B->groups, B->nts};
+  for (const auto& sh : shapes) {
+    for (int g : group_opts) {
+      for (int nt = 0; nt < 2 && e == hipSuccess; ++nt) {
+        B->tw = sh[0];
+        B->th = sh[1];
+        B->groups = g;
+        B->nts = nt;
+        if (batch_build_tiles(B) != KMI_OK) { e = hipErrorUnknown; break; }
+        if (g > 0 && g >= B->nslots) continue;  // same as one group per slot
+        if (batch_launch_step(B, x->stream) != KMI_OK) { e = hipErrorUnknown; break; }
+        double t = 1e300;
+        for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
+          e = hipEventRecord(B->ev[0], x->stream);
+          for (int64_t k = 0; k < trial && e == hipSuccess; ++k)
+            if (batch_launch_step(B, x->stream) != KMI_OK) e = hipErrorUnknown;
+          if (e == hipSuccess) e = hipEventRecord(B->ev[1], x->stream);
+          if (e == hipSuccess) e = hipEventSynchronize(B->ev[1]);
+          float ms = 0.f;
+          if (e == hipSuccess) e = hipEventElapsedTime(&ms, B->ev[0], B->ev[1]);
+          t = std::min(t, double(ms) / double(trial));
+        }
+        if (e == hipSuccess && t < best) {
+          best = t;
+          bb[0] = sh[0];
+          bb[1] = sh[1];
+          bb[2] = g;
+          bb[3] = nt;
+        }
+      }
+    }
+  }
+  B->tw = bb[0];
+  B->th = bb[1];
+  B->groups = bb[2];
+  B->nts = bb[3];
+  B->tiles_dirty = true;
+  batch_drop_graphs(B);
+  hipError_t e2 = hipMemcpyAsync(B->f[cur0], save, bytes, hipMemcpyDeviceToDevice, x->stream);
+  if (e2 == hipSuccess) e2 = hipStreamSynchronize(x->stream);
+  (void)hipFree(save);
+  B->cur = cur0;
+  if (e != hipSuccess) return hip_fail(x, e, "kmi_batch_autotune");
+  if (e2 != hipSuccess) return hip_fail(x, e2, "kmi_batch_autotune (restore)");
+  if (best_ms) *best_ms = best;
+  return batch_ensure_tiles(B);
+}
+
+int kmi_batch_active_cells(const kmi_batch* B, int64_t* cells) {
+  if (!B || !cells) return fail(nullptr, KMI_ERR_ARG, "kmi_batch_active_cells: NULL argument");
+  int64_t s = 0;
+  for (int c = 0; c < B->ncav; ++c)
+    if (B->status[c] == KMI_CAV_RUNNING) s += int64_t(B->n[c]) * B->n[c];
+  *cells = s;
+  return KMI_OK;
+}
+
+}  // extern "C"

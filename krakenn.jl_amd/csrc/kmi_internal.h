@@ -1,0 +1,51 @@
+// kmi_internal.h — host-side internals shared by the C-ABI translation units.
+#pragma once
+
+#include <string>
+
+#include "kmi_common.h"
+
+struct kmi_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  double* partials = nullptr;  // 2048 groups × 3 values
+  double* out = nullptr;       // 8 values
+};
+
+namespace kmi {
+
+// Record `msg` as the last error of `ctx` (or of the calling thread when ctx is NULL).
+int fail(kmi_ctx* ctx, int code, const std::string& msg);
+int hip_fail(kmi_ctx* ctx, hipError_t e, const char* what);
+
+hipError_t launch_step(int q, int st, int coll, bool th, bool bnd, bool nts, const StepArgs& a,
+                       dim3 g, dim3 b, hipStream_t s);
+hipError_t launch_copy_probe(const void* src, void* dst, long long bytes, hipStream_t s);
+hipError_t launch_to_storage(int Q, int st, const double* src, void* dst, long long ncells,
+                             long long plane, double shift, hipStream_t s);
+hipError_t launch_from_storage(int Q, int st, const void* src, double* dst, long long ncells,
+                               long long plane, double shift, hipStream_t s);
+hipError_t launch_init_eq(int Q, int st, void* dst, const double* rho, const double* ux,
+                          const double* uy, const double* uz, long long ncells, long long plane,
+                          hipStream_t s);
+hipError_t launch_init_temp(int st, void* dst, const double* T, long long ncells, long long plane,
+                            double T0, hipStream_t s);
+hipError_t launch_moments(int Q, int st, const void* f, const void* g, double* rho, double* ux,
+                          double* uy, double* uz, double* T, long long ncells, long long plane,
+                          double T0, hipStream_t s);
+int reduce_groups(long long ncells);
+hipError_t launch_reduce_cells(int Q, int st, const void* f, const void* g, const uint8_t* mask,
+                               double* uprev, long long ncells, long long plane, double T0,
+                               int kind, double* partials, double* out, hipStream_t s);
+hipError_t launch_nusselt(int st, const void* g, int nx, int ny, long long plane, double T0,
+                          int face, double Th, double dT, int lo, int hi, double* partials,
+                          double* out, hipStream_t s);
+
+}  // namespace kmi
+
+#define KMI_HIP(ctx, call)                                         \
+  do {                                                             \
+    hipError_t _e = (call);                                        \
+    if (_e != hipSuccess) return ::kmi::hip_fail(ctx, _e, #call);  \
+  } while (0)

@@ -18,312 +18,9 @@
 //   thermal DDF        thermal_2d.jl:16-60, 140-290, 421-480; fused_thermal_2d.jl:10-207
 // F32S / F16S store deviations from the rest state and collide them in FP32 (the
 // algebra is identical; only the rounding differs).
-#include "kmi_common.h"
+#include "kmi_cell.h"
 
 namespace kmi {
-
-template <class R>
-KMI_DEV R feq_val(R w, R rho, R cu, R usq) {
-  return w * rho * (R(1) + R(3) * cu + R(4.5) * cu * cu - R(1.5) * usq);
-}
-
-// 3cu + 4.5cu² − 1.5u² (rest: −1.5u²), for the deviation form feq − w = w(δρ + ρX).
-template <class R>
-KMI_DEV R feq_x(R cu, R usq) {
-  return R(3) * cu + R(4.5) * cu * cu - R(1.5) * usq;
-}
-
-template <int Q, class R>
-KMI_DEV R usq_of(R ux, R uy, R uz) {
-  if constexpr (Q == 9) return ux * ux + uy * uy;
-  else return ux * ux + uy * uy + uz * uz;
-}
-
-// ---- equilibria ----------------------------------------------------------------------
-// Plain: feq_q = w ρ (1 + 3cu + 4.5cu² − 1.5u²).  Shifted: deq_q = w (δρ + ρ X_q).
-template <int Q, bool SH, class R>
-KMI_DEV void equilibria(R drho_or_rho, R rho, R ux, R uy, R uz, R (&fe)[Q]) {
-  const R usq = usq_of<Q>(ux, uy, uz);
-  sfor<Q>([&](auto qc) {
-    constexpr int q = decltype(qc)::value;
-    constexpr double w = Lat<Q>::w[q];
-    if constexpr (!SH) {
-      if constexpr (q == 0) fe[q] = R(w) * rho * (R(1) - R(1.5) * usq);
-      else
-        fe[q] = feq_val(R(w), rho,
-                        cdotu<Lat<Q>::cx[q], Lat<Q>::cy[q], Lat<Q>::cz[q]>(ux, uy, uz), usq);
-    } else {
-      R x;
-      if constexpr (q == 0) x = -(R(1.5) * usq);
-      else x = feq_x(cdotu<Lat<Q>::cx[q], Lat<Q>::cy[q], Lat<Q>::cz[q]>(ux, uy, uz), usq);
-      fe[q] = R(w) * (drho_or_rho + rho * x);
-    }
-  });
-}
-
-// Moments.  Plain: ρ = Σf, u = (Σcf)/ρ (macroscopic.jl).  Shifted: δρ = Σd, ρ = 1+δρ.
-template <int Q, bool SH, class R>
-KMI_DEV void moments(const R (&f)[Q], R& m0, R& rho, R& ux, R& uy, R& uz) {
-  m0 = zeroth_moment<Q>(f);
-  rho = SH ? R(1) + m0 : m0;
-  const R inv = R(1) / rho;
-  ux = first_moment<Q, 0>(f) * inv;
-  uy = first_moment<Q, 1>(f) * inv;
-  if constexpr (Q == 19) uz = first_moment<Q, 2>(f) * inv;
-  else uz = R(0);
-}
-
-template <int Q, bool SH, class R>
-KMI_DEV void collide_bgk(const R (&fp)[Q], R (&fo)[Q], R om) {
-  R m0, rho, ux, uy, uz, fe[Q];
-  moments<Q, SH>(fp, m0, rho, ux, uy, uz);
-  equilibria<Q, SH>(m0, rho, ux, uy, uz, fe);
-  sfor<Q>([&](auto qc) {
-    constexpr int q = decltype(qc)::value;
-    fo[q] = fp[q] - om * (fp[q] - fe[q]);
-  });
-}
-
-// f* = f − a(f − feq) − b(f̄ − feq̄), a = (s⁺+s⁻)/2, b = (s⁺−s⁻)/2 (fused_trt_2d.jl:150-166).
-template <int Q, bool SH, class R>
-KMI_DEV void collide_trt(const R (&fp)[Q], R (&fo)[Q], R sp, R sm) {
-  R m0, rho, ux, uy, uz, fe[Q];
-  moments<Q, SH>(fp, m0, rho, ux, uy, uz);
-  equilibria<Q, SH>(m0, rho, ux, uy, uz, fe);
-  const R a = (sp + sm) * R(0.5);
-  const R b = (sp - sm) * R(0.5);
-  fo[0] = fp[0] - sp * (fp[0] - fe[0]);
-  sfor<Q - 1>([&](auto qc) {
-    constexpr int q = decltype(qc)::value + 1;
-    constexpr int o = Lat<Q>::opp[q];
-    fo[q] = fp[q] - a * (fp[q] - fe[q]) - b * (fp[o] - fe[o]);
-  });
-}
-
-// D2Q9 MRT (Lallemand–Luo), collide_mrt_2d.jl:56-107.  Shifted form: the moments of
-// the deviation, with m_eq shifted by the rest-state moments (e = −2, ε = 1).
-template <bool SH, class R>
-KMI_DEV void collide_mrt(const R (&fp)[9], R (&fo)[9], R s_e, R s_eps, R s_q, R s_nu) {
-  const R f1 = fp[0], f2 = fp[1], f3 = fp[2], f4 = fp[3], f5 = fp[4], f6 = fp[5], f7 = fp[6],
-          f8 = fp[7], f9 = fp[8];
-  const R m0 = f1 + f2 + f3 + f4 + f5 + f6 + f7 + f8 + f9;
-  const R e = -R(4) * f1 - f2 - f3 - f4 - f5 + R(2) * (f6 + f7 + f8 + f9);
-  const R eps = R(4) * f1 - R(2) * (f2 + f3 + f4 + f5) + f6 + f7 + f8 + f9;
-  const R jx = f2 - f4 + f6 - f7 - f8 + f9;
-  const R qx = -R(2) * f2 + R(2) * f4 + f6 - f7 - f8 + f9;
-  const R jy = f3 - f5 + f6 + f7 - f8 - f9;
-  const R qy = -R(2) * f3 + R(2) * f5 + f6 + f7 - f8 - f9;
-  const R pxx = f2 - f3 + f4 - f5;
-  const R pxy = f6 - f7 + f8 - f9;
-  const R rho = SH ? R(1) + m0 : m0;
-  const R inv = R(1) / rho;
-  const R ux = jx * inv;
-  const R uy = jy * inv;
-  const R usq = ux * ux + uy * uy;
-  R e_eq, eps_eq;
-  if constexpr (!SH) {
-    e_eq = -R(2) * rho + R(3) * rho * usq;
-    eps_eq = rho - R(3) * rho * usq;
-  } else {
-    e_eq = -R(2) * m0 + R(3) * rho * usq;  // e_eq(ρ) + 2
-    eps_eq = m0 - R(3) * rho * usq;        // ε_eq(ρ) − 1
-  }
-  const R qx_eq = -rho * ux;
-  const R qy_eq = -rho * uy;
-  const R pxx_eq = rho * (ux * ux - uy * uy);
-  const R pxy_eq = rho * ux * uy;
-  const R es = e - s_e * (e - e_eq);
-  const R ep = eps - s_eps * (eps - eps_eq);
-  const R qxs = qx - s_q * (qx - qx_eq);
-  const R qys = qy - s_q * (qy - qy_eq);
-  const R ps = pxx - s_nu * (pxx - pxx_eq);
-  const R pxys = pxy - s_nu * (pxy - pxy_eq);
-  const R r = m0, jxs = jx, jys = jy;
-  fo[0] = R(1.0 / 9) * r - R(1.0 / 9) * es + R(1.0 / 9) * ep;
-  fo[1] = R(1.0 / 9) * r - R(1.0 / 36) * es - R(1.0 / 18) * ep + R(1.0 / 6) * jxs -
-          R(1.0 / 6) * qxs + R(1.0 / 4) * ps;
-  fo[2] = R(1.0 / 9) * r - R(1.0 / 36) * es - R(1.0 / 18) * ep + R(1.0 / 6) * jys -
-          R(1.0 / 6) * qys - R(1.0 / 4) * ps;
-  fo[3] = R(1.0 / 9) * r - R(1.0 / 36) * es - R(1.0 / 18) * ep - R(1.0 / 6) * jxs +
-          R(1.0 / 6) * qxs + R(1.0 / 4) * ps;
-  fo[4] = R(1.0 / 9) * r - R(1.0 / 36) * es - R(1.0 / 18) * ep - R(1.0 / 6) * jys +
-          R(1.0 / 6) * qys - R(1.0 / 4) * ps;
-  fo[5] = R(1.0 / 9) * r + R(1.0 / 18) * es + R(1.0 / 36) * ep + R(1.0 / 6) * jxs +
-          R(1.0 / 12) * qxs + R(1.0 / 6) * jys + R(1.0 / 12) * qys + R(1.0 / 4) * pxys;
-  fo[6] = R(1.0 / 9) * r + R(1.0 / 18) * es + R(1.0 / 36) * ep - R(1.0 / 6) * jxs -
-          R(1.0 / 12) * qxs + R(1.0 / 6) * jys + R(1.0 / 12) * qys - R(1.0 / 4) * pxys;
-  fo[7] = R(1.0 / 9) * r + R(1.0 / 18) * es + R(1.0 / 36) * ep - R(1.0 / 6) * jxs -
-          R(1.0 / 12) * qxs - R(1.0 / 6) * jys - R(1.0 / 12) * qys + R(1.0 / 4) * pxys;
-  fo[8] = R(1.0 / 9) * r + R(1.0 / 18) * es + R(1.0 / 36) * ep + R(1.0 / 6) * jxs +
-          R(1.0 / 12) * qxs - R(1.0 / 6) * jys - R(1.0 / 12) * qys - R(1.0 / 4) * pxys;
-}
-
-// Thermal BGK, geq = w T (1 + 3 c·u) (thermal_2d.jl:16-60).  Shifted: g − w T0 stored,
-// geq − w T0 = w (δT + T·3c·u).
-template <bool SH, class R>
-KMI_DEV void collide_thermal(const R (&gp)[9], R (&go)[9], R ux, R uy, R omT, R T0) {
-  const R s = zeroth_moment<9>(gp);
-  if constexpr (!SH) {
-    const R Temp = s;
-    const R t3 = R(3);
-    R geq;
-    geq = R(4.0 / 9.0) * Temp;
-    go[0] = gp[0] - omT * (gp[0] - geq);
-    geq = R(1.0 / 9.0) * Temp * (R(1) + t3 * ux);
-    go[1] = gp[1] - omT * (gp[1] - geq);
-    geq = R(1.0 / 9.0) * Temp * (R(1) + t3 * uy);
-    go[2] = gp[2] - omT * (gp[2] - geq);
-    geq = R(1.0 / 9.0) * Temp * (R(1) - t3 * ux);
-    go[3] = gp[3] - omT * (gp[3] - geq);
-    geq = R(1.0 / 9.0) * Temp * (R(1) - t3 * uy);
-    go[4] = gp[4] - omT * (gp[4] - geq);
-    geq = R(1.0 / 36.0) * Temp * (R(1) + t3 * (ux + uy));
-    go[5] = gp[5] - omT * (gp[5] - geq);
-    geq = R(1.0 / 36.0) * Temp * (R(1) + t3 * (-ux + uy));
-    go[6] = gp[6] - omT * (gp[6] - geq);
-    geq = R(1.0 / 36.0) * Temp * (R(1) + t3 * (-ux - uy));
-    go[7] = gp[7] - omT * (gp[7] - geq);
-    geq = R(1.0 / 36.0) * Temp * (R(1) + t3 * (ux - uy));
-    go[8] = gp[8] - omT * (gp[8] - geq);
-  } else {
-    const R dT = s;
-    const R Tt = T0 + dT;
-    sfor<9>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      constexpr double w = Lat<9>::w[q];
-      R geq;
-      if constexpr (q == 0) geq = R(w) * dT;
-      else geq = R(w) * (dT + Tt * (R(3) * cdotu<Lat<9>::cx[q], Lat<9>::cy[q], 0>(ux, uy, R(0))));
-      go[q] = gp[q] - omT * (gp[q] - geq);
-    });
-  }
-}
-
-// BGK + Guo source for a vertical force fy (thermal_2d.jl:443-478, fused_thermal_2d.jl:116-157).
-template <bool SH, class R>
-KMI_DEV void collide_guo(const R (&fp)[9], R (&fo)[9], R m0, R rho, R ux, R uy, R fy, R om) {
-  const R fx = R(0);
-  const R usq = ux * ux + uy * uy;
-  const R guo = R(1) - om / R(2);
-  const R t3 = R(3);
-  R fe[9];
-  equilibria<9, SH>(m0, rho, ux, uy, R(0), fe);
-  R Sq;
-  Sq = R(4.0 / 9.0) * ((-ux) * fx + (-uy) * fy) * t3;
-  fo[0] = fp[0] - om * (fp[0] - fe[0]) + guo * Sq;
-  Sq = R(1.0 / 9.0) * ((R(1) - ux) * fx + (-uy) * fy) * t3 + R(1.0 / 9.0) * ux * fx * R(9);
-  fo[1] = fp[1] - om * (fp[1] - fe[1]) + guo * Sq;
-  Sq = R(1.0 / 9.0) * ((-ux) * fx + (R(1) - uy) * fy) * t3 + R(1.0 / 9.0) * uy * fy * R(9);
-  fo[2] = fp[2] - om * (fp[2] - fe[2]) + guo * Sq;
-  Sq = R(1.0 / 9.0) * ((-R(1) - ux) * fx + (-uy) * fy) * t3 + R(1.0 / 9.0) * ux * fx * R(9);
-  fo[3] = fp[3] - om * (fp[3] - fe[3]) + guo * Sq;
-  Sq = R(1.0 / 9.0) * ((-ux) * fx + (-R(1) - uy) * fy) * t3 + R(1.0 / 9.0) * uy * fy * R(9);
-  fo[4] = fp[4] - om * (fp[4] - fe[4]) + guo * Sq;
-  Sq = R(1.0 / 36.0) * ((R(1) - ux) * fx + (R(1) - uy) * fy) * t3 +
-       R(1.0 / 36.0) * (ux + uy) * (fx + fy) * R(9);
-  fo[5] = fp[5] - om * (fp[5] - fe[5]) + guo * Sq;
-  Sq = R(1.0 / 36.0) * ((-R(1) - ux) * fx + (R(1) - uy) * fy) * t3 +
-       R(1.0 / 36.0) * (-ux + uy) * (-fx + fy) * R(9);
-  fo[6] = fp[6] - om * (fp[6] - fe[6]) + guo * Sq;
-  Sq = R(1.0 / 36.0) * ((-R(1) - ux) * fx + (-R(1) - uy) * fy) * t3 +
-       R(1.0 / 36.0) * (-ux - uy) * (-fx - fy) * R(9);
-  fo[7] = fp[7] - om * (fp[7] - fe[7]) + guo * Sq;
-  Sq = R(1.0 / 36.0) * ((R(1) - ux) * fx + (-R(1) - uy) * fy) * t3 +
-       R(1.0 / 36.0) * (ux - uy) * (fx - fy) * R(9);
-  fo[8] = fp[8] - om * (fp[8] - fe[8]) + guo * Sq;
-}
-
-// ---- Zou–He closures on the node's post-stream populations (D2Q9) -----------------------
-template <bool SH, class R>
-KMI_DEV void zh_north(R (&f)[9], R u) {  // boundary_2d.jl:13-44
-  const R f1 = f[0], f2 = f[1], f3 = f[2], f4 = f[3], f6 = f[5], f7 = f[6];
-  R rw;
-  if constexpr (!SH) rw = (f1 + f2 + f4 + R(2) * (f3 + f6 + f7)) / (R(1) + u * R(0));
-  else rw = R(1) + (f1 + f2 + f4 + R(2) * (f3 + f6 + f7));
-  f[4] = f3;
-  f[8] = f7 - R(0.5) * (f2 - f4) + R(0.5) * rw * u;
-  f[7] = f6 + R(0.5) * (f2 - f4) - R(0.5) * rw * u;
-}
-template <bool SH, class R>
-KMI_DEV void zh_south(R (&f)[9], R u) {  // boundary_2d.jl:135-150
-  const R f1 = f[0], f2 = f[1], f4 = f[3], f5 = f[4], f8 = f[7], f9 = f[8];
-  R rw = f1 + f2 + f4 + R(2) * (f5 + f8 + f9);
-  if constexpr (SH) rw = R(1) + rw;
-  f[2] = f5;
-  f[5] = f8 - R(0.5) * (f2 - f4) + R(0.5) * rw * u;
-  f[6] = f9 + R(0.5) * (f2 - f4) - R(0.5) * rw * u;
-}
-template <bool SH, class R>
-KMI_DEV void zh_west(R (&f)[9], R u) {  // boundary_2d.jl:171-185
-  const R f1 = f[0], f3 = f[2], f4 = f[3], f5 = f[4], f7 = f[6], f8 = f[7];
-  R s = f1 + f3 + f5 + R(2) * (f4 + f7 + f8);
-  if constexpr (SH) s = R(1) + s;
-  const R rw = s / (R(1) - u);
-  f[1] = f4 + R(2.0 / 3.0) * rw * u;
-  f[5] = f8 - R(0.5) * (f3 - f5) + R(1.0 / 6.0) * rw * u;
-  f[8] = f7 + R(0.5) * (f3 - f5) + R(1.0 / 6.0) * rw * u;
-}
-// rho_m1 = ρ_out − 1, used by the shifted form only.
-template <bool SH, class R>
-KMI_DEV void zh_pressure_east(R (&f)[9], R rho, R rho_m1) {  // boundary_2d.jl:206-223
-  const R f1 = f[0], f2 = f[1], f3 = f[2], f5 = f[4], f6 = f[5], f9 = f[8];
-  R ux;
-  if constexpr (!SH) ux = -R(1) + (f1 + f3 + f5 + R(2) * (f2 + f6 + f9)) / rho;
-  else ux = ((f1 + f3 + f5 + R(2) * (f2 + f6 + f9)) - rho_m1) / rho;
-  f[3] = f2 - R(2.0 / 3.0) * rho * ux;
-  f[6] = f9 - R(0.5) * (f3 - f5) - R(1.0 / 6.0) * rho * ux;
-  f[7] = f6 + R(0.5) * (f3 - f5) - R(1.0 / 6.0) * rho * ux;
-}
-template <bool SH, class R>
-KMI_DEV void zh_pressure_west(R (&f)[9], R rho, R rho_m1) {  // boundary_2d.jl:264-281
-  const R f1 = f[0], f3 = f[2], f4 = f[3], f5 = f[4], f7 = f[6], f8 = f[7];
-  R ux;
-  if constexpr (!SH) ux = R(1) - (f1 + f3 + f5 + R(2) * (f4 + f7 + f8)) / rho;
-  else ux = (rho_m1 - (f1 + f3 + f5 + R(2) * (f4 + f7 + f8))) / rho;
-  f[1] = f4 + R(2.0 / 3.0) * rho * ux;
-  f[5] = f8 - R(0.5) * (f3 - f5) + R(1.0 / 6.0) * rho * ux;
-  f[8] = f7 + R(0.5) * (f3 - f5) + R(1.0 / 6.0) * rho * ux;
-}
-
-// Anti-bounce-back of one slot (thermal_2d.jl:175-178); `tl` is T_link (minus T0 when shifted).
-template <class R>
-KMI_DEV R abb(R gq, R w2, R tface, bool on_corner, R tcorner, bool skip) {
-  const R tl = on_corner ? tcorner : tface;
-  return (on_corner && skip) ? gq : -gq + w2 * tl;
-}
-
-// Dirichlet faces W, E, S, N (thermal_2d.jl:182-290 slot tables, 371-417 corner rule).
-template <bool SH, class R>
-KMI_DEV void thermal_faces(R (&g)[9], const StepArgs& a, int i, int j) {
-  const R wa = R(2) * R(1.0 / 9.0);
-  const R we = R(2) * R(1.0 / 36.0);
-  const double t0 = SH ? a.T_ref : 0.0;
-  auto tv = [&](double t) { return SH ? R(t - t0) : R(t); };
-  if (i == 0 && a.face[KMI_WEST].dirichlet) {
-    const FaceArgs& F = a.face[KMI_WEST];
-    g[1] = -g[1] + wa * tv(F.T);
-    g[5] = abb(g[5], we, tv(F.T), j == 0, tv(F.Tc[0]), F.skip[0] != 0);
-    g[8] = abb(g[8], we, tv(F.T), j == a.ny - 1, tv(F.Tc[1]), F.skip[1] != 0);
-  }
-  if (i == a.nx - 1 && a.face[KMI_EAST].dirichlet) {
-    const FaceArgs& F = a.face[KMI_EAST];
-    g[3] = -g[3] + wa * tv(F.T);
-    g[6] = abb(g[6], we, tv(F.T), j == 0, tv(F.Tc[0]), F.skip[0] != 0);
-    g[7] = abb(g[7], we, tv(F.T), j == a.ny - 1, tv(F.Tc[1]), F.skip[1] != 0);
-  }
-  if (j == 0 && a.face[KMI_SOUTH].dirichlet) {
-    const FaceArgs& F = a.face[KMI_SOUTH];
-    g[2] = -g[2] + wa * tv(F.T);
-    g[5] = abb(g[5], we, tv(F.T), i == 0, tv(F.Tc[0]), F.skip[0] != 0);
-    g[6] = abb(g[6], we, tv(F.T), i == a.nx - 1, tv(F.Tc[1]), F.skip[1] != 0);
-  }
-  if (j == a.ny - 1 && a.face[KMI_NORTH].dirichlet) {
-    const FaceArgs& F = a.face[KMI_NORTH];
-    g[4] = -g[4] + wa * tv(F.T);
-    g[8] = abb(g[8], we, tv(F.T), i == 0, tv(F.Tc[0]), F.skip[0] != 0);
-    g[7] = abb(g[7], we, tv(F.T), i == a.nx - 1, tv(F.Tc[1]), F.skip[1] != 0);
-  }
-}
 
 // ---- the step kernel ----------------------------------------------------------------------
 // BND = false: every face periodic, no mask, no closures (the Taylor–Green fast path).

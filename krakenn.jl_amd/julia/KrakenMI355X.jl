@@ -1,0 +1,132 @@
+# KrakenMI355X.jl — Julia side of the MI355X backend for Kraken.jl's stream–collide path.
+#
+# A thin `ccall` layer over libkraken_mi355x.so (include/kraken_mi355x.h).  It defines the
+# backend value `MI355XBackend()` that Kraken's drivers receive as `backend=`, and the
+# methods of the hot-path drivers for it.  No KernelAbstractions, no AMDGPU.jl: every
+# time step runs in the library's HIP kernels.  Julia is not available in this build's
+# container, so this file is exercised only through the identical C ABI from Python.
+module KrakenMI355X
+
+using Kraken
+import Kraken: run_cavity_2d, LBMConfig, D2Q9, D3Q19
+
+const libkmi = get(ENV, "KRAKEN_MI355X_LIB",
+                   joinpath(@__DIR__, "..", "kraken_mi355x", "libkraken_mi355x.so"))
+
+# ---- C structs (layout of kmi_face / kmi_desc) --------------------------------------
+struct KmiFace
+    kind::Int32
+    thermal::Int32
+    u::NTuple{3,Float64}
+    rho::Float64
+    T::Float64
+end
+
+struct KmiDesc
+    stencil::Int32; storage::Int32; collision::Int32; thermal::Int32
+    nx::Int64; ny::Int64; nz::Int64
+    omega::Float64; s_plus::Float64; s_minus::Float64
+    s_e::Float64; s_eps::Float64; s_q::Float64; s_nu::Float64
+    omega_T::Float64; beta_g::Float64; T_ref::Float64
+    coupling::Int32; n_bc_order::Int32; bc_order::NTuple{6,Int32}
+    faces::NTuple{6,KmiFace}
+end
+
+const FACE_PERIODIC, FACE_WALL, FACE_MOVING_WALL, FACE_ZOUHE_VELOCITY, FACE_ZOUHE_PRESSURE = 0, 1, 2, 3, 4
+const STORAGE = Dict(Float64 => Int32(0), Float32 => Int32(1))
+
+lasterr(ctx) = unsafe_string(ccall((:kmi_last_error, libkmi), Cstring, (Ptr{Cvoid},), ctx))
+
+function check(rc, ctx=C_NULL)
+    rc == 0 && return nothing
+    msg = lasterr(ctx)
+    # kmi_status 1 (bad argument) and 5 (unsupported) are the caller's fault
+    rc in (1, 5) ? throw(ArgumentError(msg)) : error(msg)
+end
+
+# ---- the backend value ----------------------------------------------------------------
+"""
+    MI355XBackend(device=0)
+
+Backend value for `run_simulation(...; backend=MI355XBackend())` and the drivers.  Raises
+`ArgumentError` when no MI355X is visible — never a silent CPU run
+(src/simulation_runner.jl:123-149).
+"""
+struct MI355XBackend
+    ctx::Ptr{Cvoid}
+end
+
+function MI355XBackend(device::Integer=0)
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    rc = ccall((:kmi_ctx_create, libkmi), Cint, (Cint, Ref{Ptr{Cvoid}}), device, r)
+    rc == 0 || throw(ArgumentError("Backend mi355x is unavailable: " * lasterr(C_NULL)))
+    return MI355XBackend(r[])
+end
+
+mutable struct Lattice
+    h::Ptr{Cvoid}
+    backend::MI355XBackend
+    dims::NTuple{3,Int}
+    Q::Int
+end
+
+function Lattice(b::MI355XBackend, d::KmiDesc)
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:kmi_lattice_create, libkmi), Cint, (Ptr{Cvoid}, Ref{KmiDesc}, Ref{Ptr{Cvoid}}),
+                b.ctx, d, r), b.ctx)
+    lat = Lattice(r[], b, (Int(d.nx), Int(d.ny), Int(d.nz)), d.stencil == 0 ? 9 : 19)
+    finalizer(l -> ccall((:kmi_lattice_destroy, libkmi), Cint, (Ptr{Cvoid},), l.h), lat)
+    return lat
+end
+
+step!(l::Lattice, n::Integer) =
+    check(ccall((:kmi_step, libkmi), Cint, (Ptr{Cvoid}, Int64), l.h, n), l.backend.ctx)
+
+upload!(l::Lattice, f::Array{Float64}) =
+    check(ccall((:kmi_upload, libkmi), Cint, (Ptr{Cvoid}, Int32, Ptr{Float64}, Int32),
+                l.h, 0, f, 0), l.backend.ctx)
+
+function download(l::Lattice)
+    f = Array{Float64}(undef, l.dims[1], l.dims[2], (l.Q == 19 ? (l.dims[3],) : ())..., l.Q)
+    check(ccall((:kmi_download, libkmi), Cint, (Ptr{Cvoid}, Int32, Ptr{Float64}, Int32),
+                l.h, 0, f, 0), l.backend.ctx)
+    return f
+end
+
+function moments(l::Lattice)
+    n = l.Q == 9 ? (l.dims[1], l.dims[2]) : l.dims
+    ρ, ux, uy = zeros(n), zeros(n), zeros(n)
+    uz = l.Q == 19 ? zeros(n) : nothing
+    check(ccall((:kmi_moments, libkmi), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                l.h, ρ, ux, uy, uz === nothing ? C_NULL : uz, C_NULL), l.backend.ctx)
+    return (ρ=ρ, ux=ux, uy=uy, uz=uz)
+end
+
+function desc_2d(Nx, Ny, ν, T; faces)
+    w = KmiFace(FACE_WALL, 0, (0.0, 0.0, 0.0), 1.0, 0.0)
+    fs = [w, w, w, w, w, w]
+    for (k, v) in faces
+        fs[k + 1] = v
+    end
+    sp = 1 / (3ν + 0.5); sm = 1 / ((3 / 16) / (3ν) + 0.5)
+    KmiDesc(0, STORAGE[T], 0, 0, Nx, Ny, 1, 1 / (3ν + 0.5), sp, sm, 1.4, 1.4, 1.2,
+            1 / (3ν + 0.5), 1 / (3ν + 0.5), 0.0, 0.0, 0, 0, (0, 0, 0, 0, 0, 0), Tuple(fs))
+end
+
+# ---- driver methods for the backend (src/drivers/basic.jl:112-144) ------------------------
+"""
+    run_cavity_2d(config::LBMConfig{D2Q9}; backend::MI355XBackend, T=Float64)
+
+Same result as the reference driver: Zou–He lid, halfway bounce-back walls, BGK, rest start.
+"""
+function run_cavity_2d(config::LBMConfig{D2Q9}; backend::MI355XBackend, T=Float64)
+    lid = KmiFace(FACE_ZOUHE_VELOCITY, 0, (config.u_lid, 0.0, 0.0), 1.0, 0.0)
+    d = desc_2d(config.Nx, config.Ny, config.ν, T; faces=Dict(3 => lid))
+    lat = Lattice(backend, d)
+    step!(lat, config.max_steps)
+    m = moments(lat)
+    return (ρ=m.ρ, ux=m.ux, uy=m.uy, config=config)
+end
+
+end # module

@@ -6,6 +6,7 @@ driver API; it never computes populations itself.
 """
 from . import _lib
 from ._lib import KrakenError, load
+from .batch import Batch, STATUS_NAMES, ensemble_parameters
 from .drivers import (
     D2Q9,
     D3Q19,
@@ -30,7 +31,7 @@ from .drivers import (
 from .lattice import ArgumentError, Lattice, MI355XBackend, make_desc
 
 __all__ = [
-    "ArgumentError", "D2Q9", "D3Q19", "KrakenError", "LBMConfig", "Lattice", "MI355XBackend",
+    "Batch", "STATUS_NAMES", "ensemble_parameters", "ArgumentError", "D2Q9", "D3Q19", "KrakenError", "LBMConfig", "Lattice", "MI355XBackend",
     "cavity_desc", "lbm_config", "load", "make_desc", "natural_convection_desc",
     "natural_convection_initial_temperature", "omega", "rayleigh_benard_desc",
     "rb_initial_temperature", "reynolds", "run_cavity_2d", "run_natural_convection_2d",

@@ -66,6 +66,22 @@ class KmiDesc(ctypes.Structure):
     ]
 
 
+class KmiBatchDesc(ctypes.Structure):
+    _fields_ = [
+        ("storage", c_int32),
+        ("collision", c_int32),
+        ("lid", c_int32),
+        ("pad", c_int32),
+        ("u_lid", c_double),
+        ("s_e", c_double),
+        ("s_eps", c_double),
+        ("s_q", c_double),
+        ("Lambda", c_double),
+    ]
+
+
+CAV_RUNNING, CAV_CONVERGED, CAV_DIVERGED, CAV_MAX_STEPS = 0, 1, 2, 3
+
 # name -> (restype, argtypes); every symbol declared in include/kraken_mi355x.h
 SIGNATURES = {
     "kmi_abi_version": (c_int, []),
@@ -96,6 +112,21 @@ SIGNATURES = {
     "kmi_lattice_get_launch": (c_int, [c_void_p, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)]),
     "kmi_lattice_autotune": (c_int, [c_void_p, c_int64, POINTER(c_double)]),
     "kmi_measure_copy_bandwidth": (c_int, [c_void_p, c_int64, c_int32, POINTER(c_double)]),
+    "kmi_batch_create": (c_int, [c_void_p, POINTER(KmiBatchDesc), c_int32, POINTER(c_int32),
+                                 POINTER(c_double), POINTER(c_void_p)]),
+    "kmi_batch_destroy": (c_int, [c_void_p]),
+    "kmi_batch_upload": (c_int, [c_void_p, c_int32, c_void_p, c_int32]),
+    "kmi_batch_download": (c_int, [c_void_p, c_int32, c_void_p, c_int32]),
+    "kmi_batch_init_equilibrium": (c_int, [c_void_p, c_int32] + [POINTER(c_double)] * 3),
+    "kmi_batch_moments": (c_int, [c_void_p, c_int32] + [POINTER(c_double)] * 3),
+    "kmi_batch_reset_status": (c_int, [c_void_p]),
+    "kmi_batch_step": (c_int, [c_void_p, c_int64]),
+    "kmi_batch_run": (c_int, [c_void_p, c_double, c_int64, c_int64, POINTER(c_double)]),
+    "kmi_batch_status": (c_int, [c_void_p, POINTER(c_int32), POINTER(c_int64), POINTER(c_double)]),
+    "kmi_batch_set_launch": (c_int, [c_void_p, c_int32, c_int32, c_int32, c_int32]),
+    "kmi_batch_get_launch": (c_int, [c_void_p] + [POINTER(c_int32)] * 4),
+    "kmi_batch_autotune": (c_int, [c_void_p, c_int64, POINTER(c_double)]),
+    "kmi_batch_active_cells": (c_int, [c_void_p, POINTER(c_int64)]),
 }
 
 _lib = None

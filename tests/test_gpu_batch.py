@@ -1,0 +1,109 @@
+"""Batched cavities: each cavity of the ensemble is bit-identical to the oracle's cavity chain
+(src/drivers/basic.jl:121-136 / test/test_mrt.jl:57-63) run alone."""
+import numpy as np
+import pytest
+
+from test_oracle_lattice import rng_pops
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [24, 40, 17, 64, 33, 50, 8]
+NUS = [0.05, 0.02, 0.1, 0.01, 0.03, 0.08, 0.2]
+
+
+@pytest.mark.parametrize("collision", ["mrt", "bgk", "trt"])
+@pytest.mark.parametrize("storage,dtype", [("f64", np.float64), ("f32", np.float32)])
+@pytest.mark.parametrize("lid", ["zouhe", "ladd"])
+def test_batch_cavities_bitwise(kmi, po, backend, collision, storage, dtype, lid):
+    steps = 37
+    print(f"[batch] {collision} {storage} {lid}", flush=True)
+    f0 = [rng_pops((n, n), 9, seed=10 + c, amp=0.02).astype(dtype, order="F") for c, n in enumerate(SIZES)]
+    with kmi.Batch(SIZES, NUS, storage=storage, collision=collision, lid=lid, backend=backend) as b:
+        for c in range(len(SIZES)):
+            b.upload(c, f0[c])
+        b.step(steps)
+        got = [b.download(c, dtype=dtype) for c in range(len(SIZES))]
+        st = b.status()
+    assert np.all(st["steps"] == steps)
+    for c, (n, nu) in enumerate(zip(SIZES, NUS)):
+        d = kmi.cavity_desc(n, nu, 0.1, storage=storage, collision=collision, lid=lid)
+        ref = f0[c].copy(order="F")
+        po.run(d, ref, None, steps)
+        assert np.array_equal(got[c], ref), f"cavity {c} (n={n}) max |Δ| {np.max(np.abs(got[c] - ref)):.3e}"
+
+
+def test_batch_tile_shapes_and_groups_are_invisible(kmi, backend):
+    f0 = [rng_pops((n, n), 9, seed=30 + c) for c, n in enumerate(SIZES)]
+    outs = []
+    for launch in [None, (256, 1, 0, 0), (32, 8, 16, 1), (128, 2, 8, 0), (64, 4, 1024, 1)]:
+        with kmi.Batch(SIZES, NUS, storage="f32s", backend=backend) as b:
+            if launch:
+                b.set_launch(*launch)
+            for c in range(len(SIZES)):
+                b.upload(c, f0[c])
+            b.step(25)
+            outs.append([b.download(c) for c in range(len(SIZES))])
+    for o in outs[1:]:
+        for c in range(len(SIZES)):
+            assert np.array_equal(o[c], outs[0][c])
+
+
+def test_batch_run_to_residual_matches_single_lattices(kmi, backend):
+    sizes = [16, 24, 20, 32]
+    nus = [0.1, 0.08, 0.05, 0.12]
+    tol, check = 1e-6, 50
+    with kmi.Batch(sizes, nus, storage="f64", collision="mrt", backend=backend) as b:
+        b.run(tol, check, 200_000)
+        st = b.status()
+        got = [b.download(c) for c in range(len(sizes))]
+    assert np.all(st["status"] == kmi._lib.CAV_CONVERGED), st
+    assert np.all(st["steps"] % check == 0) and np.all(st["residual"] < tol)
+    for c, (n, nu) in enumerate(zip(sizes, nus)):
+        with kmi.Lattice(kmi.cavity_desc(n, nu, 0.1, collision="mrt"), backend) as lat:
+            steps, r, ok = kmi.run_to_residual(lat, tol, check, 200_000)
+            assert ok and steps == st["steps"][c]
+            assert abs(r - st["residual"][c]) <= 1e-9 * r
+            assert np.array_equal(lat.download(), got[c])
+
+
+def test_batch_diverged_cavity_is_retired_alone(kmi, backend):
+    sizes = [16, 24, 20]
+    nus = [0.1, 0.08, 0.05]
+    clean = []
+    with kmi.Batch(sizes, nus, storage="f32s", backend=backend) as b:
+        b.run(1e-5, 20, 50_000)
+        clean = [b.download(c) for c in (0, 2)]
+        st0 = b.status()
+    with kmi.Batch(sizes, nus, storage="f32s", backend=backend) as b:
+        f = b.download(1)
+        f[3, 4, 2] = np.nan  # poison one cavity from the host
+        b.upload(1, f)
+        b.run(1e-5, 20, 50_000)
+        st = b.status()
+        assert st["status"][1] == kmi._lib.CAV_DIVERGED
+        assert st["status"][0] == st["status"][2] == kmi._lib.CAV_CONVERGED
+        assert np.array_equal(st["steps"][[0, 2]], st0["steps"][[0, 2]])
+        assert np.array_equal(b.download(0), clean[0]) and np.array_equal(b.download(2), clean[1])
+
+
+def test_batch_autotune_restores_state(kmi, backend):
+    f0 = [rng_pops((n, n), 9, seed=50 + c) for c, n in enumerate(SIZES)]
+    with kmi.Batch(SIZES, NUS, storage="f32", backend=backend) as b:
+        for c in range(len(SIZES)):
+            b.upload(c, f0[c].astype(np.float32, order="F"))
+        b.step(5)
+        before = [b.download(c) for c in range(len(SIZES))]
+        ms = b.autotune(4)
+        assert ms > 0
+        assert all(np.array_equal(b.download(c), before[c]) for c in range(len(SIZES)))
+        assert np.all(b.status()["steps"] == 5)
+
+
+def test_batch_rejects_bad_input(kmi, backend):
+    with pytest.raises(kmi.KrakenError):
+        kmi.Batch([16, 1], [0.1, 0.1], backend=backend)
+    with kmi.Batch([16], [0.1], backend=backend) as b:
+        with pytest.raises(kmi.KrakenError):
+            b.set_launch(48, 4)
+        with pytest.raises(kmi.ArgumentError):
+            b.upload(0, np.zeros((8, 8, 9)))
