@@ -264,11 +264,18 @@ int kmi_batch_init_equilibrium(kmi_batch* b, int32_t cavity, const double* rho, 
 int kmi_batch_moments(kmi_batch* b, int32_t cavity, double* rho, double* ux, double* uy);
 /* Mark every cavity running again with zero steps (populations untouched). */
 int kmi_batch_reset_status(kmi_batch* b);
+/* Put cavities retired at max_steps back to running (step counts kept), so a long run
+ * can proceed in chunks of kmi_batch_run calls with growing max_steps. */
+int kmi_batch_resume(kmi_batch* b);
 /* Advance all running cavities `nsteps` steps (no residual checks). */
 int kmi_batch_step(kmi_batch* b, int64_t nsteps);
-/* Step until every cavity is converged (r_u < tol), diverged or at max_steps, checking
- * every `check_every` (>= 2) steps; elapsed device milliseconds in *ms (may be NULL). */
-int kmi_batch_run(kmi_batch* b, double tol, int64_t check_every, int64_t max_steps, double* ms);
+/* Step until every cavity is converged (r < tol), diverged or at max_steps, checking every
+ * `check_every` (>= 2) steps; elapsed device milliseconds in *ms (may be NULL).
+ * r = ||u^k - u^(k-m)||_2 / (m ||u^k||_2) over the last `window` = m steps (1 <= m <=
+ * check_every): m = 1 is the reference's one-step r_u (gnf_periodic_steady.jl:192); a
+ * longer window averages FP32 rounding noise, whose one-step floor is ~4e-7. */
+int kmi_batch_run(kmi_batch* b, double tol, int64_t check_every, int64_t window, int64_t max_steps,
+                  double* ms);
 /* Arrays of ncav entries (any may be NULL). */
 int kmi_batch_status(const kmi_batch* b, int32_t* status, int64_t* steps, double* residual);
 /* Tiles of tw × th cells (tw*th = 256, tw in {32,64,128,256}), `groups` persistent

@@ -154,10 +154,10 @@ __global__ void __launch_bounds__(256) k_batch_resid(const BatchArgs a, const vo
   }
 }
 
-// One thread per cavity: Neumaier sum of its slots' partials (fixed order) → r_u.
+// One thread per cavity: Neumaier sum of its slots' partials (fixed order) → r.
 __global__ void k_batch_final(const double* __restrict__ partials, const int* __restrict__ cav_slots,
                               const int* __restrict__ cav_range, const int* __restrict__ active,
-                              int ncav, double* __restrict__ out) {
+                              int ncav, double window, double* __restrict__ out) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncav) return;
   if (!active[c]) return;
@@ -174,7 +174,7 @@ __global__ void k_batch_final(const double* __restrict__ partials, const int* __
   const double d2 = s[0] + comp[0], u2 = s[1] + comp[1];
   out[3 * c] = d2;
   out[3 * c + 1] = u2;
-  out[3 * c + 2] = sqrt(d2 / u2);  // gnf_periodic_steady.jl:192
+  out[3 * c + 2] = sqrt(d2 / u2) / window;  // window = 1: gnf_periodic_steady.jl:192
 }
 
 #define KMI_BATCH_ST(st, CALL)                                               \
@@ -244,7 +244,7 @@ struct kmi_batch {
   std::vector<int64_t> steps;
   std::vector<double> resid;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
-  int64_t graph_check = 0;
+  int64_t graph_check = 0, graph_window = 0;
   hipEvent_t ev[2] = {nullptr, nullptr};
 };
 
@@ -359,19 +359,21 @@ static int batch_active(const kmi_batch* B) {
   return k;
 }
 
-// One check block: (check − 1) steps, snapshot u, 1 step, residual partials, per-cavity sums.
-static int batch_record_block(kmi_batch* B, int64_t check, hipStream_t s) {
-  for (int64_t k = 0; k + 1 < check; ++k)
+// One check block: (check − m) steps, snapshot u, m steps, residual partials, per-cavity sums.
+static int batch_record_block(kmi_batch* B, int64_t check, int64_t window, hipStream_t s) {
+  for (int64_t k = 0; k + window < check; ++k)
     if (int rc = batch_launch_step(B, s)) return rc;
   BatchArgs a = batch_args(B);
   hipError_t e = launch_batch_resid(B->desc.storage, a, B->f[B->cur], 0, batch_groups(B), s);
   if (e != hipSuccess) return hip_fail(B->ctx, e, "batch snapshot");
-  if (int rc = batch_launch_step(B, s)) return rc;
+  for (int64_t k = 0; k < window; ++k)
+    if (int rc = batch_launch_step(B, s)) return rc;
   a = batch_args(B);
   e = launch_batch_resid(B->desc.storage, a, B->f[B->cur], 1, batch_groups(B), s);
   if (e != hipSuccess) return hip_fail(B->ctx, e, "batch residual");
   hipLaunchKernelGGL(k_batch_final, dim3(unsigned((B->ncav + 63) / 64)), dim3(64), 0, s,
-                     B->partials, B->cav_slots_d, B->cav_range_d, B->active_d, B->ncav, B->resid_d);
+                     B->partials, B->cav_slots_d, B->cav_range_d, B->active_d, B->ncav,
+                     double(window), B->resid_d);
   e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(B->ctx, e, "batch residual sum");
   return KMI_OK;
@@ -578,6 +580,16 @@ int kmi_batch_reset_status(kmi_batch* B) {
   return KMI_OK;
 }
 
+int kmi_batch_resume(kmi_batch* B) {
+  if (!B) return fail(nullptr, KMI_ERR_ARG, "batch is NULL");
+  for (auto& s : B->status)
+    if (s == KMI_CAV_MAX_STEPS) {
+      s = KMI_CAV_RUNNING;
+      B->tiles_dirty = true;
+    }
+  return KMI_OK;
+}
+
 int kmi_batch_step(kmi_batch* B, int64_t nsteps) {
   if (!B) return fail(nullptr, KMI_ERR_ARG, "batch is NULL");
   if (nsteps < 0) return fail(B->ctx, KMI_ERR_ARG, "kmi_batch_step: nsteps must be >= 0");
@@ -593,15 +605,19 @@ int kmi_batch_step(kmi_batch* B, int64_t nsteps) {
   return KMI_OK;
 }
 
-int kmi_batch_run(kmi_batch* B, double tol, int64_t check, int64_t max_steps, double* ms) {
+int kmi_batch_run(kmi_batch* B, double tol, int64_t check, int64_t window, int64_t max_steps,
+                  double* ms) {
   if (!B) return fail(nullptr, KMI_ERR_ARG, "batch is NULL");
   kmi_ctx* x = B->ctx;
-  if (check < 2 || max_steps < 1) return fail(x, KMI_ERR_ARG, "kmi_batch_run: check_every >= 2 and max_steps >= 1");
+  if (check < 2 || max_steps < 1 || window < 1 || window > check)
+    return fail(x, KMI_ERR_ARG,
+                "kmi_batch_run: need check_every >= 2, 1 <= window <= check_every, max_steps >= 1");
   KMI_HIP(x, hipSetDevice(x->device));
   hipStream_t s = x->stream;
-  if (B->graph_check != check) {
+  if (B->graph_check != check || B->graph_window != window) {
     batch_drop_graphs(B);
     B->graph_check = check;
+    B->graph_window = window;
   }
   KMI_HIP(x, hipEventRecord(B->ev[0], s));
   std::vector<double> r(size_t(3) * B->ncav);
@@ -611,7 +627,7 @@ int kmi_batch_run(kmi_batch* B, double tol, int64_t check, int64_t max_steps, do
     if (!B->graph[p]) {
       hipGraph_t gph = nullptr;
       KMI_HIP(x, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      int rc = batch_record_block(B, check, s);
+      int rc = batch_record_block(B, check, window, s);
       hipError_t e = hipStreamEndCapture(s, &gph);
       B->cur = p;  // capture recorded the launches only
       if (rc != KMI_OK) {

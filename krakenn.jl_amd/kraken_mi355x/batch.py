@@ -93,14 +93,21 @@ class Batch:
     def step(self, n=1):
         self._chk(L.load().kmi_batch_step(self._h, int(n)))
 
-    def run(self, tol=1e-7, check_every=100, max_steps=10_000_000):
-        """Run every cavity to r_u < tol (or divergence / max_steps); device ms elapsed."""
+    def run(self, tol=1e-7, check_every=100, max_steps=10_000_000, window=1):
+        """Run every cavity to r < tol (or divergence / max_steps); device ms elapsed.
+        r = ‖u^k − u^(k−m)‖/(m‖u^k‖) over the last m = `window` steps (m = 1: the
+        reference's one-step r_u)."""
         ms = c_double()
-        self._chk(L.load().kmi_batch_run(self._h, float(tol), int(check_every), int(max_steps), byref(ms)))
+        self._chk(L.load().kmi_batch_run(self._h, float(tol), int(check_every), int(window),
+                                         int(max_steps), byref(ms)))
         return ms.value
 
     def reset_status(self):
         self._chk(L.load().kmi_batch_reset_status(self._h))
+
+    def resume(self):
+        """Cavities stopped at max_steps run again (step counts kept)."""
+        self._chk(L.load().kmi_batch_resume(self._h))
 
     def status(self):
         k = len(self.n)

@@ -66,6 +66,26 @@ def test_batch_run_to_residual_matches_single_lattices(kmi, backend):
             assert np.array_equal(lat.download(), got[c])
 
 
+def test_batch_windowed_residual_definition(kmi, po, backend):
+    """r = ‖u^k − u^(k−m)‖₂ / (m ‖u^k‖₂) with m = window = check_every."""
+    sizes, nus, m = [16, 24], [0.1, 0.05], 10
+    f0 = [rng_pops((n, n), 9, seed=70 + c) for c, n in enumerate(sizes)]
+    with kmi.Batch(sizes, nus, storage="f64", backend=backend) as b:
+        for c in range(2):
+            b.upload(c, f0[c])
+        b.run(1.0, m, 10 * m, window=m)  # tol 1: retires at the first check
+        st = b.status()
+    assert np.all(st["steps"] == m)
+    for c, (n, nu) in enumerate(zip(sizes, nus)):
+        u0 = po.moments(f0[c])
+        f = f0[c].copy(order="F")
+        po.run(kmi.cavity_desc(n, nu, 0.1, collision="mrt"), f, None, m)
+        u1 = po.moments(f)
+        d2 = ((u1["ux"] - u0["ux"]) ** 2 + (u1["uy"] - u0["uy"]) ** 2).sum()
+        u2 = (u1["ux"] ** 2 + u1["uy"] ** 2).sum()
+        assert abs(st["residual"][c] - np.sqrt(d2 / u2) / m) <= 1e-10 * st["residual"][c]
+
+
 def test_batch_diverged_cavity_is_retired_alone(kmi, backend):
     sizes = [16, 24, 20]
     nus = [0.1, 0.08, 0.05]
