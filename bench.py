@@ -88,9 +88,61 @@ def bench_cavities(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_warmstart(args):
+    """Learned warm start: 2048 steady 128² cavities (Re log-uniform [100, 3200], lid
+    amplitude A uniform [0.06, 0.12]) from the batched solver train the conv decoder on the
+    GPU; 256 held-out cases then run to r < 1e-7 from rest and from the prediction."""
+    import numpy as np
+
+    import kraken_mi355x as kmi
+    from kraken_mi355x import warmstart as ws
+
+    log = (lambda s: print(s, file=sys.stderr, flush=True))
+    N = 128
+    Re_tr, A_tr = ws.draw_cases(args.ws_train, seed=1)
+    data = ws.generate_steady_states(Re_tr, A_tr, N=N, tol=1e-6, log=log)
+    net, loss = ws.train(data["x"], data["y"], epochs=args.ws_epochs, batch=32, lr=2e-3, log=log)
+    t_train = None
+    # BF16 FLOPs of one training sample: forward products ×3 (forward, data and weight gradients)
+    d = net.desc
+    L = d.latent_hw * d.latent_hw * d.latent_c
+    fwd = 2 * (d.in_dim * d.hidden + d.hidden * L)
+    for hi, ci, co in ((d.latent_hw, d.latent_c, d.c1), (2 * d.latent_hw, d.c1, d.c2), (4 * d.latent_hw, d.c2, d.out_c)):
+        fwd += 2 * (2 * hi) ** 2 * 9 * ci * co
+    Re_te, A_te = ws.draw_cases(args.ws_test, seed=99)
+    rest = ws.time_to_residual(Re_te, A_te, N=N, tol=1e-7)
+    warm = ws.time_to_residual(Re_te, A_te, N=N, net=net, tol=1e-7)
+    ok = (rest["status"] == 1) & (warm["status"] == 1)
+    idx = np.where(ok)[0]
+    du = [float(np.max(np.abs(warm["fields"][c]["ux"] - rest["fields"][c]["ux"])) / A_te[c]) for c in idx]
+    pred = net.predict(ws.encode(Re_te[idx], A_te[idx]))
+    perr = [float(np.linalg.norm(pred[k][..., :2] - ws.to_target(rest["fields"][c], A_te[c])[..., :2])
+                  / np.linalg.norm(ws.to_target(rest["fields"][c], A_te[c])[..., :2])) for k, c in enumerate(idx)]
+    out = {
+        "metric": "time to residual 1e-7, 256 held-out 128^2 MRT cavities, warm start vs rest init",
+        "value": round(rest["wall_s"] / warm["wall_s"], 3), "unit": "x (rest wall / warm wall)",
+        "higher_is_better": True, "n_gpus": 1, "dtype": "f32 solver / bf16 network",
+        "data": "synthetic (seeded Re, A draws; training targets from the batched solver)",
+        "rest_wall_s": round(rest["wall_s"], 2), "warm_wall_s": round(warm["wall_s"], 2),
+        "warm_inference_and_init_s": round(warm["init_s"], 3),
+        "median_step_ratio": round(float(np.median(rest["steps"][idx] / warm["steps"][idx])), 3),
+        "converged_both": int(ok.sum()), "rest_status": np.bincount(rest["status"], minlength=4).tolist(),
+        "warm_status": np.bincount(warm["status"], minlength=4).tolist(),
+        "max_dU_over_A_between_converged_fields": max(du) if du else None,
+        "median_dU_over_A": float(np.median(du)) if du else None,
+        "prediction_rel_L2_median": float(np.median(perr)) if perr else None,
+        "train": {"samples": int(len(data["index"])), "generated": args.ws_train, "data_gen_s": round(data["seconds"], 1),
+                  "epochs": args.ws_epochs, "final_loss": loss, "fwd_flop_per_sample": fwd},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="tgv", choices=["tgv", "cavities"])
+    ap.add_argument("--workload", default="tgv", choices=["tgv", "cavities", "warmstart"])
+    ap.add_argument("--ws-train", type=int, default=2048)
+    ap.add_argument("--ws-test", type=int, default=256)
+    ap.add_argument("--ws-epochs", type=int, default=150)
     ap.add_argument("--cavity-max-steps", type=int, default=600_000)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
@@ -107,6 +159,9 @@ def main():
         sys.exit(2)
     if args.workload == "cavities":
         bench_cavities(args)
+        return
+    if args.workload == "warmstart":
+        bench_warmstart(args)
         return
 
     import kraken_mi355x as kmi

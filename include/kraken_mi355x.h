@@ -289,6 +289,45 @@ int kmi_batch_autotune(kmi_batch* b, int64_t steps_per_trial, double* best_ms_pe
 /* Cells of the running cavities (the work of one step). */
 int kmi_batch_active_cells(const kmi_batch* b, int64_t* cells);
 
+/* ---- learned warm start (no reference counterpart; it produces initial fields, the
+ *      role of f_init in src/drivers/gnf_periodic_steady.jl:114-150) ------------------
+ * A conv decoder: x (in_dim) → dense+SiLU (hidden) → dense+SiLU (latent hw×hw×c0, NHWC)
+ * → 3 × [nearest ×2 up-sampling, 3×3 conv] with SiLU after the first two → an output
+ * image of (8 hw)² × out_c (NHWC).  Trained and evaluated on the GPU by implicit-GEMM
+ * kernels on BF16 MFMA tiles with FP32 accumulation; FP32 master weights, Adam.
+ * Flat parameter order: W1[in_dim][hidden], b1, W2[hidden][hw*hw*c0], b2,
+ * C1[9*c0][c1] (rows (ky, kx, c)), bc1, C2[9*c1][c2], bc2, C3[9*c2][out_c], bc3. */
+typedef struct kmi_nn kmi_nn;
+
+typedef struct {
+  int32_t in_dim, hidden, latent_hw, latent_c, c1, c2, out_c;
+  int32_t max_batch;
+  uint64_t seed; /* He-normal weight initialisation */
+} kmi_nn_desc;
+
+void kmi_nn_desc_init(kmi_nn_desc* d); /* 2, 256, 16, 64, 64, 32, 3; batch 32 */
+int kmi_nn_create(kmi_ctx* ctx, const kmi_nn_desc* d, kmi_nn** out);
+int kmi_nn_destroy(kmi_nn* nn);
+int kmi_nn_param_count(const kmi_nn* nn, int64_t* n);
+int kmi_nn_get_params(kmi_nn* nn, float* host);
+int kmi_nn_set_params(kmi_nn* nn, const float* host); /* also resets the Adam state */
+/* y: batch × (8hw)² × out_c floats (NHWC); x: batch × in_dim. */
+int kmi_nn_forward(kmi_nn* nn, int32_t batch, const float* x, float* y);
+/* MSE loss mean((y − target)²) and, when grad != NULL, d loss / d params (flat order). */
+int kmi_nn_loss_grad(kmi_nn* nn, int32_t batch, const float* x, const float* target, double* loss,
+                     float* grad);
+int kmi_nn_train_step(kmi_nn* nn, int32_t batch, const float* x, const float* target, float lr,
+                      double* loss);
+/* Device-resident training set and epochs of shuffled mini-batches (Fisher–Yates per epoch
+ * from `seed`); mean loss of the last epoch in *mean_loss. */
+int kmi_nn_set_dataset(kmi_nn* nn, int32_t n, const float* x, const float* y);
+int kmi_nn_train_epochs(kmi_nn* nn, int32_t epochs, int32_t batch, float lr, uint64_t seed,
+                        double* mean_loss);
+/* C = A·B with BF16 operands on MFMA and FP32 accumulation (host FP32 in/out); A(m, k) =
+ * A[m*sam + k*sak], B(k, n) = B[k*sbk + n*sbn]; the K range split over `splits` groups. */
+int kmi_gemm_bf16(kmi_ctx* ctx, int32_t M, int32_t N, int32_t K, const float* A, int64_t sam,
+                  int64_t sak, const float* B, int64_t sbk, int64_t sbn, int32_t splits, float* C);
+
 #ifdef __cplusplus
 }
 #endif

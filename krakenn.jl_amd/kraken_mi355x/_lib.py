@@ -7,7 +7,8 @@ from __future__ import annotations
 
 import ctypes
 import os
-from ctypes import POINTER, c_char_p, c_double, c_int, c_int32, c_int64, c_size_t, c_uint8, c_void_p
+from ctypes import (POINTER, c_char_p, c_double, c_float, c_int, c_int32, c_int64, c_size_t, c_uint8,
+                    c_uint64, c_void_p)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libkraken_mi355x.so")
@@ -128,6 +129,21 @@ SIGNATURES = {
     "kmi_batch_get_launch": (c_int, [c_void_p] + [POINTER(c_int32)] * 4),
     "kmi_batch_autotune": (c_int, [c_void_p, c_int64, POINTER(c_double)]),
     "kmi_batch_active_cells": (c_int, [c_void_p, POINTER(c_int64)]),
+    "kmi_nn_desc_init": (None, [c_void_p]),
+    "kmi_nn_create": (c_int, [c_void_p, c_void_p, POINTER(c_void_p)]),
+    "kmi_nn_destroy": (c_int, [c_void_p]),
+    "kmi_nn_param_count": (c_int, [c_void_p, POINTER(c_int64)]),
+    "kmi_nn_get_params": (c_int, [c_void_p, POINTER(c_float)]),
+    "kmi_nn_set_params": (c_int, [c_void_p, POINTER(c_float)]),
+    "kmi_nn_forward": (c_int, [c_void_p, c_int32, POINTER(c_float), POINTER(c_float)]),
+    "kmi_nn_loss_grad": (c_int, [c_void_p, c_int32, POINTER(c_float), POINTER(c_float), POINTER(c_double),
+                                 POINTER(c_float)]),
+    "kmi_nn_train_step": (c_int, [c_void_p, c_int32, POINTER(c_float), POINTER(c_float), c_float,
+                                  POINTER(c_double)]),
+    "kmi_nn_set_dataset": (c_int, [c_void_p, c_int32, POINTER(c_float), POINTER(c_float)]),
+    "kmi_nn_train_epochs": (c_int, [c_void_p, c_int32, c_int32, c_float, c_uint64, POINTER(c_double)]),
+    "kmi_gemm_bf16": (c_int, [c_void_p, c_int32, c_int32, c_int32, POINTER(c_float), c_int64, c_int64,
+                              POINTER(c_float), c_int64, c_int64, c_int32, POINTER(c_float)]),
 }
 
 _lib = None

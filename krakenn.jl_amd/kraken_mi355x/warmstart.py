@@ -1,0 +1,115 @@
+"""Learned warm start for lid-driven cavities.
+
+Pipeline (all heavy work on the GPU through the C ABI):
+  1. steady states of N×N cavities for sampled (Re, lid amplitude A) from the batched
+     solver (kmi_batch, F32S MRT, Zou–He lid of run_cavity_2d);
+  2. a conv decoder (kmi_nn, BF16 MFMA) trained to map (Re, A) to (u/A, v/A, (ρ−1)/A²);
+  3. held-out cavities started from the predicted fields (feq of the predicted ρ, u — the
+     `f_init` injection point of src/drivers/gnf_periodic_steady.jl:114-150) against
+     rest-state starts, both run by the same batched solver to the same residual.
+Rest initialisation stays the fallback: a cavity without a prediction starts at rest.
+"""
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+
+from .batch import Batch
+from .lattice import MI355XBackend
+from .nn import WarmStartNet
+
+RE_RANGE = (100.0, 3200.0)
+A_RANGE = (0.06, 0.12)
+
+
+def draw_cases(n, seed, Re_range=RE_RANGE, A_range=A_RANGE):
+    """Re log-uniform, lid amplitude A uniform."""
+    rng = np.random.default_rng(seed)
+    Re = np.exp(rng.uniform(math.log(Re_range[0]), math.log(Re_range[1]), size=n))
+    A = rng.uniform(A_range[0], A_range[1], size=n)
+    return Re, A
+
+
+def encode(Re, A, Re_range=RE_RANGE, A_range=A_RANGE):
+    """Network input in [−1, 1]²: scaled log Re and scaled amplitude."""
+    lr = (np.log(Re) - math.log(Re_range[0])) / (math.log(Re_range[1]) - math.log(Re_range[0]))
+    la = (np.asarray(A) - A_range[0]) / (A_range[1] - A_range[0])
+    return np.stack([2 * lr - 1, 2 * la - 1], axis=1).astype(np.float32)
+
+
+def to_target(m, A):
+    """Moments (arrays indexed [i, j]) → image [y, x, (u/A, v/A, (ρ−1)/A²)]."""
+    return np.stack([m["ux"].T / A, m["uy"].T / A, (m["rho"].T - 1.0) / (A * A)], axis=-1).astype(np.float32)
+
+
+def from_prediction(y, A):
+    """Image [y, x, 3] → (ρ, ux, uy) arrays indexed [i, j]."""
+    y = np.asarray(y, dtype=np.float64)
+    return 1.0 + A * A * y[..., 2].T, A * y[..., 0].T, A * y[..., 1].T
+
+
+def generate_steady_states(Re, A, N=128, tol=1e-6, check=100, window=100, max_steps=800_000,
+                           storage="f32s", collision="mrt", backend=None, log=None):
+    """Run every (Re, A) cavity from rest to r < tol; return targets of the converged ones."""
+    nu = A * N / Re
+    t0 = time.perf_counter()
+    with Batch(np.full(len(Re), N), nu, storage=storage, collision=collision, backend=backend) as b:
+        b.autotune(4)
+        cap = 0
+        while True:
+            cap += 100_000
+            b.run(tol, check, min(cap, max_steps), window=window)
+            st = b.status()
+            if log:
+                log(f"[data] cap {cap}: converged {(st['status'] == 1).sum()} diverged {(st['status'] == 2).sum()} "
+                    f"t {time.perf_counter() - t0:.1f}s")
+            if cap >= max_steps or not np.any(st["status"] == 3):
+                break
+            b.resume()
+        ok = np.where(st["status"] == 1)[0]
+        y = np.stack([to_target(b.moments(int(c)), A[c]) for c in ok]) if len(ok) else np.zeros((0, N, N, 3), np.float32)
+    return {"index": ok, "x": encode(Re[ok], A[ok]), "y": y, "status": st["status"], "steps": st["steps"],
+            "seconds": time.perf_counter() - t0}
+
+
+def train(x, y, *, epochs=300, batch=32, lr=2e-3, lr_final=1e-4, seed=0, latent_hw=None, backend=None, log=None):
+    """Train the decoder on (x, y); learning rate decays geometrically over `epochs` in 10 stages."""
+    hw = y.shape[1] // 8 if latent_hw is None else latent_hw
+    net = WarmStartNet(latent_hw=hw, max_batch=batch, seed=seed + 1, backend=backend)
+    net.set_dataset(x, y)
+    stages = 10
+    t0 = time.perf_counter()
+    loss = float("nan")
+    for k in range(stages):
+        lr_k = lr * (lr_final / lr) ** (k / (stages - 1))
+        loss = net.train_epochs(max(1, epochs // stages), min(batch, len(x)), lr_k, seed=seed + 17 * k)
+        if log:
+            log(f"[train] stage {k} lr {lr_k:.2e} loss {loss:.3e} t {time.perf_counter() - t0:.1f}s")
+    return net, loss
+
+
+def time_to_residual(Re, A, *, N=128, net=None, tol=1e-7, check=100, window=100, max_steps=1_000_000,
+                     storage="f32s", collision="mrt", backend=None):
+    """All cavities in one batch, from rest (net=None) or from the network's prediction.
+    Returns wall seconds (including inference and initialisation), per-cavity steps/status
+    and the final velocity fields."""
+    backend = backend or MI355XBackend()
+    nu = A * N / Re
+    with Batch(np.full(len(Re), N), nu, storage=storage, collision=collision, backend=backend) as b:
+        b.autotune(4)
+        backend.synchronize()
+        t0 = time.perf_counter()
+        if net is not None:
+            pred = net.predict(encode(Re, A))
+            for c in range(len(Re)):
+                rho, ux, uy = from_prediction(pred[c], A[c])
+                b.init_equilibrium(c, rho, ux, uy)
+        t_init = time.perf_counter() - t0
+        ms = b.run(tol, check, max_steps, window=window)
+        wall = time.perf_counter() - t0
+        st = b.status()
+        fields = [b.moments(c) for c in range(len(Re))]
+    return {"wall_s": wall, "init_s": t_init, "device_ms": ms, "steps": st["steps"], "status": st["status"],
+            "residual": st["residual"], "fields": fields}

@@ -1,0 +1,33 @@
+"""The learned warm start gains over rest initialisation (reduced size: 64² cavities).
+
+The full-size experiment (128² cavities, 2048 training states, 256 held-out cases) is
+`bench.py --workload warmstart`; this test runs the same pipeline small enough for the
+suite and asserts that the trained network cuts the steps to the residual."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_warm_start_beats_rest_init(kmi, backend):
+    from kraken_mi355x import warmstart as ws
+
+    N = 64
+    Re_tr, A_tr = ws.draw_cases(96, seed=1, Re_range=(100, 800))
+    data = ws.generate_steady_states(Re_tr, A_tr, N=N, tol=1e-6, max_steps=400_000, backend=backend,
+                                     log=lambda s: print(s, flush=True))
+    assert len(data["index"]) >= 80, data["status"]
+    net, loss = ws.train(data["x"], data["y"], epochs=200, batch=16, lr=3e-3, backend=backend,
+                         log=lambda s: print(s, flush=True))
+    assert loss < 2e-2, loss
+    Re_te, A_te = ws.draw_cases(16, seed=99, Re_range=(100, 800))
+    rest = ws.time_to_residual(Re_te, A_te, N=N, tol=1e-7, backend=backend)
+    warm = ws.time_to_residual(Re_te, A_te, N=N, net=net, tol=1e-7, backend=backend)
+    ok = (rest["status"] == 1) & (warm["status"] == 1)
+    assert ok.sum() >= 14
+    ratio = np.median(rest["steps"][ok] / warm["steps"][ok])
+    du = max(float(np.max(np.abs(warm["fields"][c]["ux"] - rest["fields"][c]["ux"])) / A_te[c]) for c in np.where(ok)[0])
+    print(f"[warm] median step ratio {ratio:.2f}, wall rest {rest['wall_s']:.2f}s warm {warm['wall_s']:.2f}s, "
+          f"max |Δu|/A between converged fields {du:.2e}", flush=True)
+    assert ratio > 1.5
+    assert du < 1e-2
