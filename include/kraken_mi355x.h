@@ -70,9 +70,12 @@ typedef enum { KMI_BGK = 0, KMI_TRT = 1, KMI_MRT = 2 } kmi_collision;
  *   MOVING_WALL     — halfway bounce-back with the Ladd momentum term
  *                     f_q = f*_opp(q) + 6 w_q rho_w (c_q . u_w)   (stream_periodic_3d.jl:106-166).
  *   ZOUHE_VELOCITY  — bounce-back pull, then the Zou–He velocity closure of the
- *                     reference on that face (boundary_2d.jl:13-44, 135-185).
+ *                     reference on that face (D2Q9: boundary_2d.jl:13-44, 135-185, north,
+ *                     south, west, x velocity; D3Q19: boundary_3d.jl:152-232, any face,
+ *                     any velocity).
  *   ZOUHE_PRESSURE  — bounce-back pull, then the Zou–He pressure closure
- *                     (boundary_2d.jl:206-299). */
+ *                     (D2Q9: boundary_2d.jl:206-299, west and east; D3Q19:
+ *                     boundary_3d.jl:234-290, any face). */
 typedef enum {
   KMI_FACE_PERIODIC = 0,
   KMI_FACE_WALL = 1,

@@ -173,9 +173,18 @@ static int validate(const kmi_desc& d, std::string& why, int& code) {
            why = "the thermal DDF uses BGK + Guo for the flow (thermal_2d.jl:421)", 1;
   for (int f = 0; f < nf; ++f) {
     const kmi_face& F = d.faces[f];
-    if (F.kind == KMI_FACE_ZOUHE_VELOCITY || F.kind == KMI_FACE_ZOUHE_PRESSURE) {
-      if (d.stencil != KMI_D2Q9)
-        return code = KMI_ERR_UNSUPPORTED, why = "Zou-He faces are implemented for D2Q9", 1;
+    if (F.kind == KMI_FACE_ZOUHE_PRESSURE && !(F.rho > 0.0))
+      return why = "Zou-He pressure density must be positive", 1;
+    if (F.kind == KMI_FACE_ZOUHE_VELOCITY && d.stencil == KMI_D3Q19) {
+      // ρ_w = (Σ_par + 2 Σ_out) / (1 + sign u_n) (boundary_3d.jl:186-187)
+      const double un = F.u[f / 2], sign = (f % 2) ? 1.0 : -1.0;
+      if (!(1.0 + sign * un > 0.0))
+        return why = std::string("Zou-He normal velocity on ") + kFaceName[f] +
+                     " makes 1 + sign*u_n non-positive",
+               1;
+    }
+    if ((F.kind == KMI_FACE_ZOUHE_VELOCITY || F.kind == KMI_FACE_ZOUHE_PRESSURE) &&
+        d.stencil == KMI_D2Q9) {
       if (F.kind == KMI_FACE_ZOUHE_VELOCITY) {
         if (f == KMI_EAST)
           return code = KMI_ERR_UNSUPPORTED,
@@ -193,7 +202,6 @@ static int validate(const kmi_desc& d, std::string& why, int& code) {
                  why = std::string("Pressure boundary condition on face ") + kFaceName[f] +
                        " is not supported (supported: west, east; handlers.jl:256-269)",
                  1;
-        if (!(F.rho > 0.0)) return why = "Zou-He pressure density must be positive", 1;
       }
     }
     if (d.thermal && F.thermal == KMI_THERMAL_DIRICHLET && F.kind == KMI_FACE_PERIODIC)

@@ -270,6 +270,82 @@ KMI_DEV void zh_pressure_west(R (&f)[9], R rho, R rho_m1) {  // boundary_2d.jl:2
   f[8] = f7 + R(0.5) * (f3 - f5) + R(1.0 / 6.0) * rho * ux;
 }
 
+// ---- D3Q19 Zou–He closures (boundary_3d.jl:39-300, ZouHeFace3D restated 0-based) ------------
+struct ZH3Face {
+  int axis, sign;      // normal axis (0 x, 1 y, 2 z); −1 min face, +1 max face
+  int qa, qao;         // unknown axis population and its known opposite
+  int qd[4], qdo[4];   // unknown diagonals (tangent-1 pair, tangent-2 pair) and opposites
+  int t1p, t1m, t2p, t2m;  // axis populations along the two tangents
+  int par[9];          // populations with zero normal component
+  int out[5];          // known populations leaving through the face
+};
+// Face order W, E, S, N, B, T; tangents: x-faces (y, z), y-faces (x, z), z-faces (x, y).
+constexpr ZH3Face kZH3[6] = {
+    {0, -1, 1, 2, {7, 9, 11, 13}, {10, 8, 14, 12}, 3, 4, 5, 6, {0, 3, 4, 5, 6, 15, 16, 17, 18}, {2, 8, 10, 12, 14}},
+    {0, +1, 2, 1, {8, 10, 12, 14}, {9, 7, 13, 11}, 3, 4, 5, 6, {0, 3, 4, 5, 6, 15, 16, 17, 18}, {1, 7, 9, 11, 13}},
+    {1, -1, 3, 4, {7, 8, 15, 17}, {10, 9, 18, 16}, 1, 2, 5, 6, {0, 1, 2, 5, 6, 11, 12, 13, 14}, {4, 9, 10, 16, 18}},
+    {1, +1, 4, 3, {9, 10, 16, 18}, {8, 7, 17, 15}, 1, 2, 5, 6, {0, 1, 2, 5, 6, 11, 12, 13, 14}, {3, 7, 8, 15, 17}},
+    {2, -1, 5, 6, {11, 12, 15, 16}, {14, 13, 18, 17}, 1, 2, 3, 4, {0, 1, 2, 3, 4, 7, 8, 9, 10}, {6, 13, 14, 17, 18}},
+    {2, +1, 6, 5, {13, 14, 17, 18}, {12, 11, 16, 15}, 1, 2, 3, 4, {0, 1, 2, 3, 4, 7, 8, 9, 10}, {5, 11, 12, 15, 16}}};
+
+// Σ parallel + 2 Σ outgoing, summed in table order from zero (the reference's loops).
+template <int F, class R>
+KMI_DEV R zh3_sum(const R (&f)[19]) {
+  constexpr ZH3Face Z = kZH3[F];
+  R sp = R(0), so = R(0);
+  sfor<9>([&](auto n) { sp = sp + f[Z.par[decltype(n)::value]]; });
+  sfor<5>([&](auto n) { so = so + f[Z.out[decltype(n)::value]]; });
+  return sp + R(2) * so;
+}
+
+// zou_he_velocity_3d_kernel! (boundary_3d.jl:152-232): un, ut1, ut2 are the velocity
+// components along the normal and the two tangents.
+template <bool SH, int F, class R>
+KMI_DEV void zh3_velocity(R (&f)[19], R un, R ut1, R ut2) {
+  constexpr ZH3Face Z = kZH3[F];
+  const R sg = R(Z.sign);
+  const R denom = R(1) + sg * un;
+  R s = zh3_sum<F>(f);
+  if constexpr (SH) s = R(1) + s;
+  const R rw = s / denom;
+  f[Z.qa] = f[Z.qao] - sg * R(1.0 / 3.0) * rw * un;
+  const R d1 = f[Z.par[5]], d2 = f[Z.par[6]], d3 = f[Z.par[7]], d4 = f[Z.par[8]];
+  const R t1 = (f[Z.t1p] - f[Z.t1m]) + (d1 - d2 + d3 - d4);
+  const R t2 = (f[Z.t2p] - f[Z.t2m]) + (d1 + d2 - d3 - d4);
+  f[Z.qd[0]] = f[Z.qdo[0]] - R(0.5) * t1 + R(0.5) * rw * ut1 - sg * R(1.0 / 6.0) * rw * un;
+  f[Z.qd[1]] = f[Z.qdo[1]] + R(0.5) * t1 - R(0.5) * rw * ut1 - sg * R(1.0 / 6.0) * rw * un;
+  f[Z.qd[2]] = f[Z.qdo[2]] - R(0.5) * t2 + R(0.5) * rw * ut2 - sg * R(1.0 / 6.0) * rw * un;
+  f[Z.qd[3]] = f[Z.qdo[3]] + R(0.5) * t2 - R(0.5) * rw * ut2 - sg * R(1.0 / 6.0) * rw * un;
+}
+
+// zou_he_pressure_3d_kernel! (boundary_3d.jl:234-290); rho_m1 = ρ − 1 (shifted form only).
+template <bool SH, int F, class R>
+KMI_DEV void zh3_pressure(R (&f)[19], R rho, R rho_m1) {
+  constexpr ZH3Face Z = kZH3[F];
+  const R sg = R(Z.sign);
+  const R s = zh3_sum<F>(f);
+  R un;
+  if constexpr (!SH) un = -sg * (R(1) - s / rho);
+  else un = -sg * ((rho_m1 - s) / rho);
+  f[Z.qa] = f[Z.qao] - sg * R(1.0 / 3.0) * rho * un;
+  const R t1 = f[Z.t1p] - f[Z.t1m];
+  const R t2 = f[Z.t2p] - f[Z.t2m];
+  f[Z.qd[0]] = f[Z.qdo[0]] - R(0.5) * t1 - sg * R(1.0 / 6.0) * rho * un;
+  f[Z.qd[1]] = f[Z.qdo[1]] + R(0.5) * t1 - sg * R(1.0 / 6.0) * rho * un;
+  f[Z.qd[2]] = f[Z.qdo[2]] - R(0.5) * t2 - sg * R(1.0 / 6.0) * rho * un;
+  f[Z.qd[3]] = f[Z.qdo[3]] + R(0.5) * t2 - sg * R(1.0 / 6.0) * rho * un;
+}
+
+template <bool SH, int F, class R>
+KMI_DEV void zh3_apply(R (&f)[19], const FaceArgs& Fa) {
+  constexpr ZH3Face Z = kZH3[F];
+  constexpr int a1 = Z.axis == 0 ? 1 : 0, a2 = Z.axis == 2 ? 1 : 2;
+  if (Fa.kind == KMI_FACE_ZOUHE_VELOCITY)
+    zh3_velocity<SH, F>(f, R(Fa.u[Z.axis]), R(Fa.u[a1]), R(Fa.u[a2]));
+  else
+    zh3_pressure<SH, F>(f, R(Fa.rho), R(Fa.rho - 1.0));
+}
+
 // Anti-bounce-back of one slot (thermal_2d.jl:175-178); `tl` is T_link (minus T0 when shifted).
 template <class R>
 KMI_DEV R abb(R gq, R w2, R tface, bool on_corner, R tcorner, bool skip) {

@@ -119,6 +119,22 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs a) {
       }
     }
   }
+  if constexpr (BND && Q == 19) {
+    // Face closures in the configured order; an edge node takes each of its faces in turn,
+    // as the reference's one-launch-per-face sequence does.
+    for (int n = 0; n < a.n_zh; ++n) {
+      const int fid = a.zh_face[n];
+      const FaceArgs& F = a.face[fid];
+      switch (fid) {
+        case KMI_WEST: if (i == 0) zh3_apply<SH, 0>(fp, F); break;
+        case KMI_EAST: if (i == nx - 1) zh3_apply<SH, 1>(fp, F); break;
+        case KMI_SOUTH: if (j == 0) zh3_apply<SH, 2>(fp, F); break;
+        case KMI_NORTH: if (j == ny - 1) zh3_apply<SH, 3>(fp, F); break;
+        case KMI_BOTTOM: if (k == 0) zh3_apply<SH, 4>(fp, F); break;
+        default: if (k == a.nz - 1) zh3_apply<SH, 5>(fp, F); break;
+      }
+    }
+  }
 
   const bool solid = BND && a.mask != nullptr && a.mask[self] != 0;
 

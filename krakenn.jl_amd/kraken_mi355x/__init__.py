@@ -11,6 +11,7 @@ from .drivers import (
     D2Q9,
     D3Q19,
     LBMConfig,
+    cavity_3d_desc,
     cavity_desc,
     lbm_config,
     natural_convection_desc,
@@ -20,6 +21,7 @@ from .drivers import (
     rb_initial_temperature,
     reynolds,
     run_cavity_2d,
+    run_cavity_3d,
     run_natural_convection_2d,
     run_rayleigh_benard_2d,
     run_taylor_green_3d,
@@ -32,9 +34,9 @@ from .lattice import ArgumentError, Lattice, MI355XBackend, make_desc
 
 __all__ = [
     "Batch", "STATUS_NAMES", "ensemble_parameters", "ArgumentError", "D2Q9", "D3Q19", "KrakenError", "LBMConfig", "Lattice", "MI355XBackend",
-    "cavity_desc", "lbm_config", "load", "make_desc", "natural_convection_desc",
+    "cavity_3d_desc", "cavity_desc", "lbm_config", "load", "make_desc", "natural_convection_desc",
     "natural_convection_initial_temperature", "omega", "rayleigh_benard_desc",
-    "rb_initial_temperature", "reynolds", "run_cavity_2d", "run_natural_convection_2d",
+    "rb_initial_temperature", "reynolds", "run_cavity_2d", "run_cavity_3d", "run_natural_convection_2d",
     "run_rayleigh_benard_2d", "run_taylor_green_3d", "run_to_residual", "taylor_green_3d_desc",
     "taylor_green_3d_fields", "trt_rates",
 ]

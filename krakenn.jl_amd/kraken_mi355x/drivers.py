@@ -106,6 +106,26 @@ def run_cavity_2d(config: LBMConfig, *, backend=None, T=np.float64, collision="b
     return {"ρ": m["rho"], "ux": m["ux"], "uy": m["uy"], "config": config}
 
 
+def cavity_3d_desc(Nx, Ny, Nz, nu, u_lid, *, storage="f64", collision="bgk"):
+    """Lid-driven cube of run_cavity_3d (basic.jl:153-175): halfway-BB walls from the pull,
+    Zou–He velocity lid (u_lid, 0, 0) on the top face (boundary_3d.jl:317-320)."""
+    walls = {f: {"kind": "wall"} for f in ("west", "east", "south", "north", "bottom")}
+    walls["top"] = {"kind": "zouhe_velocity", "u": (u_lid, 0.0, 0.0)}
+    return make_desc("d3q19", Nx, Ny, Nz, nu, storage=storage, collision=collision, faces=walls)
+
+
+def run_cavity_3d(config: LBMConfig, *, backend=None, T=np.float64, collision="bgk"):
+    """run_cavity_3d (basic.jl:153-175): rest start, `max_steps` steps, final moments."""
+    if not isinstance(config.lattice, D3Q19):
+        raise ArgumentError("run_cavity_3d needs a D3Q19 config")
+    d = cavity_3d_desc(config.Nx, config.Ny, config.Nz, config.nu, config.u_lid,
+                       storage=_storage(T), collision=collision)
+    with Lattice(d, _backend(backend)) as lat:
+        lat.step(config.max_steps)
+        m = lat.moments()
+    return {"ρ": m["rho"], "ux": m["ux"], "uy": m["uy"], "uz": m["uz"], "config": config}
+
+
 def taylor_green_3d_fields(N, U0):
     """u = U0 sin x cos y cos z, v = −U0 cos x sin y cos z, w = 0, ρ = 1, x = 2π(i−1)/N
     (node convention of basic.jl:259-262)."""

@@ -517,6 +517,85 @@ static void zou_he_pressure_west(T* f, T rho_in, int64_t nx, int64_t ny) {
   }
 }
 
+// ZouHeFace3D descriptors (boundary_3d.jl:56-137), kept 1-based as the reference writes
+// them; rows in the face order W, E, S, N, B, T.
+struct ZHFace3 {
+  int axis, sign, qa, qao, qd[4], qdo[4], t1p, t1m, t2p, t2m, par[9], out[5];
+};
+static const ZHFace3 ZH3_TABLE[6] = {
+    {1, -1, 2, 3, {8, 10, 12, 14}, {11, 9, 15, 13}, 4, 5, 6, 7,
+     {1, 4, 5, 6, 7, 16, 17, 18, 19}, {3, 9, 11, 13, 15}},
+    {1, +1, 3, 2, {9, 11, 13, 15}, {10, 8, 14, 12}, 4, 5, 6, 7,
+     {1, 4, 5, 6, 7, 16, 17, 18, 19}, {2, 8, 10, 12, 14}},
+    {2, -1, 4, 5, {8, 9, 16, 18}, {11, 10, 19, 17}, 2, 3, 6, 7,
+     {1, 2, 3, 6, 7, 12, 13, 14, 15}, {5, 10, 11, 17, 19}},
+    {2, +1, 5, 4, {10, 11, 17, 19}, {9, 8, 18, 16}, 2, 3, 6, 7,
+     {1, 2, 3, 6, 7, 12, 13, 14, 15}, {4, 8, 9, 16, 18}},
+    {3, -1, 6, 7, {12, 13, 16, 17}, {15, 14, 19, 18}, 2, 3, 4, 5,
+     {1, 2, 3, 4, 5, 8, 9, 10, 11}, {7, 14, 15, 18, 19}},
+    {3, +1, 7, 6, {14, 15, 18, 19}, {13, 12, 17, 16}, 2, 3, 4, 5,
+     {1, 2, 3, 4, 5, 8, 9, 10, 11}, {6, 12, 13, 16, 17}}};
+
+// Node list of a face: the fixed index on the normal axis, every node of the other two.
+template <class Fn>
+static void for_face_nodes(int face, int64_t nx, int64_t ny, int64_t nz, Fn fn) {
+  const ZHFace3& Z = ZH3_TABLE[face];
+  for (int64_t k = 0; k < nz; ++k)
+    for (int64_t j = 0; j < ny; ++j)
+      for (int64_t i = 0; i < nx; ++i) {
+        const int64_t x[3] = {i, j, k}, n[3] = {nx, ny, nz};
+        const int64_t fixed = Z.sign < 0 ? 0 : n[Z.axis - 1] - 1;
+        if (x[Z.axis - 1] == fixed) fn(I3(i, j, k, nx, ny));
+      }
+}
+
+// zou_he_velocity_3d_kernel! — boundary_3d.jl:152-232.  un, ut1, ut2: the wall velocity
+// along the normal axis and the face's two tangents (the _apply_zou_he_velocity_3d! args).
+template <class T>
+static void zou_he_velocity_3d(T* f, int face, T un, T ut1, T ut2, int64_t nx, int64_t ny,
+                               int64_t nz) {
+  const ZHFace3& Z = ZH3_TABLE[face];
+  const size_t P = size_t(nx * ny * nz);
+  for_face_nodes(face, nx, ny, nz, [&](size_t c) {
+    auto F = [&](int q1) -> T& { return f[c + size_t(q1 - 1) * P]; };
+    T sum_par = T(0), sum_out = T(0);
+    for (int n = 0; n < 9; ++n) sum_par += F(Z.par[n]);
+    for (int n = 0; n < 5; ++n) sum_out += F(Z.out[n]);
+    const T sg = T(Z.sign);
+    const T rho_w = (sum_par + T(2) * sum_out) / (T(1) + sg * un);
+    F(Z.qa) = F(Z.qao) - sg * T(1.0 / 3.0) * rho_w * un;
+    const T d1 = F(Z.par[5]), d2 = F(Z.par[6]), d3 = F(Z.par[7]), d4 = F(Z.par[8]);
+    const T t1 = (F(Z.t1p) - F(Z.t1m)) + (d1 - d2 + d3 - d4);
+    const T t2 = (F(Z.t2p) - F(Z.t2m)) + (d1 + d2 - d3 - d4);
+    F(Z.qd[0]) = F(Z.qdo[0]) - T(0.5) * t1 + T(0.5) * rho_w * ut1 - sg * T(1.0 / 6.0) * rho_w * un;
+    F(Z.qd[1]) = F(Z.qdo[1]) + T(0.5) * t1 - T(0.5) * rho_w * ut1 - sg * T(1.0 / 6.0) * rho_w * un;
+    F(Z.qd[2]) = F(Z.qdo[2]) - T(0.5) * t2 + T(0.5) * rho_w * ut2 - sg * T(1.0 / 6.0) * rho_w * un;
+    F(Z.qd[3]) = F(Z.qdo[3]) + T(0.5) * t2 - T(0.5) * rho_w * ut2 - sg * T(1.0 / 6.0) * rho_w * un;
+  });
+}
+
+// zou_he_pressure_3d_kernel! — boundary_3d.jl:234-290
+template <class T>
+static void zou_he_pressure_3d(T* f, int face, T rho_out, int64_t nx, int64_t ny, int64_t nz) {
+  const ZHFace3& Z = ZH3_TABLE[face];
+  const size_t P = size_t(nx * ny * nz);
+  for_face_nodes(face, nx, ny, nz, [&](size_t c) {
+    auto F = [&](int q1) -> T& { return f[c + size_t(q1 - 1) * P]; };
+    T sum_par = T(0), sum_out = T(0);
+    for (int n = 0; n < 9; ++n) sum_par += F(Z.par[n]);
+    for (int n = 0; n < 5; ++n) sum_out += F(Z.out[n]);
+    const T sg = T(Z.sign);
+    const T un = -sg * (T(1) - (sum_par + T(2) * sum_out) / rho_out);
+    F(Z.qa) = F(Z.qao) - sg * T(1.0 / 3.0) * rho_out * un;
+    const T t1 = F(Z.t1p) - F(Z.t1m);
+    const T t2 = F(Z.t2p) - F(Z.t2m);
+    F(Z.qd[0]) = F(Z.qdo[0]) - T(0.5) * t1 - sg * T(1.0 / 6.0) * rho_out * un;
+    F(Z.qd[1]) = F(Z.qdo[1]) + T(0.5) * t1 - sg * T(1.0 / 6.0) * rho_out * un;
+    F(Z.qd[2]) = F(Z.qdo[2]) - T(0.5) * t2 - sg * T(1.0 / 6.0) * rho_out * un;
+    F(Z.qd[3]) = F(Z.qdo[3]) + T(0.5) * t2 - sg * T(1.0 / 6.0) * rho_out * un;
+  });
+}
+
 // stream_3d_kernel! — collide_stream_3d.jl:5-43: pull, halfway BB at all six faces.
 // stream_periodic_xz_wall_y_3d_kernel! — stream_periodic_3d.jl:24-75 (px = pz = true);
 // PullHalfwayBB_3D{true} — dsl/bricks_3d.jl:106-136 (pz = true only).
@@ -892,6 +971,17 @@ static void zou_he_faces(const kmi_desc& d, T* f) {
   for (int oi = 0; oi < n; ++oi) {
     int fid = order[oi];
     const kmi_face& F = d.faces[fid];
+    if (d.stencil == KMI_D3Q19) {
+      // _apply_zou_he_velocity_3d! argument mapping: normal component, then the two
+      // tangents in axis order (boundary_3d.jl:317-390).
+      const int ax = ZH3_TABLE[fid].axis - 1;
+      const int a1 = ax == 0 ? 1 : 0, a2 = ax == 2 ? 1 : 2;
+      if (F.kind == KMI_FACE_ZOUHE_VELOCITY)
+        zou_he_velocity_3d(f, fid, T(F.u[ax]), T(F.u[a1]), T(F.u[a2]), d.nx, d.ny, d.nz);
+      else if (F.kind == KMI_FACE_ZOUHE_PRESSURE)
+        zou_he_pressure_3d(f, fid, T(F.rho), d.nx, d.ny, d.nz);
+      continue;
+    }
     if (F.kind == KMI_FACE_ZOUHE_VELOCITY) {
       if (fid == KMI_NORTH) zou_he_north(f, T(F.u[0]), d.nx, d.ny);
       else if (fid == KMI_SOUTH) zou_he_south(f, T(F.u[0]), d.nx, d.ny);
@@ -909,7 +999,7 @@ static void generic_step(const kmi_desc& d, const uint8_t* solid, T*& f, T*& f2,
   const int64_t nx = d.nx, ny = d.ny, nz = d.stencil == KMI_D2Q9 ? 1 : d.nz;
   const size_t P = size_t(nx * ny * nz);
   pull_generic(d, f2, f);
-  if (d.stencil == KMI_D2Q9) zou_he_faces(d, f2);
+  zou_he_faces(d, f2);
   if (d.thermal) {
     pull_generic(d, g2, g, false);
     bool dir[4];
@@ -1022,6 +1112,14 @@ extern "C" {
   }                                                                                      \
   void ko_zou_he_pressure_west_2d_##S(T* f, double r, int64_t nx, int64_t ny) {         \
     ko::zou_he_pressure_west(f, T(r), nx, ny);                                           \
+  }                                                                                      \
+  void ko_zou_he_velocity_3d_##S(T* f, int64_t face, double un, double ut1, double ut2,     \
+                                 int64_t nx, int64_t ny, int64_t nz) {                   \
+    ko::zou_he_velocity_3d(f, int(face), T(un), T(ut1), T(ut2), nx, ny, nz);                  \
+  }                                                                                      \
+  void ko_zou_he_pressure_3d_##S(T* f, int64_t face, double r, int64_t nx, int64_t ny,      \
+                                 int64_t nz) {                                           \
+    ko::zou_he_pressure_3d(f, int(face), T(r), nx, ny, nz);                                   \
   }                                                                                      \
   void ko_stream_3d_##S(T* fo, const T* fi, int64_t nx, int64_t ny, int64_t nz, int px,  \
                         int py, int pz) {                                                \

@@ -52,7 +52,7 @@ def _validate(kmi, d):
     ("pressure_north", 5, "Pressure boundary condition on face north"),
     ("thermal_trt", 5, "BGK"),
     ("thermal_3d", 5, "D2Q9"),
-    ("zh_3d", 5, "Zou-He"),
+    ("zh_3d_normal", 1, "1 + sign*u_n"),
     ("dirichlet_periodic", 1, "periodic"),
     ("bad_size", 1, "positive"),
 ])
@@ -70,8 +70,8 @@ def test_desc_validation_errors(kmi, case, status, needle):
         d = mk("d2q9", 8, 8, collision="trt", thermal=True)
     elif case == "thermal_3d":
         d = mk("d3q19", 8, 8, 8, thermal=True)
-    elif case == "zh_3d":
-        d = mk("d3q19", 8, 8, 8, faces={"top": {"kind": "zouhe_velocity", "u": (0.1, 0, 0)}})
+    elif case == "zh_3d_normal":
+        d = mk("d3q19", 8, 8, 8, faces={"top": {"kind": "zouhe_velocity", "u": (0.1, 0, -1.0)}})
     elif case == "dirichlet_periodic":
         d = mk("d2q9", 8, 8, thermal=True, faces={"west": {"kind": "periodic", "thermal": "dirichlet"},
                                                   "east": {"kind": "periodic"}})
@@ -99,3 +99,12 @@ def test_no_device_is_an_error_not_a_fallback(kmi):
         kmi.MI355XBackend(0)
     with pytest.raises(kmi.ArgumentError):
         kmi.run_cavity_2d(kmi.lbm_config(kmi.D2Q9(), Nx=8, Ny=8, nu=0.1, max_steps=1))
+
+
+def test_d3q19_zou_he_faces_validate(kmi):
+    """Zou–He velocity and pressure are accepted on every D3Q19 face (boundary_3d.jl:152-290)."""
+    faces = {"west": {"kind": "zouhe_velocity", "u": (0.02, 0.01, 0)}, "east": {"kind": "zouhe_pressure", "rho": 1.0},
+             "south": {"kind": "zouhe_velocity", "u": (0.01, 0, 0)}, "north": {"kind": "zouhe_pressure", "rho": 0.99},
+             "bottom": {"kind": "zouhe_velocity", "u": (0, 0, 0.01)}, "top": {"kind": "zouhe_velocity", "u": (0.1, 0, 0)}}
+    rc, msg = _validate(kmi, kmi.make_desc("d3q19", 8, 8, 8, faces=faces))
+    assert rc == 0, msg

@@ -68,13 +68,27 @@ def _case(kmi, name, storage):
     if name == "d3q19_pz_trt_walls":
         return kmi.make_desc("d3q19", 10, 8, 6, 0.02, storage=storage, collision="trt",
                              faces=per("bottom", "top")), (10, 8, 6), None, False, 33
+    if name == "d3q19_cavity_zh_lid_mask":
+        return kmi.cavity_3d_desc(12, 10, 8, 0.05, 0.08, storage=storage), (12, 10, 8), _mask_blob((12, 10, 8), (6, 5, 3), 2), False, 35
+    if name == "d3q19_channel_zh_trt":
+        z = {"kind": "zouhe_velocity", "u": (0.0, 0.0, 0.0)}
+        f = {"west": {"kind": "zouhe_velocity", "u": (0.03, 0.01, -0.005)}, "east": {"kind": "zouhe_pressure", "rho": 1.0},
+             "south": dict(z), "north": {"kind": "zouhe_velocity", "u": (0.02, 0.0, 0.01)}, "bottom": dict(z), "top": dict(z)}
+        d = kmi.make_desc("d3q19", 14, 8, 6, 0.04, storage=storage, collision="trt", faces=f,
+                          bc_order=["west", "east", "top", "bottom", "south", "north"])
+        return d, (14, 8, 6), None, False, 40
+    if name == "d3q19_zh_pressure_top_px_py":
+        f = per("west", "east", "south", "north")
+        f.update(bottom={"kind": "zouhe_velocity", "u": (0.01, -0.02, 0.015)}, top={"kind": "zouhe_pressure", "rho": 0.995})
+        return kmi.make_desc("d3q19", 8, 6, 10, 0.03, storage=storage, faces=f), (8, 6, 10), None, False, 33
     raise KeyError(name)
 
 
 CASES = ["d2q9_cavity_zh_bgk_mask", "d2q9_cavity_ladd_trt", "d2q9_cavity_mrt", "d2q9_periodic_bgk",
          "d2q9_periodic_trt", "d2q9_couette_zh", "d2q9_channel_inlet_outlet", "d2q9_rb_split",
          "d2q9_natconv_fused", "d2q9_thermal_corners_mask", "d3q19_periodic_trt",
-         "d3q19_cavity_bgk_ladd_mask", "d3q19_couette_trt_ladd", "d3q19_pz_trt_walls"]
+         "d3q19_cavity_bgk_ladd_mask", "d3q19_couette_trt_ladd", "d3q19_pz_trt_walls",
+         "d3q19_cavity_zh_lid_mask", "d3q19_channel_zh_trt", "d3q19_zh_pressure_top_px_py"]
 
 
 def _init(shape, thermal, seed):
