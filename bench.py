@@ -182,7 +182,7 @@ def main():
             lat.backend.synchronize()
             wall = time.perf_counter() - t0
             mass = lat.reduce("mass")
-            launch = lat.launch
+            launch = {**lat.launch, **lat.temporal}
         return ms, wall, mass, launch
 
     ms, wall, mass, launch = run(args.storage, args.steps, args.warmup)

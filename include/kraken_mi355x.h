@@ -39,7 +39,7 @@
 extern "C" {
 #endif
 
-#define KMI_ABI_VERSION 1
+#define KMI_ABI_VERSION 2
 
 typedef enum {
   KMI_OK = 0,
@@ -54,7 +54,9 @@ typedef enum { KMI_D2Q9 = 0, KMI_D3Q19 = 1 } kmi_stencil;
 
 /* Population storage in HBM.  Arithmetic is FP64 for F64 and FP32 otherwise.
  * F32S / F16S store the deviation f_q - w_q (thermal: g_q - w_q T_ref); F16S scales
- * it by 2^15 before rounding to IEEE binary16. */
+ * it by 2^15 before rounding to IEEE binary16.  The thermal populations of an F16S
+ * lattice are stored as F32S: binary16 cannot resolve per-step temperature increments
+ * against deviations of order w_q (T - T_ref). */
 typedef enum {
   KMI_STORE_F64 = 0,
   KMI_STORE_F32 = 1,
@@ -75,14 +77,28 @@ typedef enum { KMI_BGK = 0, KMI_TRT = 1, KMI_MRT = 2 } kmi_collision;
  *                     any velocity).
  *   ZOUHE_PRESSURE  — bounce-back pull, then the Zou–He pressure closure
  *                     (D2Q9: boundary_2d.jl:206-299, west and east; D3Q19:
- *                     boundary_3d.jl:234-290, any face). */
+ *                     boundary_3d.jl:234-290, any face).
+ *   ZOUHE_VELOCITY_PROFILE — D2Q9 north, south or west: per-node wall velocity (u_x, u_y)
+ *                     set by kmi_set_face_profile; the two corner nodes of the face keep
+ *                     the bounce-back pull (boundary_spatial_2d.jl:6-90).
+ *   ZOUHE_PRESSURE_PROFILE — D2Q9 east: per-node density, corner nodes skipped
+ *                     (boundary_spatial_2d.jl:104-131). */
 typedef enum {
   KMI_FACE_PERIODIC = 0,
   KMI_FACE_WALL = 1,
   KMI_FACE_MOVING_WALL = 2,
   KMI_FACE_ZOUHE_VELOCITY = 3,
-  KMI_FACE_ZOUHE_PRESSURE = 4
+  KMI_FACE_ZOUHE_PRESSURE = 4,
+  KMI_FACE_ZOUHE_VELOCITY_PROFILE = 5,
+  KMI_FACE_ZOUHE_PRESSURE_PROFILE = 6
 } kmi_face_kind;
+
+/* Body force on the flow populations (D2Q9 BGK).
+ *   NONE — none.
+ *   GUO  — constant force (F_x, F_y) with the Guo source term, collide_guo_2d!
+ *          (collide_guo_2d.jl:5-72); moments then report u = (sum c f + F/2) / rho,
+ *          compute_macroscopic_forced_2d! (macroscopic.jl:84-98). */
+typedef enum { KMI_FORCE_NONE = 0, KMI_FORCE_GUO = 1 } kmi_forcing;
 
 /* Faces: x-, x+, y-, y+, z-, z+ (the reference's west/east/south/north/bottom/top). */
 typedef enum {
@@ -130,6 +146,8 @@ typedef struct {
   int32_t n_bc_order;       /* 0: default order W,E,S,N,B,T; else the first n entries of */
   int32_t bc_order[6];      /*   bc_order give the order Zou–He closures are applied in   */
   kmi_face faces[6];        /* indexed by kmi_face_id; z faces ignored for D2Q9           */
+  int32_t forcing;          /* kmi_forcing                                                */
+  double force[3];          /* body force (F_x, F_y, F_z) for KMI_FORCE_GUO               */
 } kmi_desc;
 
 typedef struct kmi_ctx kmi_ctx;
@@ -189,6 +207,16 @@ int kmi_lattice_create(kmi_ctx* ctx, const kmi_desc* desc, kmi_lattice** out);
 int kmi_lattice_destroy(kmi_lattice* lat);
 /* Solid mask, one byte per cell in [i, j, k] order (nonzero = solid); NULL clears it. */
 int kmi_set_mask(kmi_lattice* lat, const uint8_t* is_solid);
+/* Per-node values of a *_PROFILE face: velocity profile (a = u_x, b = u_y) or pressure
+ * profile (a = rho, b ignored), each of face length (nx for south/north, ny for
+ * west/east), index 0 at the low end of the face — the arrays _fill_bc_arrays! builds
+ * (src/bc/handlers.jl:152-187).  Until set, a velocity profile is zero and a pressure
+ * profile equals the face's rho.  May be called between steps (time-dependent profiles,
+ * src/bc/handlers.jl:203-208). */
+int kmi_set_face_profile(kmi_lattice* lat, int32_t face, const double* a, const double* b);
+/* Replace the values (u, rho, T) of one face between steps; the face kind and thermal
+ * kind must stay the same (time-dependent scalar conditions, handlers.jl:210-218). */
+int kmi_set_face(kmi_lattice* lat, int32_t face, const kmi_face* values);
 /* Whole population field in the reference layout [i, j, (k,) q], nx*ny*nz*Q values. */
 int kmi_upload(kmi_lattice* lat, int32_t field, const void* host, int32_t host_dtype);
 int kmi_download(kmi_lattice* lat, int32_t field, void* host, int32_t host_dtype);
@@ -222,8 +250,16 @@ int kmi_lattice_device_bytes(const kmi_lattice* lat, int64_t* bytes);
  * population stores on/off.  Results do not depend on these settings. */
 int kmi_lattice_set_launch(kmi_lattice* lat, int32_t bx, int32_t by, int32_t nontemporal);
 int kmi_lattice_get_launch(const kmi_lattice* lat, int32_t* bx, int32_t* by, int32_t* nontemporal);
-/* Time every candidate shape × store policy on this lattice for ~steps_per_trial steps
- * each, keep the fastest, and restore the populations, parity and step count. */
+/* Two steps per launch (temporal blocking) for fully periodic D3Q19 BGK/TRT lattices
+ * without mask: the intermediate populations stay in LDS, halving HBM traffic per step.
+ * tile: 0 off, 1 = 32x8, 2 = 64x4, 3 = 64x8 cells per work-group (3 not for F64); nx and
+ * ny must be multiples of the tile.  zchunks: z-ranges per tile column (1..nz).  Results
+ * are bit-identical to single-step launches; odd step counts end with one single step. */
+int kmi_lattice_set_temporal(kmi_lattice* lat, int32_t tile, int32_t zchunks);
+int kmi_lattice_get_temporal(const kmi_lattice* lat, int32_t* tile, int32_t* zchunks);
+/* Time every candidate shape × store policy (and two-step variant, where it applies) on
+ * this lattice for ~steps_per_trial steps each, keep the fastest, and restore the
+ * populations, parity and step count. */
 int kmi_lattice_autotune(kmi_lattice* lat, int64_t steps_per_trial, double* best_ms_per_step);
 /* Device copy bandwidth (float4 loads, non-temporal stores) over `bytes`, GB/s counting
  * read + write, averaged over `reps` copies: the practical HBM ceiling of a 1:1 kernel. */

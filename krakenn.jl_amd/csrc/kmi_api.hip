@@ -18,16 +18,19 @@ struct kmi_lattice {
   kmi_desc desc{};
   int Q = 9;
   long long ncells = 0, plane = 0;
-  size_t esize = 8;
+  size_t esize = 8;  // bytes per flow population
+  size_t gsize = 8;  // bytes per thermal population (thermal_storage of the storage kind)
   void* f[2] = {nullptr, nullptr};
   void* g[2] = {nullptr, nullptr};
   uint8_t* mask = nullptr;
   double* uprev = nullptr;
+  double* prof[4][2] = {};  // device per-node values of *_PROFILE faces
   int cur = 0;
   int64_t steps = 0;
   StepArgs args{};
   bool bnd = true;
   int bx = 0, by = 1, nts = 0;  // launch shape and store policy (0: not chosen yet)
+  int tb_tile = 0, tb_zchunks = 1;  // two-step launches (0: off), see kmi_step2.hip
   dim3 grid, block;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   hipEvent_t ev[2] = {nullptr, nullptr};
@@ -154,7 +157,7 @@ static int validate(const kmi_desc& d, std::string& why, int& code) {
     return code = KMI_ERR_UNSUPPORTED, why = "more than 2^31 populations per field", 1;
   const int nf = d.stencil == KMI_D2Q9 ? 4 : 6;
   for (int f = 0; f < nf; ++f)
-    if (d.faces[f].kind < KMI_FACE_PERIODIC || d.faces[f].kind > KMI_FACE_ZOUHE_PRESSURE)
+    if (d.faces[f].kind < KMI_FACE_PERIODIC || d.faces[f].kind > KMI_FACE_ZOUHE_PRESSURE_PROFILE)
       return why = std::string("unknown face kind on ") + kFaceName[f], 1;
   for (int a = 0; a < nf / 2; ++a) {
     const bool p0 = d.faces[2 * a].kind == KMI_FACE_PERIODIC;
@@ -171,9 +174,29 @@ static int validate(const kmi_desc& d, std::string& why, int& code) {
   if (d.thermal && d.collision != KMI_BGK)
     return code = KMI_ERR_UNSUPPORTED,
            why = "the thermal DDF uses BGK + Guo for the flow (thermal_2d.jl:421)", 1;
+  if (d.forcing != KMI_FORCE_NONE && d.forcing != KMI_FORCE_GUO) return why = "unknown forcing", 1;
+  if (d.forcing == KMI_FORCE_GUO &&
+      (d.stencil != KMI_D2Q9 || d.collision != KMI_BGK || d.thermal))
+    return code = KMI_ERR_UNSUPPORTED,
+           why = "Guo body force is the D2Q9 BGK collision (collide_guo_2d.jl:5-72)", 1;
   for (int f = 0; f < nf; ++f) {
     const kmi_face& F = d.faces[f];
-    if (F.kind == KMI_FACE_ZOUHE_PRESSURE && !(F.rho > 0.0))
+    if (F.kind == KMI_FACE_ZOUHE_VELOCITY_PROFILE || F.kind == KMI_FACE_ZOUHE_PRESSURE_PROFILE) {
+      if (d.stencil != KMI_D2Q9)
+        return code = KMI_ERR_UNSUPPORTED, why = "profile faces are D2Q9 only (boundary_spatial_2d.jl)", 1;
+      if (F.kind == KMI_FACE_ZOUHE_VELOCITY_PROFILE && f == KMI_EAST)
+        return code = KMI_ERR_UNSUPPORTED,
+               why = "Zou-He velocity profile on the east face is not defined by the reference "
+                     "(boundary_spatial_2d.jl: north, south, west)",
+               1;
+      if (F.kind == KMI_FACE_ZOUHE_PRESSURE_PROFILE && f != KMI_EAST)
+        return code = KMI_ERR_UNSUPPORTED,
+               why = std::string("Pressure profile on face ") + kFaceName[f] +
+                     " is not supported (boundary_spatial_2d.jl: east)",
+               1;
+    }
+    if ((F.kind == KMI_FACE_ZOUHE_PRESSURE || F.kind == KMI_FACE_ZOUHE_PRESSURE_PROFILE) &&
+        !(F.rho > 0.0))
       return why = "Zou-He pressure density must be positive", 1;
     if (F.kind == KMI_FACE_ZOUHE_VELOCITY && d.stencil == KMI_D3Q19) {
       // ρ_w = (Σ_par + 2 Σ_out) / (1 + sign u_n) (boundary_3d.jl:186-187)
@@ -213,6 +236,11 @@ static int validate(const kmi_desc& d, std::string& why, int& code) {
   return 0;
 }
 
+// Nodes along a face: nx for south/north, ny for west/east (D2Q9).
+static int64_t face_length(const kmi_lattice* L, int f) {
+  return (f == KMI_SOUTH || f == KMI_NORTH) ? L->desc.nx : L->desc.ny;
+}
+
 static void build_args(kmi_lattice* L) {
   const kmi_desc& d = L->desc;
   StepArgs& a = L->args;
@@ -236,7 +264,7 @@ static void build_args(kmi_lattice* L) {
   a.n_zh = 0;
   for (int i = 0; i < n; ++i) {
     const int k = d.faces[order[i]].kind;
-    if (k == KMI_FACE_ZOUHE_VELOCITY || k == KMI_FACE_ZOUHE_PRESSURE) a.zh_face[a.n_zh++] = order[i];
+    if (k >= KMI_FACE_ZOUHE_VELOCITY) a.zh_face[a.n_zh++] = order[i];
   }
   a.omega = d.omega;
   a.s_plus = d.s_plus;
@@ -249,6 +277,9 @@ static void build_args(kmi_lattice* L) {
   a.beta_g = d.beta_g;
   a.T_ref = d.T_ref;
   a.coupling = d.coupling;
+  a.forcing = d.forcing;
+  a.fx = d.force[0];
+  a.fy = d.force[1];
   bool dir[4];
   for (int f = 0; f < 4; ++f)
     dir[f] = d.thermal && d.faces[f].kind != KMI_FACE_PERIODIC &&
@@ -268,6 +299,8 @@ static void build_args(kmi_lattice* L) {
     F.T = d.faces[f].T;
     F.Tc[0] = F.Tc[1] = d.faces[f].T;
     F.skip[0] = F.skip[1] = 0;
+    F.prof[0] = f < 4 ? L->prof[f][0] : nullptr;
+    F.prof[1] = f < 4 ? L->prof[f][1] : nullptr;
     if (f < 4 && dir[f]) {
       if (f == KMI_WEST || f == KMI_EAST) {
         F.Tc[0] = corner(f, KMI_SOUTH);
@@ -280,7 +313,7 @@ static void build_args(kmi_lattice* L) {
   }
   bool any_nonperiodic = false;
   for (int f = 0; f < nf; ++f) any_nonperiodic |= d.faces[f].kind != KMI_FACE_PERIODIC;
-  L->bnd = any_nonperiodic || L->mask != nullptr;
+  L->bnd = any_nonperiodic || L->mask != nullptr || d.forcing != KMI_FORCE_NONE;
 }
 
 static void set_geometry(kmi_lattice* L, int bx, int by, int nts) {
@@ -300,19 +333,40 @@ static void default_geometry(kmi_lattice* L) {
   int bx = int(std::min<int64_t>(256, ((d.nx + 63) / 64) * 64));
   const long long groups = ((d.nx + bx - 1) / bx) * d.ny * L->args.nz;
   if (groups < 2048 && bx > 64) bx = 64;
-  const double field_bytes = double(L->plane) * L->Q * L->esize;
+  const double field_bytes =
+      double(L->plane) * (L->Q * L->esize + (L->desc.thermal ? 9 * L->gsize : 0));
   set_geometry(L, bx, 1, field_bytes > 512e6 ? 1 : 0);
 }
 
-static int launch_one(kmi_lattice* L, hipStream_t s) {
+// Whether two-step launches with `tile` apply: fully periodic D3Q19 BGK/TRT, no mask,
+// and the x–y extent a multiple of the tile.
+static bool tb_eligible(const kmi_lattice* L, int tile) {
+  int TX, TY;
+  if (!step2_tile(tile, L->desc.storage, TX, TY)) return false;
+  return !L->bnd && L->Q == 19 && !L->desc.thermal &&
+         (L->desc.collision == KMI_BGK || L->desc.collision == KMI_TRT) &&
+         L->desc.nx % TX == 0 && L->desc.ny % TY == 0;
+}
+
+// Steps advanced by one launch.
+static int launch_span(const kmi_lattice* L) {
+  return L->tb_tile && tb_eligible(L, L->tb_tile) ? 2 : 1;
+}
+
+// One launch: a single step, or two (two = true, temporal blocking enabled).  Either way
+// the result lands in the other buffer.
+static int launch_one(kmi_lattice* L, hipStream_t s, bool two = false) {
   StepArgs a = L->args;
   a.fi = L->f[L->cur];
   a.fo = L->f[L->cur ^ 1];
   a.gi = L->g[L->cur];
   a.go = L->g[L->cur ^ 1];
   a.mask = L->mask;
-  hipError_t e = launch_step(L->Q, L->desc.storage, L->desc.collision, L->desc.thermal != 0,
-                             L->bnd, L->nts != 0, a, L->grid, L->block, s);
+  hipError_t e =
+      two ? launch_step2(L->desc.storage, L->desc.collision, L->tb_tile, L->tb_zchunks,
+                         L->nts != 0, a, s)
+          : launch_step(L->Q, L->desc.storage, L->desc.collision, L->desc.thermal != 0, L->bnd,
+                        L->nts != 0, a, L->grid, L->block, s);
   if (e != hipSuccess) return hip_fail(L->ctx, e, "step kernel launch");
   L->cur ^= 1;
   return KMI_OK;
@@ -335,8 +389,9 @@ static int run_steps(kmi_lattice* L, int64_t n) {
     if (!L->graph[p]) {
       hipGraph_t gph = nullptr;
       KMI_HIP(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      for (int k = 0; k < kGraphSteps; ++k) {
-        int rc = launch_one(L, s);
+      const int span = launch_span(L);
+      for (int k = 0; k < kGraphSteps; k += span) {
+        int rc = launch_one(L, s, span == 2);
         if (rc != KMI_OK) {
           (void)hipStreamEndCapture(s, &gph);
           if (gph) (void)hipGraphDestroy(gph);
@@ -352,6 +407,11 @@ static int run_steps(kmi_lattice* L, int64_t n) {
     KMI_HIP(c, hipGraphLaunch(L->graph[p], s));
     n -= kGraphSteps;
     L->steps += kGraphSteps;
+  }
+  for (; n >= 2 && launch_span(L) == 2; n -= 2) {
+    int rc = launch_one(L, s, true);
+    if (rc != KMI_OK) return rc;
+    L->steps += 2;
   }
   for (; n > 0; --n) {
     int rc = launch_one(L, s);
@@ -392,10 +452,24 @@ int kmi_lattice_create(kmi_ctx* ctx, const kmi_desc* desc, kmi_lattice** out) {
   L->ncells = (long long)desc->nx * desc->ny * L->desc.nz;
   L->plane = L->ncells;
   L->esize = storage_bytes(desc->storage);
+  L->gsize = storage_bytes(thermal_storage(desc->storage));
   const size_t bytes = size_t(L->plane) * L->Q * L->esize;
+  const size_t gbytes = size_t(L->plane) * 9 * L->gsize;
   hipError_t e = hipSuccess;
   for (int p = 0; p < 2 && e == hipSuccess; ++p) e = hipMalloc(&L->f[p], bytes);
-  for (int p = 0; p < 2 && e == hipSuccess && desc->thermal; ++p) e = hipMalloc(&L->g[p], bytes);
+  for (int p = 0; p < 2 && e == hipSuccess && desc->thermal; ++p) e = hipMalloc(&L->g[p], gbytes);
+  // Profile faces: velocity profiles start at zero, pressure profiles at the face's rho.
+  for (int f = 0; f < 4 && e == hipSuccess && L->Q == 9; ++f) {
+    const int kind = desc->faces[f].kind;
+    if (kind != KMI_FACE_ZOUHE_VELOCITY_PROFILE && kind != KMI_FACE_ZOUHE_PRESSURE_PROFILE) continue;
+    const size_t len = size_t(face_length(L, f));
+    const std::vector<double> init(len, kind == KMI_FACE_ZOUHE_PRESSURE_PROFILE ? desc->faces[f].rho : 0.0);
+    for (int k = 0; k < (kind == KMI_FACE_ZOUHE_VELOCITY_PROFILE ? 2 : 1) && e == hipSuccess; ++k) {
+      e = hipMalloc(&L->prof[f][k], len * sizeof(double));
+      if (e == hipSuccess)
+        e = hipMemcpy(L->prof[f][k], init.data(), len * sizeof(double), hipMemcpyHostToDevice);
+    }
+  }
   if (e == hipSuccess) e = hipEventCreate(&L->ev[0]);
   if (e == hipSuccess) e = hipEventCreate(&L->ev[1]);
   if (e != hipSuccess) {
@@ -428,6 +502,9 @@ int kmi_lattice_destroy(kmi_lattice* L) {
   }
   if (L->mask) (void)hipFree(L->mask);
   if (L->uprev) (void)hipFree(L->uprev);
+  for (auto& pf : L->prof)
+    for (double* p : pf)
+      if (p) (void)hipFree(p);
   delete L;
   return KMI_OK;
 }
@@ -449,15 +526,62 @@ int kmi_set_mask(kmi_lattice* L, const uint8_t* is_solid) {
   return KMI_OK;
 }
 
-static int field_ptrs(kmi_lattice* L, int field, void**& buf, double& shift) {
+int kmi_set_face_profile(kmi_lattice* L, int32_t face, const double* a, const double* b) {
+  if (!L) return fail(nullptr, KMI_ERR_ARG, "kmi_set_face_profile: lattice is NULL");
+  kmi_ctx* c = L->ctx;
+  if (face < 0 || face >= 4 || L->Q != 9)
+    return fail(c, KMI_ERR_ARG, "kmi_set_face_profile: face must be a D2Q9 face (0..3)");
+  const int kind = L->desc.faces[face].kind;
+  if (kind != KMI_FACE_ZOUHE_VELOCITY_PROFILE && kind != KMI_FACE_ZOUHE_PRESSURE_PROFILE)
+    return fail(c, KMI_ERR_STATE, std::string("kmi_set_face_profile: face ") + kFaceName[face] +
+                                      " is not a profile face");
+  if (!a || (kind == KMI_FACE_ZOUHE_VELOCITY_PROFILE && !b))
+    return fail(c, KMI_ERR_ARG, "kmi_set_face_profile: NULL profile array");
+  const size_t len = size_t(face_length(L, face));
+  if (kind == KMI_FACE_ZOUHE_PRESSURE_PROFILE)
+    for (size_t n = 1; n + 1 < len; ++n)
+      if (!(a[n] > 0.0)) return fail(c, KMI_ERR_ARG, "kmi_set_face_profile: density must be positive");
+  KMI_HIP(c, hipSetDevice(c->device));
+  const double* src[2] = {a, b};
+  for (int k = 0; k < (kind == KMI_FACE_ZOUHE_VELOCITY_PROFILE ? 2 : 1); ++k)
+    KMI_HIP(c, hipMemcpyAsync(L->prof[face][k], src[k], len * sizeof(double),
+                              hipMemcpyHostToDevice, c->stream));
+  KMI_HIP(c, hipStreamSynchronize(c->stream));
+  return KMI_OK;
+}
+
+int kmi_set_face(kmi_lattice* L, int32_t face, const kmi_face* v) {
+  if (!L) return fail(nullptr, KMI_ERR_ARG, "kmi_set_face: lattice is NULL");
+  kmi_ctx* c = L->ctx;
+  const int nf = L->Q == 9 ? 4 : 6;
+  if (!v || face < 0 || face >= nf) return fail(c, KMI_ERR_ARG, "kmi_set_face: bad face or NULL values");
+  const kmi_face& old = L->desc.faces[face];
+  if (v->kind != old.kind || v->thermal != old.thermal)
+    return fail(c, KMI_ERR_ARG, "kmi_set_face: the face kind and thermal kind cannot change");
+  kmi_desc d = L->desc;
+  d.faces[face] = *v;
+  std::string why;
+  int code;
+  if (validate(d, why, code)) return fail(c, code, "kmi_set_face: " + why);
+  KMI_HIP(c, hipSetDevice(c->device));
+  KMI_HIP(c, hipStreamSynchronize(c->stream));
+  drop_graphs(L);
+  L->desc = d;
+  build_args(L);
+  return KMI_OK;
+}
+
+static int field_ptrs(kmi_lattice* L, int field, void**& buf, double& shift, int& st) {
   if (field == KMI_FIELD_F) {
     buf = L->f;
     shift = storage_shift_f();
+    st = L->desc.storage;
     return KMI_OK;
   }
   if (field == KMI_FIELD_G && L->desc.thermal) {
     buf = L->g;
     shift = L->desc.T_ref;
+    st = thermal_storage(L->desc.storage);
     return KMI_OK;
   }
   return fail(L->ctx, KMI_ERR_ARG, "unknown field or lattice without thermal populations");
@@ -471,7 +595,8 @@ int kmi_upload(kmi_lattice* L, int32_t field, const void* host, int32_t host_dty
     return fail(c, KMI_ERR_ARG, "kmi_upload: unknown host dtype");
   void** buf;
   double shift;
-  if (int rc = field_ptrs(L, field, buf, shift)) return rc;
+  int st;
+  if (int rc = field_ptrs(L, field, buf, shift, st)) return rc;
   KMI_HIP(c, hipSetDevice(c->device));
   const size_t n = size_t(L->ncells) * L->Q;
   std::vector<double> tmp;
@@ -486,8 +611,7 @@ int kmi_upload(kmi_lattice* L, int32_t field, const void* host, int32_t host_dty
   KMI_HIP(c, hipMalloc(&stage, n * sizeof(double)));
   hipError_t e = hipMemcpyAsync(stage, src, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess)
-    e = launch_to_storage(L->Q, L->desc.storage, stage, buf[L->cur], L->ncells, L->plane, shift,
-                          c->stream);
+    e = launch_to_storage(L->Q, st, stage, buf[L->cur], L->ncells, L->plane, shift, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   (void)hipFree(stage);
   if (e != hipSuccess) return hip_fail(c, e, "kmi_upload");
@@ -503,7 +627,8 @@ int kmi_download(kmi_lattice* L, int32_t field, void* host, int32_t host_dtype) 
     return fail(c, KMI_ERR_ARG, "kmi_download: unknown host dtype");
   void** buf;
   double shift;
-  if (int rc = field_ptrs(L, field, buf, shift)) return rc;
+  int st;
+  if (int rc = field_ptrs(L, field, buf, shift, st)) return rc;
   KMI_HIP(c, hipSetDevice(c->device));
   const size_t n = size_t(L->ncells) * L->Q;
   double* stage = nullptr;
@@ -514,8 +639,8 @@ int kmi_download(kmi_lattice* L, int32_t field, void* host, int32_t host_dtype) 
     tmp.resize(n);
     dst = tmp.data();
   }
-  hipError_t e = launch_from_storage(L->Q, L->desc.storage, buf[L->cur], stage, L->ncells,
-                                     L->plane, shift, c->stream);
+  hipError_t e = launch_from_storage(L->Q, st, buf[L->cur], stage, L->ncells, L->plane, shift,
+                                     c->stream);
   if (e == hipSuccess)
     e = hipMemcpyAsync(dst, stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -612,12 +737,20 @@ int kmi_step_timed(kmi_lattice* L, int64_t nsteps, double* ms) {
   return KMI_OK;
 }
 
+// Body force seen by the moments (u = (Σcf + F/2)/ρ under Guo forcing).
+static double force_x(const kmi_lattice* L) {
+  return L->desc.forcing == KMI_FORCE_GUO ? L->desc.force[0] : 0.0;
+}
+static double force_y(const kmi_lattice* L) {
+  return L->desc.forcing == KMI_FORCE_GUO ? L->desc.force[1] : 0.0;
+}
+
 static int reduce_cells(kmi_lattice* L, int kind, double* out3) {
   kmi_ctx* c = L->ctx;
   hipError_t e = launch_reduce_cells(L->Q, L->desc.storage, L->f[L->cur],
                                      L->desc.thermal ? L->g[L->cur] : nullptr, L->mask, L->uprev,
-                                     L->ncells, L->plane, L->desc.T_ref, kind, c->partials, c->out,
-                                     c->stream);
+                                     L->ncells, L->plane, L->desc.T_ref, force_x(L), force_y(L), kind,
+                                     c->partials, c->out, c->stream);
   if (e != hipSuccess) return hip_fail(c, e, "reduction kernel");
   if (kind == 101) return KMI_OK;
   KMI_HIP(c, hipMemcpyAsync(out3, c->out, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -658,7 +791,8 @@ int kmi_moments(kmi_lattice* L, double* rho, double* ux, double* uy, double* uz,
   for (int k = 0; k < 5; ++k) dev[k] = host[k] ? block + size_t(slot++) * L->ncells : nullptr;
   hipError_t e = launch_moments(L->Q, L->desc.storage, L->f[L->cur],
                                 L->desc.thermal ? L->g[L->cur] : nullptr, dev[0], dev[1], dev[2],
-                                dev[3], dev[4], L->ncells, L->plane, L->desc.T_ref, c->stream);
+                                dev[3], dev[4], L->ncells, L->plane, L->desc.T_ref, force_x(L),
+                                force_y(L), c->stream);
   for (int k = 0; k < 5 && e == hipSuccess; ++k)
     if (host[k])
       e = hipMemcpyAsync(host[k], dev[k], size_t(L->ncells) * sizeof(double),
@@ -750,8 +884,34 @@ int kmi_lattice_get_launch(const kmi_lattice* L, int32_t* bx, int32_t* by, int32
   return KMI_OK;
 }
 
+int kmi_lattice_set_temporal(kmi_lattice* L, int32_t tile, int32_t zchunks) {
+  if (!L) return fail(nullptr, KMI_ERR_ARG, "kmi_lattice_set_temporal: lattice is NULL");
+  if (tile != 0 && !tb_eligible(L, tile))
+    return fail(L->ctx, KMI_ERR_UNSUPPORTED,
+                "kmi_lattice_set_temporal: two-step launches need a fully periodic D3Q19 "
+                "BGK/TRT lattice without mask whose nx, ny are multiples of the tile "
+                "(1: 32x8, 2: 64x4, 3: 64x8, not F64)");
+  if (tile != 0 && (zchunks < 1 || zchunks > L->args.nz))
+    return fail(L->ctx, KMI_ERR_ARG, "kmi_lattice_set_temporal: zchunks must be in 1..nz");
+  KMI_HIP(L->ctx, hipSetDevice(L->ctx->device));
+  KMI_HIP(L->ctx, hipStreamSynchronize(L->ctx->stream));
+  drop_graphs(L);
+  L->tb_tile = tile;
+  L->tb_zchunks = tile ? zchunks : 1;
+  return KMI_OK;
+}
+
+int kmi_lattice_get_temporal(const kmi_lattice* L, int32_t* tile, int32_t* zchunks) {
+  if (!L || !tile || !zchunks)
+    return fail(nullptr, KMI_ERR_ARG, "kmi_lattice_get_temporal: NULL argument");
+  *tile = L->tb_tile;
+  *zchunks = L->tb_zchunks;
+  return KMI_OK;
+}
+
 // Times each candidate launch shape × store policy on the lattice itself (graph-replayed
-// steps between HIP events, best of two) and keeps the fastest.  The populations, buffer
+// steps between HIP events, best of two), then the two-step variants where they apply,
+// and keeps the fastest.  The populations, buffer
 // parity and step counter are restored afterwards, so tuning leaves the run unchanged.
 int kmi_lattice_autotune(kmi_lattice* L, int64_t steps_per_trial, double* best_ms_per_step) {
   if (!L) return fail(nullptr, KMI_ERR_ARG, "kmi_lattice_autotune: lattice is NULL");
@@ -761,22 +921,39 @@ int kmi_lattice_autotune(kmi_lattice* L, int64_t steps_per_trial, double* best_m
   const int64_t trial = std::max<int64_t>(kGraphSteps, (steps_per_trial + kGraphSteps - 1) /
                                                            kGraphSteps * kGraphSteps);
   const size_t bytes = size_t(L->plane) * L->Q * L->esize;
+  const size_t gbytes = size_t(L->plane) * 9 * L->gsize;
   void* save_f = nullptr;
   void* save_g = nullptr;
   KMI_HIP(c, hipMalloc(&save_f, bytes));
   hipError_t e = hipMemcpyAsync(save_f, L->f[L->cur], bytes, hipMemcpyDeviceToDevice, c->stream);
   if (e == hipSuccess && L->desc.thermal) {
-    e = hipMalloc(&save_g, bytes);
+    e = hipMalloc(&save_g, gbytes);
     if (e == hipSuccess)
-      e = hipMemcpyAsync(save_g, L->g[L->cur], bytes, hipMemcpyDeviceToDevice, c->stream);
+      e = hipMemcpyAsync(save_g, L->g[L->cur], gbytes, hipMemcpyDeviceToDevice, c->stream);
   }
   const int cur0 = L->cur;
   const int64_t steps0 = L->steps;
-  const int bx0 = L->bx, by0 = L->by, nts0 = L->nts;
+  // Time the current configuration: one warm-up graph, then the best of two timed runs.
+  auto time_current = [&]() -> double {
+    double t = 1e300;
+    drop_graphs(L);
+    if (run_steps(L, kGraphSteps) != KMI_OK) { e = hipErrorUnknown; return t; }
+    for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
+      e = hipEventRecord(L->ev[0], c->stream);
+      if (e == hipSuccess && run_steps(L, trial) != KMI_OK) e = hipErrorUnknown;
+      if (e == hipSuccess) e = hipEventRecord(L->ev[1], c->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(L->ev[1]);
+      float ms = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, L->ev[0], L->ev[1]);
+      t = std::min(t, double(ms) / double(trial));
+    }
+    return t;
+  };
   const int shapes[][2] = {{256, 1}, {128, 1}, {128, 2}, {64, 1}, {64, 2}, {64, 4}, {32, 2},
                            {32, 4}, {32, 8}, {16, 4}, {16, 8}, {16, 16}};
   double best = 1e300;
-  int bb[3] = {bx0, by0, nts0};
+  int bb[5] = {L->bx, L->by, L->nts, L->tb_tile, L->tb_zchunks};
+  L->tb_tile = 0;
   for (const auto& sh : shapes) {
     if (e != hipSuccess) break;
     const int bx = sh[0], by = sh[1];
@@ -784,32 +961,39 @@ int kmi_lattice_autotune(kmi_lattice* L, int64_t steps_per_trial, double* best_m
     if (by > 1 && by > L->desc.ny) continue;
     if ((L->desc.ny + by - 1) / by > 65535) continue;
     for (int nts = 0; nts < 2 && e == hipSuccess; ++nts) {
-      drop_graphs(L);
       set_geometry(L, bx, by, nts);
-      if (run_steps(L, kGraphSteps) != KMI_OK) { e = hipErrorUnknown; break; }
-      double t = 1e300;
-      for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
-        e = hipEventRecord(L->ev[0], c->stream);
-        if (e == hipSuccess && run_steps(L, trial) != KMI_OK) e = hipErrorUnknown;
-        if (e == hipSuccess) e = hipEventRecord(L->ev[1], c->stream);
-        if (e == hipSuccess) e = hipEventSynchronize(L->ev[1]);
-        float ms = 0.f;
-        if (e == hipSuccess) e = hipEventElapsedTime(&ms, L->ev[0], L->ev[1]);
-        t = std::min(t, double(ms) / double(trial));
-      }
+      const double t = time_current();
       if (e == hipSuccess && t < best) {
         best = t;
-        bb[0] = bx;
-        bb[1] = by;
-        bb[2] = nts;
+        bb[0] = bx, bb[1] = by, bb[2] = nts, bb[3] = 0, bb[4] = 1;
+      }
+    }
+  }
+  // Two-step launches (the single-step shape found above serves odd remainders).
+  const int zc_list[] = {1, 2, 3, 4, 6, 8};
+  for (int tile = 1; tile <= 3 && e == hipSuccess; ++tile) {
+    if (!tb_eligible(L, tile)) continue;
+    for (int zc : zc_list) {
+      if (zc > L->args.nz || e != hipSuccess) continue;
+      for (int nts = 0; nts < 2 && e == hipSuccess; ++nts) {
+        set_geometry(L, bb[0], bb[1], nts);
+        L->tb_tile = tile;
+        L->tb_zchunks = zc;
+        const double t = time_current();
+        if (e == hipSuccess && t < best) {
+          best = t;
+          bb[2] = nts, bb[3] = tile, bb[4] = zc;
+        }
       }
     }
   }
   drop_graphs(L);
   set_geometry(L, bb[0], bb[1], bb[2]);
+  L->tb_tile = bb[3];
+  L->tb_zchunks = bb[4];
   hipError_t e2 = hipMemcpyAsync(L->f[cur0], save_f, bytes, hipMemcpyDeviceToDevice, c->stream);
   if (e2 == hipSuccess && save_g)
-    e2 = hipMemcpyAsync(L->g[cur0], save_g, bytes, hipMemcpyDeviceToDevice, c->stream);
+    e2 = hipMemcpyAsync(L->g[cur0], save_g, gbytes, hipMemcpyDeviceToDevice, c->stream);
   if (e2 == hipSuccess) e2 = hipStreamSynchronize(c->stream);
   (void)hipFree(save_f);
   if (save_g) (void)hipFree(save_g);
@@ -851,7 +1035,8 @@ int kmi_measure_copy_bandwidth(kmi_ctx* ctx, int64_t bytes, int32_t reps, double
 
 int kmi_lattice_device_bytes(const kmi_lattice* L, int64_t* bytes) {
   if (!L || !bytes) return fail(nullptr, KMI_ERR_ARG, "kmi_lattice_device_bytes: NULL argument");
-  int64_t b = int64_t(L->plane) * L->Q * int64_t(L->esize) * (L->desc.thermal ? 4 : 2);
+  int64_t b = 2 * int64_t(L->plane) * L->Q * int64_t(L->esize);
+  if (L->desc.thermal) b += 2 * int64_t(L->plane) * 9 * int64_t(L->gsize);
   if (L->mask) b += L->ncells;
   if (L->uprev) b += 3 * L->ncells * int64_t(sizeof(double));
   *bytes = b;

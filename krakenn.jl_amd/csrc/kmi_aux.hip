@@ -101,9 +101,12 @@ __global__ void k_init_temp(void* dst_, const double* __restrict__ T, long long 
 // --- per-cell moments ------------------------------------------------------------------
 // Plain storage: compute_macroscopic_2d/3d and compute_temperature_2d in the storage type
 // (macroscopic.jl:5-61, thermal_2d.jl:113-119).  Shifted: FP64 from the deviations.
+// (fx, fy): Guo body force; u = (Σcf + F/2)/ρ as compute_macroscopic_forced_2d!
+// (macroscopic.jl:84-98).  Zero without forcing (adding zero leaves Σcf unchanged).
 template <int Q, int ST>
 KMI_DEV void cell_moments(const void* f_, const void* g_, long long c, long long plane, double T0,
-                          double& rho, double& ux, double& uy, double& uz, double& T) {
+                          double fx, double fy, double& rho, double& ux, double& uy, double& uz,
+                          double& T) {
   using S = Store<ST>;
   using E = typename S::elem;
   using R = typename S::real;
@@ -114,8 +117,8 @@ KMI_DEV void cell_moments(const void* f_, const void* g_, long long c, long long
     const R r = zeroth_moment<Q>(fp);
     const R inv = R(1) / r;
     rho = r;
-    ux = first_moment<Q, 0>(fp) * inv;
-    uy = first_moment<Q, 1>(fp) * inv;
+    ux = (first_moment<Q, 0>(fp) + R(fx) / R(2)) * inv;
+    uy = (first_moment<Q, 1>(fp) + R(fy) / R(2)) * inv;
     if constexpr (Q == 19) uz = first_moment<Q, 2>(fp) * inv;
     else uz = 0.0;
   } else {
@@ -125,23 +128,24 @@ KMI_DEV void cell_moments(const void* f_, const void* g_, long long c, long long
     });
     const double r = 1.0 + zeroth_moment<Q>(fp);
     rho = r;
-    ux = first_moment<Q, 0>(fp) / r;
-    uy = first_moment<Q, 1>(fp) / r;
+    ux = (first_moment<Q, 0>(fp) + fx / 2.0) / r;
+    uy = (first_moment<Q, 1>(fp) + fy / 2.0) / r;
     if constexpr (Q == 19) uz = first_moment<Q, 2>(fp) / r;
     else uz = 0.0;
   }
   T = 0.0;
   if constexpr (Q == 9) {
     if (g_) {
-      const E* g = static_cast<const E*>(g_);
-      if constexpr (!S::shifted) {
+      using GS = Store<thermal_storage(ST)>;
+      const auto* g = static_cast<const typename GS::elem*>(g_);
+      if constexpr (!GS::shifted) {
         R gp[9];
-        sfor<9>([&](auto qc) { gp[decltype(qc)::value] = S::ld(g, c + decltype(qc)::value * plane); });
+        sfor<9>([&](auto qc) { gp[decltype(qc)::value] = GS::ld(g, c + decltype(qc)::value * plane); });
         T = zeroth_moment<9>(gp);
       } else {
         double gp[9];
         sfor<9>([&](auto qc) {
-          gp[decltype(qc)::value] = double(S::ld(g, c + decltype(qc)::value * plane));
+          gp[decltype(qc)::value] = double(GS::ld(g, c + decltype(qc)::value * plane));
         });
         T = T0 + zeroth_moment<9>(gp);
       }
@@ -151,11 +155,12 @@ KMI_DEV void cell_moments(const void* f_, const void* g_, long long c, long long
 
 template <int Q, int ST>
 __global__ void k_moments(const void* f, const void* g, double* rho, double* ux, double* uy,
-                          double* uz, double* T, long long ncells, long long plane, double T0) {
+                          double* uz, double* T, long long ncells, long long plane, double T0,
+                          double fx, double fy) {
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncells) return;
   double r, u, v, w, t;
-  cell_moments<Q, ST>(f, g, c, plane, T0, r, u, v, w, t);
+  cell_moments<Q, ST>(f, g, c, plane, T0, fx, fy, r, u, v, w, t);
   if (rho) rho[c] = r;
   if (ux) ux[c] = u;
   if (uy) uy[c] = v;
@@ -194,13 +199,14 @@ KMI_DEV void block_reduce(double (&v)[NV], bool is_max, double* partials) {
 template <int Q, int ST>
 __global__ void __launch_bounds__(kRedBlock)
     k_reduce_cells(const void* f, const void* g, const uint8_t* mask, double* uprev,
-                   long long ncells, long long plane, double T0, int kind, double* partials) {
+                   long long ncells, long long plane, double T0, double fx, double fy, int kind,
+                   double* partials) {
   double acc[3] = {0.0, 0.0, 0.0};
   const bool is_max = kind == KMI_RED_MAX_SPEED;
   for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < ncells;
        c += (long long)gridDim.x * blockDim.x) {
     double r, u, v, w, t;
-    cell_moments<Q, ST>(f, g, c, plane, T0, r, u, v, w, t);
+    cell_moments<Q, ST>(f, g, c, plane, T0, fx, fy, r, u, v, w, t);
     switch (kind) {
       case KMI_RED_MASS: acc[0] += r; break;
       case KMI_RED_MOMENTUM: acc[0] += r * u; acc[1] += r * v; acc[2] += r * w; break;
@@ -330,15 +336,15 @@ hipError_t launch_init_eq(int Q, int st, void* dst, const double* rho, const dou
 
 hipError_t launch_init_temp(int st, void* dst, const double* T, long long ncells, long long plane,
                             double T0, hipStream_t s) {
-  KMI_ST_SWITCH(st, hipLaunchKernelGGL((k_init_temp<ST>), grid1(ncells), dim3(256), 0, s, dst, T, ncells, plane, T0));
+  KMI_ST_SWITCH(thermal_storage(st), hipLaunchKernelGGL((k_init_temp<ST>), grid1(ncells), dim3(256), 0, s, dst, T, ncells, plane, T0));
   return hipGetLastError();
 }
 
 hipError_t launch_moments(int Q, int st, const void* f, const void* g, double* rho, double* ux,
                           double* uy, double* uz, double* T, long long ncells, long long plane,
-                          double T0, hipStream_t s) {
-  if (Q == 9) { KMI_ST_SWITCH(st, hipLaunchKernelGGL((k_moments<9, ST>), grid1(ncells), dim3(256), 0, s, f, g, rho, ux, uy, uz, T, ncells, plane, T0)); }
-  else { KMI_ST_SWITCH(st, hipLaunchKernelGGL((k_moments<19, ST>), grid1(ncells), dim3(256), 0, s, f, g, rho, ux, uy, uz, T, ncells, plane, T0)); }
+                          double T0, double fx, double fy, hipStream_t s) {
+  if (Q == 9) { KMI_ST_SWITCH(st, hipLaunchKernelGGL((k_moments<9, ST>), grid1(ncells), dim3(256), 0, s, f, g, rho, ux, uy, uz, T, ncells, plane, T0, fx, fy)); }
+  else { KMI_ST_SWITCH(st, hipLaunchKernelGGL((k_moments<19, ST>), grid1(ncells), dim3(256), 0, s, f, g, rho, ux, uy, uz, T, ncells, plane, T0, fx, fy)); }
   return hipGetLastError();
 }
 
@@ -349,11 +355,11 @@ int reduce_groups(long long ncells) {
 }
 
 hipError_t launch_reduce_cells(int Q, int st, const void* f, const void* g, const uint8_t* mask,
-                               double* uprev, long long ncells, long long plane, double T0,
+                               double* uprev, long long ncells, long long plane, double T0, double fx, double fy,
                                int kind, double* partials, double* out, hipStream_t s) {
   const int nb = reduce_groups(ncells);
-  if (Q == 9) { KMI_ST_SWITCH(st, hipLaunchKernelGGL((k_reduce_cells<9, ST>), dim3(nb), dim3(kRedBlock), 0, s, f, g, mask, uprev, ncells, plane, T0, kind, partials)); }
-  else { KMI_ST_SWITCH(st, hipLaunchKernelGGL((k_reduce_cells<19, ST>), dim3(nb), dim3(kRedBlock), 0, s, f, g, mask, uprev, ncells, plane, T0, kind, partials)); }
+  if (Q == 9) { KMI_ST_SWITCH(st, hipLaunchKernelGGL((k_reduce_cells<9, ST>), dim3(nb), dim3(kRedBlock), 0, s, f, g, mask, uprev, ncells, plane, T0, fx, fy, kind, partials)); }
+  else { KMI_ST_SWITCH(st, hipLaunchKernelGGL((k_reduce_cells<19, ST>), dim3(nb), dim3(kRedBlock), 0, s, f, g, mask, uprev, ncells, plane, T0, fx, fy, kind, partials)); }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || kind == 101) return e;
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, s, partials, nb, 3,
@@ -365,7 +371,7 @@ hipError_t launch_nusselt(int st, const void* g, int nx, int ny, long long plane
                           int face, double Th, double dT, int lo, int hi, double* partials,
                           double* out, hipStream_t s) {
   const int nb = (hi - lo + kRedBlock - 1) / kRedBlock;
-  KMI_ST_SWITCH(st, hipLaunchKernelGGL((k_nusselt<ST>), dim3(nb), dim3(kRedBlock), 0, s, g, nx, ny, plane, T0, face, Th, dT, lo, hi, partials));
+  KMI_ST_SWITCH(thermal_storage(st), hipLaunchKernelGGL((k_nusselt<ST>), dim3(nb), dim3(kRedBlock), 0, s, g, nx, ny, plane, T0, face, Th, dT, lo, hi, partials));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, s, partials, nb, 1, 0, out);

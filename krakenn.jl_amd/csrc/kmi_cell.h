@@ -184,10 +184,14 @@ KMI_DEV void collide_thermal(const R (&gp)[9], R (&go)[9], R ux, R uy, R omT, R 
   }
 }
 
-// BGK + Guo source for a vertical force fy (thermal_2d.jl:443-478, fused_thermal_2d.jl:116-157).
+// BGK + Guo source for a force (fx, fy): collide_guo_2d.jl:5-72 (constant force),
+// thermal_2d.jl:443-478 and fused_thermal_2d.jl:116-157 (fx = 0, buoyancy fy).  ux, uy are
+// the force-shifted velocities.  The reference writes the c_q = −e_x, −e_y terms as
+// (−u)(−F) in collide_guo_2d and as u F in the Boussinesq kernels; negation is exact, so
+// both are the same floating-point value.
 template <bool SH, class R>
-KMI_DEV void collide_guo(const R (&fp)[9], R (&fo)[9], R m0, R rho, R ux, R uy, R fy, R om) {
-  const R fx = R(0);
+KMI_DEV void collide_guo(const R (&fp)[9], R (&fo)[9], R m0, R rho, R ux, R uy, R fx, R fy,
+                         R om) {
   const R usq = ux * ux + uy * uy;
   const R guo = R(1) - om / R(2);
   const R t3 = R(3);
@@ -268,6 +272,37 @@ KMI_DEV void zh_pressure_west(R (&f)[9], R rho, R rho_m1) {  // boundary_2d.jl:2
   f[1] = f4 + R(2.0 / 3.0) * rho * ux;
   f[5] = f8 - R(0.5) * (f3 - f5) + R(1.0 / 6.0) * rho * ux;
   f[8] = f7 + R(0.5) * (f3 - f5) + R(1.0 / 6.0) * rho * ux;
+}
+
+// Per-node Zou–He velocity (u_x, u_y) on north / south / west (boundary_spatial_2d.jl:6-90).
+template <bool SH, class R>
+KMI_DEV void zh_north_profile(R (&f)[9], R ux, R uy) {
+  R s = f[0] + f[1] + f[3] + R(2) * (f[2] + f[5] + f[6]);
+  if constexpr (SH) s = R(1) + s;
+  const R rw = s / (R(1) + uy);
+  f[4] = f[2] - R(2.0 / 3.0) * rw * uy;
+  f[8] = f[6] - R(0.5) * (f[1] - f[3]) + R(0.5) * rw * ux - R(1.0 / 6.0) * rw * uy;
+  f[7] = f[5] + R(0.5) * (f[1] - f[3]) - R(0.5) * rw * ux - R(1.0 / 6.0) * rw * uy;
+}
+
+template <bool SH, class R>
+KMI_DEV void zh_south_profile(R (&f)[9], R ux, R uy) {
+  R s = f[0] + f[1] + f[3] + R(2) * (f[4] + f[7] + f[8]);
+  if constexpr (SH) s = R(1) + s;
+  const R rw = s / (R(1) - uy);
+  f[2] = f[4] + R(2.0 / 3.0) * rw * uy;
+  f[5] = f[7] - R(0.5) * (f[1] - f[3]) + R(0.5) * rw * ux + R(1.0 / 6.0) * rw * uy;
+  f[6] = f[8] + R(0.5) * (f[1] - f[3]) - R(0.5) * rw * ux + R(1.0 / 6.0) * rw * uy;
+}
+
+template <bool SH, class R>
+KMI_DEV void zh_west_profile(R (&f)[9], R ux, R uy) {
+  R s = f[0] + f[2] + f[4] + R(2) * (f[3] + f[6] + f[7]);
+  if constexpr (SH) s = R(1) + s;
+  const R rw = s / (R(1) - ux);
+  f[1] = f[3] + R(2.0 / 3.0) * rw * ux;
+  f[5] = f[7] - R(0.5) * (f[2] - f[4]) + R(1.0 / 6.0) * rw * ux + R(0.5) * rw * uy;
+  f[8] = f[6] + R(0.5) * (f[2] - f[4]) + R(1.0 / 6.0) * rw * ux - R(0.5) * rw * uy;
 }
 
 // ---- D3Q19 Zou–He closures (boundary_3d.jl:39-300, ZouHeFace3D restated 0-based) ------------

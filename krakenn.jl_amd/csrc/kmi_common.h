@@ -6,6 +6,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <type_traits>
 #include <utility>
 
 #include "kraken_mi355x.h"
@@ -174,6 +175,17 @@ inline size_t storage_bytes(int s) {
   return s == KMI_STORE_F64 ? 8 : (s == KMI_STORE_F16S ? 2 : 4);
 }
 
+// Storage of the thermal populations g.  Their deviations from w_q T_ref are O(w_q ΔT) while
+// a step changes them by O(1e-6): binary16's 11-bit significand rounds such increments away
+// and the temperature field freezes, so an F16S lattice keeps g in F32S (108 B per D2Q9
+// cell update instead of 144 B for F32).
+constexpr int thermal_storage(int s) { return s == KMI_STORE_F16S ? KMI_STORE_F32S : s; }
+
+template <class T>
+struct TypeTag {
+  using type = T;
+};
+
 // ---- kernel arguments ---------------------------------------------------------------
 struct FaceArgs {
   int kind;      // kmi_face_kind
@@ -183,6 +195,7 @@ struct FaceArgs {
   double T;
   double Tc[2];  // corner link temperatures (apply_fixed_temp_2d corner rule)
   int skip[2];   // corner slot owned by the adjacent x-face
+  const double* prof[2];  // *_PROFILE faces: per-node (u_x, u_y) or (rho, -) on the device
 };
 
 struct StepArgs {
@@ -200,6 +213,8 @@ struct StepArgs {
   double omega, s_plus, s_minus, s_e, s_eps, s_q, s_nu;
   double omega_T, beta_g, T_ref;
   int coupling;
+  int forcing;      // kmi_forcing
+  double fx, fy;    // body force (KMI_FORCE_GUO)
   FaceArgs face[6];
 };
 

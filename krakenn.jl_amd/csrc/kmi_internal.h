@@ -21,6 +21,11 @@ int hip_fail(kmi_ctx* ctx, hipError_t e, const char* what);
 
 hipError_t launch_step(int q, int st, int coll, bool th, bool bnd, bool nts, const StepArgs& a,
                        dim3 g, dim3 b, hipStream_t s);
+// Two steps per launch on a fully periodic D3Q19 BGK/TRT lattice (kmi_step2.hip); `tile`
+// selects the work-group tile (step2_tile), `zchunks` the z-ranges per column.
+bool step2_tile(int tile, int st, int& TX, int& TY);
+hipError_t launch_step2(int st, int coll, int tile, int zchunks, bool nts, const StepArgs& a,
+                        hipStream_t s);
 hipError_t launch_copy_probe(const void* src, void* dst, long long bytes, hipStream_t s);
 hipError_t launch_to_storage(int Q, int st, const double* src, void* dst, long long ncells,
                              long long plane, double shift, hipStream_t s);
@@ -33,11 +38,12 @@ hipError_t launch_init_temp(int st, void* dst, const double* T, long long ncells
                             double T0, hipStream_t s);
 hipError_t launch_moments(int Q, int st, const void* f, const void* g, double* rho, double* ux,
                           double* uy, double* uz, double* T, long long ncells, long long plane,
-                          double T0, hipStream_t s);
+                          double T0, double fx, double fy, hipStream_t s);
 int reduce_groups(long long ncells);
 hipError_t launch_reduce_cells(int Q, int st, const void* f, const void* g, const uint8_t* mask,
                                double* uprev, long long ncells, long long plane, double T0,
-                               int kind, double* partials, double* out, hipStream_t s);
+                               double fx, double fy, int kind, double* partials, double* out,
+                               hipStream_t s);
 hipError_t launch_nusselt(int st, const void* g, int nx, int ny, long long plane, double T0,
                           int face, double Th, double dT, int lo, int hi, double* partials,
                           double* out, hipStream_t s);

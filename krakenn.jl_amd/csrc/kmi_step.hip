@@ -14,7 +14,7 @@
 //   BGK                collide_stream_2d.jl:105-131, collide_stream_3d.jl:47-97
 //   TRT                fused_trt_2d.jl:101-173, dsl/bricks_3d.jl:168-229
 //   MRT                collide_mrt_2d.jl:43-109
-//   Zou–He             boundary_2d.jl:13-44, 135-185, 206-281
+//   Zou–He             boundary_2d.jl:13-44, 135-185, 206-281; boundary_3d.jl:152-290
 //   thermal DDF        thermal_2d.jl:16-60, 140-290, 421-480; fused_thermal_2d.jl:10-207
 // F32S / F16S store deviations from the rest state and collide them in FP32 (the
 // algebra is identical; only the rounding differs).
@@ -65,7 +65,10 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs a) {
   }
   const int self = i + nx * (j + ny * k);
 
-  auto pull = [&](const E* __restrict__ src, R (&out)[Q], bool ladd) {
+  // tag: TypeTag<Store<…>> of the field pulled (the thermal populations may be stored
+  // differently from the flow populations).
+  auto pull = [&](auto tag, const auto* __restrict__ src, R (&out)[Q], bool ladd) {
+    using SS = typename decltype(tag)::type;
     sfor<Q>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
       constexpr int cx = Lat<Q>::cx[q], cy = Lat<Q>::cy[q], cz = Lat<Q>::cz[q];
@@ -78,7 +81,7 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs a) {
                       (cz > 0 ? vzm : (cz < 0 ? vzp : true));
       const long long addr =
           ok ? (long long)(sx + nx * (sy + ny * sz)) + q * P : (long long)self + o * P;
-      R v = S::ld(src, addr);
+      R v = SS::ld(src, addr);
       if constexpr (BND && q != 0) {
         if (ladd && !ok) {
           // Ladd term of every crossed moving face, x then y then z.
@@ -101,21 +104,32 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs a) {
   };
 
   R fp[Q], fo_r[Q];
-  pull(fi, fp, BND && a.any_moving);
+  pull(TypeTag<S>{}, fi, fp, BND && a.any_moving);
 
   if constexpr (BND && Q == 9) {
+    // Face closures in the configured order.  Profile faces leave their two corner nodes
+    // to the bounce-back pull (boundary_spatial_2d.jl: `if 1 < i < N`).
     for (int n = 0; n < a.n_zh; ++n) {
       const int fid = a.zh_face[n];
       const FaceArgs& F = a.face[fid];
+      const bool prof = F.kind == KMI_FACE_ZOUHE_VELOCITY_PROFILE;
       if (fid == KMI_NORTH && j == ny - 1) {
-        zh_north<SH>(fp, R(F.u[0]));
+        if (!prof) zh_north<SH>(fp, R(F.u[0]));
+        else if (i > 0 && i < nx - 1) zh_north_profile<SH>(fp, R(F.prof[0][i]), R(F.prof[1][i]));
       } else if (fid == KMI_SOUTH && j == 0) {
-        zh_south<SH>(fp, R(F.u[0]));
+        if (!prof) zh_south<SH>(fp, R(F.u[0]));
+        else if (i > 0 && i < nx - 1) zh_south_profile<SH>(fp, R(F.prof[0][i]), R(F.prof[1][i]));
       } else if (fid == KMI_WEST && i == 0) {
         if (F.kind == KMI_FACE_ZOUHE_VELOCITY) zh_west<SH>(fp, R(F.u[0]));
-        else zh_pressure_west<SH>(fp, R(F.rho), R(F.rho - 1.0));
+        else if (F.kind == KMI_FACE_ZOUHE_PRESSURE) zh_pressure_west<SH>(fp, R(F.rho), R(F.rho - 1.0));
+        else if (j > 0 && j < ny - 1) zh_west_profile<SH>(fp, R(F.prof[0][j]), R(F.prof[1][j]));
       } else if (fid == KMI_EAST && i == nx - 1) {
-        zh_pressure_east<SH>(fp, R(F.rho), R(F.rho - 1.0));
+        if (F.kind == KMI_FACE_ZOUHE_PRESSURE) {
+          zh_pressure_east<SH>(fp, R(F.rho), R(F.rho - 1.0));
+        } else if (j > 0 && j < ny - 1) {
+          const double r = F.prof[0][j];  // zou_he_pressure_east_spatial_2d_kernel!
+          zh_pressure_east<SH>(fp, R(r), R(r - 1.0));
+        }
       }
     }
   }
@@ -140,10 +154,13 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs a) {
 
   if constexpr (TH) {
     static_assert(Q == 9, "thermal DDF is D2Q9");
-    const E* __restrict__ gi = static_cast<const E*>(a.gi);
-    E* __restrict__ go = static_cast<E*>(a.go);
+    using GS = Store<thermal_storage(ST)>;
+    using GE = typename GS::elem;
+    static_assert(std::is_same_v<typename GS::real, R> && GS::shifted == SH, "g and f share arithmetic");
+    const GE* __restrict__ gi = static_cast<const GE*>(a.gi);
+    GE* __restrict__ go = static_cast<GE*>(a.go);
     R gp[9], go_r[9];
-    pull(gi, gp, false);
+    pull(TypeTag<GS>{}, gi, gp, false);
     if constexpr (BND) thermal_faces<SH>(gp, a, i, j);
     const R Tl = zeroth_moment<9>(gp);  // T (shifted: T − T0)
     const R m0 = zeroth_moment<9>(fp);
@@ -169,17 +186,31 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs a) {
     if (solid) {
       sfor<9>([&](auto qc) { fo_r[decltype(qc)::value] = fp[Lat<9>::opp[decltype(qc)::value]]; });
     } else {
-      collide_guo<SH>(fp, fo_r, m0, rho, uxf, uyf, fy, R(a.omega));
+      collide_guo<SH>(fp, fo_r, m0, rho, uxf, uyf, fx, fy, R(a.omega));
     }
     sfor<9>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
-      S::template st<NTS>(go, self + q * P, go_r[q]);
+      GS::template st<NTS>(go, self + q * P, go_r[q]);
     });
   } else {
     if (solid) {
       sfor<Q>([&](auto qc) { fo_r[decltype(qc)::value] = fp[Lat<Q>::opp[decltype(qc)::value]]; });
     } else if constexpr (COLL == KMI_BGK) {
-      collide_bgk<Q, SH>(fp, fo_r, R(a.omega));
+      bool forced = false;
+      if constexpr (Q == 9 && BND) {
+        if (a.forcing == KMI_FORCE_GUO) {
+          // collide_guo_2d_kernel! (collide_guo_2d.jl:5-72): u = (Σcf + F/2)/ρ
+          forced = true;
+          const R fx = R(a.fx), fy = R(a.fy);
+          const R m0 = zeroth_moment<9>(fp);
+          const R rho = SH ? R(1) + m0 : m0;
+          const R inv = R(1) / rho;
+          const R ux = (first_moment<9, 0>(fp) + fx / R(2)) * inv;
+          const R uy = (first_moment<9, 1>(fp) + fy / R(2)) * inv;
+          collide_guo<SH>(fp, fo_r, m0, rho, ux, uy, fx, fy, R(a.omega));
+        }
+      }
+      if (!forced) collide_bgk<Q, SH>(fp, fo_r, R(a.omega));
     } else if constexpr (COLL == KMI_TRT) {
       collide_trt<Q, SH>(fp, fo_r, R(a.s_plus), R(a.s_minus));
     } else {

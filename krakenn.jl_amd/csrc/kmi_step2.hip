@@ -1,0 +1,199 @@
+// kmi_step2.hip — two time steps per launch for fully periodic D3Q19 lattices.
+//
+// A single-step kernel reads and writes every population once per step: 2·19·sizeof(elem)
+// bytes of HBM traffic per cell update, and no kernel of that shape can beat the device's
+// copy bandwidth.  This kernel advances TWO steps per pass: each work-group owns a column
+// of TX × TY cells and a range of z-planes, marches up z, and keeps the intermediate
+// (t+1) populations in LDS, so HBM sees one read and one write per population per two
+// steps.  Cost: the t+1 step is also evaluated on a one-cell x–y halo ((TX+2)(TY+2)
+// cells per plane) and on the two planes below and above the z-range.
+//
+// Per marched plane s:
+//   phase 1  t+1 populations of plane s on the (TX+2)×(TY+2) extended tile: pull from the
+//            t field in HBM (periodic wrap), collide, write to LDS;
+//   phase 2  t+2 populations of plane s−1 on the TX×TY tile: pull from LDS — c_z = +1
+//            populations from plane s−2, c_z = 0 from s−1, c_z = −1 from s — collide and
+//            store to HBM.
+// LDS holds c_z = 0 populations of 2 planes (9 each), c_z = +1 of 3 planes (5 each) and
+// c_z = −1 of 1 plane (5): 38 values per extended cell.
+//
+// The arithmetic per cell and step is the single-step kernel's (same collide functions,
+// -ffp-contract=off), and the t+1 values pass through the storage rounding before they
+// enter LDS, so the result is bit-identical to two single-step launches in every storage.
+#include "kmi_cell.h"
+
+namespace kmi {
+
+namespace tb {
+// D3Q19 populations grouped by c_z (Kraken ordering, 0-based).
+constexpr int kZ0[9] = {0, 1, 2, 3, 4, 7, 8, 9, 10};
+constexpr int kZp[5] = {5, 11, 12, 15, 16};
+constexpr int kZm[5] = {6, 13, 14, 17, 18};
+
+// Index of q inside its c_z group.
+constexpr int group_index(int q) {
+  for (int n = 0; n < 9; ++n)
+    if (kZ0[n] == q) return n;
+  for (int n = 0; n < 5; ++n)
+    if (kZp[n] == q || kZm[n] == q) return n;
+  return -1;
+}
+
+KMI_DEV int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+}  // namespace tb
+
+// Value a population takes after a store to and a load from storage ST.
+template <int ST, class R>
+KMI_DEV R storage_round(R v) {
+  if constexpr (ST == KMI_STORE_F16S) return float(_Float16(v * kF16Scale)) * kF16InvScale;
+  else return v;
+}
+
+template <int ST, int COLL, class R>
+KMI_DEV void collide19(const R (&fp)[19], R (&fo)[19], const StepArgs& a) {
+  constexpr bool SH = Store<ST>::shifted;
+  if constexpr (COLL == KMI_TRT) collide_trt<19, SH>(fp, fo, R(a.s_plus), R(a.s_minus));
+  else collide_bgk<19, SH>(fp, fo, R(a.omega));
+}
+
+// Grid: one work-group per (column tile, z-range), padded to a multiple of 8.  Work-group
+// b runs on XCD b mod 8; it takes tile (b mod 8)·per_xcd + b/8, so each XCD works through a
+// contiguous band of neighbouring tiles and their shared halo rows meet in its L2.
+template <int ST, int COLL, int TX, int TY, bool NTS>
+__global__ void __launch_bounds__(TX * TY)
+    k_step2_d3q19(const StepArgs a, int zlen, int ntx, int nty, int ntiles, int per_xcd) {
+  using S = Store<ST>;
+  using E = typename S::elem;
+  using R = typename S::real;
+  constexpr int EX = TX + 2, EY = TY + 2, NE = EX * EY, NT = TX * TY;
+  __shared__ R lds[38 * NE];
+  R* const A = lds;            // c_z = 0:  [2 planes][9][NE]
+  R* const B = lds + 18 * NE;  // c_z = +1: [3 planes][5][NE]
+  R* const C = lds + 33 * NE;  // c_z = −1: [5][NE]
+
+  const int t = int(blockIdx.x % 8u) * per_xcd + int(blockIdx.x / 8u);
+  if (t >= ntiles) return;  // padding groups (uniform per group, before any barrier)
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const long long P = a.plane;
+  const int tx = t % ntx, ty = (t / ntx) % nty, zc = t / (ntx * nty);
+  const int x0 = tx * TX, y0 = ty * TY;
+  const int z0 = zc * zlen, z1 = min(z0 + zlen, nz);
+  const E* __restrict__ fi = static_cast<const E*>(a.fi);
+  E* __restrict__ fo = static_cast<E*>(a.fo);
+  const int tid = threadIdx.x;
+
+  for (int s = z0 - 1; s <= z1; ++s) {
+    const int u = s + 3;  // non-negative plane counter for the LDS ring slots
+    // ---- phase 1: t+1 on plane s, extended tile ----
+    {
+      const int zs = tb::wrap(s, nz);
+      const int zm = tb::wrap(zs - 1, nz), zp = tb::wrap(zs + 1, nz);
+      R* const Aw = A + (u & 1) * 9 * NE;
+      R* const Bw = B + (u % 3) * 5 * NE;
+      for (int e = tid; e < NE; e += NT) {
+        const int ex = e % EX, ey = e / EX;
+        const int gx = tb::wrap(x0 - 1 + ex, nx), gy = tb::wrap(y0 - 1 + ey, ny);
+        const int xm = tb::wrap(gx - 1, nx), xp = tb::wrap(gx + 1, nx);
+        const int ym = tb::wrap(gy - 1, ny), yp = tb::wrap(gy + 1, ny);
+        R fp[19], fn[19];
+        sfor<19>([&](auto qc) {
+          constexpr int q = decltype(qc)::value;
+          constexpr int cx = Lat<19>::cx[q], cy = Lat<19>::cy[q], cz = Lat<19>::cz[q];
+          const int sx = cx > 0 ? xm : (cx < 0 ? xp : gx);
+          const int sy = cy > 0 ? ym : (cy < 0 ? yp : gy);
+          const int sz = cz > 0 ? zm : (cz < 0 ? zp : zs);
+          fp[q] = S::ld(fi, (long long)(sx + nx * (sy + ny * sz)) + q * P);
+        });
+        collide19<ST, COLL>(fp, fn, a);
+        sfor<19>([&](auto qc) {
+          constexpr int q = decltype(qc)::value;
+          constexpr int cz = Lat<19>::cz[q];
+          constexpr int n = tb::group_index(q);
+          const R v = storage_round<ST>(fn[q]);
+          if constexpr (cz == 0) Aw[n * NE + e] = v;
+          else if constexpr (cz > 0) Bw[n * NE + e] = v;
+          else C[n * NE + e] = v;
+        });
+      }
+    }
+    __syncthreads();
+    // ---- phase 2: t+2 on plane s−1, interior tile ----
+    if (s >= z0 + 1) {
+      const R* const Ar = A + ((u - 1) & 1) * 9 * NE;
+      const R* const Br = B + ((u - 2) % 3) * 5 * NE;
+      const int ix = tid % TX, iy = tid / TX;
+      R fp[19], fn[19];
+      sfor<19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int cx = Lat<19>::cx[q], cy = Lat<19>::cy[q], cz = Lat<19>::cz[q];
+        constexpr int n = tb::group_index(q);
+        const int e = (iy + 1 - cy) * EX + (ix + 1 - cx);
+        if constexpr (cz == 0) fp[q] = Ar[n * NE + e];
+        else if constexpr (cz > 0) fp[q] = Br[n * NE + e];
+        else fp[q] = C[n * NE + e];
+      });
+      collide19<ST, COLL>(fp, fn, a);
+      const long long cell = (long long)(x0 + ix) + nx * ((long long)(y0 + iy) + (long long)ny * (s - 1));
+      sfor<19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        S::template st<NTS>(fo, cell + q * P, fn[q]);
+      });
+    }
+    __syncthreads();
+  }
+}
+
+// ---- dispatch ---------------------------------------------------------------------------
+// Tile variants: 1 = 32×8, 2 = 64×4, 3 = 64×8 (the last not for F64: its LDS would exceed
+// 160 KB).
+bool step2_tile(int tile, int st, int& TX, int& TY) {
+  switch (tile) {
+    case 1: TX = 32; TY = 8; return true;
+    case 2: TX = 64; TY = 4; return true;
+    case 3: TX = 64; TY = 8; return st != KMI_STORE_F64;
+    default: return false;
+  }
+}
+
+template <int ST, int COLL, int TX, int TY>
+static hipError_t launch2(bool nts, const StepArgs& a, int zchunks, hipStream_t s) {
+  const int ntx = a.nx / TX, nty = a.ny / TY;
+  const int zlen = (a.nz + zchunks - 1) / zchunks;
+  const int nzc = (a.nz + zlen - 1) / zlen;
+  const int ntiles = ntx * nty * nzc;
+  const int per_xcd = (ntiles + 7) / 8;
+  const dim3 g(static_cast<unsigned>(8 * per_xcd)), b(static_cast<unsigned>(TX * TY));
+  if (nts) hipLaunchKernelGGL((k_step2_d3q19<ST, COLL, TX, TY, true>), g, b, 0, s, a, zlen, ntx, nty, ntiles, per_xcd);
+  else hipLaunchKernelGGL((k_step2_d3q19<ST, COLL, TX, TY, false>), g, b, 0, s, a, zlen, ntx, nty, ntiles, per_xcd);
+  return hipGetLastError();
+}
+
+template <int ST, int COLL>
+static hipError_t launch2_tile(int tile, bool nts, const StepArgs& a, int zchunks, hipStream_t s) {
+  switch (tile) {
+    case 1: return launch2<ST, COLL, 32, 8>(nts, a, zchunks, s);
+    case 2: return launch2<ST, COLL, 64, 4>(nts, a, zchunks, s);
+    default:
+      if constexpr (ST != KMI_STORE_F64) return launch2<ST, COLL, 64, 8>(nts, a, zchunks, s);
+      else return hipErrorInvalidValue;
+  }
+}
+
+template <int ST>
+static hipError_t launch2_coll(int coll, int tile, bool nts, const StepArgs& a, int zchunks,
+                               hipStream_t s) {
+  return coll == KMI_TRT ? launch2_tile<ST, KMI_TRT>(tile, nts, a, zchunks, s)
+                         : launch2_tile<ST, KMI_BGK>(tile, nts, a, zchunks, s);
+}
+
+hipError_t launch_step2(int st, int coll, int tile, int zchunks, bool nts, const StepArgs& a,
+                        hipStream_t s) {
+  switch (st) {
+    case KMI_STORE_F64: return launch2_coll<KMI_STORE_F64>(coll, tile, nts, a, zchunks, s);
+    case KMI_STORE_F32: return launch2_coll<KMI_STORE_F32>(coll, tile, nts, a, zchunks, s);
+    case KMI_STORE_F32S: return launch2_coll<KMI_STORE_F32S>(coll, tile, nts, a, zchunks, s);
+    default: return launch2_coll<KMI_STORE_F16S>(coll, tile, nts, a, zchunks, s);
+  }
+}
+
+}  // namespace kmi

@@ -30,9 +30,11 @@ struct KmiDesc
     omega_T::Float64; beta_g::Float64; T_ref::Float64
     coupling::Int32; n_bc_order::Int32; bc_order::NTuple{6,Int32}
     faces::NTuple{6,KmiFace}
+    forcing::Int32; force::NTuple{3,Float64}
 end
 
 const FACE_PERIODIC, FACE_WALL, FACE_MOVING_WALL, FACE_ZOUHE_VELOCITY, FACE_ZOUHE_PRESSURE = 0, 1, 2, 3, 4
+const FACE_ZOUHE_VELOCITY_PROFILE, FACE_ZOUHE_PRESSURE_PROFILE = 5, 6
 const STORAGE = Dict(Float64 => Int32(0), Float32 => Int32(1))
 
 lasterr(ctx) = unsafe_string(ccall((:kmi_last_error, libkmi), Cstring, (Ptr{Cvoid},), ctx))
@@ -111,7 +113,8 @@ function desc_2d(Nx, Ny, ν, T; faces)
     end
     sp = 1 / (3ν + 0.5); sm = 1 / ((3 / 16) / (3ν) + 0.5)
     KmiDesc(0, STORAGE[T], 0, 0, Nx, Ny, 1, 1 / (3ν + 0.5), sp, sm, 1.4, 1.4, 1.2,
-            1 / (3ν + 0.5), 1 / (3ν + 0.5), 0.0, 0.0, 0, 0, (0, 0, 0, 0, 0, 0), Tuple(fs))
+            1 / (3ν + 0.5), 1 / (3ν + 0.5), 0.0, 0.0, 0, 0, (0, 0, 0, 0, 0, 0), Tuple(fs),
+            0, (0.0, 0.0, 0.0))
 end
 
 # ---- driver methods for the backend (src/drivers/basic.jl:112-144) ------------------------

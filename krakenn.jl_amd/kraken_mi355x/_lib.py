@@ -18,7 +18,9 @@ KMI_OK, KMI_ERR_ARG, KMI_ERR_DEVICE, KMI_ERR_STATE, KMI_ERR_NOMEM, KMI_ERR_UNSUP
 D2Q9, D3Q19 = 0, 1
 STORE_F64, STORE_F32, STORE_F32S, STORE_F16S = 0, 1, 2, 3
 BGK, TRT, MRT = 0, 1, 2
-FACE_PERIODIC, FACE_WALL, FACE_MOVING_WALL, FACE_ZOUHE_VELOCITY, FACE_ZOUHE_PRESSURE = range(5)
+(FACE_PERIODIC, FACE_WALL, FACE_MOVING_WALL, FACE_ZOUHE_VELOCITY, FACE_ZOUHE_PRESSURE,
+ FACE_ZOUHE_VELOCITY_PROFILE, FACE_ZOUHE_PRESSURE_PROFILE) = range(7)
+FORCE_NONE, FORCE_GUO = 0, 1
 WEST, EAST, SOUTH, NORTH, BOTTOM, TOP = range(6)
 THERMAL_ADIABATIC, THERMAL_DIRICHLET = 0, 1
 COUPLING_SPLIT, COUPLING_FUSED = 0, 1
@@ -64,6 +66,8 @@ class KmiDesc(ctypes.Structure):
         ("n_bc_order", c_int32),
         ("bc_order", c_int32 * 6),
         ("faces", KmiFace * 6),
+        ("forcing", c_int32),
+        ("force", c_double * 3),
     ]
 
 
@@ -98,6 +102,8 @@ SIGNATURES = {
     "kmi_lattice_create": (c_int, [c_void_p, POINTER(KmiDesc), POINTER(c_void_p)]),
     "kmi_lattice_destroy": (c_int, [c_void_p]),
     "kmi_set_mask": (c_int, [c_void_p, POINTER(c_uint8)]),
+    "kmi_set_face_profile": (c_int, [c_void_p, c_int32, POINTER(c_double), POINTER(c_double)]),
+    "kmi_set_face": (c_int, [c_void_p, c_int32, POINTER(KmiFace)]),
     "kmi_upload": (c_int, [c_void_p, c_int32, c_void_p, c_int32]),
     "kmi_download": (c_int, [c_void_p, c_int32, c_void_p, c_int32]),
     "kmi_init_equilibrium": (c_int, [c_void_p] + [POINTER(c_double)] * 4),
@@ -111,6 +117,8 @@ SIGNATURES = {
     "kmi_lattice_device_bytes": (c_int, [c_void_p, POINTER(c_int64)]),
     "kmi_lattice_set_launch": (c_int, [c_void_p, c_int32, c_int32, c_int32]),
     "kmi_lattice_get_launch": (c_int, [c_void_p, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)]),
+    "kmi_lattice_set_temporal": (c_int, [c_void_p, c_int32, c_int32]),
+    "kmi_lattice_get_temporal": (c_int, [c_void_p, POINTER(c_int32), POINTER(c_int32)]),
     "kmi_lattice_autotune": (c_int, [c_void_p, c_int64, POINTER(c_double)]),
     "kmi_measure_copy_bandwidth": (c_int, [c_void_p, c_int64, c_int32, POINTER(c_double)]),
     "kmi_batch_create": (c_int, [c_void_p, POINTER(KmiBatchDesc), c_int32, POINTER(c_int32),

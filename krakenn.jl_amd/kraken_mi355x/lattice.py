@@ -203,8 +203,20 @@ class Lattice:
         self._chk(L.load().kmi_lattice_get_launch(self._h, byref(bx), byref(by), byref(nt)))
         return {"bx": bx.value, "by": by.value, "nontemporal": bool(nt.value)}
 
+    def set_temporal(self, tile, zchunks=1):
+        """Two steps per launch (tile 1: 32×8, 2: 64×4, 3: 64×8 cells; 0: off) on a fully
+        periodic D3Q19 BGK/TRT lattice; bit-identical to single steps."""
+        self._chk(L.load().kmi_lattice_set_temporal(self._h, int(tile), int(zchunks)))
+
+    @property
+    def temporal(self):
+        t, z = ctypes.c_int32(), ctypes.c_int32()
+        self._chk(L.load().kmi_lattice_get_temporal(self._h, byref(t), byref(z)))
+        return {"tile": t.value, "zchunks": z.value}
+
     def autotune(self, steps_per_trial=64):
-        """Pick the fastest launch shape / store policy on this lattice; state is restored."""
+        """Pick the fastest launch shape / store policy / two-step variant on this lattice;
+        state is restored."""
         ms = c_double()
         self._chk(L.load().kmi_lattice_autotune(self._h, int(steps_per_trial), byref(ms)))
         return ms.value

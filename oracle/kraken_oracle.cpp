@@ -285,6 +285,37 @@ static void guo9(const T* fp, T* fo, T rho, T ux, T uy, T fy, T om) {
   fo[8] = fp[8] - om * (fp[8] - feq<T>(1.0 / 36.0, rho, ux - uy, usq)) + guo * Sq;
 }
 
+// collide_guo_2d_kernel! per-cell body (collide_guo_2d.jl:21-70), constant force (fx, fy);
+// written with the reference's (−u)(−F) factors for the c = −e_x, −e_y populations.
+template <class T>
+static void guo9_force(const T* fp, T* fo, T rho, T ux, T uy, T fx, T fy, T om) {
+  T usq = ux * ux + uy * uy;
+  T pref = T(1) - om / T(2);
+  T Sq;
+  Sq = T(4.0 / 9.0) * ((-ux) * fx + (-uy) * fy) * T(3);
+  fo[0] = fp[0] - om * (fp[0] - T(4.0 / 9.0) * rho * (T(1) - T(1.5) * usq)) + pref * Sq;
+  Sq = T(1.0 / 9.0) * ((T(1) - ux) * fx + (-uy) * fy) * T(3) + T(1.0 / 9.0) * (ux) * (fx) * T(9);
+  fo[1] = fp[1] - om * (fp[1] - feq<T>(1.0 / 9.0, rho, ux, usq)) + pref * Sq;
+  Sq = T(1.0 / 9.0) * ((-ux) * fx + (T(1) - uy) * fy) * T(3) + T(1.0 / 9.0) * (uy) * (fy) * T(9);
+  fo[2] = fp[2] - om * (fp[2] - feq<T>(1.0 / 9.0, rho, uy, usq)) + pref * Sq;
+  Sq = T(1.0 / 9.0) * ((-T(1) - ux) * fx + (-uy) * fy) * T(3) + T(1.0 / 9.0) * (-ux) * (-fx) * T(9);
+  fo[3] = fp[3] - om * (fp[3] - feq<T>(1.0 / 9.0, rho, -ux, usq)) + pref * Sq;
+  Sq = T(1.0 / 9.0) * ((-ux) * fx + (-T(1) - uy) * fy) * T(3) + T(1.0 / 9.0) * (-uy) * (-fy) * T(9);
+  fo[4] = fp[4] - om * (fp[4] - feq<T>(1.0 / 9.0, rho, -uy, usq)) + pref * Sq;
+  Sq = T(1.0 / 36.0) * ((T(1) - ux) * fx + (T(1) - uy) * fy) * T(3) +
+       T(1.0 / 36.0) * (ux + uy) * (fx + fy) * T(9);
+  fo[5] = fp[5] - om * (fp[5] - feq<T>(1.0 / 36.0, rho, ux + uy, usq)) + pref * Sq;
+  Sq = T(1.0 / 36.0) * ((-T(1) - ux) * fx + (T(1) - uy) * fy) * T(3) +
+       T(1.0 / 36.0) * (-ux + uy) * (-fx + fy) * T(9);
+  fo[6] = fp[6] - om * (fp[6] - feq<T>(1.0 / 36.0, rho, -ux + uy, usq)) + pref * Sq;
+  Sq = T(1.0 / 36.0) * ((-T(1) - ux) * fx + (-T(1) - uy) * fy) * T(3) +
+       T(1.0 / 36.0) * (-ux - uy) * (-fx - fy) * T(9);
+  fo[7] = fp[7] - om * (fp[7] - feq<T>(1.0 / 36.0, rho, -ux - uy, usq)) + pref * Sq;
+  Sq = T(1.0 / 36.0) * ((T(1) - ux) * fx + (-T(1) - uy) * fy) * T(3) +
+       T(1.0 / 36.0) * (ux - uy) * (fx - fy) * T(9);
+  fo[8] = fp[8] - om * (fp[8] - feq<T>(1.0 / 36.0, rho, ux - uy, usq)) + pref * Sq;
+}
+
 // ----------------------------------------------------------------------------------
 // Reference kernels restated one for one (whole-array operations)
 // ----------------------------------------------------------------------------------
@@ -514,6 +545,73 @@ static void zou_he_pressure_west(T* f, T rho_in, int64_t nx, int64_t ny) {
     f[c + P] = f4 + T(2.0 / 3.0) * rho_in * ux;
     f[c + 5 * P] = f8 - T(0.5) * (f3 - f5) + T(1.0 / 6.0) * rho_in * ux;
     f[c + 8 * P] = f7 + T(0.5) * (f3 - f5) + T(1.0 / 6.0) * rho_in * ux;
+  }
+}
+
+// Spatial Zou–He closures, per-node velocity / density arrays, corner nodes skipped
+// (boundary_spatial_2d.jl).  f1..f9 of the reference are f[c + (q−1)P].
+// zou_he_velocity_north_spatial_2d_kernel! — boundary_spatial_2d.jl:6-24
+template <class T>
+static void zou_he_north_spatial(T* f, const double* uxa, const double* uya, int64_t nx, int64_t ny) {
+  const size_t P = size_t(nx * ny);
+  for (int64_t i = 1; i < nx - 1; ++i) {
+    size_t c = I2(i, ny - 1, nx);
+    T u_x = T(uxa[i]), u_y = T(uya[i]);
+    T f1 = f[c], f2 = f[c + P], f3 = f[c + 2 * P], f4 = f[c + 3 * P], f6 = f[c + 5 * P],
+      f7 = f[c + 6 * P];
+    T rho_wall = (f1 + f2 + f4 + T(2) * (f3 + f6 + f7)) / (T(1) + u_y);
+    f[c + 4 * P] = f3 - T(2.0 / 3.0) * rho_wall * u_y;
+    f[c + 8 * P] = f7 - T(0.5) * (f2 - f4) + T(0.5) * rho_wall * u_x - T(1.0 / 6.0) * rho_wall * u_y;
+    f[c + 7 * P] = f6 + T(0.5) * (f2 - f4) - T(0.5) * rho_wall * u_x - T(1.0 / 6.0) * rho_wall * u_y;
+  }
+}
+
+// zou_he_velocity_south_spatial_2d_kernel! — boundary_spatial_2d.jl:40-57
+template <class T>
+static void zou_he_south_spatial(T* f, const double* uxa, const double* uya, int64_t nx, int64_t ny) {
+  const size_t P = size_t(nx * ny);
+  for (int64_t i = 1; i < nx - 1; ++i) {
+    size_t c = I2(i, 0, nx);
+    T u_x = T(uxa[i]), u_y = T(uya[i]);
+    T f1 = f[c], f2 = f[c + P], f4 = f[c + 3 * P], f5 = f[c + 4 * P], f8 = f[c + 7 * P],
+      f9 = f[c + 8 * P];
+    T rho_wall = (f1 + f2 + f4 + T(2) * (f5 + f8 + f9)) / (T(1) - u_y);
+    f[c + 2 * P] = f5 + T(2.0 / 3.0) * rho_wall * u_y;
+    f[c + 5 * P] = f8 - T(0.5) * (f2 - f4) + T(0.5) * rho_wall * u_x + T(1.0 / 6.0) * rho_wall * u_y;
+    f[c + 6 * P] = f9 + T(0.5) * (f2 - f4) - T(0.5) * rho_wall * u_x + T(1.0 / 6.0) * rho_wall * u_y;
+  }
+  (void)ny;
+}
+
+// zou_he_velocity_west_spatial_2d_kernel! — boundary_spatial_2d.jl:72-89
+template <class T>
+static void zou_he_west_spatial(T* f, const double* uxa, const double* uya, int64_t nx, int64_t ny) {
+  const size_t P = size_t(nx * ny);
+  for (int64_t j = 1; j < ny - 1; ++j) {
+    size_t c = I2(0, j, nx);
+    T u_x = T(uxa[j]), u_y = T(uya[j]);
+    T f1 = f[c], f3 = f[c + 2 * P], f4 = f[c + 3 * P], f5 = f[c + 4 * P], f7 = f[c + 6 * P],
+      f8 = f[c + 7 * P];
+    T rho_wall = (f1 + f3 + f5 + T(2) * (f4 + f7 + f8)) / (T(1) - u_x);
+    f[c + P] = f4 + T(2.0 / 3.0) * rho_wall * u_x;
+    f[c + 5 * P] = f8 - T(0.5) * (f3 - f5) + T(1.0 / 6.0) * rho_wall * u_x + T(0.5) * rho_wall * u_y;
+    f[c + 8 * P] = f7 + T(0.5) * (f3 - f5) + T(1.0 / 6.0) * rho_wall * u_x - T(0.5) * rho_wall * u_y;
+  }
+}
+
+// zou_he_pressure_east_spatial_2d_kernel! — boundary_spatial_2d.jl:104-121
+template <class T>
+static void zou_he_pressure_east_spatial(T* f, const double* rhoa, int64_t nx, int64_t ny) {
+  const size_t P = size_t(nx * ny);
+  for (int64_t j = 1; j < ny - 1; ++j) {
+    size_t c = I2(nx - 1, j, nx);
+    T rho_out = T(rhoa[j]);
+    T f1 = f[c], f2 = f[c + P], f3 = f[c + 2 * P], f5 = f[c + 4 * P], f6 = f[c + 5 * P],
+      f9 = f[c + 8 * P];
+    T ux = -T(1) + (f1 + f3 + f5 + T(2) * (f2 + f6 + f9)) / rho_out;
+    f[c + 3 * P] = f2 - T(2.0 / 3.0) * rho_out * ux;
+    f[c + 6 * P] = f9 - T(0.5) * (f3 - f5) - T(1.0 / 6.0) * rho_out * ux;
+    f[c + 7 * P] = f6 + T(0.5) * (f3 - f5) - T(1.0 / 6.0) * rho_out * ux;
   }
 }
 
@@ -844,6 +942,44 @@ static void collide_boussinesq_2d(T* f, const T* Temp, const uint8_t* solid, int
   }
 }
 
+// collide_guo_2d_kernel! — collide_guo_2d.jl:5-72 (constant body force)
+template <class T>
+static void collide_guo_2d(T* f, const uint8_t* solid, int64_t nx, int64_t ny, T om, T Fx, T Fy) {
+  const size_t P = size_t(nx * ny);
+#pragma omp parallel for
+  for (int64_t c = 0; c < int64_t(P); ++c) {
+    T fp[9], fo[9];
+    for (int q = 0; q < 9; ++q) fp[q] = f[c + q * P];
+    if (solid && solid[c]) {
+      for (int q = 0; q < 9; ++q) fo[q] = fp[OPP9[q]];
+    } else {
+      T rho = fp[0] + fp[1] + fp[2] + fp[3] + fp[4] + fp[5] + fp[6] + fp[7] + fp[8];
+      T inv = T(1) / rho;
+      T ux = ((fp[1] - fp[3] + fp[5] - fp[6] - fp[7] + fp[8]) + Fx / T(2)) * inv;
+      T uy = ((fp[2] - fp[4] + fp[5] + fp[6] - fp[7] - fp[8]) + Fy / T(2)) * inv;
+      guo9_force(fp, fo, rho, ux, uy, Fx, Fy, om);
+    }
+    for (int q = 0; q < 9; ++q) f[c + q * P] = fo[q];
+  }
+}
+
+// compute_macroscopic_forced_2d_kernel! — macroscopic.jl:84-98
+template <class T>
+static void macroscopic_forced_2d(T* rho, T* ux, T* uy, const T* f, int64_t nx, int64_t ny, T Fx,
+                                  T Fy) {
+  const size_t P = size_t(nx * ny);
+#pragma omp parallel for
+  for (int64_t c = 0; c < int64_t(P); ++c) {
+    T fp[9];
+    for (int q = 0; q < 9; ++q) fp[q] = f[c + q * P];
+    T r = fp[0] + fp[1] + fp[2] + fp[3] + fp[4] + fp[5] + fp[6] + fp[7] + fp[8];
+    T inv = T(1) / r;
+    rho[c] = r;
+    ux[c] = ((fp[1] - fp[3] + fp[5] - fp[6] - fp[7] + fp[8]) + Fx / T(2)) * inv;
+    uy[c] = ((fp[2] - fp[4] + fp[5] + fp[6] - fp[7] - fp[8]) + Fy / T(2)) * inv;
+  }
+}
+
 // fused_natconv_step_kernel! — fused_thermal_2d.jl:178-207 (Dirichlet west/east, adiabatic
 // south/north, no solids).
 template <class T>
@@ -960,8 +1096,13 @@ static void pull_generic(const kmi_desc& d, T* fo, const T* fi, bool ladd = true
       }
 }
 
+// Per-node arrays of the profile faces, indexed [face][0: u_x or rho, 1: u_y].
+struct Profiles {
+  const double* p[4][2] = {};
+};
+
 template <class T>
-static void zou_he_faces(const kmi_desc& d, T* f) {
+static void zou_he_faces(const kmi_desc& d, T* f, const Profiles* pr = nullptr) {
   int order[6] = {0, 1, 2, 3, 4, 5};
   int n = 6;
   if (d.n_bc_order > 0) {
@@ -982,6 +1123,13 @@ static void zou_he_faces(const kmi_desc& d, T* f) {
         zou_he_pressure_3d(f, fid, T(F.rho), d.nx, d.ny, d.nz);
       continue;
     }
+    if (F.kind == KMI_FACE_ZOUHE_VELOCITY_PROFILE && pr) {
+      if (fid == KMI_NORTH) zou_he_north_spatial(f, pr->p[fid][0], pr->p[fid][1], d.nx, d.ny);
+      else if (fid == KMI_SOUTH) zou_he_south_spatial(f, pr->p[fid][0], pr->p[fid][1], d.nx, d.ny);
+      else if (fid == KMI_WEST) zou_he_west_spatial(f, pr->p[fid][0], pr->p[fid][1], d.nx, d.ny);
+    } else if (F.kind == KMI_FACE_ZOUHE_PRESSURE_PROFILE && pr) {
+      if (fid == KMI_EAST) zou_he_pressure_east_spatial(f, pr->p[fid][0], d.nx, d.ny);
+    }
     if (F.kind == KMI_FACE_ZOUHE_VELOCITY) {
       if (fid == KMI_NORTH) zou_he_north(f, T(F.u[0]), d.nx, d.ny);
       else if (fid == KMI_SOUTH) zou_he_south(f, T(F.u[0]), d.nx, d.ny);
@@ -995,11 +1143,11 @@ static void zou_he_faces(const kmi_desc& d, T* f) {
 
 template <class T>
 static void generic_step(const kmi_desc& d, const uint8_t* solid, T*& f, T*& f2, T*& g, T*& g2,
-                         std::vector<T>& scratch) {
+                         std::vector<T>& scratch, const Profiles* pr = nullptr) {
   const int64_t nx = d.nx, ny = d.ny, nz = d.stencil == KMI_D2Q9 ? 1 : d.nz;
   const size_t P = size_t(nx * ny * nz);
   pull_generic(d, f2, f);
-  zou_he_faces(d, f2);
+  zou_he_faces(d, f2, pr);
   if (d.thermal) {
     pull_generic(d, g2, g, false);
     bool dir[4];
@@ -1034,7 +1182,9 @@ static void generic_step(const kmi_desc& d, const uint8_t* solid, T*& f, T*& f2,
     collide_boussinesq_2d(f2, Temp, solid, nx, ny, T(d.omega), T(d.beta_g), T(d.T_ref));
     std::swap(g, g2);
   } else if (d.stencil == KMI_D2Q9) {
-    if (d.collision == KMI_BGK) collide_2d(f2, solid, nx, ny, T(d.omega));
+    if (d.collision == KMI_BGK && d.forcing == KMI_FORCE_GUO)
+      collide_guo_2d(f2, solid, nx, ny, T(d.omega), T(d.force[0]), T(d.force[1]));
+    else if (d.collision == KMI_BGK) collide_2d(f2, solid, nx, ny, T(d.omega));
     else if (d.collision == KMI_TRT) collide_trt_2d(f2, solid, nx, ny, T(d.s_plus), T(d.s_minus));
     else collide_mrt_2d(f2, solid, nx, ny, T(d.s_e), T(d.s_eps), T(d.s_q), T(d.s_nu));
   } else {
@@ -1180,6 +1330,36 @@ extern "C" {
     for (int64_t s = 0; s < nsteps; ++s) ko::generic_step(*d, solid, fa, fb, ga, gb, scratch); \
     if (fa != f) std::memcpy(f, fa, n * sizeof(T));                                      \
     if (d->thermal && ga != g) std::memcpy(g, ga, n * sizeof(T));                        \
+  }                                                                                      \
+  /* as ko_run, with the per-node arrays of profile faces: prof[2*face + k] */           \
+  void ko_run_profiles_##S(const kmi_desc* d, const uint8_t* solid, T* f,                \
+                           const double* const* prof, int64_t nsteps) {                  \
+    const size_t n = size_t(d->nx * d->ny) * 9;                                          \
+    std::vector<T> b1(n), scratch;                                                       \
+    ko::Profiles pr;                                                                     \
+    for (int k = 0; k < 8; ++k) pr.p[k / 2][k % 2] = prof[k];                            \
+    T *fa = f, *fb = b1.data(), *ga = nullptr, *gb = nullptr;                            \
+    for (int64_t s = 0; s < nsteps; ++s)                                                 \
+      ko::generic_step(*d, solid, fa, fb, ga, gb, scratch, &pr);                         \
+    if (fa != f) std::memcpy(f, fa, n * sizeof(T));                                      \
+  }                                                                                      \
+  void ko_collide_guo_2d_##S(T* f, const uint8_t* s, int64_t nx, int64_t ny, double om, \
+                             double fx, double fy) {                                     \
+    ko::collide_guo_2d(f, s, nx, ny, T(om), T(fx), T(fy));                               \
+  }                                                                                      \
+  void ko_macroscopic_forced_2d_##S(T* r, T* u, T* v, const T* f, int64_t nx,           \
+                                    int64_t ny, double fx, double fy) {                  \
+    ko::macroscopic_forced_2d(r, u, v, f, nx, ny, T(fx), T(fy));                         \
+  }                                                                                      \
+  void ko_zou_he_velocity_spatial_2d_##S(T* f, int64_t face, const double* ux,          \
+                                         const double* uy, int64_t nx, int64_t ny) {     \
+    if (face == KMI_NORTH) ko::zou_he_north_spatial(f, ux, uy, nx, ny);                  \
+    else if (face == KMI_SOUTH) ko::zou_he_south_spatial(f, ux, uy, nx, ny);             \
+    else if (face == KMI_WEST) ko::zou_he_west_spatial(f, ux, uy, nx, ny);               \
+  }                                                                                      \
+  void ko_zou_he_pressure_east_spatial_2d_##S(T* f, const double* rho, int64_t nx,      \
+                                              int64_t ny) {                              \
+    ko::zou_he_pressure_east_spatial(f, rho, nx, ny);                                    \
   }
 
 KO_BOTH(KO_STREAM)

@@ -27,7 +27,7 @@ def test_every_declared_symbol_is_exported(kmi):
 
 def test_abi_version_and_layout(kmi):
     lib = kmi.load()
-    assert lib.kmi_abi_version() == 1
+    assert lib.kmi_abi_version() == 2
     assert lib.kmi_desc_sizeof() == ctypes.sizeof(kmi._lib.KmiDesc)
 
 

@@ -269,3 +269,53 @@ def test_invalid_descriptor_raises(kmi, backend):
     with pytest.raises(kmi.KrakenError) as e:
         kmi.Lattice(d, backend)
     assert e.value.status == 5 and "MRT" in str(e.value)
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32", "f32s", "f16s"])
+@pytest.mark.parametrize("collision", ["trt", "bgk"])
+def test_two_step_launches_bit_identical(kmi, backend, storage, collision):
+    """Temporal blocking (two steps per launch, t+1 populations in LDS) reproduces single-step
+    launches bit for bit, for every tile and z split, odd step counts included."""
+    shape = (64, 16, 12)
+    per = {f: {"kind": "periodic"} for f in ("west", "east", "south", "north", "bottom", "top")}
+    desc = kmi.make_desc("d3q19", *shape, 0.02, storage=storage, collision=collision, faces=per)
+    f = rng_pops(shape, 19, seed=5, amp=0.05)
+    steps = 37
+    print(f"[two-step] {storage} {collision} shape={shape} steps={steps}", flush=True)
+    with kmi.Lattice(desc, backend) as lat:
+        lat.upload(f)
+        lat.step(steps)
+        ref = lat.download()
+    modes = [(1, 1), (1, 3), (2, 2), (2, 12)] + ([(3, 5)] if storage != "f64" else [])
+    for tile, zc in modes:
+        with kmi.Lattice(desc, backend) as lat:
+            lat.upload(f)
+            lat.set_temporal(tile, zc)
+            assert lat.temporal == {"tile": tile, "zchunks": zc}
+            lat.step(steps)
+            got = lat.download()
+        assert np.array_equal(got, ref), (tile, zc, float(np.max(np.abs(got - ref))))
+
+
+def test_two_step_tuned_and_rejected_where_it_does_not_apply(kmi, backend):
+    N = 64
+    desc = kmi.taylor_green_3d_desc(N, 0.01, "f32")
+    fields = kmi.taylor_green_3d_fields(N, 0.05)
+    with kmi.Lattice(desc, backend) as lat:
+        lat.init_equilibrium(*fields)
+        lat.step(3)
+        before = lat.download()
+        lat.autotune(32)
+        assert lat.steps == 3 and np.array_equal(lat.download(), before)
+        lat.step(40)
+        tuned = lat.download()
+        print(f"[two-step] tuned launch {lat.launch} temporal {lat.temporal}", flush=True)
+    with kmi.Lattice(desc, backend) as lat:
+        lat.init_equilibrium(*fields)
+        lat.step(43)
+        assert np.array_equal(lat.download(), tuned)
+    walls = kmi.make_desc("d3q19", 64, 16, 8, 0.02, collision="trt")
+    with kmi.Lattice(walls, backend) as lat:
+        with pytest.raises(kmi.KrakenError) as e:
+            lat.set_temporal(1, 1)
+        assert e.value.status == 5
