@@ -96,9 +96,16 @@ typedef enum {
 /* Body force on the flow populations (D2Q9 BGK).
  *   NONE — none.
  *   GUO  — constant force (F_x, F_y) with the Guo source term, collide_guo_2d!
- *          (collide_guo_2d.jl:5-72); moments then report u = (sum c f + F/2) / rho,
- *          compute_macroscopic_forced_2d! (macroscopic.jl:84-98). */
-typedef enum { KMI_FORCE_NONE = 0, KMI_FORCE_GUO = 1 } kmi_forcing;
+ *          (collide_guo_2d.jl:5-72); moments and reductions report
+ *          u = (sum c f + F/2) / rho, compute_macroscopic_forced_2d! (macroscopic.jl:84-98),
+ *          as the generic runner does (simulation_runner.jl:402-406).
+ *   GUO_PLAIN_MOMENTS — the same collision, moments without the F/2 shift, as
+ *          run_poiseuille_2d reports them (basic.jl:197-199). */
+typedef enum {
+  KMI_FORCE_NONE = 0,
+  KMI_FORCE_GUO = 1,
+  KMI_FORCE_GUO_PLAIN_MOMENTS = 2
+} kmi_forcing;
 
 /* Faces: x-, x+, y-, y+, z-, z+ (the reference's west/east/south/north/bottom/top). */
 typedef enum {

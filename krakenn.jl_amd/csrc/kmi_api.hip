@@ -174,8 +174,9 @@ static int validate(const kmi_desc& d, std::string& why, int& code) {
   if (d.thermal && d.collision != KMI_BGK)
     return code = KMI_ERR_UNSUPPORTED,
            why = "the thermal DDF uses BGK + Guo for the flow (thermal_2d.jl:421)", 1;
-  if (d.forcing != KMI_FORCE_NONE && d.forcing != KMI_FORCE_GUO) return why = "unknown forcing", 1;
-  if (d.forcing == KMI_FORCE_GUO &&
+  if (d.forcing < KMI_FORCE_NONE || d.forcing > KMI_FORCE_GUO_PLAIN_MOMENTS)
+    return why = "unknown forcing", 1;
+  if (d.forcing != KMI_FORCE_NONE &&
       (d.stencil != KMI_D2Q9 || d.collision != KMI_BGK || d.thermal))
     return code = KMI_ERR_UNSUPPORTED,
            why = "Guo body force is the D2Q9 BGK collision (collide_guo_2d.jl:5-72)", 1;
@@ -277,7 +278,7 @@ static void build_args(kmi_lattice* L) {
   a.beta_g = d.beta_g;
   a.T_ref = d.T_ref;
   a.coupling = d.coupling;
-  a.forcing = d.forcing;
+  a.forcing = d.forcing != KMI_FORCE_NONE;
   a.fx = d.force[0];
   a.fy = d.force[1];
   bool dir[4];

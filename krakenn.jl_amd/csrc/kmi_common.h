@@ -213,7 +213,7 @@ struct StepArgs {
   double omega, s_plus, s_minus, s_e, s_eps, s_q, s_nu;
   double omega_T, beta_g, T_ref;
   int coupling;
-  int forcing;      // kmi_forcing
+  int forcing;      // 1: Guo body force (any kmi_forcing other than NONE)
   double fx, fy;    // body force (KMI_FORCE_GUO)
   FaceArgs face[6];
 };

@@ -198,7 +198,7 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs a) {
     } else if constexpr (COLL == KMI_BGK) {
       bool forced = false;
       if constexpr (Q == 9 && BND) {
-        if (a.forcing == KMI_FORCE_GUO) {
+        if (a.forcing) {
           // collide_guo_2d_kernel! (collide_guo_2d.jl:5-72): u = (Σcf + F/2)/ρ
           forced = true;
           const R fx = R(a.fx), fy = R(a.fy);

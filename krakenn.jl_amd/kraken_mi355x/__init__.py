@@ -22,21 +22,29 @@ from .drivers import (
     reynolds,
     run_cavity_2d,
     run_cavity_3d,
+    run_couette_2d,
     run_natural_convection_2d,
+    run_poiseuille_2d,
     run_rayleigh_benard_2d,
+    run_taylor_green_2d,
     run_taylor_green_3d,
     run_to_residual,
+    taylor_green_2d_fields,
     taylor_green_3d_desc,
     taylor_green_3d_fields,
     trt_rates,
 )
 from .lattice import ArgumentError, Lattice, MI355XBackend, make_desc
+from .runner import (BoundarySetup, DomainSetup, GeometryRegion, InitialSetup, PhysicsSetup,
+                     SimulationSetup, plan_simulation, run_simulation)
 
 __all__ = [
+    "BoundarySetup", "DomainSetup", "GeometryRegion", "InitialSetup", "PhysicsSetup", "SimulationSetup",
+    "plan_simulation", "run_simulation",
     "Batch", "STATUS_NAMES", "ensemble_parameters", "ArgumentError", "D2Q9", "D3Q19", "KrakenError", "LBMConfig", "Lattice", "MI355XBackend",
     "cavity_3d_desc", "cavity_desc", "lbm_config", "load", "make_desc", "natural_convection_desc",
     "natural_convection_initial_temperature", "omega", "rayleigh_benard_desc",
-    "rb_initial_temperature", "reynolds", "run_cavity_2d", "run_cavity_3d", "run_natural_convection_2d",
-    "run_rayleigh_benard_2d", "run_taylor_green_3d", "run_to_residual", "taylor_green_3d_desc",
-    "taylor_green_3d_fields", "trt_rates",
+    "rb_initial_temperature", "reynolds", "run_cavity_2d", "run_cavity_3d", "run_couette_2d", "run_natural_convection_2d",
+    "run_poiseuille_2d", "run_rayleigh_benard_2d", "run_taylor_green_2d", "run_taylor_green_3d", "run_to_residual", "taylor_green_3d_desc",
+    "taylor_green_2d_fields", "taylor_green_3d_fields", "trt_rates",
 ]

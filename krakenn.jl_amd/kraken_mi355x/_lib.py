@@ -20,7 +20,7 @@ STORE_F64, STORE_F32, STORE_F32S, STORE_F16S = 0, 1, 2, 3
 BGK, TRT, MRT = 0, 1, 2
 (FACE_PERIODIC, FACE_WALL, FACE_MOVING_WALL, FACE_ZOUHE_VELOCITY, FACE_ZOUHE_PRESSURE,
  FACE_ZOUHE_VELOCITY_PROFILE, FACE_ZOUHE_PRESSURE_PROFILE) = range(7)
-FORCE_NONE, FORCE_GUO = 0, 1
+FORCE_NONE, FORCE_GUO, FORCE_GUO_PLAIN_MOMENTS = 0, 1, 2
 WEST, EAST, SOUTH, NORTH, BOTTOM, TOP = range(6)
 THERMAL_ADIABATIC, THERMAL_DIRICHLET = 0, 1
 COUPLING_SPLIT, COUPLING_FUSED = 0, 1

@@ -126,6 +126,68 @@ def run_cavity_3d(config: LBMConfig, *, backend=None, T=np.float64, collision="b
     return {"ρ": m["rho"], "ux": m["ux"], "uy": m["uy"], "uz": m["uz"], "config": config}
 
 
+def run_poiseuille_2d(*, Nx=4, Ny=32, nu=None, ν=0.1, Fx=1e-5, max_steps=10000, backend=None,
+                      T=np.float64):
+    """run_poiseuille_2d (basic.jl:185-204): periodic x, halfway-BB walls in y, BGK with the
+    Guo force (F_x, 0); moments as compute_macroscopic_2d! (no F/2 shift)."""
+    nu = ν if nu is None else nu
+    per = {"west": {"kind": "periodic"}, "east": {"kind": "periodic"}}
+    d = make_desc("d2q9", Nx, Ny, 1, nu, storage=_storage(T), faces=per, forcing="guo_plain",
+                  force=(Fx, 0.0))
+    with Lattice(d, _backend(backend)) as lat:
+        lat.step(max_steps)
+        m = lat.moments()
+    return {"ρ": m["rho"], "ux": m["ux"], "uy": m["uy"],
+            "config": lbm_config(D2Q9(), Nx=Nx, Ny=Ny, nu=nu, u_lid=0.0, max_steps=max_steps)}
+
+
+def run_couette_2d(*, Nx=4, Ny=32, nu=None, ν=0.1, u_wall=0.05, max_steps=10000, backend=None,
+                   T=np.float64):
+    """run_couette_2d (basic.jl:213-233): periodic x, Zou–He south wall moving at u_wall
+    then Zou–He north wall at rest, BGK."""
+    nu = ν if nu is None else nu
+    faces = {"west": {"kind": "periodic"}, "east": {"kind": "periodic"},
+             "south": {"kind": "zouhe_velocity", "u": (u_wall, 0.0, 0.0)},
+             "north": {"kind": "zouhe_velocity", "u": (0.0, 0.0, 0.0)}}
+    d = make_desc("d2q9", Nx, Ny, 1, nu, storage=_storage(T), faces=faces, bc_order=["south", "north"])
+    with Lattice(d, _backend(backend)) as lat:
+        lat.step(max_steps)
+        m = lat.moments()
+    return {"ρ": m["rho"], "ux": m["ux"], "uy": m["uy"],
+            "config": lbm_config(D2Q9(), Nx=Nx, Ny=Ny, nu=nu, u_lid=0.0, max_steps=max_steps)}
+
+
+def taylor_green_2d_fields(N, u0, T=np.float64):
+    """Initial (ρ, u_x, u_y) of initialize_taylor_green_2d (basic.jl:242-274), evaluated in
+    the element type T at nodes x = i − 1: u_x = −u0 cos kx sin ky, u_y = u0 sin kx cos ky,
+    ρ = 1 − 3u0²/4 (cos 2kx + cos 2ky), k = 2π/N."""
+    T = np.dtype(T).type
+    k = T(2 * np.pi / N)
+    x = np.arange(N, dtype=T)[:, None] * np.ones((1, N), dtype=T)
+    y = np.ones((N, 1), dtype=T) * np.arange(N, dtype=T)[None, :]
+    u0 = T(u0)
+    ux = -u0 * np.cos(k * x) * np.sin(k * y)
+    uy = u0 * np.sin(k * x) * np.cos(k * y)
+    rho = T(1) - T(3) * u0 ** 2 / T(4) * (np.cos(T(2) * k * x) + np.cos(T(2) * k * y))
+    return rho, ux, uy
+
+
+def run_taylor_green_2d(*, N=64, nu=None, ν=0.01, u0=0.01, max_steps=1000, backend=None,
+                        T=np.float64):
+    """run_taylor_green_2d (basic.jl:281-300): fully periodic BGK decay from the analytic
+    vortex; returns the final moments with `k` and `u0`."""
+    nu = ν if nu is None else nu
+    per = {f: {"kind": "periodic"} for f in ("west", "east", "south", "north")}
+    d = make_desc("d2q9", N, N, 1, nu, storage=_storage(T), faces=per)
+    with Lattice(d, _backend(backend)) as lat:
+        lat.init_equilibrium(*taylor_green_2d_fields(N, u0, T))
+        lat.step(max_steps)
+        m = lat.moments()
+    return {"ρ": m["rho"], "ux": m["ux"], "uy": m["uy"], "u0": u0, "k": 2 * np.pi / N,
+            "max_steps": max_steps,
+            "config": lbm_config(D2Q9(), Nx=N, Ny=N, nu=nu, u_lid=0.0, max_steps=0)}
+
+
 def taylor_green_3d_fields(N, U0):
     """u = U0 sin x cos y cos z, v = −U0 cos x sin y cos z, w = 0, ρ = 1, x = 2π(i−1)/N
     (node convention of basic.jl:259-262)."""

@@ -65,9 +65,10 @@ class MI355XBackend:
 def make_desc(stencil, nx, ny, nz=1, nu=0.1, storage="f64", collision="bgk", faces=None,
               thermal=False, omega=None, s_plus=None, s_minus=None, Lambda=3.0 / 16.0,
               s_e=1.4, s_eps=1.4, s_q=1.2, omega_T=None, beta_g=0.0, T_ref=0.0,
-              coupling="split", bc_order=None) -> L.KmiDesc:
+              coupling="split", bc_order=None, forcing=None, force=(0.0, 0.0, 0.0)) -> L.KmiDesc:
     """Build a kmi_desc.  `faces` maps face name -> dict(kind=..., u=..., rho=..., T=...,
-    thermal='dirichlet'|'adiabatic')."""
+    thermal='dirichlet'|'adiabatic').  forcing: None, 'guo' (moments with the F/2 shift) or
+    'guo_plain' (moments without it), with the body force `force`."""
     st = {"d2q9": L.D2Q9, "d3q19": L.D3Q19}.get(str(stencil).lower(), stencil)
     d = L.desc_init(st, nx, ny, nz, nu)
     d.storage = L.STORAGE_NAMES[storage] if isinstance(storage, str) else storage
@@ -84,23 +85,35 @@ def make_desc(stencil, nx, ny, nz=1, nu=0.1, storage="f64", collision="bgk", fac
     d.beta_g = beta_g
     d.T_ref = T_ref
     d.coupling = L.COUPLING_FUSED if coupling == "fused" else L.COUPLING_SPLIT
-    kinds = {"periodic": L.FACE_PERIODIC, "wall": L.FACE_WALL, "moving_wall": L.FACE_MOVING_WALL,
-             "zouhe_velocity": L.FACE_ZOUHE_VELOCITY, "zouhe_pressure": L.FACE_ZOUHE_PRESSURE}
     for name, spec in (faces or {}).items():
-        f = d.faces[L.FACE_NAMES[name]]
-        k = spec.get("kind", "wall")
-        f.kind = kinds[k] if isinstance(k, str) else k
-        u = list(spec.get("u", (0.0, 0.0, 0.0))) + [0.0, 0.0, 0.0]
-        for c in range(3):
-            f.u[c] = float(u[c])
-        f.rho = float(spec.get("rho", 1.0))
-        f.T = float(spec.get("T", 0.0))
-        f.thermal = L.THERMAL_DIRICHLET if spec.get("thermal") == "dirichlet" else L.THERMAL_ADIABATIC
+        _fill_face(d.faces[L.FACE_NAMES[name]], spec)
     if bc_order:
         d.n_bc_order = len(bc_order)
         for n, name in enumerate(bc_order):
             d.bc_order[n] = L.FACE_NAMES[name] if isinstance(name, str) else name
+    d.forcing = {None: L.FORCE_NONE, "none": L.FORCE_NONE, "guo": L.FORCE_GUO,
+                 "guo_plain": L.FORCE_GUO_PLAIN_MOMENTS}[forcing]
+    fv = list(force) + [0.0, 0.0, 0.0]
+    for c in range(3):
+        d.force[c] = float(fv[c])
     return d
+
+
+FACE_KINDS = {"periodic": L.FACE_PERIODIC, "wall": L.FACE_WALL, "moving_wall": L.FACE_MOVING_WALL,
+              "zouhe_velocity": L.FACE_ZOUHE_VELOCITY, "zouhe_pressure": L.FACE_ZOUHE_PRESSURE,
+              "zouhe_velocity_profile": L.FACE_ZOUHE_VELOCITY_PROFILE,
+              "zouhe_pressure_profile": L.FACE_ZOUHE_PRESSURE_PROFILE}
+
+
+def _fill_face(f, spec):
+    k = spec.get("kind", "wall")
+    f.kind = FACE_KINDS[k] if isinstance(k, str) else k
+    u = list(spec.get("u", (0.0, 0.0, 0.0))) + [0.0, 0.0, 0.0]
+    for c in range(3):
+        f.u[c] = float(u[c])
+    f.rho = float(spec.get("rho", 1.0))
+    f.T = float(spec.get("T", 0.0))
+    f.thermal = L.THERMAL_DIRICHLET if spec.get("thermal") == "dirichlet" else L.THERMAL_ADIABATIC
 
 
 class Lattice:
@@ -168,6 +181,22 @@ class Lattice:
         arrs = [None if a is None else np.asfortranarray(np.broadcast_to(a, self.shape), dtype=np.float64)
                 for a in (rho, ux, uy, uz)]
         self._chk(L.load().kmi_init_equilibrium(self._h, *[_dptr(a) for a in arrs]))
+
+    def set_face_profile(self, face, a, b=None):
+        """Per-node values of a profile face: (u_x, u_y) arrays for a velocity profile, or the
+        density array `a` for a pressure profile (length nx on south/north, ny on west/east)."""
+        fid = L.FACE_NAMES[face] if isinstance(face, str) else int(face)
+        n = self.shape[0] if fid in (L.SOUTH, L.NORTH) else self.shape[1]
+        pa = np.ascontiguousarray(np.broadcast_to(a, (n,)), dtype=np.float64)
+        pb = None if b is None else np.ascontiguousarray(np.broadcast_to(b, (n,)), dtype=np.float64)
+        self._chk(L.load().kmi_set_face_profile(self._h, fid, _dptr(pa), _dptr(pb)))
+
+    def set_face(self, face, spec):
+        """Replace a face's values (u, rho, T) between steps; its kind must not change."""
+        fid = L.FACE_NAMES[face] if isinstance(face, str) else int(face)
+        f = L.KmiFace()
+        _fill_face(f, spec)
+        self._chk(L.load().kmi_set_face(self._h, fid, byref(f)))
 
     def init_temperature(self, T=None):
         a = None if T is None else np.asfortranarray(np.broadcast_to(T, self.shape), dtype=np.float64)

@@ -1182,7 +1182,7 @@ static void generic_step(const kmi_desc& d, const uint8_t* solid, T*& f, T*& f2,
     collide_boussinesq_2d(f2, Temp, solid, nx, ny, T(d.omega), T(d.beta_g), T(d.T_ref));
     std::swap(g, g2);
   } else if (d.stencil == KMI_D2Q9) {
-    if (d.collision == KMI_BGK && d.forcing == KMI_FORCE_GUO)
+    if (d.collision == KMI_BGK && d.forcing != KMI_FORCE_NONE)
       collide_guo_2d(f2, solid, nx, ny, T(d.omega), T(d.force[0]), T(d.force[1]));
     else if (d.collision == KMI_BGK) collide_2d(f2, solid, nx, ny, T(d.omega));
     else if (d.collision == KMI_TRT) collide_trt_2d(f2, solid, nx, ny, T(d.s_plus), T(d.s_minus));

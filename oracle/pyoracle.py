@@ -88,6 +88,34 @@ def run(desc, f, g=None, nsteps=1, mask=None):
     return f, g
 
 
+def run_profiles(desc, f, profiles, nsteps=1, mask=None):
+    """As run(), for D2Q9 lattices with profile faces: `profiles` maps face name -> (a, b)
+    arrays (u_x, u_y of a velocity profile, or rho and None of a pressure profile)."""
+    names = {"west": 0, "east": 1, "south": 2, "north": 3}
+    keep, ptrs = [], (ctypes.c_void_p * 8)()
+    for name, (a, b) in profiles.items():
+        for k, arr in enumerate((a, b)):
+            if arr is None:
+                continue
+            arr = np.ascontiguousarray(arr, dtype=np.float64)
+            keep.append(arr)
+            ptrs[2 * names[name] + k] = arr.ctypes.data
+    fn = getattr(lib(), f"ko_run_profiles_{_sfx(f)}")
+    fn.restype = None
+    m = None if mask is None else np.asfortranarray(mask, dtype=np.uint8)
+    fn(ctypes.byref(desc), None if m is None else m.ctypes.data_as(c_void_p), _p(f), ptrs,
+       c_int64(int(nsteps)))
+    return f
+
+
+def moments_forced(f, fx, fy):
+    """compute_macroscopic_forced_2d of a D2Q9 population array."""
+    nx, ny, _ = f.shape
+    r, u, v = (np.empty((nx, ny), dtype=f.dtype, order="F") for _ in range(3))
+    call("macroscopic_forced_2d", r, u, v, f, nx, ny, float(fx), float(fy), dtype=f.dtype)
+    return {"rho": r, "ux": u, "uy": v}
+
+
 def pull(desc, f):
     out = np.empty_like(f, order="F")
     fn = getattr(lib(), f"ko_pull_generic_{_sfx(f)}")
