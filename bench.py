@@ -1,9 +1,11 @@
 """Benchmark: D3Q19 TRT (Λ = 3/16) triply periodic Taylor–Green vortex, 256³, FP32 storage.
 
-One "step" = one single-pass stream–collide launch over the whole 256³ lattice (populations
-resident in HBM).  Prints ONE JSON line with MLUPS, the HBM roofline of the step kernel
-(algorithmic 152 B per cell update in FP32 = 19 reads + 19 writes of 4 B) and a CPU
-baseline (the FP32 oracle restatement, OpenMP on the host cores, bounded sample).
+One "step" = one stream–collide time step over the whole 256³ lattice (populations resident
+in HBM).  The autotuner picks the launch: single-step launches, or two steps per launch with
+the intermediate populations in LDS.  Prints ONE JSON line with MLUPS, the HBM roofline of the
+dominant kernel (algorithmic 152 B per cell update in FP32 = 19 reads + 19 writes of 4 B,
+times the updates one launch performs) and a CPU baseline (the FP32 oracle restatement,
+OpenMP on the host cores, bounded sample).
 """
 import argparse
 import json
@@ -151,6 +153,8 @@ def main():
     ap.add_argument("--storage", default="f32", choices=sorted(BYTES_PER_POP))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--temporal", default=None,
+                    help="TILE,ZCHUNKS[,NT]: fixed two-step launch (implies --no-autotune)")
     ap.add_argument("--variants", action="store_true", help="also time f64 and f16s storage")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -173,7 +177,11 @@ def main():
         desc = kmi.taylor_green_3d_desc(N, nu, storage)
         with kmi.Lattice(desc) as lat:
             lat.init_equilibrium(*fields)
-            if not args.no_autotune:
+            if args.temporal:
+                tile, zc, *nt = (int(v) for v in args.temporal.split(","))
+                lat.set_launch(256, 1, bool(nt[0]) if nt else False)
+                lat.set_temporal(tile, zc)
+            elif not args.no_autotune:
                 lat.autotune(64)  # restores the populations afterwards
             lat.step(warmup)
             lat.backend.synchronize()
@@ -189,17 +197,21 @@ def main():
     copy_gbps = kmi.MI355XBackend().copy_bandwidth(2 * 1024 ** 3, 20)
     # HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated on a
     # copy kernel of known size), recorded for this workload in profiles/traffic.json.
+    # Steps per launch of the dominant kernel: 2 for the two-step kernel (k_step2_d3q19),
+    # else 1 (k_step).  Traffic records are per kernel.
+    span = 2 if launch.get("tile", 0) else 1
+    kernel = "k_step2_d3q19" if span == 2 else "k_step"
     traffic = None
     try:
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as fh:
-            rec = json.load(fh).get(f"d3q19_trt_{args.storage}_{N}")
+            rec = json.load(fh).get(f"{kernel}_trt_{args.storage}_{N}")
         traffic = rec["bytes_per_launch"] if rec else None
     except (OSError, ValueError, KeyError):
         traffic = None
     cells = N ** 3
     mlups = cells * args.steps / wall / 1e6
-    per_launch_s = ms / 1e3 / args.steps
-    alg_bytes = cells * 19 * 2 * BYTES_PER_POP[args.storage]
+    per_launch_s = ms / 1e3 / (args.steps / span)
+    alg_bytes = span * cells * 19 * 2 * BYTES_PER_POP[args.storage]
     achieved = alg_bytes / per_launch_s / 1e9
     out = {
         "metric": "MLUPS (D3Q19 TRT 256^3 Taylor-Green, FP32 storage) and % of HBM roofline",
@@ -219,7 +231,10 @@ def main():
                    "cells": cells, "storage": args.storage, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
+                     "kernel": kernel, "steps_per_launch": span,
                      "bytes_per_launch": alg_bytes, "avg_launch_ms": round(per_launch_s * 1e3, 5),
+                     "hbm_GBps_measured_traffic": (round(traffic / per_launch_s / 1e9, 1)
+                                                   if traffic else None),
                      "copy_measured": round(copy_gbps, 1),
                      "frac_of_copy": round(achieved / copy_gbps, 4)},
         "launch": launch,

@@ -259,8 +259,8 @@ int kmi_lattice_set_launch(kmi_lattice* lat, int32_t bx, int32_t by, int32_t non
 int kmi_lattice_get_launch(const kmi_lattice* lat, int32_t* bx, int32_t* by, int32_t* nontemporal);
 /* Two steps per launch (temporal blocking) for fully periodic D3Q19 BGK/TRT lattices
  * without mask: the intermediate populations stay in LDS, halving HBM traffic per step.
- * tile: 0 off, 1 = 32x8, 2 = 64x4, 3 = 64x8 cells per work-group (3 not for F64); nx and
- * ny must be multiples of the tile.  zchunks: z-ranges per tile column (1..nz).  Results
+ * tile: 0 off, 1 = 32x8, 2 = 64x4, 3 = 64x8, 4 = 32x16, 5 = 64x16 (not F64) cells per
+ * work-group; nx and ny must be multiples of the tile.  zchunks: z-ranges per tile column (1..nz).  Results
  * are bit-identical to single-step launches; odd step counts end with one single step. */
 int kmi_lattice_set_temporal(kmi_lattice* lat, int32_t tile, int32_t zchunks);
 int kmi_lattice_get_temporal(const kmi_lattice* lat, int32_t* tile, int32_t* zchunks);

@@ -891,7 +891,7 @@ int kmi_lattice_set_temporal(kmi_lattice* L, int32_t tile, int32_t zchunks) {
     return fail(L->ctx, KMI_ERR_UNSUPPORTED,
                 "kmi_lattice_set_temporal: two-step launches need a fully periodic D3Q19 "
                 "BGK/TRT lattice without mask whose nx, ny are multiples of the tile "
-                "(1: 32x8, 2: 64x4, 3: 64x8, not F64)");
+                "(1: 32x8, 2: 64x4, 3: 64x8, 4: 32x16, 5: 64x16)");
   if (tile != 0 && (zchunks < 1 || zchunks > L->args.nz))
     return fail(L->ctx, KMI_ERR_ARG, "kmi_lattice_set_temporal: zchunks must be in 1..nz");
   KMI_HIP(L->ctx, hipSetDevice(L->ctx->device));
@@ -971,8 +971,8 @@ int kmi_lattice_autotune(kmi_lattice* L, int64_t steps_per_trial, double* best_m
     }
   }
   // Two-step launches (the single-step shape found above serves odd remainders).
-  const int zc_list[] = {1, 2, 3, 4, 6, 8};
-  for (int tile = 1; tile <= 3 && e == hipSuccess; ++tile) {
+  const int zc_list[] = {1, 2, 3, 4, 6, 8, 12, 16};
+  for (int tile = 1; tile <= 5 && e == hipSuccess; ++tile) {
     if (!tb_eligible(L, tile)) continue;
     for (int zc : zc_list) {
       if (zc > L->args.nz || e != hipSuccess) continue;

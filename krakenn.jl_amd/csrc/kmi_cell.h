@@ -7,17 +7,6 @@
 
 namespace kmi {
 
-template <class R>
-KMI_DEV R feq_val(R w, R rho, R cu, R usq) {
-  return w * rho * (R(1) + R(3) * cu + R(4.5) * cu * cu - R(1.5) * usq);
-}
-
-// 3cu + 4.5cu² − 1.5u² (rest: −1.5u²), for the deviation form feq − w = w(δρ + ρX).
-template <class R>
-KMI_DEV R feq_x(R cu, R usq) {
-  return R(3) * cu + R(4.5) * cu * cu - R(1.5) * usq;
-}
-
 template <int Q, class R>
 KMI_DEV R usq_of(R ux, R uy, R uz) {
   if constexpr (Q == 9) return ux * ux + uy * uy;
@@ -26,22 +15,32 @@ KMI_DEV R usq_of(R ux, R uy, R uz) {
 
 // ---- equilibria ----------------------------------------------------------------------
 // Plain: feq_q = w ρ (1 + 3cu + 4.5cu² − 1.5u²).  Shifted: deq_q = w (δρ + ρ X_q).
+// Each population q and its opposite share c·u up to sign: c_opp·u = −(c_q·u) exactly (the
+// sums are negated term by term and round-to-nearest is symmetric), hence 3c_opp·u = −3c_q·u
+// and (4.5 c_opp·u)(c_opp·u) = (4.5 c_q·u)(c_q·u) bit for bit, and (1 + (−t)) = (1 − t).  The
+// pair is evaluated once; every value equals the reference's per-q expression.
 template <int Q, bool SH, class R>
 KMI_DEV void equilibria(R drho_or_rho, R rho, R ux, R uy, R uz, R (&fe)[Q]) {
   const R usq = usq_of<Q>(ux, uy, uz);
+  const R u15 = R(1.5) * usq;
+  if constexpr (!SH) fe[0] = R(Lat<Q>::w[0]) * rho * (R(1) - u15);
+  else fe[0] = R(Lat<Q>::w[0]) * (drho_or_rho + rho * (-u15));
   sfor<Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    constexpr double w = Lat<Q>::w[q];
-    if constexpr (!SH) {
-      if constexpr (q == 0) fe[q] = R(w) * rho * (R(1) - R(1.5) * usq);
-      else
-        fe[q] = feq_val(R(w), rho,
-                        cdotu<Lat<Q>::cx[q], Lat<Q>::cy[q], Lat<Q>::cz[q]>(ux, uy, uz), usq);
-    } else {
-      R x;
-      if constexpr (q == 0) x = -(R(1.5) * usq);
-      else x = feq_x(cdotu<Lat<Q>::cx[q], Lat<Q>::cy[q], Lat<Q>::cz[q]>(ux, uy, uz), usq);
-      fe[q] = R(w) * (drho_or_rho + rho * x);
+    constexpr int o = Lat<Q>::opp[q];
+    if constexpr (q != 0 && q < o) {
+      constexpr double w = Lat<Q>::w[q];
+      const R cu = cdotu<Lat<Q>::cx[q], Lat<Q>::cy[q], Lat<Q>::cz[q]>(ux, uy, uz);
+      const R t3 = R(3) * cu;
+      const R sq = R(4.5) * cu * cu;
+      if constexpr (!SH) {
+        const R wr = R(w) * rho;
+        fe[q] = wr * (((R(1) + t3) + sq) - u15);
+        fe[o] = wr * (((R(1) - t3) + sq) - u15);
+      } else {
+        fe[q] = R(w) * (drho_or_rho + rho * ((t3 + sq) - u15));
+        fe[o] = R(w) * (drho_or_rho + rho * (((-t3) + sq) - u15));
+      }
     }
   });
 }

@@ -10,12 +10,16 @@
 //
 // Per marched plane s:
 //   phase 1  t+1 populations of plane s on the (TX+2)×(TY+2) extended tile: pull from the
-//            t field in HBM (periodic wrap), collide, write to LDS;
+//            t field in HBM (periodic wrap), collide; the c_z = −1 populations go to LDS at
+//            once, the c_z = 0 and +1 populations stay in registers;
 //   phase 2  t+2 populations of plane s−1 on the TX×TY tile: pull from LDS — c_z = +1
-//            populations from plane s−2, c_z = 0 from s−1, c_z = −1 from s — collide and
-//            store to HBM.
-// LDS holds c_z = 0 populations of 2 planes (9 each), c_z = +1 of 3 planes (5 each) and
-// c_z = −1 of 1 plane (5): 38 values per extended cell.
+//            populations of plane s−2, c_z = 0 of s−1, c_z = −1 of s — collide, store to HBM;
+//   phase 3  the registers kept in phase 1 replace the LDS values phase 2 has consumed
+//            (c_z = 0 of s−1, c_z = +1 of s−2).
+// LDS per extended cell: 9 (c_z = 0, one plane) + 10 (c_z = +1, two planes) + 5 (c_z = −1)
+// = 24 values; two barriers per plane (after phases 1 and 2; phase 3's writes are read only
+// after the next phase-1 barrier, and phase 1 writes only the c_z = −1 slot, whose readers
+// finished before the phase-2 barrier).
 //
 // The arithmetic per cell and step is the single-step kernel's (same collide functions,
 // -ffp-contract=off), and the t+1 values pass through the storage rounding before they
@@ -40,6 +44,19 @@ constexpr int group_index(int q) {
 }
 
 KMI_DEV int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+// Element at byte offset `off` of a population plane.  Plane bases are uniform (scalar
+// registers) and the offsets 32-bit, so each access is one saddr-form global instruction
+// with no 64-bit address arithmetic.  A field holds fewer than 2^31 / 19 cells, so every
+// byte offset fits in 32 bits.
+template <class E>
+KMI_DEV const E& at(const E* plane, unsigned off) {
+  return *reinterpret_cast<const E*>(reinterpret_cast<const char*>(plane) + off);
+}
+template <class E>
+KMI_DEV E* at(E* plane, unsigned off) {
+  return reinterpret_cast<E*>(reinterpret_cast<char*>(plane) + off);
+}
 }  // namespace tb
 
 // Value a population takes after a store to and a load from storage ST.
@@ -66,10 +83,12 @@ __global__ void __launch_bounds__(TX * TY)
   using E = typename S::elem;
   using R = typename S::real;
   constexpr int EX = TX + 2, EY = TY + 2, NE = EX * EY, NT = TX * TY;
-  __shared__ R lds[38 * NE];
-  R* const A = lds;            // c_z = 0:  [2 planes][9][NE]
-  R* const B = lds + 18 * NE;  // c_z = +1: [3 planes][5][NE]
-  R* const C = lds + 33 * NE;  // c_z = −1: [5][NE]
+  constexpr int EPT = (NE + NT - 1) / NT;  // extended cells per work-item in phase 1
+  constexpr unsigned ES = sizeof(E);
+  __shared__ R lds[24 * NE];
+  R* const A = lds;            // c_z = 0:  [9][NE]
+  R* const B = lds + 9 * NE;   // c_z = +1: [2 planes][5][NE]
+  R* const C = lds + 19 * NE;  // c_z = −1: [5][NE]
 
   const int t = int(blockIdx.x % 8u) * per_xcd + int(blockIdx.x / 8u);
   if (t >= ntiles) return;  // padding groups (uniform per group, before any barrier)
@@ -82,36 +101,62 @@ __global__ void __launch_bounds__(TX * TY)
   E* __restrict__ fo = static_cast<E*>(a.fo);
   const int tid = threadIdx.x;
 
-  for (int s = z0 - 1; s <= z1; ++s) {
-    const int u = s + 3;  // non-negative plane counter for the LDS ring slots
-    // ---- phase 1: t+1 on plane s, extended tile ----
-    {
-      const int zs = tb::wrap(s, nz);
-      const int zm = tb::wrap(zs - 1, nz), zp = tb::wrap(zs + 1, nz);
-      R* const Aw = A + (u & 1) * 9 * NE;
-      R* const Bw = B + (u % 3) * 5 * NE;
-      for (int e = tid; e < NE; e += NT) {
+  // Pull of the t populations of plane s for the extended cells this work-item owns.  With
+  // FP32 arithmetic the pull for plane s+1 is issued before plane s is processed, so its HBM
+  // latency overlaps the collisions, the barriers and the phase-2 stores of plane s; FP64
+  // and 1024-wide groups have no registers to spare for it.
+  constexpr bool kPrefetch = sizeof(R) == 4 && NT <= 512;
+  R nxt[EPT][19];
+  auto pull_plane = [&](int s) {
+    const int zs = tb::wrap(s, nz);
+    const int zm = tb::wrap(zs - 1, nz), zp = tb::wrap(zs + 1, nz);
+#pragma unroll
+    for (int r = 0; r < EPT; ++r) {
+      const int e = tid + r * NT;
+      if (e < NE) {
         const int ex = e % EX, ey = e / EX;
         const int gx = tb::wrap(x0 - 1 + ex, nx), gy = tb::wrap(y0 - 1 + ey, ny);
         const int xm = tb::wrap(gx - 1, nx), xp = tb::wrap(gx + 1, nx);
         const int ym = tb::wrap(gy - 1, ny), yp = tb::wrap(gy + 1, ny);
-        R fp[19], fn[19];
+        // byte offsets of the three source columns and of the nine source rows
+        const unsigned bx[3] = {unsigned(xm) * ES, unsigned(gx) * ES, unsigned(xp) * ES};
+        const int yy[3] = {ym, gy, yp}, zz[3] = {zm, zs, zp};
         sfor<19>([&](auto qc) {
           constexpr int q = decltype(qc)::value;
           constexpr int cx = Lat<19>::cx[q], cy = Lat<19>::cy[q], cz = Lat<19>::cz[q];
-          const int sx = cx > 0 ? xm : (cx < 0 ? xp : gx);
-          const int sy = cy > 0 ? ym : (cy < 0 ? yp : gy);
-          const int sz = cz > 0 ? zm : (cz < 0 ? zp : zs);
-          fp[q] = S::ld(fi, (long long)(sx + nx * (sy + ny * sz)) + q * P);
+          const unsigned row = unsigned(nx) * unsigned(yy[1 - cy] + ny * zz[1 - cz]) * ES;
+          nxt[r][q] = S::ld(&tb::at(fi + q * P, row + bx[1 - cx]), 0);
         });
-        collide19<ST, COLL>(fp, fn, a);
+      }
+    }
+  };
+  if constexpr (kPrefetch) pull_plane(z0 - 1);
+
+  for (int s = z0 - 1; s <= z1; ++s) {
+    const int u = s + 3;  // non-negative plane counter for the LDS ring slot
+    // ---- phase 1: t+1 on plane s, extended tile ----
+    if constexpr (!kPrefetch) pull_plane(s);
+    R cur[EPT][19];
+#pragma unroll
+    for (int r = 0; r < EPT; ++r)
+#pragma unroll
+      for (int q = 0; q < 19; ++q) cur[r][q] = nxt[r][q];
+    if constexpr (kPrefetch)
+      if (s < z1) pull_plane(s + 1);
+    R keep0[EPT][9], keepp[EPT][5];
+#pragma unroll
+    for (int r = 0; r < EPT; ++r) {
+      const int e = tid + r * NT;
+      if (e < NE) {
+        R fn[19];
+        collide19<ST, COLL>(cur[r], fn, a);
         sfor<19>([&](auto qc) {
           constexpr int q = decltype(qc)::value;
           constexpr int cz = Lat<19>::cz[q];
           constexpr int n = tb::group_index(q);
           const R v = storage_round<ST>(fn[q]);
-          if constexpr (cz == 0) Aw[n * NE + e] = v;
-          else if constexpr (cz > 0) Bw[n * NE + e] = v;
+          if constexpr (cz == 0) keep0[r][n] = v;
+          else if constexpr (cz > 0) keepp[r][n] = v;
           else C[n * NE + e] = v;
         });
       }
@@ -119,8 +164,7 @@ __global__ void __launch_bounds__(TX * TY)
     __syncthreads();
     // ---- phase 2: t+2 on plane s−1, interior tile ----
     if (s >= z0 + 1) {
-      const R* const Ar = A + ((u - 1) & 1) * 9 * NE;
-      const R* const Br = B + ((u - 2) % 3) * 5 * NE;
+      const R* const Br = B + ((u - 2) & 1) * 5 * NE;
       const int ix = tid % TX, iy = tid / TX;
       R fp[19], fn[19];
       sfor<19>([&](auto qc) {
@@ -128,29 +172,46 @@ __global__ void __launch_bounds__(TX * TY)
         constexpr int cx = Lat<19>::cx[q], cy = Lat<19>::cy[q], cz = Lat<19>::cz[q];
         constexpr int n = tb::group_index(q);
         const int e = (iy + 1 - cy) * EX + (ix + 1 - cx);
-        if constexpr (cz == 0) fp[q] = Ar[n * NE + e];
+        if constexpr (cz == 0) fp[q] = A[n * NE + e];
         else if constexpr (cz > 0) fp[q] = Br[n * NE + e];
         else fp[q] = C[n * NE + e];
       });
       collide19<ST, COLL>(fp, fn, a);
-      const long long cell = (long long)(x0 + ix) + nx * ((long long)(y0 + iy) + (long long)ny * (s - 1));
+      const unsigned cell = (unsigned(x0 + ix) + unsigned(nx) * unsigned(y0 + iy + ny * (s - 1))) * ES;
       sfor<19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        S::template st<NTS>(fo, cell + q * P, fn[q]);
+        S::template st<NTS>(tb::at(fo + q * P, cell), 0, fn[q]);
       });
     }
     __syncthreads();
+    // ---- phase 3: plane s's c_z = 0 and +1 populations into the slots phase 2 released ----
+    {
+      R* const Bw = B + (u & 1) * 5 * NE;
+#pragma unroll
+      for (int r = 0; r < EPT; ++r) {
+        const int e = tid + r * NT;
+        if (e < NE) {
+#pragma unroll
+          for (int n = 0; n < 9; ++n) A[n * NE + e] = keep0[r][n];
+#pragma unroll
+          for (int n = 0; n < 5; ++n) Bw[n * NE + e] = keepp[r][n];
+        }
+      }
+    }
   }
 }
 
 // ---- dispatch ---------------------------------------------------------------------------
-// Tile variants: 1 = 32×8, 2 = 64×4, 3 = 64×8 (the last not for F64: its LDS would exceed
-// 160 KB).
+// Tile variants (TX × TY interior cells per work-group): 1 = 32×8, 2 = 64×4, 3 = 64×8,
+// 4 = 32×16, 5 = 64×16, not for FP64 (the larger the tile, the smaller the share of halo recomputation:
+// (TX+2)(TY+2)/(TX·TY) = 1.55, 1.55, 1.29, 1.32, 1.16).
 bool step2_tile(int tile, int st, int& TX, int& TY) {
   switch (tile) {
     case 1: TX = 32; TY = 8; return true;
     case 2: TX = 64; TY = 4; return true;
-    case 3: TX = 64; TY = 8; return st != KMI_STORE_F64;
+    case 3: TX = 64; TY = 8; return true;
+    case 4: TX = 32; TY = 16; return true;
+    case 5: TX = 64; TY = 16; return st != KMI_STORE_F64;  // FP64: > 160 KB of LDS
     default: return false;
   }
 }
@@ -173,8 +234,10 @@ static hipError_t launch2_tile(int tile, bool nts, const StepArgs& a, int zchunk
   switch (tile) {
     case 1: return launch2<ST, COLL, 32, 8>(nts, a, zchunks, s);
     case 2: return launch2<ST, COLL, 64, 4>(nts, a, zchunks, s);
+    case 3: return launch2<ST, COLL, 64, 8>(nts, a, zchunks, s);
+    case 4: return launch2<ST, COLL, 32, 16>(nts, a, zchunks, s);
     default:
-      if constexpr (ST != KMI_STORE_F64) return launch2<ST, COLL, 64, 8>(nts, a, zchunks, s);
+      if constexpr (ST != KMI_STORE_F64) return launch2<ST, COLL, 64, 16>(nts, a, zchunks, s);
       else return hipErrorInvalidValue;
   }
 }

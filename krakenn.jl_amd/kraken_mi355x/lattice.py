@@ -233,7 +233,7 @@ class Lattice:
         return {"bx": bx.value, "by": by.value, "nontemporal": bool(nt.value)}
 
     def set_temporal(self, tile, zchunks=1):
-        """Two steps per launch (tile 1: 32×8, 2: 64×4, 3: 64×8 cells; 0: off) on a fully
+        """Two steps per launch (tile 1: 32×8, 2: 64×4, 3: 64×8, 4: 32×16, 5: 64×16 cells; 0: off) on a fully
         periodic D3Q19 BGK/TRT lattice; bit-identical to single steps."""
         self._chk(L.load().kmi_lattice_set_temporal(self._h, int(tile), int(zchunks)))
 

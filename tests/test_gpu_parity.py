@@ -286,7 +286,7 @@ def test_two_step_launches_bit_identical(kmi, backend, storage, collision):
         lat.upload(f)
         lat.step(steps)
         ref = lat.download()
-    modes = [(1, 1), (1, 3), (2, 2), (2, 12)] + ([(3, 5)] if storage != "f64" else [])
+    modes = [(1, 1), (1, 3), (2, 2), (2, 12), (3, 5), (4, 1)] + ([(5, 2)] if storage != "f64" else [])
     for tile, zc in modes:
         with kmi.Lattice(desc, backend) as lat:
             lat.upload(f)
