@@ -107,6 +107,12 @@ typedef enum {
   KMI_FORCE_GUO_PLAIN_MOMENTS = 2
 } kmi_forcing;
 
+/* Cells flagged solid by kmi_set_mask.
+ *   BOUNCE_BACK — the cell swaps its pulled populations (bounce_back_2d!, collide_2d.jl).
+ *   INERT       — the cell holds the rest equilibrium f_q = w_q (SolidInert brick,
+ *                 dsl/bricks.jl:133-154): the wall is represented by the cut links. */
+typedef enum { KMI_SOLID_BOUNCE_BACK = 0, KMI_SOLID_INERT = 1 } kmi_solid_mode;
+
 /* Faces: x-, x+, y-, y+, z-, z+ (the reference's west/east/south/north/bottom/top). */
 typedef enum {
   KMI_WEST = 0,
@@ -155,6 +161,7 @@ typedef struct {
   kmi_face faces[6];        /* indexed by kmi_face_id; z faces ignored for D2Q9           */
   int32_t forcing;          /* kmi_forcing                                                */
   double force[3];          /* body force (F_x, F_y, F_z) for KMI_FORCE_GUO               */
+  int32_t solid_mode;       /* kmi_solid_mode                                             */
 } kmi_desc;
 
 typedef struct kmi_ctx kmi_ctx;
@@ -174,14 +181,21 @@ typedef enum { KMI_HOST_F64 = 0, KMI_HOST_F32 = 1 } kmi_host_dtype;
  *                   faces, the hot one named by `arg` (a kmi_face_id) — estimator of
  *                   thermal.jl:225-234 (second-order one-sided gradient, rows 2..N-1;
  *                   all rows when the wall-parallel axis is periodic)
- *   MAX_SPEED       out[0] = max_cells |u|  */
+ *   MAX_SPEED       out[0] = max_cells |u|
+ *   DRAG_MEA        out[0..2] = force on the cut-link walls, Mei–Luo–Shyy momentum
+ *                   exchange with the Bouzidi arriving population, per link
+ *                   c_q (f_q + f~_qbar)   (cylinder_libb.jl:61-104, drag_gpu.jl:346-403)
+ *   DRAG_HALFWAY    out[0..2] = sum over cut links of 2 c_q f_q   (halfway-BB momentum
+ *                   exchange, cylinder_libb.jl:27-42, li_bb_3d_v2.jl:167-206) */
 typedef enum {
   KMI_RED_MASS = 0,
   KMI_RED_MOMENTUM = 1,
   KMI_RED_KINETIC_ENERGY = 2,
   KMI_RED_THERMAL = 3,
   KMI_RED_NUSSELT = 4,
-  KMI_RED_MAX_SPEED = 5
+  KMI_RED_MAX_SPEED = 5,
+  KMI_RED_DRAG_MEA = 6,
+  KMI_RED_DRAG_HALFWAY = 7
 } kmi_reduction;
 
 /* ---- library / context ---------------------------------------------------------- */
@@ -221,6 +235,19 @@ int kmi_set_mask(kmi_lattice* lat, const uint8_t* is_solid);
  * profile equals the face's rho.  May be called between steps (time-dependent profiles,
  * src/bc/handlers.jl:203-208). */
 int kmi_set_face_profile(kmi_lattice* lat, int32_t face, const double* a, const double* b);
+/* Interpolated bounce-back on cut links (Bouzidi, applied to the pulled populations
+ * before the collision: fused_trt_libb_v2_step!, li_bb_2d_v2.jl:35-110, dsl/bricks.jl:516-581;
+ * D3Q19 li_bb_3d_v2.jl:13-43, dsl/bricks_3d.jl:366-473).  Link n leaves fluid cell
+ * cell[n] (index i + nx (j + ny k)) along direction q[n] (0-based) and meets the wall at
+ * the fraction qw[n] in (0, 1] of its length; uw[3n..3n+2] is the wall velocity there.
+ * The population opp(q) pulled from across the wall is replaced by
+ *   qw <= 1/2:  2qw f_q(x) + (1 - 2qw) f~_q + d
+ *   qw >  1/2:  s f_q(x) + (1 - s) f_opp(x) + d s,   s = 1/(2qw)
+ * with f the cell's previous post-collision populations, f~_q the pulled population q and
+ * d = -6 w_q (c_q . uw), links taken in the reference's pair order.  Each call replaces the previous list; n = 0 clears it.  The
+ * cells of the list must not be masked solid.  D2Q9 or D3Q19, BGK or TRT. */
+int kmi_set_cut_links(kmi_lattice* lat, int64_t n, const int32_t* cell, const int32_t* q,
+                      const double* qw, const double* uw);
 /* Replace the values (u, rho, T) of one face between steps; the face kind and thermal
  * kind must stay the same (time-dependent scalar conditions, handlers.jl:210-218). */
 int kmi_set_face(kmi_lattice* lat, int32_t face, const kmi_face* values);

@@ -1,4 +1,5 @@
 // kmi_api.hip — host implementation of the kraken_mi355x C ABI.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -25,6 +26,15 @@ struct kmi_lattice {
   uint8_t* mask = nullptr;
   double* uprev = nullptr;
   double* prof[4][2] = {};  // device per-node values of *_PROFILE faces
+  std::vector<uint8_t> solid_h;  // host copy of the solid mask (empty: none)
+  std::vector<int> cut_cells;    // cells holding cut links (sorted)
+  int* cut_slot = nullptr;       // dense per-cell slot of cut cells (device)
+  double* cut_rec = nullptr;     // per cut cell: Q wall fractions, Q × 3 wall velocities
+  long long nlinks = 0;          // cut-link list for the drag reductions (device)
+  int* link_cell = nullptr;
+  int* link_q = nullptr;
+  double* link_qw = nullptr;
+  double* link_uw = nullptr;
   int cur = 0;
   int64_t steps = 0;
   StepArgs args{};
@@ -174,6 +184,10 @@ static int validate(const kmi_desc& d, std::string& why, int& code) {
   if (d.thermal && d.collision != KMI_BGK)
     return code = KMI_ERR_UNSUPPORTED,
            why = "the thermal DDF uses BGK + Guo for the flow (thermal_2d.jl:421)", 1;
+  if (d.solid_mode != KMI_SOLID_BOUNCE_BACK && d.solid_mode != KMI_SOLID_INERT)
+    return why = "unknown solid mode", 1;
+  if (d.solid_mode == KMI_SOLID_INERT && d.thermal)
+    return code = KMI_ERR_UNSUPPORTED, why = "inert solids are for the flow populations only", 1;
   if (d.forcing < KMI_FORCE_NONE || d.forcing > KMI_FORCE_GUO_PLAIN_MOMENTS)
     return why = "unknown forcing", 1;
   if (d.forcing != KMI_FORCE_NONE &&
@@ -279,6 +293,9 @@ static void build_args(kmi_lattice* L) {
   a.T_ref = d.T_ref;
   a.coupling = d.coupling;
   a.forcing = d.forcing != KMI_FORCE_NONE;
+  a.solid_inert = d.solid_mode == KMI_SOLID_INERT;
+  a.cut_slot = L->cut_slot;
+  a.cut_rec = L->cut_rec;
   a.fx = d.force[0];
   a.fy = d.force[1];
   bool dir[4];
@@ -506,24 +523,111 @@ int kmi_lattice_destroy(kmi_lattice* L) {
   for (auto& pf : L->prof)
     for (double* p : pf)
       if (p) (void)hipFree(p);
+  for (void* p : {(void*)L->cut_slot, (void*)L->cut_rec, (void*)L->link_cell, (void*)L->link_q,
+                  (void*)L->link_qw, (void*)L->link_uw})
+    if (p) (void)hipFree(p);
   delete L;
   return KMI_OK;
+}
+
+// Device mask = solid flags combined with the cut-cell class; none when both are empty.
+static hipError_t upload_mask(kmi_lattice* L) {
+  if (L->solid_h.empty() && L->cut_cells.empty()) {
+    if (L->mask) (void)hipFree(L->mask);
+    L->mask = nullptr;
+    return hipSuccess;
+  }
+  std::vector<uint8_t> m(size_t(L->ncells), kCellFluid);
+  if (!L->solid_h.empty())
+    for (size_t c = 0; c < m.size(); ++c) m[c] = L->solid_h[c] ? kCellSolid : kCellFluid;
+  for (int c : L->cut_cells) m[size_t(c)] = kCellCut;
+  hipError_t e = hipSuccess;
+  if (!L->mask) e = hipMalloc(&L->mask, m.size());
+  if (e == hipSuccess) e = hipMemcpy(L->mask, m.data(), m.size(), hipMemcpyHostToDevice);
+  return e;
 }
 
 int kmi_set_mask(kmi_lattice* L, const uint8_t* is_solid) {
   if (!L) return fail(nullptr, KMI_ERR_ARG, "kmi_set_mask: lattice is NULL");
   kmi_ctx* c = L->ctx;
+  if (is_solid)
+    for (int cell : L->cut_cells)
+      if (is_solid[cell])
+        return fail(c, KMI_ERR_ARG, "kmi_set_mask: a cell holding cut links cannot be solid");
   KMI_HIP(c, hipSetDevice(c->device));
   KMI_HIP(c, hipStreamSynchronize(c->stream));
   drop_graphs(L);
-  if (!is_solid) {
-    if (L->mask) KMI_HIP(c, hipFree(L->mask));
-    L->mask = nullptr;
-  } else {
-    if (!L->mask) KMI_HIP(c, hipMalloc(&L->mask, size_t(L->ncells)));
-    KMI_HIP(c, hipMemcpy(L->mask, is_solid, size_t(L->ncells), hipMemcpyHostToDevice));
-  }
+  if (!is_solid) L->solid_h.clear();
+  else L->solid_h.assign(is_solid, is_solid + L->ncells);
+  KMI_HIP(c, upload_mask(L));
   build_args(L);
+  return KMI_OK;
+}
+
+int kmi_set_cut_links(kmi_lattice* L, int64_t n, const int32_t* cell, const int32_t* q,
+                      const double* qw, const double* uw) {
+  if (!L) return fail(nullptr, KMI_ERR_ARG, "kmi_set_cut_links: lattice is NULL");
+  kmi_ctx* c = L->ctx;
+  const int Q = L->Q;
+  if (n < 0 || (n > 0 && (!cell || !q || !qw || !uw)))
+    return fail(c, KMI_ERR_ARG, "kmi_set_cut_links: bad count or NULL array");
+  if (L->desc.thermal)
+    return fail(c, KMI_ERR_UNSUPPORTED, "kmi_set_cut_links: cut links act on the flow populations of "
+                                        "non-thermal lattices");
+  if (L->desc.collision == KMI_MRT)
+    return fail(c, KMI_ERR_UNSUPPORTED, "kmi_set_cut_links: the LI-BB step collides with BGK or TRT");
+  for (int64_t k = 0; k < n; ++k) {
+    if (cell[k] < 0 || cell[k] >= L->ncells || q[k] < 1 || q[k] >= Q)
+      return fail(c, KMI_ERR_ARG, "kmi_set_cut_links: link " + std::to_string(k) +
+                                      " has a cell or direction out of range");
+    if (!(qw[k] > 0.0 && qw[k] <= 1.0))
+      return fail(c, KMI_ERR_ARG, "kmi_set_cut_links: wall fraction must be in (0, 1]");
+    if (!L->solid_h.empty() && L->solid_h[size_t(cell[k])])
+      return fail(c, KMI_ERR_ARG, "kmi_set_cut_links: link " + std::to_string(k) +
+                                      " leaves a solid cell");
+  }
+  // per-cell records (duplicates: the last entry of a (cell, q) pair wins)
+  std::vector<int> cells(cell, cell + n);
+  std::sort(cells.begin(), cells.end());
+  cells.erase(std::unique(cells.begin(), cells.end()), cells.end());
+  std::vector<double> rec(cells.size() * 4 * Q, 0.0);
+  for (int64_t k = 0; k < n; ++k) {
+    const size_t s = size_t(std::lower_bound(cells.begin(), cells.end(), cell[k]) - cells.begin());
+    double* r = rec.data() + s * 4 * Q;
+    r[q[k]] = qw[k];
+    for (int a = 0; a < 3; ++a) r[Q + 3 * q[k] + a] = uw[3 * k + a];
+  }
+  KMI_HIP(c, hipSetDevice(c->device));
+  KMI_HIP(c, hipStreamSynchronize(c->stream));
+  drop_graphs(L);
+  for (void** p : {(void**)&L->cut_rec, (void**)&L->link_cell, (void**)&L->link_q,
+                   (void**)&L->link_qw, (void**)&L->link_uw}) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+  L->cut_cells = cells;
+  L->nlinks = n;
+  hipError_t e = hipSuccess;
+  if (!cells.empty()) {
+    std::vector<int> slot_h(size_t(L->ncells), -1);  // dense slots, read only for cut cells
+    for (size_t s = 0; s < cells.size(); ++s) slot_h[size_t(cells[s])] = int(s);
+    if (!L->cut_slot) e = hipMalloc(&L->cut_slot, size_t(L->ncells) * sizeof(int));
+    if (e == hipSuccess)
+      e = hipMemcpy(L->cut_slot, slot_h.data(), slot_h.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&L->cut_rec, rec.size() * sizeof(double));
+    if (e == hipSuccess) e = hipMemcpy(L->cut_rec, rec.data(), rec.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&L->link_cell, size_t(n) * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&L->link_q, size_t(n) * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&L->link_qw, size_t(n) * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&L->link_uw, size_t(n) * 3 * sizeof(double));
+    if (e == hipSuccess) e = hipMemcpy(L->link_cell, cell, size_t(n) * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(L->link_q, q, size_t(n) * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(L->link_qw, qw, size_t(n) * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(L->link_uw, uw, size_t(n) * 3 * sizeof(double), hipMemcpyHostToDevice);
+  }
+  if (e == hipSuccess) e = upload_mask(L);
+  build_args(L);
+  if (e != hipSuccess) return hip_fail(c, e, "kmi_set_cut_links");
   return KMI_OK;
 }
 
@@ -851,6 +955,23 @@ int kmi_reduce(kmi_lattice* L, int32_t kind, int32_t arg, double* out) {
       KMI_HIP(c, hipMemcpyAsync(r, c->out, sizeof(double), hipMemcpyDeviceToHost, c->stream));
       KMI_HIP(c, hipStreamSynchronize(c->stream));
       out[0] = r[0] / double(hi - lo);
+      return KMI_OK;
+    }
+    case KMI_RED_DRAG_MEA:
+    case KMI_RED_DRAG_HALFWAY: {
+      if (L->nlinks > 0) {
+        const kmi_desc& d = L->desc;
+        hipError_t e = launch_drag(L->Q, d.storage, L->f[L->cur], L->nlinks, L->link_cell,
+                                   L->link_q, L->link_qw, L->link_uw, int(d.nx), int(d.ny),
+                                   L->args.nz, L->plane, kind == KMI_RED_DRAG_MEA, c->partials,
+                                   c->out, c->stream);
+        if (e != hipSuccess) return hip_fail(c, e, "drag kernel");
+        KMI_HIP(c, hipMemcpyAsync(r, c->out, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        KMI_HIP(c, hipStreamSynchronize(c->stream));
+      }
+      out[0] = r[0];
+      out[1] = r[1];
+      out[2] = r[2];
       return KMI_OK;
     }
     default:

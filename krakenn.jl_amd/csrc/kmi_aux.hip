@@ -214,7 +214,7 @@ __global__ void __launch_bounds__(kRedBlock)
       case KMI_RED_THERMAL: acc[0] += t; break;
       case KMI_RED_MAX_SPEED: acc[0] = fmax(acc[0], sqrt(u * u + v * v + w * w)); break;
       case 100:
-        if (!(mask && mask[c])) {
+        if (!(mask && mask[c] == kCellSolid)) {
           const double du = u - uprev[c], dv = v - uprev[c + ncells], dw = w - uprev[c + 2 * ncells];
           acc[0] += du * du + dv * dv + dw * dw;
           acc[1] += u * u + v * v + w * w;
@@ -266,6 +266,75 @@ __global__ void __launch_bounds__(kRedBlock)
     acc[0] += -H * (-8 * Th + 9 * T1 - T2) / 3.0 / dT;
   }
   block_reduce<1>(acc, false, partials);
+}
+
+// Force on the cut-link walls, accumulated in FP64 over the link list.
+//   mea = 1: Mei–Luo–Shyy momentum exchange with the Bouzidi arriving population,
+//            per link c_q (f_q + f~_qbar)   (_compute_drag_libb_mei_2d_host,
+//            cylinder_libb.jl:61-104; drag_gpu.jl:346-376 for the link form);
+//   mea = 0: halfway-BB exchange 2 c_q f_q   (cylinder_libb.jl:27-42, li_bb_3d_v2.jl:183-201).
+// f is the stored (post-collision) field.  The neighbour behind the link is looked up inside
+// the domain only, falling back to the cell's own opposite population, as the reference does.
+template <int Q, int ST>
+__global__ void __launch_bounds__(kRedBlock)
+    k_drag(const void* f_, long long nlinks, const int* __restrict__ cell,
+           const int* __restrict__ qv, const double* __restrict__ qwv,
+           const double* __restrict__ uwv, int nx, int ny, int nz, long long plane, int mea,
+           double* partials) {
+  using S = Store<ST>;
+  using E = typename S::elem;
+  const E* f = static_cast<const E*>(f_);
+  auto full = [&](long long idx, int q) {
+    double v = double(S::ld(f, idx + q * plane));
+    if constexpr (S::shifted) {
+      double w = 0.0;
+      sfor<Q>([&](auto qc) { if (decltype(qc)::value == q) w = Lat<Q>::w[decltype(qc)::value]; });
+      v = w + v;
+    }
+    return v;
+  };
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x; n < nlinks;
+       n += (long long)gridDim.x * blockDim.x) {
+    const long long c = cell[n];
+    const int q = qv[n];
+    int cx = 0, cy = 0, cz = 0, o = 0;
+    double w = 0.0;
+    sfor<Q>([&](auto qc) {
+      constexpr int k = decltype(qc)::value;
+      if (k == q) {
+        cx = Lat<Q>::cx[k], cy = Lat<Q>::cy[k], cz = Lat<Q>::cz[k], o = Lat<Q>::opp[k];
+        w = Lat<Q>::w[k];
+      }
+    });
+    const double fq = full(c, q);
+    double F;
+    if (!mea) {
+      F = 2.0 * fq;
+    } else {
+      const int i = int(c % nx), j = int((c / nx) % ny), k = int(c / ((long long)nx * ny));
+      const int im = i - cx, jm = j - cy, km = k - cz;
+      const bool in = im >= 0 && im < nx && jm >= 0 && jm < ny && km >= 0 && km < nz;
+      const double back = in ? full(im + (long long)nx * (jm + (long long)ny * km), q) : full(c, o);
+      const double bar = full(c, o);
+      const double qw = qwv[n];
+      const double* u = uwv + 3 * n;
+      const double cu = double(cx) * u[0] + double(cy) * u[1] + (Q == 19 ? double(cz) * u[2] : 0.0);
+      const double d = -6.0 * w * cu;
+      double arriving;
+      if (qw <= 0.5) {
+        arriving = 2 * qw * fq + (1 - 2 * qw) * back + d;
+      } else {
+        const double inv2q = 1.0 / (2 * qw);
+        arriving = inv2q * fq + (1 - inv2q) * bar + inv2q * d;
+      }
+      F = fq + arriving;
+    }
+    acc[0] += double(cx) * F;
+    acc[1] += double(cy) * F;
+    acc[2] += double(cz) * F;
+  }
+  block_reduce<3>(acc, false, partials);
 }
 
 // Combine `nb` partials of NV values each, in order, with Neumaier compensation.
@@ -364,6 +433,18 @@ hipError_t launch_reduce_cells(int Q, int st, const void* f, const void* g, cons
   if (e != hipSuccess || kind == 101) return e;
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, s, partials, nb, 3,
                      kind == KMI_RED_MAX_SPEED ? 1 : 0, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_drag(int Q, int st, const void* f, long long nlinks, const int* cell,
+                       const int* q, const double* qw, const double* uw, int nx, int ny, int nz,
+                       long long plane, int mea, double* partials, double* out, hipStream_t s) {
+  const int nb = reduce_groups(nlinks);
+  if (Q == 9) { KMI_ST_SWITCH(st, hipLaunchKernelGGL((k_drag<9, ST>), dim3(nb), dim3(kRedBlock), 0, s, f, nlinks, cell, q, qw, uw, nx, ny, nz, plane, mea, partials)); }
+  else { KMI_ST_SWITCH(st, hipLaunchKernelGGL((k_drag<19, ST>), dim3(nb), dim3(kRedBlock), 0, s, f, nlinks, cell, q, qw, uw, nx, ny, nz, plane, mea, partials)); }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, s, partials, nb, 3, 0, out);
   return hipGetLastError();
 }
 

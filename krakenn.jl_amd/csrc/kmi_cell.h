@@ -380,6 +380,51 @@ KMI_DEV void zh3_apply(R (&f)[19], const FaceArgs& Fa) {
     zh3_pressure<SH, F>(f, R(Fa.rho), R(Fa.rho - 1.0));
 }
 
+// ---- interpolated bounce-back on cut links (LI-BB V2 pre-phase) ------------------------
+// Order in which the reference's pre-phase visits the links: opposite pairs, q before
+// opp(q) (dsl/bricks.jl:518-581, dsl/bricks_3d.jl:368-473).  A pair with both links cut
+// substitutes the second population from the first substitution, as the reference does.
+template <int Q>
+struct LibbOrder;
+template <>
+struct LibbOrder<9> {
+  static constexpr int q[8] = {1, 3, 2, 4, 5, 7, 6, 8};
+};
+template <>
+struct LibbOrder<19> {
+  static constexpr int q[18] = {1, 2, 3, 4, 5, 6, 7, 10, 8, 9, 11, 14, 12, 13, 15, 18, 16, 17};
+};
+
+// _libb_branch (li_bb_2d.jl:70-83).
+template <class R>
+KMI_DEV R libb_branch(R qw, R here, R back, R bar, R d) {
+  if (qw <= R(0.5)) return R(2) * qw * here + (R(1) - R(2) * qw) * back + d;
+  const R inv = R(1) / (R(2) * qw);
+  return inv * here + (R(1) - inv) * bar + d * inv;
+}
+
+// fp: pulled populations of a cut cell; fi/self/P locate the cell's previous populations;
+// rec: the cell's record (Q wall fractions, Q × 3 wall velocities).  The rule is linear with
+// coefficients summing to one, so shifted storage applies it to the deviations unchanged.
+template <int Q, class S, class R, class E>
+KMI_DEV void libb_prephase(R (&fp)[Q], const E* __restrict__ fi, long long self, long long P,
+                           const double* __restrict__ rec) {
+  sfor<Q - 1>([&](auto nc) {
+    constexpr int q = LibbOrder<Q>::q[decltype(nc)::value];
+    constexpr int o = Lat<Q>::opp[q];
+    constexpr int cx = Lat<Q>::cx[q], cy = Lat<Q>::cy[q], cz = Lat<Q>::cz[q];
+    constexpr bool axis = cx * cx + cy * cy + cz * cz == 1;
+    // 6 w_q: 2/3 (D2Q9 axis), 1/3 (D3Q19 axis), 1/6 (diagonals), written as the reference
+    constexpr double six_wq = axis ? (Q == 9 ? 2.0 / 3.0 : 1.0 / 3.0) : 1.0 / 6.0;
+    const R qw = R(rec[q]);
+    if (qw > R(0)) {
+      const double* u = rec + Q + 3 * q;
+      const R d = R(-six_wq) * cdotu<cx, cy, cz>(R(u[0]), R(u[1]), R(u[2]));
+      fp[o] = libb_branch(qw, S::ld(fi, self + q * P), fp[q], S::ld(fi, self + o * P), d);
+    }
+  });
+}
+
 // Anti-bounce-back of one slot (thermal_2d.jl:175-178); `tl` is T_link (minus T0 when shifted).
 template <class R>
 KMI_DEV R abb(R gq, R w2, R tface, bool on_corner, R tcorner, bool skip) {

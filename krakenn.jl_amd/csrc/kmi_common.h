@@ -215,7 +215,15 @@ struct StepArgs {
   int coupling;
   int forcing;      // 1: Guo body force (any kmi_forcing other than NONE)
   double fx, fy;    // body force (KMI_FORCE_GUO)
+  int solid_inert;  // masked cells hold the rest equilibrium (KMI_SOLID_INERT)
+  // Cut links: mask value 2 marks a fluid cell with cut links; cut_slot[cell] indexes its
+  // record in cut_rec: Q wall fractions (0: link not cut), then Q × 3 wall velocities.
+  const int* cut_slot;
+  const double* cut_rec;
   FaceArgs face[6];
 };
+
+// Cell classes held in the mask byte.
+enum : unsigned char { kCellFluid = 0, kCellSolid = 1, kCellCut = 2 };
 
 }  // namespace kmi

@@ -44,6 +44,9 @@ hipError_t launch_reduce_cells(int Q, int st, const void* f, const void* g, cons
                                double* uprev, long long ncells, long long plane, double T0,
                                double fx, double fy, int kind, double* partials, double* out,
                                hipStream_t s);
+hipError_t launch_drag(int Q, int st, const void* f, long long nlinks, const int* cell,
+                       const int* q, const double* qw, const double* uw, int nx, int ny, int nz,
+                       long long plane, int mea, double* partials, double* out, hipStream_t s);
 hipError_t launch_nusselt(int st, const void* g, int nx, int ny, long long plane, double T0,
                           int face, double Th, double dT, int lo, int hi, double* partials,
                           double* out, hipStream_t s);

@@ -150,7 +150,11 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs a) {
     }
   }
 
-  const bool solid = BND && a.mask != nullptr && a.mask[self] != 0;
+  const unsigned char cls = (BND && a.mask != nullptr) ? a.mask[self] : kCellFluid;
+  const bool solid = cls == kCellSolid;
+  if constexpr (BND) {
+    if (cls == kCellCut) libb_prephase<Q, S>(fp, fi, self, P, a.cut_rec + (long long)a.cut_slot[self] * (4 * Q));
+  }
 
   if constexpr (TH) {
     static_assert(Q == 9, "thermal DDF is D2Q9");
@@ -193,7 +197,10 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs a) {
       GS::template st<NTS>(go, self + q * P, go_r[q]);
     });
   } else {
-    if (solid) {
+    if (solid && a.solid_inert) {
+      // SolidInert (dsl/bricks.jl:133-154): rest equilibrium, w_q (0 as a deviation)
+      sfor<Q>([&](auto qc) { fo_r[decltype(qc)::value] = SH ? R(0) : R(Lat<Q>::w[decltype(qc)::value]); });
+    } else if (solid) {
       sfor<Q>([&](auto qc) { fo_r[decltype(qc)::value] = fp[Lat<Q>::opp[decltype(qc)::value]]; });
     } else if constexpr (COLL == KMI_BGK) {
       bool forced = false;

@@ -31,6 +31,7 @@ struct KmiDesc
     coupling::Int32; n_bc_order::Int32; bc_order::NTuple{6,Int32}
     faces::NTuple{6,KmiFace}
     forcing::Int32; force::NTuple{3,Float64}
+    solid_mode::Int32
 end
 
 const FACE_PERIODIC, FACE_WALL, FACE_MOVING_WALL, FACE_ZOUHE_VELOCITY, FACE_ZOUHE_PRESSURE = 0, 1, 2, 3, 4
@@ -114,7 +115,7 @@ function desc_2d(Nx, Ny, ν, T; faces)
     sp = 1 / (3ν + 0.5); sm = 1 / ((3 / 16) / (3ν) + 0.5)
     KmiDesc(0, STORAGE[T], 0, 0, Nx, Ny, 1, 1 / (3ν + 0.5), sp, sm, 1.4, 1.4, 1.2,
             1 / (3ν + 0.5), 1 / (3ν + 0.5), 0.0, 0.0, 0, 0, (0, 0, 0, 0, 0, 0), Tuple(fs),
-            0, (0.0, 0.0, 0.0))
+            0, (0.0, 0.0, 0.0), 0)
 end
 
 # ---- driver methods for the backend (src/drivers/basic.jl:112-144) ------------------------

@@ -4,7 +4,7 @@ The solver runs in hand-written CDNA4 HIP kernels (libkraken_mi355x.so, C ABI in
 include/kraken_mi355x.h).  This package is the host-side mirror of the reference's
 driver API; it never computes populations itself.
 """
-from . import _lib
+from . import _lib, libb
 from ._lib import KrakenError, load
 from .batch import Batch, STATUS_NAMES, ensemble_parameters
 from .drivers import (
@@ -35,12 +35,15 @@ from .drivers import (
     trt_rates,
 )
 from .lattice import ArgumentError, Lattice, MI355XBackend, make_desc
+from .libb import (cut_links_from_dense, precompute_q_wall_cylinder, precompute_q_wall_sphere_3d,
+                   wall_velocity_rotating_cylinder)
 from .runner import (BoundarySetup, DomainSetup, GeometryRegion, InitialSetup, PhysicsSetup,
                      SimulationSetup, plan_simulation, run_simulation)
 
 __all__ = [
     "BoundarySetup", "DomainSetup", "GeometryRegion", "InitialSetup", "PhysicsSetup", "SimulationSetup",
-    "plan_simulation", "run_simulation",
+    "plan_simulation", "run_simulation", "cut_links_from_dense", "precompute_q_wall_cylinder",
+    "precompute_q_wall_sphere_3d", "wall_velocity_rotating_cylinder",
     "Batch", "STATUS_NAMES", "ensemble_parameters", "ArgumentError", "D2Q9", "D3Q19", "KrakenError", "LBMConfig", "Lattice", "MI355XBackend",
     "cavity_3d_desc", "cavity_desc", "lbm_config", "load", "make_desc", "natural_convection_desc",
     "natural_convection_initial_temperature", "omega", "rayleigh_benard_desc",

@@ -26,7 +26,9 @@ THERMAL_ADIABATIC, THERMAL_DIRICHLET = 0, 1
 COUPLING_SPLIT, COUPLING_FUSED = 0, 1
 FIELD_F, FIELD_G = 0, 1
 HOST_F64, HOST_F32 = 0, 1
-RED_MASS, RED_MOMENTUM, RED_KINETIC_ENERGY, RED_THERMAL, RED_NUSSELT, RED_MAX_SPEED = range(6)
+(RED_MASS, RED_MOMENTUM, RED_KINETIC_ENERGY, RED_THERMAL, RED_NUSSELT, RED_MAX_SPEED, RED_DRAG_MEA,
+ RED_DRAG_HALFWAY) = range(8)
+SOLID_BOUNCE_BACK, SOLID_INERT = 0, 1
 
 STORAGE_NAMES = {"f64": STORE_F64, "f32": STORE_F32, "f32s": STORE_F32S, "f16s": STORE_F16S}
 COLLISION_NAMES = {"bgk": BGK, "trt": TRT, "mrt": MRT}
@@ -68,6 +70,7 @@ class KmiDesc(ctypes.Structure):
         ("faces", KmiFace * 6),
         ("forcing", c_int32),
         ("force", c_double * 3),
+        ("solid_mode", c_int32),
     ]
 
 
@@ -104,6 +107,8 @@ SIGNATURES = {
     "kmi_set_mask": (c_int, [c_void_p, POINTER(c_uint8)]),
     "kmi_set_face_profile": (c_int, [c_void_p, c_int32, POINTER(c_double), POINTER(c_double)]),
     "kmi_set_face": (c_int, [c_void_p, c_int32, POINTER(KmiFace)]),
+    "kmi_set_cut_links": (c_int, [c_void_p, c_int64, POINTER(c_int32), POINTER(c_int32), POINTER(c_double),
+                                  POINTER(c_double)]),
     "kmi_upload": (c_int, [c_void_p, c_int32, c_void_p, c_int32]),
     "kmi_download": (c_int, [c_void_p, c_int32, c_void_p, c_int32]),
     "kmi_init_equilibrium": (c_int, [c_void_p] + [POINTER(c_double)] * 4),

@@ -65,10 +65,12 @@ class MI355XBackend:
 def make_desc(stencil, nx, ny, nz=1, nu=0.1, storage="f64", collision="bgk", faces=None,
               thermal=False, omega=None, s_plus=None, s_minus=None, Lambda=3.0 / 16.0,
               s_e=1.4, s_eps=1.4, s_q=1.2, omega_T=None, beta_g=0.0, T_ref=0.0,
-              coupling="split", bc_order=None, forcing=None, force=(0.0, 0.0, 0.0)) -> L.KmiDesc:
+              coupling="split", bc_order=None, forcing=None, force=(0.0, 0.0, 0.0),
+              solid_mode="bounce_back") -> L.KmiDesc:
     """Build a kmi_desc.  `faces` maps face name -> dict(kind=..., u=..., rho=..., T=...,
     thermal='dirichlet'|'adiabatic').  forcing: None, 'guo' (moments with the F/2 shift) or
-    'guo_plain' (moments without it), with the body force `force`."""
+    'guo_plain' (moments without it), with the body force `force`.  solid_mode: masked cells
+    swap their populations ('bounce_back') or hold the rest equilibrium ('inert', LI-BB)."""
     st = {"d2q9": L.D2Q9, "d3q19": L.D3Q19}.get(str(stencil).lower(), stencil)
     d = L.desc_init(st, nx, ny, nz, nu)
     d.storage = L.STORAGE_NAMES[storage] if isinstance(storage, str) else storage
@@ -96,6 +98,7 @@ def make_desc(stencil, nx, ny, nz=1, nu=0.1, storage="f64", collision="bgk", fac
     fv = list(force) + [0.0, 0.0, 0.0]
     for c in range(3):
         d.force[c] = float(fv[c])
+    d.solid_mode = {"bounce_back": L.SOLID_BOUNCE_BACK, "inert": L.SOLID_INERT}[solid_mode]
     return d
 
 
@@ -275,7 +278,24 @@ class Lattice:
     def reduce(self, kind, arg=0):
         out = (c_double * 3)()
         k = {"mass": L.RED_MASS, "momentum": L.RED_MOMENTUM, "kinetic_energy": L.RED_KINETIC_ENERGY,
-             "thermal": L.RED_THERMAL, "nusselt": L.RED_NUSSELT, "max_speed": L.RED_MAX_SPEED}.get(kind, kind)
+             "thermal": L.RED_THERMAL, "nusselt": L.RED_NUSSELT, "max_speed": L.RED_MAX_SPEED,
+             "drag_mea": L.RED_DRAG_MEA, "drag_halfway": L.RED_DRAG_HALFWAY}.get(kind, kind)
         a = L.FACE_NAMES[arg] if isinstance(arg, str) else arg
         self._chk(L.load().kmi_reduce(self._h, k, a, out))
-        return tuple(out) if k == L.RED_MOMENTUM else out[0]
+        if k in (L.RED_MOMENTUM, L.RED_DRAG_MEA, L.RED_DRAG_HALFWAY):
+            return tuple(out)
+        return out[0]
+
+    def set_cut_links(self, cell, q, qw, uw=None):
+        """Interpolated bounce-back links: cell index (x fastest), direction q (0-based),
+        wall fraction in (0, 1], wall velocity (n × 3, default zero)."""
+        cell = np.ascontiguousarray(cell, dtype=np.int32)
+        q = np.ascontiguousarray(q, dtype=np.int32)
+        qw = np.ascontiguousarray(qw, dtype=np.float64)
+        n = len(cell)
+        uw = np.zeros((n, 3)) if uw is None else np.asarray(uw, dtype=np.float64).reshape(n, -1)
+        if uw.shape[1] < 3:
+            uw = np.hstack([uw, np.zeros((n, 3 - uw.shape[1]))])
+        uw = np.ascontiguousarray(uw)
+        ip = lambda a: a.ctypes.data_as(POINTER(ctypes.c_int32))
+        self._chk(L.load().kmi_set_cut_links(self._h, n, ip(cell), ip(q), _dptr(qw), _dptr(uw)))

@@ -1194,6 +1194,153 @@ static void generic_step(const kmi_desc& d, const uint8_t* solid, T*& f, T*& f2,
   std::swap(f, f2);
 }
 
+// ---- LI-BB V2 (li_bb_2d_v2.jl:35-110, li_bb_3d_v2.jl:13-43) --------------------------------
+// _libb_branch — li_bb_2d.jl:70-83
+template <class T>
+static T libb_branch(T q_w, T f_post_here, T f_post_back, T f_bar_post_here, T delta) {
+  const T half = T(0.5);
+  if (q_w <= half) return T(2) * q_w * f_post_here + (T(1) - T(2) * q_w) * f_post_back + delta;
+  const T inv_two_q = T(1) / (T(2) * q_w);
+  return inv_two_q * f_post_here + (T(1) - inv_two_q) * f_bar_post_here + delta * inv_two_q;
+}
+
+// One flagged link of the pre-phase brick: 1-based q, its opposite, and the moving-wall
+// term as the brick writes it: sign · coef · (s_a u_a  ±  s_b u_b) with u the link's wall
+// velocity components (0 x, 1 y, 2 z; b = -1: single component).
+struct LibbLink {
+  int q, qbar, neg;       // neg: the leading constant is −coef
+  double coef;
+  int a, a_neg, b, b_sub;  // (±u_a) then (+ or −) u_b
+};
+// ApplyLiBBPrePhase — dsl/bricks.jl:518-581 (pairs (2,4), (3,5), (6,8), (7,9))
+static const LibbLink LIBB9[8] = {
+    {2, 4, 1, 2.0 / 3.0, 0, 0, -1, 0}, {4, 2, 0, 2.0 / 3.0, 0, 0, -1, 0},
+    {3, 5, 1, 2.0 / 3.0, 1, 0, -1, 0}, {5, 3, 0, 2.0 / 3.0, 1, 0, -1, 0},
+    {6, 8, 1, 1.0 / 6.0, 0, 0, 1, 0},  {8, 6, 0, 1.0 / 6.0, 0, 0, 1, 0},
+    {7, 9, 1, 1.0 / 6.0, 0, 1, 1, 0},  {9, 7, 0, 1.0 / 6.0, 0, 1, 1, 0}};
+// ApplyLiBBPrePhase_3D — dsl/bricks_3d.jl:368-473
+static const LibbLink LIBB19[18] = {
+    {2, 3, 1, 1.0 / 3.0, 0, 0, -1, 0},   {3, 2, 0, 1.0 / 3.0, 0, 0, -1, 0},
+    {4, 5, 1, 1.0 / 3.0, 1, 0, -1, 0},   {5, 4, 0, 1.0 / 3.0, 1, 0, -1, 0},
+    {6, 7, 1, 1.0 / 3.0, 2, 0, -1, 0},   {7, 6, 0, 1.0 / 3.0, 2, 0, -1, 0},
+    {8, 11, 1, 1.0 / 6.0, 0, 0, 1, 0},   {11, 8, 0, 1.0 / 6.0, 0, 0, 1, 0},
+    {9, 10, 1, 1.0 / 6.0, 0, 1, 1, 0},   {10, 9, 1, 1.0 / 6.0, 0, 0, 1, 1},
+    {12, 15, 1, 1.0 / 6.0, 0, 0, 2, 0},  {15, 12, 0, 1.0 / 6.0, 0, 0, 2, 0},
+    {13, 14, 1, 1.0 / 6.0, 0, 1, 2, 0},  {14, 13, 1, 1.0 / 6.0, 0, 0, 2, 1},
+    {16, 19, 1, 1.0 / 6.0, 1, 0, 2, 0},  {19, 16, 0, 1.0 / 6.0, 1, 0, 2, 0},
+    {17, 18, 1, 1.0 / 6.0, 1, 1, 2, 0},  {18, 17, 1, 1.0 / 6.0, 1, 0, 2, 1}};
+
+// Pre-phase substitution on the pulled field f2 (fluid cells), reading the previous field fi;
+// q_wall / uw arrays dense in the reference layout [cell + (q−1)P].
+template <class T>
+static void libb_prephase(const kmi_desc& d, T* f2, const T* fi, const uint8_t* solid,
+                          const T* q_wall, const T* const* uw) {
+  const bool d3 = d.stencil == KMI_D3Q19;
+  const size_t P = size_t(d.nx * d.ny * (d3 ? d.nz : 1));
+  const LibbLink* tab = d3 ? LIBB19 : LIBB9;
+  const int nl = d3 ? 18 : 8;
+#pragma omp parallel for
+  for (int64_t c = 0; c < int64_t(P); ++c) {
+    if (solid && solid[c]) continue;
+    for (int n = 0; n < nl; ++n) {
+      const LibbLink& L = tab[n];
+      const size_t iq = c + size_t(L.q - 1) * P, ib = c + size_t(L.qbar - 1) * P;
+      const T qw = q_wall[iq];
+      if (!(qw > T(0))) continue;
+      T ua = uw[L.a][iq];
+      if (L.a_neg) ua = -ua;
+      T comb = ua;
+      if (L.b >= 0) comb = L.b_sub ? ua - uw[L.b][iq] : ua + uw[L.b][iq];
+      const T k = L.neg ? -T(L.coef) : T(L.coef);
+      const T delta = k * comb;
+      f2[ib] = libb_branch(qw, fi[iq], f2[iq], fi[ib], delta);
+    }
+  }
+}
+
+// fused_trt_libb_v2_step! chain on the descriptor's faces: pull (halfway BB / periodic),
+// pre-phase on fluid cells, collision, SolidInert rest equilibrium on solid cells.
+template <class T>
+static void libb_step(const kmi_desc& d, const uint8_t* solid, T*& f, T*& f2, const T* q_wall,
+                      const T* const* uw) {
+  const bool d3 = d.stencil == KMI_D3Q19;
+  const int64_t nx = d.nx, ny = d.ny, nz = d3 ? d.nz : 1;
+  const size_t P = size_t(nx * ny * nz);
+  pull_generic(d, f2, f);
+  zou_he_faces(d, f2);
+  libb_prephase(d, f2, f, solid, q_wall, uw);
+  if (!d3) {
+    if (d.collision == KMI_BGK) collide_2d(f2, (const uint8_t*)nullptr, nx, ny, T(d.omega));
+    else collide_trt_2d(f2, (const uint8_t*)nullptr, nx, ny, T(d.s_plus), T(d.s_minus));
+  } else {
+    if (d.collision == KMI_BGK) collide_3d(f2, (const uint8_t*)nullptr, nx, ny, nz, T(d.omega));
+    else collide_trt_3d(f2, (const uint8_t*)nullptr, nx, ny, nz, T(d.s_plus), T(d.s_minus));
+  }
+  const int Q = d3 ? 19 : 9;
+  const double* w = d3 ? W19 : W9;
+  for (size_t c = 0; c < P; ++c)  // SolidInert — dsl/bricks.jl:133-154: feq(ρ=1, u=0) = w_q
+    if (solid && solid[c])
+      for (int q = 0; q < Q; ++q) f2[c + q * P] = T(w[q]) * T(1) * (T(1) + T(0) + T(0) - T(0));
+  std::swap(f, f2);
+}
+
+// _compute_drag_libb_mei_2d_host — cylinder_libb.jl:61-104 (FP64 accumulation)
+template <class T>
+static void drag_mei_2d(const T* f, const T* qw, const T* uwx, const T* uwy, int64_t Nx,
+                        int64_t Ny, double* out) {
+  const size_t P = size_t(Nx * Ny);
+  double Fx = 0.0, Fy = 0.0;
+  for (int64_t j = 0; j < Ny; ++j)
+    for (int64_t i = 0; i < Nx; ++i)
+      for (int q = 1; q < 9; ++q) {
+        const size_t c = I2(i, j, Nx);
+        const double q_w = double(qw[c + q * P]);
+        if (!(q_w > 0)) continue;
+        const int qbar = OPP9[q];
+        const int64_t im = i - CX9[q], jm = j - CY9[q];
+        const double back = (im >= 0 && im < Nx && jm >= 0 && jm < Ny)
+                                ? double(f[I2(im, jm, Nx) + q * P])
+                                : double(f[c + qbar * P]);
+        const double here = double(f[c + q * P]), bar = double(f[c + qbar * P]);
+        const double w_q = (q <= 4) ? 1.0 / 9.0 : 1.0 / 36.0;
+        const double cu = double(CX9[q]) * double(uwx[c + q * P]) + double(CY9[q]) * double(uwy[c + q * P]);
+        const double delta = -6.0 * w_q * cu;
+        double arriving;
+        if (q_w <= 0.5) {
+          arriving = 2 * q_w * here + (1 - 2 * q_w) * back + delta;
+        } else {
+          const double inv2q = 1.0 / (2 * q_w);
+          arriving = inv2q * here + (1 - inv2q) * bar + inv2q * delta;
+        }
+        const double F = here + arriving;
+        Fx += double(CX9[q]) * F;
+        Fy += double(CY9[q]) * F;
+      }
+  out[0] = Fx;
+  out[1] = Fy;
+}
+
+// _compute_drag_libb_3d_host — li_bb_3d_v2.jl:183-201 (2D: compute_drag_libb_2d,
+// cylinder_libb.jl:27-42): Σ 2 c_q f_q over cut links.
+template <class T>
+static void drag_halfway(const kmi_desc& d, const T* f, const T* qw, double* out) {
+  const bool d3 = d.stencil == KMI_D3Q19;
+  const size_t P = size_t(d.nx * d.ny * (d3 ? d.nz : 1));
+  const int Q = d3 ? 19 : 9;
+  const int* cx = d3 ? CX19 : CX9;
+  const int* cy = d3 ? CY19 : CY9;
+  double F[3] = {0, 0, 0};
+  for (size_t c = 0; c < P; ++c)
+    for (int q = 1; q < Q; ++q)
+      if (qw[c + q * P] > 0) {
+        const double fv = double(f[c + q * P]);
+        F[0] += 2.0 * double(cx[q]) * fv;
+        F[1] += 2.0 * double(cy[q]) * fv;
+        if (d3) F[2] += 2.0 * double(CZ19[q]) * fv;
+      }
+  out[0] = F[0], out[1] = F[1], out[2] = F[2];
+}
+
 // Neumaier-compensated sum.
 struct KSum {
   double s = 0.0, c = 0.0;
@@ -1362,7 +1509,27 @@ extern "C" {
     ko::zou_he_pressure_east_spatial(f, rho, nx, ny);                                    \
   }
 
+#define KO_LIBB(S, T)                                                                     \
+  void ko_run_libb_##S(const kmi_desc* d, const uint8_t* solid, T* f, const T* qw,       \
+                       const T* uwx, const T* uwy, const T* uwz, int64_t nsteps) {        \
+    const int Q = d->stencil == KMI_D2Q9 ? 9 : 19;                                       \
+    const size_t n = size_t(d->nx * d->ny * (d->stencil == KMI_D2Q9 ? 1 : d->nz)) * Q;   \
+    std::vector<T> b1(n);                                                                \
+    const T* uw[3] = {uwx, uwy, uwz};                                                    \
+    T *fa = f, *fb = b1.data();                                                          \
+    for (int64_t s = 0; s < nsteps; ++s) ko::libb_step(*d, solid, fa, fb, qw, uw);       \
+    if (fa != f) std::memcpy(f, fa, n * sizeof(T));                                      \
+  }                                                                                      \
+  void ko_drag_mei_2d_##S(const T* f, const T* qw, const T* uwx, const T* uwy,          \
+                          int64_t nx, int64_t ny, double* out) {                         \
+    ko::drag_mei_2d(f, qw, uwx, uwy, nx, ny, out);                                       \
+  }                                                                                      \
+  void ko_drag_halfway_##S(const kmi_desc* d, const T* f, const T* qw, double* out) {   \
+    ko::drag_halfway(*d, f, qw, out);                                                    \
+  }
+
 KO_BOTH(KO_STREAM)
+KO_BOTH(KO_LIBB)
 
 /* Hot-wall Nusselt number — thermal.jl:225-234 (west wall, rows 2..Ny-1), generalised to
  * the hot face `face`; `periodic_parallel` != 0 averages over every wall-parallel index. */

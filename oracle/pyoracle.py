@@ -108,6 +108,40 @@ def run_profiles(desc, f, profiles, nsteps=1, mask=None):
     return f
 
 
+def run_libb(desc, f, solid, q_wall, uw=(None, None, None), nsteps=1):
+    """fused_trt_libb_v2_step! chain (LI-BB pre-phase, inert solids) on the descriptor's faces;
+    q_wall and the wall velocities are dense arrays of f's element type and shape."""
+    z = np.zeros_like(q_wall, order="F")
+    arrs = [np.asfortranarray(a if a is not None else z, dtype=f.dtype) for a in uw]
+    qw = np.asfortranarray(q_wall, dtype=f.dtype)
+    m = None if solid is None else np.asfortranarray(solid, dtype=np.uint8)
+    fn = getattr(lib(), f"ko_run_libb_{_sfx(f)}")
+    fn.restype = None
+    fn(ctypes.byref(desc), None if m is None else m.ctypes.data_as(c_void_p), _p(f), _p(qw),
+       _p(arrs[0]), _p(arrs[1]), _p(arrs[2]), c_int64(int(nsteps)))
+    return f
+
+
+def drag_mei_2d(f, q_wall, uwx=None, uwy=None):
+    z = np.zeros_like(q_wall, order="F")
+    out = np.zeros(2)
+    nx, ny, _ = f.shape
+    fn = getattr(lib(), f"ko_drag_mei_2d_{_sfx(f)}")
+    fn.restype = None
+    a = [np.asfortranarray(v if v is not None else z, dtype=f.dtype) for v in (q_wall, uwx, uwy)]
+    fn(_p(f), _p(a[0]), _p(a[1]), _p(a[2]), c_int64(nx), c_int64(ny), _p(out))
+    return out
+
+
+def drag_halfway(desc, f, q_wall):
+    out = np.zeros(3)
+    fn = getattr(lib(), f"ko_drag_halfway_{_sfx(f)}")
+    fn.restype = None
+    qw = np.asfortranarray(q_wall, dtype=f.dtype)
+    fn(ctypes.byref(desc), _p(f), _p(qw), _p(out))
+    return out
+
+
 def moments_forced(f, fx, fy):
     """compute_macroscopic_forced_2d of a D2Q9 population array."""
     nx, ny, _ = f.shape
