@@ -205,7 +205,10 @@ def main():
     try:
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as fh:
             rec = json.load(fh).get(f"{kernel}_trt_{args.storage}_{N}")
-        traffic = rec["bytes_per_launch"] if rec else None
+        # a two-step record holds for the tile and z-split it was measured with only
+        same = rec is not None and all(launch.get(k) == v for k, v in rec.get("launch", {}).items()
+                                       if k in ("tile", "zchunks"))
+        traffic = rec["bytes_per_launch"] if same else None
     except (OSError, ValueError, KeyError):
         traffic = None
     cells = N ** 3
