@@ -93,18 +93,24 @@ typedef enum {
   KMI_FACE_ZOUHE_PRESSURE_PROFILE = 6
 } kmi_face_kind;
 
-/* Body force on the flow populations (D2Q9 BGK).
+/* Body force on the flow populations (BGK, D2Q9 or D3Q19).
  *   NONE — none.
- *   GUO  — constant force (F_x, F_y) with the Guo source term, collide_guo_2d!
- *          (collide_guo_2d.jl:5-72); moments and reductions report
- *          u = (sum c f + F/2) / rho, compute_macroscopic_forced_2d! (macroscopic.jl:84-98),
- *          as the generic runner does (simulation_runner.jl:402-406).
+ *   GUO  — constant force desc.force with the Guo source term, collide_guo_2d!
+ *          (collide_guo_2d.jl:5-72) / collide_guo_3d! (collide_guo_3d.jl:5-190); moments
+ *          and reductions report u = (sum c f + F/2) / rho, compute_macroscopic_forced_2d!
+ *          / _3d! (macroscopic.jl:84-160), as the generic runner does
+ *          (simulation_runner.jl:402-406).
  *   GUO_PLAIN_MOMENTS — the same collision, moments without the F/2 shift, as
- *          run_poiseuille_2d reports them (basic.jl:197-199). */
+ *          run_poiseuille_2d reports them (basic.jl:197-199).
+ *   GUO_FIELD — per-node force set by kmi_set_force_field (zero until set):
+ *          collide_guo_field_2d! / collide_guo_field_3d! (collide_guo_2d.jl:93-140,
+ *          collide_guo_3d.jl:199-325); moments with the per-node F/2 shift,
+ *          compute_macroscopic_forced_field_3d! (macroscopic.jl:164-200). */
 typedef enum {
   KMI_FORCE_NONE = 0,
   KMI_FORCE_GUO = 1,
-  KMI_FORCE_GUO_PLAIN_MOMENTS = 2
+  KMI_FORCE_GUO_PLAIN_MOMENTS = 2,
+  KMI_FORCE_GUO_FIELD = 3
 } kmi_forcing;
 
 /* Cells flagged solid by kmi_set_mask.
@@ -125,12 +131,15 @@ typedef enum {
 
 /* Thermal (g) populations on a non-periodic face: adiabatic = bounce-back,
  * Dirichlet = halfway anti-bounce-back g_q = -g*_opp(q) + 2 w_q T_w with the
- * reference's corner rule (thermal_2d.jl:140-172, 371-417). */
+ * reference's corner rule in 2D (thermal_2d.jl:140-172, 371-417) and edge rule in 3D
+ * (thermal_3d.jl:164-230, 494-535): a diagonal slot whose link crosses two Dirichlet
+ * faces is flipped once, with the mean of the two wall temperatures. */
 typedef enum { KMI_THERMAL_ADIABATIC = 0, KMI_THERMAL_DIRICHLET = 1 } kmi_thermal_face;
 
 /* Velocity seen by the thermal collision.
- *   SPLIT — the reference's unfused Rayleigh–Bénard chain: u from the post-stream
- *           populations without the Guo half-force shift (thermal.jl:123-128).
+ *   SPLIT — the reference's unfused chains: u from the post-stream populations without
+ *           the Guo half-force shift (D2Q9: thermal.jl:123-128; D3Q19, the only 3D
+ *           chain: run_natural_convection_3d, thermal.jl:500-515).
  *   FUSED — the reference's fused natural-convection kernel: u with the F/2 shift
  *           (fused_thermal_2d.jl:61-72, 197-200). */
 typedef enum { KMI_COUPLING_SPLIT = 0, KMI_COUPLING_FUSED = 1 } kmi_thermal_coupling;
@@ -147,20 +156,21 @@ typedef struct {
   int32_t stencil;    /* kmi_stencil */
   int32_t storage;    /* kmi_storage */
   int32_t collision;  /* kmi_collision; MRT is D2Q9 only */
-  int32_t thermal;    /* 0 / 1: D2Q9 double-distribution temperature field g */
+  int32_t thermal;    /* 0 / 1: double-distribution temperature field g (Q populations) */
   int64_t nx, ny, nz; /* nz = 1 for D2Q9 */
   double omega;             /* BGK rate  1/(3 nu + 1/2)                     (basic.jl:30) */
   double s_plus, s_minus;   /* TRT rates, see kmi_trt_rates              (fused_trt_2d.jl:201) */
   double s_e, s_eps, s_q, s_nu; /* MRT rates                            (collide_mrt_2d.jl:118) */
   double omega_T;           /* thermal BGK rate 1/(3 alpha + 1/2)           (thermal.jl:65) */
-  double beta_g;            /* Boussinesq  F_y = beta_g (T - T_ref)         (thermal_2d.jl:435) */
+  double beta_g;            /* Boussinesq  F_y = beta_g (T - T_ref), y up (thermal_2d.jl:435,
+                               thermal_3d.jl:583) */
   double T_ref;             /* buoyancy reference temperature (also the g storage shift) */
   int32_t coupling;         /* kmi_thermal_coupling */
   int32_t n_bc_order;       /* 0: default order W,E,S,N,B,T; else the first n entries of */
   int32_t bc_order[6];      /*   bc_order give the order Zou–He closures are applied in   */
   kmi_face faces[6];        /* indexed by kmi_face_id; z faces ignored for D2Q9           */
   int32_t forcing;          /* kmi_forcing                                                */
-  double force[3];          /* body force (F_x, F_y, F_z) for KMI_FORCE_GUO               */
+  double force[3];          /* body force (F_x, F_y, F_z) for KMI_FORCE_GUO(_PLAIN_MOMENTS) */
   int32_t solid_mode;       /* kmi_solid_mode                                             */
 } kmi_desc;
 
@@ -169,8 +179,11 @@ typedef struct kmi_lattice kmi_lattice;
 
 /* Population fields of a lattice. */
 typedef enum { KMI_FIELD_F = 0, KMI_FIELD_G = 1 } kmi_field;
-/* Element type of a host buffer. */
-typedef enum { KMI_HOST_F64 = 0, KMI_HOST_F32 = 1 } kmi_host_dtype;
+/* Element type of a host buffer.  STORAGE: the storage elements themselves, no conversion
+ * (F64: double; F32 and F32S: float, F32S holding f - w_q; F16S: the IEEE binary16 bits of
+ * 2^15 (f - w_q) as uint16; thermal g of an F16S lattice: F32S floats) — the exact state,
+ * for checkpoints that restart bit for bit. */
+typedef enum { KMI_HOST_F64 = 0, KMI_HOST_F32 = 1, KMI_HOST_STORAGE = 2 } kmi_host_dtype;
 
 /* Reductions, all accumulated in FP64 with compensated (Neumaier) sums.
  *   MASS            out[0] = sum_cells rho
@@ -179,8 +192,10 @@ typedef enum { KMI_HOST_F64 = 0, KMI_HOST_F32 = 1 } kmi_host_dtype;
  *   THERMAL         out[0] = sum_cells T
  *   NUSSELT         out[0] = hot-wall Nusselt number; requires two opposite Dirichlet
  *                   faces, the hot one named by `arg` (a kmi_face_id) — estimator of
- *                   thermal.jl:225-234 (second-order one-sided gradient, rows 2..N-1;
- *                   all rows when the wall-parallel axis is periodic)
+ *                   thermal.jl:225-234 and 525-533 (second-order one-sided gradient
+ *                   -H (-8 T_w + 9 T_1 - T_2) / (3 dT), H the cell count along the
+ *                   normal, averaged over the face without its edge rows; a periodic
+ *                   wall-parallel axis keeps all its rows)
  *   MAX_SPEED       out[0] = max_cells |u|
  *   DRAG_MEA        out[0..2] = force on the cut-link walls, Mei–Luo–Shyy momentum
  *                   exchange with the Bouzidi arriving population, per link
@@ -248,6 +263,13 @@ int kmi_set_face_profile(kmi_lattice* lat, int32_t face, const double* a, const 
  * cells of the list must not be masked solid.  D2Q9 or D3Q19, BGK or TRT. */
 int kmi_set_cut_links(kmi_lattice* lat, int64_t n, const int32_t* cell, const int32_t* q,
                       const double* qw, const double* uw);
+/* Per-node body force for KMI_FORCE_GUO_FIELD: the Fx_field, Fy_field (, Fz_field)
+ * arrays of collide_guo_field_2d! / collide_guo_field_3d! (collide_guo_2d.jl:93-160,
+ * collide_guo_3d.jl:199-332), nx*ny*nz values each in the reference layout.  The values
+ * are rounded once to the collision arithmetic (FP64 for F64 storage, FP32 otherwise),
+ * as the reference's element-typed field arrays are.  fz is ignored for D2Q9 (may be
+ * NULL).  The field persists until the next call. */
+int kmi_set_force_field(kmi_lattice* lat, const double* fx, const double* fy, const double* fz);
 /* Replace the values (u, rho, T) of one face between steps; the face kind and thermal
  * kind must stay the same (time-dependent scalar conditions, handlers.jl:210-218). */
 int kmi_set_face(kmi_lattice* lat, int32_t face, const kmi_face* values);
@@ -275,6 +297,8 @@ int kmi_moments(kmi_lattice* lat, double* rho, double* ux, double* uy, double* u
 int kmi_reduce(kmi_lattice* lat, int32_t kind, int32_t arg, double* out);
 /* Steps taken since creation (or since the last upload/init). */
 int kmi_lattice_steps(const kmi_lattice* lat, int64_t* steps);
+/* Set the step counter (a restored checkpoint resumes its cycle count, state.jl:58-70). */
+int kmi_lattice_set_steps(kmi_lattice* lat, int64_t steps);
 /* Bytes of device memory held by the lattice. */
 int kmi_lattice_device_bytes(const kmi_lattice* lat, int64_t* bytes);
 

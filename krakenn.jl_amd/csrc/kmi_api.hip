@@ -35,6 +35,7 @@ struct kmi_lattice {
   int* link_q = nullptr;
   double* link_qw = nullptr;
   double* link_uw = nullptr;
+  void* ffield = nullptr;        // per-node body force (KMI_FORCE_GUO_FIELD), Store::real planes
   int cur = 0;
   int64_t steps = 0;
   StepArgs args{};
@@ -179,8 +180,11 @@ static int validate(const kmi_desc& d, std::string& why, int& code) {
   }
   if (d.collision == KMI_MRT && d.stencil != KMI_D2Q9)
     return code = KMI_ERR_UNSUPPORTED, why = "MRT is defined for D2Q9 only", 1;
-  if (d.thermal && d.stencil != KMI_D2Q9)
-    return code = KMI_ERR_UNSUPPORTED, why = "the thermal DDF is D2Q9 only", 1;
+  if (d.thermal && d.stencil == KMI_D3Q19 && d.coupling != KMI_COUPLING_SPLIT)
+    return code = KMI_ERR_UNSUPPORTED,
+           why = "the D3Q19 thermal chain couples through the unforced velocity only "
+                 "(run_natural_convection_3d, thermal.jl:500-515)",
+           1;
   if (d.thermal && d.collision != KMI_BGK)
     return code = KMI_ERR_UNSUPPORTED,
            why = "the thermal DDF uses BGK + Guo for the flow (thermal_2d.jl:421)", 1;
@@ -188,12 +192,13 @@ static int validate(const kmi_desc& d, std::string& why, int& code) {
     return why = "unknown solid mode", 1;
   if (d.solid_mode == KMI_SOLID_INERT && d.thermal)
     return code = KMI_ERR_UNSUPPORTED, why = "inert solids are for the flow populations only", 1;
-  if (d.forcing < KMI_FORCE_NONE || d.forcing > KMI_FORCE_GUO_PLAIN_MOMENTS)
+  if (d.forcing < KMI_FORCE_NONE || d.forcing > KMI_FORCE_GUO_FIELD)
     return why = "unknown forcing", 1;
-  if (d.forcing != KMI_FORCE_NONE &&
-      (d.stencil != KMI_D2Q9 || d.collision != KMI_BGK || d.thermal))
+  if (d.forcing != KMI_FORCE_NONE && (d.collision != KMI_BGK || d.thermal))
     return code = KMI_ERR_UNSUPPORTED,
-           why = "Guo body force is the D2Q9 BGK collision (collide_guo_2d.jl:5-72)", 1;
+           why = "Guo body force is a BGK collision (collide_guo_2d.jl:5-140, "
+                 "collide_guo_3d.jl:5-325) and not combined with the thermal buoyancy",
+           1;
   for (int f = 0; f < nf; ++f) {
     const kmi_face& F = d.faces[f];
     if (F.kind == KMI_FACE_ZOUHE_VELOCITY_PROFILE || F.kind == KMI_FACE_ZOUHE_PRESSURE_PROFILE) {
@@ -298,11 +303,14 @@ static void build_args(kmi_lattice* L) {
   a.cut_rec = L->cut_rec;
   a.fx = d.force[0];
   a.fy = d.force[1];
-  bool dir[4];
-  for (int f = 0; f < 4; ++f)
-    dir[f] = d.thermal && d.faces[f].kind != KMI_FACE_PERIODIC &&
+  a.fz = L->Q == 19 ? d.force[2] : 0.0;
+  a.ffield = d.forcing == KMI_FORCE_GUO_FIELD ? L->ffield : nullptr;
+  bool dir[6];
+  for (int f = 0; f < 6; ++f)
+    dir[f] = d.thermal && f < nf && d.faces[f].kind != KMI_FACE_PERIODIC &&
              d.faces[f].thermal == KMI_THERMAL_DIRICHLET;
-  // Corner link temperature, in the storage arithmetic (_corner_temp, thermal_2d.jl:299-300).
+  // Corner / edge link temperature, in the storage arithmetic (_corner_temp,
+  // thermal_2d.jl:356-357).
   auto corner = [&](int fa, int fb) -> double {
     if (!dir[fb]) return d.faces[fa].T;
     if (d.storage == KMI_STORE_F32) return double((float(d.faces[fa].T) + float(d.faces[fb].T)) / 2.0f);
@@ -311,22 +319,33 @@ static void build_args(kmi_lattice* L) {
   for (int f = 0; f < 6; ++f) {
     FaceArgs& F = a.face[f];
     F.kind = d.faces[f].kind;
-    F.dirichlet = f < 4 ? dir[f] : 0;
+    F.dirichlet = dir[f];
     for (int c = 0; c < 3; ++c) F.u[c] = d.faces[f].u[c];
     F.rho = d.faces[f].rho;
     F.T = d.faces[f].T;
-    F.Tc[0] = F.Tc[1] = d.faces[f].T;
-    F.skip[0] = F.skip[1] = 0;
+    for (int e = 0; e < 4; ++e) F.Tc[e] = d.faces[f].T, F.skip[e] = 0;
     F.prof[0] = f < 4 ? L->prof[f][0] : nullptr;
     F.prof[1] = f < 4 ? L->prof[f][1] : nullptr;
-    if (f < 4 && dir[f]) {
-      if (f == KMI_WEST || f == KMI_EAST) {
-        F.Tc[0] = corner(f, KMI_SOUTH);
-        F.Tc[1] = corner(f, KMI_NORTH);
-      } else {
-        F.skip[0] = dir[KMI_WEST];
-        F.skip[1] = dir[KMI_EAST];
-      }
+    if (!dir[f]) continue;
+    // Edge slots in the order of the face's tangents (y then z for x-faces, x then z for
+    // y-faces, x then y for z-faces; low index first): x-faces own every edge, y-faces
+    // leave the edges with Dirichlet x-faces alone, z-faces every edge with a Dirichlet x-
+    // or y-face (_apply_fixed_temp_faces_3d!, thermal_3d.jl:494-535; 2D: the first two).
+    if (f == KMI_WEST || f == KMI_EAST) {
+      F.Tc[0] = corner(f, KMI_SOUTH);
+      F.Tc[1] = corner(f, KMI_NORTH);
+      F.Tc[2] = corner(f, KMI_BOTTOM);
+      F.Tc[3] = corner(f, KMI_TOP);
+    } else if (f == KMI_SOUTH || f == KMI_NORTH) {
+      F.skip[0] = dir[KMI_WEST];
+      F.skip[1] = dir[KMI_EAST];
+      F.Tc[2] = corner(f, KMI_BOTTOM);
+      F.Tc[3] = corner(f, KMI_TOP);
+    } else {
+      F.skip[0] = dir[KMI_WEST];
+      F.skip[1] = dir[KMI_EAST];
+      F.skip[2] = dir[KMI_SOUTH];
+      F.skip[3] = dir[KMI_NORTH];
     }
   }
   bool any_nonperiodic = false;
@@ -352,7 +371,7 @@ static void default_geometry(kmi_lattice* L) {
   const long long groups = ((d.nx + bx - 1) / bx) * d.ny * L->args.nz;
   if (groups < 2048 && bx > 64) bx = 64;
   const double field_bytes =
-      double(L->plane) * (L->Q * L->esize + (L->desc.thermal ? 9 * L->gsize : 0));
+      double(L->plane) * (L->Q * L->esize + (L->desc.thermal ? L->Q * L->gsize : 0));
   set_geometry(L, bx, 1, field_bytes > 512e6 ? 1 : 0);
 }
 
@@ -441,6 +460,13 @@ static int run_steps(kmi_lattice* L, int64_t n) {
 
 static double storage_shift_f() { return 1.0; }
 
+// Bytes of the per-node force field: 2 (D2Q9) or 3 (D3Q19) planes in the collision arithmetic.
+static size_t ffield_bytes(const kmi_lattice* L) {
+  return size_t(L->Q == 19 ? 3 : 2) * size_t(L->plane) *
+         (L->desc.storage == KMI_STORE_F64 ? sizeof(double) : sizeof(float));
+}
+
+
 extern "C" {
 
 int kmi_desc_validate(const kmi_desc* desc) {
@@ -472,7 +498,7 @@ int kmi_lattice_create(kmi_ctx* ctx, const kmi_desc* desc, kmi_lattice** out) {
   L->esize = storage_bytes(desc->storage);
   L->gsize = storage_bytes(thermal_storage(desc->storage));
   const size_t bytes = size_t(L->plane) * L->Q * L->esize;
-  const size_t gbytes = size_t(L->plane) * 9 * L->gsize;
+  const size_t gbytes = size_t(L->plane) * L->Q * L->gsize;
   hipError_t e = hipSuccess;
   for (int p = 0; p < 2 && e == hipSuccess; ++p) e = hipMalloc(&L->f[p], bytes);
   for (int p = 0; p < 2 && e == hipSuccess && desc->thermal; ++p) e = hipMalloc(&L->g[p], gbytes);
@@ -487,6 +513,11 @@ int kmi_lattice_create(kmi_ctx* ctx, const kmi_desc* desc, kmi_lattice** out) {
       if (e == hipSuccess)
         e = hipMemcpy(L->prof[f][k], init.data(), len * sizeof(double), hipMemcpyHostToDevice);
     }
+  }
+  // Per-node force field, zero until kmi_set_force_field.
+  if (e == hipSuccess && desc->forcing == KMI_FORCE_GUO_FIELD) {
+    e = hipMalloc(&L->ffield, ffield_bytes(L));
+    if (e == hipSuccess) e = hipMemset(L->ffield, 0, ffield_bytes(L));
   }
   if (e == hipSuccess) e = hipEventCreate(&L->ev[0]);
   if (e == hipSuccess) e = hipEventCreate(&L->ev[1]);
@@ -526,6 +557,7 @@ int kmi_lattice_destroy(kmi_lattice* L) {
   for (void* p : {(void*)L->cut_slot, (void*)L->cut_rec, (void*)L->link_cell, (void*)L->link_q,
                   (void*)L->link_qw, (void*)L->link_uw})
     if (p) (void)hipFree(p);
+  if (L->ffield) (void)hipFree(L->ffield);
   delete L;
   return KMI_OK;
 }
@@ -561,6 +593,33 @@ int kmi_set_mask(kmi_lattice* L, const uint8_t* is_solid) {
   else L->solid_h.assign(is_solid, is_solid + L->ncells);
   KMI_HIP(c, upload_mask(L));
   build_args(L);
+  return KMI_OK;
+}
+
+int kmi_set_force_field(kmi_lattice* L, const double* fx, const double* fy, const double* fz) {
+  if (!L) return fail(nullptr, KMI_ERR_ARG, "kmi_set_force_field: lattice is NULL");
+  kmi_ctx* c = L->ctx;
+  if (L->desc.forcing != KMI_FORCE_GUO_FIELD)
+    return fail(c, KMI_ERR_STATE, "kmi_set_force_field: the lattice's forcing is not KMI_FORCE_GUO_FIELD");
+  if (!fx || !fy || (L->Q == 19 && !fz))
+    return fail(c, KMI_ERR_ARG, "kmi_set_force_field: fx and fy (and fz for D3Q19) are required");
+  const double* src[3] = {fx, fy, fz};
+  const int nc = L->Q == 19 ? 3 : 2;
+  const bool dbl = L->desc.storage == KMI_STORE_F64;
+  std::vector<unsigned char> h(ffield_bytes(L));
+  for (int k = 0; k < nc; ++k)
+    for (long long i = 0; i < L->ncells; ++i) {
+      const size_t at = size_t(k) * size_t(L->plane) + size_t(i);
+      if (dbl) {
+        std::memcpy(h.data() + at * sizeof(double), &src[k][i], sizeof(double));
+      } else {
+        const float v = float(src[k][i]);  // rounded once, as a Float32 field array
+        std::memcpy(h.data() + at * sizeof(float), &v, sizeof(float));
+      }
+    }
+  KMI_HIP(c, hipSetDevice(c->device));
+  KMI_HIP(c, hipStreamSynchronize(c->stream));
+  KMI_HIP(c, hipMemcpy(L->ffield, h.data(), h.size(), hipMemcpyHostToDevice));
   return KMI_OK;
 }
 
@@ -696,7 +755,7 @@ int kmi_upload(kmi_lattice* L, int32_t field, const void* host, int32_t host_dty
   if (!L) return fail(nullptr, KMI_ERR_ARG, "kmi_upload: lattice is NULL");
   kmi_ctx* c = L->ctx;
   if (!host) return fail(c, KMI_ERR_ARG, "kmi_upload: host buffer is NULL");
-  if (host_dtype != KMI_HOST_F64 && host_dtype != KMI_HOST_F32)
+  if (host_dtype != KMI_HOST_F64 && host_dtype != KMI_HOST_F32 && host_dtype != KMI_HOST_STORAGE)
     return fail(c, KMI_ERR_ARG, "kmi_upload: unknown host dtype");
   void** buf;
   double shift;
@@ -704,6 +763,12 @@ int kmi_upload(kmi_lattice* L, int32_t field, const void* host, int32_t host_dty
   if (int rc = field_ptrs(L, field, buf, shift, st)) return rc;
   KMI_HIP(c, hipSetDevice(c->device));
   const size_t n = size_t(L->ncells) * L->Q;
+  if (host_dtype == KMI_HOST_STORAGE) {  // storage elements verbatim (planes are unpadded)
+    KMI_HIP(c, hipStreamSynchronize(c->stream));
+    KMI_HIP(c, hipMemcpy(buf[L->cur], host, n * storage_bytes(st), hipMemcpyHostToDevice));
+    if (field == KMI_FIELD_F) L->steps = 0;
+    return KMI_OK;
+  }
   std::vector<double> tmp;
   const double* src = static_cast<const double*>(host);
   if (host_dtype == KMI_HOST_F32) {
@@ -728,7 +793,7 @@ int kmi_download(kmi_lattice* L, int32_t field, void* host, int32_t host_dtype) 
   if (!L) return fail(nullptr, KMI_ERR_ARG, "kmi_download: lattice is NULL");
   kmi_ctx* c = L->ctx;
   if (!host) return fail(c, KMI_ERR_ARG, "kmi_download: host buffer is NULL");
-  if (host_dtype != KMI_HOST_F64 && host_dtype != KMI_HOST_F32)
+  if (host_dtype != KMI_HOST_F64 && host_dtype != KMI_HOST_F32 && host_dtype != KMI_HOST_STORAGE)
     return fail(c, KMI_ERR_ARG, "kmi_download: unknown host dtype");
   void** buf;
   double shift;
@@ -736,6 +801,11 @@ int kmi_download(kmi_lattice* L, int32_t field, void* host, int32_t host_dtype) 
   if (int rc = field_ptrs(L, field, buf, shift, st)) return rc;
   KMI_HIP(c, hipSetDevice(c->device));
   const size_t n = size_t(L->ncells) * L->Q;
+  if (host_dtype == KMI_HOST_STORAGE) {
+    KMI_HIP(c, hipStreamSynchronize(c->stream));
+    KMI_HIP(c, hipMemcpy(host, buf[L->cur], n * storage_bytes(st), hipMemcpyDeviceToHost));
+    return KMI_OK;
+  }
   double* stage = nullptr;
   KMI_HIP(c, hipMalloc(&stage, n * sizeof(double)));
   std::vector<double> tmp;
@@ -810,8 +880,8 @@ int kmi_init_temperature(kmi_lattice* L, const double* T) {
   double* dev[1];
   double* block;
   if (int rc = stage_fields(L, host, 1, dev, block)) return rc;
-  hipError_t e = launch_init_temp(L->desc.storage, L->g[L->cur], dev[0], L->ncells, L->plane,
-                                  L->desc.T_ref, c->stream);
+  hipError_t e = launch_init_temp(L->Q, L->desc.storage, L->g[L->cur], dev[0], L->ncells,
+                                  L->plane, L->desc.T_ref, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (block) (void)hipFree(block);
   if (e != hipSuccess) return hip_fail(c, e, "kmi_init_temperature");
@@ -842,19 +912,21 @@ int kmi_step_timed(kmi_lattice* L, int64_t nsteps, double* ms) {
   return KMI_OK;
 }
 
-// Body force seen by the moments (u = (Σcf + F/2)/ρ under Guo forcing).
-static double force_x(const kmi_lattice* L) {
-  return L->desc.forcing == KMI_FORCE_GUO ? L->desc.force[0] : 0.0;
-}
-static double force_y(const kmi_lattice* L) {
-  return L->desc.forcing == KMI_FORCE_GUO ? L->desc.force[1] : 0.0;
+// Body force seen by the moments (u = (Σcf + F/2)/ρ under Guo forcing): none for
+// GUO_PLAIN_MOMENTS, the per-node field for GUO_FIELD.
+static MomForce mom_force(const kmi_lattice* L) {
+  MomForce F{{0.0, 0.0, 0.0}, nullptr};
+  if (L->desc.forcing == KMI_FORCE_GUO)
+    for (int k = 0; k < 3; ++k) F.f[k] = (k < 2 || L->Q == 19) ? L->desc.force[k] : 0.0;
+  if (L->desc.forcing == KMI_FORCE_GUO_FIELD) F.field = L->ffield;
+  return F;
 }
 
 static int reduce_cells(kmi_lattice* L, int kind, double* out3) {
   kmi_ctx* c = L->ctx;
   hipError_t e = launch_reduce_cells(L->Q, L->desc.storage, L->f[L->cur],
                                      L->desc.thermal ? L->g[L->cur] : nullptr, L->mask, L->uprev,
-                                     L->ncells, L->plane, L->desc.T_ref, force_x(L), force_y(L), kind,
+                                     L->ncells, L->plane, L->desc.T_ref, mom_force(L), kind,
                                      c->partials, c->out, c->stream);
   if (e != hipSuccess) return hip_fail(c, e, "reduction kernel");
   if (kind == 101) return KMI_OK;
@@ -896,8 +968,8 @@ int kmi_moments(kmi_lattice* L, double* rho, double* ux, double* uy, double* uz,
   for (int k = 0; k < 5; ++k) dev[k] = host[k] ? block + size_t(slot++) * L->ncells : nullptr;
   hipError_t e = launch_moments(L->Q, L->desc.storage, L->f[L->cur],
                                 L->desc.thermal ? L->g[L->cur] : nullptr, dev[0], dev[1], dev[2],
-                                dev[3], dev[4], L->ncells, L->plane, L->desc.T_ref, force_x(L),
-                                force_y(L), c->stream);
+                                dev[3], dev[4], L->ncells, L->plane, L->desc.T_ref, mom_force(L),
+                                c->stream);
   for (int k = 0; k < 5 && e == hipSuccess; ++k)
     if (host[k])
       e = hipMemcpyAsync(host[k], dev[k], size_t(L->ncells) * sizeof(double),
@@ -935,26 +1007,39 @@ int kmi_reduce(kmi_lattice* L, int32_t kind, int32_t arg, double* out) {
     case KMI_RED_NUSSELT: {
       const kmi_desc& d = L->desc;
       if (!d.thermal) return fail(c, KMI_ERR_STATE, "kmi_reduce: lattice is not thermal");
-      if (arg < 0 || arg > 3) return fail(c, KMI_ERR_ARG, "kmi_reduce: Nusselt face must be W, E, S or N");
+      const int nf = L->Q == 9 ? 4 : 6;
+      if (arg < 0 || arg >= nf)
+        return fail(c, KMI_ERR_ARG, L->Q == 9 ? "kmi_reduce: Nusselt face must be W, E, S or N"
+                                              : "kmi_reduce: Nusselt face must be W, E, S, N, B or T");
       const int opp = arg ^ 1;
       auto dirichlet = [&](int f) {
         return d.faces[f].kind != KMI_FACE_PERIODIC && d.faces[f].thermal == KMI_THERMAL_DIRICHLET;
       };
       if (!dirichlet(arg) || !dirichlet(opp))
         return fail(c, KMI_ERR_STATE, "kmi_reduce: Nusselt needs Dirichlet temperatures on the named face and its opposite");
-      const bool xface = arg == KMI_WEST || arg == KMI_EAST;
-      const int npar = int(xface ? d.ny : d.nx);
-      const bool per = d.faces[xface ? KMI_SOUTH : KMI_WEST].kind == KMI_FACE_PERIODIC;
-      const int lo = per ? 0 : 1, hi = per ? npar : npar - 1;
-      if (hi - lo < 1 || (xface ? d.nx : d.ny) < 2)
+      // Wall nodes: the face's tangent indices without the edge rows of a non-periodic
+      // tangent axis (thermal.jl:230-233, 530-533).
+      const int64_t n[3] = {d.nx, d.ny, L->Q == 9 ? 1 : d.nz};
+      const int ax = arg / 2, t1 = ax == 0 ? 1 : 0, t2 = ax == 2 ? 1 : 2;
+      auto range = [&](int t, int& lo, int& hi) {
+        if (t >= nf / 2) { lo = 0, hi = 1; return; }  // 2D: no second tangent
+        const bool per = d.faces[2 * t].kind == KMI_FACE_PERIODIC;
+        lo = per ? 0 : 1;
+        hi = int(per ? n[t] : n[t] - 1);
+      };
+      int lo1, hi1, lo2, hi2;
+      range(t1, lo1, hi1);
+      range(t2, lo2, hi2);
+      if (hi1 - lo1 < 1 || hi2 - lo2 < 1 || n[ax] < 2)
         return fail(c, KMI_ERR_STATE, "kmi_reduce: grid too small for the Nusselt estimator");
       const double Th = d.faces[arg].T, dT = d.faces[arg].T - d.faces[opp].T;
-      hipError_t e = launch_nusselt(d.storage, L->g[L->cur], int(d.nx), int(d.ny), L->plane,
-                                    d.T_ref, arg, Th, dT, lo, hi, c->partials, c->out, c->stream);
+      hipError_t e = launch_nusselt(L->Q, d.storage, L->g[L->cur], int(d.nx), int(d.ny),
+                                    int(n[2]), L->plane, d.T_ref, arg, Th, dT, lo1, hi1, lo2, hi2,
+                                    c->partials, c->out, c->stream);
       if (e != hipSuccess) return hip_fail(c, e, "Nusselt kernel");
       KMI_HIP(c, hipMemcpyAsync(r, c->out, sizeof(double), hipMemcpyDeviceToHost, c->stream));
       KMI_HIP(c, hipStreamSynchronize(c->stream));
-      out[0] = r[0] / double(hi - lo);
+      out[0] = r[0] / double(int64_t(hi1 - lo1) * (hi2 - lo2));
       return KMI_OK;
     }
     case KMI_RED_DRAG_MEA:
@@ -982,6 +1067,13 @@ int kmi_reduce(kmi_lattice* L, int32_t kind, int32_t arg, double* out) {
 int kmi_lattice_steps(const kmi_lattice* L, int64_t* steps) {
   if (!L || !steps) return fail(nullptr, KMI_ERR_ARG, "kmi_lattice_steps: NULL argument");
   *steps = L->steps;
+  return KMI_OK;
+}
+
+int kmi_lattice_set_steps(kmi_lattice* L, int64_t steps) {
+  if (!L) return fail(nullptr, KMI_ERR_ARG, "kmi_lattice_set_steps: lattice is NULL");
+  if (steps < 0) return fail(L->ctx, KMI_ERR_ARG, "kmi_lattice_set_steps: steps must be >= 0");
+  L->steps = steps;
   return KMI_OK;
 }
 
@@ -1043,7 +1135,7 @@ int kmi_lattice_autotune(kmi_lattice* L, int64_t steps_per_trial, double* best_m
   const int64_t trial = std::max<int64_t>(kGraphSteps, (steps_per_trial + kGraphSteps - 1) /
                                                            kGraphSteps * kGraphSteps);
   const size_t bytes = size_t(L->plane) * L->Q * L->esize;
-  const size_t gbytes = size_t(L->plane) * 9 * L->gsize;
+  const size_t gbytes = size_t(L->plane) * L->Q * L->gsize;
   void* save_f = nullptr;
   void* save_g = nullptr;
   KMI_HIP(c, hipMalloc(&save_f, bytes));
@@ -1158,7 +1250,7 @@ int kmi_measure_copy_bandwidth(kmi_ctx* ctx, int64_t bytes, int32_t reps, double
 int kmi_lattice_device_bytes(const kmi_lattice* L, int64_t* bytes) {
   if (!L || !bytes) return fail(nullptr, KMI_ERR_ARG, "kmi_lattice_device_bytes: NULL argument");
   int64_t b = 2 * int64_t(L->plane) * L->Q * int64_t(L->esize);
-  if (L->desc.thermal) b += 2 * int64_t(L->plane) * 9 * int64_t(L->gsize);
+  if (L->desc.thermal) b += 2 * int64_t(L->plane) * L->Q * int64_t(L->gsize);
   if (L->mask) b += L->ncells;
   if (L->uprev) b += 3 * L->ncells * int64_t(sizeof(double));
   *bytes = b;

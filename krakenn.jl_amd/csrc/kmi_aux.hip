@@ -80,8 +80,8 @@ __global__ void k_init_eq(void* dst_, const double* __restrict__ rho, const doub
   });
 }
 
-// g_q = w_q T (thermal.jl:94-96).
-template <int ST>
+// g_q = w_q T (thermal.jl:94-96, 489-496).
+template <int Q, int ST>
 __global__ void k_init_temp(void* dst_, const double* __restrict__ T, long long ncells,
                             long long plane, double T0) {
   using S = Store<ST>;
@@ -91,35 +91,43 @@ __global__ void k_init_temp(void* dst_, const double* __restrict__ T, long long 
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncells) return;
   const double t = T ? T[c] : T0;
-  sfor<9>([&](auto qc) {
+  sfor<Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    if constexpr (!S::shifted) S::st(dst, c + q * plane, R(Lat<9>::w[q]) * R(t));
-    else S::st(dst, c + q * plane, R(Lat<9>::w[q] * (t - T0)));
+    if constexpr (!S::shifted) S::st(dst, c + q * plane, R(Lat<Q>::w[q]) * R(t));
+    else S::st(dst, c + q * plane, R(Lat<Q>::w[q] * (t - T0)));
   });
 }
 
 // --- per-cell moments ------------------------------------------------------------------
-// Plain storage: compute_macroscopic_2d/3d and compute_temperature_2d in the storage type
-// (macroscopic.jl:5-61, thermal_2d.jl:113-119).  Shifted: FP64 from the deviations.
-// (fx, fy): Guo body force; u = (Σcf + F/2)/ρ as compute_macroscopic_forced_2d!
-// (macroscopic.jl:84-98).  Zero without forcing (adding zero leaves Σcf unchanged).
+// Plain storage: compute_macroscopic_2d/3d and compute_temperature_2d/3d in the storage type
+// (macroscopic.jl:5-61, thermal_2d.jl:113-119, thermal_3d.jl:128-136).  Shifted: FP64 from
+// the deviations.  MomForce: Guo body force, u = (Σcf + F/2)/ρ as
+// compute_macroscopic_forced_2d!/_3d!/_field_3d! (macroscopic.jl:84-200); zero without
+// forcing (adding zero leaves Σcf unchanged).
 template <int Q, int ST>
 KMI_DEV void cell_moments(const void* f_, const void* g_, long long c, long long plane, double T0,
-                          double fx, double fy, double& rho, double& ux, double& uy, double& uz,
+                          const MomForce& F, double& rho, double& ux, double& uy, double& uz,
                           double& T) {
   using S = Store<ST>;
   using E = typename S::elem;
   using R = typename S::real;
   const E* f = static_cast<const E*>(f_);
+  R fx = R(F.f[0]), fy = R(F.f[1]), fz = R(F.f[2]);
+  if (F.field) {
+    const R* fld = static_cast<const R*>(F.field);
+    fx = fld[c];
+    fy = fld[c + plane];
+    fz = Q == 19 ? fld[c + 2 * plane] : R(0);
+  }
   if constexpr (!S::shifted) {
     R fp[Q];
     sfor<Q>([&](auto qc) { fp[decltype(qc)::value] = S::ld(f, c + decltype(qc)::value * plane); });
     const R r = zeroth_moment<Q>(fp);
     const R inv = R(1) / r;
     rho = r;
-    ux = (first_moment<Q, 0>(fp) + R(fx) / R(2)) * inv;
-    uy = (first_moment<Q, 1>(fp) + R(fy) / R(2)) * inv;
-    if constexpr (Q == 19) uz = first_moment<Q, 2>(fp) * inv;
+    ux = (first_moment<Q, 0>(fp) + fx / R(2)) * inv;
+    uy = (first_moment<Q, 1>(fp) + fy / R(2)) * inv;
+    if constexpr (Q == 19) uz = (first_moment<Q, 2>(fp) + fz / R(2)) * inv;
     else uz = 0.0;
   } else {
     double fp[Q];
@@ -128,27 +136,25 @@ KMI_DEV void cell_moments(const void* f_, const void* g_, long long c, long long
     });
     const double r = 1.0 + zeroth_moment<Q>(fp);
     rho = r;
-    ux = (first_moment<Q, 0>(fp) + fx / 2.0) / r;
-    uy = (first_moment<Q, 1>(fp) + fy / 2.0) / r;
-    if constexpr (Q == 19) uz = first_moment<Q, 2>(fp) / r;
+    ux = (first_moment<Q, 0>(fp) + double(fx) / 2.0) / r;
+    uy = (first_moment<Q, 1>(fp) + double(fy) / 2.0) / r;
+    if constexpr (Q == 19) uz = (first_moment<Q, 2>(fp) + double(fz) / 2.0) / r;
     else uz = 0.0;
   }
   T = 0.0;
-  if constexpr (Q == 9) {
-    if (g_) {
-      using GS = Store<thermal_storage(ST)>;
-      const auto* g = static_cast<const typename GS::elem*>(g_);
-      if constexpr (!GS::shifted) {
-        R gp[9];
-        sfor<9>([&](auto qc) { gp[decltype(qc)::value] = GS::ld(g, c + decltype(qc)::value * plane); });
-        T = zeroth_moment<9>(gp);
-      } else {
-        double gp[9];
-        sfor<9>([&](auto qc) {
-          gp[decltype(qc)::value] = double(GS::ld(g, c + decltype(qc)::value * plane));
-        });
-        T = T0 + zeroth_moment<9>(gp);
-      }
+  if (g_) {
+    using GS = Store<thermal_storage(ST)>;
+    const auto* g = static_cast<const typename GS::elem*>(g_);
+    if constexpr (!GS::shifted) {
+      R gp[Q];
+      sfor<Q>([&](auto qc) { gp[decltype(qc)::value] = GS::ld(g, c + decltype(qc)::value * plane); });
+      T = zeroth_moment<Q>(gp);
+    } else {
+      double gp[Q];
+      sfor<Q>([&](auto qc) {
+        gp[decltype(qc)::value] = double(GS::ld(g, c + decltype(qc)::value * plane));
+      });
+      T = T0 + zeroth_moment<Q>(gp);
     }
   }
 }
@@ -156,11 +162,11 @@ KMI_DEV void cell_moments(const void* f_, const void* g_, long long c, long long
 template <int Q, int ST>
 __global__ void k_moments(const void* f, const void* g, double* rho, double* ux, double* uy,
                           double* uz, double* T, long long ncells, long long plane, double T0,
-                          double fx, double fy) {
+                          MomForce F) {
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncells) return;
   double r, u, v, w, t;
-  cell_moments<Q, ST>(f, g, c, plane, T0, fx, fy, r, u, v, w, t);
+  cell_moments<Q, ST>(f, g, c, plane, T0, F, r, u, v, w, t);
   if (rho) rho[c] = r;
   if (ux) ux[c] = u;
   if (uy) uy[c] = v;
@@ -199,14 +205,14 @@ KMI_DEV void block_reduce(double (&v)[NV], bool is_max, double* partials) {
 template <int Q, int ST>
 __global__ void __launch_bounds__(kRedBlock)
     k_reduce_cells(const void* f, const void* g, const uint8_t* mask, double* uprev,
-                   long long ncells, long long plane, double T0, double fx, double fy, int kind,
+                   long long ncells, long long plane, double T0, MomForce F, int kind,
                    double* partials) {
   double acc[3] = {0.0, 0.0, 0.0};
   const bool is_max = kind == KMI_RED_MAX_SPEED;
   for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < ncells;
        c += (long long)gridDim.x * blockDim.x) {
     double r, u, v, w, t;
-    cell_moments<Q, ST>(f, g, c, plane, T0, fx, fy, r, u, v, w, t);
+    cell_moments<Q, ST>(f, g, c, plane, T0, F, r, u, v, w, t);
     switch (kind) {
       case KMI_RED_MASS: acc[0] += r; break;
       case KMI_RED_MOMENTUM: acc[0] += r * u; acc[1] += r * v; acc[2] += r * w; break;
@@ -232,36 +238,43 @@ __global__ void __launch_bounds__(kRedBlock)
   block_reduce<3>(acc, is_max, partials);
 }
 
-// Hot-wall Nusselt number per wall-parallel index p (thermal.jl:225-234):
-// Nu_p = −H (−8 T_w + 9 T_1 − T_2) / (3 dx) / ΔT, with T_1, T_2 the first two nodes.
-template <int ST>
+// Hot-wall Nusselt number per wall node (thermal.jl:225-234, 525-533):
+// Nu = −H (−8 T_w + 9 T_1 − T_2) / (3 dx) / ΔT, with T_1, T_2 the first two nodes along the
+// normal and H the cell count along it.  The wall nodes are the rectangle
+// [lo1, hi1) × [lo2, hi2) of the face's two tangent indices (axis order; 2D: one tangent).
+template <int Q, int ST>
 __global__ void __launch_bounds__(kRedBlock)
-    k_nusselt(const void* g_, int nx, int ny, long long plane, double T0, int face, double Th,
-              double dT, int lo, int hi, double* partials) {
+    k_nusselt(const void* g_, int nx, int ny, int nz, long long plane, double T0, int face,
+              double Th, double dT, int lo1, int hi1, int lo2, int hi2, double* partials) {
   using S = Store<ST>;
   using E = typename S::elem;
   const E* g = static_cast<const E*>(g_);
   double acc[1] = {0.0};
-  const bool xface = face == KMI_WEST || face == KMI_EAST;
-  const double H = xface ? nx : ny;
-  for (int p = lo + blockIdx.x * blockDim.x + threadIdx.x; p < hi; p += gridDim.x * blockDim.x) {
-    long long c1, c2;
-    if (face == KMI_WEST) { c1 = (long long)p * nx; c2 = c1 + 1; }
-    else if (face == KMI_EAST) { c1 = (long long)p * nx + nx - 1; c2 = c1 - 1; }
-    else if (face == KMI_SOUTH) { c1 = p; c2 = p + nx; }
-    else { c1 = (long long)(ny - 1) * nx + p; c2 = c1 - nx; }
+  const int n[3] = {nx, ny, nz};
+  const int ax = face / 2, t1 = ax == 0 ? 1 : 0, t2 = ax == 2 ? 1 : 2;
+  const double H = n[ax];
+  const int w1 = hi1 - lo1;
+  const long long count = (long long)w1 * (hi2 - lo2);
+  const long long stride[3] = {1, nx, (long long)nx * ny};
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < count;
+       p += (long long)gridDim.x * blockDim.x) {
+    const long long a = lo1 + p % w1, b = lo2 + p / w1;
+    const long long base = a * stride[t1] + b * stride[t2];
+    const bool hi = face % 2 == 1;
+    const long long c1 = base + (hi ? (n[ax] - 1) : 0) * stride[ax];
+    const long long c2 = c1 + (hi ? -stride[ax] : stride[ax]);
     double T1 = 0.0, T2 = 0.0;
     if constexpr (!S::shifted) {
       using R = typename S::real;
-      R a = S::ld(g, c1), b = S::ld(g, c2);
-      for (int q = 1; q < 9; ++q) { a = a + S::ld(g, c1 + q * plane); b = b + S::ld(g, c2 + q * plane); }
-      T1 = a;
-      T2 = b;
+      R x = S::ld(g, c1), y = S::ld(g, c2);
+      for (int q = 1; q < Q; ++q) { x = x + S::ld(g, c1 + q * plane); y = y + S::ld(g, c2 + q * plane); }
+      T1 = x;
+      T2 = y;
     } else {
-      double a = 0.0, b = 0.0;
-      for (int q = 0; q < 9; ++q) { a += double(S::ld(g, c1 + q * plane)); b += double(S::ld(g, c2 + q * plane)); }
-      T1 = T0 + a;
-      T2 = T0 + b;
+      double x = 0.0, y = 0.0;
+      for (int q = 0; q < Q; ++q) { x += double(S::ld(g, c1 + q * plane)); y += double(S::ld(g, c2 + q * plane)); }
+      T1 = T0 + x;
+      T2 = T0 + y;
     }
     acc[0] += -H * (-8 * Th + 9 * T1 - T2) / 3.0 / dT;
   }
@@ -403,17 +416,18 @@ hipError_t launch_init_eq(int Q, int st, void* dst, const double* rho, const dou
   return hipGetLastError();
 }
 
-hipError_t launch_init_temp(int st, void* dst, const double* T, long long ncells, long long plane,
-                            double T0, hipStream_t s) {
-  KMI_ST_SWITCH(thermal_storage(st), hipLaunchKernelGGL((k_init_temp<ST>), grid1(ncells), dim3(256), 0, s, dst, T, ncells, plane, T0));
+hipError_t launch_init_temp(int Q, int st, void* dst, const double* T, long long ncells,
+                            long long plane, double T0, hipStream_t s) {
+  if (Q == 9) { KMI_ST_SWITCH(thermal_storage(st), hipLaunchKernelGGL((k_init_temp<9, ST>), grid1(ncells), dim3(256), 0, s, dst, T, ncells, plane, T0)); }
+  else { KMI_ST_SWITCH(thermal_storage(st), hipLaunchKernelGGL((k_init_temp<19, ST>), grid1(ncells), dim3(256), 0, s, dst, T, ncells, plane, T0)); }
   return hipGetLastError();
 }
 
 hipError_t launch_moments(int Q, int st, const void* f, const void* g, double* rho, double* ux,
                           double* uy, double* uz, double* T, long long ncells, long long plane,
-                          double T0, double fx, double fy, hipStream_t s) {
-  if (Q == 9) { KMI_ST_SWITCH(st, hipLaunchKernelGGL((k_moments<9, ST>), grid1(ncells), dim3(256), 0, s, f, g, rho, ux, uy, uz, T, ncells, plane, T0, fx, fy)); }
-  else { KMI_ST_SWITCH(st, hipLaunchKernelGGL((k_moments<19, ST>), grid1(ncells), dim3(256), 0, s, f, g, rho, ux, uy, uz, T, ncells, plane, T0, fx, fy)); }
+                          double T0, const MomForce& F, hipStream_t s) {
+  if (Q == 9) { KMI_ST_SWITCH(st, hipLaunchKernelGGL((k_moments<9, ST>), grid1(ncells), dim3(256), 0, s, f, g, rho, ux, uy, uz, T, ncells, plane, T0, F)); }
+  else { KMI_ST_SWITCH(st, hipLaunchKernelGGL((k_moments<19, ST>), grid1(ncells), dim3(256), 0, s, f, g, rho, ux, uy, uz, T, ncells, plane, T0, F)); }
   return hipGetLastError();
 }
 
@@ -424,11 +438,12 @@ int reduce_groups(long long ncells) {
 }
 
 hipError_t launch_reduce_cells(int Q, int st, const void* f, const void* g, const uint8_t* mask,
-                               double* uprev, long long ncells, long long plane, double T0, double fx, double fy,
-                               int kind, double* partials, double* out, hipStream_t s) {
+                               double* uprev, long long ncells, long long plane, double T0,
+                               const MomForce& F, int kind, double* partials, double* out,
+                               hipStream_t s) {
   const int nb = reduce_groups(ncells);
-  if (Q == 9) { KMI_ST_SWITCH(st, hipLaunchKernelGGL((k_reduce_cells<9, ST>), dim3(nb), dim3(kRedBlock), 0, s, f, g, mask, uprev, ncells, plane, T0, fx, fy, kind, partials)); }
-  else { KMI_ST_SWITCH(st, hipLaunchKernelGGL((k_reduce_cells<19, ST>), dim3(nb), dim3(kRedBlock), 0, s, f, g, mask, uprev, ncells, plane, T0, fx, fy, kind, partials)); }
+  if (Q == 9) { KMI_ST_SWITCH(st, hipLaunchKernelGGL((k_reduce_cells<9, ST>), dim3(nb), dim3(kRedBlock), 0, s, f, g, mask, uprev, ncells, plane, T0, F, kind, partials)); }
+  else { KMI_ST_SWITCH(st, hipLaunchKernelGGL((k_reduce_cells<19, ST>), dim3(nb), dim3(kRedBlock), 0, s, f, g, mask, uprev, ncells, plane, T0, F, kind, partials)); }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || kind == 101) return e;
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, s, partials, nb, 3,
@@ -448,11 +463,12 @@ hipError_t launch_drag(int Q, int st, const void* f, long long nlinks, const int
   return hipGetLastError();
 }
 
-hipError_t launch_nusselt(int st, const void* g, int nx, int ny, long long plane, double T0,
-                          int face, double Th, double dT, int lo, int hi, double* partials,
-                          double* out, hipStream_t s) {
-  const int nb = (hi - lo + kRedBlock - 1) / kRedBlock;
-  KMI_ST_SWITCH(thermal_storage(st), hipLaunchKernelGGL((k_nusselt<ST>), dim3(nb), dim3(kRedBlock), 0, s, g, nx, ny, plane, T0, face, Th, dT, lo, hi, partials));
+hipError_t launch_nusselt(int Q, int st, const void* g, int nx, int ny, int nz, long long plane,
+                          double T0, int face, double Th, double dT, int lo1, int hi1, int lo2,
+                          int hi2, double* partials, double* out, hipStream_t s) {
+  const int nb = reduce_groups((long long)(hi1 - lo1) * (hi2 - lo2));
+  if (Q == 9) { KMI_ST_SWITCH(thermal_storage(st), hipLaunchKernelGGL((k_nusselt<9, ST>), dim3(nb), dim3(kRedBlock), 0, s, g, nx, ny, nz, plane, T0, face, Th, dT, lo1, hi1, lo2, hi2, partials)); }
+  else { KMI_ST_SWITCH(thermal_storage(st), hipLaunchKernelGGL((k_nusselt<19, ST>), dim3(nb), dim3(kRedBlock), 0, s, g, nx, ny, nz, plane, T0, face, Th, dT, lo1, hi1, lo2, hi2, partials)); }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, s, partials, nb, 1, 0, out);

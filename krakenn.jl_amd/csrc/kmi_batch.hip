@@ -561,7 +561,8 @@ int kmi_batch_moments(kmi_batch* B, int32_t c, double* rho, double* ux, double* 
   KMI_HIP(x, hipMalloc(&block, size_t(3) * nc * sizeof(double)));
   const char* base = static_cast<const char*>(B->f[B->cur]) + B->cav[c].foff * B->esize;
   hipError_t e = launch_moments(9, B->desc.storage, base, nullptr, block, block + nc, block + 2 * nc,
-                                nullptr, nullptr, nc, nc, 0.0, 0.0, 0.0, x->stream);
+                                nullptr, nullptr, nc, nc, 0.0, MomForce{{0.0, 0.0, 0.0}, nullptr},
+                                x->stream);
   double* host[3] = {rho, ux, uy};
   for (int k = 0; k < 3 && e == hipSuccess; ++k)
     if (host[k]) e = hipMemcpyAsync(host[k], block + k * nc, size_t(nc) * sizeof(double), hipMemcpyDeviceToHost, x->stream);

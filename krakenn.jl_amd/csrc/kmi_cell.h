@@ -183,6 +183,70 @@ KMI_DEV void collide_thermal(const R (&gp)[9], R (&go)[9], R ux, R uy, R omT, R 
   }
 }
 
+// Thermal BGK on D3Q19 g (collide_thermal_3d_kernel!, thermal_3d.jl:12-107):
+// T = g1 + … + g19, geq = w T (1 + 3 c·u) with c·u in the reference's written form
+// (1 + 3(−u) equals the written 1 − 3u bit for bit).  Shifted: as collide_thermal.
+template <bool SH, class R>
+KMI_DEV void collide_thermal19(const R (&gp)[19], R (&go)[19], R ux, R uy, R uz, R omT, R T0) {
+  const R s = zeroth_moment<19>(gp);
+  const R Tt = SH ? T0 + s : s;
+  sfor<19>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr double w = Lat<19>::w[q];
+    R geq;
+    if constexpr (q == 0) geq = R(w) * s;
+    else if constexpr (!SH)
+      geq = R(w) * s * (R(1) + R(3) * cdotu<Lat<19>::cx[q], Lat<19>::cy[q], Lat<19>::cz[q]>(ux, uy, uz));
+    else
+      geq = R(w) * (s + Tt * (R(3) * cdotu<Lat<19>::cx[q], Lat<19>::cy[q], Lat<19>::cz[q]>(ux, uy, uz)));
+    go[q] = gp[q] - omT * (gp[q] - geq);
+  });
+}
+
+// (c_a − u_a) as the reference writes it: `(-ux)`, `(one(T)-ux)`, `(-one(T)-ux)`.
+template <int C, class R>
+KMI_DEV R c_minus_u(R u) {
+  if constexpr (C == 0) return -u;
+  else if constexpr (C > 0) return R(1) - u;
+  else return -R(1) - u;
+}
+
+// BGK + Guo source on D3Q19: collide_guo_3d_kernel! (collide_guo_3d.jl:21-185), its
+// per-node-field twin (:213-320) and collide_boussinesq_3d_kernel! (thermal_3d.jl:574-712):
+//   S_q = w ((c−u)·F) 3 + w (c·u)(c·F) 9,   f* = f − ω(f − feq) + (1 − ω/2) S_q,
+// ux, uy, uz the force-shifted velocities; equilibria paired as in `equilibria`.
+template <bool SH, class R>
+KMI_DEV void collide_guo19(const R (&fp)[19], R (&fo)[19], R m0, R rho, R ux, R uy, R uz, R fx,
+                           R fy, R fz, R om) {
+  const R guo = R(1) - om / R(2);
+  R fe[19];
+  equilibria<19, SH>(m0, rho, ux, uy, uz, fe);
+  sfor<19>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr int cx = Lat<19>::cx[q], cy = Lat<19>::cy[q], cz = Lat<19>::cz[q];
+    constexpr double w = Lat<19>::w[q];
+    R Sq = R(w) * (c_minus_u<cx>(ux) * fx + c_minus_u<cy>(uy) * fy + c_minus_u<cz>(uz) * fz) * R(3);
+    if constexpr (q != 0)
+      Sq = Sq + R(w) * cdotu<cx, cy, cz>(ux, uy, uz) * cdotu<cx, cy, cz>(fx, fy, fz) * R(9);
+    fo[q] = fp[q] - om * (fp[q] - fe[q]) + guo * Sq;
+  });
+}
+
+// Body force at cell c: the per-node field when one is set, else the constant force.
+template <int Q, class R>
+KMI_DEV void node_force(const StepArgs& a, long long c, R& fx, R& fy, R& fz) {
+  if (a.ffield) {
+    const R* F = static_cast<const R*>(a.ffield);
+    fx = F[c];
+    fy = F[c + a.plane];
+    fz = Q == 19 ? F[c + 2 * a.plane] : R(0);
+  } else {
+    fx = R(a.fx);
+    fy = R(a.fy);
+    fz = R(a.fz);
+  }
+}
+
 // BGK + Guo source for a force (fx, fy): collide_guo_2d.jl:5-72 (constant force),
 // thermal_2d.jl:443-478 and fused_thermal_2d.jl:116-157 (fx = 0, buoyancy fy).  ux, uy are
 // the force-shifted velocities.  The reference writes the c_q = −e_x, −e_y terms as
@@ -463,6 +527,40 @@ KMI_DEV void thermal_faces(R (&g)[9], const StepArgs& a, int i, int j) {
     g[8] = abb(g[8], we, tv(F.T), i == 0, tv(F.Tc[0]), F.skip[0] != 0);
     g[7] = abb(g[7], we, tv(F.T), i == a.nx - 1, tv(F.Tc[1]), F.skip[1] != 0);
   }
+}
+
+// Dirichlet faces of D3Q19 g (apply_fixed_temp_*_3d_kernel!, thermal_3d.jl:193-480): the
+// axis slot pointing into the domain and four edge slots per face, 0-based; Tc/skip carry
+// the edge rule of _apply_fixed_temp_faces_3d! (thermal_3d.jl:494-535).
+template <bool SH, class R>
+KMI_DEV void thermal_faces3(R (&g)[19], const StepArgs& a, int i, int j, int k) {
+  const R wa = R(2) * R(1.0 / 18.0);
+  const R we = R(2) * R(1.0 / 36.0);
+  const double t0 = SH ? a.T_ref : 0.0;
+  auto tv = [&](double t) { return SH ? R(t - t0) : R(t); };
+  const int xm = a.nx - 1, ym = a.ny - 1, zm = a.nz - 1;
+  auto face = [&](int fid, int ax, int e0, int e1, int e2, int e3, bool c0, bool c1, bool c2,
+                  bool c3) {
+    const FaceArgs& F = a.face[fid];
+    const R t = tv(F.T);
+    g[ax] = -g[ax] + wa * t;
+    g[e0] = abb(g[e0], we, t, c0, tv(F.Tc[0]), F.skip[0] != 0);
+    g[e1] = abb(g[e1], we, t, c1, tv(F.Tc[1]), F.skip[1] != 0);
+    g[e2] = abb(g[e2], we, t, c2, tv(F.Tc[2]), F.skip[2] != 0);
+    g[e3] = abb(g[e3], we, t, c3, tv(F.Tc[3]), F.skip[3] != 0);
+  };
+  if (i == 0 && a.face[KMI_WEST].dirichlet)
+    face(KMI_WEST, 1, 7, 9, 11, 13, j == 0, j == ym, k == 0, k == zm);
+  if (i == xm && a.face[KMI_EAST].dirichlet)
+    face(KMI_EAST, 2, 8, 10, 12, 14, j == 0, j == ym, k == 0, k == zm);
+  if (j == 0 && a.face[KMI_SOUTH].dirichlet)
+    face(KMI_SOUTH, 3, 7, 8, 15, 17, i == 0, i == xm, k == 0, k == zm);
+  if (j == ym && a.face[KMI_NORTH].dirichlet)
+    face(KMI_NORTH, 4, 9, 10, 16, 18, i == 0, i == xm, k == 0, k == zm);
+  if (k == 0 && a.face[KMI_BOTTOM].dirichlet)
+    face(KMI_BOTTOM, 5, 11, 12, 15, 16, i == 0, i == xm, j == 0, j == ym);
+  if (k == zm && a.face[KMI_TOP].dirichlet)
+    face(KMI_TOP, 6, 13, 14, 17, 18, i == 0, i == xm, j == 0, j == ym);
 }
 
 }  // namespace kmi

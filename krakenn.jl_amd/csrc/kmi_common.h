@@ -193,8 +193,11 @@ struct FaceArgs {
   double u[3];
   double rho;
   double T;
-  double Tc[2];  // corner link temperatures (apply_fixed_temp_2d corner rule)
-  int skip[2];   // corner slot owned by the adjacent x-face
+  // Corner (2D: entries 0, 1) or edge (3D: 0..3) link temperatures of the Dirichlet
+  // face, in the order of the face's edge slots, and whether an adjacent face owns the
+  // slot instead (apply_fixed_temp_2d corner rule, apply_fixed_temp_3d edge rule).
+  double Tc[4];
+  int skip[4];
   const double* prof[2];  // *_PROFILE faces: per-node (u_x, u_y) or (rho, -) on the device
 };
 
@@ -214,13 +217,23 @@ struct StepArgs {
   double omega_T, beta_g, T_ref;
   int coupling;
   int forcing;      // 1: Guo body force (any kmi_forcing other than NONE)
-  double fx, fy;    // body force (KMI_FORCE_GUO)
+  double fx, fy, fz;  // constant body force (KMI_FORCE_GUO)
+  // Per-node body force (KMI_FORCE_GUO_FIELD): planes x, y(, z) of `plane` values in the
+  // collision arithmetic (Store<ST>::real); nullptr: the constant force applies.
+  const void* ffield;
   int solid_inert;  // masked cells hold the rest equilibrium (KMI_SOLID_INERT)
   // Cut links: mask value 2 marks a fluid cell with cut links; cut_slot[cell] indexes its
   // record in cut_rec: Q wall fractions (0: link not cut), then Q × 3 wall velocities.
   const int* cut_slot;
   const double* cut_rec;
   FaceArgs face[6];
+};
+
+// Body force seen by the moment kernels: constant f[3], or per-node values (`field`, planes
+// x, y(, z) in Store<ST>::real, as StepArgs::ffield) when field != nullptr.
+struct MomForce {
+  double f[3];
+  const void* field;
 };
 
 // Cell classes held in the mask byte.

@@ -34,22 +34,22 @@ hipError_t launch_from_storage(int Q, int st, const void* src, double* dst, long
 hipError_t launch_init_eq(int Q, int st, void* dst, const double* rho, const double* ux,
                           const double* uy, const double* uz, long long ncells, long long plane,
                           hipStream_t s);
-hipError_t launch_init_temp(int st, void* dst, const double* T, long long ncells, long long plane,
-                            double T0, hipStream_t s);
+hipError_t launch_init_temp(int Q, int st, void* dst, const double* T, long long ncells,
+                            long long plane, double T0, hipStream_t s);
 hipError_t launch_moments(int Q, int st, const void* f, const void* g, double* rho, double* ux,
                           double* uy, double* uz, double* T, long long ncells, long long plane,
-                          double T0, double fx, double fy, hipStream_t s);
+                          double T0, const MomForce& F, hipStream_t s);
 int reduce_groups(long long ncells);
 hipError_t launch_reduce_cells(int Q, int st, const void* f, const void* g, const uint8_t* mask,
                                double* uprev, long long ncells, long long plane, double T0,
-                               double fx, double fy, int kind, double* partials, double* out,
+                               const MomForce& F, int kind, double* partials, double* out,
                                hipStream_t s);
 hipError_t launch_drag(int Q, int st, const void* f, long long nlinks, const int* cell,
                        const int* q, const double* qw, const double* uw, int nx, int ny, int nz,
                        long long plane, int mea, double* partials, double* out, hipStream_t s);
-hipError_t launch_nusselt(int st, const void* g, int nx, int ny, long long plane, double T0,
-                          int face, double Th, double dT, int lo, int hi, double* partials,
-                          double* out, hipStream_t s);
+hipError_t launch_nusselt(int Q, int st, const void* g, int nx, int ny, int nz, long long plane,
+                          double T0, int face, double Th, double dT, int lo1, int hi1, int lo2,
+                          int hi2, double* partials, double* out, hipStream_t s);
 
 }  // namespace kmi
 

@@ -15,7 +15,9 @@
 //   TRT                fused_trt_2d.jl:101-173, dsl/bricks_3d.jl:168-229
 //   MRT                collide_mrt_2d.jl:43-109
 //   Zou–He             boundary_2d.jl:13-44, 135-185, 206-281; boundary_3d.jl:152-290
-//   thermal DDF        thermal_2d.jl:16-60, 140-290, 421-480; fused_thermal_2d.jl:10-207
+//   thermal DDF        thermal_2d.jl:16-60, 140-290, 421-480; fused_thermal_2d.jl:10-207;
+//                      thermal_3d.jl:12-136, 164-535, 559-712
+//   Guo forcing        collide_guo_2d.jl:5-140, collide_guo_3d.jl:5-325
 // F32S / F16S store deviations from the rest state and collide them in FP32 (the
 // algebra is identical; only the rounding differs).
 #include "kmi_cell.h"
@@ -157,42 +159,59 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs a) {
   }
 
   if constexpr (TH) {
-    static_assert(Q == 9, "thermal DDF is D2Q9");
     using GS = Store<thermal_storage(ST)>;
     using GE = typename GS::elem;
     static_assert(std::is_same_v<typename GS::real, R> && GS::shifted == SH, "g and f share arithmetic");
     const GE* __restrict__ gi = static_cast<const GE*>(a.gi);
     GE* __restrict__ go = static_cast<GE*>(a.go);
-    R gp[9], go_r[9];
+    R gp[Q], go_r[Q];
     pull(TypeTag<GS>{}, gi, gp, false);
-    if constexpr (BND) thermal_faces<SH>(gp, a, i, j);
-    const R Tl = zeroth_moment<9>(gp);  // T (shifted: T − T0)
-    const R m0 = zeroth_moment<9>(fp);
+    if constexpr (BND) {
+      if constexpr (Q == 9) thermal_faces<SH>(gp, a, i, j);
+      else thermal_faces3<SH>(gp, a, i, j, k);
+    }
+    const R Tl = zeroth_moment<Q>(gp);  // T (shifted: T − T0)
+    const R m0 = zeroth_moment<Q>(fp);
     const R rho = SH ? R(1) + m0 : m0;
     const R inv = R(1) / rho;
-    const R jx = first_moment<9, 0>(fp);
-    const R jy = first_moment<9, 1>(fp);
+    const R jx = first_moment<Q, 0>(fp);
+    const R jy = first_moment<Q, 1>(fp);
     const R fy = SH ? R(a.beta_g) * Tl : R(a.beta_g) * (Tl - R(a.T_ref));
     const R fx = R(0);
-    R uxf, uyf, uxt, uyt;
-    if (a.coupling == KMI_COUPLING_SPLIT) {
-      uxt = jx * inv;  // compute_macroscopic_2d! (thermal.jl:124)
-      uyt = jy * inv;
-      uxf = (jx + fx / R(2)) * inv;  // collide_boussinesq_2d_kernel! (thermal_2d.jl:440-441)
-      uyf = (jy + fy / R(2)) * inv;
+    if constexpr (Q == 9) {
+      R uxf, uyf, uxt, uyt;
+      if (a.coupling == KMI_COUPLING_SPLIT) {
+        uxt = jx * inv;  // compute_macroscopic_2d! (thermal.jl:124)
+        uyt = jy * inv;
+        uxf = (jx + fx / R(2)) * inv;  // collide_boussinesq_2d_kernel! (thermal_2d.jl:440-441)
+        uyf = (jy + fy / R(2)) * inv;
+      } else {
+        uxf = jx * inv;  // macroscopic_boussinesq (fused_thermal_2d.jl:67-69)
+        uyf = (jy + fy / R(2)) * inv;
+        uxt = uxf;
+        uyt = uyf;
+      }
+      collide_thermal<SH>(gp, go_r, uxt, uyt, R(a.omega_T), R(a.T_ref));
+      if (solid) {
+        sfor<9>([&](auto qc) { fo_r[decltype(qc)::value] = fp[Lat<9>::opp[decltype(qc)::value]]; });
+      } else {
+        collide_guo<SH>(fp, fo_r, m0, rho, uxf, uyf, fx, fy, R(a.omega));
+      }
     } else {
-      uxf = jx * inv;  // macroscopic_boussinesq (fused_thermal_2d.jl:67-69)
-      uyf = (jy + fy / R(2)) * inv;
-      uxt = uxf;
-      uyt = uyf;
+      // run_natural_convection_3d (thermal.jl:500-515): the thermal collision sees
+      // compute_macroscopic_3d!'s u, the Boussinesq collision the F/2-shifted one
+      // (thermal_3d.jl:593-596), F = (0, β_g (T − T_ref), 0).
+      const R jz = first_moment<Q, 2>(fp);
+      const R fz = R(0);
+      collide_thermal19<SH>(gp, go_r, jx * inv, jy * inv, jz * inv, R(a.omega_T), R(a.T_ref));
+      if (solid) {
+        sfor<Q>([&](auto qc) { fo_r[decltype(qc)::value] = fp[Lat<Q>::opp[decltype(qc)::value]]; });
+      } else {
+        collide_guo19<SH>(fp, fo_r, m0, rho, (jx + fx / R(2)) * inv, (jy + fy / R(2)) * inv,
+                          (jz + fz / R(2)) * inv, fx, fy, fz, R(a.omega));
+      }
     }
-    collide_thermal<SH>(gp, go_r, uxt, uyt, R(a.omega_T), R(a.T_ref));
-    if (solid) {
-      sfor<9>([&](auto qc) { fo_r[decltype(qc)::value] = fp[Lat<9>::opp[decltype(qc)::value]]; });
-    } else {
-      collide_guo<SH>(fp, fo_r, m0, rho, uxf, uyf, fx, fy, R(a.omega));
-    }
-    sfor<9>([&](auto qc) {
+    sfor<Q>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
       GS::template st<NTS>(go, self + q * P, go_r[q]);
     });
@@ -204,17 +223,24 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs a) {
       sfor<Q>([&](auto qc) { fo_r[decltype(qc)::value] = fp[Lat<Q>::opp[decltype(qc)::value]]; });
     } else if constexpr (COLL == KMI_BGK) {
       bool forced = false;
-      if constexpr (Q == 9 && BND) {
+      if constexpr (BND) {
         if (a.forcing) {
-          // collide_guo_2d_kernel! (collide_guo_2d.jl:5-72): u = (Σcf + F/2)/ρ
+          // collide_guo_2d_kernel! / collide_guo_3d_kernel! and their per-node-field twins
+          // (collide_guo_2d.jl:5-140, collide_guo_3d.jl:5-325): u = (Σcf + F/2)/ρ
           forced = true;
-          const R fx = R(a.fx), fy = R(a.fy);
-          const R m0 = zeroth_moment<9>(fp);
+          R fx, fy, fz;
+          node_force<Q>(a, self, fx, fy, fz);
+          const R m0 = zeroth_moment<Q>(fp);
           const R rho = SH ? R(1) + m0 : m0;
           const R inv = R(1) / rho;
-          const R ux = (first_moment<9, 0>(fp) + fx / R(2)) * inv;
-          const R uy = (first_moment<9, 1>(fp) + fy / R(2)) * inv;
-          collide_guo<SH>(fp, fo_r, m0, rho, ux, uy, fx, fy, R(a.omega));
+          const R ux = (first_moment<Q, 0>(fp) + fx / R(2)) * inv;
+          const R uy = (first_moment<Q, 1>(fp) + fy / R(2)) * inv;
+          if constexpr (Q == 9) {
+            collide_guo<SH>(fp, fo_r, m0, rho, ux, uy, fx, fy, R(a.omega));
+          } else {
+            const R uz = (first_moment<Q, 2>(fp) + fz / R(2)) * inv;
+            collide_guo19<SH>(fp, fo_r, m0, rho, ux, uy, uz, fx, fy, fz, R(a.omega));
+          }
         }
       }
       if (!forced) collide_bgk<Q, SH>(fp, fo_r, R(a.omega));
@@ -243,9 +269,9 @@ static hipError_t launch(bool nts, const StepArgs& a, dim3 g, dim3 b, hipStream_
 template <int Q, int ST>
 static hipError_t launch_st(int coll, bool th, bool bnd, bool nts, const StepArgs& a, dim3 g,
                             dim3 b, hipStream_t s) {
+  if (th) return bnd ? launch<Q, ST, KMI_BGK, true, true>(nts, a, g, b, s)
+                     : launch<Q, ST, KMI_BGK, true, false>(nts, a, g, b, s);
   if constexpr (Q == 9) {
-    if (th) return bnd ? launch<9, ST, KMI_BGK, true, true>(nts, a, g, b, s)
-                       : launch<9, ST, KMI_BGK, true, false>(nts, a, g, b, s);
     if (coll == KMI_MRT) return bnd ? launch<9, ST, KMI_MRT, false, true>(nts, a, g, b, s)
                                     : launch<9, ST, KMI_MRT, false, false>(nts, a, g, b, s);
   }
@@ -269,7 +295,7 @@ static hipError_t launch_q(int st, int coll, bool th, bool bnd, bool nts, const 
 hipError_t launch_step(int q, int st, int coll, bool th, bool bnd, bool nts, const StepArgs& a,
                        dim3 g, dim3 b, hipStream_t s) {
   return q == 9 ? launch_q<9>(st, coll, th, bnd, nts, a, g, b, s)
-                : launch_q<19>(st, coll, false, bnd, nts, a, g, b, s);
+                : launch_q<19>(st, coll, th, bnd, nts, a, g, b, s);
 }
 
 }  // namespace kmi

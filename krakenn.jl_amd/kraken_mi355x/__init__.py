@@ -14,6 +14,8 @@ from .drivers import (
     cavity_3d_desc,
     cavity_desc,
     lbm_config,
+    natural_convection_3d_desc,
+    natural_convection_3d_initial_temperature,
     natural_convection_desc,
     natural_convection_initial_temperature,
     omega,
@@ -24,6 +26,7 @@ from .drivers import (
     run_cavity_3d,
     run_couette_2d,
     run_natural_convection_2d,
+    run_natural_convection_3d,
     run_poiseuille_2d,
     run_rayleigh_benard_2d,
     run_taylor_green_2d,
@@ -45,9 +48,9 @@ __all__ = [
     "plan_simulation", "run_simulation", "cut_links_from_dense", "precompute_q_wall_cylinder",
     "precompute_q_wall_sphere_3d", "wall_velocity_rotating_cylinder",
     "Batch", "STATUS_NAMES", "ensemble_parameters", "ArgumentError", "D2Q9", "D3Q19", "KrakenError", "LBMConfig", "Lattice", "MI355XBackend",
-    "cavity_3d_desc", "cavity_desc", "lbm_config", "load", "make_desc", "natural_convection_desc",
+    "cavity_3d_desc", "cavity_desc", "lbm_config", "load", "make_desc", "natural_convection_3d_desc", "natural_convection_3d_initial_temperature", "natural_convection_desc",
     "natural_convection_initial_temperature", "omega", "rayleigh_benard_desc",
-    "rb_initial_temperature", "reynolds", "run_cavity_2d", "run_cavity_3d", "run_couette_2d", "run_natural_convection_2d",
+    "rb_initial_temperature", "reynolds", "run_cavity_2d", "run_cavity_3d", "run_couette_2d", "run_natural_convection_2d", "run_natural_convection_3d",
     "run_poiseuille_2d", "run_rayleigh_benard_2d", "run_taylor_green_2d", "run_taylor_green_3d", "run_to_residual", "taylor_green_3d_desc",
     "taylor_green_2d_fields", "taylor_green_3d_fields", "trt_rates",
 ]

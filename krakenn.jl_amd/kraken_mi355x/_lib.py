@@ -20,12 +20,12 @@ STORE_F64, STORE_F32, STORE_F32S, STORE_F16S = 0, 1, 2, 3
 BGK, TRT, MRT = 0, 1, 2
 (FACE_PERIODIC, FACE_WALL, FACE_MOVING_WALL, FACE_ZOUHE_VELOCITY, FACE_ZOUHE_PRESSURE,
  FACE_ZOUHE_VELOCITY_PROFILE, FACE_ZOUHE_PRESSURE_PROFILE) = range(7)
-FORCE_NONE, FORCE_GUO, FORCE_GUO_PLAIN_MOMENTS = 0, 1, 2
+FORCE_NONE, FORCE_GUO, FORCE_GUO_PLAIN_MOMENTS, FORCE_GUO_FIELD = 0, 1, 2, 3
 WEST, EAST, SOUTH, NORTH, BOTTOM, TOP = range(6)
 THERMAL_ADIABATIC, THERMAL_DIRICHLET = 0, 1
 COUPLING_SPLIT, COUPLING_FUSED = 0, 1
 FIELD_F, FIELD_G = 0, 1
-HOST_F64, HOST_F32 = 0, 1
+HOST_F64, HOST_F32, HOST_STORAGE = 0, 1, 2
 (RED_MASS, RED_MOMENTUM, RED_KINETIC_ENERGY, RED_THERMAL, RED_NUSSELT, RED_MAX_SPEED, RED_DRAG_MEA,
  RED_DRAG_HALFWAY) = range(8)
 SOLID_BOUNCE_BACK, SOLID_INERT = 0, 1
@@ -109,6 +109,7 @@ SIGNATURES = {
     "kmi_set_face": (c_int, [c_void_p, c_int32, POINTER(KmiFace)]),
     "kmi_set_cut_links": (c_int, [c_void_p, c_int64, POINTER(c_int32), POINTER(c_int32), POINTER(c_double),
                                   POINTER(c_double)]),
+    "kmi_set_force_field": (c_int, [c_void_p, POINTER(c_double), POINTER(c_double), POINTER(c_double)]),
     "kmi_upload": (c_int, [c_void_p, c_int32, c_void_p, c_int32]),
     "kmi_download": (c_int, [c_void_p, c_int32, c_void_p, c_int32]),
     "kmi_init_equilibrium": (c_int, [c_void_p] + [POINTER(c_double)] * 4),
@@ -119,6 +120,7 @@ SIGNATURES = {
     "kmi_moments": (c_int, [c_void_p] + [POINTER(c_double)] * 5),
     "kmi_reduce": (c_int, [c_void_p, c_int32, c_int32, POINTER(c_double)]),
     "kmi_lattice_steps": (c_int, [c_void_p, POINTER(c_int64)]),
+    "kmi_lattice_set_steps": (c_int, [c_void_p, c_int64]),
     "kmi_lattice_device_bytes": (c_int, [c_void_p, POINTER(c_int64)]),
     "kmi_lattice_set_launch": (c_int, [c_void_p, c_int32, c_int32, c_int32]),
     "kmi_lattice_get_launch": (c_int, [c_void_p, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)]),

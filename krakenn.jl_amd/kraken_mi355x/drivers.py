@@ -299,6 +299,46 @@ def run_natural_convection_2d(*, N=128, Ra=1e4, Pr=0.71, T_hot=1.0, T_cold=0.0, 
     return {"ρ": m["rho"], "ux": m["ux"], "uy": m["uy"], "Temp": m["Temp"], "Nu": Nu, "Ra": Ra, "Pr": Pr}
 
 
+def natural_convection_3d_desc(N, Ra, Pr, T_hot=1.0, T_cold=0.0, storage="f64"):
+    """Differentially heated cube (thermal.jl:455-538): D3Q19, halfway bounce-back on all six
+    faces, hot west / cold east anti-bounce-back, adiabatic elsewhere, Boussinesq
+    F_y = β_g (T − T_ref), the unfused chain's velocity coupling."""
+    dT = T_hot - T_cold
+    nu = 0.05
+    a = nu / Pr
+    beta_g = Ra * nu * a / (dT * float(N) ** 3)
+    faces = {"west": {"kind": "wall", "thermal": "dirichlet", "T": T_hot},
+             "east": {"kind": "wall", "thermal": "dirichlet", "T": T_cold},
+             "south": {"kind": "wall"}, "north": {"kind": "wall"},
+             "bottom": {"kind": "wall"}, "top": {"kind": "wall"}}
+    d = make_desc("d3q19", N, N, N, nu, storage=storage, collision="bgk", faces=faces, thermal=True,
+                  omega_T=1.0 / (3.0 * a + 0.5), beta_g=beta_g, T_ref=(T_hot + T_cold) / 2,
+                  coupling="split")
+    return d, {"nu": nu, "alpha": a, "beta_g": beta_g, "T_ref": (T_hot + T_cold) / 2}
+
+
+def natural_convection_3d_initial_temperature(N, T_hot=1.0, T_cold=0.0):
+    """Linear profile between the halfway walls, T = T_hot − ΔT (i − ½)/N (thermal.jl:489-496)."""
+    dT = T_hot - T_cold
+    i = np.arange(1, N + 1, dtype=np.float64)
+    T = T_hot - dT * (i - 0.5) / N
+    return np.asfortranarray(np.broadcast_to(T[:, None, None], (N, N, N)))
+
+
+def run_natural_convection_3d(*, N=32, Ra=1e3, Pr=0.71, T_hot=1.0, T_cold=0.0, max_steps=10000,
+                              storage="f64", backend=None):
+    """run_natural_convection_3d (thermal.jl:455-538) on the device: every step one launch;
+    Nu from the hot-wall estimator averaged over the face without its edge rows."""
+    d, p = natural_convection_3d_desc(N, Ra, Pr, T_hot, T_cold, storage)
+    with Lattice(d, _backend(backend)) as lat:
+        lat.init_temperature(natural_convection_3d_initial_temperature(N, T_hot, T_cold))
+        lat.step(max_steps)
+        m = lat.moments(temperature=True)
+        Nu = lat.reduce("nusselt", "west")
+    return {"ρ": m["rho"], "ux": m["ux"], "uy": m["uy"], "uz": m["uz"], "Temp": m["Temp"],
+            "Nu": Nu, "Ra": Ra, "Pr": Pr, "ν": p["nu"], "α": p["alpha"]}
+
+
 def run_to_residual(lat: Lattice, tol=1e-7, check_every=100, max_steps=10_000_000, max_wall=math.inf):
     """Step until r_u = ‖u^k − u^(k−1)‖₂/‖u^k‖₂ < tol, checked every `check_every` steps
     (the stop rule of gnf_periodic_steady.jl:114-227).  Returns (steps, r_u, converged)."""

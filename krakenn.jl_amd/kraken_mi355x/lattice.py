@@ -69,7 +69,8 @@ def make_desc(stencil, nx, ny, nz=1, nu=0.1, storage="f64", collision="bgk", fac
               solid_mode="bounce_back") -> L.KmiDesc:
     """Build a kmi_desc.  `faces` maps face name -> dict(kind=..., u=..., rho=..., T=...,
     thermal='dirichlet'|'adiabatic').  forcing: None, 'guo' (moments with the F/2 shift) or
-    'guo_plain' (moments without it), with the body force `force`.  solid_mode: masked cells
+    'guo_plain' (moments without it), with the body force `force`, or 'guo_field' (per-node
+    force, Lattice.set_force_field).  solid_mode: masked cells
     swap their populations ('bounce_back') or hold the rest equilibrium ('inert', LI-BB)."""
     st = {"d2q9": L.D2Q9, "d3q19": L.D3Q19}.get(str(stencil).lower(), stencil)
     d = L.desc_init(st, nx, ny, nz, nu)
@@ -94,7 +95,7 @@ def make_desc(stencil, nx, ny, nz=1, nu=0.1, storage="f64", collision="bgk", fac
         for n, name in enumerate(bc_order):
             d.bc_order[n] = L.FACE_NAMES[name] if isinstance(name, str) else name
     d.forcing = {None: L.FORCE_NONE, "none": L.FORCE_NONE, "guo": L.FORCE_GUO,
-                 "guo_plain": L.FORCE_GUO_PLAIN_MOMENTS}[forcing]
+                 "guo_plain": L.FORCE_GUO_PLAIN_MOMENTS, "guo_field": L.FORCE_GUO_FIELD}[forcing]
     fv = list(force) + [0.0, 0.0, 0.0]
     for c in range(3):
         d.force[c] = float(fv[c])
@@ -133,6 +134,9 @@ class Lattice:
         h = c_void_p()
         L.check(lib.kmi_lattice_create(self.backend.ctx, byref(desc), byref(h)), self.backend.ctx)
         self._h = h
+        # Host copies of the state set through the setters below (mask, cut links, force
+        # field, face profiles and face values), for checkpoints of the whole lattice.
+        self.aux = {}
 
     def close(self):
         if getattr(self, "_h", None):
@@ -171,14 +175,40 @@ class Lattice:
         self._chk(L.load().kmi_download(self._h, fld, out.ctypes.data_as(c_void_p), dt))
         return out
 
+    def storage_dtype(self, field="f"):
+        """numpy type of the stored elements of a field (KMI_HOST_STORAGE transfers)."""
+        st = self.desc.storage
+        if field == "g" and st == L.STORE_F16S:
+            st = L.STORE_F32S  # thermal populations of an F16S lattice are F32S
+        return {L.STORE_F64: np.float64, L.STORE_F32: np.float32, L.STORE_F32S: np.float32,
+                L.STORE_F16S: np.uint16}[st]
+
+    def download_raw(self, field="f"):
+        """The stored elements verbatim (exact state: F32S/F16S deviations, F16S as binary16 bits)."""
+        fld = L.FIELD_G if field == "g" else L.FIELD_F
+        out = np.empty(self.shape + (self.Q,), dtype=self.storage_dtype(field), order="F")
+        self._chk(L.load().kmi_download(self._h, fld, out.ctypes.data_as(c_void_p), L.HOST_STORAGE))
+        return out
+
+    def upload_raw(self, a, field="f"):
+        fld = L.FIELD_G if field == "g" else L.FIELD_F
+        want = self.storage_dtype(field)
+        a = np.asarray(a)
+        if a.dtype != want or a.size != self.ncells * self.Q:
+            raise ArgumentError(f"raw {field} needs {self.ncells * self.Q} values of {np.dtype(want)}")
+        a = np.asfortranarray(a.reshape(self.shape + (self.Q,), order="F"))
+        self._chk(L.load().kmi_upload(self._h, fld, a.ctypes.data_as(c_void_p), L.HOST_STORAGE))
+
     def set_mask(self, is_solid):
         if is_solid is None:
             self._chk(L.load().kmi_set_mask(self._h, None))
+            self.aux.pop("is_solid", None)
             return
         m = np.asfortranarray(np.asarray(is_solid, dtype=np.uint8))
         if m.size != self.ncells:
             raise ArgumentError("mask size does not match the lattice")
         self._chk(L.load().kmi_set_mask(self._h, m.ctypes.data_as(POINTER(c_uint8))))
+        self.aux["is_solid"] = m.reshape(self.shape, order="F") != 0
 
     def init_equilibrium(self, rho=None, ux=None, uy=None, uz=None):
         arrs = [None if a is None else np.asfortranarray(np.broadcast_to(a, self.shape), dtype=np.float64)
@@ -193,6 +223,7 @@ class Lattice:
         pa = np.ascontiguousarray(np.broadcast_to(a, (n,)), dtype=np.float64)
         pb = None if b is None else np.ascontiguousarray(np.broadcast_to(b, (n,)), dtype=np.float64)
         self._chk(L.load().kmi_set_face_profile(self._h, fid, _dptr(pa), _dptr(pb)))
+        self.aux.setdefault("profiles", {})[fid] = (pa.copy(), None if pb is None else pb.copy())
 
     def set_face(self, face, spec):
         """Replace a face's values (u, rho, T) between steps; its kind must not change."""
@@ -200,6 +231,7 @@ class Lattice:
         f = L.KmiFace()
         _fill_face(f, spec)
         self._chk(L.load().kmi_set_face(self._h, fid, byref(f)))
+        self.aux.setdefault("faces", {})[fid] = f
 
     def init_temperature(self, T=None):
         a = None if T is None else np.asfortranarray(np.broadcast_to(T, self.shape), dtype=np.float64)
@@ -224,6 +256,10 @@ class Lattice:
         s = c_int64()
         self._chk(L.load().kmi_lattice_steps(self._h, byref(s)))
         return s.value
+
+    @steps.setter
+    def steps(self, n):
+        self._chk(L.load().kmi_lattice_set_steps(self._h, int(n)))
 
     # --- launch tuning (results never depend on it) ---
     def set_launch(self, bx, by=1, nontemporal=False):
@@ -286,6 +322,26 @@ class Lattice:
             return tuple(out)
         return out[0]
 
+    def set_force_field(self, fx, fy, fz=None):
+        """Per-node body force of a 'guo_field' lattice (collide_guo_field_2d!/_3d!), arrays
+        of the lattice shape in the reference layout; rounded once to the collision
+        arithmetic."""
+        shp = self.shape
+        arrs = []
+        for a in (fx, fy, fz):
+            if a is None:
+                arrs.append(None)
+                continue
+            a = np.asarray(a, dtype=np.float64)
+            if a.shape != shp:
+                raise ArgumentError(f"force field shape {a.shape} != lattice shape {shp}")
+            arrs.append(np.asfortranarray(a))
+        if arrs[0] is None or arrs[1] is None or (len(shp) == 3 and arrs[2] is None):
+            raise ArgumentError("set_force_field needs every component of the lattice's dimension")
+        self._chk(L.load().kmi_set_force_field(self._h, _dptr(arrs[0]), _dptr(arrs[1]),
+                                               _dptr(arrs[2]) if arrs[2] is not None else None))
+        self.aux["force_field"] = tuple(a.copy(order="F") for a in arrs[:len(shp)])
+
     def set_cut_links(self, cell, q, qw, uw=None):
         """Interpolated bounce-back links: cell index (x fastest), direction q (0-based),
         wall fraction in (0, 1], wall velocity (n × 3, default zero)."""
@@ -299,3 +355,7 @@ class Lattice:
         uw = np.ascontiguousarray(uw)
         ip = lambda a: a.ctypes.data_as(POINTER(ctypes.c_int32))
         self._chk(L.load().kmi_set_cut_links(self._h, n, ip(cell), ip(q), _dptr(qw), _dptr(uw)))
+        if n:
+            self.aux["cut_links"] = (cell.copy(), q.copy(), qw.copy(), uw.copy())
+        else:
+            self.aux.pop("cut_links", None)

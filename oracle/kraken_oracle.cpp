@@ -316,6 +316,52 @@ static void guo9_force(const T* fp, T* fo, T rho, T ux, T uy, T fx, T fy, T om) 
   fo[8] = fp[8] - om * (fp[8] - feq<T>(1.0 / 36.0, rho, ux - uy, usq)) + pref * Sq;
 }
 
+// Thermal BGK on g, D3Q19 — collide_thermal_3d_kernel! (thermal_3d.jl:12-107):
+// T = g1 + … + g19 left to right, geq = w T (1 + 3 c·u) with c·u written as the reference
+// does (`u_x`, `-u_x + u_y`, …; 1 + 3(−u) equals the written 1 − 3u exactly).
+template <class T>
+static void thermal19(const T* gp, T* go, T ux, T uy, T uz, T omT) {
+  T Temp = gp[0];
+  for (int q = 1; q < 19; ++q) Temp = Temp + gp[q];
+  T t3 = T(3);
+  T geq = T(1.0 / 3.0) * Temp;
+  go[0] = gp[0] - omT * (gp[0] - geq);
+  for (int q = 1; q < 19; ++q) {
+    geq = T(W19[q]) * Temp * (T(1) + t3 * cdotu<T>(CX19[q], CY19[q], CZ19[q], ux, uy, uz));
+    go[q] = gp[q] - omT * (gp[q] - geq);
+  }
+}
+
+// (c_a − u_a) as the reference writes it: `(-ux)`, `(one(T)-ux)`, `(-one(T)-ux)`.
+template <class T>
+static inline T c_minus_u(int c, T u) {
+  return c == 0 ? -u : (c > 0 ? T(1) - u : -T(1) - u);
+}
+
+// BGK + Guo source, D3Q19, one cell — the common body of collide_guo_3d_kernel!
+// (collide_guo_3d.jl:21-185), collide_guo_field_3d_kernel! (:213-320) and
+// collide_boussinesq_3d_kernel! (thermal_3d.jl:574-712, F = (0, fy, 0)):
+//   S_q = w ((c−u)·F) 3 + w (c·u)(c·F) 9,  f* = f − ω(f − feq) + (1 − ω/2) S_q,
+// with ρ, u the force-shifted moments of the caller.
+template <class T>
+static void guo19(const T* fp, T* fo, T rho, T ux, T uy, T uz, T fx, T fy, T fz, T om) {
+  T usq = ux * ux + uy * uy + uz * uz;
+  T pref = T(1) - om / T(2);
+  T t3 = T(3), t9 = T(9);
+  T fe = T(W19[0]) * rho * (T(1) - T(1.5) * usq);
+  T Sq = T(W19[0]) * ((-ux) * fx + (-uy) * fy + (-uz) * fz) * t3;
+  fo[0] = fp[0] - om * (fp[0] - fe) + pref * Sq;
+  for (int q = 1; q < 19; ++q) {
+    const int cx = CX19[q], cy = CY19[q], cz = CZ19[q];
+    T cu = cdotu<T>(cx, cy, cz, ux, uy, uz);
+    T cf = cdotu<T>(cx, cy, cz, fx, fy, fz);
+    fe = feq<T>(W19[q], rho, cu, usq);
+    Sq = T(W19[q]) * (c_minus_u(cx, ux) * fx + c_minus_u(cy, uy) * fy + c_minus_u(cz, uz) * fz) * t3 +
+         T(W19[q]) * cu * cf * t9;
+    fo[q] = fp[q] - om * (fp[q] - fe) + pref * Sq;
+  }
+}
+
 // ----------------------------------------------------------------------------------
 // Reference kernels restated one for one (whole-array operations)
 // ----------------------------------------------------------------------------------
@@ -1026,6 +1072,204 @@ static void fused_natconv_step(T* fo, const T* fi, T* go, const T* gi, T* Temp, 
     }
 }
 
+// --- D3Q19 thermal and forced kernels ------------------------------------------------
+
+// collide_thermal_3d_kernel! — thermal_3d.jl:12-107 (u arrays from compute_macroscopic_3d!)
+template <class T>
+static void collide_thermal_3d(T* g, const T* ux, const T* uy, const T* uz, int64_t nx,
+                               int64_t ny, int64_t nz, T omT) {
+  const size_t P = size_t(nx * ny * nz);
+#pragma omp parallel for
+  for (int64_t c = 0; c < int64_t(P); ++c) {
+    T gp[19], go[19];
+    for (int q = 0; q < 19; ++q) gp[q] = g[c + q * P];
+    thermal19(gp, go, ux[c], uy[c], uz[c], omT);
+    for (int q = 0; q < 19; ++q) g[c + q * P] = go[q];
+  }
+}
+
+// compute_temperature_3d_kernel! — thermal_3d.jl:128-136
+template <class T>
+static void compute_temperature_3d(T* Temp, const T* g, int64_t nx, int64_t ny, int64_t nz) {
+  const size_t P = size_t(nx * ny * nz);
+  for (size_t c = 0; c < P; ++c) {
+    T t = g[c];
+    for (int q = 1; q < 19; ++q) t = t + g[c + q * P];
+    Temp[c] = t;
+  }
+}
+
+// apply_fixed_temp_{west,east,south,north,bottom,top}_3d_kernel! — thermal_3d.jl:193-480.
+// Per face: the axis slot (pointing into the domain) and the four edge slots with the
+// tangent index at which each is an edge slot, 0-based.  Edge n of a face uses T_e[n]
+// and is left alone when skip_e[n] is set (_abb_slot_3d, thermal_3d.jl:185-188).
+struct FixedTempFace3 {
+  int axis_slot;
+  int edge_slot[4];
+  int edge_axis[4];  // 0: i, 1: j, 2: k
+  int edge_hi[4];    // 0: index 0, 1: index n − 1
+};
+static const FixedTempFace3 FT3[6] = {
+    {1, {7, 9, 11, 13}, {1, 1, 2, 2}, {0, 1, 0, 1}},     // west   (i = 0)
+    {2, {8, 10, 12, 14}, {1, 1, 2, 2}, {0, 1, 0, 1}},    // east   (i = nx − 1)
+    {3, {7, 8, 15, 17}, {0, 0, 2, 2}, {0, 1, 0, 1}},     // south  (j = 0)
+    {4, {9, 10, 16, 18}, {0, 0, 2, 2}, {0, 1, 0, 1}},    // north  (j = ny − 1)
+    {5, {11, 12, 15, 16}, {0, 0, 1, 1}, {0, 1, 0, 1}},   // bottom (k = 0)
+    {6, {13, 14, 17, 18}, {0, 0, 1, 1}, {0, 1, 0, 1}},   // top    (k = nz − 1)
+};
+
+template <class T>
+static void fixed_temp_face_3d(T* g, int face, T Tw, const T* Te, const bool* skip, int64_t nx,
+                               int64_t ny, int64_t nz) {
+  const int64_t n[3] = {nx, ny, nz};
+  const size_t P = size_t(nx * ny * nz);
+  const int ax = face / 2;
+  const int64_t fixed = (face % 2) ? n[ax] - 1 : 0;
+  const FixedTempFace3& F = FT3[face];
+  const T wa = T(2) * T(1.0 / 18.0);
+  const T we = T(2) * T(1.0 / 36.0);
+  for (int64_t k = 0; k < nz; ++k)
+    for (int64_t j = 0; j < ny; ++j)
+      for (int64_t i = 0; i < nx; ++i) {
+        const int64_t x[3] = {i, j, k};
+        if (x[ax] != fixed) continue;
+        const size_t c = I3(i, j, k, nx, ny);
+        T& ga = g[c + F.axis_slot * P];
+        ga = -ga + wa * Tw;
+        for (int e = 0; e < 4; ++e) {
+          const int a = F.edge_axis[e];
+          const bool on_edge = x[a] == (F.edge_hi[e] ? n[a] - 1 : 0);
+          T& gq = g[c + F.edge_slot[e] * P];
+          gq = abb_slot(gq, we, Tw, on_edge, Te[e], skip[e]);
+        }
+      }
+}
+
+// _apply_fixed_temp_faces_3d! — thermal_3d.jl:494-535: x-faces own their edges (mean
+// temperature where the other face is Dirichlet), y-faces leave the edges with Dirichlet
+// x-faces alone, z-faces leave every edge with a Dirichlet x- or y-face alone.
+template <class T>
+static void apply_fixed_temp_3d(T* g, const bool* dir, const double* Tf, int64_t nx, int64_t ny,
+                                int64_t nz) {
+  auto corner = [&](int a, int b) { return dir[b] ? (T(Tf[a]) + T(Tf[b])) / T(2) : T(Tf[a]); };
+  for (int f = 0; f < 6; ++f) {
+    if (!dir[f]) continue;
+    T Te[4];
+    bool sk[4] = {false, false, false, false};
+    const T t = T(Tf[f]);
+    if (f < 2) {
+      Te[0] = corner(f, KMI_SOUTH), Te[1] = corner(f, KMI_NORTH);
+      Te[2] = corner(f, KMI_BOTTOM), Te[3] = corner(f, KMI_TOP);
+    } else if (f < 4) {
+      Te[0] = t, Te[1] = t, Te[2] = corner(f, KMI_BOTTOM), Te[3] = corner(f, KMI_TOP);
+      sk[0] = dir[KMI_WEST], sk[1] = dir[KMI_EAST];
+    } else {
+      Te[0] = Te[1] = Te[2] = Te[3] = t;
+      sk[0] = dir[KMI_WEST], sk[1] = dir[KMI_EAST], sk[2] = dir[KMI_SOUTH], sk[3] = dir[KMI_NORTH];
+    }
+    fixed_temp_face_3d(g, f, t, Te, sk, nx, ny, nz);
+  }
+}
+
+// Solid cell of the 3D collide kernels: swap opposite pairs (collide_guo_3d.jl:9-19).
+template <class T>
+static inline void swap_pairs_19(T* f, size_t c, size_t P) {
+  for (int q = 1; q < 19; ++q)
+    if (q < OPP19[q]) std::swap(f[c + q * P], f[c + OPP19[q] * P]);
+}
+
+// Force-shifted moments, D3Q19 — the common head of the 3D Guo kernels
+// (collide_guo_3d.jl:34-39) and compute_macroscopic_forced(_field)_3d! (macroscopic.jl:121-200).
+template <class T>
+static inline void moments19_forced(const T* f, T fx, T fy, T fz, T& rho, T& ux, T& uy, T& uz) {
+  rho = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8] + f[9] + f[10] +
+        f[11] + f[12] + f[13] + f[14] + f[15] + f[16] + f[17] + f[18];
+  T inv = T(1) / rho;
+  ux = ((f[1] - f[2] + f[7] - f[8] + f[9] - f[10] + f[11] - f[12] + f[13] - f[14]) + fx / T(2)) * inv;
+  uy = ((f[3] - f[4] + f[7] + f[8] - f[9] - f[10] + f[15] - f[16] + f[17] - f[18]) + fy / T(2)) * inv;
+  uz = ((f[5] - f[6] + f[11] + f[12] - f[13] - f[14] + f[15] + f[16] - f[17] - f[18]) + fz / T(2)) * inv;
+}
+
+// collide_guo_3d! (constant force) and collide_guo_field_3d! (per-node force):
+// collide_guo_3d.jl:5-190, 199-332.  Fxa == nullptr selects the constant force.
+template <class T>
+static void collide_guo_3d(T* f, const uint8_t* solid, int64_t nx, int64_t ny, int64_t nz, T om,
+                           T Fx, T Fy, T Fz, const T* Fxa = nullptr, const T* Fya = nullptr,
+                           const T* Fza = nullptr) {
+  const size_t P = size_t(nx * ny * nz);
+#pragma omp parallel for
+  for (int64_t c = 0; c < int64_t(P); ++c) {
+    if (solid && solid[c]) {
+      swap_pairs_19(f, c, P);
+      continue;
+    }
+    T fx = Fxa ? Fxa[c] : Fx, fy = Fxa ? Fya[c] : Fy, fz = Fxa ? Fza[c] : Fz;
+    T fp[19], fo[19], rho, ux, uy, uz;
+    for (int q = 0; q < 19; ++q) fp[q] = f[c + q * P];
+    moments19_forced(fp, fx, fy, fz, rho, ux, uy, uz);
+    guo19(fp, fo, rho, ux, uy, uz, fx, fy, fz, om);
+    for (int q = 0; q < 19; ++q) f[c + q * P] = fo[q];
+  }
+}
+
+// collide_boussinesq_3d_kernel! — thermal_3d.jl:559-712: F = (0, β_g (T − T_ref), 0).
+template <class T>
+static void collide_boussinesq_3d(T* f, const T* Temp, const uint8_t* solid, int64_t nx,
+                                  int64_t ny, int64_t nz, T om, T beta_g, T T_ref) {
+  const size_t P = size_t(nx * ny * nz);
+#pragma omp parallel for
+  for (int64_t c = 0; c < int64_t(P); ++c) {
+    if (solid && solid[c]) {
+      swap_pairs_19(f, c, P);
+      continue;
+    }
+    T fp[19], fo[19], rho, ux, uy, uz;
+    for (int q = 0; q < 19; ++q) fp[q] = f[c + q * P];
+    T fy = beta_g * (Temp[c] - T_ref), fx = T(0), fz = T(0);
+    moments19_forced(fp, fx, fy, fz, rho, ux, uy, uz);
+    guo19(fp, fo, rho, ux, uy, uz, fx, fy, fz, om);
+    for (int q = 0; q < 19; ++q) f[c + q * P] = fo[q];
+  }
+}
+
+// compute_macroscopic_forced_3d! / compute_macroscopic_forced_field_3d! — macroscopic.jl:121-200
+template <class T>
+static void macroscopic_forced_3d(T* rho, T* ux, T* uy, T* uz, const T* f, int64_t nx, int64_t ny,
+                                  int64_t nz, T Fx, T Fy, T Fz, const T* Fxa = nullptr,
+                                  const T* Fya = nullptr, const T* Fza = nullptr) {
+  const size_t P = size_t(nx * ny * nz);
+  for (size_t c = 0; c < P; ++c) {
+    T fp[19];
+    for (int q = 0; q < 19; ++q) fp[q] = f[c + q * P];
+    T fx = Fxa ? Fxa[c] : Fx, fy = Fxa ? Fya[c] : Fy, fz = Fxa ? Fza[c] : Fz;
+    moments19_forced(fp, fx, fy, fz, rho[c], ux[c], uy[c], uz[c]);
+  }
+}
+
+// collide_guo_field_2d_kernel! — collide_guo_2d.jl:93-140 (the constant-force body with
+// per-node (fx, fy)).
+template <class T>
+static void collide_guo_field_2d(T* f, const uint8_t* solid, int64_t nx, int64_t ny, T om,
+                                 const T* Fxa, const T* Fya) {
+  const size_t P = size_t(nx * ny);
+#pragma omp parallel for
+  for (int64_t c = 0; c < int64_t(P); ++c) {
+    T fp[9], fo[9];
+    for (int q = 0; q < 9; ++q) fp[q] = f[c + q * P];
+    if (solid && solid[c]) {
+      for (int q = 0; q < 9; ++q) f[c + q * P] = fp[OPP9[q]];
+      continue;
+    }
+    const T fx = Fxa[c], fy = Fya[c];
+    T rho = fp[0] + fp[1] + fp[2] + fp[3] + fp[4] + fp[5] + fp[6] + fp[7] + fp[8];
+    T inv = T(1) / rho;
+    T ux = ((fp[1] - fp[3] + fp[5] - fp[6] - fp[7] + fp[8]) + fx / T(2)) * inv;
+    T uy = ((fp[2] - fp[4] + fp[5] + fp[6] - fp[7] - fp[8]) + fy / T(2)) * inv;
+    guo9_force(fp, fo, rho, ux, uy, fx, fy, om);
+    for (int q = 0; q < 9; ++q) f[c + q * P] = fo[q];
+  }
+}
+
 // ----------------------------------------------------------------------------------
 // Generic descriptor-driven step: the semantics the MI355X kernels implement.
 // Each piece is one of the restated reference kernels above, chained in the order of
@@ -1143,25 +1387,41 @@ static void zou_he_faces(const kmi_desc& d, T* f, const Profiles* pr = nullptr) 
 
 template <class T>
 static void generic_step(const kmi_desc& d, const uint8_t* solid, T*& f, T*& f2, T*& g, T*& g2,
-                         std::vector<T>& scratch, const Profiles* pr = nullptr) {
-  const int64_t nx = d.nx, ny = d.ny, nz = d.stencil == KMI_D2Q9 ? 1 : d.nz;
+                         std::vector<T>& scratch, const Profiles* pr = nullptr,
+                         const T* const* ffield = nullptr) {
+  const bool d3 = d.stencil == KMI_D3Q19;
+  const int64_t nx = d.nx, ny = d.ny, nz = d3 ? d.nz : 1;
   const size_t P = size_t(nx * ny * nz);
   pull_generic(d, f2, f);
   zou_he_faces(d, f2, pr);
   if (d.thermal) {
     pull_generic(d, g2, g, false);
-    bool dir[4];
-    double Tf[4];
-    for (int a = 0; a < 4; ++a) {
-      dir[a] = d.faces[a].kind != KMI_FACE_PERIODIC && d.faces[a].thermal == KMI_THERMAL_DIRICHLET;
+    bool dir[6];
+    double Tf[6];
+    for (int a = 0; a < 6; ++a) {
+      dir[a] = (d3 || a < 4) && d.faces[a].kind != KMI_FACE_PERIODIC &&
+               d.faces[a].thermal == KMI_THERMAL_DIRICHLET;
       Tf[a] = d.faces[a].T;
     }
-    apply_fixed_temp_2d(g2, dir, Tf, nx, ny);
-    scratch.resize(4 * P);
+    scratch.resize(5 * P);
     T* Temp = scratch.data();
     T* rho = Temp + P;
     T* ux = rho + P;
     T* uy = ux + P;
+    T* uz = uy + P;
+    if (d3) {
+      // run_natural_convection_3d step (thermal.jl:500-515): ABB faces, T, unforced u for
+      // the thermal collision, Boussinesq collision with the force-shifted u.
+      apply_fixed_temp_3d(g2, dir, Tf, nx, ny, nz);
+      compute_temperature_3d(Temp, g2, nx, ny, nz);
+      macroscopic_3d(rho, ux, uy, uz, f2, nx, ny, nz);
+      collide_thermal_3d(g2, ux, uy, uz, nx, ny, nz, T(d.omega_T));
+      collide_boussinesq_3d(f2, Temp, solid, nx, ny, nz, T(d.omega), T(d.beta_g), T(d.T_ref));
+      std::swap(g, g2);
+      std::swap(f, f2);
+      return;
+    }
+    apply_fixed_temp_2d(g2, dir, Tf, nx, ny);
     compute_temperature_2d(Temp, g2, nx, ny);
     if (d.coupling == KMI_COUPLING_SPLIT) {
       // thermal.jl:123-125: u from compute_macroscopic_2d! (no force shift)
@@ -1181,14 +1441,22 @@ static void generic_step(const kmi_desc& d, const uint8_t* solid, T*& f, T*& f2,
     collide_thermal_2d(g2, ux, uy, nx, ny, T(d.omega_T));
     collide_boussinesq_2d(f2, Temp, solid, nx, ny, T(d.omega), T(d.beta_g), T(d.T_ref));
     std::swap(g, g2);
-  } else if (d.stencil == KMI_D2Q9) {
-    if (d.collision == KMI_BGK && d.forcing != KMI_FORCE_NONE)
+  } else if (!d3) {
+    if (d.collision == KMI_BGK && d.forcing == KMI_FORCE_GUO_FIELD)
+      collide_guo_field_2d(f2, solid, nx, ny, T(d.omega), ffield[0], ffield[1]);
+    else if (d.collision == KMI_BGK && d.forcing != KMI_FORCE_NONE)
       collide_guo_2d(f2, solid, nx, ny, T(d.omega), T(d.force[0]), T(d.force[1]));
     else if (d.collision == KMI_BGK) collide_2d(f2, solid, nx, ny, T(d.omega));
     else if (d.collision == KMI_TRT) collide_trt_2d(f2, solid, nx, ny, T(d.s_plus), T(d.s_minus));
     else collide_mrt_2d(f2, solid, nx, ny, T(d.s_e), T(d.s_eps), T(d.s_q), T(d.s_nu));
   } else {
-    if (d.collision == KMI_BGK) collide_3d(f2, solid, nx, ny, nz, T(d.omega));
+    if (d.collision == KMI_BGK && d.forcing == KMI_FORCE_GUO_FIELD)
+      collide_guo_3d(f2, solid, nx, ny, nz, T(d.omega), T(0), T(0), T(0), ffield[0], ffield[1],
+                     ffield[2]);
+    else if (d.collision == KMI_BGK && d.forcing != KMI_FORCE_NONE)
+      collide_guo_3d(f2, solid, nx, ny, nz, T(d.omega), T(d.force[0]), T(d.force[1]),
+                     T(d.force[2]));
+    else if (d.collision == KMI_BGK) collide_3d(f2, solid, nx, ny, nz, T(d.omega));
     else collide_trt_3d(f2, solid, nx, ny, nz, T(d.s_plus), T(d.s_minus));
   }
   std::swap(f, f2);
@@ -1509,6 +1777,68 @@ extern "C" {
     ko::zou_he_pressure_east_spatial(f, rho, nx, ny);                                    \
   }
 
+#define KO_3D(S, T)                                                                       \
+  void ko_collide_thermal_3d_##S(T* g, const T* u, const T* v, const T* w, int64_t nx,   \
+                                 int64_t ny, int64_t nz, double om) {                    \
+    ko::collide_thermal_3d(g, u, v, w, nx, ny, nz, T(om));                               \
+  }                                                                                      \
+  void ko_compute_temperature_3d_##S(T* t, const T* g, int64_t nx, int64_t ny,           \
+                                     int64_t nz) {                                       \
+    ko::compute_temperature_3d(t, g, nx, ny, nz);                                        \
+  }                                                                                      \
+  /* dir[6], Tf[6] in face order W, E, S, N, B, T */                                     \
+  void ko_apply_fixed_temp_3d_##S(T* g, const int* dir, const double* Tf, int64_t nx,   \
+                                  int64_t ny, int64_t nz) {                              \
+    bool b[6];                                                                           \
+    for (int a = 0; a < 6; ++a) b[a] = dir[a] != 0;                                      \
+    ko::apply_fixed_temp_3d(g, b, Tf, nx, ny, nz);                                       \
+  }                                                                                      \
+  /* the single-face kernel with its own temperature on every edge (thermal_3d.jl:225) */\
+  void ko_apply_fixed_temp_face_3d_##S(T* g, int64_t face, double Tw, int64_t nx,       \
+                                       int64_t ny, int64_t nz) {                         \
+    T t = T(Tw), te[4] = {t, t, t, t};                                                   \
+    bool sk[4] = {false, false, false, false};                                           \
+    ko::fixed_temp_face_3d(g, int(face), t, te, sk, nx, ny, nz);                         \
+  }                                                                                      \
+  void ko_collide_boussinesq_3d_##S(T* f, const T* t, const uint8_t* s, int64_t nx,     \
+                                    int64_t ny, int64_t nz, double om, double bg,        \
+                                    double tr) {                                         \
+    ko::collide_boussinesq_3d(f, t, s, nx, ny, nz, T(om), T(bg), T(tr));                 \
+  }                                                                                      \
+  void ko_collide_guo_3d_##S(T* f, const uint8_t* s, int64_t nx, int64_t ny, int64_t nz, \
+                             double om, double fx, double fy, double fz) {               \
+    ko::collide_guo_3d(f, s, nx, ny, nz, T(om), T(fx), T(fy), T(fz));                    \
+  }                                                                                      \
+  void ko_collide_guo_field_3d_##S(T* f, const uint8_t* s, const T* fx, const T* fy,    \
+                                   const T* fz, int64_t nx, int64_t ny, int64_t nz,      \
+                                   double om) {                                          \
+    ko::collide_guo_3d(f, s, nx, ny, nz, T(om), T(0), T(0), T(0), fx, fy, fz);           \
+  }                                                                                      \
+  void ko_collide_guo_field_2d_##S(T* f, const uint8_t* s, const T* fx, const T* fy,    \
+                                   int64_t nx, int64_t ny, double om) {                  \
+    ko::collide_guo_field_2d(f, s, nx, ny, T(om), fx, fy);                               \
+  }                                                                                      \
+  /* fxa == NULL: constant force (fx, fy, fz) */                                         \
+  void ko_macroscopic_forced_3d_##S(T* r, T* u, T* v, T* w, const T* f, int64_t nx,     \
+                                    int64_t ny, int64_t nz, double fx, double fy,        \
+                                    double fz, const T* fxa, const T* fya,               \
+                                    const T* fza) {                                      \
+    ko::macroscopic_forced_3d(r, u, v, w, f, nx, ny, nz, T(fx), T(fy), T(fz), fxa, fya,  \
+                              fza);                                                      \
+  }                                                                                      \
+  /* ko_run with a per-node force field (KMI_FORCE_GUO_FIELD); fz unused in 2D */        \
+  void ko_run_field_##S(const kmi_desc* d, const uint8_t* solid, T* f, const T* fx,     \
+                        const T* fy, const T* fz, int64_t nsteps) {                      \
+    const int Q = d->stencil == KMI_D2Q9 ? 9 : 19;                                       \
+    const size_t n = size_t(d->nx * d->ny * (d->stencil == KMI_D2Q9 ? 1 : d->nz)) * Q;   \
+    std::vector<T> b1(n), scratch;                                                       \
+    const T* ff[3] = {fx, fy, fz};                                                       \
+    T *fa = f, *fb = b1.data(), *ga = nullptr, *gb = nullptr;                            \
+    for (int64_t s = 0; s < nsteps; ++s)                                                 \
+      ko::generic_step(*d, solid, fa, fb, ga, gb, scratch, nullptr, ff);                 \
+    if (fa != f) std::memcpy(f, fa, n * sizeof(T));                                      \
+  }
+
 #define KO_LIBB(S, T)                                                                     \
   void ko_run_libb_##S(const kmi_desc* d, const uint8_t* solid, T* f, const T* qw,       \
                        const T* uwx, const T* uwy, const T* uwz, int64_t nsteps) {        \
@@ -1529,6 +1859,7 @@ extern "C" {
   }
 
 KO_BOTH(KO_STREAM)
+KO_BOTH(KO_3D)
 KO_BOTH(KO_LIBB)
 
 /* Hot-wall Nusselt number — thermal.jl:225-234 (west wall, rows 2..Ny-1), generalised to
@@ -1552,6 +1883,32 @@ double ko_nusselt_d(const double* Temp, int64_t nx, int64_t ny, int face, double
     sum += nu;
   }
   return sum / double(hi - lo);
+}
+
+/* 3D hot-wall Nusselt number — thermal.jl:525-533 (hot face `face`, T_1 and T_2 the first
+ * two nodes along the normal, H the cell count along it), averaged over the face without
+ * its edge rows (a periodic tangent axis keeps all of them). */
+double ko_nusselt_3d_d(const double* Temp, int64_t nx, int64_t ny, int64_t nz, int face,
+                       double T_hot, double T_cold, int per_t1, int per_t2) {
+  const int64_t n[3] = {nx, ny, nz};
+  const int ax = face / 2, t1 = ax == 0 ? 1 : 0, t2 = ax == 2 ? 1 : 2;
+  const double H = double(n[ax]), dT = T_hot - T_cold;
+  const int64_t lo1 = per_t1 ? 0 : 1, hi1 = per_t1 ? n[t1] : n[t1] - 1;
+  const int64_t lo2 = per_t2 ? 0 : 1, hi2 = per_t2 ? n[t2] : n[t2] - 1;
+  double sum = 0.0;
+  for (int64_t b = lo2; b < hi2; ++b)
+    for (int64_t a = lo1; a < hi1; ++a) {
+      int64_t x1[3], x2[3];
+      x1[t1] = x2[t1] = a;
+      x1[t2] = x2[t2] = b;
+      x1[ax] = (face % 2) ? n[ax] - 1 : 0;
+      x2[ax] = (face % 2) ? n[ax] - 2 : 1;
+      const double T1 = Temp[ko::I3(x1[0], x1[1], x1[2], nx, ny)];
+      const double T2 = Temp[ko::I3(x2[0], x2[1], x2[2], nx, ny)];
+      sum += -H * ((-8 * T_hot + 9 * T1 - T2) / 3.0) / dT;
+    }
+  const int64_t cnt = (hi1 - lo1) * (hi2 - lo2);
+  return sum / double(cnt > 1 ? cnt : 1);
 }
 
 /* Compensated sum of an array (for mass / energy checks). */

@@ -32,6 +32,7 @@ def lib():
             build()
         _lib = ctypes.CDLL(LIB)
         _lib.ko_nusselt_d.restype = c_double
+        _lib.ko_nusselt_3d_d.restype = c_double
         _lib.ko_ksum_d.restype = c_double
         _lib.ko_omp_threads.restype = c_int
     return _lib
@@ -150,6 +151,59 @@ def moments_forced(f, fx, fy):
     return {"rho": r, "ux": u, "uy": v}
 
 
+def run_field(desc, f, fx, fy, fz=None, nsteps=1, mask=None):
+    """As run(), for a KMI_FORCE_GUO_FIELD descriptor: per-node force arrays (f's element
+    type and the lattice shape)."""
+    T = f.dtype
+    arr = [np.asfortranarray(a, dtype=T) for a in (fx, fy, fz if fz is not None else fx)]
+    fn = getattr(lib(), f"ko_run_field_{_sfx(f)}")
+    fn.restype = None
+    m = None if mask is None else np.asfortranarray(mask, dtype=np.uint8)
+    fn(ctypes.byref(desc), None if m is None else m.ctypes.data_as(c_void_p), _p(f), _p(arr[0]),
+       _p(arr[1]), _p(arr[2]), c_int64(int(nsteps)))
+    return f
+
+
+def moments_forced_3d(f, fx=0.0, fy=0.0, fz=0.0, field=None):
+    """compute_macroscopic_forced_3d! (constant force) or _forced_field_3d! (field = (Fx, Fy, Fz))."""
+    nx, ny, nz, _ = f.shape
+    r, u, v, w = (np.empty((nx, ny, nz), dtype=f.dtype, order="F") for _ in range(4))
+    fa = [None] * 3 if field is None else [np.asfortranarray(a, dtype=f.dtype) for a in field]
+    call("macroscopic_forced_3d", r, u, v, w, f, nx, ny, nz, float(fx), float(fy), float(fz),
+         fa[0], fa[1], fa[2], dtype=f.dtype)
+    return {"rho": r, "ux": u, "uy": v, "uz": w}
+
+
+def apply_fixed_temp_3d(g, temps):
+    """apply_fixed_temp_3d! on a copy of g: temps maps face name -> wall temperature."""
+    names = ["west", "east", "south", "north", "bottom", "top"]
+    dirs = np.array([int(n in temps) for n in names], dtype=np.int32)
+    Tf = np.array([float(temps.get(n, 0.0)) for n in names])
+    out = np.array(g, order="F", copy=True)
+    nx, ny, nz, _ = g.shape
+    fn = getattr(lib(), f"ko_apply_fixed_temp_3d_{_sfx(g)}")
+    fn.restype = None
+    fn(_p(out), dirs.ctypes.data_as(c_void_p), Tf.ctypes.data_as(c_void_p), c_int64(nx),
+       c_int64(ny), c_int64(nz))
+    return out
+
+
+def apply_fixed_temp_face_3d(g, face, Tw):
+    """The single-face kernel (apply_fixed_temp_<face>_3d!) on a copy of g."""
+    out = np.array(g, order="F", copy=True)
+    nx, ny, nz, _ = g.shape
+    call("apply_fixed_temp_face_3d", out, int(face), float(Tw), nx, ny, nz, dtype=g.dtype)
+    return out
+
+
+def nusselt_3d(Temp, face, T_hot, T_cold, per_t1=False, per_t2=False):
+    nx, ny, nz = Temp.shape
+    t = np.asfortranarray(Temp, dtype=np.float64)
+    return lib().ko_nusselt_3d_d(t.ctypes.data_as(POINTER(c_double)), c_int64(nx), c_int64(ny),
+                                 c_int64(nz), c_int(face), c_double(T_hot), c_double(T_cold),
+                                 c_int(int(per_t1)), c_int(int(per_t2)))
+
+
 def pull(desc, f):
     out = np.empty_like(f, order="F")
     fn = getattr(lib(), f"ko_pull_generic_{_sfx(f)}")
@@ -172,6 +226,11 @@ def moments(f):
 
 
 def temperature(g):
+    if g.ndim == 4:
+        nx, ny, nz, _ = g.shape
+        t = np.empty((nx, ny, nz), dtype=g.dtype, order="F")
+        call("compute_temperature_3d", t, g, nx, ny, nz, dtype=g.dtype)
+        return t
     nx, ny, _ = g.shape
     t = np.empty((nx, ny), dtype=g.dtype, order="F")
     call("compute_temperature_2d", t, g, nx, ny, dtype=g.dtype)

@@ -51,7 +51,9 @@ def _validate(kmi, d):
     ("zh_velocity_east", 5, "east"),
     ("pressure_north", 5, "Pressure boundary condition on face north"),
     ("thermal_trt", 5, "BGK"),
-    ("thermal_3d", 5, "D2Q9"),
+    ("thermal_3d_fused", 5, "unforced velocity"),
+    ("guo_trt", 5, "BGK"),
+    ("guo_thermal", 5, "thermal"),
     ("zh_3d_normal", 1, "1 + sign*u_n"),
     ("dirichlet_periodic", 1, "periodic"),
     ("bad_size", 1, "positive"),
@@ -68,8 +70,12 @@ def test_desc_validation_errors(kmi, case, status, needle):
         d = mk("d2q9", 8, 8, faces={"north": {"kind": "zouhe_pressure", "rho": 1.0}})
     elif case == "thermal_trt":
         d = mk("d2q9", 8, 8, collision="trt", thermal=True)
-    elif case == "thermal_3d":
-        d = mk("d3q19", 8, 8, 8, thermal=True)
+    elif case == "thermal_3d_fused":
+        d = mk("d3q19", 8, 8, 8, thermal=True, coupling="fused")
+    elif case == "guo_trt":
+        d = mk("d3q19", 8, 8, 8, collision="trt", forcing="guo_field")
+    elif case == "guo_thermal":
+        d = mk("d3q19", 8, 8, 8, thermal=True, forcing="guo")
     elif case == "zh_3d_normal":
         d = mk("d3q19", 8, 8, 8, faces={"top": {"kind": "zouhe_velocity", "u": (0.1, 0, -1.0)}})
     elif case == "dirichlet_periodic":
@@ -108,3 +114,16 @@ def test_d3q19_zou_he_faces_validate(kmi):
              "bottom": {"kind": "zouhe_velocity", "u": (0, 0, 0.01)}, "top": {"kind": "zouhe_velocity", "u": (0.1, 0, 0)}}
     rc, msg = _validate(kmi, kmi.make_desc("d3q19", 8, 8, 8, faces=faces))
     assert rc == 0, msg
+
+
+def test_3d_thermal_and_force_fields_validate(kmi):
+    """The D3Q19 thermal chain (split coupling, Dirichlet on any face) and the Guo force in 3D
+    and per node are accepted."""
+    faces = {"bottom": {"kind": "wall", "thermal": "dirichlet", "T": 1.0},
+             "top": {"kind": "wall", "thermal": "dirichlet", "T": 0.0}}
+    for d in (kmi.make_desc("d3q19", 8, 6, 5, thermal=True, faces=faces),
+              kmi.make_desc("d3q19", 8, 6, 5, forcing="guo", force=(1e-5, 0.0, 2e-5)),
+              kmi.make_desc("d3q19", 8, 6, 5, forcing="guo_field"),
+              kmi.make_desc("d2q9", 8, 6, 1, forcing="guo_field")):
+        rc, msg = _validate(kmi, d)
+        assert rc == 0, msg
