@@ -82,7 +82,18 @@ typedef enum { KMI_BGK = 0, KMI_TRT = 1, KMI_MRT = 2 } kmi_collision;
  *                     set by kmi_set_face_profile; the two corner nodes of the face keep
  *                     the bounce-back pull (boundary_spatial_2d.jl:6-90).
  *   ZOUHE_PRESSURE_PROFILE — D2Q9 east: per-node density, corner nodes skipped
- *                     (boundary_spatial_2d.jl:104-131). */
+ *                     (boundary_spatial_2d.jl:104-131).
+ *   ZOUHE_VELOCITY_REBUILD — west, D2Q9 or D3Q19: the face nodes off the edges
+ *                     (j = 2..Ny-1, k = 2..Nz-1) are rebuilt from the step's pulled
+ *                     populations — Zou–He closure for the normal velocity of the per-node
+ *                     profile set by kmi_set_face_profile (length ny, or ny*nz with index
+ *                     j + ny k), then a local TRT collision at the lattice's rates (D2Q9:
+ *                     the regularised form of _trt_collide_local) — replacing the node's
+ *                     own update; the edge nodes keep the bounce-back pull
+ *                     (apply_bc_rebuild_2d!/_3d!, src/bc/rebuild_2d.jl:1-23,
+ *                     rebuild_3d.jl:1-45, specs.jl:7-108).  TRT lattices only.
+ *   ZOUHE_PRESSURE_REBUILD — east, D2Q9 or D3Q19: the same with the Zou–He pressure
+ *                     closure for `rho` (rebuild_2d.jl:25-50, rebuild_3d.jl:47-89). */
 typedef enum {
   KMI_FACE_PERIODIC = 0,
   KMI_FACE_WALL = 1,
@@ -90,7 +101,9 @@ typedef enum {
   KMI_FACE_ZOUHE_VELOCITY = 3,
   KMI_FACE_ZOUHE_PRESSURE = 4,
   KMI_FACE_ZOUHE_VELOCITY_PROFILE = 5,
-  KMI_FACE_ZOUHE_PRESSURE_PROFILE = 6
+  KMI_FACE_ZOUHE_PRESSURE_PROFILE = 6,
+  KMI_FACE_ZOUHE_VELOCITY_REBUILD = 7,
+  KMI_FACE_ZOUHE_PRESSURE_REBUILD = 8
 } kmi_face_kind;
 
 /* Body force on the flow populations (BGK, D2Q9 or D3Q19).

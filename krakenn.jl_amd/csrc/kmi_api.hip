@@ -168,7 +168,7 @@ static int validate(const kmi_desc& d, std::string& why, int& code) {
     return code = KMI_ERR_UNSUPPORTED, why = "more than 2^31 populations per field", 1;
   const int nf = d.stencil == KMI_D2Q9 ? 4 : 6;
   for (int f = 0; f < nf; ++f)
-    if (d.faces[f].kind < KMI_FACE_PERIODIC || d.faces[f].kind > KMI_FACE_ZOUHE_PRESSURE_PROFILE)
+    if (d.faces[f].kind < KMI_FACE_PERIODIC || d.faces[f].kind > KMI_FACE_ZOUHE_PRESSURE_REBUILD)
       return why = std::string("unknown face kind on ") + kFaceName[f], 1;
   for (int a = 0; a < nf / 2; ++a) {
     const bool p0 = d.faces[2 * a].kind == KMI_FACE_PERIODIC;
@@ -201,6 +201,18 @@ static int validate(const kmi_desc& d, std::string& why, int& code) {
            1;
   for (int f = 0; f < nf; ++f) {
     const kmi_face& F = d.faces[f];
+    if (F.kind == KMI_FACE_ZOUHE_VELOCITY_REBUILD || F.kind == KMI_FACE_ZOUHE_PRESSURE_REBUILD) {
+      const bool vel = F.kind == KMI_FACE_ZOUHE_VELOCITY_REBUILD;
+      if (f != (vel ? KMI_WEST : KMI_EAST))
+        return code = KMI_ERR_UNSUPPORTED,
+               why = std::string(vel ? "the velocity rebuild is a west face" : "the pressure rebuild is an east face") +
+                     " (apply_bc_rebuild_2d!/_3d!: west inlet, east outlet)",
+               1;
+      if (d.collision != KMI_TRT || d.thermal)
+        return code = KMI_ERR_UNSUPPORTED,
+               why = "rebuilt faces collide with TRT rates (src/bc/specs.jl:7-108): TRT, isothermal lattices only", 1;
+      if (!vel && !(F.rho > 0.0)) return why = "rebuild pressure density must be positive", 1;
+    }
     if (F.kind == KMI_FACE_ZOUHE_VELOCITY_PROFILE || F.kind == KMI_FACE_ZOUHE_PRESSURE_PROFILE) {
       if (d.stencil != KMI_D2Q9)
         return code = KMI_ERR_UNSUPPORTED, why = "profile faces are D2Q9 only (boundary_spatial_2d.jl)", 1;
@@ -256,9 +268,13 @@ static int validate(const kmi_desc& d, std::string& why, int& code) {
   return 0;
 }
 
-// Nodes along a face: nx for south/north, ny for west/east (D2Q9).
+// Nodes of a face: nx / ny along a D2Q9 face; the product of the two tangent extents in 3D.
 static int64_t face_length(const kmi_lattice* L, int f) {
-  return (f == KMI_SOUTH || f == KMI_NORTH) ? L->desc.nx : L->desc.ny;
+  const kmi_desc& d = L->desc;
+  if (L->Q == 9) return (f == KMI_SOUTH || f == KMI_NORTH) ? d.nx : d.ny;
+  const int64_t n[3] = {d.nx, d.ny, d.nz};
+  const int ax = f / 2;
+  return n[ax == 0 ? 1 : 0] * n[ax == 2 ? 1 : 2];
 }
 
 static void build_args(kmi_lattice* L) {
@@ -284,7 +300,7 @@ static void build_args(kmi_lattice* L) {
   a.n_zh = 0;
   for (int i = 0; i < n; ++i) {
     const int k = d.faces[order[i]].kind;
-    if (k >= KMI_FACE_ZOUHE_VELOCITY) a.zh_face[a.n_zh++] = order[i];
+    if (k >= KMI_FACE_ZOUHE_VELOCITY && k <= KMI_FACE_ZOUHE_PRESSURE_PROFILE) a.zh_face[a.n_zh++] = order[i];
   }
   a.omega = d.omega;
   a.s_plus = d.s_plus;
@@ -503,9 +519,11 @@ int kmi_lattice_create(kmi_ctx* ctx, const kmi_desc* desc, kmi_lattice** out) {
   for (int p = 0; p < 2 && e == hipSuccess; ++p) e = hipMalloc(&L->f[p], bytes);
   for (int p = 0; p < 2 && e == hipSuccess && desc->thermal; ++p) e = hipMalloc(&L->g[p], gbytes);
   // Profile faces: velocity profiles start at zero, pressure profiles at the face's rho.
-  for (int f = 0; f < 4 && e == hipSuccess && L->Q == 9; ++f) {
+  for (int f = 0; f < 4 && e == hipSuccess; ++f) {
     const int kind = desc->faces[f].kind;
-    if (kind != KMI_FACE_ZOUHE_VELOCITY_PROFILE && kind != KMI_FACE_ZOUHE_PRESSURE_PROFILE) continue;
+    if (kind != KMI_FACE_ZOUHE_VELOCITY_PROFILE && kind != KMI_FACE_ZOUHE_PRESSURE_PROFILE &&
+        kind != KMI_FACE_ZOUHE_VELOCITY_REBUILD)
+      continue;
     const size_t len = size_t(face_length(L, f));
     const std::vector<double> init(len, kind == KMI_FACE_ZOUHE_PRESSURE_PROFILE ? desc->faces[f].rho : 0.0);
     for (int k = 0; k < (kind == KMI_FACE_ZOUHE_VELOCITY_PROFILE ? 2 : 1) && e == hipSuccess; ++k) {
@@ -693,10 +711,11 @@ int kmi_set_cut_links(kmi_lattice* L, int64_t n, const int32_t* cell, const int3
 int kmi_set_face_profile(kmi_lattice* L, int32_t face, const double* a, const double* b) {
   if (!L) return fail(nullptr, KMI_ERR_ARG, "kmi_set_face_profile: lattice is NULL");
   kmi_ctx* c = L->ctx;
-  if (face < 0 || face >= 4 || L->Q != 9)
-    return fail(c, KMI_ERR_ARG, "kmi_set_face_profile: face must be a D2Q9 face (0..3)");
+  if (face < 0 || face >= (L->Q == 9 ? 4 : 6))
+    return fail(c, KMI_ERR_ARG, "kmi_set_face_profile: unknown face");
   const int kind = L->desc.faces[face].kind;
-  if (kind != KMI_FACE_ZOUHE_VELOCITY_PROFILE && kind != KMI_FACE_ZOUHE_PRESSURE_PROFILE)
+  if (kind != KMI_FACE_ZOUHE_VELOCITY_PROFILE && kind != KMI_FACE_ZOUHE_PRESSURE_PROFILE &&
+      kind != KMI_FACE_ZOUHE_VELOCITY_REBUILD)
     return fail(c, KMI_ERR_STATE, std::string("kmi_set_face_profile: face ") + kFaceName[face] +
                                       " is not a profile face");
   if (!a || (kind == KMI_FACE_ZOUHE_VELOCITY_PROFILE && !b))

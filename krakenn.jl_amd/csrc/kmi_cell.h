@@ -444,6 +444,117 @@ KMI_DEV void zh3_apply(R (&f)[19], const FaceArgs& Fa) {
     zh3_pressure<SH, F>(f, R(Fa.rho), R(Fa.rho - 1.0));
 }
 
+// ---- face rebuild (apply_bc_rebuild_2d!/_3d!, src/bc/rebuild_2d.jl, rebuild_3d.jl) ---------
+// A rebuilt face node takes its pulled populations (the unknown ones are overwritten), the
+// Zou–He closure of the rebuild kernels and a local TRT collision; plain populations.
+
+// Regularised TRT of one D2Q9 node (_trt_collide_local, src/bc/specs.jl:7-50): the
+// non-equilibrium part is rebuilt from the stress Π alone, then relaxed with TRT rates.
+template <class R>
+KMI_DEV void rebuild_collide9(const R (&f)[9], R (&o)[9], R sp, R sm) {
+  const R rho = zeroth_moment<9>(f);
+  const R ux = (f[1] - f[3] + f[5] - f[7] + f[8] - f[6]) / rho;
+  const R uy = (f[2] - f[4] + f[5] - f[7] + f[6] - f[8]) / rho;
+  R fe[9];
+  equilibria<9, false>(rho, rho, ux, uy, R(0), fe);
+  const R Pxx = (f[1] - fe[1]) + (f[3] - fe[3]) + (f[5] - fe[5]) + (f[6] - fe[6]) + (f[7] - fe[7]) + (f[8] - fe[8]);
+  const R Pyy = (f[2] - fe[2]) + (f[4] - fe[4]) + (f[5] - fe[5]) + (f[6] - fe[6]) + (f[7] - fe[7]) + (f[8] - fe[8]);
+  const R Pxy = (f[5] - fe[5]) - (f[6] - fe[6]) + (f[7] - fe[7]) - (f[8] - fe[8]);
+  const R h = R(0.5);
+  const R n0 = -h * R(2.0 / 9.0) * (Pxx + Pyy);
+  const R n1 = h * R(1.0 / 9.0) * (R(2) * Pxx - Pyy);
+  const R n2 = h * R(1.0 / 9.0) * (-Pxx + R(2) * Pyy);
+  const R n5 = h * R(1.0 / 36.0) * (Pxx + Pyy) + R(1.0 / 4.0) * Pxy;
+  const R n6 = h * R(1.0 / 36.0) * (Pxx + Pyy) - R(1.0 / 4.0) * Pxy;
+  const R a = (sp + sm) * h, b = (sp - sm) * h;
+  // n3 = n1, n4 = n2, n7 = n5, n8 = n6
+  o[0] = fe[0] + (R(1) - sp) * n0;
+  o[1] = fe[1] + (R(1) - a) * n1 - b * n1;
+  o[2] = fe[2] + (R(1) - a) * n2 - b * n2;
+  o[3] = fe[3] + (R(1) - a) * n1 - b * n1;
+  o[4] = fe[4] + (R(1) - a) * n2 - b * n2;
+  o[5] = fe[5] + (R(1) - a) * n5 - b * n5;
+  o[6] = fe[6] + (R(1) - a) * n6 - b * n6;
+  o[7] = fe[7] + (R(1) - a) * n5 - b * n5;
+  o[8] = fe[8] + (R(1) - a) * n6 - b * n6;
+}
+
+// TRT of one D3Q19 node with the rebuild kernels' moments (_trt_collide_local_3d,
+// src/bc/specs.jl:52-108): the velocity sums are grouped by opposite pairs and divided by ρ.
+template <class R>
+KMI_DEV void rebuild_collide19(const R (&f)[19], R (&o)[19], R sp, R sm) {
+  const R rho = zeroth_moment<19>(f);
+  const R ux = ((f[1] - f[2]) + (f[7] - f[10]) + (f[9] - f[8]) + (f[11] - f[14]) + (f[13] - f[12])) / rho;
+  const R uy = ((f[3] - f[4]) + (f[7] - f[10]) + (f[8] - f[9]) + (f[15] - f[18]) + (f[17] - f[16])) / rho;
+  const R uz = ((f[5] - f[6]) + (f[11] - f[14]) + (f[12] - f[13]) + (f[15] - f[18]) + (f[16] - f[17])) / rho;
+  const R a = (sp + sm) * R(0.5), b = (sp - sm) * R(0.5);
+  R fe[19];
+  equilibria<19, false>(rho, rho, ux, uy, uz, fe);
+  o[0] = f[0] - sp * (f[0] - fe[0]);
+  sfor<18>([&](auto qc) {
+    constexpr int q = decltype(qc)::value + 1;
+    constexpr int p = Lat<19>::opp[q];
+    o[q] = f[q] - a * (f[q] - fe[q]) - b * (f[p] - fe[p]);
+  });
+}
+
+// Zou–He closures of the rebuild kernels (rebuild_2d.jl:1-50, rebuild_3d.jl:1-89):
+// west normal velocity u / east density rho, zero tangential velocity.
+template <class R>
+KMI_DEV void rebuild_zh9(R (&f)[9], int kind, R v) {
+  const R c23 = R(2.0 / 3.0), c16 = R(1.0 / 6.0), h = R(0.5);
+  if (kind == 1) {
+    const R rw = (f[0] + f[2] + f[4] + R(2) * (f[3] + f[6] + f[7])) / (R(1) - v);
+    f[1] = f[3] + c23 * rw * v;
+    f[5] = f[7] - h * (f[2] - f[4]) + c16 * rw * v;
+    f[8] = f[6] + h * (f[2] - f[4]) + c16 * rw * v;
+  } else {
+    const R u = -R(1) + (f[0] + f[2] + f[4] + R(2) * (f[1] + f[5] + f[8])) / v;
+    f[3] = f[1] - c23 * v * u;
+    f[6] = f[8] - h * (f[2] - f[4]) - c16 * v * u;
+    f[7] = f[5] + h * (f[2] - f[4]) - c16 * v * u;
+  }
+}
+
+template <class R>
+KMI_DEV void rebuild_zh19(R (&f)[19], int kind, R v) {
+  const R c13 = R(1.0 / 3.0), c16 = R(1.0 / 6.0), h = R(0.5);
+  const R par = f[0] + f[3] + f[4] + f[5] + f[6] + f[15] + f[16] + f[17] + f[18];
+  const R t1 = f[3] - f[4], t2 = f[5] - f[6];
+  if (kind == 1) {
+    const R out = f[2] + f[8] + f[10] + f[12] + f[14];
+    const R rw = (par + R(2) * out) / (R(1) - v);
+    f[1] = f[2] + c13 * rw * v;
+    f[7] = f[10] - h * t1 + c16 * rw * v;
+    f[9] = f[8] + h * t1 + c16 * rw * v;
+    f[11] = f[14] - h * t2 + c16 * rw * v;
+    f[13] = f[12] + h * t2 + c16 * rw * v;
+  } else {
+    const R out = f[1] + f[7] + f[9] + f[11] + f[13];
+    const R u = -(R(1) - (par + R(2) * out) / v);
+    f[2] = f[1] - c13 * v * u;
+    f[8] = f[9] - h * t1 - c16 * v * u;
+    f[10] = f[7] + h * t1 - c16 * v * u;
+    f[12] = f[13] - h * t2 - c16 * v * u;
+    f[14] = f[11] + h * t2 - c16 * v * u;
+  }
+}
+
+// kind 1: west velocity v = u_n; kind 2: east pressure v = rho.  Shifted storage is
+// converted to plain populations and back around the rebuild.
+template <int Q, bool SH, class R>
+KMI_DEV void rebuild_node(R (&fp)[Q], R (&fo)[Q], int kind, R v, R sp, R sm) {
+  if constexpr (SH) sfor<Q>([&](auto qc) { fp[decltype(qc)::value] += R(Lat<Q>::w[decltype(qc)::value]); });
+  if constexpr (Q == 9) {
+    rebuild_zh9(fp, kind, v);
+    rebuild_collide9(fp, fo, sp, sm);
+  } else {
+    rebuild_zh19(fp, kind, v);
+    rebuild_collide19(fp, fo, sp, sm);
+  }
+  if constexpr (SH) sfor<Q>([&](auto qc) { fo[decltype(qc)::value] -= R(Lat<Q>::w[decltype(qc)::value]); });
+}
+
 // ---- interpolated bounce-back on cut links (LI-BB V2 pre-phase) ------------------------
 // Order in which the reference's pre-phase visits the links: opposite pairs, q before
 // opp(q) (dsl/bricks.jl:518-581, dsl/bricks_3d.jl:368-473).  A pair with both links cut

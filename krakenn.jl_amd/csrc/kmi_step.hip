@@ -18,6 +18,7 @@
 //   thermal DDF        thermal_2d.jl:16-60, 140-290, 421-480; fused_thermal_2d.jl:10-207;
 //                      thermal_3d.jl:12-136, 164-535, 559-712
 //   Guo forcing        collide_guo_2d.jl:5-140, collide_guo_3d.jl:5-325
+//   face rebuild       bc/rebuild_2d.jl:1-50, bc/rebuild_3d.jl:1-89, bc/specs.jl:7-108
 // F32S / F16S store deviations from the rest state and collide them in FP32 (the
 // algebra is identical; only the rounding differs).
 #include "kmi_cell.h"
@@ -152,10 +153,17 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs a) {
     }
   }
 
+  // Rebuilt face node (apply_bc_rebuild_2d!/_3d!): the west / east face off its edges.
+  int rb = 0;
+  if constexpr (BND) {
+    const bool inner = j > 0 && j < ny - 1 && (Q == 9 || (k > 0 && k < a.nz - 1));
+    if (inner && i == 0 && a.face[KMI_WEST].kind == KMI_FACE_ZOUHE_VELOCITY_REBUILD) rb = 1;
+    if (inner && i == nx - 1 && a.face[KMI_EAST].kind == KMI_FACE_ZOUHE_PRESSURE_REBUILD) rb = 2;
+  }
   const unsigned char cls = (BND && a.mask != nullptr) ? a.mask[self] : kCellFluid;
   const bool solid = cls == kCellSolid;
   if constexpr (BND) {
-    if (cls == kCellCut) libb_prephase<Q, S>(fp, fi, self, P, a.cut_rec + (long long)a.cut_slot[self] * (4 * Q));
+    if (cls == kCellCut && !rb) libb_prephase<Q, S>(fp, fi, self, P, a.cut_rec + (long long)a.cut_slot[self] * (4 * Q));
   }
 
   if constexpr (TH) {
@@ -216,7 +224,10 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs a) {
       GS::template st<NTS>(go, self + q * P, go_r[q]);
     });
   } else {
-    if (solid && a.solid_inert) {
+    if (rb) {
+      const R v = rb == 1 ? R(a.face[KMI_WEST].prof[0][j + (Q == 19 ? ny * k : 0)]) : R(a.face[KMI_EAST].rho);
+      rebuild_node<Q, SH>(fp, fo_r, rb, v, R(a.s_plus), R(a.s_minus));
+    } else if (solid && a.solid_inert) {
       // SolidInert (dsl/bricks.jl:133-154): rest equilibrium, w_q (0 as a deviation)
       sfor<Q>([&](auto qc) { fo_r[decltype(qc)::value] = SH ? R(0) : R(Lat<Q>::w[decltype(qc)::value]); });
     } else if (solid) {

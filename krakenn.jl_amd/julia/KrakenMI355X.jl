@@ -37,6 +37,8 @@ end
 const FACE_PERIODIC, FACE_WALL, FACE_MOVING_WALL, FACE_ZOUHE_VELOCITY, FACE_ZOUHE_PRESSURE = 0, 1, 2, 3, 4
 const FACE_ZOUHE_VELOCITY_PROFILE, FACE_ZOUHE_PRESSURE_PROFILE = 5, 6
 const STORAGE = Dict(Float64 => Int32(0), Float32 => Int32(1))
+const FORCE_NONE, FORCE_GUO, FORCE_GUO_PLAIN_MOMENTS, FORCE_GUO_FIELD = 0, 1, 2, 3
+const HOST_F64, HOST_F32, HOST_STORAGE = 0, 1, 2
 
 lasterr(ctx) = unsafe_string(ccall((:kmi_last_error, libkmi), Cstring, (Ptr{Cvoid},), ctx))
 
@@ -105,6 +107,26 @@ function moments(l::Lattice)
                 l.h, ρ, ux, uy, uz === nothing ? C_NULL : uz, C_NULL), l.backend.ctx)
     return (ρ=ρ, ux=ux, uy=uy, uz=uz)
 end
+
+# Per-node force of a FORCE_GUO_FIELD lattice: the Fx_field, Fy_field(, Fz_field) arrays of
+# collide_guo_field_2d!/_3d! (collide_guo_2d.jl:93-160, collide_guo_3d.jl:199-332).
+function set_force_field!(l::Lattice, Fx::Array{Float64}, Fy::Array{Float64},
+                          Fz::Union{Nothing,Array{Float64}}=nothing)
+    check(ccall((:kmi_set_force_field, libkmi), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                l.h, Fx, Fy, Fz === nothing ? C_NULL : Fz), l.backend.ctx)
+end
+
+# The stored populations verbatim (F64 storage: Float64, F32: Float32), for checkpoints of a
+# lattice (src/platform/state.jl) that restart bit for bit; set_steps! restores the cycle.
+function download_raw(l::Lattice, ::Type{E}) where {E<:Union{Float64,Float32}}
+    f = Array{E}(undef, l.dims[1], l.dims[2], (l.Q == 19 ? (l.dims[3],) : ())..., l.Q)
+    check(ccall((:kmi_download, libkmi), Cint, (Ptr{Cvoid}, Int32, Ptr{E}, Int32),
+                l.h, 0, f, HOST_STORAGE), l.backend.ctx)
+    return f
+end
+set_steps!(l::Lattice, n::Integer) =
+    check(ccall((:kmi_lattice_set_steps, libkmi), Cint, (Ptr{Cvoid}, Int64), l.h, n), l.backend.ctx)
 
 function desc_2d(Nx, Ny, ν, T; faces)
     w = KmiFace(FACE_WALL, 0, (0.0, 0.0, 0.0), 1.0, 0.0)

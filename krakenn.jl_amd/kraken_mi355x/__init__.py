@@ -39,13 +39,14 @@ from .drivers import (
 )
 from .lattice import ArgumentError, Lattice, MI355XBackend, make_desc
 from .libb import (cut_links_from_dense, precompute_q_wall_cylinder, precompute_q_wall_sphere_3d,
+                   run_cylinder_libb_2d, run_sphere_libb_3d,
                    wall_velocity_rotating_cylinder)
 from .runner import (BoundarySetup, DomainSetup, GeometryRegion, InitialSetup, PhysicsSetup,
                      SimulationSetup, plan_simulation, run_simulation)
 
 __all__ = [
     "BoundarySetup", "DomainSetup", "GeometryRegion", "InitialSetup", "PhysicsSetup", "SimulationSetup",
-    "plan_simulation", "run_simulation", "cut_links_from_dense", "precompute_q_wall_cylinder",
+    "plan_simulation", "run_simulation", "run_cylinder_libb_2d", "run_sphere_libb_3d", "cut_links_from_dense", "precompute_q_wall_cylinder",
     "precompute_q_wall_sphere_3d", "wall_velocity_rotating_cylinder",
     "Batch", "STATUS_NAMES", "ensemble_parameters", "ArgumentError", "D2Q9", "D3Q19", "KrakenError", "LBMConfig", "Lattice", "MI355XBackend",
     "cavity_3d_desc", "cavity_desc", "lbm_config", "load", "make_desc", "natural_convection_3d_desc", "natural_convection_3d_initial_temperature", "natural_convection_desc",

@@ -19,7 +19,8 @@ D2Q9, D3Q19 = 0, 1
 STORE_F64, STORE_F32, STORE_F32S, STORE_F16S = 0, 1, 2, 3
 BGK, TRT, MRT = 0, 1, 2
 (FACE_PERIODIC, FACE_WALL, FACE_MOVING_WALL, FACE_ZOUHE_VELOCITY, FACE_ZOUHE_PRESSURE,
- FACE_ZOUHE_VELOCITY_PROFILE, FACE_ZOUHE_PRESSURE_PROFILE) = range(7)
+ FACE_ZOUHE_VELOCITY_PROFILE, FACE_ZOUHE_PRESSURE_PROFILE, FACE_ZOUHE_VELOCITY_REBUILD,
+ FACE_ZOUHE_PRESSURE_REBUILD) = range(9)
 FORCE_NONE, FORCE_GUO, FORCE_GUO_PLAIN_MOMENTS, FORCE_GUO_FIELD = 0, 1, 2, 3
 WEST, EAST, SOUTH, NORTH, BOTTOM, TOP = range(6)
 THERMAL_ADIABATIC, THERMAL_DIRICHLET = 0, 1

@@ -106,7 +106,9 @@ def make_desc(stencil, nx, ny, nz=1, nu=0.1, storage="f64", collision="bgk", fac
 FACE_KINDS = {"periodic": L.FACE_PERIODIC, "wall": L.FACE_WALL, "moving_wall": L.FACE_MOVING_WALL,
               "zouhe_velocity": L.FACE_ZOUHE_VELOCITY, "zouhe_pressure": L.FACE_ZOUHE_PRESSURE,
               "zouhe_velocity_profile": L.FACE_ZOUHE_VELOCITY_PROFILE,
-              "zouhe_pressure_profile": L.FACE_ZOUHE_PRESSURE_PROFILE}
+              "zouhe_pressure_profile": L.FACE_ZOUHE_PRESSURE_PROFILE,
+              "zouhe_velocity_rebuild": L.FACE_ZOUHE_VELOCITY_REBUILD,
+              "zouhe_pressure_rebuild": L.FACE_ZOUHE_PRESSURE_REBUILD}
 
 
 def _fill_face(f, spec):
@@ -216,12 +218,19 @@ class Lattice:
         self._chk(L.load().kmi_init_equilibrium(self._h, *[_dptr(a) for a in arrs]))
 
     def set_face_profile(self, face, a, b=None):
-        """Per-node values of a profile face: (u_x, u_y) arrays for a velocity profile, or the
-        density array `a` for a pressure profile (length nx on south/north, ny on west/east)."""
+        """Per-node values of a profile face: (u_x, u_y) arrays for a velocity profile, the
+        density array `a` for a pressure profile, or the normal velocity `a` of a rebuilt
+        inlet (length nx on south/north, ny on west/east; on a D3Q19 face an array of the two
+        tangent extents, e.g. (ny, nz) on west)."""
         fid = L.FACE_NAMES[face] if isinstance(face, str) else int(face)
-        n = self.shape[0] if fid in (L.SOUTH, L.NORTH) else self.shape[1]
-        pa = np.ascontiguousarray(np.broadcast_to(a, (n,)), dtype=np.float64)
-        pb = None if b is None else np.ascontiguousarray(np.broadcast_to(b, (n,)), dtype=np.float64)
+        if self.Q == 9:
+            tshape = (self.shape[0],) if fid in (L.SOUTH, L.NORTH) else (self.shape[1],)
+        else:
+            ax = fid // 2
+            tshape = tuple(self.shape[t] for t in range(3) if t != ax)
+        flat = lambda v: np.asfortranarray(np.broadcast_to(v, tshape), dtype=np.float64).ravel(order="F")
+        pa = flat(a)
+        pb = None if b is None else flat(b)
         self._chk(L.load().kmi_set_face_profile(self._h, fid, _dptr(pa), _dptr(pb)))
         self.aux.setdefault("profiles", {})[fid] = (pa.copy(), None if pb is None else pb.copy())
 

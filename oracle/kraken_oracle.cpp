@@ -1385,6 +1385,160 @@ static void zou_he_faces(const kmi_desc& d, T* f, const Profiles* pr = nullptr) 
   }
 }
 
+// ---- face rebuild: apply_bc_rebuild_2d!/_3d! (src/bc/rebuild_2d.jl, rebuild_3d.jl) ----------
+// _trt_collide_local — src/bc/specs.jl:7-50 (regularised TRT, 1-based names in comments)
+template <class T>
+static void trt_collide_local_2d(const T* f, T* o, T sp, T sm) {
+  T rho = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8];
+  T ux = (f[1] - f[3] + f[5] - f[7] + f[8] - f[6]) / rho;  // f2 - f4 + f6 - f8 + f9 - f7
+  T uy = (f[2] - f[4] + f[5] - f[7] + f[6] - f[8]) / rho;  // f3 - f5 + f6 - f8 + f7 - f9
+  T usq = ux * ux + uy * uy;
+  T fe[9];
+  fe[0] = T(W9[0]) * rho * (T(1) - T(1.5) * usq);
+  for (int q = 1; q < 9; ++q) fe[q] = feq<T>(W9[q], rho, cdotu<T>(CX9[q], CY9[q], 0, ux, uy, T(0)), usq);
+  T Pxx = (f[1] - fe[1]) + (f[3] - fe[3]) + (f[5] - fe[5]) + (f[6] - fe[6]) + (f[7] - fe[7]) + (f[8] - fe[8]);
+  T Pyy = (f[2] - fe[2]) + (f[4] - fe[4]) + (f[5] - fe[5]) + (f[6] - fe[6]) + (f[7] - fe[7]) + (f[8] - fe[8]);
+  T Pxy = (f[5] - fe[5]) - (f[6] - fe[6]) + (f[7] - fe[7]) - (f[8] - fe[8]);
+  T h = T(0.5);
+  T fn[9];
+  fn[0] = -h * T(2.0 / 9.0) * (Pxx + Pyy);
+  fn[1] = h * T(1.0 / 9.0) * (T(2) * Pxx - Pyy);
+  fn[2] = h * T(1.0 / 9.0) * (-Pxx + T(2) * Pyy);
+  fn[3] = fn[1];
+  fn[4] = fn[2];
+  fn[5] = h * T(1.0 / 36.0) * (Pxx + Pyy) + T(1.0 / 4.0) * Pxy;
+  fn[6] = h * T(1.0 / 36.0) * (Pxx + Pyy) - T(1.0 / 4.0) * Pxy;
+  fn[7] = fn[5];
+  fn[8] = fn[6];
+  T a = (sp + sm) * h, b = (sp - sm) * h;
+  o[0] = fe[0] + (T(1) - sp) * fn[0];
+  for (int q = 1; q < 9; ++q) o[q] = fe[q] + (T(1) - a) * fn[q] - b * fn[OPP9[q]];
+}
+
+// _trt_collide_local_3d — src/bc/specs.jl:52-108
+template <class T>
+static void trt_collide_local_3d(const T* f, T* o, T sp, T sm) {
+  T rho = f[0];
+  for (int q = 1; q < 19; ++q) rho = rho + f[q];
+  T ux = ((f[1] - f[2]) + (f[7] - f[10]) + (f[9] - f[8]) + (f[11] - f[14]) + (f[13] - f[12])) / rho;
+  T uy = ((f[3] - f[4]) + (f[7] - f[10]) + (f[8] - f[9]) + (f[15] - f[18]) + (f[17] - f[16])) / rho;
+  T uz = ((f[5] - f[6]) + (f[11] - f[14]) + (f[12] - f[13]) + (f[15] - f[18]) + (f[16] - f[17])) / rho;
+  T a = (sp + sm) * T(0.5), b = (sp - sm) * T(0.5);
+  T usq = ux * ux + uy * uy + uz * uz;
+  T fe[19];
+  fe[0] = T(W19[0]) * rho * (T(1) - T(1.5) * usq);
+  for (int q = 1; q < 19; ++q)
+    fe[q] = feq<T>(W19[q], rho, cdotu<T>(CX19[q], CY19[q], CZ19[q], ux, uy, uz), usq);
+  o[0] = f[0] - sp * (f[0] - fe[0]);
+  for (int q = 1; q < 19; ++q) o[q] = f[q] - a * (f[q] - fe[q]) - b * (f[OPP19[q]] - fe[OPP19[q]]);
+}
+
+// _bc_west_zh_velocity_2d! / _bc_east_zh_pressure_2d! — rebuild_2d.jl:1-50, rows j = 2..Ny-1;
+// reads the pre-step populations fi at the reference's explicit neighbour indices.
+template <class T>
+static void rebuild_2d(T* fo, const T* fi, const kmi_desc& d, const double* prof, T sp, T sm) {
+  const int64_t nx = d.nx, ny = d.ny;
+  const size_t P = size_t(nx * ny);
+  auto F = [&](int64_t i, int64_t j, int q) { return fi[I2(i, j, nx) + size_t(q) * P]; };
+  for (int64_t j = 1; j < ny - 1; ++j) {
+    T fp[9], o[9];
+    if (d.faces[KMI_WEST].kind == KMI_FACE_ZOUHE_VELOCITY_REBUILD) {
+      fp[0] = F(0, j, 0); fp[2] = F(0, j - 1, 2); fp[3] = F(1, j, 3);
+      fp[4] = F(0, j + 1, 4); fp[6] = F(1, j - 1, 6); fp[7] = F(1, j + 1, 7);
+      T u_in = T(prof[j]);
+      T rho_w = (fp[0] + fp[2] + fp[4] + T(2) * (fp[3] + fp[6] + fp[7])) / (T(1) - u_in);
+      fp[1] = fp[3] + T(2.0 / 3.0) * rho_w * u_in;
+      fp[5] = fp[7] - T(0.5) * (fp[2] - fp[4]) + T(1.0 / 6.0) * rho_w * u_in;
+      fp[8] = fp[6] + T(0.5) * (fp[2] - fp[4]) + T(1.0 / 6.0) * rho_w * u_in;
+      trt_collide_local_2d(fp, o, sp, sm);
+      for (int q = 0; q < 9; ++q) fo[I2(0, j, nx) + size_t(q) * P] = o[q];
+    }
+    if (d.faces[KMI_EAST].kind == KMI_FACE_ZOUHE_PRESSURE_REBUILD) {
+      const int64_t e = nx - 1;
+      fp[0] = F(e, j, 0); fp[1] = F(e - 1, j, 1); fp[2] = F(e, j - 1, 2);
+      fp[4] = F(e, j + 1, 4); fp[5] = F(e - 1, j - 1, 5); fp[8] = F(e - 1, j + 1, 8);
+      T rho = T(d.faces[KMI_EAST].rho);
+      T u_x = -T(1) + (fp[0] + fp[2] + fp[4] + T(2) * (fp[1] + fp[5] + fp[8])) / rho;
+      fp[3] = fp[1] - T(2.0 / 3.0) * rho * u_x;
+      fp[6] = fp[8] - T(0.5) * (fp[2] - fp[4]) - T(1.0 / 6.0) * rho * u_x;
+      fp[7] = fp[5] + T(0.5) * (fp[2] - fp[4]) - T(1.0 / 6.0) * rho * u_x;
+      trt_collide_local_2d(fp, o, sp, sm);
+      for (int q = 0; q < 9; ++q) fo[I2(e, j, nx) + size_t(q) * P] = o[q];
+    }
+  }
+}
+
+// _bc_west_zh_velocity_3d! / _bc_east_zh_pressure_3d! — rebuild_3d.jl:1-89, j = 2..Ny-1,
+// k = 2..Nz-1; the west profile is the (Ny, Nz) matrix, index j + Ny k.
+template <class T>
+static void rebuild_3d(T* fo, const T* fi, const kmi_desc& d, const double* prof, T sp, T sm) {
+  const int64_t nx = d.nx, ny = d.ny, nz = d.nz;
+  const size_t P = size_t(nx * ny * nz);
+  auto F = [&](int64_t i, int64_t j, int64_t k, int q) { return fi[I3(i, j, k, nx, ny) + size_t(q) * P]; };
+  for (int64_t k = 1; k < nz - 1; ++k)
+    for (int64_t j = 1; j < ny - 1; ++j) {
+      for (int side = 0; side < 2; ++side) {
+        const bool west = side == 0;
+        if (west && d.faces[KMI_WEST].kind != KMI_FACE_ZOUHE_VELOCITY_REBUILD) continue;
+        if (!west && d.faces[KMI_EAST].kind != KMI_FACE_ZOUHE_PRESSURE_REBUILD) continue;
+        const int64_t b = west ? 0 : nx - 1, n = west ? 1 : nx - 2;  // boundary and next column
+        T fp[19], o[19];
+        fp[0] = F(b, j, k, 0);
+        fp[3] = F(b, j - 1, k, 3);
+        fp[4] = F(b, j + 1, k, 4);
+        fp[5] = F(b, j, k - 1, 5);
+        fp[6] = F(b, j, k + 1, 6);
+        fp[15] = F(b, j - 1, k - 1, 15);
+        fp[16] = F(b, j + 1, k - 1, 16);
+        fp[17] = F(b, j - 1, k + 1, 17);
+        fp[18] = F(b, j + 1, k + 1, 18);
+        T sum_par = fp[0] + fp[3] + fp[4] + fp[5] + fp[6] + fp[15] + fp[16] + fp[17] + fp[18];
+        T t1 = fp[3] - fp[4], t2 = fp[5] - fp[6];
+        if (west) {
+          fp[2] = F(n, j, k, 2);
+          fp[8] = F(n, j - 1, k, 8);
+          fp[10] = F(n, j + 1, k, 10);
+          fp[12] = F(n, j, k - 1, 12);
+          fp[14] = F(n, j, k + 1, 14);
+          T sum_out = fp[2] + fp[8] + fp[10] + fp[12] + fp[14];
+          T u_n = T(prof[j + ny * k]);
+          T rho_w = (sum_par + T(2) * sum_out) / (T(1) - u_n);
+          fp[1] = fp[2] + T(1.0 / 3.0) * rho_w * u_n;
+          fp[7] = fp[10] - T(0.5) * t1 + T(1.0 / 6.0) * rho_w * u_n;
+          fp[9] = fp[8] + T(0.5) * t1 + T(1.0 / 6.0) * rho_w * u_n;
+          fp[11] = fp[14] - T(0.5) * t2 + T(1.0 / 6.0) * rho_w * u_n;
+          fp[13] = fp[12] + T(0.5) * t2 + T(1.0 / 6.0) * rho_w * u_n;
+        } else {
+          fp[1] = F(n, j, k, 1);
+          fp[7] = F(n, j - 1, k, 7);
+          fp[9] = F(n, j + 1, k, 9);
+          fp[11] = F(n, j, k - 1, 11);
+          fp[13] = F(n, j, k + 1, 13);
+          T sum_out = fp[1] + fp[7] + fp[9] + fp[11] + fp[13];
+          T rho = T(d.faces[KMI_EAST].rho);
+          T u_n = -(T(1) - (sum_par + T(2) * sum_out) / rho);
+          fp[2] = fp[1] - T(1.0 / 3.0) * rho * u_n;
+          fp[8] = fp[9] - T(0.5) * t1 - T(1.0 / 6.0) * rho * u_n;
+          fp[10] = fp[7] + T(0.5) * t1 - T(1.0 / 6.0) * rho * u_n;
+          fp[12] = fp[13] - T(0.5) * t2 - T(1.0 / 6.0) * rho * u_n;
+          fp[14] = fp[11] + T(0.5) * t2 - T(1.0 / 6.0) * rho * u_n;
+        }
+        trt_collide_local_3d(fp, o, sp, sm);
+        for (int q = 0; q < 19; ++q) fo[I3(b, j, k, nx, ny) + size_t(q) * P] = o[q];
+      }
+    }
+}
+
+// apply_bc_rebuild_2d!/_3d! after the fused step, at the descriptor's TRT rates.
+template <class T>
+static void apply_rebuilds(const kmi_desc& d, T* fo, const T* fi, const double* west_prof) {
+  const bool any = d.faces[KMI_WEST].kind == KMI_FACE_ZOUHE_VELOCITY_REBUILD ||
+                   d.faces[KMI_EAST].kind == KMI_FACE_ZOUHE_PRESSURE_REBUILD;
+  if (!any) return;
+  if (d.stencil == KMI_D2Q9) rebuild_2d(fo, fi, d, west_prof, T(d.s_plus), T(d.s_minus));
+  else rebuild_3d(fo, fi, d, west_prof, T(d.s_plus), T(d.s_minus));
+}
+
 template <class T>
 static void generic_step(const kmi_desc& d, const uint8_t* solid, T*& f, T*& f2, T*& g, T*& g2,
                          std::vector<T>& scratch, const Profiles* pr = nullptr,
@@ -1459,6 +1613,7 @@ static void generic_step(const kmi_desc& d, const uint8_t* solid, T*& f, T*& f2,
     else if (d.collision == KMI_BGK) collide_3d(f2, solid, nx, ny, nz, T(d.omega));
     else collide_trt_3d(f2, solid, nx, ny, nz, T(d.s_plus), T(d.s_minus));
   }
+  apply_rebuilds(d, f2, f, pr ? pr->p[KMI_WEST][0] : nullptr);
   std::swap(f, f2);
 }
 
@@ -1530,7 +1685,7 @@ static void libb_prephase(const kmi_desc& d, T* f2, const T* fi, const uint8_t* 
 // pre-phase on fluid cells, collision, SolidInert rest equilibrium on solid cells.
 template <class T>
 static void libb_step(const kmi_desc& d, const uint8_t* solid, T*& f, T*& f2, const T* q_wall,
-                      const T* const* uw) {
+                      const T* const* uw, const double* west_prof = nullptr) {
   const bool d3 = d.stencil == KMI_D3Q19;
   const int64_t nx = d.nx, ny = d.ny, nz = d3 ? d.nz : 1;
   const size_t P = size_t(nx * ny * nz);
@@ -1549,6 +1704,7 @@ static void libb_step(const kmi_desc& d, const uint8_t* solid, T*& f, T*& f2, co
   for (size_t c = 0; c < P; ++c)  // SolidInert — dsl/bricks.jl:133-154: feq(ρ=1, u=0) = w_q
     if (solid && solid[c])
       for (int q = 0; q < Q; ++q) f2[c + q * P] = T(w[q]) * T(1) * (T(1) + T(0) + T(0) - T(0));
+  apply_rebuilds(d, f2, f, west_prof);  // run_cylinder_libb_2d / run_sphere_libb_3d
   std::swap(f, f2);
 }
 
@@ -1749,7 +1905,8 @@ extern "C" {
   /* as ko_run, with the per-node arrays of profile faces: prof[2*face + k] */           \
   void ko_run_profiles_##S(const kmi_desc* d, const uint8_t* solid, T* f,                \
                            const double* const* prof, int64_t nsteps) {                  \
-    const size_t n = size_t(d->nx * d->ny) * 9;                                          \
+    const int Q = d->stencil == KMI_D2Q9 ? 9 : 19;                                       \
+    const size_t n = size_t(d->nx * d->ny * (d->stencil == KMI_D2Q9 ? 1 : d->nz)) * Q;   \
     std::vector<T> b1(n), scratch;                                                       \
     ko::Profiles pr;                                                                     \
     for (int k = 0; k < 8; ++k) pr.p[k / 2][k % 2] = prof[k];                            \
@@ -1826,6 +1983,11 @@ extern "C" {
     ko::macroscopic_forced_3d(r, u, v, w, f, nx, ny, nz, T(fx), T(fy), T(fz), fxa, fya,  \
                               fza);                                                      \
   }                                                                                      \
+  /* _trt_collide_local(_3d) of one node (specs.jl:7-108): f, o of 9 or 19 values */     \
+  void ko_trt_collide_local_##S(const T* f, T* o, int64_t Q, double sp, double sm) {    \
+    if (Q == 9) ko::trt_collide_local_2d(f, o, T(sp), T(sm));                            \
+    else ko::trt_collide_local_3d(f, o, T(sp), T(sm));                                   \
+  }                                                                                      \
   /* ko_run with a per-node force field (KMI_FORCE_GUO_FIELD); fz unused in 2D */        \
   void ko_run_field_##S(const kmi_desc* d, const uint8_t* solid, T* f, const T* fx,     \
                         const T* fy, const T* fz, int64_t nsteps) {                      \
@@ -1841,13 +2003,14 @@ extern "C" {
 
 #define KO_LIBB(S, T)                                                                     \
   void ko_run_libb_##S(const kmi_desc* d, const uint8_t* solid, T* f, const T* qw,       \
-                       const T* uwx, const T* uwy, const T* uwz, int64_t nsteps) {        \
+                       const T* uwx, const T* uwy, const T* uwz, int64_t nsteps,          \
+                       const double* west_prof) {                                        \
     const int Q = d->stencil == KMI_D2Q9 ? 9 : 19;                                       \
     const size_t n = size_t(d->nx * d->ny * (d->stencil == KMI_D2Q9 ? 1 : d->nz)) * Q;   \
     std::vector<T> b1(n);                                                                \
     const T* uw[3] = {uwx, uwy, uwz};                                                    \
     T *fa = f, *fb = b1.data();                                                          \
-    for (int64_t s = 0; s < nsteps; ++s) ko::libb_step(*d, solid, fa, fb, qw, uw);       \
+    for (int64_t s = 0; s < nsteps; ++s) ko::libb_step(*d, solid, fa, fb, qw, uw, west_prof); \
     if (fa != f) std::memcpy(f, fa, n * sizeof(T));                                      \
   }                                                                                      \
   void ko_drag_mei_2d_##S(const T* f, const T* qw, const T* uwx, const T* uwy,          \

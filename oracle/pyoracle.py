@@ -93,12 +93,13 @@ def run_profiles(desc, f, profiles, nsteps=1, mask=None):
     """As run(), for D2Q9 lattices with profile faces: `profiles` maps face name -> (a, b)
     arrays (u_x, u_y of a velocity profile, or rho and None of a pressure profile)."""
     names = {"west": 0, "east": 1, "south": 2, "north": 3}
+    profiles = {k: (v if isinstance(v, tuple) else (v, None)) for k, v in profiles.items()}
     keep, ptrs = [], (ctypes.c_void_p * 8)()
     for name, (a, b) in profiles.items():
         for k, arr in enumerate((a, b)):
             if arr is None:
                 continue
-            arr = np.ascontiguousarray(arr, dtype=np.float64)
+            arr = np.asfortranarray(arr, dtype=np.float64)  # (Ny, Nz) profiles: j + Ny k
             keep.append(arr)
             ptrs[2 * names[name] + k] = arr.ctypes.data
     fn = getattr(lib(), f"ko_run_profiles_{_sfx(f)}")
@@ -109,9 +110,12 @@ def run_profiles(desc, f, profiles, nsteps=1, mask=None):
     return f
 
 
-def run_libb(desc, f, solid, q_wall, uw=(None, None, None), nsteps=1):
+def run_libb(desc, f, solid, q_wall, uw=(None, None, None), nsteps=1, west_profile=None):
     """fused_trt_libb_v2_step! chain (LI-BB pre-phase, inert solids) on the descriptor's faces;
-    q_wall and the wall velocities are dense arrays of f's element type and shape."""
+    q_wall and the wall velocities are dense arrays of f's element type and shape.  Rebuilt
+    faces (run_cylinder_libb_2d / run_sphere_libb_3d) take the west inlet profile
+    `west_profile` (Ny, or (Ny, Nz))."""
+    wp = None if west_profile is None else np.asfortranarray(west_profile, dtype=np.float64)
     z = np.zeros_like(q_wall, order="F")
     arrs = [np.asfortranarray(a if a is not None else z, dtype=f.dtype) for a in uw]
     qw = np.asfortranarray(q_wall, dtype=f.dtype)
@@ -119,7 +123,7 @@ def run_libb(desc, f, solid, q_wall, uw=(None, None, None), nsteps=1):
     fn = getattr(lib(), f"ko_run_libb_{_sfx(f)}")
     fn.restype = None
     fn(ctypes.byref(desc), None if m is None else m.ctypes.data_as(c_void_p), _p(f), _p(qw),
-       _p(arrs[0]), _p(arrs[1]), _p(arrs[2]), c_int64(int(nsteps)))
+       _p(arrs[0]), _p(arrs[1]), _p(arrs[2]), c_int64(int(nsteps)), _p(wp))
     return f
 
 

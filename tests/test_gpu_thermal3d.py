@@ -23,7 +23,7 @@ W19 = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
 def _thermal_case(kmi, name, storage):
     """(desc, shape, mask, steps) of a D3Q19 thermal case."""
     if name == "natconv_box":
-        d, _ = kmi.natural_convection_3d_desc(10, 1e5, 0.71, storage=storage)
+        d, _ = kmi.natural_convection_3d_desc(10, 2e3, 0.71, storage=storage)  # β_g = 7e-3
         return d, (10, 10, 10), None, 30
     if name == "all_dirichlet_edges_mask":
         T = dict(west=0.9, east=0.1, south=0.7, north=0.3, bottom=0.6, top=0.2)

@@ -143,3 +143,126 @@ def test_mei_drag_is_halfway_at_half_cut(kmi, po):
     mea = po.drag_mei_2d(f, half)
     hw = po.drag_halfway(d, f, half)
     np.testing.assert_allclose(mea, hw[:2], rtol=1e-14, atol=1e-16)
+
+
+# ---- rebuilt inlet / outlet faces (apply_bc_rebuild_2d!/_3d!, src/bc/rebuild_*.jl) ----------
+# Pins: the regularised collision's output depends on ρ, u and the stress Π only (adding the
+# D2Q9 ghost modes ε and q_x changes nothing) and keeps equilibria fixed; both local
+# collisions conserve momentum (the 3D one mass too, and agrees with the bulk TRT); the
+# rebuilt node carries the Zou–He values — the normal momentum ρ_w u_n (and in 3D the mass
+# ρ_w), ρ_w from the textbook Zou–He relation on the populations streamed in from inside, and
+# zero tangential momentum in 2D (in 3D, the in-face diagonals' share, which the reference's
+# closure leaves uncorrected).
+
+def _local(po, f, sp, sm):
+    o = np.empty_like(f)
+    po.call("trt_collide_local", f, o, len(f), sp, sm)
+    return o
+
+
+def test_regularised_collision_sees_only_the_stress(po):
+    r = np.random.default_rng(41)
+    W = po.W9
+    f = np.ascontiguousarray(W * (1 + 0.05 * r.uniform(-1, 1, 9)))
+    sp, sm = 1.7, 0.9
+    o = _local(po, f, sp, sm)
+    eps = np.array([4, -2, -2, -2, -2, 1, 1, 1, 1], dtype=float)
+    qx = np.array([0, -2, 0, 2, 0, 1, -1, -1, 1], dtype=float)
+    qy = np.array([0, 0, -2, 0, 2, 1, 1, -1, -1], dtype=float)
+    for g in (eps, qx, qy):
+        assert abs(g.sum()) + abs((g * po.CX9).sum()) + abs((g * po.CY9).sum()) == 0
+        assert (g * po.CX9 * po.CX9).sum() == 0 and (g * po.CY9 * po.CY9).sum() == 0
+        assert (g * po.CX9 * po.CY9).sum() == 0
+        o2 = _local(po, np.ascontiguousarray(f + 1e-3 * g), sp, sm)
+        assert np.max(np.abs(o2 - o)) < 1e-15
+    for a in (po.CX9, po.CY9):
+        assert abs((o * a).sum() - (f * a).sum()) < 1e-15
+    feq = po.equilibrium(np.array([1.02]), np.array([0.03]), np.array([-0.01]))[0]
+    assert np.max(np.abs(_local(po, np.ascontiguousarray(feq), sp, sm) - feq)) < 1e-15
+
+
+def test_local_trt_3d_is_the_bulk_trt(po, kmi):
+    r = np.random.default_rng(42)
+    f = np.asfortranarray(po.W19 * (1 + 0.05 * r.uniform(-1, 1, 19)))
+    sp, sm = 1.6, 0.4
+    o = _local(po, np.ascontiguousarray(f), sp, sm)
+    g = f.reshape(1, 1, 1, 19).copy(order="F")
+    po.call("collide_trt_3d", g, None, 1, 1, 1, sp, sm)
+    assert np.max(np.abs(o - g.ravel())) < 1e-15
+    assert abs(o.sum() - f.sum()) < 1e-15
+    for a in (po.CX19, po.CY19, po.CZ19):
+        assert abs((o * a).sum() - (f * a).sum()) < 1e-15
+
+
+def _rebuild_desc(kmi, stencil, shape, rho_out=1.002):
+    faces = {"west": {"kind": "zouhe_velocity_rebuild"},
+             "east": {"kind": "zouhe_pressure_rebuild", "rho": rho_out}}
+    nz = shape[2] if len(shape) == 3 else 1
+    return kmi.make_desc(stencil, shape[0], shape[1], nz, 0.05, collision="trt", faces=faces)
+
+
+@pytest.mark.parametrize("stencil,shape", [("d2q9", (6, 7)), ("d3q19", (5, 6, 7))])
+def test_rebuilt_nodes_carry_the_zou_he_values(po, kmi, stencil, shape):
+    d3 = stencil == "d3q19"
+    Q = 19 if d3 else 9
+    C = np.array([po.CX19, po.CY19, po.CZ19] if d3 else [po.CX9, po.CY9])
+    W = po.W19 if d3 else po.W9
+    r = np.random.default_rng(43)
+    f0 = np.asfortranarray(W * (1 + 0.03 * r.uniform(-1, 1, shape + (Q,))))
+    tshape = shape[1:]
+    prof = np.asfortranarray(0.03 + 0.01 * r.uniform(-1, 1, tshape))
+    d = _rebuild_desc(kmi, stencil, shape)
+    f = f0.copy(order="F")
+    po.run_profiles(d, f, {"west": (prof, None)}, nsteps=1)
+    inner = [range(1, n - 1) for n in shape[1:]]
+    for side in ("west", "east"):
+        i = 0 if side == "west" else shape[0] - 1
+        for t in np.ndindex(*[len(x) for x in inner]):
+            node = (i,) + tuple(rg[k] for rg, k in zip(inner, t))
+            o = f[node]
+            # populations streamed in from inside the domain: f0(x − c_q) for inside sources
+            par, out = 0.0, 0.0
+            diag = np.zeros(len(shape))  # in-face diagonals (c_x = 0, two tangents nonzero)
+            for q in range(Q):
+                src = tuple(node[a] - C[a][q] for a in range(len(shape)))
+                if 0 <= src[0] < shape[0]:
+                    v = f0[src + (q,)]
+                    if C[0][q] == 0:
+                        par += v
+                        if d3 and C[1][q] != 0 and C[2][q] != 0:
+                            diag += v * C[:, q]
+                    elif (C[0][q] < 0) == (side == "west"):
+                        out += v
+            # Tangential momentum: zero in D2Q9; in D3Q19 the reference's closure corrects the
+            # axis populations only (rebuild_3d.jl:22-29), leaving the in-face diagonals' part.
+            for a in range(1, len(shape)):
+                assert abs((o * C[a]).sum() - diag[a]) < 1e-15, (side, node, a)
+            if side == "west":
+                un = prof[node[1:]]
+                rho_w = (par + 2 * out) / (1 - un)
+                assert abs((o * C[0]).sum() - rho_w * un) < 1e-15, node
+                if d3:
+                    assert abs(o.sum() - rho_w) < 1e-14
+            else:
+                rho = d.faces[1].rho
+                u = -1 + (par + 2 * out) / rho
+                assert abs((o * C[0]).sum() - rho * u) < 1e-15, node
+                if d3:
+                    assert abs(o.sum() - rho) < 1e-14
+
+
+def test_rebuilt_channel_keeps_the_inlet_profile(po, kmi):
+    """A channel with the rebuilt parabolic inlet and pressure outlet settles on a profile
+    close to the inlet's (halfway walls, so within a few per cent off the wall rows)."""
+    Nx, Ny, u = 40, 17, 0.04
+    j = np.arange(1, Ny + 1)
+    prof = 4 * u * (j - 1) * (Ny - j) / (Ny - 1) ** 2
+    faces = {"west": {"kind": "zouhe_velocity_rebuild"}, "east": {"kind": "zouhe_pressure_rebuild", "rho": 1.0}}
+    d = kmi.make_desc("d2q9", Nx, Ny, 1, 0.1, collision="trt", faces=faces)
+    from kraken_mi355x.libb import _equilibrium_x
+    f = _equilibrium_x(9, np.broadcast_to(prof[None, :], (Nx, Ny)))
+    po.run_profiles(d, f, {"west": (prof, None)}, nsteps=4000)
+    m = po.moments(f)
+    mid = m["ux"][Nx // 2, 2:Ny - 2]
+    assert np.max(np.abs(mid - prof[2:Ny - 2])) / u < 0.06
+    assert np.all(np.isfinite(f))
