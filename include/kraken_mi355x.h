@@ -52,7 +52,9 @@ typedef enum {
 
 typedef enum { KMI_D2Q9 = 0, KMI_D3Q19 = 1 } kmi_stencil;
 
-/* Population storage in HBM.  Arithmetic is FP64 for F64 and FP32 otherwise.
+/* Population storage in HBM.  Arithmetic is FP64 for F64 and FP32 otherwise; F64 and F32
+ * follow the reference's unfused evaluation order exactly, F32S and F16S collide with
+ * fused multiply-adds.
  * F32S / F16S store the deviation f_q - w_q (thermal: g_q - w_q T_ref); F16S scales
  * it by 2^15 before rounding to IEEE binary16.  The thermal populations of an F16S
  * lattice are stored as F32S: binary16 cannot resolve per-step temperature increments

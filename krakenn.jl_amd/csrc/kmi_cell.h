@@ -7,6 +7,16 @@
 
 namespace kmi {
 
+// Fused multiply-add a·b + c, rounded once.  The shifted storages (F32S, F16S), whose
+// results are checked against FP64 with tolerances, use it explicitly in their collision
+// arithmetic; the code is compiled without automatic contraction, so every kernel evaluates
+// the same operations and the plain storages keep the reference's unfused order.
+template <class R>
+KMI_DEV R fmad(R a, R b, R c) {
+  if constexpr (sizeof(R) == 4) return __builtin_fmaf(a, b, c);
+  else return __builtin_fma(a, b, c);
+}
+
 template <int Q, class R>
 KMI_DEV R usq_of(R ux, R uy, R uz) {
   if constexpr (Q == 9) return ux * ux + uy * uy;
@@ -21,10 +31,12 @@ KMI_DEV R usq_of(R ux, R uy, R uz) {
 // pair is evaluated once; every value equals the reference's per-q expression.
 template <int Q, bool SH, class R>
 KMI_DEV void equilibria(R drho_or_rho, R rho, R ux, R uy, R uz, R (&fe)[Q]) {
-  const R usq = usq_of<Q>(ux, uy, uz);
+  R usq;
+  if constexpr (!SH) usq = usq_of<Q>(ux, uy, uz);
+  else usq = Q == 9 ? fmad(uy, uy, ux * ux) : fmad(uz, uz, fmad(uy, uy, ux * ux));
   const R u15 = R(1.5) * usq;
   if constexpr (!SH) fe[0] = R(Lat<Q>::w[0]) * rho * (R(1) - u15);
-  else fe[0] = R(Lat<Q>::w[0]) * (drho_or_rho + rho * (-u15));
+  else fe[0] = R(Lat<Q>::w[0]) * fmad(rho, -u15, drho_or_rho);
   sfor<Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
     constexpr int o = Lat<Q>::opp[q];
@@ -32,14 +44,15 @@ KMI_DEV void equilibria(R drho_or_rho, R rho, R ux, R uy, R uz, R (&fe)[Q]) {
       constexpr double w = Lat<Q>::w[q];
       const R cu = cdotu<Lat<Q>::cx[q], Lat<Q>::cy[q], Lat<Q>::cz[q]>(ux, uy, uz);
       const R t3 = R(3) * cu;
-      const R sq = R(4.5) * cu * cu;
       if constexpr (!SH) {
+        const R sq = R(4.5) * cu * cu;
         const R wr = R(w) * rho;
         fe[q] = wr * (((R(1) + t3) + sq) - u15);
         fe[o] = wr * (((R(1) - t3) + sq) - u15);
       } else {
-        fe[q] = R(w) * (drho_or_rho + rho * ((t3 + sq) - u15));
-        fe[o] = R(w) * (drho_or_rho + rho * (((-t3) + sq) - u15));
+        const R h = R(4.5) * cu;  // X_q = 3cu + 4.5cu² − 1.5u², deq = w (δρ + ρ X_q)
+        fe[q] = R(w) * fmad(rho, fmad(h, cu, t3) - u15, drho_or_rho);
+        fe[o] = R(w) * fmad(rho, fmad(h, cu, -t3) - u15, drho_or_rho);
       }
     }
   });
@@ -50,7 +63,9 @@ template <int Q, bool SH, class R>
 KMI_DEV void moments(const R (&f)[Q], R& m0, R& rho, R& ux, R& uy, R& uz) {
   m0 = zeroth_moment<Q>(f);
   rho = SH ? R(1) + m0 : m0;
-  const R inv = R(1) / rho;
+  R inv;
+  if constexpr (SH && sizeof(R) == 4) inv = __builtin_amdgcn_rcpf(rho);  // 1 ulp, shifted only
+  else inv = R(1) / rho;
   ux = first_moment<Q, 0>(f) * inv;
   uy = first_moment<Q, 1>(f) * inv;
   if constexpr (Q == 19) uz = first_moment<Q, 2>(f) * inv;
@@ -64,7 +79,8 @@ KMI_DEV void collide_bgk(const R (&fp)[Q], R (&fo)[Q], R om) {
   equilibria<Q, SH>(m0, rho, ux, uy, uz, fe);
   sfor<Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    fo[q] = fp[q] - om * (fp[q] - fe[q]);
+    if constexpr (!SH) fo[q] = fp[q] - om * (fp[q] - fe[q]);
+    else fo[q] = fmad(-om, fp[q] - fe[q], fp[q]);
   });
 }
 
@@ -76,12 +92,21 @@ KMI_DEV void collide_trt(const R (&fp)[Q], R (&fo)[Q], R sp, R sm) {
   equilibria<Q, SH>(m0, rho, ux, uy, uz, fe);
   const R a = (sp + sm) * R(0.5);
   const R b = (sp - sm) * R(0.5);
-  fo[0] = fp[0] - sp * (fp[0] - fe[0]);
-  sfor<Q - 1>([&](auto qc) {
-    constexpr int q = decltype(qc)::value + 1;
-    constexpr int o = Lat<Q>::opp[q];
-    fo[q] = fp[q] - a * (fp[q] - fe[q]) - b * (fp[o] - fe[o]);
-  });
+  if constexpr (!SH) {
+    fo[0] = fp[0] - sp * (fp[0] - fe[0]);
+    sfor<Q - 1>([&](auto qc) {
+      constexpr int q = decltype(qc)::value + 1;
+      constexpr int o = Lat<Q>::opp[q];
+      fo[q] = fp[q] - a * (fp[q] - fe[q]) - b * (fp[o] - fe[o]);
+    });
+  } else {
+    fo[0] = fmad(-sp, fp[0] - fe[0], fp[0]);
+    sfor<Q - 1>([&](auto qc) {
+      constexpr int q = decltype(qc)::value + 1;
+      constexpr int o = Lat<Q>::opp[q];
+      fo[q] = fmad(-b, fp[o] - fe[o], fmad(-a, fp[q] - fe[q], fp[q]));
+    });
+  }
 }
 
 // D2Q9 MRT (Lallemand–Luo), collide_mrt_2d.jl:56-107.  Shifted form: the moments of

@@ -113,3 +113,42 @@ def time_to_residual(Re, A, *, N=128, net=None, tol=1e-7, check=100, window=100,
         fields = [b.moments(c) for c in range(len(Re))]
     return {"wall_s": wall, "init_s": t_init, "device_ms": ms, "steps": st["steps"], "status": st["status"],
             "residual": st["residual"], "fields": fields}
+
+
+def plain_time_to_residual(Re, A, *, N=128, tol=1e-7, check=100, window=100, max_steps=1_000_000,
+                           storage="f32s", backend=None):
+    """Rest-initialised plain runs: one lattice at a time (the single-cavity path, one launch
+    per step), each run to the batch's stop rule r = ‖u^k − u^(k−m)‖/(m‖u^k‖) < tol checked
+    every `check` steps (m = `window`).  Returns per-case wall seconds, steps and status
+    (1 converged, 2 diverged, 3 max_steps)."""
+    from .drivers import cavity_desc
+
+    backend = backend or MI355XBackend()
+    walls, steps, status = [], [], []
+    for Re_c, A_c in zip(Re, A):
+        nu = float(A_c) * N / float(Re_c)
+        d = cavity_desc(N, nu, float(A_c), storage=storage, collision="mrt")
+        from .lattice import Lattice
+
+        t0 = time.perf_counter()
+        n, st = 0, 3
+        with Lattice(d, backend) as lat:
+            while n < max_steps:
+                lat.step(check - window)
+                m0 = lat.moments()
+                lat.step(window)
+                m1 = lat.moments()
+                n += check
+                du = math.sqrt(float(np.sum((m1["ux"] - m0["ux"]) ** 2 + (m1["uy"] - m0["uy"]) ** 2)))
+                u = math.sqrt(float(np.sum(m1["ux"] ** 2 + m1["uy"] ** 2)))
+                r = du / u / window if u > 0 else math.inf
+                if not math.isfinite(r):
+                    st = 2
+                    break
+                if r < tol:
+                    st = 1
+                    break
+        walls.append(time.perf_counter() - t0)
+        steps.append(n)
+        status.append(st)
+    return {"wall_s": np.array(walls), "steps": np.array(steps), "status": np.array(status)}
