@@ -381,6 +381,15 @@ int kmi_batch_reset_status(kmi_batch* b);
 int kmi_batch_resume(kmi_batch* b);
 /* Advance all running cavities `nsteps` steps (no residual checks). */
 int kmi_batch_step(kmi_batch* b, int64_t nsteps);
+/* Consistent initialisation of a prescribed velocity field (Mei et al., Comput. Fluids 35
+ * (2006) 855): the current velocity of every held cavity (hold[c] != 0; hold = NULL holds
+ * all) is kept while the batch runs `iterations` steps, after each of which a held cell's
+ * populations get f_q += feq_q(rho, u_held) - feq_q(rho, u).  Density and the
+ * non-equilibrium part relax to values consistent with the held field (an feq-only start
+ * lacks them at the walls and excites slowly decaying modes).  The steps count for every
+ * running cavity, held or not.  No reference counterpart: the reference initialises from
+ * feq only (initialize_2d, basic.jl:51-70; gnf_periodic_steady.jl:114-150). */
+int kmi_batch_consistent_init(kmi_batch* b, const int32_t* hold, int32_t iterations);
 /* Step until every cavity is converged (r < tol), diverged or at max_steps, checking every
  * `check_every` (>= 2) steps; elapsed device milliseconds in *ms (may be NULL).
  * r = ||u^k - u^(k-m)||_2 / (m ||u^k||_2) over the last `window` = m steps (1 <= m <=

@@ -154,6 +154,46 @@ __global__ void __launch_bounds__(256) k_batch_resid(const BatchArgs a, const vo
   }
 }
 
+// Consistent initialisation of a prescribed velocity field (Mei, Luo, Lallemand and
+// d'Humières, "Consistent initial conditions for lattice Boltzmann simulations", Comput.
+// Fluids 35 (2006) 855-862): between steps, every cell of a held cavity gets the equilibrium
+// part of its populations moved back to the held velocity u_h (in uprev), keeping the density
+// and the non-equilibrium part that the step produced:
+//   f_q ← f_q + feq_q(ρ, u_h) − feq_q(ρ, u),   (ρ, u) the moments of f.
+// Iterated, ρ and f − feq relax to the values consistent with u_h, walls and lid included;
+// an feq-only start lacks them and seeds slowly decaying modes.  In place, FP64 arithmetic.
+template <int ST>
+__global__ void __launch_bounds__(256) k_batch_hold(const BatchArgs a, void* f, const int* __restrict__ hold) {
+  using S = Store<ST>;
+  using E = typename S::elem;
+  using R = typename S::real;
+  const int lx = threadIdx.x % a.tw, ly = threadIdx.x / a.tw;
+  for (int t = blockIdx.x; t < a.nslots; t += gridDim.x) {
+    const int c = a.slots[3 * t];
+    if (c < 0 || !hold[c]) continue;
+    const CavRec cv = a.cav[c];
+    const int i = a.slots[3 * t + 1] + lx, j = a.slots[3 * t + 2] + ly;
+    if (i >= cv.n || j >= cv.n) continue;
+    const long long P = (long long)cv.n * cv.n;
+    E* p = static_cast<E*>(f) + cv.foff;
+    const int self = i + cv.n * j;
+    double fp[9];
+    sfor<9>([&](auto qc) { fp[decltype(qc)::value] = double(S::ld(p, self + decltype(qc)::value * P)); });
+    const double r = S::shifted ? 1.0 + zeroth_moment<9>(fp) : zeroth_moment<9>(fp);
+    const double ux = first_moment<9, 0>(fp) / r, uy = first_moment<9, 1>(fp) / r;
+    const long long k = cv.coff + self;
+    const double hx = a.uprev[k], hy = a.uprev[a.total_cells + k];
+    const double du2 = (hx * hx + hy * hy) - (ux * ux + uy * uy);
+    sfor<9>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      constexpr double cx = Lat<9>::cx[q], cy = Lat<9>::cy[q];
+      const double cu = cx * ux + cy * uy, ch = cx * hx + cy * hy;
+      const double d = Lat<9>::w[q] * r * (3.0 * (ch - cu) + 4.5 * (ch * ch - cu * cu) - 1.5 * du2);
+      S::template st<false>(p, self + q * P, R(fp[q] + d));
+    });
+  }
+}
+
 // One thread per cavity: Neumaier sum of its slots' partials (fixed order) → r.
 __global__ void k_batch_final(const double* __restrict__ partials, const int* __restrict__ cav_slots,
                               const int* __restrict__ cav_range, const int* __restrict__ active,
@@ -207,6 +247,12 @@ static hipError_t launch_batch_step(int st, int coll, bool nts, const BatchArgs&
 static hipError_t launch_batch_resid(int st, const BatchArgs& a, const void* f, int mode,
                                      int groups, hipStream_t s) {
   KMI_BATCH_ST(st, hipLaunchKernelGGL((k_batch_resid<ST>), dim3(unsigned(groups)), dim3(256), 0, s, a, f, mode));
+  return hipGetLastError();
+}
+
+static hipError_t launch_batch_hold(int st, const BatchArgs& a, void* f, const int* hold, int groups,
+                                    hipStream_t s) {
+  KMI_BATCH_ST(st, hipLaunchKernelGGL((k_batch_hold<ST>), dim3(unsigned(groups)), dim3(256), 0, s, a, f, hold));
   return hipGetLastError();
 }
 
@@ -603,6 +649,40 @@ int kmi_batch_step(kmi_batch* B, int64_t nsteps) {
   for (int c = 0; c < B->ncav; ++c)
     if (B->status[c] == KMI_CAV_RUNNING) B->steps[c] += nsteps;
   KMI_HIP(x, hipStreamSynchronize(x->stream));
+  return KMI_OK;
+}
+
+int kmi_batch_consistent_init(kmi_batch* B, const int32_t* hold, int32_t iterations) {
+  if (!B) return fail(nullptr, KMI_ERR_ARG, "batch is NULL");
+  kmi_ctx* x = B->ctx;
+  if (iterations < 0) return fail(x, KMI_ERR_ARG, "kmi_batch_consistent_init: iterations must be >= 0");
+  KMI_HIP(x, hipSetDevice(x->device));
+  if (int rc = batch_ensure_tiles(B)) return rc;
+  if (iterations == 0 || batch_active(B) == 0) return KMI_OK;
+  std::vector<int> h(B->ncav, 1);
+  if (hold)
+    for (int c = 0; c < B->ncav; ++c) h[c] = hold[c] != 0;
+  int* hold_d = nullptr;
+  KMI_HIP(x, hipMalloc(&hold_d, sizeof(int) * B->ncav));
+  hipStream_t s = x->stream;
+  hipError_t e = hipMemcpyAsync(hold_d, h.data(), sizeof(int) * B->ncav, hipMemcpyHostToDevice, s);
+  // the held velocities: the current moments, into uprev (scratch between residual checks)
+  BatchArgs a = batch_args(B);
+  if (e == hipSuccess) e = launch_batch_resid(B->desc.storage, a, B->f[B->cur], 0, batch_groups(B), s);
+  int rc = KMI_OK;
+  for (int32_t k = 0; k < iterations && e == hipSuccess && rc == KMI_OK; ++k) {
+    rc = batch_launch_step(B, s);
+    if (rc == KMI_OK) {
+      a = batch_args(B);
+      e = launch_batch_hold(B->desc.storage, a, B->f[B->cur], hold_d, batch_groups(B), s);
+    }
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(hold_d);
+  if (rc != KMI_OK) return rc;
+  if (e != hipSuccess) return hip_fail(x, e, "kmi_batch_consistent_init");
+  for (int c = 0; c < B->ncav; ++c)
+    if (B->status[c] == KMI_CAV_RUNNING) B->steps[c] += iterations;
   return KMI_OK;
 }
 

@@ -139,6 +139,7 @@ SIGNATURES = {
     "kmi_batch_reset_status": (c_int, [c_void_p]),
     "kmi_batch_resume": (c_int, [c_void_p]),
     "kmi_batch_step": (c_int, [c_void_p, c_int64]),
+    "kmi_batch_consistent_init": (c_int, [c_void_p, c_void_p, c_int32]),
     "kmi_batch_run": (c_int, [c_void_p, c_double, c_int64, c_int64, c_int64, POINTER(c_double)]),
     "kmi_batch_status": (c_int, [c_void_p, POINTER(c_int32), POINTER(c_int64), POINTER(c_double)]),
     "kmi_batch_set_launch": (c_int, [c_void_p, c_int32, c_int32, c_int32, c_int32]),

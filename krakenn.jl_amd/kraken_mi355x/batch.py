@@ -93,6 +93,18 @@ class Batch:
     def step(self, n=1):
         self._chk(L.load().kmi_batch_step(self._h, int(n)))
 
+    def consistent_init(self, iterations, hold=None):
+        """Hold the current velocity of the cavities flagged in `hold` (all when None) for
+        `iterations` steps, re-imposing it after each step, so that density and the
+        non-equilibrium populations become consistent with it (Mei et al. 2006)."""
+        h = None
+        if hold is not None:
+            h = np.ascontiguousarray(hold, dtype=np.int32)
+            if h.shape != self.n.shape:
+                raise ArgumentError("hold must have one entry per cavity")
+        self._chk(L.load().kmi_batch_consistent_init(
+            self._h, None if h is None else h.ctypes.data_as(c_void_p), int(iterations)))
+
     def run(self, tol=1e-7, check_every=100, max_steps=10_000_000, window=1):
         """Run every cavity to r < tol (or divergence / max_steps); device ms elapsed.
         r = ‖u^k − u^(k−m)‖/(m‖u^k‖) over the last m = `window` steps (m = 1: the

@@ -5,8 +5,10 @@ Pipeline (all heavy work on the GPU through the C ABI):
      solver (kmi_batch, F32S MRT, Zou–He lid of run_cavity_2d);
   2. a conv decoder (kmi_nn, BF16 MFMA) trained to map (Re, A) to (u/A, v/A, (ρ−1)/A²);
   3. held-out cavities started from the predicted fields (feq of the predicted ρ, u — the
-     `f_init` injection point of src/drivers/gnf_periodic_steady.jl:114-150) against
-     rest-state starts, both run by the same batched solver to the same residual.
+     `f_init` injection point of src/drivers/gnf_periodic_steady.jl:114-150 — optionally
+     followed by consistent-initialisation steps that hold the predicted velocity while
+     density and non-equilibrium populations settle) against rest-state starts, both run by
+     the same batched solver to the same residual.
 Rest initialisation stays the fallback: a cavity without a prediction starts at rest.
 """
 from __future__ import annotations
@@ -91,10 +93,12 @@ def train(x, y, *, epochs=300, batch=32, lr=2e-3, lr_final=1e-4, seed=0, latent_
 
 
 def time_to_residual(Re, A, *, N=128, net=None, tol=1e-7, check=100, window=100, max_steps=1_000_000,
-                     storage="f32s", collision="mrt", backend=None):
+                     storage="f32s", collision="mrt", consistent=0, backend=None):
     """All cavities in one batch, from rest (net=None) or from the network's prediction.
-    Returns wall seconds (including inference and initialisation), per-cavity steps/status
-    and the final velocity fields."""
+    With `consistent` > 0 the predicted velocity is held for that many steps so that density
+    and non-equilibrium populations settle to it (Batch.consistent_init) before the run;
+    those steps are counted.  Returns wall seconds (including inference and initialisation),
+    per-cavity steps/status and the final velocity fields."""
     backend = backend or MI355XBackend()
     nu = A * N / Re
     with Batch(np.full(len(Re), N), nu, storage=storage, collision=collision, backend=backend) as b:
@@ -106,6 +110,8 @@ def time_to_residual(Re, A, *, N=128, net=None, tol=1e-7, check=100, window=100,
             for c in range(len(Re)):
                 rho, ux, uy = from_prediction(pred[c], A[c])
                 b.init_equilibrium(c, rho, ux, uy)
+            if consistent > 0:
+                b.consistent_init(consistent)
         t_init = time.perf_counter() - t0
         ms = b.run(tol, check, max_steps, window=window)
         wall = time.perf_counter() - t0
@@ -113,6 +119,28 @@ def time_to_residual(Re, A, *, N=128, net=None, tol=1e-7, check=100, window=100,
         fields = [b.moments(c) for c in range(len(Re))]
     return {"wall_s": wall, "init_s": t_init, "device_ms": ms, "steps": st["steps"], "status": st["status"],
             "residual": st["residual"], "fields": fields}
+
+
+def converged_agreement(Re, A, net, *, N=128, tol=1e-11, check=100, window=100, max_steps=3_000_000,
+                        storage="f64", consistent=0, backend=None):
+    """Whether the warm start changes the answer: the same cavities run from rest and from the
+    prediction, in FP64, to a tight residual.  A field stopped at r < tol still sits about
+    r·τ from the steady state (τ the slowest mode's decay time in steps), so two starts agree
+    only to that distance; at tol = 1e-11 it is far below the fields' 1e-6 agreement bound.
+    Returns per-case max |Δu|/A and relative L2 difference of u over the cases converged from
+    both starts, with steps and status of both runs."""
+    rest = time_to_residual(Re, A, N=N, tol=tol, check=check, window=window, max_steps=max_steps,
+                            storage=storage, backend=backend)
+    warm = time_to_residual(Re, A, N=N, net=net, tol=tol, check=check, window=window,
+                            max_steps=max_steps, storage=storage, consistent=consistent, backend=backend)
+    ok = np.where((rest["status"] == 1) & (warm["status"] == 1))[0]
+    du = np.array([max(float(np.max(np.abs(warm["fields"][c][k] - rest["fields"][c][k]))) for k in ("ux", "uy"))
+                   / float(A[c]) for c in ok])
+    l2 = np.array([math.sqrt(sum(float(np.sum((warm["fields"][c][k] - rest["fields"][c][k]) ** 2)) for k in ("ux", "uy"))
+                             / sum(float(np.sum(rest["fields"][c][k] ** 2)) for k in ("ux", "uy"))) for c in ok])
+    return {"index": ok, "dU_over_A": du, "rel_L2": l2, "rest_steps": rest["steps"], "warm_steps": warm["steps"],
+            "rest_status": rest["status"], "warm_status": warm["status"],
+            "wall_s": rest["wall_s"] + warm["wall_s"]}
 
 
 def plain_time_to_residual(Re, A, *, N=128, tol=1e-7, check=100, window=100, max_steps=1_000_000,

@@ -289,3 +289,18 @@ def rest(shape, dtype=np.float64):
     for q in range(len(W)):
         f[..., q] = dtype(W[q])
     return f
+
+
+def consistent_init(desc, f, iterations):
+    """Consistent initialisation of a prescribed velocity field (Mei, Luo, Lallemand and
+    d'Humières, Comput. Fluids 35 (2006) 855-862), FP64, on a D2Q9 state in place: the
+    velocity of f is held while the chain of `desc` runs `iterations` steps, each followed by
+    f_q += feq_q(ρ, u_held) − feq_q(ρ, u) with (ρ, u) the moments of f.  No reference
+    counterpart (the reference starts from feq, src/drivers/basic.jl:51-70)."""
+    m = moments(f)
+    uh, vh = m["ux"].copy(order="F"), m["uy"].copy(order="F")
+    for _ in range(iterations):
+        run(desc, f, nsteps=1)
+        m = moments(f)
+        f += equilibrium(m["rho"], uh, vh) - equilibrium(m["rho"], m["ux"], m["uy"])
+    return f

@@ -127,3 +127,51 @@ def test_batch_rejects_bad_input(kmi, backend):
             b.set_launch(48, 4)
         with pytest.raises(kmi.ArgumentError):
             b.upload(0, np.zeros((8, 8, 9)))
+
+
+@pytest.mark.parametrize("storage,tol", [("f64", 1e-13), ("f32s", 2e-6)])
+def test_batch_consistent_init_matches_oracle(kmi, po, backend, storage, tol):
+    """Held cavities follow the oracle's consistent initialisation (Mei et al. 2006), the
+    others plain steps; FP64 within round-off (the hold is evaluated in another order),
+    F32S within its storage tolerance."""
+    iters = 9
+    hold = [1, 0, 1, 1, 0, 1, 1]
+    f0 = []
+    for c, n in enumerate(SIZES):
+        r = np.random.default_rng(50 + c)
+        rho = np.asfortranarray(1 + 1e-3 * r.standard_normal((n, n)))
+        u, v = (np.asfortranarray(0.02 * r.standard_normal((n, n))) for _ in range(2))
+        f0.append(po.equilibrium(rho, u, v))
+    print(f"[batch consistent] {storage}", flush=True)
+    with kmi.Batch(SIZES, NUS, storage=storage, collision="mrt", backend=backend) as b:
+        for c in range(len(SIZES)):
+            b.upload(c, f0[c])
+        b.consistent_init(iters, hold=hold)
+        got = [b.download(c) for c in range(len(SIZES))]
+        st = b.status()
+    assert np.all(st["steps"] == iters)
+    for c, (n, nu) in enumerate(zip(SIZES, NUS)):
+        d = kmi.cavity_desc(n, nu, 0.1, storage="f64", collision="mrt")
+        ref = f0[c].copy(order="F")
+        if hold[c]:
+            po.consistent_init(d, ref, iters)
+        else:
+            po.run(d, ref, None, iters)
+        err = np.max(np.abs(got[c] - ref))
+        assert err < tol, f"cavity {c} (n={n}, hold={hold[c]}) max |Δ| {err:.3e}"
+    if storage == "f64":
+        # the held velocity is the uploaded one
+        for c in (0, 2):
+            m = po.moments(np.asfortranarray(got[c]))
+            m0 = po.moments(f0[c])
+            assert np.max(np.abs(m["ux"] - m0["ux"])) < 1e-15
+
+
+def test_batch_consistent_init_arguments(kmi, backend):
+    with kmi.Batch([8, 9], [0.1, 0.1], backend=backend) as b:
+        with pytest.raises(kmi.ArgumentError):
+            b.consistent_init(3, hold=[1, 0, 1])
+        with pytest.raises(kmi.KrakenError):
+            b.consistent_init(-1)
+        b.consistent_init(0)
+        assert np.all(b.status()["steps"] == 0)
