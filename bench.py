@@ -94,6 +94,88 @@ def bench_cavities(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_small(args):
+    """The two small single-lattice configurations, timed as whole runs from their initial
+    state: `cavity` = D2Q9 BGK lid-driven cavity 128², Re = 100 (u_lid = 0.1, ν = 0.128),
+    Zou–He lid, FP64, rest init, 20000 steps (run_cavity_2d); `rb` = D2Q9 thermal DDF
+    Rayleigh–Bénard 512×256, Ra = 1e5, Pr = 0.71, F32 shifted storage (g too), linear T plus
+    1e-3 noise, 20000 steps.  Both are launch-bound (2–30 MB of state per step): the steps
+    run as captured HIP graphs of single-step launches."""
+    import numpy as np
+
+    import kraken_mi355x as kmi
+
+    sys.path.insert(0, ROOT)
+    if args.workload == "cavity":
+        N, nu, ul = 128, 0.128, 0.1
+        st = "f64" if args.storage == "f32s" else args.storage  # the configuration is FP64
+        desc = kmi.cavity_desc(N, nu, ul, storage=st)
+        cpu_desc = kmi.cavity_desc(N, nu, ul, storage="f64" if st == "f64" else "f32")
+        cells, bpc, T0 = N * N, 9 * 2 * BYTES_PER_POP[st], None
+        name = f"D2Q9 BGK lid-driven cavity {N}^2, Re=100 (u_lid=0.1, nu=0.128), Zou-He lid, rest init"
+    else:
+        Nx, Ny = 512, 256
+        st = args.storage
+        desc, _ = kmi.rayleigh_benard_desc(Nx, Ny, 1e5, 0.71, storage=st)
+        cpu_desc, _ = kmi.rayleigh_benard_desc(Nx, Ny, 1e5, 0.71, storage="f64" if st == "f64" else "f32")
+        T0 = kmi.rb_initial_temperature(Nx, Ny, 1.0, 0.0, seed="noise")
+        # g of an F16S lattice is stored F32S
+        gb = 4 if st == "f16s" else BYTES_PER_POP[st]
+        cells, bpc = Nx * Ny, 9 * 2 * (BYTES_PER_POP[st] + gb)
+        name = f"D2Q9 thermal DDF Rayleigh-Benard {Nx}x{Ny}, Ra=1e5, Pr=0.71, linear T + 1e-3 noise"
+    steps = args.steps if args.steps != 1000 else 20000
+    with kmi.Lattice(desc) as lat:
+        if T0 is not None:
+            lat.init_temperature(T0)
+        lat.step(args.warmup)  # graph capture and warm-up, then back to the initial state
+        lat.backend.synchronize()
+    with kmi.Lattice(desc) as lat:
+        if T0 is not None:
+            lat.init_temperature(T0)
+        lat.backend.synchronize()
+        t0 = time.perf_counter()
+        ms = lat.step_timed(steps)
+        lat.backend.synchronize()
+        wall = time.perf_counter() - t0
+        mass = lat.reduce("mass")
+        extra = {"Nu": lat.reduce("nusselt", "south")} if T0 is not None else {}
+    per_step = ms / 1e3 / steps
+    achieved = cells * bpc / per_step / 1e9
+    out = {
+        "metric": f"MLUPS, {name}, {st}",
+        "value": round(cells * steps / wall / 1e6, 1), "unit": "MLUPS", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / steps, 5), "wall_s": round(wall, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if st == "f64" else "f32",
+        "data": "synthetic (rest / analytic initial state)",
+        "config": {"workload": name, "cells": cells, "storage": st, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None, "kernel": "k_step",
+                     "bytes_per_launch": cells * bpc, "avg_step_us": round(per_step * 1e6, 3),
+                     "note": "launch-bound: the state is 2-30 MB, one step is a few microseconds"},
+        "mass_after": mass, **extra,
+    }
+    if not args.no_cpu_baseline:
+        from oracle import pyoracle as po
+
+        dt = np.float64 if st == "f64" else np.float32
+        with kmi.Lattice(desc) as lat:
+            if T0 is not None:
+                lat.init_temperature(T0)
+            f = np.asfortranarray(lat.download().astype(dt))
+            g = np.asfortranarray(lat.download("g").astype(dt)) if T0 is not None else None
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 10.0 and n < steps:
+            po.run(cpu_desc, f, g, 200)
+            n += 200
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(cells * n / el / 1e6, 2), "unit": "MLUPS", "cores": po.threads(),
+                               "kind": "port",
+                               "sample": f"{n} steps ({el:.1f} s) of the C++ oracle chain, "
+                                         f"{'FP64' if dt is np.float64 else 'FP32'}, same configuration"}
+    print(json.dumps(out), flush=True)
+
+
 def bench_warmstart(args):
     """Learned warm start: 2048 steady 128² cavities (Re log-uniform [100, 3200], lid
     amplitude A uniform [0.06, 0.12]) from the batched solver train the conv decoder on the
@@ -197,7 +279,7 @@ def bench_warmstart(args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="tgv", choices=["tgv", "cavities", "warmstart"])
+    ap.add_argument("--workload", default="tgv", choices=["tgv", "cavities", "warmstart", "cavity", "rb"])
     ap.add_argument("--ws-train", type=int, default=2048)
     ap.add_argument("--ws-test", type=int, default=256)
     ap.add_argument("--ws-epochs", type=int, default=150)
@@ -226,6 +308,9 @@ def main():
         return
     if args.workload == "warmstart":
         bench_warmstart(args)
+        return
+    if args.workload in ("cavity", "rb"):
+        bench_small(args)
         return
 
     import kraken_mi355x as kmi
