@@ -3,8 +3,8 @@ storage (populations stored as FP32 deviations f − w_q, collision in FP32); th
 shifted and FP64 storages are timed alongside as variants.
 
 One "step" = one stream–collide time step over the whole 256³ lattice (populations resident
-in HBM).  The autotuner picks the launch: single-step launches, or two steps per launch with
-the intermediate populations in LDS.  Prints ONE JSON line with MLUPS, the HBM roofline of the
+in HBM).  The autotuner picks the launch: single-step launches, or two or three steps per
+launch with the intermediate populations in LDS.  Prints ONE JSON line with MLUPS, the HBM roofline of the
 dominant kernel (algorithmic 152 B per cell update in FP32 = 19 reads + 19 writes of 4 B,
 times the updates one launch performs) and a CPU baseline (the FP32 oracle restatement,
 OpenMP on the host cores, bounded sample).
@@ -22,6 +22,12 @@ HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E, published
 BYTES_PER_POP = {"f64": 8, "f32": 4, "f32s": 4, "f16s": 2}
 STORAGE_LABEL = {"f64": "FP64 storage", "f32": "FP32 storage",
                  "f32s": "FP32 shifted storage f-w", "f16s": "FP16 shifted storage"}
+
+
+def launch_span(launch):
+    """Steps one launch advances for a lattice launch record (temporal tile 0: 1, 1-5: 2, 6-8: 3)."""
+    t = launch.get("tile", 0)
+    return 3 if t >= 6 else (2 if t >= 1 else 1)
 
 
 def cpu_baseline(N, nu, storage, budget_s=12.0, max_steps=30):
@@ -342,10 +348,11 @@ def main():
     copy_gbps = kmi.MI355XBackend().copy_bandwidth(2 * 1024 ** 3, 20)
     # HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated on a
     # copy kernel of known size), recorded for this workload in profiles/traffic.json.
-    # Steps per launch of the dominant kernel: 2 for the two-step kernel (k_step2_d3q19),
-    # else 1 (k_step).  Traffic records are per kernel.
-    span = 2 if launch.get("tile", 0) else 1
-    kernel = "k_step2_d3q19" if span == 2 else "k_step"
+    # Steps per launch of the dominant kernel: 3 for the three-step kernel (k_step3_d3q19,
+    # tiles 6-8), 2 for the two-step kernel (k_step2_d3q19, tiles 1-5), else 1 (k_step).
+    # Traffic records are per kernel.
+    span = launch_span(launch)
+    kernel = {3: "k_step3_d3q19", 2: "k_step2_d3q19", 1: "k_step"}[span]
     traffic = None
     try:
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as fh:
@@ -398,7 +405,7 @@ def main():
             if st == args.storage:
                 continue
             m2, w2, _, l2 = run(st, args.steps, args.warmup)
-            sp2 = 2 if l2.get("tile", 0) else 1
+            sp2 = launch_span(l2)
             var[st] = {"MLUPS": round(cells * args.steps / w2 / 1e6, 1),
                        "frac": round(sp2 * cells * 38 * BYTES_PER_POP[st] / (m2 / 1e3 / (args.steps / sp2)) / 1e9
                                      / HBM_PEAK_GBPS, 4),

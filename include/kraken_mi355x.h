@@ -323,11 +323,14 @@ int kmi_lattice_device_bytes(const kmi_lattice* lat, int64_t* bytes);
  * population stores on/off.  Results do not depend on these settings. */
 int kmi_lattice_set_launch(kmi_lattice* lat, int32_t bx, int32_t by, int32_t nontemporal);
 int kmi_lattice_get_launch(const kmi_lattice* lat, int32_t* bx, int32_t* by, int32_t* nontemporal);
-/* Two steps per launch (temporal blocking) for fully periodic D3Q19 BGK/TRT lattices
- * without mask: the intermediate populations stay in LDS, halving HBM traffic per step.
- * tile: 0 off, 1 = 32x8, 2 = 64x4, 3 = 64x8, 4 = 32x16, 5 = 64x16 (not F64) cells per
- * work-group; nx and ny must be multiples of the tile.  zchunks: z-ranges per tile column (1..nz).  Results
- * are bit-identical to single-step launches; odd step counts end with one single step. */
+/* Two or three steps per launch (temporal blocking) for fully periodic D3Q19 BGK/TRT
+ * lattices without mask and with nz >= 4: the intermediate populations stay in LDS, so HBM
+ * sees one read and one write of the field per launch.  tile: 0 off; two steps: 1 = 32x8,
+ * 2 = 64x4, 3 = 64x8, 4 = 32x16, 5 = 64x16 (not F64); three steps: 6 = 64x8 (not F64),
+ * 7 = 32x8, 8 = 32x16 (not F64) cells per work-group; nx and ny must be multiples of the
+ * tile.  zchunks: z-ranges per tile column (1..nz).  Results are bit-identical to
+ * single-step launches; step counts that are not a multiple of the span end with single
+ * steps. */
 int kmi_lattice_set_temporal(kmi_lattice* lat, int32_t tile, int32_t zchunks);
 int kmi_lattice_get_temporal(const kmi_lattice* lat, int32_t* tile, int32_t* zchunks);
 /* Time every candidate shape × store policy (and two-step variant, where it applies) on

@@ -391,24 +391,28 @@ static void default_geometry(kmi_lattice* L) {
   set_geometry(L, bx, 1, field_bytes > 512e6 ? 1 : 0);
 }
 
-// Whether two-step launches with `tile` apply: fully periodic D3Q19 BGK/TRT, no mask,
-// and the x–y extent a multiple of the tile.
+// Whether multi-step launches with `tile` apply: fully periodic D3Q19 BGK/TRT, no mask,
+// the x–y extent a multiple of the tile, and at least 4 planes in z.
 static bool tb_eligible(const kmi_lattice* L, int tile) {
   int TX, TY;
   if (!step2_tile(tile, L->desc.storage, TX, TY)) return false;
   return !L->bnd && L->Q == 19 && !L->desc.thermal &&
          (L->desc.collision == KMI_BGK || L->desc.collision == KMI_TRT) &&
-         L->desc.nx % TX == 0 && L->desc.ny % TY == 0;
+         L->desc.nx % TX == 0 && L->desc.ny % TY == 0 && L->args.nz >= 4;
 }
 
 // Steps advanced by one launch.
 static int launch_span(const kmi_lattice* L) {
-  return L->tb_tile && tb_eligible(L, L->tb_tile) ? 2 : 1;
+  return L->tb_tile && tb_eligible(L, L->tb_tile) ? temporal_span(L->tb_tile) : 1;
 }
 
-// One launch: a single step, or two (two = true, temporal blocking enabled).  Either way
-// the result lands in the other buffer.
-static int launch_one(kmi_lattice* L, hipStream_t s, bool two = false) {
+// Steps per captured graph: a multiple of the span with an even number of launches, so the
+// buffer parity returns to where the capture started.
+static int graph_steps(const kmi_lattice* L) { return launch_span(L) == 3 ? 30 : kGraphSteps; }
+
+// One launch: a single step, or the temporal-blocking launch (multi = true: two or three
+// steps).  Either way the result lands in the other buffer.
+static int launch_one(kmi_lattice* L, hipStream_t s, bool multi = false) {
   StepArgs a = L->args;
   a.fi = L->f[L->cur];
   a.fo = L->f[L->cur ^ 1];
@@ -416,10 +420,10 @@ static int launch_one(kmi_lattice* L, hipStream_t s, bool two = false) {
   a.go = L->g[L->cur ^ 1];
   a.mask = L->mask;
   hipError_t e =
-      two ? launch_step2(L->desc.storage, L->desc.collision, L->tb_tile, L->tb_zchunks,
-                         L->nts != 0, a, s)
-          : launch_step(L->Q, L->desc.storage, L->desc.collision, L->desc.thermal != 0, L->bnd,
-                        L->nts != 0, a, L->grid, L->block, s);
+      multi ? launch_step2(L->desc.storage, L->desc.collision, L->tb_tile, L->tb_zchunks,
+                           L->nts != 0, a, s)
+            : launch_step(L->Q, L->desc.storage, L->desc.collision, L->desc.thermal != 0, L->bnd,
+                          L->nts != 0, a, L->grid, L->block, s);
   if (e != hipSuccess) return hip_fail(L->ctx, e, "step kernel launch");
   L->cur ^= 1;
   return KMI_OK;
@@ -433,18 +437,19 @@ static void drop_graphs(kmi_lattice* L) {
     }
 }
 
-// kGraphSteps launches captured once per buffer parity and replayed.
+// graph_steps(L) steps captured once per buffer parity and replayed.
 static int run_steps(kmi_lattice* L, int64_t n) {
   kmi_ctx* c = L->ctx;
   hipStream_t s = c->stream;
-  while (n >= kGraphSteps) {
+  const int span = launch_span(L);
+  const int gsteps = graph_steps(L);
+  while (n >= gsteps) {
     const int p = L->cur;
     if (!L->graph[p]) {
       hipGraph_t gph = nullptr;
       KMI_HIP(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      const int span = launch_span(L);
-      for (int k = 0; k < kGraphSteps; k += span) {
-        int rc = launch_one(L, s, span == 2);
+      for (int k = 0; k < gsteps; k += span) {
+        int rc = launch_one(L, s, span > 1);
         if (rc != KMI_OK) {
           (void)hipStreamEndCapture(s, &gph);
           if (gph) (void)hipGraphDestroy(gph);
@@ -458,13 +463,13 @@ static int run_steps(kmi_lattice* L, int64_t n) {
       // capture only recorded the launches; parity is back where it started
     }
     KMI_HIP(c, hipGraphLaunch(L->graph[p], s));
-    n -= kGraphSteps;
-    L->steps += kGraphSteps;
+    n -= gsteps;
+    L->steps += gsteps;
   }
-  for (; n >= 2 && launch_span(L) == 2; n -= 2) {
+  for (; span > 1 && n >= span; n -= span) {
     int rc = launch_one(L, s, true);
     if (rc != KMI_OK) return rc;
-    L->steps += 2;
+    L->steps += span;
   }
   for (; n > 0; --n) {
     int rc = launch_one(L, s);
@@ -1121,9 +1126,10 @@ int kmi_lattice_set_temporal(kmi_lattice* L, int32_t tile, int32_t zchunks) {
   if (!L) return fail(nullptr, KMI_ERR_ARG, "kmi_lattice_set_temporal: lattice is NULL");
   if (tile != 0 && !tb_eligible(L, tile))
     return fail(L->ctx, KMI_ERR_UNSUPPORTED,
-                "kmi_lattice_set_temporal: two-step launches need a fully periodic D3Q19 "
-                "BGK/TRT lattice without mask whose nx, ny are multiples of the tile "
-                "(1: 32x8, 2: 64x4, 3: 64x8, 4: 32x16, 5: 64x16)");
+                "kmi_lattice_set_temporal: multi-step launches need a fully periodic D3Q19 "
+                "BGK/TRT lattice without mask, nz >= 4, whose nx, ny are multiples of the tile "
+                "(two steps 1: 32x8, 2: 64x4, 3: 64x8, 4: 32x16, 5: 64x16 (not FP64); "
+                "three steps 6: 64x8 (not FP64), 7: 32x8, 8: 32x16 (not FP64))");
   if (tile != 0 && (zchunks < 1 || zchunks > L->args.nz))
     return fail(L->ctx, KMI_ERR_ARG, "kmi_lattice_set_temporal: zchunks must be in 1..nz");
   KMI_HIP(L->ctx, hipSetDevice(L->ctx->device));
@@ -1202,9 +1208,9 @@ int kmi_lattice_autotune(kmi_lattice* L, int64_t steps_per_trial, double* best_m
       }
     }
   }
-  // Two-step launches (the single-step shape found above serves odd remainders).
+  // Two- and three-step launches (the single-step shape found above serves remainders).
   const int zc_list[] = {1, 2, 3, 4, 6, 8, 12, 16};
-  for (int tile = 1; tile <= 5 && e == hipSuccess; ++tile) {
+  for (int tile = 1; tile <= 8 && e == hipSuccess; ++tile) {
     if (!tb_eligible(L, tile)) continue;
     for (int zc : zc_list) {
       if (zc > L->args.nz || e != hipSuccess) continue;

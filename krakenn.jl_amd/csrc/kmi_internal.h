@@ -21,9 +21,12 @@ int hip_fail(kmi_ctx* ctx, hipError_t e, const char* what);
 
 hipError_t launch_step(int q, int st, int coll, bool th, bool bnd, bool nts, const StepArgs& a,
                        dim3 g, dim3 b, hipStream_t s);
-// Two steps per launch on a fully periodic D3Q19 BGK/TRT lattice (kmi_step2.hip); `tile`
-// selects the work-group tile (step2_tile), `zchunks` the z-ranges per column.
+// Two or three steps per launch on a fully periodic D3Q19 BGK/TRT lattice (kmi_step2.hip);
+// `tile` selects the work-group tile and the steps per launch (step2_tile, temporal_span),
+// `zchunks` the z-ranges per column.
 bool step2_tile(int tile, int st, int& TX, int& TY);
+// Steps one launch of `tile` advances: 2 (tiles 1–5), 3 (tiles 6–8), 1 (tile 0).
+int temporal_span(int tile);
 hipError_t launch_step2(int st, int coll, int tile, int zchunks, bool nts, const StepArgs& a,
                         hipStream_t s);
 hipError_t launch_copy_probe(const void* src, void* dst, long long bytes, hipStream_t s);

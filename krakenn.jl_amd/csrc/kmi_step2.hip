@@ -1,4 +1,4 @@
-// kmi_step2.hip — two time steps per launch for fully periodic D3Q19 lattices.
+// kmi_step2.hip — two or three time steps per launch for fully periodic D3Q19 lattices.
 //
 // A single-step kernel reads and writes every population once per step: 2·19·sizeof(elem)
 // bytes of HBM traffic per cell update, and no kernel of that shape can beat the device's
@@ -201,10 +201,202 @@ __global__ void __launch_bounds__(TX * TY)
   }
 }
 
+// ---- three steps per launch ------------------------------------------------------------
+// The same march with two LDS levels: the t+1 populations on a two-cell x–y halo
+// ((TX+4)(TY+4) cells, level 1) and the t+2 populations on a one-cell halo ((TX+2)(TY+2),
+// level 2), each held as the 24 values per cell of the two-step kernel.  HBM sees one read
+// and one write per population per three steps; the extra evaluations are the two halos and
+// the two planes below and above the z-range at each level.  Per marched plane s:
+//   P1  t+1 on plane s, level-1 tile, pulled from the t field in HBM: c_z = −1 → L1.C at
+//       once, c_z = 0 and +1 kept in registers;                                    barrier
+//   P2  t+2 on plane s−1, level-2 tile, pulled from L1 (c_z = +1 of s−2, 0 of s−1, −1 of s):
+//       c_z = −1 → L2.C, c_z = 0 and +1 kept;                                      barrier
+//   P3  t+3 on plane s−2, interior, pulled from L2 (c_z = +1 of s−3, 0 of s−2, −1 of s−1),
+//       stored to HBM; the values kept in P1 replace the L1 slots P2 consumed;     barrier
+//   P4  the values kept in P2 replace the L2 slots P3 consumed.
+// P1 writes only L1.C, whose readers (P2) finished before the second barrier; P4's L2
+// writes are read after the next iteration's second barrier.  Every t+1 and t+2 value
+// passes through the storage rounding, so the result is bit-identical to three single-step
+// launches in every storage.
+template <int ST, int COLL, int TX, int TY, bool NTS>
+__global__ void __launch_bounds__(TX * TY)
+    k_step3_d3q19(const StepArgs a, int zlen, int ntx, int nty, int ntiles, int per_xcd) {
+  using S = Store<ST>;
+  using E = typename S::elem;
+  using R = typename S::real;
+  constexpr int E1X = TX + 4, E1Y = TY + 4, N1 = E1X * E1Y;  // level 1 (t+1)
+  constexpr int E2X = TX + 2, E2Y = TY + 2, N2 = E2X * E2Y;  // level 2 (t+2)
+  constexpr int NT = TX * TY;
+  constexpr int EPT1 = (N1 + NT - 1) / NT, EPT2 = (N2 + NT - 1) / NT;
+  constexpr unsigned ES = sizeof(E);
+  __shared__ R lds[24 * (N1 + N2)];
+  R* const A1 = lds;             // c_z = 0:  [9][N1]
+  R* const B1 = lds + 9 * N1;    // c_z = +1: [2 planes][5][N1]
+  R* const C1 = lds + 19 * N1;   // c_z = −1: [5][N1]
+  R* const L2 = lds + 24 * N1;
+  R* const A2 = L2;              // [9][N2]
+  R* const B2 = L2 + 9 * N2;     // [2][5][N2]
+  R* const C2 = L2 + 19 * N2;    // [5][N2]
+
+  const int t = int(blockIdx.x % 8u) * per_xcd + int(blockIdx.x / 8u);
+  if (t >= ntiles) return;  // padding groups (uniform per group, before any barrier)
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const long long P = a.plane;
+  const int tx = t % ntx, ty = (t / ntx) % nty, zc = t / (ntx * nty);
+  const int x0 = tx * TX, y0 = ty * TY;
+  const int z0 = zc * zlen, z1 = min(z0 + zlen, nz);
+  const E* __restrict__ fi = static_cast<const E*>(a.fi);
+  E* __restrict__ fo = static_cast<E*>(a.fo);
+  const int tid = threadIdx.x;
+
+  constexpr bool kPrefetch = sizeof(R) == 4 && NT <= 512;
+  R nxt[EPT1][19];
+  auto pull_plane = [&](int s) {
+    const int zs = tb::wrap(s, nz);
+    const int zm = tb::wrap(zs - 1, nz), zp = tb::wrap(zs + 1, nz);
+#pragma unroll
+    for (int r = 0; r < EPT1; ++r) {
+      const int e = tid + r * NT;
+      if (e < N1) {
+        const int ex = e % E1X, ey = e / E1X;
+        const int gx = tb::wrap(x0 - 2 + ex, nx), gy = tb::wrap(y0 - 2 + ey, ny);
+        const int xm = tb::wrap(gx - 1, nx), xp = tb::wrap(gx + 1, nx);
+        const int ym = tb::wrap(gy - 1, ny), yp = tb::wrap(gy + 1, ny);
+        const unsigned bx[3] = {unsigned(xm) * ES, unsigned(gx) * ES, unsigned(xp) * ES};
+        const int yy[3] = {ym, gy, yp}, zz[3] = {zm, zs, zp};
+        sfor<19>([&](auto qc) {
+          constexpr int q = decltype(qc)::value;
+          constexpr int cx = Lat<19>::cx[q], cy = Lat<19>::cy[q], cz = Lat<19>::cz[q];
+          const unsigned row = unsigned(nx) * unsigned(yy[1 - cy] + ny * zz[1 - cz]) * ES;
+          nxt[r][q] = S::ld(&tb::at(fi + q * P, row + bx[1 - cx]), 0);
+        });
+      }
+    }
+  };
+  if constexpr (kPrefetch) pull_plane(z0 - 2);
+
+  for (int s = z0 - 2; s <= z1 + 1; ++s) {
+    const int u = s + 4;  // non-negative plane counter for the ring slots
+    // ---- P1: t+1 on plane s, level-1 tile ----
+    if constexpr (!kPrefetch) pull_plane(s);
+    R cur[EPT1][19];
+#pragma unroll
+    for (int r = 0; r < EPT1; ++r)
+#pragma unroll
+      for (int q = 0; q < 19; ++q) cur[r][q] = nxt[r][q];
+    if constexpr (kPrefetch)
+      if (s <= z1) pull_plane(s + 1);
+    R k1z[EPT1][9], k1p[EPT1][5];
+#pragma unroll
+    for (int r = 0; r < EPT1; ++r) {
+      const int e = tid + r * NT;
+      if (e < N1) {
+        R fn[19];
+        collide19<ST, COLL>(cur[r], fn, a);
+        sfor<19>([&](auto qc) {
+          constexpr int q = decltype(qc)::value;
+          constexpr int cz = Lat<19>::cz[q];
+          constexpr int n = tb::group_index(q);
+          const R v = storage_round<ST>(fn[q]);
+          if constexpr (cz == 0) k1z[r][n] = v;
+          else if constexpr (cz > 0) k1p[r][n] = v;
+          else C1[n * N1 + e] = v;
+        });
+      }
+    }
+    __syncthreads();
+    // ---- P2: t+2 on plane s−1, level-2 tile ----
+    R k2z[EPT2][9], k2p[EPT2][5];
+    if (s >= z0) {
+      const R* const B1r = B1 + ((u - 2) & 1) * 5 * N1;
+#pragma unroll
+      for (int r = 0; r < EPT2; ++r) {
+        const int e = tid + r * NT;
+        if (e < N2) {
+          const int ex = e % E2X, ey = e / E2X;
+          R fp[19], fn[19];
+          sfor<19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            constexpr int cx = Lat<19>::cx[q], cy = Lat<19>::cy[q], cz = Lat<19>::cz[q];
+            constexpr int n = tb::group_index(q);
+            const int src = (ey + 1 - cy) * E1X + (ex + 1 - cx);
+            if constexpr (cz == 0) fp[q] = A1[n * N1 + src];
+            else if constexpr (cz > 0) fp[q] = B1r[n * N1 + src];
+            else fp[q] = C1[n * N1 + src];
+          });
+          collide19<ST, COLL>(fp, fn, a);
+          sfor<19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            constexpr int cz = Lat<19>::cz[q];
+            constexpr int n = tb::group_index(q);
+            const R v = storage_round<ST>(fn[q]);
+            if constexpr (cz == 0) k2z[r][n] = v;
+            else if constexpr (cz > 0) k2p[r][n] = v;
+            else C2[n * N2 + e] = v;
+          });
+        }
+      }
+    }
+    __syncthreads();
+    // ---- P3: t+3 on plane s−2, interior; then the P1 values into level 1 ----
+    if (s >= z0 + 2) {
+      const R* const B2r = B2 + ((u - 3) & 1) * 5 * N2;
+      const int ix = tid % TX, iy = tid / TX;
+      R fp[19], fn[19];
+      sfor<19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int cx = Lat<19>::cx[q], cy = Lat<19>::cy[q], cz = Lat<19>::cz[q];
+        constexpr int n = tb::group_index(q);
+        const int src = (iy + 1 - cy) * E2X + (ix + 1 - cx);
+        if constexpr (cz == 0) fp[q] = A2[n * N2 + src];
+        else if constexpr (cz > 0) fp[q] = B2r[n * N2 + src];
+        else fp[q] = C2[n * N2 + src];
+      });
+      collide19<ST, COLL>(fp, fn, a);
+      const unsigned cell = (unsigned(x0 + ix) + unsigned(nx) * unsigned(y0 + iy + ny * (s - 2))) * ES;
+      sfor<19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        S::template st<NTS>(tb::at(fo + q * P, cell), 0, fn[q]);
+      });
+    }
+    {
+      R* const B1w = B1 + (u & 1) * 5 * N1;
+#pragma unroll
+      for (int r = 0; r < EPT1; ++r) {
+        const int e = tid + r * NT;
+        if (e < N1) {
+#pragma unroll
+          for (int n = 0; n < 9; ++n) A1[n * N1 + e] = k1z[r][n];
+#pragma unroll
+          for (int n = 0; n < 5; ++n) B1w[n * N1 + e] = k1p[r][n];
+        }
+      }
+    }
+    __syncthreads();
+    // ---- P4: the P2 values (plane s−1) into level 2 ----
+    if (s >= z0) {
+      R* const B2w = B2 + ((u - 1) & 1) * 5 * N2;
+#pragma unroll
+      for (int r = 0; r < EPT2; ++r) {
+        const int e = tid + r * NT;
+        if (e < N2) {
+#pragma unroll
+          for (int n = 0; n < 9; ++n) A2[n * N2 + e] = k2z[r][n];
+#pragma unroll
+          for (int n = 0; n < 5; ++n) B2w[n * N2 + e] = k2p[r][n];
+        }
+      }
+    }
+  }
+}
+
 // ---- dispatch ---------------------------------------------------------------------------
-// Tile variants (TX × TY interior cells per work-group): 1 = 32×8, 2 = 64×4, 3 = 64×8,
-// 4 = 32×16, 5 = 64×16, not for FP64 (the larger the tile, the smaller the share of halo recomputation:
-// (TX+2)(TY+2)/(TX·TY) = 1.55, 1.55, 1.29, 1.32, 1.16).
+// Tile variants (TX × TY interior cells per work-group).  Two steps per launch: 1 = 32×8,
+// 2 = 64×4, 3 = 64×8, 4 = 32×16, 5 = 64×16, not for FP64 (the larger the tile, the smaller
+// the share of halo recomputation: (TX+2)(TY+2)/(TX·TY) = 1.55, 1.55, 1.29, 1.32, 1.16).
+// Three steps per launch: 6 = 64×8, 7 = 32×8, 8 = 32×16; FP64 only 7 (LDS: 24 values per
+// cell of both levels, 141 / 74 / 128 KB in FP32; evaluations per update
+// ((TX+4)(TY+4) + (TX+2)(TY+2) + TX·TY) / (3·TX·TY) = 1.29, 1.34, 1.20).
 bool step2_tile(int tile, int st, int& TX, int& TY) {
   switch (tile) {
     case 1: TX = 32; TY = 8; return true;
@@ -212,9 +404,14 @@ bool step2_tile(int tile, int st, int& TX, int& TY) {
     case 3: TX = 64; TY = 8; return true;
     case 4: TX = 32; TY = 16; return true;
     case 5: TX = 64; TY = 16; return st != KMI_STORE_F64;  // FP64: > 160 KB of LDS
+    case 6: TX = 64; TY = 8; return st != KMI_STORE_F64;
+    case 7: TX = 32; TY = 8; return true;
+    case 8: TX = 32; TY = 16; return st != KMI_STORE_F64;
     default: return false;
   }
 }
+
+int temporal_span(int tile) { return tile >= 6 ? 3 : (tile >= 1 ? 2 : 1); }
 
 template <int ST, int COLL, int TX, int TY>
 static hipError_t launch2(bool nts, const StepArgs& a, int zchunks, hipStream_t s) {
@@ -229,16 +426,39 @@ static hipError_t launch2(bool nts, const StepArgs& a, int zchunks, hipStream_t 
   return hipGetLastError();
 }
 
+template <int ST, int COLL, int TX, int TY>
+static hipError_t launch3(bool nts, const StepArgs& a, int zchunks, hipStream_t s) {
+  const int ntx = a.nx / TX, nty = a.ny / TY;
+  const int zlen = (a.nz + zchunks - 1) / zchunks;
+  const int nzc = (a.nz + zlen - 1) / zlen;
+  const int ntiles = ntx * nty * nzc;
+  const int per_xcd = (ntiles + 7) / 8;
+  const dim3 g(static_cast<unsigned>(8 * per_xcd)), b(static_cast<unsigned>(TX * TY));
+  if (nts) hipLaunchKernelGGL((k_step3_d3q19<ST, COLL, TX, TY, true>), g, b, 0, s, a, zlen, ntx, nty, ntiles, per_xcd);
+  else hipLaunchKernelGGL((k_step3_d3q19<ST, COLL, TX, TY, false>), g, b, 0, s, a, zlen, ntx, nty, ntiles, per_xcd);
+  return hipGetLastError();
+}
+
 template <int ST, int COLL>
 static hipError_t launch2_tile(int tile, bool nts, const StepArgs& a, int zchunks, hipStream_t s) {
   switch (tile) {
+    case 7: return launch3<ST, COLL, 32, 8>(nts, a, zchunks, s);
+    case 6:
+    case 8:
+      if constexpr (ST != KMI_STORE_F64) {
+        if (tile == 6) return launch3<ST, COLL, 64, 8>(nts, a, zchunks, s);
+        return launch3<ST, COLL, 32, 16>(nts, a, zchunks, s);
+      } else {
+        return hipErrorInvalidValue;
+      }
     case 1: return launch2<ST, COLL, 32, 8>(nts, a, zchunks, s);
     case 2: return launch2<ST, COLL, 64, 4>(nts, a, zchunks, s);
     case 3: return launch2<ST, COLL, 64, 8>(nts, a, zchunks, s);
     case 4: return launch2<ST, COLL, 32, 16>(nts, a, zchunks, s);
-    default:
+    case 5:
       if constexpr (ST != KMI_STORE_F64) return launch2<ST, COLL, 64, 16>(nts, a, zchunks, s);
       else return hipErrorInvalidValue;
+    default: return hipErrorInvalidValue;
   }
 }
 

@@ -281,8 +281,9 @@ class Lattice:
         return {"bx": bx.value, "by": by.value, "nontemporal": bool(nt.value)}
 
     def set_temporal(self, tile, zchunks=1):
-        """Two steps per launch (tile 1: 32×8, 2: 64×4, 3: 64×8, 4: 32×16, 5: 64×16 cells; 0: off) on a fully
-        periodic D3Q19 BGK/TRT lattice; bit-identical to single steps."""
+        """Temporal blocking on a fully periodic D3Q19 BGK/TRT lattice: two steps per launch
+        (tile 1: 32×8, 2: 64×4, 3: 64×8, 4: 32×16, 5: 64×16 cells) or three (6: 64×8, 7: 32×8,
+        8: 32×16; FP64 only 7); 0: off.  Bit-identical to single steps."""
         self._chk(L.load().kmi_lattice_set_temporal(self._h, int(tile), int(zchunks)))
 
     @property

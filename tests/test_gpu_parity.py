@@ -297,6 +297,45 @@ def test_two_step_launches_bit_identical(kmi, backend, storage, collision):
         assert np.array_equal(got, ref), (tile, zc, float(np.max(np.abs(got - ref))))
 
 
+@pytest.mark.parametrize("storage", ["f64", "f32", "f32s", "f16s"])
+@pytest.mark.parametrize("collision", ["trt", "bgk"])
+def test_three_step_launches_bit_identical(kmi, backend, storage, collision):
+    """Three steps per launch (t+1 and t+2 populations in two LDS levels) reproduce single-step
+    launches bit for bit, for every tile and z split down to one plane per range, with step
+    counts that leave one- and two-step remainders, and on a four-plane lattice."""
+    per = {f: {"kind": "periodic"} for f in ("west", "east", "south", "north", "bottom", "top")}
+    for shape in ((64, 16, 12), (32, 16, 4)):
+        desc = kmi.make_desc("d3q19", *shape, 0.02, storage=storage, collision=collision, faces=per)
+        f = rng_pops(shape, 19, seed=6, amp=0.05)
+        for steps in (37, 65):
+            print(f"[three-step] {storage} {collision} shape={shape} steps={steps}", flush=True)
+            with kmi.Lattice(desc, backend) as lat:
+                lat.upload(f)
+                lat.step(steps)
+                ref = lat.download()
+            modes = [(7, 1), (7, min(5, shape[2] - 1)), (7, shape[2])]
+            if storage != "f64":
+                modes += [(8, 2), (8, 3)] + ([(6, 1), (6, 4)] if shape[0] % 64 == 0 else [])
+            for tile, zc in modes:
+                with kmi.Lattice(desc, backend) as lat:
+                    lat.upload(f)
+                    lat.set_temporal(tile, zc)
+                    assert lat.temporal == {"tile": tile, "zchunks": zc}
+                    lat.step(steps)
+                    assert lat.steps == steps
+                    got = lat.download()
+                assert np.array_equal(got, ref), (shape, steps, tile, zc, float(np.max(np.abs(got - ref))))
+    with kmi.Lattice(kmi.make_desc("d3q19", 32, 8, 3, 0.02, storage=storage, collision=collision,
+                                   faces=per), backend) as lat:
+        with pytest.raises(kmi.KrakenError):
+            lat.set_temporal(7, 1)  # fewer than four planes
+    if storage == "f64":
+        with kmi.Lattice(kmi.make_desc("d3q19", 64, 16, 8, 0.02, storage="f64", faces=per), backend) as lat:
+            for tile in (6, 8):
+                with pytest.raises(kmi.KrakenError):
+                    lat.set_temporal(tile, 1)  # FP64 LDS budget
+
+
 def test_two_step_tuned_and_rejected_where_it_does_not_apply(kmi, backend):
     N = 64
     desc = kmi.taylor_green_3d_desc(N, 0.01, "f32")
