@@ -402,11 +402,16 @@ int kmi_batch_run(kmi_batch* b, double tol, int64_t check_every, int64_t window,
                   double* ms);
 /* Arrays of ncav entries (any may be NULL). */
 int kmi_batch_status(const kmi_batch* b, int32_t* status, int64_t* steps, double* residual);
-/* Tiles of tw × th cells (tw*th = 256, tw in {32,64,128,256}), `groups` persistent
+/* Tiles of tw × th cells (tw*th = 256, tw in {16,32,64,128,256}), `groups` persistent
  * work-groups (0: one per tile slot), non-temporal stores on/off. */
 int kmi_batch_set_launch(kmi_batch* b, int32_t tw, int32_t th, int32_t groups, int32_t nontemporal);
 int kmi_batch_get_launch(const kmi_batch* b, int32_t* tw, int32_t* th, int32_t* groups,
                          int32_t* nontemporal);
+/* Steps per launch: 1, or 2 (the tile's step t+1, with a one-cell ring, kept in LDS; HBM
+ * sees one read and one write per population per two steps).  Bit-identical to single
+ * steps; odd step counts end with a single step.  Chosen by kmi_batch_autotune. */
+int kmi_batch_set_temporal(kmi_batch* b, int32_t steps_per_launch);
+int kmi_batch_get_temporal(const kmi_batch* b, int32_t* steps_per_launch);
 /* Time every candidate tile shape × group count × store policy and keep the fastest;
  * populations, parity, status and step counts are restored. */
 int kmi_batch_autotune(kmi_batch* b, int64_t steps_per_trial, double* best_ms_per_step);

@@ -91,6 +91,94 @@ __global__ void __launch_bounds__(256) k_batch_step(const BatchArgs a) {
   }
 }
 
+// Pull of cell (i, j) of an n × n cavity: halfway bounce-back on the four walls (a link whose
+// source lies outside takes the node's own opposite population), plus the Ladd term on the
+// north lid when it is a moving wall — the rule of k_batch_step, with the source read through
+// ld(q, i, j) so the same code pulls from HBM or from LDS.
+template <class R, class LD>
+KMI_DEV void cavity_pull(const BatchArgs& a, int n, int i, int j, R (&fp)[9], LD&& ld) {
+  sfor<9>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr int cx = Lat<9>::cx[q], cy = Lat<9>::cy[q];
+    constexpr int o = Lat<9>::opp[q];
+    const bool ok = (cx > 0 ? i > 0 : (cx < 0 ? i < n - 1 : true)) &&
+                    (cy > 0 ? j > 0 : (cy < 0 ? j < n - 1 : true));
+    R v = ld(ok ? q : o, ok ? i - cx : i, ok ? j - cy : j);
+    if constexpr (cy < 0) {
+      if (!ok && a.lid == KMI_FACE_MOVING_WALL && j == n - 1)
+        v = v + R(six_w(Lat<9>::w[q])) * R(1.0) * cdotu<cx, cy, 0>(R(a.u_lid), R(0.0), R(0.0));
+    }
+    fp[q] = v;
+  });
+}
+
+// Lid closure and collision of a pulled cell (as in k_batch_step).
+template <int ST, int COLL, class R>
+KMI_DEV void cavity_collide(const BatchArgs& a, const CavRec& cv, int n, int j, R (&fp)[9], R (&fo)[9]) {
+  constexpr bool SH = Store<ST>::shifted;
+  if (a.lid == KMI_FACE_ZOUHE_VELOCITY && j == n - 1) zh_north<SH>(fp, R(a.u_lid));
+  if constexpr (COLL == KMI_BGK) collide_bgk<9, SH>(fp, fo, R(cv.om));
+  else if constexpr (COLL == KMI_TRT) collide_trt<9, SH>(fp, fo, R(cv.om), R(cv.sm));
+  else collide_mrt<SH>(fp, fo, R(a.s_e), R(a.s_eps), R(a.s_q), R(cv.om));
+}
+
+// Two steps per launch over the same tile slots.  Phase 1 evaluates step t+1 on the tile and
+// its one-cell ring (cells inside the cavity only), pulled from HBM, into LDS (9 values per
+// cell, in the storage rounding); phase 2 evaluates step t+2 on the tile from LDS and writes
+// HBM.  Bit-identical to two k_batch_step launches; HBM sees one read and one write of each
+// population per two steps.  Dynamic LDS: 9 · (tw+2)(th+2) reals.
+extern __shared__ double kmi_batch_lds[];
+
+template <int ST, int COLL, bool NTS>
+__global__ void __launch_bounds__(256) k_batch_step2(const BatchArgs a) {
+  using S = Store<ST>;
+  using E = typename S::elem;
+  using R = typename S::real;
+  R* const L = reinterpret_cast<R*>(kmi_batch_lds);
+  const int tw = a.tw, th = int(blockDim.x) / a.tw;
+  const int ew = tw + 2, ne = ew * (th + 2);
+  const int lx = threadIdx.x % tw, ly = threadIdx.x / tw;
+  for (int t = blockIdx.x; t < a.nslots; t += gridDim.x) {
+    const int c = a.slots[3 * t];  // uniform over the work-group
+    if (c < 0) continue;
+    const CavRec cv = a.cav[c];
+    const int n = cv.n;
+    const int x0 = a.slots[3 * t + 1], y0 = a.slots[3 * t + 2];
+    const long long P = (long long)n * n;
+    const E* __restrict__ src = static_cast<const E*>(a.fi) + cv.foff;
+    E* __restrict__ dst = static_cast<E*>(a.fo) + cv.foff;
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+      const int i = x0 - 1 + e % ew, j = y0 - 1 + e / ew;
+      if (i < 0 || j < 0 || i >= n || j >= n) continue;
+      R fp[9], fo[9];
+      cavity_pull(a, n, i, j, fp, [&](int q, int si, int sj) {
+        return S::ld(src, (long long)(si + n * sj) + q * P);
+      });
+      cavity_collide<ST, COLL>(a, cv, n, j, fp, fo);
+      sfor<9>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        L[q * ne + e] = storage_round<ST>(fo[q]);
+      });
+    }
+    __syncthreads();
+    const int i = x0 + lx, j = y0 + ly;
+    if (i < n && j < n) {
+      const int es = (ly + 1) * ew + lx + 1;
+      R fp[9], fo[9];
+      cavity_pull(a, n, i, j, fp, [&](int q, int si, int sj) {
+        return L[q * ne + es + (si - i) + (sj - j) * ew];
+      });
+      cavity_collide<ST, COLL>(a, cv, n, j, fp, fo);
+      const int self = i + n * j;
+      sfor<9>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        S::template st<NTS>(dst, self + q * P, fo[q]);
+      });
+    }
+    __syncthreads();  // the next slot's phase 1 overwrites LDS
+  }
+}
+
 // Velocity of a cell from the stored populations, FP64 (compute_macroscopic_2d order).
 template <int ST>
 KMI_DEV void cell_u(const void* f, const CavRec& cv, int self, double& ux, double& uy) {
@@ -244,6 +332,29 @@ static hipError_t launch_batch_step(int st, int coll, bool nts, const BatchArgs&
   return hipGetLastError();
 }
 
+static hipError_t launch_batch_step2(int st, int coll, bool nts, const BatchArgs& a, int groups,
+                                     hipStream_t s) {
+  const dim3 g(static_cast<unsigned>(groups)), b(256);
+  const int ne = (a.tw + 2) * (256 / a.tw + 2);
+#define KMI_BATCH_COLL2(ST)                                                                          \
+  {                                                                                                 \
+    const size_t sh = size_t(9) * ne * sizeof(typename Store<ST>::real);                            \
+    if (coll == KMI_BGK) {                                                                          \
+      if (nts) hipLaunchKernelGGL((k_batch_step2<ST, KMI_BGK, true>), g, b, sh, s, a);              \
+      else hipLaunchKernelGGL((k_batch_step2<ST, KMI_BGK, false>), g, b, sh, s, a);                 \
+    } else if (coll == KMI_TRT) {                                                                   \
+      if (nts) hipLaunchKernelGGL((k_batch_step2<ST, KMI_TRT, true>), g, b, sh, s, a);              \
+      else hipLaunchKernelGGL((k_batch_step2<ST, KMI_TRT, false>), g, b, sh, s, a);                 \
+    } else {                                                                                        \
+      if (nts) hipLaunchKernelGGL((k_batch_step2<ST, KMI_MRT, true>), g, b, sh, s, a);              \
+      else hipLaunchKernelGGL((k_batch_step2<ST, KMI_MRT, false>), g, b, sh, s, a);                 \
+    }                                                                                               \
+  }
+  KMI_BATCH_ST(st, KMI_BATCH_COLL2(ST));
+#undef KMI_BATCH_COLL2
+  return hipGetLastError();
+}
+
 static hipError_t launch_batch_resid(int st, const BatchArgs& a, const void* f, int mode,
                                      int groups, hipStream_t s) {
   KMI_BATCH_ST(st, hipLaunchKernelGGL((k_batch_resid<ST>), dim3(unsigned(groups)), dim3(256), 0, s, a, f, mode));
@@ -274,6 +385,7 @@ struct kmi_batch {
   int cur = 0;
   // tiles
   int tw = 64, th = 4, groups = 0, nts = 0;
+  int span = 1;  // steps per launch: 1, or 2 (k_batch_step2)
   bool tiles_dirty = true;
   int nslots = 0;
   int* slots_d = nullptr;
@@ -290,6 +402,7 @@ struct kmi_batch {
   std::vector<int64_t> steps;
   std::vector<double> resid;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
+  int graph_next[2] = {0, 1};  // buffer parity after a replay of graph[p]
   int64_t graph_check = 0, graph_window = 0;
   hipEvent_t ev[2] = {nullptr, nullptr};
 };
@@ -395,6 +508,25 @@ static int batch_launch_step(kmi_batch* B, hipStream_t s) {
   return KMI_OK;
 }
 
+static int batch_launch_step2(kmi_batch* B, hipStream_t s) {
+  BatchArgs a = batch_args(B);
+  hipError_t e = launch_batch_step2(B->desc.storage, B->desc.collision, B->nts != 0, a,
+                                    batch_groups(B), s);
+  if (e != hipSuccess) return hip_fail(B->ctx, e, "batch two-step launch");
+  B->cur ^= 1;
+  return KMI_OK;
+}
+
+// n steps: two-step launches when enabled, single steps for the rest.
+static int batch_launch_n(kmi_batch* B, int64_t n, hipStream_t s) {
+  if (B->span == 2)
+    for (; n >= 2; n -= 2)
+      if (int rc = batch_launch_step2(B, s)) return rc;
+  for (; n > 0; --n)
+    if (int rc = batch_launch_step(B, s)) return rc;
+  return KMI_OK;
+}
+
 static int batch_ensure_tiles(kmi_batch* B) {
   return B->tiles_dirty ? batch_build_tiles(B) : KMI_OK;
 }
@@ -407,13 +539,11 @@ static int batch_active(const kmi_batch* B) {
 
 // One check block: (check − m) steps, snapshot u, m steps, residual partials, per-cavity sums.
 static int batch_record_block(kmi_batch* B, int64_t check, int64_t window, hipStream_t s) {
-  for (int64_t k = 0; k + window < check; ++k)
-    if (int rc = batch_launch_step(B, s)) return rc;
+  if (int rc = batch_launch_n(B, check - window, s)) return rc;
   BatchArgs a = batch_args(B);
   hipError_t e = launch_batch_resid(B->desc.storage, a, B->f[B->cur], 0, batch_groups(B), s);
   if (e != hipSuccess) return hip_fail(B->ctx, e, "batch snapshot");
-  for (int64_t k = 0; k < window; ++k)
-    if (int rc = batch_launch_step(B, s)) return rc;
+  if (int rc = batch_launch_n(B, window, s)) return rc;
   a = batch_args(B);
   e = launch_batch_resid(B->desc.storage, a, B->f[B->cur], 1, batch_groups(B), s);
   if (e != hipSuccess) return hip_fail(B->ctx, e, "batch residual");
@@ -644,8 +774,7 @@ int kmi_batch_step(kmi_batch* B, int64_t nsteps) {
   KMI_HIP(x, hipSetDevice(x->device));
   if (int rc = batch_ensure_tiles(B)) return rc;
   if (batch_active(B) == 0) return KMI_OK;
-  for (int64_t k = 0; k < nsteps; ++k)
-    if (int rc = batch_launch_step(B, x->stream)) return rc;
+  if (int rc = batch_launch_n(B, nsteps, x->stream)) return rc;
   for (int c = 0; c < B->ncav; ++c)
     if (B->status[c] == KMI_CAV_RUNNING) B->steps[c] += nsteps;
   KMI_HIP(x, hipStreamSynchronize(x->stream));
@@ -710,6 +839,7 @@ int kmi_batch_run(kmi_batch* B, double tol, int64_t check, int64_t window, int64
       KMI_HIP(x, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       int rc = batch_record_block(B, check, window, s);
       hipError_t e = hipStreamEndCapture(s, &gph);
+      B->graph_next[p] = B->cur;
       B->cur = p;  // capture recorded the launches only
       if (rc != KMI_OK) {
         if (gph) (void)hipGraphDestroy(gph);
@@ -721,7 +851,7 @@ int kmi_batch_run(kmi_batch* B, double tol, int64_t check, int64_t window, int64
       if (e != hipSuccess) return hip_fail(x, e, "batch graph instantiate");
     }
     KMI_HIP(x, hipGraphLaunch(B->graph[p], s));
-    B->cur = int((p + check) & 1);
+    B->cur = B->graph_next[p];
     KMI_HIP(x, hipMemcpyAsync(r.data(), B->resid_d, r.size() * sizeof(double), hipMemcpyDeviceToHost, s));
     KMI_HIP(x, hipStreamSynchronize(s));
     for (int c = 0; c < B->ncav; ++c) {
@@ -736,6 +866,13 @@ int kmi_batch_run(kmi_batch* B, double tol, int64_t check, int64_t window, int64
       if (st != KMI_CAV_RUNNING) {
         B->status[c] = st;
         B->tiles_dirty = true;
+        // A retired cavity is no longer stepped while the buffer parity keeps changing:
+        // mirror its final populations into the other buffer, so both hold them.
+        const size_t off = size_t(B->cav[c].foff) * B->esize;
+        const size_t bytes = size_t(9) * B->n[c] * B->n[c] * B->esize;
+        KMI_HIP(x, hipMemcpyAsync(static_cast<char*>(B->f[B->cur ^ 1]) + off,
+                                  static_cast<const char*>(B->f[B->cur]) + off, bytes,
+                                  hipMemcpyDeviceToDevice, s));
       }
     }
   }
@@ -759,8 +896,8 @@ int kmi_batch_status(const kmi_batch* B, int32_t* status, int64_t* steps, double
 
 int kmi_batch_set_launch(kmi_batch* B, int32_t tw, int32_t th, int32_t groups, int32_t nts) {
   if (!B) return fail(nullptr, KMI_ERR_ARG, "batch is NULL");
-  if (!((tw == 32 || tw == 64 || tw == 128 || tw == 256) && tw * th == 256) || groups < 0)
-    return fail(B->ctx, KMI_ERR_ARG, "kmi_batch_set_launch: tiles of tw x th = 256 cells, tw in {32,64,128,256}");
+  if (!((tw == 16 || tw == 32 || tw == 64 || tw == 128 || tw == 256) && tw * th == 256) || groups < 0)
+    return fail(B->ctx, KMI_ERR_ARG, "kmi_batch_set_launch: tiles of tw x th = 256 cells, tw in {16,32,64,128,256}");
   KMI_HIP(B->ctx, hipSetDevice(B->ctx->device));
   KMI_HIP(B->ctx, hipStreamSynchronize(B->ctx->stream));
   B->tw = tw;
@@ -769,6 +906,23 @@ int kmi_batch_set_launch(kmi_batch* B, int32_t tw, int32_t th, int32_t groups, i
   B->nts = nts ? 1 : 0;
   B->tiles_dirty = true;
   batch_drop_graphs(B);
+  return KMI_OK;
+}
+
+int kmi_batch_set_temporal(kmi_batch* B, int32_t steps_per_launch) {
+  if (!B) return fail(nullptr, KMI_ERR_ARG, "batch is NULL");
+  if (steps_per_launch != 1 && steps_per_launch != 2)
+    return fail(B->ctx, KMI_ERR_ARG, "kmi_batch_set_temporal: steps per launch must be 1 or 2");
+  KMI_HIP(B->ctx, hipSetDevice(B->ctx->device));
+  KMI_HIP(B->ctx, hipStreamSynchronize(B->ctx->stream));
+  B->span = steps_per_launch;
+  batch_drop_graphs(B);
+  return KMI_OK;
+}
+
+int kmi_batch_get_temporal(const kmi_batch* B, int32_t* steps_per_launch) {
+  if (!B || !steps_per_launch) return fail(nullptr, KMI_ERR_ARG, "kmi_batch_get_temporal: NULL argument");
+  *steps_per_launch = B->span;
   return KMI_OK;
 }
 
@@ -787,31 +941,32 @@ int kmi_batch_autotune(kmi_batch* B, int64_t steps_per_trial, double* best_ms) {
   KMI_HIP(x, hipSetDevice(x->device));
   KMI_HIP(x, hipStreamSynchronize(x->stream));
   if (batch_active(B) == 0) return fail(x, KMI_ERR_STATE, "kmi_batch_autotune: no running cavity");
-  const int64_t trial = std::max<int64_t>(2, steps_per_trial);
+  int64_t trial = std::max<int64_t>(2, steps_per_trial);
+  trial += trial & 1;  // even, so two-step variants time whole launches
   const size_t bytes = size_t(B->total_pop) * B->esize;
   void* save = nullptr;
   KMI_HIP(x, hipMalloc(&save, bytes));
   hipError_t e = hipMemcpyAsync(save, B->f[B->cur], bytes, hipMemcpyDeviceToDevice, x->stream);
   const int cur0 = B->cur;
-  const int shapes[][2] = {{256, 1}, {128, 2}, {64, 4}, {32, 8}};
+  const int shapes[][2] = {{256, 1}, {128, 2}, {64, 4}, {32, 8}, {16, 16}};
   const int group_opts[] = {0, 8192, 4096, 2048, 1024};
   double best = 1e300;
-  int bb[4] = {B->tw, B->th, B->groups, B->nts};
+  int bb[5] = {B->tw, B->th, B->groups, B->nts, B->span};
   for (const auto& sh : shapes) {
     for (int g : group_opts) {
-      for (int nt = 0; nt < 2 && e == hipSuccess; ++nt) {
+      for (int nt = 0; nt < 4 && e == hipSuccess; ++nt) {
         B->tw = sh[0];
         B->th = sh[1];
         B->groups = g;
-        B->nts = nt;
+        B->nts = nt & 1;
+        B->span = 1 + (nt >> 1);
         if (batch_build_tiles(B) != KMI_OK) { e = hipErrorUnknown; break; }
         if (g > 0 && g >= B->nslots) continue;  // same as one group per slot
-        if (batch_launch_step(B, x->stream) != KMI_OK) { e = hipErrorUnknown; break; }
+        if (batch_launch_n(B, B->span, x->stream) != KMI_OK) { e = hipErrorUnknown; break; }
         double t = 1e300;
         for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
           e = hipEventRecord(B->ev[0], x->stream);
-          for (int64_t k = 0; k < trial && e == hipSuccess; ++k)
-            if (batch_launch_step(B, x->stream) != KMI_OK) e = hipErrorUnknown;
+          if (e == hipSuccess && batch_launch_n(B, trial, x->stream) != KMI_OK) e = hipErrorUnknown;
           if (e == hipSuccess) e = hipEventRecord(B->ev[1], x->stream);
           if (e == hipSuccess) e = hipEventSynchronize(B->ev[1]);
           float ms = 0.f;
@@ -823,7 +978,8 @@ int kmi_batch_autotune(kmi_batch* B, int64_t steps_per_trial, double* best_ms) {
           bb[0] = sh[0];
           bb[1] = sh[1];
           bb[2] = g;
-          bb[3] = nt;
+          bb[3] = nt & 1;
+          bb[4] = 1 + (nt >> 1);
         }
       }
     }
@@ -832,6 +988,7 @@ int kmi_batch_autotune(kmi_batch* B, int64_t steps_per_trial, double* best_ms) {
   B->th = bb[1];
   B->groups = bb[2];
   B->nts = bb[3];
+  B->span = bb[4];
   B->tiles_dirty = true;
   batch_drop_graphs(B);
   hipError_t e2 = hipMemcpyAsync(B->f[cur0], save, bytes, hipMemcpyDeviceToDevice, x->stream);

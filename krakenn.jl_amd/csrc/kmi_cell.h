@@ -17,6 +17,14 @@ KMI_DEV R fmad(R a, R b, R c) {
   else return __builtin_fma(a, b, c);
 }
 
+// Value a population takes after a store to and a load from storage ST (the multi-step
+// kernels keep intermediate populations on chip in this rounding).
+template <int ST, class R>
+KMI_DEV R storage_round(R v) {
+  if constexpr (ST == KMI_STORE_F16S) return float(_Float16(v * kF16Scale)) * kF16InvScale;
+  else return v;
+}
+
 template <int Q, class R>
 KMI_DEV R usq_of(R ux, R uy, R uz) {
   if constexpr (Q == 9) return ux * ux + uy * uy;

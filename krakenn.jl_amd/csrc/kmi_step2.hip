@@ -59,13 +59,6 @@ KMI_DEV E* at(E* plane, unsigned off) {
 }
 }  // namespace tb
 
-// Value a population takes after a store to and a load from storage ST.
-template <int ST, class R>
-KMI_DEV R storage_round(R v) {
-  if constexpr (ST == KMI_STORE_F16S) return float(_Float16(v * kF16Scale)) * kF16InvScale;
-  else return v;
-}
-
 template <int ST, int COLL, class R>
 KMI_DEV void collide19(const R (&fp)[19], R (&fo)[19], const StepArgs& a) {
   constexpr bool SH = Store<ST>::shifted;

@@ -144,6 +144,8 @@ SIGNATURES = {
     "kmi_batch_status": (c_int, [c_void_p, POINTER(c_int32), POINTER(c_int64), POINTER(c_double)]),
     "kmi_batch_set_launch": (c_int, [c_void_p, c_int32, c_int32, c_int32, c_int32]),
     "kmi_batch_get_launch": (c_int, [c_void_p] + [POINTER(c_int32)] * 4),
+    "kmi_batch_set_temporal": (c_int, [c_void_p, c_int32]),
+    "kmi_batch_get_temporal": (c_int, [c_void_p, POINTER(c_int32)]),
     "kmi_batch_autotune": (c_int, [c_void_p, c_int64, POINTER(c_double)]),
     "kmi_batch_active_cells": (c_int, [c_void_p, POINTER(c_int64)]),
     "kmi_nn_desc_init": (None, [c_void_p]),

@@ -138,7 +138,15 @@ class Batch:
     def launch(self):
         v = [c_int32() for _ in range(4)]
         self._chk(L.load().kmi_batch_get_launch(self._h, *[byref(x) for x in v]))
-        return {"tw": v[0].value, "th": v[1].value, "groups": v[2].value, "nontemporal": bool(v[3].value)}
+        sp = c_int32()
+        self._chk(L.load().kmi_batch_get_temporal(self._h, byref(sp)))
+        return {"tw": v[0].value, "th": v[1].value, "groups": v[2].value, "nontemporal": bool(v[3].value),
+                "steps_per_launch": sp.value}
+
+    def set_temporal(self, steps_per_launch):
+        """1: one step per launch; 2: two steps per launch (step t+1 of each tile and its ring
+        kept in LDS), bit-identical to single steps."""
+        self._chk(L.load().kmi_batch_set_temporal(self._h, int(steps_per_launch)))
 
     def autotune(self, steps_per_trial=8):
         ms = c_double()

@@ -48,10 +48,13 @@ def test_batch_tile_shapes_and_groups_are_invisible(kmi, backend):
             assert np.array_equal(o[c], outs[0][c])
 
 
-def test_batch_run_to_residual_matches_single_lattices(kmi, backend):
+@pytest.mark.parametrize("check", [50, 51])
+def test_batch_run_to_residual_matches_single_lattices(kmi, backend, check):
+    """Odd check intervals included: a cavity retired while the others keep stepping still
+    reads back its final populations."""
     sizes = [16, 24, 20, 32]
     nus = [0.1, 0.08, 0.05, 0.12]
-    tol, check = 1e-6, 50
+    tol = 1e-6
     with kmi.Batch(sizes, nus, storage="f64", collision="mrt", backend=backend) as b:
         b.run(tol, check, 200_000)
         st = b.status()
@@ -175,3 +178,62 @@ def test_batch_consistent_init_arguments(kmi, backend):
             b.consistent_init(-1)
         b.consistent_init(0)
         assert np.all(b.status()["steps"] == 0)
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32", "f32s", "f16s"])
+@pytest.mark.parametrize("collision,lid", [("mrt", "zouhe"), ("bgk", "ladd"), ("trt", "zouhe")])
+def test_batch_two_step_launches_bit_identical(kmi, backend, storage, collision, lid):
+    """Two steps per launch (step t+1 of a tile and its ring in LDS) reproduce single steps bit
+    for bit on every tile shape, with uneven cavity sizes, odd step counts and persistent
+    groups."""
+    f0 = [rng_pops((n, n), 9, seed=70 + c, amp=0.02) for c, n in enumerate(SIZES)]
+    print(f"[batch two-step] {storage} {collision} {lid}", flush=True)
+    with kmi.Batch(SIZES, NUS, storage=storage, collision=collision, lid=lid, backend=backend) as b:
+        for c in range(len(SIZES)):
+            b.upload(c, f0[c])
+        b.step(37)
+        ref = [b.download(c) for c in range(len(SIZES))]
+    for tw, th, groups in ((64, 4, 0), (32, 8, 16), (16, 16, 0), (256, 1, 0), (128, 2, 8)):
+        with kmi.Batch(SIZES, NUS, storage=storage, collision=collision, lid=lid, backend=backend) as b:
+            b.set_launch(tw, th, groups)
+            b.set_temporal(2)
+            assert b.launch["steps_per_launch"] == 2
+            for c in range(len(SIZES)):
+                b.upload(c, f0[c])
+            b.step(37)
+            assert np.all(b.status()["steps"] == 37)
+            for c in range(len(SIZES)):
+                assert np.array_equal(b.download(c), ref[c]), (tw, th, groups, c)
+
+
+def test_batch_two_step_run_to_residual(kmi, backend):
+    """Graph-replayed check blocks with two-step launches (odd and even block lengths) give
+    the single-step run's steps, residuals and fields."""
+    sizes, nus = [16, 24, 20, 32], [0.1, 0.08, 0.05, 0.12]
+    out = []
+    for span in (1, 2):
+        for check, window in ((50, 50), (51, 7)):
+            with kmi.Batch(sizes, nus, storage="f32s", collision="mrt", backend=backend) as b:
+                b.set_temporal(span)
+                b.run(1e-6, check, 60_000, window=window)
+                st = b.status()
+                out.append((span, check, st, [b.download(c) for c in range(len(sizes))]))
+    for span, check, st, f in out[2:]:
+        ref = next(o for o in out[:2] if o[1] == check)
+        assert np.array_equal(st["steps"], ref[2]["steps"]) and np.array_equal(st["residual"], ref[2]["residual"])
+        assert np.array_equal(st["status"], ref[2]["status"])
+        for a, r in zip(f, ref[3]):
+            assert np.array_equal(a, r)
+
+
+def test_batch_autotune_considers_two_step(kmi, backend):
+    sizes = [64, 96, 128]
+    with kmi.Batch(sizes, [0.02, 0.03, 0.04], backend=backend) as b:
+        b.step(5)
+        before = [b.download(c) for c in range(len(sizes))]
+        b.autotune(4)
+        print(f"[batch two-step] tuned {b.launch}", flush=True)
+        assert b.launch["steps_per_launch"] in (1, 2)
+        assert all(np.array_equal(b.download(c), before[c]) for c in range(len(sizes)))
+        with pytest.raises(kmi.KrakenError):
+            b.set_temporal(3)
