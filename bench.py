@@ -75,7 +75,21 @@ def bench_cavities(args):
     n, nu, Re = kmi.ensemble_parameters(256, 2024)
     cells = int((n.astype(np.int64) ** 2).sum())
     with kmi.Batch(n, nu, storage="f32s", collision="mrt") as b:
-        full_ms = b.autotune(6)
+        if args.batch_launch:
+            tw, th, groups, nt, span = (int(v) for v in args.batch_launch.split(","))
+            b.set_launch(tw, th, groups, bool(nt))
+            b.set_temporal(span)
+            b.step(2 * span)  # warm-up, then time a full-load launch sequence
+            b.backend.synchronize()
+            t0 = time.perf_counter()
+            b.step(20 * span)
+            b.backend.synchronize()
+            full_ms = (time.perf_counter() - t0) * 1e3 / (20 * span)
+            b.reset_status()
+            for c in range(len(n)):
+                b.init_equilibrium(c)
+        else:
+            full_ms = b.autotune(6)
         t0 = time.perf_counter()
         ms = b.run(1e-7, 100, args.cavity_max_steps, window=100)
         wall = time.perf_counter() - t0
@@ -294,6 +308,8 @@ def main():
     ap.add_argument("--ws-consistent", type=int, default=256,
                     help="consistent-initialisation steps of the variant warm start")
     ap.add_argument("--cavity-max-steps", type=int, default=600_000)
+    ap.add_argument("--batch-launch", default=None,
+                    help="TW,TH,GROUPS,NT,SPAN: fixed batch launch for the cavities workload (no autotune)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=64)
