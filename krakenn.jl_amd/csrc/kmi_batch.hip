@@ -647,6 +647,17 @@ int kmi_batch_destroy(kmi_batch* B) {
   return KMI_OK;
 }
 
+// A cavity that is not running keeps its populations in both buffers (the parity moves on
+// while the others step): after writing f[cur], copy the region to the other buffer.
+static hipError_t batch_mirror_if_retired(kmi_batch* B, int32_t c, hipStream_t s) {
+  if (B->status[c] == KMI_CAV_RUNNING) return hipSuccess;
+  const size_t off = size_t(B->cav[c].foff) * B->esize;
+  const size_t bytes = size_t(9) * B->n[c] * B->n[c] * B->esize;
+  return hipMemcpyAsync(static_cast<char*>(B->f[B->cur ^ 1]) + off,
+                        static_cast<const char*>(B->f[B->cur]) + off, bytes,
+                        hipMemcpyDeviceToDevice, s);
+}
+
 static int batch_check_cav(kmi_batch* B, int32_t c) {
   if (!B) return fail(nullptr, KMI_ERR_ARG, "batch is NULL");
   if (c < 0 || c >= B->ncav) return fail(B->ctx, KMI_ERR_ARG, "cavity index out of range");
@@ -671,6 +682,7 @@ int kmi_batch_upload(kmi_batch* B, int32_t c, const void* host, int32_t dt) {
   hipError_t e = hipMemcpyAsync(stage, src, size_t(9 * nc) * sizeof(double), hipMemcpyHostToDevice, x->stream);
   char* base = static_cast<char*>(B->f[B->cur]) + B->cav[c].foff * B->esize;
   if (e == hipSuccess) e = launch_to_storage(9, B->desc.storage, stage, base, nc, nc, 1.0, x->stream);
+  if (e == hipSuccess) e = batch_mirror_if_retired(B, c, x->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(x->stream);
   (void)hipFree(stage);
   if (e != hipSuccess) return hip_fail(x, e, "kmi_batch_upload");
@@ -722,6 +734,7 @@ int kmi_batch_init_equilibrium(kmi_batch* B, int32_t c, const double* rho, const
   }
   char* base = static_cast<char*>(B->f[B->cur]) + B->cav[c].foff * B->esize;
   if (e == hipSuccess) e = launch_init_eq(9, B->desc.storage, base, dev[0], dev[1], dev[2], nullptr, nc, nc, x->stream);
+  if (e == hipSuccess) e = batch_mirror_if_retired(B, c, x->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(x->stream);
   if (block) (void)hipFree(block);
   if (e != hipSuccess) return hip_fail(x, e, "kmi_batch_init_equilibrium");
@@ -866,13 +879,8 @@ int kmi_batch_run(kmi_batch* B, double tol, int64_t check, int64_t window, int64
       if (st != KMI_CAV_RUNNING) {
         B->status[c] = st;
         B->tiles_dirty = true;
-        // A retired cavity is no longer stepped while the buffer parity keeps changing:
-        // mirror its final populations into the other buffer, so both hold them.
-        const size_t off = size_t(B->cav[c].foff) * B->esize;
-        const size_t bytes = size_t(9) * B->n[c] * B->n[c] * B->esize;
-        KMI_HIP(x, hipMemcpyAsync(static_cast<char*>(B->f[B->cur ^ 1]) + off,
-                                  static_cast<const char*>(B->f[B->cur]) + off, bytes,
-                                  hipMemcpyDeviceToDevice, s));
+        // no longer stepped while the buffer parity keeps changing: both buffers hold it
+        KMI_HIP(x, batch_mirror_if_retired(B, c, s));
       }
     }
   }

@@ -237,3 +237,21 @@ def test_batch_autotune_considers_two_step(kmi, backend):
         assert all(np.array_equal(b.download(c), before[c]) for c in range(len(sizes)))
         with pytest.raises(kmi.KrakenError):
             b.set_temporal(3)
+
+
+def test_batch_retired_cavity_keeps_written_state(kmi, po, backend):
+    """A cavity that is no longer running reads back what was last written to it, however
+    many steps the running cavities take afterwards (odd counts included)."""
+    with kmi.Batch([8, 16], [0.2, 0.01], storage="f64", collision="mrt", backend=backend) as b:
+        b.run(1e-3, 9, 300, window=3)  # the small viscous cavity converges first
+        st = b.status()
+        assert st["status"][0] == kmi._lib.CAV_CONVERGED
+        assert st["status"][1] == kmi._lib.CAV_MAX_STEPS, st
+        f0 = rng_pops((8, 8), 9, seed=91, amp=0.02)
+        b.upload(0, f0)
+        b.resume()  # cavity 1 (max_steps) runs on, cavity 0 stays retired
+        b.step(7)
+        assert np.array_equal(b.download(0), f0)
+        b.init_equilibrium(0)
+        b.step(3)
+        assert np.array_equal(b.download(0), po.rest((8, 8)))
