@@ -61,16 +61,20 @@ def test_config2_tgv_256_fp64_bitwise_and_reduced_precision(kmi, po, backend):
     """D3Q19 TRT Λ = 3/16, triply periodic TGV 256³, U0 = 0.05, ν = 0.01."""
     N, nu, U0 = 256, 0.01, 0.05
     fields = kmi.taylor_green_3d_fields(N, U0)
-    print("[config2] TGV 256^3 FP64 2-step full-size parity", flush=True)
     d64 = kmi.taylor_green_3d_desc(N, nu, "f64")
-    f0 = po.equilibrium(*fields)
-    with kmi.Lattice(d64, backend) as lat:
-        lat.upload(f0)
-        lat.step(2)
-        fg = lat.download()
-    po.run(d64, f0, None, 2)
-    assert np.array_equal(fg, f0)
-    del fg, f0
+    # single-step, two-step (tile 3) and three-step (tile 7) launches at full size
+    for tile, zc, nst in ((0, 1, 2), (3, 2, 2), (7, 2, 3)):
+        print(f"[config2] TGV 256^3 FP64 full-size parity, temporal tile {tile}, {nst} steps", flush=True)
+        f0 = po.equilibrium(*fields)
+        with kmi.Lattice(d64, backend) as lat:
+            lat.upload(f0)
+            if tile:
+                lat.set_temporal(tile, zc)
+            lat.step(nst)
+            fg = lat.download()
+        po.run(d64, f0, None, nst)
+        assert np.array_equal(fg, f0), tile
+        del fg, f0
     steps = 1000
     res = {}
     for st in ("f64", "f32", "f32s", "f16s"):
