@@ -93,17 +93,20 @@ __global__ void __launch_bounds__(256) k_batch_step(const BatchArgs a) {
 
 // Pull of cell (i, j) of an n × n cavity: halfway bounce-back on the four walls (a link whose
 // source lies outside takes the node's own opposite population), plus the Ladd term on the
-// north lid when it is a moving wall — the rule of k_batch_step, with the source read through
-// ld(q, i, j) so the same code pulls from HBM or from LDS.
+// north lid when it is a moving wall — the rule of k_batch_step.  The source is element
+// ld(off) of a store laid out as 9 planes of `plane` elements with rows of `stride`
+// (HBM: the cavity, n and n²; LDS: the extended tile); `self` is the cell's own index.
 template <class R, class LD>
-KMI_DEV void cavity_pull(const BatchArgs& a, int n, int i, int j, R (&fp)[9], LD&& ld) {
+KMI_DEV void cavity_pull(const BatchArgs& a, int n, int i, int j, unsigned self, unsigned stride,
+                         unsigned plane, R (&fp)[9], LD&& ld) {
   sfor<9>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
     constexpr int cx = Lat<9>::cx[q], cy = Lat<9>::cy[q];
     constexpr int o = Lat<9>::opp[q];
     const bool ok = (cx > 0 ? i > 0 : (cx < 0 ? i < n - 1 : true)) &&
                     (cy > 0 ? j > 0 : (cy < 0 ? j < n - 1 : true));
-    R v = ld(ok ? q : o, ok ? i - cx : i, ok ? j - cy : j);
+    const unsigned back = unsigned(cx) + unsigned(cy) * stride;  // source = self − back
+    R v = ld(ok ? self - back + unsigned(q) * plane : self + unsigned(o) * plane);
     if constexpr (cy < 0) {
       if (!ok && a.lid == KMI_FACE_MOVING_WALL && j == n - 1)
         v = v + R(six_w(Lat<9>::w[q])) * R(1.0) * cdotu<cx, cy, 0>(R(a.u_lid), R(0.0), R(0.0));
@@ -151,9 +154,8 @@ __global__ void __launch_bounds__(256) k_batch_step2(const BatchArgs a) {
       const int i = x0 - 1 + e % ew, j = y0 - 1 + e / ew;
       if (i < 0 || j < 0 || i >= n || j >= n) continue;
       R fp[9], fo[9];
-      cavity_pull(a, n, i, j, fp, [&](int q, int si, int sj) {
-        return S::ld(src, (long long)(si + n * sj) + q * P);
-      });
+      cavity_pull(a, n, i, j, unsigned(i + n * j), unsigned(n), unsigned(P), fp,
+                  [&](unsigned off) { return S::ld(src, off); });
       cavity_collide<ST, COLL>(a, cv, n, j, fp, fo);
       sfor<9>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
@@ -165,9 +167,8 @@ __global__ void __launch_bounds__(256) k_batch_step2(const BatchArgs a) {
     if (i < n && j < n) {
       const int es = (ly + 1) * ew + lx + 1;
       R fp[9], fo[9];
-      cavity_pull(a, n, i, j, fp, [&](int q, int si, int sj) {
-        return L[q * ne + es + (si - i) + (sj - j) * ew];
-      });
+      cavity_pull(a, n, i, j, unsigned(es), unsigned(ew), unsigned(ne), fp,
+                  [&](unsigned off) { return L[off]; });
       cavity_collide<ST, COLL>(a, cv, n, j, fp, fo);
       const int self = i + n * j;
       sfor<9>([&](auto qc) {

@@ -143,10 +143,65 @@ KMI_DEV void collide_trt(const R (&fp)[Q], R (&fo)[Q], R sp, R sm) {
   }
 }
 
+// Shifted-storage D2Q9 MRT: the same relaxation as below on the moments of the deviations
+// d = f − w, with the transforms written as butterflies over opposite pairs (axis pairs
+// (1,3), (2,4), diagonal pairs (5,7), (6,8), 0-based) and fused multiply-adds, and
+// q_eq = −ρu taken as −j; about half the floating-point operations of the reference's
+// row-by-row products, which the plain storages keep.
+template <class R>
+KMI_DEV void collide_mrt_shifted(const R (&d)[9], R (&fo)[9], R s_e, R s_eps, R s_q, R s_nu) {
+  const R a = d[1] + d[3], b = d[2] + d[4];  // axis pairs
+  const R c = d[5] + d[7], g = d[6] + d[8];  // diagonal pairs
+  const R dx = d[1] - d[3], dy = d[2] - d[4];
+  const R dc = d[5] - d[7], dg = d[6] - d[8];
+  const R sa = a + b, sd = c + g;
+  const R m0 = d[0] + sa + sd;
+  const R e = fmad(R(2), sd, fmad(R(-4), d[0], -sa));
+  const R eps = fmad(R(4), d[0], fmad(R(-2), sa, sd));
+  const R u1 = dc - dg, u2 = dc + dg;
+  const R jx = dx + u1, jy = dy + u2;
+  const R qx = fmad(R(-2), dx, u1), qy = fmad(R(-2), dy, u2);
+  const R pxx = a - b, pxy = c - g;
+  const R rho = R(1) + m0;
+  R inv;
+  if constexpr (sizeof(R) == 4) inv = __builtin_amdgcn_rcpf(rho);
+  else inv = R(1) / rho;
+  const R ux = jx * inv, uy = jy * inv;
+  const R t = R(3) * rho * fmad(uy, uy, ux * ux);  // 3ρu²
+  const R es = fmad(-s_e, e - fmad(R(-2), m0, t), e);  // e_eq + 2 = −2δρ + 3ρu²
+  const R ep = fmad(-s_eps, eps - (m0 - t), eps);       // ε_eq − 1 = δρ − 3ρu²
+  const R qxs = fmad(-s_q, qx + jx, qx);
+  const R qys = fmad(-s_q, qy + jy, qy);
+  const R ps = fmad(-s_nu, pxx - fmad(jx, ux, -(jy * uy)), pxx);  // p_xx,eq = ρ(ux² − uy²)
+  const R pxys = fmad(-s_nu, pxy - jx * uy, pxy);                  // p_xy,eq = ρ ux uy
+  const R r9 = R(1.0 / 9) * m0;
+  fo[0] = fmad(R(1.0 / 9), ep - es, r9);
+  const R A = fmad(R(-1.0 / 36), es, fmad(R(-1.0 / 18), ep, r9));
+  const R B = fmad(R(1.0 / 18), es, fmad(R(1.0 / 36), ep, r9));
+  const R tx = R(1.0 / 6) * (jx - qxs), ty = R(1.0 / 6) * (jy - qys), p4 = R(0.25) * ps;
+  const R ap = A + p4, am = A - p4;
+  fo[1] = ap + tx;
+  fo[3] = ap - tx;
+  fo[2] = am + ty;
+  fo[4] = am - ty;
+  const R X = fmad(R(1.0 / 6), jx, R(1.0 / 12) * qxs);
+  const R Y = fmad(R(1.0 / 6), jy, R(1.0 / 12) * qys);
+  const R P = R(0.25) * pxys;
+  const R bp = B + X, bm = B - X, yp = Y + P, ym = Y - P;
+  fo[5] = bp + yp;
+  fo[6] = bm + ym;
+  fo[7] = bm - ym;
+  fo[8] = bp - yp;
+}
+
 // D2Q9 MRT (Lallemand–Luo), collide_mrt_2d.jl:56-107.  Shifted form: the moments of
 // the deviation, with m_eq shifted by the rest-state moments (e = −2, ε = 1).
 template <bool SH, class R>
 KMI_DEV void collide_mrt(const R (&fp)[9], R (&fo)[9], R s_e, R s_eps, R s_q, R s_nu) {
+  if constexpr (SH) {
+    collide_mrt_shifted(fp, fo, s_e, s_eps, s_q, s_nu);
+    return;
+  }
   const R f1 = fp[0], f2 = fp[1], f3 = fp[2], f4 = fp[3], f5 = fp[4], f6 = fp[5], f7 = fp[6],
           f8 = fp[7], f9 = fp[8];
   const R m0 = f1 + f2 + f3 + f4 + f5 + f6 + f7 + f8 + f9;
