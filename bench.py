@@ -19,6 +19,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "krakenn.jl_amd"))
 
 HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E, published
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA, published
 BYTES_PER_POP = {"f64": 8, "f32": 4, "f32s": 4, "f16s": 2}
 STORAGE_LABEL = {"f64": "FP64 storage", "f32": "FP32 storage",
                  "f32s": "FP32 shifted storage f-w", "f16s": "FP16 shifted storage"}
@@ -196,6 +197,70 @@ def bench_small(args):
     print(json.dumps(out), flush=True)
 
 
+def decoder_forward_flop(d):
+    """BF16 product FLOPs of one forward sample of the warm-start decoder: the MLP
+    (in → hidden → latent) and three 3×3 convolutions after 2× upsampling."""
+    lat = d.latent_hw * d.latent_hw * d.latent_c
+    fwd = 2 * (d.in_dim * d.hidden + d.hidden * lat)
+    for hi, ci, co in ((d.latent_hw, d.latent_c, d.c1), (2 * d.latent_hw, d.c1, d.c2), (4 * d.latent_hw, d.c2, d.out_c)):
+        fwd += 2 * (2 * hi) ** 2 * 9 * ci * co
+    return fwd
+
+
+def bench_nn(args):
+    """BF16-MFMA throughput of the warm-start decoder's own kernels: training epochs (forward,
+    both gradients, Adam) over 2048 samples of 128×128×3 targets at batch 32, then inference of
+    256 samples.  Targets are smooth synthetic fields of the inputs (the kernels' work does not
+    depend on their values).  A step is one epoch."""
+    import numpy as np
+
+    from kraken_mi355x import warmstart as ws
+    from kraken_mi355x.nn import WarmStartNet
+
+    n, B, N = 2048, 32, 128
+    Re, A = ws.draw_cases(n, seed=1)
+    x = ws.encode(Re, A)
+    yy, xx = np.meshgrid(np.linspace(0, 1, N), np.linspace(0, 1, N), indexing="ij")
+    y = np.stack([np.stack([np.sin(np.pi * (1 + c) * xx + x[k, 0]) * np.cos(np.pi * yy * (1 + x[k, 1]))
+                            for c in range(3)], axis=-1) for k in range(n)]).astype(np.float32)
+    net = WarmStartNet(latent_hw=N // 8, max_batch=B, seed=1)
+    net.set_dataset(x, y)
+    net.train_epochs(max(args.warmup // 32, 1), B, 1e-3, seed=3)
+    net.backend.synchronize()
+    epochs = max(args.steps // 200, 1)
+    t0 = time.perf_counter()
+    loss = net.train_epochs(epochs, B, 1e-3, seed=5)
+    net.backend.synchronize()
+    dt = time.perf_counter() - t0
+    fwd = decoder_forward_flop(net.desc)
+    flop = 3.0 * fwd * n * epochs
+    xt = ws.encode(*ws.draw_cases(256, seed=99))
+    net.predict(xt)
+    net.backend.synchronize()
+    t1 = time.perf_counter()
+    net.predict(xt)
+    net.backend.synchronize()
+    dti = time.perf_counter() - t1
+    achieved = flop / dt / 1e12
+    out = {
+        "metric": "BF16 TFLOP/s of the warm-start decoder training (forward + gradients + Adam), batch 32",
+        "value": round(achieved, 2), "unit": "TFLOP/s", "n_gpus": 1, "steps": epochs, "warmup": 1,
+        "ms_per_step": round(dt * 1e3 / epochs, 2), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16 products, fp32 accumulate",
+        "data": "synthetic (seeded inputs, smooth synthetic 128x128x3 targets)",
+        "config": {"workload": f"warm-start decoder 2 -> {net.desc.hidden} -> {N // 8}^2x{net.desc.latent_c} -> 3 conv "
+                               f"(2x upsample) -> {N}x{N}x3, {n} samples, batch {B}", "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / BF16_PEAK_TFLOPS, 4), "traffic": None,
+                     "fwd_flop_per_sample": fwd, "flop_per_epoch": 3.0 * fwd * n,
+                     "note": "latency-bound: 2.6e8 FLOP per sample, batch 32 (8.4 GFLOP per training step)"},
+        "inference": {"samples": 256, "ms": round(dti * 1e3, 2),
+                      "TFLOPs": round(fwd * 256 / dti / 1e12, 2)},
+        "final_loss": loss,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_warmstart(args):
     """Learned warm start: 2048 steady 128² cavities (Re log-uniform [100, 3200], lid
     amplitude A uniform [0.06, 0.12]) from the batched solver train the conv decoder on the
@@ -211,14 +276,13 @@ def bench_warmstart(args):
     N = 128
     Re_tr, A_tr = ws.draw_cases(args.ws_train, seed=1)
     data = ws.generate_steady_states(Re_tr, A_tr, N=N, tol=1e-6, log=log)
+    t0 = time.perf_counter()
     net, loss = ws.train(data["x"], data["y"], epochs=args.ws_epochs, batch=32, lr=2e-3, log=log)
-    t_train = None
-    # BF16 FLOPs of one training sample: forward products ×3 (forward, data and weight gradients)
-    d = net.desc
-    L = d.latent_hw * d.latent_hw * d.latent_c
-    fwd = 2 * (d.in_dim * d.hidden + d.hidden * L)
-    for hi, ci, co in ((d.latent_hw, d.latent_c, d.c1), (2 * d.latent_hw, d.c1, d.c2), (4 * d.latent_hw, d.c2, d.out_c)):
-        fwd += 2 * (2 * hi) ** 2 * 9 * ci * co
+    net.backend.synchronize()
+    t_train = time.perf_counter() - t0
+    fwd = decoder_forward_flop(net.desc)
+    # an epoch runs floor(samples / batch) full batches
+    train_flop = 3.0 * fwd * (len(data["index"]) // 32 * 32) * max(1, args.ws_epochs // 10) * 10
     Re_te, A_te = ws.draw_cases(args.ws_test, seed=99)
     K = args.ws_consistent
     rest = ws.time_to_residual(Re_te, A_te, N=N, tol=1e-7)
@@ -292,14 +356,16 @@ def bench_warmstart(args):
             "median_steps_warm": int(np.median(agree["warm_steps"])), "wall_s": round(agree["wall_s"], 2)},
         "prediction_rel_L2_median": float(np.median(perr)) if perr else None,
         "train": {"samples": int(len(data["index"])), "generated": args.ws_train, "data_gen_s": round(data["seconds"], 1),
-                  "epochs": args.ws_epochs, "final_loss": loss, "fwd_flop_per_sample": fwd},
+                  "epochs": args.ws_epochs, "final_loss": loss, "fwd_flop_per_sample": fwd,
+                  "train_s": round(t_train, 2), "bf16_TFLOPs": round(train_flop / t_train / 1e12, 2),
+                  "frac_of_bf16_peak": round(train_flop / t_train / 1e12 / BF16_PEAK_TFLOPS, 4)},
     }
     print(json.dumps(out), flush=True)
 
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="tgv", choices=["tgv", "cavities", "warmstart", "cavity", "rb"])
+    ap.add_argument("--workload", default="tgv", choices=["tgv", "cavities", "warmstart", "cavity", "rb", "nn"])
     ap.add_argument("--ws-train", type=int, default=2048)
     ap.add_argument("--ws-test", type=int, default=256)
     ap.add_argument("--ws-epochs", type=int, default=150)
@@ -333,6 +399,9 @@ def main():
         return
     if args.workload in ("cavity", "rb"):
         bench_small(args)
+        return
+    if args.workload == "nn":
+        bench_nn(args)
         return
 
     import kraken_mi355x as kmi

@@ -181,22 +181,47 @@ __global__ void k_act_grad(const float* __restrict__ dH, const float* __restrict
 }
 
 // Column sums of X[rows, N] (bias gradients): partial over row chunks, then fixed-order sum.
-__global__ void k_colsum_partial(const float* __restrict__ X, long long rows, int N, int chunk,
-                                 float* __restrict__ part) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
+// Wide rows (N > 128): one thread per column walks the chunk.  Narrow rows: the 256 threads
+// of a group split into G = 256 / N row lanes per column (consecutive threads read
+// consecutive elements), and the G lane sums of a column are added in lane order.
+__global__ void __launch_bounds__(256) k_colsum_partial(const float* __restrict__ X, long long rows,
+                                                        int N, int chunk, float* __restrict__ part) {
   const long long r0 = (long long)blockIdx.y * chunk, r1 = min(rows, r0 + chunk);
+  if (N > 128) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    float s = 0.f;
+    for (long long r = r0; r < r1; ++r) s += X[r * N + n];
+    part[(long long)blockIdx.y * N + n] = s;
+    return;
+  }
+  __shared__ float lds[256];
+  const int G = 256 / N, t = threadIdx.x, n = t % N, g = t / N;
   float s = 0.f;
-  for (long long r = r0; r < r1; ++r) s += X[r * N + n];
-  part[(long long)blockIdx.y * N + n] = s;
+  if (g < G)
+    for (long long r = r0 + g; r < r1; r += G) s += X[r * N + n];
+  lds[t] = s;
+  __syncthreads();
+  if (t < N) {
+    float acc = 0.f;
+    for (int k = 0; k < G; ++k) acc += lds[k * N + t];
+    part[(long long)blockIdx.y * N + t] = acc;
+  }
 }
-__global__ void k_colsum_final(const float* __restrict__ part, int nchunk, int N,
-                               float* __restrict__ out) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
+// One group per column: strided partial sums in fixed order, then a fixed-order tree.
+__global__ void __launch_bounds__(256) k_colsum_final(const float* __restrict__ part, int nchunk, int N,
+                                                      float* __restrict__ out) {
+  __shared__ float lds[256];
+  const int n = blockIdx.x, t = threadIdx.x;
   float s = 0.f;
-  for (int c = 0; c < nchunk; ++c) s += part[(long long)c * N + n];
-  out[n] = s;
+  for (int c = t; c < nchunk; c += 256) s += part[(long long)c * N + n];
+  lds[t] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) lds[t] += lds[t + w];
+    __syncthreads();
+  }
+  if (t == 0) out[n] = lds[0];
 }
 
 // MSE loss partials and dY = 2 (Y − T) / count.
@@ -217,11 +242,12 @@ __global__ void __launch_bounds__(256) k_mse(const float* __restrict__ Y, const 
   __syncthreads();
   if (threadIdx.x == 0) part[blockIdx.x] = (lds[0] + lds[1] + lds[2] + lds[3]) / double(n);
 }
-__global__ void k_sum_parts(const double* __restrict__ part, int n, double* __restrict__ out) {
-  if (threadIdx.x != 0) return;
+// One wave: lane-strided sums in fixed order, then a fixed butterfly.
+__global__ void __launch_bounds__(64) k_sum_parts(const double* __restrict__ part, int n, double* __restrict__ out) {
   double s = 0.0;
-  for (int i = 0; i < n; ++i) s += part[i];
-  out[0] = s;
+  for (int i = threadIdx.x; i < n; i += 64) s += part[i];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (threadIdx.x == 0) out[0] = s;
 }
 
 __global__ void k_to_bf16(const float* __restrict__ x, long long n, uint16_t* __restrict__ y) {
@@ -380,9 +406,9 @@ static int pick_splits(const kmi_nn* N, long long M, long long Nn, long long K) 
 static int colsum(kmi_nn* N, const float* X, long long rows, int n, float* out, hipStream_t s) {
   const int chunk = int(std::max<long long>(256, (rows + 1023) / 1024));
   const int nchunk = int((rows + chunk - 1) / chunk);
-  hipLaunchKernelGGL(k_colsum_partial, dim3(unsigned((n + 255) / 256), unsigned(nchunk)), dim3(256), 0, s,
-                     X, rows, n, chunk, N->colpart);
-  hipLaunchKernelGGL(k_colsum_final, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, N->colpart, nchunk, n, out);
+  hipLaunchKernelGGL(k_colsum_partial, dim3(n > 128 ? unsigned((n + 255) / 256) : 1u, unsigned(nchunk)),
+                     dim3(256), 0, s, X, rows, n, chunk, N->colpart);
+  hipLaunchKernelGGL(k_colsum_final, dim3(unsigned(n)), dim3(256), 0, s, N->colpart, nchunk, n, out);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? KMI_OK : hip_fail(N->ctx, e, "bias gradient");
 }
