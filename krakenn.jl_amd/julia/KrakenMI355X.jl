@@ -128,6 +128,94 @@ end
 set_steps!(l::Lattice, n::Integer) =
     check(ccall((:kmi_lattice_set_steps, libkmi), Cint, (Ptr{Cvoid}, Int64), l.h, n), l.backend.ctx)
 
+# Launch choice for the lattice (results do not depend on it): the autotuner times the
+# single-step shapes and, on fully periodic D3Q19 BGK/TRT lattices, the two- and three-step
+# kernels; set_temporal! fixes a multi-step tile by hand (0: single steps).
+function autotune!(l::Lattice, steps_per_trial::Integer=64)
+    ms = Ref{Float64}(0.0)
+    check(ccall((:kmi_lattice_autotune, libkmi), Cint, (Ptr{Cvoid}, Int64, Ref{Float64}),
+                l.h, steps_per_trial, ms), l.backend.ctx)
+    return ms[]
+end
+set_temporal!(l::Lattice, tile::Integer, zchunks::Integer=1) =
+    check(ccall((:kmi_lattice_set_temporal, libkmi), Cint, (Ptr{Cvoid}, Int32, Int32),
+                l.h, tile, zchunks), l.backend.ctx)
+
+# ---- batched cavities (kmi_batch_*) -------------------------------------------------------
+# Hundreds of square D2Q9 lid-driven cavities of uneven side and viscosity, advanced by one
+# launch per step (or per two steps) and each run to its own residual: the ensemble form of
+# run_cavity_2d / the MRT loop of test/test_mrt.jl:57-63.
+struct KmiBatchDesc
+    storage::Int32; collision::Int32; lid::Int32; pad::Int32
+    u_lid::Float64; s_e::Float64; s_eps::Float64; s_q::Float64; Lambda::Float64
+end
+
+const CAV_RUNNING, CAV_CONVERGED, CAV_DIVERGED, CAV_MAX_STEPS = 0, 1, 2, 3
+
+mutable struct Batch
+    h::Ptr{Cvoid}
+    backend::MI355XBackend
+    n::Vector{Int32}
+end
+
+function Batch(b::MI355XBackend, n::AbstractVector{<:Integer}, ν::AbstractVector{<:Real};
+               storage::Integer=2, collision::Integer=2, u_lid::Real=0.1, zou_he_lid::Bool=true)
+    d = KmiBatchDesc(storage, collision, zou_he_lid ? FACE_ZOUHE_VELOCITY : FACE_MOVING_WALL, 0,
+                     u_lid, 1.4, 1.4, 1.2, 3 / 16)
+    nn, νν = Int32.(collect(n)), Float64.(collect(ν))
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:kmi_batch_create, libkmi), Cint,
+                (Ptr{Cvoid}, Ref{KmiBatchDesc}, Int32, Ptr{Int32}, Ptr{Float64}, Ref{Ptr{Cvoid}}),
+                b.ctx, d, length(nn), nn, νν, r), b.ctx)
+    bt = Batch(r[], b, nn)
+    finalizer(x -> ccall((:kmi_batch_destroy, libkmi), Cint, (Ptr{Cvoid},), x.h), bt)
+    return bt
+end
+
+function autotune!(bt::Batch, steps_per_trial::Integer=8)
+    ms = Ref{Float64}(0.0)
+    check(ccall((:kmi_batch_autotune, libkmi), Cint, (Ptr{Cvoid}, Int64, Ref{Float64}),
+                bt.h, steps_per_trial, ms), bt.backend.ctx)
+    return ms[]
+end
+
+set_temporal!(bt::Batch, steps_per_launch::Integer) =
+    check(ccall((:kmi_batch_set_temporal, libkmi), Cint, (Ptr{Cvoid}, Int32), bt.h, steps_per_launch),
+          bt.backend.ctx)
+
+# Warm start of cavity c (1-based): populations set to feq of the given fields (Nc × Nc).
+init_equilibrium!(bt::Batch, c::Integer, ρ::Matrix{Float64}, ux::Matrix{Float64}, uy::Matrix{Float64}) =
+    check(ccall((:kmi_batch_init_equilibrium, libkmi), Cint,
+                (Ptr{Cvoid}, Int32, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                bt.h, c - 1, ρ, ux, uy), bt.backend.ctx)
+
+# Step every running cavity to r < tol (checked every `check_every` steps, r over the last
+# `window` steps), divergence or max_steps; returns the device milliseconds.
+function run!(bt::Batch, tol::Real; check_every::Integer=100, window::Integer=1,
+              max_steps::Integer=10_000_000)
+    ms = Ref{Float64}(0.0)
+    check(ccall((:kmi_batch_run, libkmi), Cint, (Ptr{Cvoid}, Float64, Int64, Int64, Int64, Ref{Float64}),
+                bt.h, tol, check_every, window, max_steps, ms), bt.backend.ctx)
+    return ms[]
+end
+
+function status(bt::Batch)
+    k = length(bt.n)
+    st, steps, r = zeros(Int32, k), zeros(Int64, k), zeros(Float64, k)
+    check(ccall((:kmi_batch_status, libkmi), Cint, (Ptr{Cvoid}, Ptr{Int32}, Ptr{Int64}, Ptr{Float64}),
+                bt.h, st, steps, r), bt.backend.ctx)
+    return (status=st, steps=steps, residual=r)
+end
+
+function moments(bt::Batch, c::Integer)
+    m = Int(bt.n[c])
+    ρ, ux, uy = zeros(m, m), zeros(m, m), zeros(m, m)
+    check(ccall((:kmi_batch_moments, libkmi), Cint,
+                (Ptr{Cvoid}, Int32, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                bt.h, c - 1, ρ, ux, uy), bt.backend.ctx)
+    return (ρ=ρ, ux=ux, uy=uy)
+end
+
 function desc_2d(Nx, Ny, ν, T; faces)
     w = KmiFace(FACE_WALL, 0, (0.0, 0.0, 0.0), 1.0, 0.0)
     fs = [w, w, w, w, w, w]
