@@ -194,6 +194,10 @@ def bench_small(args):
                                "kind": "port",
                                "sample": f"{n} steps ({el:.1f} s) of the C++ oracle chain, "
                                          f"{'FP64' if dt is np.float64 else 'FP32'}, same configuration"}
+        if args.workload == "cavity":
+            # the reference's own published CPU figure for this case (Julia, one Apple M2 core)
+            out["reference_published_cpu"] = {"value": "26.5-31.9", "unit": "MLUPS",
+                                              "note": "D2Q9 FP64 lid-driven cavity, Kraken.jl CPU backend"}
     print(json.dumps(out), flush=True)
 
 
@@ -498,6 +502,8 @@ def main():
         out["variants"] = var
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(N, nu, args.storage)
+        # the FP64 restatement on the same host cores, for the FP64 variant and reference
+        out["cpu_baseline_f64"] = cpu_baseline(N, nu, "f64", budget_s=6.0, max_steps=10)
     print(json.dumps(out), flush=True)
 
 
