@@ -80,6 +80,23 @@ KMI_DEV void moments(const R (&f)[Q], R& m0, R& rho, R& ux, R& uy, R& uz) {
   else uz = R(0);
 }
 
+// Shifted BGK with deviations in the F16S storage scale (see collide_trt_shifted): bit for bit
+// the scaled result of collide_bgk<Q, true>.
+template <int Q, class R>
+KMI_DEV void collide_bgk_f16u(const R (&fp)[Q], R (&fo)[Q], R om) {
+  const R m0 = zeroth_moment<Q>(fp);
+  const R rho = R(1) + m0 * R(kF16InvScale);
+  const R inv = __builtin_amdgcn_rcpf(rho) * R(kF16InvScale);
+  const R ux = first_moment<Q, 0>(fp) * inv, uy = first_moment<Q, 1>(fp) * inv;
+  const R uz = Q == 19 ? first_moment<Q, 2>(fp) * inv : R(0);
+  R fe[Q];
+  equilibria<Q, true>(m0, rho * R(kF16Scale), ux, uy, uz, fe);
+  sfor<Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    fo[q] = fmad(-om, fp[q] - fe[q], fp[q]);
+  });
+}
+
 template <int Q, bool SH, class R>
 KMI_DEV void collide_bgk(const R (&fp)[Q], R (&fo)[Q], R om) {
   R m0, rho, ux, uy, uz, fe[Q];
@@ -99,13 +116,28 @@ KMI_DEV void collide_bgk(const R (&fp)[Q], R (&fo)[Q], R om) {
 // and 3wρ c_q·u its odd part; rest population d_0* = d_0 − s⁺(d_0 − w_0(δρ − 1.5ρu²)).
 // About 16 % fewer operations per D3Q19 cell than evaluating both equilibria and relaxing
 // each population; the plain storages keep the reference's order below.
-template <int Q, class R>
+//
+// F16U = true: the deviations come in and go out in the F16S storage scale (d · 2^15), so
+// the kernels skip a multiply per population on load, store and on-chip rounding.  Every
+// operation is then the unscaled one times an exact power of two (ρ, u and c·u are formed
+// as before), so the results are the unscaled collision's, scaled, bit for bit.
+template <int Q, bool F16U = false, class R>
 KMI_DEV void collide_trt_shifted(const R (&fp)[Q], R (&fo)[Q], R sp, R sm) {
   R m0, rho, ux, uy, uz;
-  moments<Q, true>(fp, m0, rho, ux, uy, uz);
+  if constexpr (!F16U) {
+    moments<Q, true>(fp, m0, rho, ux, uy, uz);
+  } else {
+    m0 = zeroth_moment<Q>(fp);  // 2^15 δρ
+    rho = R(1) + m0 * R(kF16InvScale);
+    const R inv = __builtin_amdgcn_rcpf(rho) * R(kF16InvScale);
+    ux = first_moment<Q, 0>(fp) * inv;
+    uy = first_moment<Q, 1>(fp) * inv;
+    uz = Q == 19 ? first_moment<Q, 2>(fp) * inv : R(0);
+  }
+  const R rs = F16U ? rho * R(kF16Scale) : rho;  // ρ in the deviations' scale
   const R usq = Q == 9 ? fmad(uy, uy, ux * ux) : fmad(uz, uz, fmad(uy, uy, ux * ux));
-  const R K = fmad(R(-1.5) * rho, usq, m0);  // δρ − 1.5ρu²
-  const R r45 = R(4.5) * rho;
+  const R K = fmad(R(-1.5) * rs, usq, m0);  // δρ − 1.5ρu²
+  const R r45 = R(4.5) * rs;
   const R hp = R(0.5) * sp, hm = R(0.5) * sm;
   fo[0] = fmad(-sp, fp[0] - R(Lat<Q>::w[0]) * K, fp[0]);
   sfor<Q>([&](auto qc) {
@@ -116,7 +148,7 @@ KMI_DEV void collide_trt_shifted(const R (&fp)[Q], R (&fo)[Q], R sp, R sm) {
       const R cu = cdotu<Lat<Q>::cx[q], Lat<Q>::cy[q], Lat<Q>::cz[q]>(ux, uy, uz);
       const R E = fmad(r45, cu * cu, K);
       const R pe = fmad(hp, fp[q] + fp[o], -(R(w) * sp) * E);
-      const R po = fmad(hm, fp[q] - fp[o], -((R(3.0 * w) * sm) * rho) * cu);
+      const R po = fmad(hm, fp[q] - fp[o], -((R(3.0 * w) * sm) * rs) * cu);
       fo[q] = fp[q] - (pe + po);
       fo[o] = fp[o] - (pe - po);
     }

@@ -59,11 +59,50 @@ KMI_DEV E* at(E* plane, unsigned off) {
 }
 }  // namespace tb
 
+// Population access of the multi-step kernels.  F16S keeps the populations in the storage
+// scale (d · 2^15) from load to store: the load and the on-chip rounding skip the scale
+// multiplies and the collisions run scaled (collide_trt_shifted<.., true>,
+// collide_bgk_f16u), which reproduces the single-step kernel's values bit for bit.
+template <int ST>
+struct Pops {
+  using S = Store<ST>;
+  using E = typename S::elem;
+  using R = typename S::real;
+  static constexpr bool kScaled = ST == KMI_STORE_F16S;
+  KMI_DEV static R ld(const E* p) {
+    if constexpr (kScaled) return float(*p);
+    else return S::ld(p, 0);
+  }
+  // The collision's last operation is often a fused multiply-add; converting its FP32
+  // result straight to binary16 would let the compiler merge the two into one mixed-precision
+  // FMA (v_fma_mix, a single rounding to binary16) and change the stored value.  The empty
+  // register constraint keeps the FP32 rounding in between, as in the unscaled path.
+  KMI_DEV static _Float16 to_half(R v) {
+    asm volatile("" : "+v"(v));
+    return _Float16(v);
+  }
+  KMI_DEV static R round(R v) {
+    if constexpr (kScaled) return float(to_half(v));
+    else return storage_round<ST>(v);
+  }
+  template <bool NT>
+  KMI_DEV static void st(E* p, R v) {
+    if constexpr (kScaled) store_elem(p, 0, to_half(v), NT);
+    else S::template st<NT>(p, 0, v);
+  }
+};
+
 template <int ST, int COLL, class R>
 KMI_DEV void collide19(const R (&fp)[19], R (&fo)[19], const StepArgs& a) {
   constexpr bool SH = Store<ST>::shifted;
-  if constexpr (COLL == KMI_TRT) collide_trt<19, SH>(fp, fo, R(a.s_plus), R(a.s_minus));
-  else collide_bgk<19, SH>(fp, fo, R(a.omega));
+  if constexpr (ST == KMI_STORE_F16S) {
+    if constexpr (COLL == KMI_TRT) collide_trt_shifted<19, true>(fp, fo, R(a.s_plus), R(a.s_minus));
+    else collide_bgk_f16u<19>(fp, fo, R(a.omega));
+  } else if constexpr (COLL == KMI_TRT) {
+    collide_trt<19, SH>(fp, fo, R(a.s_plus), R(a.s_minus));
+  } else {
+    collide_bgk<19, SH>(fp, fo, R(a.omega));
+  }
 }
 
 // Grid: one work-group per (column tile, z-range), padded to a multiple of 8.  Work-group
@@ -118,7 +157,7 @@ __global__ void __launch_bounds__(TX * TY)
           constexpr int q = decltype(qc)::value;
           constexpr int cx = Lat<19>::cx[q], cy = Lat<19>::cy[q], cz = Lat<19>::cz[q];
           const unsigned row = unsigned(nx) * unsigned(yy[1 - cy] + ny * zz[1 - cz]) * ES;
-          nxt[r][q] = S::ld(&tb::at(fi + q * P, row + bx[1 - cx]), 0);
+          nxt[r][q] = Pops<ST>::ld(&tb::at(fi + q * P, row + bx[1 - cx]));
         });
       }
     }
@@ -147,7 +186,7 @@ __global__ void __launch_bounds__(TX * TY)
           constexpr int q = decltype(qc)::value;
           constexpr int cz = Lat<19>::cz[q];
           constexpr int n = tb::group_index(q);
-          const R v = storage_round<ST>(fn[q]);
+          const R v = Pops<ST>::round(fn[q]);
           if constexpr (cz == 0) keep0[r][n] = v;
           else if constexpr (cz > 0) keepp[r][n] = v;
           else C[n * NE + e] = v;
@@ -173,7 +212,7 @@ __global__ void __launch_bounds__(TX * TY)
       const unsigned cell = (unsigned(x0 + ix) + unsigned(nx) * unsigned(y0 + iy + ny * (s - 1))) * ES;
       sfor<19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        S::template st<NTS>(tb::at(fo + q * P, cell), 0, fn[q]);
+        Pops<ST>::template st<NTS>(tb::at(fo + q * P, cell), fn[q]);
       });
     }
     __syncthreads();
@@ -261,7 +300,7 @@ __global__ void __launch_bounds__(TX * TY)
           constexpr int q = decltype(qc)::value;
           constexpr int cx = Lat<19>::cx[q], cy = Lat<19>::cy[q], cz = Lat<19>::cz[q];
           const unsigned row = unsigned(nx) * unsigned(yy[1 - cy] + ny * zz[1 - cz]) * ES;
-          nxt[r][q] = S::ld(&tb::at(fi + q * P, row + bx[1 - cx]), 0);
+          nxt[r][q] = Pops<ST>::ld(&tb::at(fi + q * P, row + bx[1 - cx]));
         });
       }
     }
@@ -290,7 +329,7 @@ __global__ void __launch_bounds__(TX * TY)
           constexpr int q = decltype(qc)::value;
           constexpr int cz = Lat<19>::cz[q];
           constexpr int n = tb::group_index(q);
-          const R v = storage_round<ST>(fn[q]);
+          const R v = Pops<ST>::round(fn[q]);
           if constexpr (cz == 0) k1z[r][n] = v;
           else if constexpr (cz > 0) k1p[r][n] = v;
           else C1[n * N1 + e] = v;
@@ -322,7 +361,7 @@ __global__ void __launch_bounds__(TX * TY)
             constexpr int q = decltype(qc)::value;
             constexpr int cz = Lat<19>::cz[q];
             constexpr int n = tb::group_index(q);
-            const R v = storage_round<ST>(fn[q]);
+            const R v = Pops<ST>::round(fn[q]);
             if constexpr (cz == 0) k2z[r][n] = v;
             else if constexpr (cz > 0) k2p[r][n] = v;
             else C2[n * N2 + e] = v;
@@ -349,7 +388,7 @@ __global__ void __launch_bounds__(TX * TY)
       const unsigned cell = (unsigned(x0 + ix) + unsigned(nx) * unsigned(y0 + iy + ny * (s - 2))) * ES;
       sfor<19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        S::template st<NTS>(tb::at(fo + q * P, cell), 0, fn[q]);
+        Pops<ST>::template st<NTS>(tb::at(fo + q * P, cell), fn[q]);
       });
     }
     {
