@@ -101,6 +101,8 @@ def run_cavity_2d(config: LBMConfig, *, backend=None, T=np.float64, collision="b
     d = cavity_desc(config.Nx, config.nu, config.u_lid, storage=_storage(T), collision=collision,
                     lid=lid, Ny=config.Ny)
     with Lattice(d, _backend(backend)) as lat:
+        if config.max_steps >= 2000:
+            lat.autotune(64)  # launch choice only: results are identical for every choice
         lat.step(config.max_steps)
         m = lat.moments()
     return {"ρ": m["rho"], "ux": m["ux"], "uy": m["uy"], "config": config}

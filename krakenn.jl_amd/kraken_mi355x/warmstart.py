@@ -145,8 +145,8 @@ def converged_agreement(Re, A, net, *, N=128, tol=1e-11, check=100, window=100, 
 
 def plain_time_to_residual(Re, A, *, N=128, tol=1e-7, check=100, window=100, max_steps=1_000_000,
                            storage="f32s", backend=None):
-    """Rest-initialised plain runs: one lattice at a time (the single-cavity path, one launch
-    per step), each run to the batch's stop rule r = ‖u^k − u^(k−m)‖/(m‖u^k‖) < tol checked
+    """Rest-initialised plain runs: one lattice at a time (the single-cavity path with its
+    autotuned launch), each run to the batch's stop rule r = ‖u^k − u^(k−m)‖/(m‖u^k‖) < tol checked
     every `check` steps (m = `window`).  Returns per-case wall seconds, steps and status
     (1 converged, 2 diverged, 3 max_steps)."""
     from .drivers import cavity_desc
@@ -161,6 +161,7 @@ def plain_time_to_residual(Re, A, *, N=128, tol=1e-7, check=100, window=100, max
         t0 = time.perf_counter()
         n, st = 0, 3
         with Lattice(d, backend) as lat:
+            lat.autotune(64)  # the fastest single-lattice launch (several steps per launch here)
             while n < max_steps:
                 lat.step(check - window)
                 m0 = lat.moments()

@@ -358,3 +358,79 @@ def test_two_step_tuned_and_rejected_where_it_does_not_apply(kmi, backend):
         with pytest.raises(kmi.KrakenError) as e:
             lat.set_temporal(1, 1)
         assert e.value.status == 5
+
+
+def _tb2d_cases(kmi, storage):
+    """D2Q9 lattices in the multi-step kernel's scope: (name, desc, shape, mask)."""
+    N, M = 70, 45
+    yield "cavity_zh_bgk", kmi.cavity_desc(N, 0.05, 0.1, storage=storage, Ny=M), (N, M), None
+    yield "cavity_ladd_trt_mask", kmi.cavity_desc(N, 0.04, 0.1, storage=storage, collision="trt", lid="ladd",
+                                                  Ny=M), (N, M), _mask_blob((N, M), (20, 18), 4)
+    yield "cavity_mrt", kmi.cavity_desc(N, 0.02, 0.1, storage=storage, collision="mrt", Ny=M), (N, M), None
+    per = {"west": {"kind": "periodic"}, "east": {"kind": "periodic"}}
+    yield "channel_guo", kmi.make_desc("d2q9", N, M, 1, 0.05, storage=storage, faces=per, forcing="guo",
+                                       force=(1e-5, 0.0, 0.0)), (N, M), _mask_blob((N, M), (25, 21), 5)
+    zh = {"west": {"kind": "zouhe_velocity", "u": (0.04, 0.0, 0.0)}, "east": {"kind": "zouhe_pressure", "rho": 1.0}}
+    yield "channel_zh_inert", kmi.make_desc("d2q9", N, M, 1, 0.05, storage=storage, faces=zh,
+                                            solid_mode="inert"), (N, M), _mask_blob((N, M), (18, 20), 4)
+    full = {f: {"kind": "periodic"} for f in ("west", "east", "south", "north")}
+    yield "periodic_trt", kmi.make_desc("d2q9", N, M, 1, 0.02, storage=storage, collision="trt", faces=full), \
+        (N, M), None
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32", "f32s", "f16s"])
+def test_d2q9_multi_step_launches_bit_identical(kmi, backend, storage):
+    """K steps per launch on D2Q9 lattices (tile plus a K-cell ring in LDS) reproduce single
+    steps bit for bit for every variant: walls, Ladd and Zou–He lids, Zou–He inlet/outlet,
+    periodic axes, masks (bounce-back and inert), Guo forcing, BGK / TRT / MRT, and lattice
+    sizes that are not multiples of the tile; step counts leave remainders."""
+    steps = 37
+    for name, desc, shape, mask in _tb2d_cases(kmi, storage):
+        f = rng_pops(shape, 9, seed=8, amp=0.02)
+        print(f"[tb2d] {storage} {name} shape={shape} steps={steps}", flush=True)
+        with kmi.Lattice(desc, backend) as lat:
+            if mask is not None:
+                lat.set_mask(mask)
+            lat.upload(f)
+            lat.step(steps)
+            ref = lat.download()
+        for tile in (1, 2, 3, 4, 5):
+            with kmi.Lattice(desc, backend) as lat:
+                if mask is not None:
+                    lat.set_mask(mask)
+                lat.upload(f)
+                lat.set_temporal(tile, 1)
+                lat.step(steps)
+                assert lat.steps == steps
+                got = lat.download()
+            assert np.array_equal(got, ref), (name, tile, float(np.max(np.abs(got - ref))))
+
+
+def test_d2q9_multi_step_rejected_out_of_scope(kmi, backend):
+    # thermal populations, per-node faces and lattices smaller than tile + ring stay single-step
+    d, _ = kmi.rayleigh_benard_desc(64, 32, 1e4, 0.71)
+    with kmi.Lattice(d, backend) as lat:
+        with pytest.raises(kmi.KrakenError):
+            lat.set_temporal(1, 1)
+    with kmi.Lattice(kmi.cavity_desc(40, 0.05, 0.1), backend) as lat:
+        lat.set_temporal(1, 1)  # 40 >= 28 + 4
+        with pytest.raises(kmi.KrakenError):
+            lat.set_temporal(3, 1)  # 40 < 56 + 8
+    prof = {"west": {"kind": "zouhe_velocity_profile"}}
+    with kmi.Lattice(kmi.make_desc("d2q9", 40, 40, 1, 0.05, faces=prof), backend) as lat:
+        with pytest.raises(kmi.KrakenError):
+            lat.set_temporal(1, 1)
+
+
+def test_d2q9_multi_step_autotuned_cavity_matches(kmi, backend):
+    """The autotuner may pick a multi-step variant for a 128² cavity; the run is unchanged."""
+    d = kmi.cavity_desc(128, 0.128, 0.1)
+    with kmi.Lattice(d, backend) as lat:
+        lat.step(10)
+        lat.autotune(64)
+        print(f"[tb2d] cavity 128^2 tuned temporal {lat.temporal}", flush=True)
+        lat.step(290)
+        a = lat.download()
+    with kmi.Lattice(d, backend) as lat:
+        lat.step(300)
+        assert np.array_equal(lat.download(), a)
