@@ -281,9 +281,11 @@ class Lattice:
         return {"bx": bx.value, "by": by.value, "nontemporal": bool(nt.value)}
 
     def set_temporal(self, tile, zchunks=1):
-        """Temporal blocking on a fully periodic D3Q19 BGK/TRT lattice: two steps per launch
+        """Temporal blocking.  Fully periodic D3Q19 BGK/TRT lattice: two steps per launch
         (tile 1: 32×8, 2: 64×4, 3: 64×8, 4: 32×16, 5: 64×16 cells) or three (6: 64×8, 7: 32×8,
-        8: 32×16; FP64 only 7); 0: off.  Bit-identical to single steps."""
+        8: 32×16; FP64 only 7).  D2Q9 lattice without thermal populations, cut links or
+        per-node faces: 1: 28×28 two steps, 2: 24×24 four, 3: 56×8 four, 4: 16×16 eight,
+        5: 60×12 two.  0: off.  Bit-identical to single steps."""
         self._chk(L.load().kmi_lattice_set_temporal(self._h, int(tile), int(zchunks)))
 
     @property
