@@ -410,9 +410,11 @@ int kmi_batch_status(const kmi_batch* b, int32_t* status, int64_t* steps, double
 int kmi_batch_set_launch(kmi_batch* b, int32_t tw, int32_t th, int32_t groups, int32_t nontemporal);
 int kmi_batch_get_launch(const kmi_batch* b, int32_t* tw, int32_t* th, int32_t* groups,
                          int32_t* nontemporal);
-/* Steps per launch: 1, or 2 (the tile's step t+1, with a one-cell ring, kept in LDS; HBM
- * sees one read and one write per population per two steps).  Bit-identical to single
- * steps; odd step counts end with a single step.  Chosen by kmi_batch_autotune. */
+/* Steps per launch: 1; 2 (the tile's step t+1, with a one-cell ring, kept in LDS; HBM
+ * sees one read and one write per population per two steps); 3 or 4 (the tile and a
+ * 3- or 4-cell ring advanced in LDS, one work-item per cell of that region; tiles of width
+ * 16, 32 or 64).  Bit-identical to single steps; step counts that are not a multiple end
+ * with two-step and single-step launches.  Chosen by kmi_batch_autotune. */
 int kmi_batch_set_temporal(kmi_batch* b, int32_t steps_per_launch);
 int kmi_batch_get_temporal(const kmi_batch* b, int32_t* steps_per_launch);
 /* Time every candidate tile shape × group count × store policy and keep the fastest;

@@ -180,6 +180,75 @@ __global__ void __launch_bounds__(256) k_batch_step2(const BatchArgs a) {
   }
 }
 
+// K steps per launch (K = 3 or 4) over the same tile slots: the tile and a K-cell ring are
+// loaded into LDS with one work-item per cell of that region ((tw+2K)(th+2K) ≤ 1024), then
+// advanced K steps in place (step s updates the region less s cells on each side: read,
+// barrier, write back in the storage rounding, barrier); the tile's t+K populations go to
+// HBM.  Bit-identical to K single steps.  Dynamic LDS: 9 · (tw+2K)(th+2K) reals.
+template <int ST, int COLL, int K>
+__global__ void __launch_bounds__(1024) k_batch_stepK(const BatchArgs a) {
+  using S = Store<ST>;
+  using E = typename S::elem;
+  using R = typename S::real;
+  R* const L = reinterpret_cast<R*>(kmi_batch_lds);
+  const int tw = a.tw, th = 256 / a.tw;
+  const int ew = tw + 2 * K, eh = th + 2 * K, ne = ew * eh;
+  const int e = threadIdx.x;  // one work-item per region cell (blockDim.x == ne)
+  const int ex = e % ew, ey = e / ew;
+  for (int t = blockIdx.x; t < a.nslots; t += gridDim.x) {
+    const int c = a.slots[3 * t];  // uniform over the work-group
+    if (c < 0) continue;
+    const CavRec cv = a.cav[c];
+    const int n = cv.n;
+    const int i = a.slots[3 * t + 1] - K + ex, j = a.slots[3 * t + 2] - K + ey;
+    const bool in = i >= 0 && j >= 0 && i < n && j < n;
+    const unsigned P = unsigned(n) * unsigned(n);
+    const E* __restrict__ src = static_cast<const E*>(a.fi) + cv.foff;
+    E* __restrict__ dst = static_cast<E*>(a.fo) + cv.foff;
+    if (in) {
+      const unsigned self = unsigned(i + n * j);
+      sfor<9>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        L[q * ne + e] = S::ld(src, self + unsigned(q) * P);
+      });
+    }
+    __syncthreads();
+    R fr[9];
+    for (int s = 1; s <= K; ++s) {
+      const bool valid = in && ex >= s && ex < ew - s && ey >= s && ey < eh - s;
+      if (valid) {
+        R fp[9];
+        cavity_pull(a, n, i, j, unsigned(e), unsigned(ew), unsigned(ne), fp,
+                    [&](unsigned off) { return L[off]; });
+        cavity_collide<ST, COLL>(a, cv, n, j, fp, fr);
+      }
+      if (s == K) break;
+      __syncthreads();  // every step-s read of L is done
+      if (valid) {
+        sfor<9>([&](auto qc) {
+          constexpr int q = decltype(qc)::value;
+          L[q * ne + e] = storage_round<ST>(fr[q]);
+        });
+      }
+      __syncthreads();
+    }
+    if (in && ex >= K && ex < K + tw && ey >= K && ey < K + th) {
+      const unsigned self = unsigned(i + n * j);
+      sfor<9>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        S::template st<false>(dst, self + unsigned(q) * P, fr[q]);
+      });
+    }
+    __syncthreads();  // the next slot's load overwrites LDS
+  }
+}
+
+// Work-items of a K-step region for tile width tw (0: the region exceeds 1024).
+KMI_HD inline int stepK_region(int tw, int K) {
+  const int r = (tw + 2 * K) * (256 / tw + 2 * K);
+  return r <= 1024 ? r : 0;
+}
+
 // Velocity of a cell from the stored populations, FP64 (compute_macroscopic_2d order).
 template <int ST>
 KMI_DEV void cell_u(const void* f, const CavRec& cv, int self, double& ux, double& uy) {
@@ -356,6 +425,28 @@ static hipError_t launch_batch_step2(int st, int coll, bool nts, const BatchArgs
   return hipGetLastError();
 }
 
+static hipError_t launch_batch_stepK(int st, int coll, int K, const BatchArgs& a, int groups, hipStream_t s) {
+  const int ne = stepK_region(a.tw, K);
+  if (ne == 0) return hipErrorInvalidValue;
+  const dim3 g(static_cast<unsigned>(groups)), b(static_cast<unsigned>(ne));
+#define KMI_BATCH_COLLK(ST)                                                                      \
+  {                                                                                             \
+    const size_t sh = size_t(9) * ne * sizeof(typename Store<ST>::real);                        \
+    if (K == 3) {                                                                               \
+      if (coll == KMI_BGK) hipLaunchKernelGGL((k_batch_stepK<ST, KMI_BGK, 3>), g, b, sh, s, a); \
+      else if (coll == KMI_TRT) hipLaunchKernelGGL((k_batch_stepK<ST, KMI_TRT, 3>), g, b, sh, s, a); \
+      else hipLaunchKernelGGL((k_batch_stepK<ST, KMI_MRT, 3>), g, b, sh, s, a);                 \
+    } else {                                                                                    \
+      if (coll == KMI_BGK) hipLaunchKernelGGL((k_batch_stepK<ST, KMI_BGK, 4>), g, b, sh, s, a); \
+      else if (coll == KMI_TRT) hipLaunchKernelGGL((k_batch_stepK<ST, KMI_TRT, 4>), g, b, sh, s, a); \
+      else hipLaunchKernelGGL((k_batch_stepK<ST, KMI_MRT, 4>), g, b, sh, s, a);                 \
+    }                                                                                           \
+  }
+  KMI_BATCH_ST(st, KMI_BATCH_COLLK(ST));
+#undef KMI_BATCH_COLLK
+  return hipGetLastError();
+}
+
 static hipError_t launch_batch_resid(int st, const BatchArgs& a, const void* f, int mode,
                                      int groups, hipStream_t s) {
   KMI_BATCH_ST(st, hipLaunchKernelGGL((k_batch_resid<ST>), dim3(unsigned(groups)), dim3(256), 0, s, a, f, mode));
@@ -386,7 +477,7 @@ struct kmi_batch {
   int cur = 0;
   // tiles
   int tw = 64, th = 4, groups = 0, nts = 0;
-  int span = 1;  // steps per launch: 1, or 2 (k_batch_step2)
+  int span = 1;  // steps per launch: 1, 2 (k_batch_step2), 3 or 4 (k_batch_stepK)
   bool tiles_dirty = true;
   int nslots = 0;
   int* slots_d = nullptr;
@@ -518,9 +609,20 @@ static int batch_launch_step2(kmi_batch* B, hipStream_t s) {
   return KMI_OK;
 }
 
-// n steps: two-step launches when enabled, single steps for the rest.
+static int batch_launch_stepK(kmi_batch* B, hipStream_t s) {
+  BatchArgs a = batch_args(B);
+  hipError_t e = launch_batch_stepK(B->desc.storage, B->desc.collision, B->span, a, batch_groups(B), s);
+  if (e != hipSuccess) return hip_fail(B->ctx, e, "batch multi-step launch");
+  B->cur ^= 1;
+  return KMI_OK;
+}
+
+// n steps: multi-step launches when enabled, then two-step and single steps for the rest.
 static int batch_launch_n(kmi_batch* B, int64_t n, hipStream_t s) {
-  if (B->span == 2)
+  if (B->span >= 3)
+    for (; n >= B->span; n -= B->span)
+      if (int rc = batch_launch_stepK(B, s)) return rc;
+  if (B->span >= 2)
     for (; n >= 2; n -= 2)
       if (int rc = batch_launch_step2(B, s)) return rc;
   for (; n > 0; --n)
@@ -907,6 +1009,8 @@ int kmi_batch_set_launch(kmi_batch* B, int32_t tw, int32_t th, int32_t groups, i
   if (!B) return fail(nullptr, KMI_ERR_ARG, "batch is NULL");
   if (!((tw == 16 || tw == 32 || tw == 64 || tw == 128 || tw == 256) && tw * th == 256) || groups < 0)
     return fail(B->ctx, KMI_ERR_ARG, "kmi_batch_set_launch: tiles of tw x th = 256 cells, tw in {16,32,64,128,256}");
+  if (B->span >= 3 && stepK_region(tw, B->span) == 0)
+    return fail(B->ctx, KMI_ERR_ARG, "kmi_batch_set_launch: tile too wide for the current steps per launch");
   KMI_HIP(B->ctx, hipSetDevice(B->ctx->device));
   KMI_HIP(B->ctx, hipStreamSynchronize(B->ctx->stream));
   B->tw = tw;
@@ -920,8 +1024,11 @@ int kmi_batch_set_launch(kmi_batch* B, int32_t tw, int32_t th, int32_t groups, i
 
 int kmi_batch_set_temporal(kmi_batch* B, int32_t steps_per_launch) {
   if (!B) return fail(nullptr, KMI_ERR_ARG, "batch is NULL");
-  if (steps_per_launch != 1 && steps_per_launch != 2)
-    return fail(B->ctx, KMI_ERR_ARG, "kmi_batch_set_temporal: steps per launch must be 1 or 2");
+  if (steps_per_launch < 1 || steps_per_launch > 4)
+    return fail(B->ctx, KMI_ERR_ARG, "kmi_batch_set_temporal: steps per launch must be 1, 2, 3 or 4");
+  if (steps_per_launch >= 3 && stepK_region(B->tw, steps_per_launch) == 0)
+    return fail(B->ctx, KMI_ERR_ARG,
+                "kmi_batch_set_temporal: 3 or 4 steps per launch need tiles of width 16, 32 or 64");
   KMI_HIP(B->ctx, hipSetDevice(B->ctx->device));
   KMI_HIP(B->ctx, hipStreamSynchronize(B->ctx->stream));
   B->span = steps_per_launch;
@@ -951,7 +1058,7 @@ int kmi_batch_autotune(kmi_batch* B, int64_t steps_per_trial, double* best_ms) {
   KMI_HIP(x, hipStreamSynchronize(x->stream));
   if (batch_active(B) == 0) return fail(x, KMI_ERR_STATE, "kmi_batch_autotune: no running cavity");
   int64_t trial = std::max<int64_t>(2, steps_per_trial);
-  trial += trial & 1;  // even, so two-step variants time whole launches
+  trial = (trial + 11) / 12 * 12;  // whole launches of every span (1, 2, 3, 4)
   const size_t bytes = size_t(B->total_pop) * B->esize;
   void* save = nullptr;
   KMI_HIP(x, hipMalloc(&save, bytes));
@@ -963,12 +1070,13 @@ int kmi_batch_autotune(kmi_batch* B, int64_t steps_per_trial, double* best_ms) {
   int bb[5] = {B->tw, B->th, B->groups, B->nts, B->span};
   for (const auto& sh : shapes) {
     for (int g : group_opts) {
-      for (int nt = 0; nt < 4 && e == hipSuccess; ++nt) {
+      for (int nt = 0; nt < 8 && e == hipSuccess; ++nt) {
         B->tw = sh[0];
         B->th = sh[1];
         B->groups = g;
         B->nts = nt & 1;
         B->span = 1 + (nt >> 1);
+        if (B->span >= 3 && (B->nts || stepK_region(B->tw, B->span) == 0)) continue;  // K-step stores are plain
         if (batch_build_tiles(B) != KMI_OK) { e = hipErrorUnknown; break; }
         if (g > 0 && g >= B->nslots) continue;  // same as one group per slot
         if (batch_launch_n(B, B->span, x->stream) != KMI_OK) { e = hipErrorUnknown; break; }
@@ -988,7 +1096,7 @@ int kmi_batch_autotune(kmi_batch* B, int64_t steps_per_trial, double* best_ms) {
           bb[1] = sh[1];
           bb[2] = g;
           bb[3] = nt & 1;
-          bb[4] = 1 + (nt >> 1);
+          bb[4] = B->span;
         }
       }
     }

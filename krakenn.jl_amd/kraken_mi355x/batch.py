@@ -145,7 +145,8 @@ class Batch:
 
     def set_temporal(self, steps_per_launch):
         """1: one step per launch; 2: two steps per launch (step t+1 of each tile and its ring
-        kept in LDS), bit-identical to single steps."""
+        kept in LDS); 3 or 4: the tile and a 3- or 4-cell ring advanced in LDS (tile width 16,
+        32 or 64).  Bit-identical to single steps."""
         self._chk(L.load().kmi_batch_set_temporal(self._h, int(steps_per_launch)))
 
     def autotune(self, steps_per_trial=8):

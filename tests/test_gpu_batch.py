@@ -193,17 +193,18 @@ def test_batch_two_step_launches_bit_identical(kmi, backend, storage, collision,
             b.upload(c, f0[c])
         b.step(37)
         ref = [b.download(c) for c in range(len(SIZES))]
-    for tw, th, groups in ((64, 4, 0), (32, 8, 16), (16, 16, 0), (256, 1, 0), (128, 2, 8)):
+    for tw, th, groups, span in ((64, 4, 0, 2), (32, 8, 16, 2), (16, 16, 0, 2), (256, 1, 0, 2), (128, 2, 8, 2),
+                                 (32, 8, 0, 3), (64, 4, 16, 3), (16, 16, 0, 4), (32, 8, 8, 4)):
         with kmi.Batch(SIZES, NUS, storage=storage, collision=collision, lid=lid, backend=backend) as b:
             b.set_launch(tw, th, groups)
-            b.set_temporal(2)
-            assert b.launch["steps_per_launch"] == 2
+            b.set_temporal(span)
+            assert b.launch["steps_per_launch"] == span
             for c in range(len(SIZES)):
                 b.upload(c, f0[c])
             b.step(37)
             assert np.all(b.status()["steps"] == 37)
             for c in range(len(SIZES)):
-                assert np.array_equal(b.download(c), ref[c]), (tw, th, groups, c)
+                assert np.array_equal(b.download(c), ref[c]), (tw, th, groups, span, c)
 
 
 def test_batch_two_step_run_to_residual(kmi, backend):
@@ -211,14 +212,14 @@ def test_batch_two_step_run_to_residual(kmi, backend):
     the single-step run's steps, residuals and fields."""
     sizes, nus = [16, 24, 20, 32], [0.1, 0.08, 0.05, 0.12]
     out = []
-    for span in (1, 2):
+    for span in (1, 2, 3):
         for check, window in ((50, 50), (51, 7)):
             with kmi.Batch(sizes, nus, storage="f32s", collision="mrt", backend=backend) as b:
                 b.set_temporal(span)
                 b.run(1e-6, check, 60_000, window=window)
                 st = b.status()
                 out.append((span, check, st, [b.download(c) for c in range(len(sizes))]))
-    for span, check, st, f in out[2:]:
+    for span, check, st, f in out[2:]:  # spans 2 and 3 against span 1
         ref = next(o for o in out[:2] if o[1] == check)
         assert np.array_equal(st["steps"], ref[2]["steps"]) and np.array_equal(st["residual"], ref[2]["residual"])
         assert np.array_equal(st["status"], ref[2]["status"])
@@ -233,10 +234,14 @@ def test_batch_autotune_considers_two_step(kmi, backend):
         before = [b.download(c) for c in range(len(sizes))]
         b.autotune(4)
         print(f"[batch two-step] tuned {b.launch}", flush=True)
-        assert b.launch["steps_per_launch"] in (1, 2)
+        assert b.launch["steps_per_launch"] in (1, 2, 3, 4)
         assert all(np.array_equal(b.download(c), before[c]) for c in range(len(sizes)))
         with pytest.raises(kmi.KrakenError):
-            b.set_temporal(3)
+            b.set_temporal(5)
+        b.set_temporal(1)
+        b.set_launch(256, 1)
+        with pytest.raises(kmi.KrakenError):
+            b.set_temporal(3)  # a 262 x 7 region exceeds one work-group
 
 
 def test_batch_retired_cavity_keeps_written_state(kmi, po, backend):
