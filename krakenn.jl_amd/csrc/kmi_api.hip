@@ -393,12 +393,13 @@ static void default_geometry(kmi_lattice* L) {
 
 // Whether multi-step launches with `tile` apply.  D3Q19: fully periodic BGK/TRT, no mask,
 // the x–y extent a multiple of the tile, and at least 4 planes in z.  D2Q9 (k_step_tb2d):
-// no thermal populations, no cut links, faces periodic / wall / moving wall / constant
-// Zou–He only, and a lattice of at least the tile plus its ring on each axis.
+// no cut links, faces periodic / wall / moving wall / constant Zou–He only (thermal lattices
+// included), and a lattice of at least the tile plus its ring on each axis.
 static bool tb_eligible(const kmi_lattice* L, int tile) {
   if (L->Q == 9) {
     int TW, TH, K;
-    if (!tb2d_variant(tile, TW, TH, K) || L->desc.thermal || !L->cut_cells.empty()) return false;
+    if (!tb2d_variant(tile, TW, TH, K) || !L->cut_cells.empty()) return false;
+    if (L->desc.thermal && L->desc.storage == KMI_STORE_F64) return false;  // see k_step_tb2d
     for (int f = 0; f < 4; ++f)
       if (L->desc.faces[f].kind > KMI_FACE_ZOUHE_PRESSURE) return false;
     return L->desc.nx >= TW + 2 * K && L->desc.ny >= TH + 2 * K;
@@ -435,7 +436,8 @@ static int launch_one(kmi_lattice* L, hipStream_t s, bool multi = false) {
   a.go = L->g[L->cur ^ 1];
   a.mask = L->mask;
   hipError_t e =
-      multi ? (L->Q == 9 ? launch_step_tb2d(L->desc.storage, L->desc.collision, L->tb_tile, a, s)
+      multi ? (L->Q == 9 ? launch_step_tb2d(L->desc.storage, L->desc.collision, L->tb_tile,
+                                            L->desc.thermal != 0, a, s)
                          : launch_step2(L->desc.storage, L->desc.collision, L->tb_tile, L->tb_zchunks,
                                         L->nts != 0, a, s))
             : launch_step(L->Q, L->desc.storage, L->desc.collision, L->desc.thermal != 0, L->bnd,
@@ -1146,7 +1148,7 @@ int kmi_lattice_set_temporal(kmi_lattice* L, int32_t tile, int32_t zchunks) {
                 "BGK/TRT lattice without mask, nz >= 4, whose nx, ny are multiples of the tile "
                 "(two steps 1: 32x8, 2: 64x4, 3: 64x8, 4: 32x16, 5: 64x16 (not FP64); "
                 "three steps 6: 64x8 (not FP64), 7: 32x8, 8: 32x16 (not FP64)), or a D2Q9 "
-                "lattice without thermal populations, cut links or per-node faces, at least "
+                "lattice without cut links or per-node faces (thermal: not FP64), at least "
                 "the tile plus its ring on each axis (1: 28x28 2 steps, 2: 24x24 4 steps, "
                 "3: 56x8 4 steps, 4: 16x16 8 steps, 5: 60x12 2 steps)");
   if (tile != 0 && (zchunks < 1 || zchunks > L->args.nz))

@@ -32,7 +32,7 @@ hipError_t launch_step2(int st, int coll, int tile, int zchunks, bool nts, const
 // K steps per launch on a D2Q9 lattice (kmi_step_tb2d.hip): `tile` selects tile and K
 // (tb2d_variant).
 bool tb2d_variant(int tile, int& TW, int& TH, int& K);
-hipError_t launch_step_tb2d(int st, int coll, int tile, const StepArgs& a, hipStream_t s);
+hipError_t launch_step_tb2d(int st, int coll, int tile, bool th, const StepArgs& a, hipStream_t s);
 hipError_t launch_copy_probe(const void* src, void* dst, long long bytes, hipStream_t s);
 hipError_t launch_to_storage(int Q, int st, const double* src, void* dst, long long ncells,
                              long long plane, double shift, hipStream_t s);

@@ -11,8 +11,9 @@
 //
 // Scope: D2Q9 BGK / TRT / MRT with periodic, wall, moving-wall and constant Zou–He
 // (velocity or pressure) faces, bounce-back or inert masks and Guo forcing (constant or
-// per-node) — the lattices of run_cavity_2d, run_poiseuille_2d, the .krk 2D loop without
-// per-node faces.  Per-cell arithmetic and order are the single-step kernel's (kmi_cell.h),
+// per-node), and the thermal double distribution (THM: g beside f in LDS, Dirichlet /
+// adiabatic walls, Boussinesq coupling split or fused; not in FP64) — the lattices of run_cavity_2d,
+// run_poiseuille_2d, run_rayleigh_benard_2d and the .krk 2D loop without per-node faces.  Per-cell arithmetic and order are the single-step kernel's (kmi_cell.h),
 // and every intermediate value passes through the storage rounding, so the result is
 // bit-identical to K single-step launches in every storage.
 #include "kmi_cell.h"
@@ -20,15 +21,20 @@
 
 namespace kmi {
 
-template <int ST, int COLL, int TW, int TH, int K>
+template <int ST, int COLL, int TW, int TH, int K, bool THM>
 __global__ void __launch_bounds__((TW + 2 * K) * (TH + 2 * K)) k_step_tb2d(const StepArgs a) {
   using S = Store<ST>;
   using E = typename S::elem;
   using R = typename S::real;
+  using GS = Store<thermal_storage(ST)>;
+  using GE = typename GS::elem;
   constexpr bool SH = S::shifted;
   constexpr int EW = TW + 2 * K, EH = TH + 2 * K, NE = EW * EH, NT = NE;
   constexpr int EPT = 1;
-  __shared__ R L[9 * NE];
+  __shared__ R L[(THM ? 18 : 9) * NE];
+  R* const G = L + 9 * NE;  // thermal populations (THM)
+  const GE* __restrict__ gi = static_cast<const GE*>(a.gi);
+  GE* __restrict__ go = static_cast<GE*>(a.go);
   const int nx = a.nx, ny = a.ny;
   const long long P = a.plane;
   const E* __restrict__ fi = static_cast<const E*>(a.fi);
@@ -50,11 +56,12 @@ __global__ void __launch_bounds__((TW + 2 * K) * (TH + 2 * K)) k_step_tb2d(const
     sfor<9>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
       L[q * NE + e] = S::ld(fi, self + q * P);
+      if constexpr (THM) G[q * NE + e] = GS::ld(gi, self + q * P);
     });
   }
   __syncthreads();
 
-  R nv[EPT][9];
+  R nv[EPT][9], gv[EPT][9];
   for (int s = 1; s <= K; ++s) {
 #pragma unroll
     for (int r = 0; r < EPT; ++r) {
@@ -67,12 +74,13 @@ __global__ void __launch_bounds__((TW + 2 * K) * (TH + 2 * K)) k_step_tb2d(const
       // a link whose source lies beyond a non-periodic face bounces back (k_step's rule)
       const bool vxm = perx || i > 0, vxp = perx || i < nx - 1;
       const bool vym = pery || j > 0, vyp = pery || j < ny - 1;
-      R fp[9], fr[9];
+      R fp[9], fr[9], gp[9], gr[9];
       sfor<9>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         constexpr int cx = Lat<9>::cx[q], cy = Lat<9>::cy[q];
         constexpr int o = Lat<9>::opp[q];
         const bool ok = (cx > 0 ? vxm : (cx < 0 ? vxp : true)) && (cy > 0 ? vym : (cy < 0 ? vyp : true));
+        if constexpr (THM) gp[q] = ok ? G[q * NE + e - cx - cy * EW] : G[o * NE + e];
         R v = ok ? L[q * NE + e - cx - cy * EW] : L[o * NE + e];
         if constexpr (q != 0) {
           if (a.any_moving && !ok) {
@@ -102,7 +110,38 @@ __global__ void __launch_bounds__((TW + 2 * K) * (TH + 2 * K)) k_step_tb2d(const
         }
       }
       const bool solid = a.mask != nullptr && a.mask[self] == kCellSolid;
-      if (solid && a.solid_inert) {
+      if constexpr (THM) {
+        // the thermal step of k_step: anti-bounce-back faces, T, Boussinesq force, thermal
+        // BGK with the unforced (split) or forced (fused) u, Guo collision of f
+        thermal_faces<SH>(gp, a, i, j);
+        const R Tl = zeroth_moment<9>(gp);
+        const R m0 = zeroth_moment<9>(fp);
+        const R rho = SH ? R(1) + m0 : m0;
+        const R inv = R(1) / rho;
+        const R jx = first_moment<9, 0>(fp);
+        const R jy = first_moment<9, 1>(fp);
+        const R fy = SH ? R(a.beta_g) * Tl : R(a.beta_g) * (Tl - R(a.T_ref));
+        const R fx = R(0);
+        R uxf, uyf, uxt, uyt;
+        if (a.coupling == KMI_COUPLING_SPLIT) {
+          uxt = jx * inv;
+          uyt = jy * inv;
+          uxf = (jx + fx / R(2)) * inv;
+          uyf = (jy + fy / R(2)) * inv;
+        } else {
+          uxf = jx * inv;
+          uyf = (jy + fy / R(2)) * inv;
+          uxt = uxf;
+          uyt = uyf;
+        }
+        collide_thermal<SH>(gp, gr, uxt, uyt, R(a.omega_T), R(a.T_ref));
+        if (solid) {
+          sfor<9>([&](auto qc) { fr[decltype(qc)::value] = fp[Lat<9>::opp[decltype(qc)::value]]; });
+        } else {
+          collide_guo<SH>(fp, fr, m0, rho, uxf, uyf, fx, fy, R(a.omega));
+        }
+        sfor<9>([&](auto qc) { gv[r][decltype(qc)::value] = gr[decltype(qc)::value]; });
+      } else if (solid && a.solid_inert) {
         sfor<9>([&](auto qc) { fr[decltype(qc)::value] = SH ? R(0) : R(Lat<9>::w[decltype(qc)::value]); });
       } else if (solid) {
         sfor<9>([&](auto qc) { fr[decltype(qc)::value] = fp[Lat<9>::opp[decltype(qc)::value]]; });
@@ -139,6 +178,7 @@ __global__ void __launch_bounds__((TW + 2 * K) * (TH + 2 * K)) k_step_tb2d(const
       sfor<9>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         L[q * NE + e] = storage_round<ST>(nv[r][q]);
+        if constexpr (THM) G[q * NE + e] = storage_round<thermal_storage(ST)>(gv[r][q]);
       });
     }
     __syncthreads();
@@ -156,6 +196,7 @@ __global__ void __launch_bounds__((TW + 2 * K) * (TH + 2 * K)) k_step_tb2d(const
     sfor<9>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
       S::template st<false>(fo, self + q * P, nv[r][q]);
+      if constexpr (THM) GS::template st<false>(go, self + q * P, gv[r][q]);
     });
   }
 }
@@ -175,37 +216,47 @@ bool tb2d_variant(int tile, int& TW, int& TH, int& K) {
 }
 
 template <int ST, int COLL, int TW, int TH, int K>
-static hipError_t launch_tb(const StepArgs& a, hipStream_t s) {
+static hipError_t launch_tb(const StepArgs& a, bool th, hipStream_t s) {
   const dim3 g(unsigned((a.nx + TW - 1) / TW), unsigned((a.ny + TH - 1) / TH)), b((TW + 2 * K) * (TH + 2 * K));
-  hipLaunchKernelGGL((k_step_tb2d<ST, COLL, TW, TH, K>), g, b, 0, s, a);
+  if (th) {
+    // the thermal chain collides f with Guo BGK; FP64 f and g exceed the registers of a
+    // 1024-wide group, so FP64 thermal lattices keep single-step launches
+    if constexpr (COLL == KMI_BGK && ST != KMI_STORE_F64)
+      hipLaunchKernelGGL((k_step_tb2d<ST, COLL, TW, TH, K, true>), g, b, 0, s, a);
+    else
+      return hipErrorInvalidValue;
+  } else {
+    hipLaunchKernelGGL((k_step_tb2d<ST, COLL, TW, TH, K, false>), g, b, 0, s, a);
+  }
   return hipGetLastError();
 }
 
 template <int ST, int COLL>
-static hipError_t launch_tb_tile(int tile, const StepArgs& a, hipStream_t s) {
+static hipError_t launch_tb_tile(int tile, const StepArgs& a, bool th, hipStream_t s) {
   switch (tile) {
-    case 1: return launch_tb<ST, COLL, 28, 28, 2>(a, s);
-    case 2: return launch_tb<ST, COLL, 24, 24, 4>(a, s);
-    case 3: return launch_tb<ST, COLL, 56, 8, 4>(a, s);
-    case 4: return launch_tb<ST, COLL, 16, 16, 8>(a, s);
-    case 5: return launch_tb<ST, COLL, 60, 12, 2>(a, s);
+    case 1: return launch_tb<ST, COLL, 28, 28, 2>(a, th, s);
+    case 2: return launch_tb<ST, COLL, 24, 24, 4>(a, th, s);
+    case 3: return launch_tb<ST, COLL, 56, 8, 4>(a, th, s);
+    case 4: return launch_tb<ST, COLL, 16, 16, 8>(a, th, s);
+    case 5: return launch_tb<ST, COLL, 60, 12, 2>(a, th, s);
     default: return hipErrorInvalidValue;
   }
 }
 
 template <int ST>
-static hipError_t launch_tb_coll(int coll, int tile, const StepArgs& a, hipStream_t s) {
-  if (coll == KMI_MRT) return launch_tb_tile<ST, KMI_MRT>(tile, a, s);
-  if (coll == KMI_TRT) return launch_tb_tile<ST, KMI_TRT>(tile, a, s);
-  return launch_tb_tile<ST, KMI_BGK>(tile, a, s);
+static hipError_t launch_tb_coll(int coll, int tile, const StepArgs& a, bool th, hipStream_t s) {
+  if (th) return launch_tb_tile<ST, KMI_BGK>(tile, a, true, s);
+  if (coll == KMI_MRT) return launch_tb_tile<ST, KMI_MRT>(tile, a, false, s);
+  if (coll == KMI_TRT) return launch_tb_tile<ST, KMI_TRT>(tile, a, false, s);
+  return launch_tb_tile<ST, KMI_BGK>(tile, a, false, s);
 }
 
-hipError_t launch_step_tb2d(int st, int coll, int tile, const StepArgs& a, hipStream_t s) {
+hipError_t launch_step_tb2d(int st, int coll, int tile, bool th, const StepArgs& a, hipStream_t s) {
   switch (st) {
-    case KMI_STORE_F64: return launch_tb_coll<KMI_STORE_F64>(coll, tile, a, s);
-    case KMI_STORE_F32: return launch_tb_coll<KMI_STORE_F32>(coll, tile, a, s);
-    case KMI_STORE_F32S: return launch_tb_coll<KMI_STORE_F32S>(coll, tile, a, s);
-    default: return launch_tb_coll<KMI_STORE_F16S>(coll, tile, a, s);
+    case KMI_STORE_F64: return launch_tb_coll<KMI_STORE_F64>(coll, tile, a, th, s);
+    case KMI_STORE_F32: return launch_tb_coll<KMI_STORE_F32>(coll, tile, a, th, s);
+    case KMI_STORE_F32S: return launch_tb_coll<KMI_STORE_F32S>(coll, tile, a, th, s);
+    default: return launch_tb_coll<KMI_STORE_F16S>(coll, tile, a, th, s);
   }
 }
 

@@ -434,3 +434,33 @@ def test_d2q9_multi_step_autotuned_cavity_matches(kmi, backend):
     with kmi.Lattice(d, backend) as lat:
         lat.step(300)
         assert np.array_equal(lat.download(), a)
+
+
+@pytest.mark.parametrize("storage", ["f32", "f32s", "f16s"])
+def test_d2q9_thermal_multi_step_launches_bit_identical(kmi, backend, storage):
+    """The thermal double distribution in the multi-step kernel: Rayleigh–Bénard (split
+    coupling, periodic x, Dirichlet south/north) and natural convection in a box (fused
+    coupling, Dirichlet west/east with the corner rule, adiabatic south/north), f and g
+    bit-identical to single steps for every variant."""
+    N, M, steps = 70, 45, 29
+    rb, _ = kmi.rayleigh_benard_desc(N, M, 1e4, 0.71, storage=storage)
+    nc = kmi.natural_convection_desc(64, 1e5, 0.71, storage=storage)
+    x = (np.arange(64)[:, None] + 0.5) / 64 + 0.0 * np.arange(64)[None, :]
+    cases = [("rb", rb, kmi.rb_initial_temperature(N, M, 1.0, 0.0, seed="noise")),
+             ("natconv", nc, np.asfortranarray(1.0 - x))]
+    for name, desc, T0 in cases:
+        print(f"[tb2d thermal] {storage} {name}", flush=True)
+        outs = []
+        for tile in (0, 1, 2, 3, 4, 5):
+            with kmi.Lattice(desc, backend) as lat:
+                if T0 is not None:
+                    lat.init_temperature(T0)
+                if tile:
+                    lat.set_temporal(tile, 1)
+                lat.step(steps)
+                outs.append((lat.download(), lat.download("g")))
+        for tile, (f, g) in zip((1, 2, 3, 4, 5), outs[1:]):
+            assert np.array_equal(f, outs[0][0]) and np.array_equal(g, outs[0][1]), (name, tile)
+    with kmi.Lattice(kmi.rayleigh_benard_desc(N, M, 1e4, 0.71, storage="f64")[0], backend) as lat:
+        with pytest.raises(kmi.KrakenError):
+            lat.set_temporal(1, 1)  # FP64 thermal lattices keep single steps
