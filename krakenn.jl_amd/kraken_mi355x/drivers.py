@@ -94,6 +94,16 @@ def cavity_desc(N, nu, u_lid, *, storage="f64", collision="bgk", lid="zouhe", Ny
                             "south": {"kind": "wall"}, "north": top})
 
 
+def _tune(lat, steps):
+    """Pick a D2Q9 lattice's launch (single-step shape, or several steps per launch) for a
+    long run: small 2D lattices are bound by the fixed cost of a launch, and the ~30 trials
+    of the tuner (a few thousand steps) pay off over tens of thousands.  The populations are
+    restored and every choice gives bit-identical results.  D3Q19 runs keep the default
+    launch here (their tuner times ~130 variants at full size)."""
+    if lat.Q == 9 and steps >= 2000:
+        lat.autotune(64)
+
+
 def run_cavity_2d(config: LBMConfig, *, backend=None, T=np.float64, collision="bgk", lid="zouhe"):
     """run_cavity_2d (basic.jl:112-144): rest start, `max_steps` steps, final moments."""
     if not isinstance(config.lattice, D2Q9):
@@ -101,8 +111,7 @@ def run_cavity_2d(config: LBMConfig, *, backend=None, T=np.float64, collision="b
     d = cavity_desc(config.Nx, config.nu, config.u_lid, storage=_storage(T), collision=collision,
                     lid=lid, Ny=config.Ny)
     with Lattice(d, _backend(backend)) as lat:
-        if config.max_steps >= 2000:
-            lat.autotune(64)  # launch choice only: results are identical for every choice
+        _tune(lat, config.max_steps)
         lat.step(config.max_steps)
         m = lat.moments()
     return {"ρ": m["rho"], "ux": m["ux"], "uy": m["uy"], "config": config}
@@ -123,6 +132,7 @@ def run_cavity_3d(config: LBMConfig, *, backend=None, T=np.float64, collision="b
     d = cavity_3d_desc(config.Nx, config.Ny, config.Nz, config.nu, config.u_lid,
                        storage=_storage(T), collision=collision)
     with Lattice(d, _backend(backend)) as lat:
+        _tune(lat, config.max_steps)
         lat.step(config.max_steps)
         m = lat.moments()
     return {"ρ": m["rho"], "ux": m["ux"], "uy": m["uy"], "uz": m["uz"], "config": config}
@@ -137,6 +147,7 @@ def run_poiseuille_2d(*, Nx=4, Ny=32, nu=None, ν=0.1, Fx=1e-5, max_steps=10000,
     d = make_desc("d2q9", Nx, Ny, 1, nu, storage=_storage(T), faces=per, forcing="guo_plain",
                   force=(Fx, 0.0))
     with Lattice(d, _backend(backend)) as lat:
+        _tune(lat, max_steps)
         lat.step(max_steps)
         m = lat.moments()
     return {"ρ": m["rho"], "ux": m["ux"], "uy": m["uy"],
@@ -153,6 +164,7 @@ def run_couette_2d(*, Nx=4, Ny=32, nu=None, ν=0.1, u_wall=0.05, max_steps=10000
              "north": {"kind": "zouhe_velocity", "u": (0.0, 0.0, 0.0)}}
     d = make_desc("d2q9", Nx, Ny, 1, nu, storage=_storage(T), faces=faces, bc_order=["south", "north"])
     with Lattice(d, _backend(backend)) as lat:
+        _tune(lat, max_steps)
         lat.step(max_steps)
         m = lat.moments()
     return {"ρ": m["rho"], "ux": m["ux"], "uy": m["uy"],
@@ -183,6 +195,7 @@ def run_taylor_green_2d(*, N=64, nu=None, ν=0.01, u0=0.01, max_steps=1000, back
     d = make_desc("d2q9", N, N, 1, nu, storage=_storage(T), faces=per)
     with Lattice(d, _backend(backend)) as lat:
         lat.init_equilibrium(*taylor_green_2d_fields(N, u0, T))
+        _tune(lat, max_steps)
         lat.step(max_steps)
         m = lat.moments()
     return {"ρ": m["rho"], "ux": m["ux"], "uy": m["uy"], "u0": u0, "k": 2 * np.pi / N,
@@ -212,6 +225,7 @@ def run_taylor_green_3d(*, N=256, nu=0.01, U0=0.05, max_steps=1000, storage="f32
     d = taylor_green_3d_desc(N, nu, storage)
     with Lattice(d, _backend(backend)) as lat:
         lat.init_equilibrium(*taylor_green_3d_fields(N, U0))
+        _tune(lat, max_steps)
         lat.step(max_steps)
         m = lat.moments()
     m["max_steps"] = max_steps
@@ -259,6 +273,7 @@ def run_rayleigh_benard_2d(*, Nx=128, Ny=32, Ra=2000.0, Pr=1.0, T_hot=1.0, T_col
     d, p = rayleigh_benard_desc(Nx, Ny, Ra, Pr, T_hot, T_cold, nu, alpha, storage)
     with Lattice(d, _backend(backend)) as lat:
         lat.init_temperature(rb_initial_temperature(Nx, Ny, T_hot, T_cold, seed))
+        _tune(lat, max_steps)
         lat.step(max_steps)
         m = lat.moments(temperature=True)
         Nu = lat.reduce("nusselt", "south")
@@ -295,6 +310,7 @@ def run_natural_convection_2d(*, N=128, Ra=1e4, Pr=0.71, T_hot=1.0, T_cold=0.0, 
     d = natural_convection_desc(N, Ra, Pr, T_hot, T_cold, storage)
     with Lattice(d, _backend(backend)) as lat:
         lat.init_temperature(natural_convection_initial_temperature(N, T_hot, T_cold))
+        _tune(lat, max_steps)
         lat.step(max_steps)
         m = lat.moments(temperature=True)
         Nu = lat.reduce("nusselt", "west")
@@ -334,6 +350,7 @@ def run_natural_convection_3d(*, N=32, Ra=1e3, Pr=0.71, T_hot=1.0, T_cold=0.0, m
     d, p = natural_convection_3d_desc(N, Ra, Pr, T_hot, T_cold, storage)
     with Lattice(d, _backend(backend)) as lat:
         lat.init_temperature(natural_convection_3d_initial_temperature(N, T_hot, T_cold))
+        _tune(lat, max_steps)
         lat.step(max_steps)
         m = lat.moments(temperature=True)
         Nu = lat.reduce("nusselt", "west")
