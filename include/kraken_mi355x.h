@@ -302,6 +302,11 @@ int kmi_step(kmi_lattice* lat, int64_t nsteps);
 /* As kmi_step, and also the relative one-step velocity change of the last step,
  * r_u = ||u^k - u^(k-1)||_2 / ||u^k||_2 over fluid cells (gnf_periodic_steady.jl:186-196). */
 int kmi_step_residual(kmi_lattice* lat, int64_t nsteps, double* r_u);
+/* As kmi_step_residual over the last `window` = m steps (1 <= m <= nsteps):
+ * r = ||u^k - u^(k-m)||_2 / (m ||u^k||_2), the stop rule of kmi_batch_run; m = 1 is r_u.
+ * No reference counterpart: for single-lattice runs to a residual in FP32 storage, whose
+ * one-step r_u has a rounding floor near 4e-7. */
+int kmi_step_residual_window(kmi_lattice* lat, int64_t nsteps, int64_t window, double* r_u);
 /* Device-timed kmi_step: elapsed milliseconds between HIP events on the lattice stream. */
 int kmi_step_timed(kmi_lattice* lat, int64_t nsteps, double* ms);
 /* Moments of the stored (post-collision) populations, as compute_macroscopic_2d/3d and

@@ -992,6 +992,22 @@ int kmi_step_residual(kmi_lattice* L, int64_t nsteps, double* r_u) {
   return KMI_OK;
 }
 
+int kmi_step_residual_window(kmi_lattice* L, int64_t nsteps, int64_t window, double* r_u) {
+  if (!L) return fail(nullptr, KMI_ERR_ARG, "kmi_step_residual_window: lattice is NULL");
+  kmi_ctx* c = L->ctx;
+  if (window < 1 || nsteps < window || !r_u)
+    return fail(c, KMI_ERR_ARG, "kmi_step_residual_window: need 1 <= window <= nsteps and r_u");
+  KMI_HIP(c, hipSetDevice(c->device));
+  if (!L->uprev) KMI_HIP(c, hipMalloc(&L->uprev, size_t(3) * L->ncells * sizeof(double)));
+  if (int rc = run_steps(L, nsteps - window)) return rc;
+  double tmp[3];
+  if (int rc = reduce_cells(L, 101, tmp)) return rc;  // u^(k-m) → uprev
+  if (int rc = run_steps(L, window)) return rc;
+  if (int rc = reduce_cells(L, 100, tmp)) return rc;
+  *r_u = std::sqrt(tmp[0] / tmp[1]) / double(window);
+  return KMI_OK;
+}
+
 int kmi_moments(kmi_lattice* L, double* rho, double* ux, double* uy, double* uz, double* T) {
   if (!L) return fail(nullptr, KMI_ERR_ARG, "kmi_moments: lattice is NULL");
   kmi_ctx* c = L->ctx;

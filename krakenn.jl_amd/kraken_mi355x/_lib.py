@@ -117,6 +117,7 @@ SIGNATURES = {
     "kmi_init_temperature": (c_int, [c_void_p, POINTER(c_double)]),
     "kmi_step": (c_int, [c_void_p, c_int64]),
     "kmi_step_residual": (c_int, [c_void_p, c_int64, POINTER(c_double)]),
+    "kmi_step_residual_window": (c_int, [c_void_p, c_int64, c_int64, POINTER(c_double)]),
     "kmi_step_timed": (c_int, [c_void_p, c_int64, POINTER(c_double)]),
     "kmi_moments": (c_int, [c_void_p] + [POINTER(c_double)] * 5),
     "kmi_reduce": (c_int, [c_void_p, c_int32, c_int32, POINTER(c_double)]),

@@ -255,9 +255,14 @@ class Lattice:
         self._chk(L.load().kmi_step_timed(self._h, int(n), byref(ms)))
         return ms.value
 
-    def step_residual(self, n):
+    def step_residual(self, n, window=1):
+        """Advance n steps; the relative velocity change over the last `window` steps,
+        ‖u^k − u^(k−m)‖/(m‖u^k‖) (m = 1: the reference's one-step r_u)."""
         r = c_double()
-        self._chk(L.load().kmi_step_residual(self._h, int(n), byref(r)))
+        if window == 1:
+            self._chk(L.load().kmi_step_residual(self._h, int(n), byref(r)))
+        else:
+            self._chk(L.load().kmi_step_residual_window(self._h, int(n), int(window), byref(r)))
         return r.value
 
     @property

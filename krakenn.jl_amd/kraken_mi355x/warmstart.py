@@ -146,9 +146,10 @@ def converged_agreement(Re, A, net, *, N=128, tol=1e-11, check=100, window=100, 
 def plain_time_to_residual(Re, A, *, N=128, tol=1e-7, check=100, window=100, max_steps=1_000_000,
                            storage="f32s", backend=None):
     """Rest-initialised plain runs: one lattice at a time (the single-cavity path with its
-    autotuned launch), each run to the batch's stop rule r = ‖u^k − u^(k−m)‖/(m‖u^k‖) < tol checked
-    every `check` steps (m = `window`).  Returns per-case wall seconds, steps and status
-    (1 converged, 2 diverged, 3 max_steps)."""
+    autotuned launch), each run to the batch's stop rule r = ‖u^k − u^(k−m)‖/(m‖u^k‖) < tol
+    checked every `check` steps (m = `window`), evaluated on the device
+    (kmi_step_residual_window).  Returns per-case wall seconds, steps and status (1 converged,
+    2 diverged, 3 max_steps)."""
     from .drivers import cavity_desc
 
     backend = backend or MI355XBackend()
@@ -163,14 +164,8 @@ def plain_time_to_residual(Re, A, *, N=128, tol=1e-7, check=100, window=100, max
         with Lattice(d, backend) as lat:
             lat.autotune(64)  # the fastest single-lattice launch (several steps per launch here)
             while n < max_steps:
-                lat.step(check - window)
-                m0 = lat.moments()
-                lat.step(window)
-                m1 = lat.moments()
+                r = lat.step_residual(check, window)  # on the device, FP64
                 n += check
-                du = math.sqrt(float(np.sum((m1["ux"] - m0["ux"]) ** 2 + (m1["uy"] - m0["uy"]) ** 2)))
-                u = math.sqrt(float(np.sum(m1["ux"] ** 2 + m1["uy"] ** 2)))
-                r = du / u / window if u > 0 else math.inf
                 if not math.isfinite(r):
                     st = 2
                     break

@@ -260,3 +260,22 @@ def test_batch_retired_cavity_keeps_written_state(kmi, po, backend):
         b.init_equilibrium(0)
         b.step(3)
         assert np.array_equal(b.download(0), po.rest((8, 8)))
+
+
+def test_single_lattice_windowed_residual_matches_batch(kmi, backend):
+    """kmi_step_residual_window follows the batch's stop rule: the same cavity reaches
+    r < tol after the same number of steps alone and in a batch (F32S, m = 100)."""
+    n, nu, tol, check, window = 48, 0.05, 1e-6, 200, 100
+    with kmi.Batch([n], [nu], storage="f32s", collision="mrt", backend=backend) as b:
+        b.run(tol, check, 200_000, window=window)
+        st = b.status()
+    assert st["status"][0] == kmi._lib.CAV_CONVERGED
+    with kmi.Lattice(kmi.cavity_desc(n, nu, 0.1, storage="f32s", collision="mrt"), backend) as lat:
+        steps, r = 0, 1.0
+        while r >= tol:
+            r = lat.step_residual(check, window)
+            steps += check
+        assert steps == st["steps"][0]
+        assert abs(r - st["residual"][0]) <= 1e-6 * st["residual"][0]
+        with pytest.raises(kmi.KrakenError):
+            lat.step_residual(10, 20)
