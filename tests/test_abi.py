@@ -57,6 +57,7 @@ def _validate(kmi, d):
     ("zh_3d_normal", 1, "1 + sign*u_n"),
     ("dirichlet_periodic", 1, "periodic"),
     ("bad_size", 1, "positive"),
+    ("too_many_populations", 5, "2^31"),
 ])
 def test_desc_validation_errors(kmi, case, status, needle):
     mk = kmi.make_desc
@@ -81,6 +82,8 @@ def test_desc_validation_errors(kmi, case, status, needle):
     elif case == "dirichlet_periodic":
         d = mk("d2q9", 8, 8, thermal=True, faces={"west": {"kind": "periodic", "thermal": "dirichlet"},
                                                   "east": {"kind": "periodic"}})
+    elif case == "too_many_populations":
+        d = mk("d3q19", 1024, 1024, 128)  # 2.55e9 populations: 32-bit plane offsets would overflow
     else:
         d = mk("d2q9", 0, 8)
     rc, msg = _validate(kmi, d)
