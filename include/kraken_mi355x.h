@@ -332,8 +332,9 @@ int kmi_lattice_get_launch(const kmi_lattice* lat, int32_t* bx, int32_t* by, int
  * lattices without mask and with nz >= 4: the intermediate populations stay in LDS, so HBM
  * sees one read and one write of the field per launch.  tile: 0 off; two steps: 1 = 32x8,
  * 2 = 64x4, 3 = 64x8, 4 = 32x16, 5 = 64x16 (not F64); three steps: 6 = 64x8 (not F64),
- * 7 = 32x8, 8 = 32x16 (not F64) cells per work-group; nx and ny must be multiples of the
- * tile.  zchunks: z-ranges per tile column (1..nz).  D2Q9 lattices without thermal
+ * 7 = 32x8, 8 = 32x16 (not F64) cells per work-group with one work-item per cell, and
+ * 9 = 32x8, 10 = 32x16 (not F64), 11 = 64x8 (not F64) with one work-item per cell of the
+ * tile plus its two-cell ring; nx and ny must be multiples of the tile.  zchunks: z-ranges per tile column (1..nz).  D2Q9 lattices without thermal
  * populations, cut links or per-node faces (at least the tile plus its ring on each axis):
  * tile 1 = 28x28 two steps, 2 = 24x24 four, 3 = 56x8 four, 4 = 16x16 eight, 5 = 60x12 two
  * (the tile and a ring of that many cells are advanced in LDS), zchunks 1.  Results are

@@ -250,15 +250,20 @@ __global__ void __launch_bounds__(TX * TY)
 // writes are read after the next iteration's second barrier.  Every t+1 and t+2 value
 // passes through the storage rounding, so the result is bit-identical to three single-step
 // launches in every storage.
-template <int ST, int COLL, int TX, int TY, bool NTS>
-__global__ void __launch_bounds__(TX * TY)
+// NT threads per work-group (at least TX·TY): with NT = TX·TY a thread evaluates up to two
+// level-1 and two level-2 cells one after the other; with NT ≥ (TX+4)(TY+4) every cell of a
+// level has its own thread, which halves the registers a thread holds (more resident waves
+// per CU) and runs the halo evaluations side by side; threads beyond TX·TY idle in P3.
+template <int ST, int COLL, int TX, int TY, int NT, bool NTS>
+__global__ void __launch_bounds__(NT)
     k_step3_d3q19(const StepArgs a, int zlen, int ntx, int nty, int ntiles, int per_xcd) {
   using S = Store<ST>;
   using E = typename S::elem;
   using R = typename S::real;
   constexpr int E1X = TX + 4, E1Y = TY + 4, N1 = E1X * E1Y;  // level 1 (t+1)
   constexpr int E2X = TX + 2, E2Y = TY + 2, N2 = E2X * E2Y;  // level 2 (t+2)
-  constexpr int NT = TX * TY;
+  constexpr int NI = TX * TY;
+  static_assert(NT >= NI && NT % 64 == 0, "one thread per interior cell at least");
   constexpr int EPT1 = (N1 + NT - 1) / NT, EPT2 = (N2 + NT - 1) / NT;
   constexpr unsigned ES = sizeof(E);
   __shared__ R lds[24 * (N1 + N2)];
@@ -281,7 +286,7 @@ __global__ void __launch_bounds__(TX * TY)
   E* __restrict__ fo = static_cast<E*>(a.fo);
   const int tid = threadIdx.x;
 
-  constexpr bool kPrefetch = sizeof(R) == 4 && NT <= 512;
+  constexpr bool kPrefetch = sizeof(R) == 4 && (NT <= 512 || EPT1 == 1);
   R nxt[EPT1][19];
   auto pull_plane = [&](int s) {
     const int zs = tb::wrap(s, nz);
@@ -371,7 +376,7 @@ __global__ void __launch_bounds__(TX * TY)
     }
     __syncthreads();
     // ---- P3: t+3 on plane s−2, interior; then the P1 values into level 1 ----
-    if (s >= z0 + 2) {
+    if (s >= z0 + 2 && tid < NI) {
       const R* const B2r = B2 + ((u - 3) & 1) * 5 * N2;
       const int ix = tid % TX, iy = tid / TX;
       R fp[19], fn[19];
@@ -426,9 +431,11 @@ __global__ void __launch_bounds__(TX * TY)
 // Tile variants (TX × TY interior cells per work-group).  Two steps per launch: 1 = 32×8,
 // 2 = 64×4, 3 = 64×8, 4 = 32×16, 5 = 64×16, not for FP64 (the larger the tile, the smaller
 // the share of halo recomputation: (TX+2)(TY+2)/(TX·TY) = 1.55, 1.55, 1.29, 1.32, 1.16).
-// Three steps per launch: 6 = 64×8, 7 = 32×8, 8 = 32×16; FP64 only 7 (LDS: 24 values per
-// cell of both levels, 141 / 74 / 128 KB in FP32; evaluations per update
-// ((TX+4)(TY+4) + (TX+2)(TY+2) + TX·TY) / (3·TX·TY) = 1.29, 1.34, 1.20).
+// Three steps per launch: 6 = 64×8, 7 = 32×8, 8 = 32×16 with one thread per interior cell;
+// 9 = 32×8 (448 threads), 10 = 32×16 (768), 11 = 64×8 (832) with one thread per level-1
+// cell; FP64 only 7 and 9 (LDS: 24 values per cell of both levels, 141 / 74 / 128 KB in
+// FP32; evaluations per update ((TX+4)(TY+4) + (TX+2)(TY+2) + TX·TY) / (3·TX·TY) = 1.29,
+// 1.34, 1.20).
 bool step2_tile(int tile, int st, int& TX, int& TY) {
   switch (tile) {
     case 1: TX = 32; TY = 8; return true;
@@ -439,6 +446,9 @@ bool step2_tile(int tile, int st, int& TX, int& TY) {
     case 6: TX = 64; TY = 8; return st != KMI_STORE_F64;
     case 7: TX = 32; TY = 8; return true;
     case 8: TX = 32; TY = 16; return st != KMI_STORE_F64;
+    case 9: TX = 32; TY = 8; return true;
+    case 10: TX = 32; TY = 16; return st != KMI_STORE_F64;
+    case 11: TX = 64; TY = 8; return st != KMI_STORE_F64;
     default: return false;
   }
 }
@@ -458,16 +468,16 @@ static hipError_t launch2(bool nts, const StepArgs& a, int zchunks, hipStream_t 
   return hipGetLastError();
 }
 
-template <int ST, int COLL, int TX, int TY>
+template <int ST, int COLL, int TX, int TY, int NT = TX * TY>
 static hipError_t launch3(bool nts, const StepArgs& a, int zchunks, hipStream_t s) {
   const int ntx = a.nx / TX, nty = a.ny / TY;
   const int zlen = (a.nz + zchunks - 1) / zchunks;
   const int nzc = (a.nz + zlen - 1) / zlen;
   const int ntiles = ntx * nty * nzc;
   const int per_xcd = (ntiles + 7) / 8;
-  const dim3 g(static_cast<unsigned>(8 * per_xcd)), b(static_cast<unsigned>(TX * TY));
-  if (nts) hipLaunchKernelGGL((k_step3_d3q19<ST, COLL, TX, TY, true>), g, b, 0, s, a, zlen, ntx, nty, ntiles, per_xcd);
-  else hipLaunchKernelGGL((k_step3_d3q19<ST, COLL, TX, TY, false>), g, b, 0, s, a, zlen, ntx, nty, ntiles, per_xcd);
+  const dim3 g(static_cast<unsigned>(8 * per_xcd)), b(static_cast<unsigned>(NT));
+  if (nts) hipLaunchKernelGGL((k_step3_d3q19<ST, COLL, TX, TY, NT, true>), g, b, 0, s, a, zlen, ntx, nty, ntiles, per_xcd);
+  else hipLaunchKernelGGL((k_step3_d3q19<ST, COLL, TX, TY, NT, false>), g, b, 0, s, a, zlen, ntx, nty, ntiles, per_xcd);
   return hipGetLastError();
 }
 
@@ -475,11 +485,16 @@ template <int ST, int COLL>
 static hipError_t launch2_tile(int tile, bool nts, const StepArgs& a, int zchunks, hipStream_t s) {
   switch (tile) {
     case 7: return launch3<ST, COLL, 32, 8>(nts, a, zchunks, s);
+    case 9: return launch3<ST, COLL, 32, 8, 448>(nts, a, zchunks, s);
     case 6:
     case 8:
+    case 10:
+    case 11:
       if constexpr (ST != KMI_STORE_F64) {
         if (tile == 6) return launch3<ST, COLL, 64, 8>(nts, a, zchunks, s);
-        return launch3<ST, COLL, 32, 16>(nts, a, zchunks, s);
+        if (tile == 8) return launch3<ST, COLL, 32, 16>(nts, a, zchunks, s);
+        if (tile == 10) return launch3<ST, COLL, 32, 16, 768>(nts, a, zchunks, s);
+        return launch3<ST, COLL, 64, 8, 832>(nts, a, zchunks, s);
       } else {
         return hipErrorInvalidValue;
       }

@@ -288,7 +288,8 @@ class Lattice:
     def set_temporal(self, tile, zchunks=1):
         """Temporal blocking.  Fully periodic D3Q19 BGK/TRT lattice: two steps per launch
         (tile 1: 32×8, 2: 64×4, 3: 64×8, 4: 32×16, 5: 64×16 cells) or three (6: 64×8, 7: 32×8,
-        8: 32×16; FP64 only 7).  D2Q9 lattice without thermal populations, cut links or
+        8: 32×16 with a thread per cell; 9: 32×8, 10: 32×16, 11: 64×8 with a thread per cell
+        of the tile and its two-cell ring; FP64 only 7 and 9).  D2Q9 lattice without thermal populations, cut links or
         per-node faces: 1: 28×28 two steps, 2: 24×24 four, 3: 56×8 four, 4: 16×16 eight,
         5: 60×12 two.  0: off.  Bit-identical to single steps."""
         self._chk(L.load().kmi_lattice_set_temporal(self._h, int(tile), int(zchunks)))

@@ -301,7 +301,8 @@ def test_two_step_launches_bit_identical(kmi, backend, storage, collision):
 @pytest.mark.parametrize("collision", ["trt", "bgk"])
 def test_three_step_launches_bit_identical(kmi, backend, storage, collision):
     """Three steps per launch (t+1 and t+2 populations in two LDS levels) reproduce single-step
-    launches bit for bit, for every tile and z split down to one plane per range, with step
+    launches bit for bit, for every tile and thread count (one thread per interior cell, tiles
+    6-8, or per level-1 cell, tiles 9-11) and z split down to one plane per range, with step
     counts that leave one- and two-step remainders, and on a four-plane lattice."""
     per = {f: {"kind": "periodic"} for f in ("west", "east", "south", "north", "bottom", "top")}
     for shape in ((64, 16, 12), (32, 16, 4)):
@@ -313,9 +314,10 @@ def test_three_step_launches_bit_identical(kmi, backend, storage, collision):
                 lat.upload(f)
                 lat.step(steps)
                 ref = lat.download()
-            modes = [(7, 1), (7, min(5, shape[2] - 1)), (7, shape[2])]
+            modes = [(7, 1), (7, min(5, shape[2] - 1)), (7, shape[2]), (9, 1), (9, 3)]
             if storage != "f64":
-                modes += [(8, 2), (8, 3)] + ([(6, 1), (6, 4)] if shape[0] % 64 == 0 else [])
+                modes += [(8, 2), (8, 3), (10, 1), (10, 4)]
+                modes += [(6, 1), (6, 4), (11, 2), (11, shape[2])] if shape[0] % 64 == 0 else []
             for tile, zc in modes:
                 with kmi.Lattice(desc, backend) as lat:
                     lat.upload(f)
@@ -331,7 +333,7 @@ def test_three_step_launches_bit_identical(kmi, backend, storage, collision):
             lat.set_temporal(7, 1)  # fewer than four planes
     if storage == "f64":
         with kmi.Lattice(kmi.make_desc("d3q19", 64, 16, 8, 0.02, storage="f64", faces=per), backend) as lat:
-            for tile in (6, 8):
+            for tile in (6, 8, 10, 11):
                 with pytest.raises(kmi.KrakenError):
                     lat.set_temporal(tile, 1)  # FP64 LDS budget
 
