@@ -26,9 +26,10 @@ STORAGE_LABEL = {"f64": "FP64 storage", "f32": "FP32 storage",
 
 
 def launch_span(launch):
-    """Steps one launch advances for a lattice launch record (temporal tile 0: 1, 1-5: 2, 6-11: 3)."""
+    """Steps one launch advances for a lattice launch record (temporal tile 0: 1, 1-5 and
+    12-14: 2, 6-11: 3)."""
     t = launch.get("tile", 0)
-    return 3 if t >= 6 else (2 if t >= 1 else 1)
+    return 3 if 6 <= t <= 11 else (2 if t >= 1 else 1)
 
 
 def cpu_baseline(N, nu, storage, budget_s=12.0, max_steps=30):
@@ -449,7 +450,7 @@ def main():
     # HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated on a
     # copy kernel of known size), recorded for this workload in profiles/traffic.json.
     # Steps per launch of the dominant kernel: 3 for the three-step kernel (k_step3_d3q19,
-    # tiles 6-11), 2 for the two-step kernel (k_step2_d3q19, tiles 1-5), else 1 (k_step).
+    # tiles 6-11), 2 for the two-step kernel (k_step2_d3q19, tiles 1-5, 12-14), else 1 (k_step).
     # Traffic records are per kernel.
     span = launch_span(launch)
     kernel = {3: "k_step3_d3q19", 2: "k_step2_d3q19", 1: "k_step"}[span]

@@ -331,7 +331,9 @@ int kmi_lattice_get_launch(const kmi_lattice* lat, int32_t* bx, int32_t* by, int
 /* Two or three steps per launch (temporal blocking) for fully periodic D3Q19 BGK/TRT
  * lattices without mask and with nz >= 4: the intermediate populations stay in LDS, so HBM
  * sees one read and one write of the field per launch.  tile: 0 off; two steps: 1 = 32x8,
- * 2 = 64x4, 3 = 64x8, 4 = 32x16, 5 = 64x16 (not F64); three steps: 6 = 64x8 (not F64),
+ * 2 = 64x4, 3 = 64x8, 4 = 32x16, 5 = 64x16 (not F64) with one work-item per cell, and
+ * 12 = 64x8, 13 = 32x16, 14 = 32x8 with one work-item per cell of the tile plus its
+ * one-cell ring; three steps: 6 = 64x8 (not F64),
  * 7 = 32x8, 8 = 32x16 (not F64) cells per work-group with one work-item per cell, and
  * 9 = 32x8, 10 = 32x16 (not F64), 11 = 64x8 (not F64) with one work-item per cell of the
  * tile plus its two-cell ring; nx and ny must be multiples of the tile.  zchunks: z-ranges per tile column (1..nz).  D2Q9 lattices without thermal

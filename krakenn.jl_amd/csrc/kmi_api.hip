@@ -1247,7 +1247,7 @@ int kmi_lattice_autotune(kmi_lattice* L, int64_t steps_per_trial, double* best_m
   }
   // Two- and three-step launches (the single-step shape found above serves remainders).
   const int zc_list[] = {1, 2, 3, 4, 6, 8, 12, 16};
-  for (int tile = 1; tile <= 11 && e == hipSuccess; ++tile) {
+  for (int tile = 1; tile <= 14 && e == hipSuccess; ++tile) {
     if (!tb_eligible(L, tile)) continue;
     for (int zc : zc_list) {
       if (zc > L->args.nz || e != hipSuccess) continue;

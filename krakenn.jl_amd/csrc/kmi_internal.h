@@ -25,7 +25,7 @@ hipError_t launch_step(int q, int st, int coll, bool th, bool bnd, bool nts, con
 // `tile` selects the work-group tile and the steps per launch (step2_tile, temporal_span),
 // `zchunks` the z-ranges per column.
 bool step2_tile(int tile, int st, int& TX, int& TY);
-// Steps one launch of `tile` advances: 2 (tiles 1–5), 3 (tiles 6–11), 1 (tile 0).
+// Steps one launch of `tile` advances: 2 (tiles 1–5, 12–14), 3 (tiles 6–11), 1 (tile 0).
 int temporal_span(int tile);
 hipError_t launch_step2(int st, int coll, int tile, int zchunks, bool nts, const StepArgs& a,
                         hipStream_t s);

@@ -108,13 +108,17 @@ KMI_DEV void collide19(const R (&fp)[19], R (&fo)[19], const StepArgs& a) {
 // Grid: one work-group per (column tile, z-range), padded to a multiple of 8.  Work-group
 // b runs on XCD b mod 8; it takes tile (b mod 8)·per_xcd + b/8, so each XCD works through a
 // contiguous band of neighbouring tiles and their shared halo rows meet in its L2.
-template <int ST, int COLL, int TX, int TY, bool NTS>
-__global__ void __launch_bounds__(TX * TY)
+// NT threads per work-group, at least TX·TY: one per interior cell (a thread then evaluates
+// up to two extended cells in phase 1), or at least (TX+2)(TY+2), one per extended cell
+// (threads beyond TX·TY idle in phase 2).
+template <int ST, int COLL, int TX, int TY, int NT, bool NTS>
+__global__ void __launch_bounds__(NT)
     k_step2_d3q19(const StepArgs a, int zlen, int ntx, int nty, int ntiles, int per_xcd) {
   using S = Store<ST>;
   using E = typename S::elem;
   using R = typename S::real;
-  constexpr int EX = TX + 2, EY = TY + 2, NE = EX * EY, NT = TX * TY;
+  constexpr int EX = TX + 2, EY = TY + 2, NE = EX * EY, NI = TX * TY;
+  static_assert(NT >= NI && NT % 64 == 0, "one thread per interior cell at least");
   constexpr int EPT = (NE + NT - 1) / NT;  // extended cells per work-item in phase 1
   constexpr unsigned ES = sizeof(E);
   __shared__ R lds[24 * NE];
@@ -137,7 +141,7 @@ __global__ void __launch_bounds__(TX * TY)
   // FP32 arithmetic the pull for plane s+1 is issued before plane s is processed, so its HBM
   // latency overlaps the collisions, the barriers and the phase-2 stores of plane s; FP64
   // and 1024-wide groups have no registers to spare for it.
-  constexpr bool kPrefetch = sizeof(R) == 4 && NT <= 512;
+  constexpr bool kPrefetch = sizeof(R) == 4 && (NT <= 512 || EPT == 1);
   R nxt[EPT][19];
   auto pull_plane = [&](int s) {
     const int zs = tb::wrap(s, nz);
@@ -195,7 +199,7 @@ __global__ void __launch_bounds__(TX * TY)
     }
     __syncthreads();
     // ---- phase 2: t+2 on plane s−1, interior tile ----
-    if (s >= z0 + 1) {
+    if (s >= z0 + 1 && tid < NI) {
       const R* const Br = B + ((u - 2) & 1) * 5 * NE;
       const int ix = tid % TX, iy = tid / TX;
       R fp[19], fn[19];
@@ -430,7 +434,9 @@ __global__ void __launch_bounds__(NT)
 // ---- dispatch ---------------------------------------------------------------------------
 // Tile variants (TX × TY interior cells per work-group).  Two steps per launch: 1 = 32×8,
 // 2 = 64×4, 3 = 64×8, 4 = 32×16, 5 = 64×16, not for FP64 (the larger the tile, the smaller
-// the share of halo recomputation: (TX+2)(TY+2)/(TX·TY) = 1.55, 1.55, 1.29, 1.32, 1.16).
+// the share of halo recomputation: (TX+2)(TY+2)/(TX·TY) = 1.55, 1.55, 1.29, 1.32, 1.16),
+// with one thread per interior cell; 12 = 64×8 (704 threads), 13 = 32×16 (640), 14 = 32×8
+// (384) with one thread per extended cell.
 // Three steps per launch: 6 = 64×8, 7 = 32×8, 8 = 32×16 with one thread per interior cell;
 // 9 = 32×8 (448 threads), 10 = 32×16 (768), 11 = 64×8 (832) with one thread per level-1
 // cell; FP64 only 7 and 9 (LDS: 24 values per cell of both levels, 141 / 74 / 128 KB in
@@ -449,22 +455,28 @@ bool step2_tile(int tile, int st, int& TX, int& TY) {
     case 9: TX = 32; TY = 8; return true;
     case 10: TX = 32; TY = 16; return st != KMI_STORE_F64;
     case 11: TX = 64; TY = 8; return st != KMI_STORE_F64;
+    case 12: TX = 64; TY = 8; return true;
+    case 13: TX = 32; TY = 16; return true;
+    case 14: TX = 32; TY = 8; return true;
     default: return false;
   }
 }
 
-int temporal_span(int tile) { return tile >= 6 ? 3 : (tile >= 1 ? 2 : 1); }
+int temporal_span(int tile) {
+  if (tile >= 6 && tile <= 11) return 3;
+  return tile >= 1 ? 2 : 1;
+}
 
-template <int ST, int COLL, int TX, int TY>
+template <int ST, int COLL, int TX, int TY, int NT = TX * TY>
 static hipError_t launch2(bool nts, const StepArgs& a, int zchunks, hipStream_t s) {
   const int ntx = a.nx / TX, nty = a.ny / TY;
   const int zlen = (a.nz + zchunks - 1) / zchunks;
   const int nzc = (a.nz + zlen - 1) / zlen;
   const int ntiles = ntx * nty * nzc;
   const int per_xcd = (ntiles + 7) / 8;
-  const dim3 g(static_cast<unsigned>(8 * per_xcd)), b(static_cast<unsigned>(TX * TY));
-  if (nts) hipLaunchKernelGGL((k_step2_d3q19<ST, COLL, TX, TY, true>), g, b, 0, s, a, zlen, ntx, nty, ntiles, per_xcd);
-  else hipLaunchKernelGGL((k_step2_d3q19<ST, COLL, TX, TY, false>), g, b, 0, s, a, zlen, ntx, nty, ntiles, per_xcd);
+  const dim3 g(static_cast<unsigned>(8 * per_xcd)), b(static_cast<unsigned>(NT));
+  if (nts) hipLaunchKernelGGL((k_step2_d3q19<ST, COLL, TX, TY, NT, true>), g, b, 0, s, a, zlen, ntx, nty, ntiles, per_xcd);
+  else hipLaunchKernelGGL((k_step2_d3q19<ST, COLL, TX, TY, NT, false>), g, b, 0, s, a, zlen, ntx, nty, ntiles, per_xcd);
   return hipGetLastError();
 }
 
@@ -502,6 +514,9 @@ static hipError_t launch2_tile(int tile, bool nts, const StepArgs& a, int zchunk
     case 2: return launch2<ST, COLL, 64, 4>(nts, a, zchunks, s);
     case 3: return launch2<ST, COLL, 64, 8>(nts, a, zchunks, s);
     case 4: return launch2<ST, COLL, 32, 16>(nts, a, zchunks, s);
+    case 12: return launch2<ST, COLL, 64, 8, 704>(nts, a, zchunks, s);
+    case 13: return launch2<ST, COLL, 32, 16, 640>(nts, a, zchunks, s);
+    case 14: return launch2<ST, COLL, 32, 8, 384>(nts, a, zchunks, s);
     case 5:
       if constexpr (ST != KMI_STORE_F64) return launch2<ST, COLL, 64, 16>(nts, a, zchunks, s);
       else return hipErrorInvalidValue;

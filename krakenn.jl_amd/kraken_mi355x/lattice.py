@@ -287,7 +287,8 @@ class Lattice:
 
     def set_temporal(self, tile, zchunks=1):
         """Temporal blocking.  Fully periodic D3Q19 BGK/TRT lattice: two steps per launch
-        (tile 1: 32×8, 2: 64×4, 3: 64×8, 4: 32×16, 5: 64×16 cells) or three (6: 64×8, 7: 32×8,
+        (tile 1: 32×8, 2: 64×4, 3: 64×8, 4: 32×16, 5: 64×16 cells with a thread per cell;
+        12: 64×8, 13: 32×16, 14: 32×8 with a thread per cell of the tile and its ring) or three (6: 64×8, 7: 32×8,
         8: 32×16 with a thread per cell; 9: 32×8, 10: 32×16, 11: 64×8 with a thread per cell
         of the tile and its two-cell ring; FP64 only 7 and 9).  D2Q9 lattice without thermal populations, cut links or
         per-node faces: 1: 28×28 two steps, 2: 24×24 four, 3: 56×8 four, 4: 16×16 eight,

@@ -18,6 +18,7 @@ def test_launch_span():
     assert bench.launch_span({}) == 1
     assert all(bench.launch_span({"tile": t}) == 2 for t in range(1, 6))
     assert all(bench.launch_span({"tile": t}) == 3 for t in range(6, 12))
+    assert all(bench.launch_span({"tile": t}) == 2 for t in (1, 5, 12, 14))
 
 
 def test_decoder_flop_count():

@@ -275,7 +275,8 @@ def test_invalid_descriptor_raises(kmi, backend):
 @pytest.mark.parametrize("collision", ["trt", "bgk"])
 def test_two_step_launches_bit_identical(kmi, backend, storage, collision):
     """Temporal blocking (two steps per launch, t+1 populations in LDS) reproduces single-step
-    launches bit for bit, for every tile and z split, odd step counts included."""
+    launches bit for bit, for every tile and thread count and z split, odd step counts
+    included."""
     shape = (64, 16, 12)
     per = {f: {"kind": "periodic"} for f in ("west", "east", "south", "north", "bottom", "top")}
     desc = kmi.make_desc("d3q19", *shape, 0.02, storage=storage, collision=collision, faces=per)
@@ -287,6 +288,7 @@ def test_two_step_launches_bit_identical(kmi, backend, storage, collision):
         lat.step(steps)
         ref = lat.download()
     modes = [(1, 1), (1, 3), (2, 2), (2, 12), (3, 5), (4, 1)] + ([(5, 2)] if storage != "f64" else [])
+    modes += [(12, 1), (12, 4), (13, 2), (14, 1), (14, 12)]  # one thread per extended cell
     for tile, zc in modes:
         with kmi.Lattice(desc, backend) as lat:
             lat.upload(f)
