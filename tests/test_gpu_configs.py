@@ -67,8 +67,8 @@ def test_config2_tgv_256_fp64_bitwise_and_reduced_precision(kmi, po, backend):
     N, nu, U0 = 256, 0.01, 0.05
     fields = kmi.taylor_green_3d_fields(N, U0)
     d64 = kmi.taylor_green_3d_desc(N, nu, "f64")
-    # single-step, two-step (tile 3) and three-step (tile 7) launches at full size
-    for tile, zc, nst in ((0, 1, 2), (3, 2, 2), (7, 2, 3)):
+    # single-step, two-step (tiles 3 and 12) and three-step (tiles 7 and 9) launches at full size
+    for tile, zc, nst in ((0, 1, 2), (3, 2, 2), (12, 2, 2), (7, 2, 3), (9, 2, 3)):
         print(f"[config2] TGV 256^3 FP64 full-size parity, temporal tile {tile}, {nst} steps", flush=True)
         f0 = po.equilibrium(*fields)
         with kmi.Lattice(d64, backend) as lat:
@@ -80,6 +80,21 @@ def test_config2_tgv_256_fp64_bitwise_and_reduced_precision(kmi, po, backend):
         po.run(d64, f0, None, nst)
         assert np.array_equal(fg, f0), tile
         del fg, f0
+    # the benchmark's launches (F32S three steps on 64x8 tiles, 832 threads; F16S two steps on
+    # 64x8 tiles, 704 threads) against single-step launches of the same storage, 6 steps
+    for st, tile in (("f32s", 11), ("f16s", 12)):
+        print(f"[config2] TGV 256^3 {st} tile {tile} against single steps", flush=True)
+        d = kmi.taylor_green_3d_desc(N, nu, st)
+        out = []
+        for t in (0, tile):
+            with kmi.Lattice(d, backend) as lat:
+                lat.init_equilibrium(*fields)
+                if t:
+                    lat.set_temporal(t, 2)
+                lat.step(6)
+                out.append(lat.download_raw())  # the stored elements verbatim
+        assert np.array_equal(out[0], out[1]), (st, tile)
+        del out
     steps = 1000
     res = {}
     for st in ("f64", "f32", "f32s", "f16s"):
