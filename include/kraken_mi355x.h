@@ -328,20 +328,24 @@ int kmi_lattice_device_bytes(const kmi_lattice* lat, int64_t* bytes);
  * population stores on/off.  Results do not depend on these settings. */
 int kmi_lattice_set_launch(kmi_lattice* lat, int32_t bx, int32_t by, int32_t nontemporal);
 int kmi_lattice_get_launch(const kmi_lattice* lat, int32_t* bx, int32_t* by, int32_t* nontemporal);
-/* Two or three steps per launch (temporal blocking) for fully periodic D3Q19 BGK/TRT
- * lattices without mask and with nz >= 4: the intermediate populations stay in LDS, so HBM
- * sees one read and one write of the field per launch.  tile: 0 off; two steps: 1 = 32x8,
- * 2 = 64x4, 3 = 64x8, 4 = 32x16, 5 = 64x16 (not F64) with one work-item per cell, and
- * 12 = 64x8, 13 = 32x16, 14 = 32x8 with one work-item per cell of the tile plus its
- * one-cell ring; three steps: 6 = 64x8 (not F64),
- * 7 = 32x8, 8 = 32x16 (not F64) cells per work-group with one work-item per cell, and
- * 9 = 32x8, 10 = 32x16 (not F64), 11 = 64x8 (not F64) with one work-item per cell of the
- * tile plus its two-cell ring; nx and ny must be multiples of the tile.  zchunks: z-ranges per tile column (1..nz).  D2Q9 lattices without thermal
- * populations, cut links or per-node faces (at least the tile plus its ring on each axis):
- * tile 1 = 28x28 two steps, 2 = 24x24 four, 3 = 56x8 four, 4 = 16x16 eight, 5 = 60x12 two
- * (the tile and a ring of that many cells are advanced in LDS), zchunks 1.  Results are
- * bit-identical to single-step launches; step counts that are not a multiple of the span
- * end with single steps. */
+/* Several steps per launch (temporal blocking); the intermediate populations stay in LDS,
+ * so HBM sees one read and one write of the field per launch.  tile 0: off.
+ * Fully periodic D3Q19 BGK/TRT lattices without mask and with nz >= 4 (nx and ny multiples
+ * of the tile; zchunks: z-ranges per tile column, 1..nz):
+ *   two steps, one work-item per interior cell: 1 = 32x8, 2 = 64x4, 3 = 64x8, 4 = 32x16,
+ *     5 = 64x16 (not F64);
+ *   two steps, one work-item per cell of the tile and its one-cell ring: 12 = 64x8,
+ *     13 = 32x16, 14 = 32x8;
+ *   three steps, one work-item per interior cell: 6 = 64x8 (not F64), 7 = 32x8,
+ *     8 = 32x16 (not F64);
+ *   three steps, one work-item per cell of the tile and its two-cell ring: 9 = 32x8,
+ *     10 = 32x16 (not F64), 11 = 64x8 (not F64).
+ * D2Q9 lattices without cut links or per-node faces (thermal: not F64), at least the tile
+ * plus its ring on each axis, zchunks 1; the tile and a ring of as many cells as steps are
+ * advanced in LDS, one work-item per cell: 1 = 28x28 two steps, 2 = 24x24 four, 3 = 56x8
+ * four, 4 = 16x16 eight, 5 = 60x12 two, 6 = 8x8 four, 7 = 16x16 four, 8 = 8x8 eight.
+ * Results are bit-identical to single-step launches; step counts that are not a multiple
+ * of the span end with single steps. */
 int kmi_lattice_set_temporal(kmi_lattice* lat, int32_t tile, int32_t zchunks);
 int kmi_lattice_get_temporal(const kmi_lattice* lat, int32_t* tile, int32_t* zchunks);
 /* Time every candidate shape × store policy (and two-step variant, where it applies) on

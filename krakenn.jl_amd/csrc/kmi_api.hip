@@ -1162,11 +1162,13 @@ int kmi_lattice_set_temporal(kmi_lattice* L, int32_t tile, int32_t zchunks) {
     return fail(L->ctx, KMI_ERR_UNSUPPORTED,
                 "kmi_lattice_set_temporal: multi-step launches need a fully periodic D3Q19 "
                 "BGK/TRT lattice without mask, nz >= 4, whose nx, ny are multiples of the tile "
-                "(two steps 1: 32x8, 2: 64x4, 3: 64x8, 4: 32x16, 5: 64x16 (not FP64); "
-                "three steps 6: 64x8 (not FP64), 7: 32x8, 8: 32x16 (not FP64)), or a D2Q9 "
-                "lattice without cut links or per-node faces (thermal: not FP64), at least "
-                "the tile plus its ring on each axis (1: 28x28 2 steps, 2: 24x24 4 steps, "
-                "3: 56x8 4 steps, 4: 16x16 8 steps, 5: 60x12 2 steps)");
+                "(two steps 1: 32x8, 2: 64x4, 3: 64x8, 4: 32x16, 5: 64x16 (not FP64), "
+                "12: 64x8, 13: 32x16, 14: 32x8; three steps 6: 64x8 (not FP64), 7: 32x8, "
+                "8: 32x16 (not FP64), 9: 32x8, 10: 32x16 (not FP64), 11: 64x8 (not FP64)), "
+                "or a D2Q9 lattice without cut links or per-node faces (thermal: not FP64), "
+                "at least the tile plus its ring on each axis (1: 28x28 2 steps, 2: 24x24 4 "
+                "steps, 3: 56x8 4 steps, 4: 16x16 8 steps, 5: 60x12 2 steps, 6: 8x8 4 steps, "
+                "7: 16x16 4 steps, 8: 8x8 8 steps)");
   if (tile != 0 && (zchunks < 1 || zchunks > L->args.nz))
     return fail(L->ctx, KMI_ERR_ARG, "kmi_lattice_set_temporal: zchunks must be in 1..nz");
   KMI_HIP(L->ctx, hipSetDevice(L->ctx->device));

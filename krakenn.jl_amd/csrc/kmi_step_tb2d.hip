@@ -202,8 +202,10 @@ __global__ void __launch_bounds__((TW + 2 * K) * (TH + 2 * K)) k_step_tb2d(const
 }
 
 // ---- dispatch ---------------------------------------------------------------------------
-// Variants (regions of 1024 cells): 1 = 28×28 tile, 2 steps; 2 = 24×24, 4 steps;
-// 3 = 56×8, 4 steps; 4 = 16×16, 8 steps; 5 = 60×12, 2 steps.
+// Variants: regions of 1024 cells, 1 = 28×28 tile, 2 steps; 2 = 24×24, 4 steps;
+// 3 = 56×8, 4 steps; 4 = 16×16, 8 steps; 5 = 60×12, 2 steps; smaller regions, which spread
+// a small lattice over more CUs: 6 = 8×8, 4 steps (256 cells); 7 = 16×16, 4 steps (576);
+// 8 = 8×8, 8 steps (576).
 bool tb2d_variant(int tile, int& TW, int& TH, int& K) {
   switch (tile) {
     case 1: TW = 28; TH = 28; K = 2; return true;
@@ -211,6 +213,9 @@ bool tb2d_variant(int tile, int& TW, int& TH, int& K) {
     case 3: TW = 56; TH = 8; K = 4; return true;
     case 4: TW = 16; TH = 16; K = 8; return true;
     case 5: TW = 60; TH = 12; K = 2; return true;
+    case 6: TW = 8; TH = 8; K = 4; return true;
+    case 7: TW = 16; TH = 16; K = 4; return true;
+    case 8: TW = 8; TH = 8; K = 8; return true;
     default: return false;
   }
 }
@@ -239,6 +244,9 @@ static hipError_t launch_tb_tile(int tile, const StepArgs& a, bool th, hipStream
     case 3: return launch_tb<ST, COLL, 56, 8, 4>(a, th, s);
     case 4: return launch_tb<ST, COLL, 16, 16, 8>(a, th, s);
     case 5: return launch_tb<ST, COLL, 60, 12, 2>(a, th, s);
+    case 6: return launch_tb<ST, COLL, 8, 8, 4>(a, th, s);
+    case 7: return launch_tb<ST, COLL, 16, 16, 4>(a, th, s);
+    case 8: return launch_tb<ST, COLL, 8, 8, 8>(a, th, s);
     default: return hipErrorInvalidValue;
   }
 }

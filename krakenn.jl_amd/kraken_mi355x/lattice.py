@@ -292,7 +292,7 @@ class Lattice:
         8: 32×16 with a thread per cell; 9: 32×8, 10: 32×16, 11: 64×8 with a thread per cell
         of the tile and its two-cell ring; FP64 only 7 and 9).  D2Q9 lattice without thermal populations, cut links or
         per-node faces: 1: 28×28 two steps, 2: 24×24 four, 3: 56×8 four, 4: 16×16 eight,
-        5: 60×12 two.  0: off.  Bit-identical to single steps."""
+        5: 60×12 two, 6: 8×8 four, 7: 16×16 four, 8: 8×8 eight.  0: off.  Bit-identical to single steps."""
         self._chk(L.load().kmi_lattice_set_temporal(self._h, int(tile), int(zchunks)))
 
     @property

@@ -47,11 +47,13 @@ def test_config1_cavity_fp64_full_run_bitwise(kmi, po, backend):
     assert np.array_equal(fg, fr)
     mr = po.moments(fr)
     assert np.array_equal(m["ux"], mr["ux"]) and np.array_equal(m["rho"], mr["rho"])
-    # the same full run with eight steps per launch (tile and ring in LDS)
-    with kmi.Lattice(d, backend) as lat:
-        lat.set_temporal(4, 1)
-        lat.step(steps)
-        assert np.array_equal(lat.download(), fr)
+    # the same full run with eight (16x16 tile) and four (8x8 tile) steps per launch, the
+    # tile and its ring in LDS
+    for tile in (4, 6):
+        with kmi.Lattice(d, backend) as lat:
+            lat.set_temporal(tile, 1)
+            lat.step(steps)
+            assert np.array_equal(lat.download(), fr), tile
     # the Ladd-lid variant (BASELINE's "moving-lid bounce-back") against its oracle chain
     d2 = kmi.cavity_desc(N, nu, ul, lid="ladd")
     with kmi.Lattice(d2, backend) as lat:

@@ -398,7 +398,7 @@ def test_d2q9_multi_step_launches_bit_identical(kmi, backend, storage):
             lat.upload(f)
             lat.step(steps)
             ref = lat.download()
-        for tile in (1, 2, 3, 4, 5):
+        for tile in (1, 2, 3, 4, 5, 6, 7, 8):
             with kmi.Lattice(desc, backend) as lat:
                 if mask is not None:
                     lat.set_mask(mask)
@@ -455,7 +455,7 @@ def test_d2q9_thermal_multi_step_launches_bit_identical(kmi, backend, storage):
     for name, desc, T0 in cases:
         print(f"[tb2d thermal] {storage} {name}", flush=True)
         outs = []
-        for tile in (0, 1, 2, 3, 4, 5):
+        for tile in (0, 1, 2, 3, 4, 5, 6, 7, 8):
             with kmi.Lattice(desc, backend) as lat:
                 if T0 is not None:
                     lat.init_temperature(T0)
@@ -463,7 +463,7 @@ def test_d2q9_thermal_multi_step_launches_bit_identical(kmi, backend, storage):
                     lat.set_temporal(tile, 1)
                 lat.step(steps)
                 outs.append((lat.download(), lat.download("g")))
-        for tile, (f, g) in zip((1, 2, 3, 4, 5), outs[1:]):
+        for tile, (f, g) in zip((1, 2, 3, 4, 5, 6, 7, 8), outs[1:]):
             assert np.array_equal(f, outs[0][0]) and np.array_equal(g, outs[0][1]), (name, tile)
     with kmi.Lattice(kmi.rayleigh_benard_desc(N, M, 1e4, 0.71, storage="f64")[0], backend) as lat:
         with pytest.raises(kmi.KrakenError):
