@@ -87,6 +87,112 @@ struct LoadConvWeightT {
   }
 };
 
+// ---- four consecutive elements (i, j..j+3), 0 at j+e >= jend, packed into a uint2 ---------
+KMI_DEV uint2 pack4(const uint16_t (&v)[4]) {
+  return make_uint2(uint32_t(v[0]) | (uint32_t(v[1]) << 16), uint32_t(v[2]) | (uint32_t(v[3]) << 16));
+}
+KMI_DEV bool aligned8(const uint16_t* p) { return (reinterpret_cast<uintptr_t>(p) & 7u) == 0; }
+KMI_DEV uint2 ld8(const uint16_t* p) { return *reinterpret_cast<const uint2*>(p); }
+
+// Generic: four calls of the loader.
+template <class LD>
+KMI_DEV uint2 load4(const LD& ld, int i, int j, int jend) {
+  uint16_t v[4];
+  for (int e = 0; e < 4; ++e) v[e] = j + e < jend ? ld(i, j + e) : uint16_t(0);
+  return pack4(v);
+}
+// Dense row-major operand: one 8-byte load when the four elements are contiguous.
+KMI_DEV uint2 load4(const LoadDense& ld, int i, int j, int jend) {
+  if (i >= ld.n0) return make_uint2(0u, 0u);
+  const uint16_t* q = ld.p + (long long)i * ld.s0 + (long long)j * ld.s1;
+  if (ld.s1 == 1 && j + 3 < min(jend, ld.n1) && aligned8(q)) return ld8(q);
+  uint16_t v[4];
+  for (int e = 0; e < 4; ++e) v[e] = (j + e < jend && j + e < ld.n1) ? q[(long long)e * ld.s1] : uint16_t(0);
+  return pack4(v);
+}
+// Forward conv operand: the pixel m is decoded once; four consecutive k = (tap, c..c+3) of
+// one tap are four consecutive channels of one input pixel (C a multiple of 4).
+KMI_DEV uint2 load4(const LoadConvUp<false>& ld, int m, int k, int jend) {
+  const int Ho = 2 * ld.Hi, C = ld.C;
+  if (m >= ld.B * Ho * Ho) return make_uint2(0u, 0u);
+  const int b = m / (Ho * Ho), r = m % (Ho * Ho), y = r / Ho, x = r % Ho;
+  const uint16_t* base = ld.in + (long long)b * ld.Hi * ld.Hi * C;
+  if (C % 4 == 0 && k % 4 == 0 && k + 3 < min(jend, 9 * C)) {
+    const int t = k / C, c = k % C, yy = y + t / 3 - 1, xx = x + t % 3 - 1;
+    if (yy < 0 || yy >= Ho || xx < 0 || xx >= Ho) return make_uint2(0u, 0u);
+    const uint16_t* q = base + ((long long)(yy >> 1) * ld.Hi + (xx >> 1)) * C + c;
+    if (aligned8(q)) return ld8(q);
+  }
+  uint16_t v[4];
+  for (int e = 0; e < 4; ++e) {
+    const int kk = k + e;
+    v[e] = 0;
+    if (kk >= jend || kk >= 9 * C) continue;
+    const int t = kk / C, c = kk % C, yy = y + t / 3 - 1, xx = x + t % 3 - 1;
+    if (yy < 0 || yy >= Ho || xx < 0 || xx >= Ho) continue;
+    v[e] = base[((long long)(yy >> 1) * ld.Hi + (xx >> 1)) * C + c];
+  }
+  return pack4(v);
+}
+// Data-gradient operand: four consecutive k = (d, tap, o..o+3) are four consecutive output
+// channels of one output pixel (Co a multiple of 4).
+KMI_DEV uint2 load4(const LoadConvUpGradIn& ld, int m, int k, int jend) {
+  const int Hi = ld.Hi, Ho = 2 * Hi, Co = ld.Co;
+  if (m >= ld.B * Hi * Hi) return make_uint2(0u, 0u);
+  const int b = m / (Hi * Hi), r = m % (Hi * Hi), yi = r / Hi, xi = r % Hi;
+  const uint16_t* base = ld.dz + (long long)b * Ho * Ho * Co;
+  if (Co % 4 == 0 && k % 4 == 0 && k + 3 < min(jend, 36 * Co)) {
+    const int o = k % Co, s = k / Co, t = s % 9, d = s / 9;
+    const int y = 2 * yi + (d >> 1) - t / 3 + 1, x = 2 * xi + (d & 1) - t % 3 + 1;
+    if (y < 0 || y >= Ho || x < 0 || x >= Ho) return make_uint2(0u, 0u);
+    const uint16_t* q = base + ((long long)y * Ho + x) * Co + o;
+    if (aligned8(q)) return ld8(q);
+  }
+  uint16_t v[4];
+  for (int e = 0; e < 4; ++e) {
+    const int kk = k + e;
+    v[e] = 0;
+    if (kk >= jend || kk >= 36 * Co) continue;
+    const int o = kk % Co, s = kk / Co, t = s % 9, d = s / 9;
+    const int y = 2 * yi + (d >> 1) - t / 3 + 1, x = 2 * xi + (d & 1) - t % 3 + 1;
+    if (y < 0 || y >= Ho || x < 0 || x >= Ho) continue;
+    v[e] = base[((long long)y * Ho + x) * Co + o];
+  }
+  return pack4(v);
+}
+
+// Weight-gradient operand A(k, m..m+3): the tap and channel are decoded once; four
+// consecutive pixels (m a multiple of 4, Ho a multiple of 4) lie on one output row.
+KMI_DEV uint2 load4(const LoadConvUp<true>& ld, int k, int m, int jend) {
+  const int Hi = ld.Hi, Ho = 2 * Hi, C = ld.C, HW = Ho * Ho;
+  if (k >= 9 * C) return make_uint2(0u, 0u);
+  const int t = k / C, c = k % C, dy = t / 3 - 1, dx = t % 3 - 1;
+  uint16_t v[4] = {0, 0, 0, 0};
+  if (m % 4 == 0 && Ho % 4 == 0 && m + 3 < min(jend, ld.B * HW)) {
+    const int b = m / HW, r = m % HW, y = r / Ho, x = r % Ho, yy = y + dy;
+    if (yy >= 0 && yy < Ho) {
+      const uint16_t* row = ld.in + ((long long)b * Hi + (yy >> 1)) * Hi * C + c;
+      for (int e = 0; e < 4; ++e) {
+        const int xx = x + e + dx;
+        if (xx >= 0 && xx < Ho) v[e] = row[(long long)(xx >> 1) * C];
+      }
+    }
+    return pack4(v);
+  }
+  for (int e = 0; e < 4; ++e) v[e] = m + e < jend ? ld(k, m + e) : uint16_t(0);
+  return pack4(v);
+}
+// Data-gradient operand B(k, n..n+3) = W[(tap, n), o]: tap and o decoded once.
+KMI_DEV uint2 load4(const LoadConvWeightT& ld, int k, int n, int jend) {
+  if (k >= 36 * ld.Co) return make_uint2(0u, 0u);
+  const int o = k % ld.Co, t = (k / ld.Co) % 9;
+  const uint16_t* q = ld.w + (long long)t * ld.Ci * ld.Co + o;
+  uint16_t v[4];
+  for (int e = 0; e < 4; ++e)
+    v[e] = (n + e < jend && n + e < ld.Ci) ? q[(long long)(n + e) * ld.Co] : uint16_t(0);
+  return pack4(v);
+}
+
 // ---- split-K GEMM: P[z][m][n] = Σ_{k in split z} A(m,k) B(k,n) --------------------------------
 // 256 threads = 4 waves; block tile 64×64; each wave 32×32 = 2×2 MFMA 16×16×16 tiles.
 // MFMA operand layout (lane l): A row l%16, k = 4(l/16)..+3; B col l%16, same k;
@@ -94,8 +200,8 @@ struct LoadConvWeightT {
 template <class LA, class LB>
 __global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, int kchunk, LA la, LB lb,
                                               float* __restrict__ out) {
-  __shared__ uint16_t As[64][16 + 4];
-  __shared__ uint16_t Bs[16][64 + 4];
+  __shared__ __align__(8) uint16_t As[64][16 + 4];  // rows of 40 B: 8-byte aligned quads
+  __shared__ __align__(8) uint16_t Bs[16][64 + 4];  // rows of 136 B
   const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
   const int kbeg = blockIdx.z * kchunk, kend = min(K, kbeg + kchunk);
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -107,16 +213,13 @@ __global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, int kchunk, L
     // A tile 64×16: thread → row t/4, 4 consecutive k
     {
       const int r = t >> 2, kk = (t & 3) * 4;
-      for (int e = 0; e < 4; ++e) {
-        const int k = k0 + kk + e;
-        As[r][kk + e] = (k < kend) ? la(m0 + r, k) : uint16_t(0);
-      }
+      *reinterpret_cast<uint2*>(&As[r][kk]) = load4(la, m0 + r, k0 + kk, kend);
     }
     // B tile 16×64: thread → k row t/16, 4 consecutive n
     {
       const int kr = t >> 4, nn = (t & 15) * 4;
       const int k = k0 + kr;
-      for (int e = 0; e < 4; ++e) Bs[kr][nn + e] = (k < kend) ? lb(k, n0 + nn + e) : uint16_t(0);
+      *reinterpret_cast<uint2*>(&Bs[kr][nn]) = k < kend ? load4(lb, k, n0 + nn, 1 << 30) : make_uint2(0u, 0u);
     }
     __syncthreads();
     const int kq = (lane >> 4) * 4;
