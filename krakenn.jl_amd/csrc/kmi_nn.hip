@@ -268,6 +268,26 @@ __global__ void k_epilogue(const float* __restrict__ P, int splits, long long MN
   if (Hb) Hb[i] = f2bf(h);
 }
 
+// Σ over split-K partials of products with a long reduction and a small output (weight
+// gradients): 16 outputs × 16 split lanes per work-group, lane g summing splits g, g+16, …,
+// the lane sums added in lane order (a fixed order: the result is deterministic).
+__global__ void __launch_bounds__(256) k_split_sum(const float* __restrict__ P, int splits, long long MN,
+                                                    float* __restrict__ out) {
+  __shared__ float part[16][17];
+  const int o = threadIdx.x & 15, g = threadIdx.x >> 4;
+  const long long i = (long long)blockIdx.x * 16 + o;
+  float z = 0.f;
+  if (i < MN)
+    for (int sp = g; sp < splits; sp += 16) z += P[(long long)sp * MN + i];
+  part[g][o] = z;
+  __syncthreads();
+  if (g == 0 && i < MN) {
+    float t = 0.f;
+    for (int k = 0; k < 16; ++k) t += part[k][o];
+    out[i] = t;
+  }
+}
+
 // dZ = dH ⊙ SiLU'(Z) (or dH when !silu); FP32 and BF16 copies.  nearest-×2 folding of the
 // up-sampled data gradient happens inside LoadConvUpGradIn, so dH is already per input pixel.
 __global__ void k_act_grad(const float* __restrict__ dH, const float* __restrict__ Z, long long n,
@@ -587,8 +607,7 @@ static int nn_backward(kmi_nn* N, int B) {
       kchunk = (kchunk + 15) / 16 * 16;
       const int splits = (Mo + kchunk - 1) / kchunk;
       const long long mn = (long long)Mw * S.Co;
-      hipLaunchKernelGGL(k_epilogue, g1(mn), dim3(256), 0, s, N->P, splits, mn, S.Co, (const float*)nullptr, 0,
-                         g + S.ow, (float*)nullptr, (uint16_t*)nullptr);
+      hipLaunchKernelGGL(k_split_sum, dim3(unsigned((mn + 15) / 16)), dim3(256), 0, s, N->P, splits, mn, g + S.ow);
     }
     // data gradient into the stage input (per input pixel, up-sampling folded in)
     const int Mi = B * S.Hi * S.Hi;
@@ -610,8 +629,7 @@ static int nn_backward(kmi_nn* N, int B) {
     int kchunk = ((B + sp - 1) / sp + 15) / 16 * 16;
     const int splits = (B + kchunk - 1) / kchunk;
     const long long mn = (long long)d.hidden * N->L;
-    hipLaunchKernelGGL(k_epilogue, g1(mn), dim3(256), 0, s, N->P, splits, mn, N->L, (const float*)nullptr, 0,
-                       g + N->oW2, (float*)nullptr, (uint16_t*)nullptr);
+    hipLaunchKernelGGL(k_split_sum, dim3(unsigned((mn + 15) / 16)), dim3(256), 0, s, N->P, splits, mn, g + N->oW2);
   }
   // dH1 = dZ2 W2ᵀ (long reduction over the latent: split-K)
   const long long nh = (long long)B * d.hidden;
@@ -621,8 +639,7 @@ static int nn_backward(kmi_nn* N, int B) {
                   LoadDense{wb + N->oW2, 1, N->L, N->L, d.hidden}, N->P, sp, s));
     int kchunk = ((N->L + sp - 1) / sp + 15) / 16 * 16;
     const int splits = (N->L + kchunk - 1) / kchunk;
-    hipLaunchKernelGGL(k_epilogue, g1(nh), dim3(256), 0, s, N->P, splits, nh, d.hidden, (const float*)nullptr, 0,
-                       N->dh, (float*)nullptr, (uint16_t*)nullptr);
+    hipLaunchKernelGGL(k_split_sum, dim3(unsigned((nh + 15) / 16)), dim3(256), 0, s, N->P, splits, nh, N->dh);
   }
   hipLaunchKernelGGL(k_act_grad, g1(nh), dim3(256), 0, s, N->dh, N->z1, nh, 1, N->dz, N->dzb);
   // dense 1
