@@ -209,19 +209,23 @@ __global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, int kchunk, L
   v4f acc[2][2];
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+  // A tile 64×16: thread → row t/4, 4 consecutive k; B tile 16×64: thread → k row t/16,
+  // 4 consecutive n.  The quads of K-step k0+16 are loaded into registers while the MFMAs of
+  // K-step k0 run, so the operand latency overlaps the products.
+  const int ar = t >> 2, ak = (t & 3) * 4, bk = t >> 4, bn = (t & 15) * 4;
+  auto fetch_a = [&](int k0) { return load4(la, m0 + ar, k0 + ak, kend); };
+  auto fetch_b = [&](int k0) {
+    return k0 + bk < kend ? load4(lb, k0 + bk, n0 + bn, 1 << 30) : make_uint2(0u, 0u);
+  };
+  uint2 qa = fetch_a(kbeg), qb = fetch_b(kbeg);
   for (int k0 = kbeg; k0 < kend; k0 += 16) {
-    // A tile 64×16: thread → row t/4, 4 consecutive k
-    {
-      const int r = t >> 2, kk = (t & 3) * 4;
-      *reinterpret_cast<uint2*>(&As[r][kk]) = load4(la, m0 + r, k0 + kk, kend);
-    }
-    // B tile 16×64: thread → k row t/16, 4 consecutive n
-    {
-      const int kr = t >> 4, nn = (t & 15) * 4;
-      const int k = k0 + kr;
-      *reinterpret_cast<uint2*>(&Bs[kr][nn]) = k < kend ? load4(lb, k, n0 + nn, 1 << 30) : make_uint2(0u, 0u);
-    }
+    *reinterpret_cast<uint2*>(&As[ar][ak]) = qa;
+    *reinterpret_cast<uint2*>(&Bs[bk][bn]) = qb;
     __syncthreads();
+    if (k0 + 16 < kend) {
+      qa = fetch_a(k0 + 16);
+      qb = fetch_b(k0 + 16);
+    }
     const int kq = (lane >> 4) * 4;
     for (int a = 0; a < 2; ++a) {
       v4s af;
