@@ -1,7 +1,7 @@
 """Batched lid-driven cavities (kmi_batch): hundreds of uneven D2Q9 cavities per launch.
 
 The reference runs one cavity per driver call (run_cavity_2d, src/drivers/basic.jl:112-144;
-the MRT loop of test/test_mrt.jl:57-63).  Here every cavity of the ensemble advances in the
+the MRT loop of test/test_mrt.jl:19-25).  Here every cavity of the ensemble advances in the
 same single launch per step, each checked for the one-step residual r_u
 (src/drivers/gnf_periodic_steady.jl:186-196) and retired once converged or diverged.
 """

@@ -5,7 +5,8 @@ Three layers, as in the reference:
 
 - ``StateSnapshot``: a plain host image of a solver state (arrays, histories, scalars and
   four configuration dictionaries) with the reference's content rules
-  (``validate_content``, ``check_finite``, ``check_compatible``; state.jl:58-215, 367-445);
+  (``validate_content``, ``check_finite``, ``check_compatible``: the contract of
+  state.jl:58-215, 367-445, written here as a table of rules);
 - ``write_checkpoint`` / ``read_checkpoint`` / ``checkpoint_info``: one HDF5 file per
   checkpoint in the reference's container layout, version 1 (checkpoint_hdf5.jl:23-257):
   root attributes ``container_version, schema_version, solver, cycle, kraken_version,
@@ -76,81 +77,107 @@ class StateSnapshot:
         validate_content(self)
 
 
-def _check_array_name(cls, name):
-    segs = name.split("/")
-    ok = (isinstance(name, str) and name != "" and not any(ord(ch) < 32 or ord(ch) == 127 for ch in name)
-          and all(s not in ("", ".", "..") for s in segs))
-    if not ok:
-        raise CheckpointError(f"{cls}: invalid name {name!r} (non-empty `/`-separated segments required)")
+# A snapshot entry is checked against a table of rules.  Each rule names the groups it
+# covers, a test that is true when the entry breaks it, and the reason given in the error
+# ("<group>/<name>: <reason>").  The contract the rules encode is the reference's (allowed
+# element and scalar types, the reserved attribute name, path-like array names that may not
+# double as group prefixes, vectors only in `series`, None only in the configuration
+# classes); the table form and the wording are this module's.
+_CONTROL_CHARS = {chr(c) for c in range(32)} | {chr(127)}
 
 
-def _check_scalar_key(cls, key):
-    ok = (isinstance(key, str) and key != "" and "/" not in key
-          and not any(ord(ch) < 32 or ord(ch) == 127 for ch in key) and key != _KEYS_ATTR)
-    if not ok:
-        raise CheckpointError(f"{cls}: invalid key {key!r} (non-empty, no `/`, no control character, "
-                              "not the reserved name `keys`)")
+@dataclass(frozen=True)
+class _Rule:
+    groups: tuple
+    broken: object   # (name, value) -> bool
+    reason: object   # (name, value) -> str
 
 
-def _check_arrays(cls, d):
-    names = sorted(d)
-    for name in names:
-        _check_array_name(cls, name)
-        a = d[name]
-        if not isinstance(a, np.ndarray) or a.dtype.type not in SNAPSHOT_ELTYPES:
-            t = getattr(a, "dtype", type(a))
-            raise CheckpointError(f"{cls}/{name}: element type {t} is not allowed "
-                                  "(allowed: Float32, Float64, Int32, Int64, Bool)")
-        if a.ndim < 1:
-            raise CheckpointError(f"{cls}/{name}: zero-dimensional arrays are not allowed (use `scalars`)")
-        if cls == "series" and a.ndim != 1:
-            raise CheckpointError(f"series/{name}: must be a vector, found size {a.shape}")
-    for name in names:
-        for other in names:
-            if other.startswith(name + "/"):
-                raise CheckpointError(f"{cls}: name {name!r} is also used as a prefix by {other!r}")
+def _bad_path(name) -> bool:
+    if not isinstance(name, str) or not name or _CONTROL_CHARS & set(name):
+        return True
+    return any(seg in ("", ".", "..") for seg in name.split("/"))
 
 
-def _scalar_ok(v, allow_none):
+def _storable_scalar(v, group) -> bool:
     if v is None:
-        return allow_none
-    return isinstance(v, (str, bool, int, float, np.floating, np.integer, np.bool_)) and \
-        not isinstance(v, complex)
+        return group in SNAPSHOT_CONFIG_CLASSES
+    return isinstance(v, (str, bool, int, float, np.floating, np.integer, np.bool_))
 
 
-def _check_scalars(cls, d):
-    allow_none = cls in SNAPSHOT_CONFIG_CLASSES
-    for key, v in d.items():
-        _check_scalar_key(cls, key)
-        if not _scalar_ok(v, allow_none):
-            raise CheckpointError(f"{cls}/{key}: value of type {type(v).__name__} is not allowed "
-                                  "(allowed: String, Bool, Int, Float32, Float64"
-                                  + (", nothing" if allow_none else "") + ")")
+_ARRAY_RULES = (
+    _Rule(_ARRAY_GROUPS, lambda n, a: _bad_path(n),
+          lambda n, a: "array names are `/`-joined paths of non-empty segments other than "
+                       "`.` and `..`, without control characters"),
+    _Rule(_ARRAY_GROUPS, lambda n, a: not isinstance(a, np.ndarray) or a.dtype.type not in SNAPSHOT_ELTYPES,
+          lambda n, a: f"dtype {getattr(a, 'dtype', type(a).__name__)} cannot be stored "
+                       "(Float32, Float64, Int32, Int64 or Bool arrays only)"),
+    _Rule(_ARRAY_GROUPS, lambda n, a: a.ndim == 0,
+          lambda n, a: "a 0-d array is a scalar: put it in `scalars`"),
+    _Rule(("series",), lambda n, a: a.ndim != 1,
+          lambda n, a: f"series entries are 1-D, this one has shape {a.shape}"),
+)
+
+_SCALAR_RULES = (
+    _Rule(_SCALAR_GROUPS, lambda k, v: not isinstance(k, str) or not k or "/" in k or bool(_CONTROL_CHARS & set(k)),
+          lambda k, v: "scalar keys are single non-empty names without `/` or control characters"),
+    _Rule(_SCALAR_GROUPS, lambda k, v: k == _KEYS_ATTR,
+          lambda k, v: f"`{_KEYS_ATTR}` is the group's key-list attribute and cannot name an entry"),
+)
+
+
+def _scalar_type_rule(group):
+    extra = ", or None" if group in SNAPSHOT_CONFIG_CLASSES else ""
+    return _Rule((group,), lambda k, v: not _storable_scalar(v, group),
+                 lambda k, v: f"{type(v).__name__} is not a storable scalar (str, bool, int or float{extra})")
+
+
+def _enforce(rules, group, entries):
+    for name, value in entries.items():
+        for r in rules:
+            if group in r.groups and r.broken(name, value):
+                raise CheckpointError(f"{group}/{name}: {r.reason(name, value)}")
+
+
+def _enforce_no_prefix_clash(group, names):
+    # an array name cannot also be the group that holds another array ("a" and "a/b")
+    ordered = sorted(names)
+    for i, name in enumerate(ordered):
+        for later in ordered[i + 1:]:
+            if later.startswith(name + "/"):
+                raise CheckpointError(f"{group}/{name}: the name is also the parent group of {group}/{later}")
+            if not later.startswith(name):
+                break
 
 
 def validate_content(snap: StateSnapshot) -> StateSnapshot:
-    """Names, keys, value and element types of every entry (state.jl:191-204)."""
+    """Refuse a snapshot whose names, keys, element or value types the container cannot hold
+    (the content contract of state.jl:139-215)."""
     if not snap.solver:
-        raise CheckpointError("solver: name must not be empty")
+        raise CheckpointError("snapshot: the solver name is empty")
     if snap.cycle < 0:
-        raise CheckpointError(f"cycle: must be >= 0, got {snap.cycle}")
-    _check_arrays("fields", snap.fields)
-    _check_arrays("series", snap.series)
-    for cls in _SCALAR_GROUPS:
-        _check_scalars(cls, getattr(snap, cls))
+        raise CheckpointError(f"snapshot: cycle {snap.cycle} is negative")
+    for group in _ARRAY_GROUPS:
+        entries = getattr(snap, group)
+        _enforce(_ARRAY_RULES, group, entries)
+        _enforce_no_prefix_clash(group, entries)
+    for group in _SCALAR_GROUPS:
+        _enforce(_SCALAR_RULES + (_scalar_type_rule(group),), group, getattr(snap, group))
     return snap
 
 
 def check_finite(snap: StateSnapshot) -> StateSnapshot:
-    """Refuse NaN / Inf in fields and series, naming the first offending index (state.jl:219-238)."""
-    for cls in _ARRAY_GROUPS:
-        for name, a in getattr(snap, cls).items():
-            if a.dtype.kind == "f":
-                bad = np.argwhere(~np.isfinite(a))
-                if len(bad):
-                    i = tuple(int(v) for v in bad[0])
-                    raise CheckpointError(f"{cls}/{name}: non-finite value {a[i]} at index "
-                                          f"{tuple(v + 1 for v in i)}; snapshot refused")
+    """Refuse NaN or Inf in any floating-point array; the error gives the first bad element
+    with 1-based (column-major, Julia) indices."""
+    for group in _ARRAY_GROUPS:
+        for name, a in getattr(snap, group).items():
+            if a.dtype.kind != "f":
+                continue
+            bad = np.argwhere(~np.isfinite(a))
+            if len(bad):
+                at = tuple(int(v) for v in bad[0])
+                raise CheckpointError(f"{group}/{name}: {a[at]} at 1-based index {tuple(v + 1 for v in at)}; "
+                                      "a snapshot must hold finite values only")
     return snap
 
 
@@ -162,29 +189,36 @@ def _same_scalar(a, b):
     return a == b
 
 
+def _typed(v):
+    return f"{v!r}::{type(v).__name__}"
+
+
 def check_compatible(snap: StateSnapshot, *, solver, schema_version, identity, identity_defaults=None):
-    """Solver name, schema version and identity key by key, same type and value
-    (state.jl:409-437); run_control and parameters are not compared."""
-    identity_defaults = identity_defaults or {}
+    """A snapshot may restore a run only if it comes from the same solver and schema version
+    and its identity entries equal the run's, type included; keys the run declares with a
+    default may be absent.  run_control and parameters are free to differ (the compatibility
+    contract of state.jl:409-441)."""
+    defaults = identity_defaults or {}
     if snap.solver != solver:
-        raise CheckpointError(f"solver: expected {solver!r}, found {snap.solver!r}")
+        raise CheckpointError(f"solver: the snapshot was written by {snap.solver!r}, this run is {solver!r}")
     if snap.schema_version != schema_version:
-        raise CheckpointError(f"schema_version: expected {schema_version}, found {snap.schema_version}")
-    for key, expected in identity.items():
-        if key in snap.identity:
-            found = snap.identity[key]
-        elif key in identity_defaults:
-            found = identity_defaults[key]
-        else:
-            raise CheckpointError(f"identity/{key}: expected {expected!r}::{type(expected).__name__}, key "
-                                  "missing from the snapshot and no default declared")
-        if not _same_scalar(found, expected):
-            raise CheckpointError(f"identity/{key}: expected {expected!r}::{type(expected).__name__}, "
-                                  f"found {found!r}::{type(found).__name__}")
-    for key in snap.identity:
-        if key not in identity:
-            raise CheckpointError(f"identity/{key}: found {snap.identity[key]!r}, key not expected by the caller")
+        raise CheckpointError(f"schema_version: the snapshot has {snap.schema_version}, "
+                              f"this run reads {schema_version}")
+    for key, want in identity.items():
+        have = snap.identity.get(key, defaults.get(key, _MISSING))
+        if have is _MISSING:
+            raise CheckpointError(f"identity/{key}: the run needs {_typed(want)}; the snapshot lacks the key "
+                                  "and the run declares no default")
+        if not _same_scalar(have, want):
+            raise CheckpointError(f"identity/{key}: the snapshot has {_typed(have)}, the run needs {_typed(want)}")
+    extra = [k for k in snap.identity if k not in identity]
+    if extra:
+        raise CheckpointError(f"identity/{extra[0]}: the snapshot has {_typed(snap.identity[extra[0]])}, "
+                              "a key this run does not declare")
     return snap
+
+
+_MISSING = object()
 
 
 # --------------------------------------------------------------------------------------

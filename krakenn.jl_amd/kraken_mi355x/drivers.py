@@ -61,7 +61,7 @@ def reynolds(config: LBMConfig) -> float:
 
 
 def trt_rates(nu: float, Lambda: float = 3.0 / 16.0):
-    """(s_plus, s_minus) (fused_trt_2d.jl:201-205)."""
+    """(s_plus, s_minus) (fused_trt_2d.jl:129-133)."""
     return L.trt_rates(nu, Lambda)
 
 

@@ -1273,7 +1273,7 @@ static void collide_guo_field_2d(T* f, const uint8_t* solid, int64_t nx, int64_t
 // ----------------------------------------------------------------------------------
 // Generic descriptor-driven step: the semantics the MI355X kernels implement.
 // Each piece is one of the restated reference kernels above, chained in the order of
-// the reference drivers (basic.jl:121-136, thermal.jl:100-133, test_mrt.jl:57-63).
+// the reference drivers (basic.jl:121-136, thermal.jl:100-133, test_mrt.jl:19-25).
 // ----------------------------------------------------------------------------------
 struct Axis {
   int64_t n;

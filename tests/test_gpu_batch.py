@@ -1,5 +1,5 @@
 """Batched cavities: each cavity of the ensemble is bit-identical to the oracle's cavity chain
-(src/drivers/basic.jl:121-136 / test/test_mrt.jl:57-63) run alone."""
+(src/drivers/basic.jl:121-136 / test/test_mrt.jl:19-25) run alone."""
 import numpy as np
 import pytest
 

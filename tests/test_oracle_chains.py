@@ -96,7 +96,7 @@ def test_generic_cavity_equals_run_cavity_2d_chain(po, kmi):
 
 
 def test_generic_mrt_equals_test_mrt_loop(po, kmi):
-    """test/test_mrt.jl:57-63: stream_2d! → apply_zou_he_north_2d! → collide_mrt_2d!."""
+    """test/test_mrt.jl:19-25: stream_2d! → apply_zou_he_north_2d! → collide_mrt_2d!."""
     nx, ny, nu, ul = 11, 9, 0.02, 0.1
     d = kmi.cavity_desc(nx, nu, ul, Ny=ny, collision="mrt")
     f = rng_pops((nx, ny), 9, seed=16)
