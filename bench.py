@@ -4,10 +4,21 @@ shifted and FP64 storages are timed alongside as variants.
 
 One "step" = one stream–collide time step over the whole 256³ lattice (populations resident
 in HBM).  The autotuner picks the launch: single-step launches, or two or three steps per
-launch with the intermediate populations in LDS.  Prints ONE JSON line with MLUPS, the HBM roofline of the
-dominant kernel (algorithmic 152 B per cell update in FP32 = 19 reads + 19 writes of 4 B,
-times the updates one launch performs) and a CPU baseline (the FP32 oracle restatement,
-OpenMP on the host cores, bounded sample).
+launch with the intermediate populations in LDS.  Prints ONE JSON line with MLUPS, the HBM
+roofline of the dominant kernel and a CPU baseline (the FP32 oracle restatement, OpenMP on the
+host cores, bounded sample).
+
+Roofline of the dominant kernel: `achieved` = the least HBM traffic one launch can have — one
+read and one write of every population, 152 B per cell in FP32 (19 × 4 B × 2), whatever the
+steps the launch advances — divided by the launch's average duration, timed with HIP events on
+the lattice stream over whole launches only.  `traffic` is the rocprofv3 counter figure for
+the same kernel and launch shape (FETCH_SIZE and WRITE_SIZE, calibrated on a copy kernel),
+`frac_counter_bytes` the counted bytes over the same time.  The per-step figure (152 B per
+cell update × steps per launch) is reported apart as `per_step_GBps`.
+
+The same line carries the learned warm-start record (`warm_start`): 2048 steady 128² MRT
+cavities from the batched solver train the decoder, 256 held-out cases run to the residual
+1e-7 warm-started in one batch against rest-initialised plain single-lattice runs.
 """
 import argparse
 import json
@@ -21,6 +32,9 @@ sys.path.insert(0, os.path.join(ROOT, "krakenn.jl_amd"))
 HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E, published
 BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA, published
 BYTES_PER_POP = {"f64": 8, "f32": 4, "f32s": 4, "f16s": 2}
+ARITH_DTYPE = {"f64": "f64", "f32": "f32", "f32s": "f32", "f16s": "f32"}
+STORAGE_DTYPE = {"f64": "f64", "f32": "f32", "f32s": "f32 (f - w_q)", "f16s": "f16 (2^15 (f - w_q))"}
+LAUNCHES_TIMED = 20
 STORAGE_LABEL = {"f64": "FP64 storage", "f32": "FP32 storage",
                  "f32s": "FP32 shifted storage f-w", "f16s": "FP16 shifted storage"}
 
@@ -30,6 +44,22 @@ def launch_span(launch):
     12-14: 2, 6-11: 3)."""
     t = launch.get("tile", 0)
     return 3 if 6 <= t <= 11 else (2 if t >= 1 else 1)
+
+
+def traffic_record(kernel, storage, N, launch):
+    """Counted HBM bytes per launch for this kernel, storage and launch shape, or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as fh:
+            recs = json.load(fh).get(f"{kernel}_trt_{storage}_{N}")
+    except (OSError, ValueError):
+        return None
+    recs = recs if isinstance(recs, list) else ([recs] if recs else [])
+    for rec in recs:
+        shape = rec.get("launch", {})
+        if all(launch.get(k) == shape.get(k) for k in ("tile", "zchunks", "nontemporal")) or \
+                (not shape and not launch.get("tile")):
+            return rec["bytes_per_launch"]
+    return None
 
 
 def cpu_baseline(N, nu, storage, budget_s=12.0, max_steps=30):
@@ -274,106 +304,24 @@ def bench_nn(args):
 
 
 def bench_warmstart(args):
-    """Learned warm start: 2048 steady 128² cavities (Re log-uniform [100, 3200], lid
+    """Learned warm start alone: steady 128² cavities (Re log-uniform [100, 1000], lid
     amplitude A uniform [0.06, 0.12]) from the batched solver train the conv decoder on the
-    GPU; 256 held-out cases then run to r < 1e-7 from rest and from the prediction (feq of
-    the predicted ρ, u; a variant that then holds the predicted velocity for consistent-
-    initialisation steps is reported beside it)."""
-    import numpy as np
-
-    import kraken_mi355x as kmi
+    GPU; the held-out cases then run to r < 1e-7 from rest and from the prediction (feq of the
+    predicted ρ, u), batched, against rest-initialised plain single-lattice runs; FP64 runs of
+    both starts to a tight residual compare the converged fields (all held-out cases with
+    --ws-agree-all)."""
     from kraken_mi355x import warmstart as ws
 
-    log = (lambda s: print(s, file=sys.stderr, flush=True))
-    N = 128
-    Re_tr, A_tr = ws.draw_cases(args.ws_train, seed=1)
-    data = ws.generate_steady_states(Re_tr, A_tr, N=N, tol=1e-6, log=log)
-    t0 = time.perf_counter()
-    net, loss = ws.train(data["x"], data["y"], epochs=args.ws_epochs, batch=32, lr=2e-3, log=log)
-    net.backend.synchronize()
-    t_train = time.perf_counter() - t0
-    fwd = decoder_forward_flop(net.desc)
-    # an epoch runs floor(samples / batch) full batches
-    train_flop = 3.0 * fwd * (len(data["index"]) // 32 * 32) * max(1, args.ws_epochs // 10) * 10
-    Re_te, A_te = ws.draw_cases(args.ws_test, seed=99)
-    K = args.ws_consistent
-    rest = ws.time_to_residual(Re_te, A_te, N=N, tol=1e-7)
-    warm = ws.time_to_residual(Re_te, A_te, N=N, net=net, tol=1e-7)
-    warm_ci = ws.time_to_residual(Re_te, A_te, N=N, net=net, tol=1e-7, consistent=K)
-    ok = (rest["status"] == 1) & (warm["status"] == 1) & (warm_ci["status"] == 1)
-    idx = np.where(ok)[0]
-
-    def dU(a, b):
-        return [max(float(np.max(np.abs(a["fields"][c][k] - b["fields"][c][k]))) for k in ("ux", "uy")) / A_te[c]
-                for c in idx]
-
-    du, du_ci = dU(warm, rest), dU(warm_ci, rest)
-    pred = net.predict(ws.encode(Re_te[idx], A_te[idx]))
-    perr = [float(np.linalg.norm(pred[k][..., :2] - ws.to_target(rest["fields"][c], A_te[c])[..., :2])
-                  / np.linalg.norm(ws.to_target(rest["fields"][c], A_te[c])[..., :2])) for k, c in enumerate(idx)]
-    # Time-to-residual is defined for cases that converge: the batch's wall time is set by
-    # its slowest member, and the cases that diverge or stall at max_steps from either start
-    # would hold both runs to the step limit.  Re-time the batches on the converging set,
-    # and time rest-initialised plain runs (one lattice at a time) on a sample of it.
-    log(f"[warmstart] converging set: {len(idx)} of {len(Re_te)}")
-    rest_c = ws.time_to_residual(Re_te[idx], A_te[idx], N=N, tol=1e-7)
-    warm_c = ws.time_to_residual(Re_te[idx], A_te[idx], N=N, net=net, tol=1e-7)
-    warm_ci_c = ws.time_to_residual(Re_te[idx], A_te[idx], N=N, net=net, tol=1e-7, consistent=K)
-    pick = idx[np.linspace(0, len(idx) - 1, min(args.ws_plain, len(idx))).astype(int)]
-    plain = ws.plain_time_to_residual(Re_te[pick], A_te[pick], N=N, tol=1e-7)
-    plain_total = float(np.mean(plain["wall_s"])) * len(idx)
-    log(f"[warmstart] plain sample {len(pick)}: mean {np.mean(plain['wall_s']):.3f} s per cavity")
-    # The fields' agreement at 1e-7 is bounded by the stop rule (a field stopped at r < 1e-7
-    # is still ~r·τ from the steady state); run a sample in FP64 from both starts to a tight r.
-    pick_a = idx[np.linspace(0, len(idx) - 1, min(args.ws_agree, len(idx))).astype(int)]
-    agree = ws.converged_agreement(Re_te[pick_a], A_te[pick_a], net, N=N, tol=args.ws_agree_tol,
-                                   max_steps=args.ws_agree_max_steps)
-    log(f"[warmstart] FP64 r<{args.ws_agree_tol:g} agreement on {len(pick_a)} cases: max dU/A "
-        f"{agree['dU_over_A'].max() if len(agree['dU_over_A']) else float('nan'):.2e}")
-    med = (lambda a, b: round(float(np.median(a["steps"][idx] / b["steps"][idx])), 3))
-    out = {
-        "metric": "time to residual 1e-7, held-out 128^2 MRT cavities: warm-started batched runs vs "
-                  "rest-initialised plain runs",
-        "value": round(plain_total / warm_c["wall_s"], 3), "unit": "x (plain rest wall / batched warm wall)",
-        "warm_start": "feq of the network's predicted rho, u",
-        "batched_rest_over_warm_converging": round(rest_c["wall_s"] / warm_c["wall_s"], 3),
-        "batched_rest_over_warm_all": round(rest["wall_s"] / warm["wall_s"], 3),
-        "converging_set": int(len(idx)), "batched_rest_wall_converging_s": round(rest_c["wall_s"], 2),
-        "batched_warm_wall_converging_s": round(warm_c["wall_s"], 2),
-        "plain_rest_wall_estimate_s": round(plain_total, 2),
-        "plain_sample": {"cases": int(len(pick)), "mean_wall_s": round(float(np.mean(plain["wall_s"])), 4),
-                         "status": np.bincount(plain["status"], minlength=4).tolist(),
-                         "median_steps": int(np.median(plain["steps"]))},
-        "higher_is_better": True, "n_gpus": 1, "dtype": "f32 solver / bf16 network",
-        "data": "synthetic (seeded Re, A draws; training targets from the batched solver)",
-        "rest_wall_s": round(rest["wall_s"], 2), "warm_wall_s": round(warm["wall_s"], 2),
-        "warm_inference_and_init_s": round(warm["init_s"], 3),
-        "median_step_ratio": med(rest, warm),
-        "converged_all_three": int(ok.sum()), "rest_status": np.bincount(rest["status"], minlength=4).tolist(),
-        "warm_status": np.bincount(warm["status"], minlength=4).tolist(),
-        "max_dU_over_A_between_converged_fields": max(du) if du else None,
-        "median_dU_over_A": float(np.median(du)) if du else None,
-        f"consistent_init_{K}_warm_start": {
-            "batched_rest_over_warm_converging": round(rest_c["wall_s"] / warm_ci_c["wall_s"], 3),
-            "batched_rest_over_warm_all": round(rest["wall_s"] / warm_ci["wall_s"], 3),
-            "plain_over_warm": round(plain_total / warm_ci_c["wall_s"], 3),
-            "median_step_ratio": med(rest, warm_ci), "max_dU_over_A": max(du_ci) if du_ci else None,
-            "status": np.bincount(warm_ci["status"], minlength=4).tolist()},
-        "fp64_agreement": {
-            "tol": args.ws_agree_tol, "max_steps": args.ws_agree_max_steps,
-            "cases": int(len(pick_a)), "converged_both": int(len(agree["index"])),
-            "max_dU_over_A": float(agree["dU_over_A"].max()) if len(agree["dU_over_A"]) else None,
-            "median_dU_over_A": float(np.median(agree["dU_over_A"])) if len(agree["dU_over_A"]) else None,
-            "below_1e-6": int(np.sum(agree["dU_over_A"] < 1e-6)),
-            "max_rel_L2": float(agree["rel_L2"].max()) if len(agree["rel_L2"]) else None,
-            "median_steps_rest": int(np.median(agree["rest_steps"])),
-            "median_steps_warm": int(np.median(agree["warm_steps"])), "wall_s": round(agree["wall_s"], 2)},
-        "prediction_rel_L2_median": float(np.median(perr)) if perr else None,
-        "train": {"samples": int(len(data["index"])), "generated": args.ws_train, "data_gen_s": round(data["seconds"], 1),
-                  "epochs": args.ws_epochs, "final_loss": loss, "fwd_flop_per_sample": fwd,
-                  "train_s": round(t_train, 2), "bf16_TFLOPs": round(train_flop / t_train / 1e12, 2),
-                  "frac_of_bf16_peak": round(train_flop / t_train / 1e12 / BF16_PEAK_TFLOPS, 4)},
-    }
+    log = (lambda m: print(m, file=sys.stderr, flush=True))
+    rec = ws.experiment(n_train=args.ws_train, n_test=args.ws_test, epochs=args.ws_epochs,
+                        target_tol=args.ws_target_tol, plain_sample=args.ws_plain, agree_sample=args.ws_agree,
+                        agree_tol=args.ws_agree_tol, agree_max_steps=args.ws_agree_max_steps,
+                        agree_all=args.ws_agree_all, log=log)
+    out = {"metric": "time to residual 1e-7, held-out 128^2 MRT cavities: warm-started batched runs vs "
+                     "rest-initialised plain runs",
+           "value": rec.pop("value"), "unit": rec.pop("unit"), "higher_is_better": True, "n_gpus": 1,
+           "dtype": "f32 solver / bf16 network", "data": "synthetic (seeded Re, A draws; training targets "
+                                                           "from the batched solver)", **rec}
     print(json.dumps(out), flush=True)
 
 
@@ -387,8 +335,8 @@ def main():
     ap.add_argument("--ws-agree", type=int, default=16, help="cases run in FP64 to a tight residual from both starts")
     ap.add_argument("--ws-agree-tol", type=float, default=1e-12)
     ap.add_argument("--ws-agree-max-steps", type=int, default=4_000_000)
-    ap.add_argument("--ws-consistent", type=int, default=256,
-                    help="consistent-initialisation steps of the variant warm start")
+    ap.add_argument("--ws-target-tol", type=float, default=1e-8, help="residual of the training states")
+    ap.add_argument("--ws-agree-all", action="store_true", help="FP64 agreement over every held-out case")
     ap.add_argument("--cavity-max-steps", type=int, default=600_000)
     ap.add_argument("--batch-launch", default=None,
                     help="TW,TH,GROUPS,NT,SPAN: fixed batch launch for the cavities workload (no autotune)")
@@ -402,6 +350,7 @@ def main():
     ap.add_argument("--temporal", default=None,
                     help="TILE,ZCHUNKS[,NT]: fixed two-step launch (implies --no-autotune)")
     ap.add_argument("--no-variants", action="store_true", help="skip timing the other storages")
+    ap.add_argument("--no-warmstart", action="store_true", help="skip the learned warm-start record")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
@@ -443,33 +392,28 @@ def main():
             wall = time.perf_counter() - t0
             mass = lat.reduce("mass")
             launch = {**lat.launch, **lat.temporal}
-        return ms, wall, mass, launch
+            # the dominant kernel alone: whole launches only (a step count that is a multiple
+            # of the steps per launch), HIP events on the lattice stream
+            span = launch_span(launch)
+            launch_ms = lat.step_timed(span * LAUNCHES_TIMED) / LAUNCHES_TIMED
+        return ms, wall, mass, launch, launch_ms
 
-    ms, wall, mass, launch = run(args.storage, args.steps, args.warmup)
+    log = (lambda m: print(m, file=sys.stderr, flush=True))
+    ms, wall, mass, launch, launch_ms = run(args.storage, args.steps, args.warmup)
     copy_gbps = kmi.MI355XBackend().copy_bandwidth(2 * 1024 ** 3, 20)
+    log(f"[bench] {args.storage}: {wall * 1e3 / args.steps:.4f} ms/step, launch {launch}, "
+        f"{launch_ms:.4f} ms per launch")
     # HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated on a
-    # copy kernel of known size), recorded for this workload in profiles/traffic.json.
-    # Steps per launch of the dominant kernel: 3 for the three-step kernel (k_step3_d3q19,
-    # tiles 6-11), 2 for the two-step kernel (k_step2_d3q19, tiles 1-5, 12-14), else 1 (k_step).
-    # Traffic records are per kernel.
+    # copy kernel of known size), recorded in profiles/traffic.json per kernel, storage and
+    # launch shape (tile, z-ranges, non-temporal stores): a record holds for its shape only.
     span = launch_span(launch)
     kernel = {3: "k_step3_d3q19", 2: "k_step2_d3q19", 1: "k_step"}[span]
-    traffic = None
-    try:
-        with open(os.path.join(ROOT, "profiles", "traffic.json")) as fh:
-            recs = json.load(fh).get(f"{kernel}_trt_{args.storage}_{N}")
-        recs = recs if isinstance(recs, list) else ([recs] if recs else [])
-        # a two-step record holds for the tile and z-split it was measured with only
-        for rec in recs:
-            if all(launch.get(k) == v for k, v in rec.get("launch", {}).items() if k in ("tile", "zchunks")):
-                traffic = rec["bytes_per_launch"]
-    except (OSError, ValueError, KeyError):
-        traffic = None
+    traffic = traffic_record(kernel, args.storage, N, launch)
     cells = N ** 3
     mlups = cells * args.steps / wall / 1e6
-    per_launch_s = ms / 1e3 / (args.steps / span)
-    alg_bytes = span * cells * 19 * 2 * BYTES_PER_POP[args.storage]
-    achieved = alg_bytes / per_launch_s / 1e9
+    launch_s = launch_ms / 1e3
+    min_bytes = cells * 19 * 2 * BYTES_PER_POP[args.storage]  # one read + one write per launch
+    achieved = min_bytes / launch_s / 1e9
     out = {
         "metric": ("MLUPS (D3Q19 TRT 256^3 Taylor-Green, " + STORAGE_LABEL[args.storage] +
                    ") and % of HBM roofline"),
@@ -482,17 +426,20 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if args.storage in ("f32", "f32s", "f16s") else "f64",
+        "dtype": ARITH_DTYPE[args.storage],
+        "storage_dtype": STORAGE_DTYPE[args.storage],
         "data": "synthetic (analytic Taylor-Green initial field, no dataset)",
         "config": {"workload": f"D3Q19 TRT Lambda=3/16 triply periodic Taylor-Green {N}^3, "
                                f"U0={U0}, nu={nu}, storage={args.storage}",
                    "cells": cells, "storage": args.storage, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
-                     "kernel": kernel, "steps_per_launch": span,
-                     "bytes_per_launch": alg_bytes, "avg_launch_ms": round(per_launch_s * 1e3, 5),
-                     "hbm_GBps_measured_traffic": (round(traffic / per_launch_s / 1e9, 1)
-                                                   if traffic else None),
+                     "kernel": kernel, "steps_per_launch": span, "launch": launch,
+                     "bytes_per_launch": min_bytes, "avg_launch_ms": round(launch_ms, 5),
+                     "launches_timed": LAUNCHES_TIMED,
+                     "frac_counter_bytes": (round(traffic / launch_s / 1e9 / HBM_PEAK_GBPS, 4)
+                                            if traffic else None),
+                     "per_step_GBps": round(span * achieved, 1),
                      "copy_measured": round(copy_gbps, 1),
                      "frac_of_copy": round(achieved / copy_gbps, 4)},
         "launch": launch,
@@ -505,13 +452,17 @@ def main():
         for st in ("f32", "f16s", "f64"):
             if st == args.storage:
                 continue
-            m2, w2, _, l2 = run(st, args.steps, args.warmup)
-            sp2 = launch_span(l2)
+            m2, w2, _, l2, lms2 = run(st, args.steps, args.warmup)
             var[st] = {"MLUPS": round(cells * args.steps / w2 / 1e6, 1),
-                       "frac": round(sp2 * cells * 38 * BYTES_PER_POP[st] / (m2 / 1e3 / (args.steps / sp2)) / 1e9
-                                     / HBM_PEAK_GBPS, 4),
-                       "launch": l2}
+                       "frac": round(cells * 38 * BYTES_PER_POP[st] / (lms2 / 1e3) / 1e9 / HBM_PEAK_GBPS, 4),
+                       "avg_launch_ms": round(lms2, 5), "steps_per_launch": launch_span(l2),
+                       "dtype": ARITH_DTYPE[st], "storage_dtype": STORAGE_DTYPE[st], "launch": l2}
+            log(f"[bench] {st}: {var[st]['MLUPS']} MLUPS, launch {l2}")
         out["variants"] = var
+    if not args.no_warmstart:
+        from kraken_mi355x import warmstart as ws
+
+        out["warm_start"] = ws.experiment(log=log)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(N, nu, args.storage)
         # the FP64 restatement on the same host cores, for the FP64 variant and reference

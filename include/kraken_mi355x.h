@@ -11,7 +11,7 @@
  *   - backend value / dispatch:   src/simulation_runner.jl:110-160, 235-443
  *                                  (`_backend_from_krk_symbol`, `run_simulation`)
  *   - state allocation / init:     src/drivers/basic.jl:51-101 (`initialize_2d/3d`)
- *   - fused single-pass step:      src/kernels/fused_bgk_2d.jl:67, fused_trt_2d.jl:217,
+ *   - fused single-pass step:      src/kernels/fused_bgk_2d.jl:67, fused_trt_2d.jl:145,
  *                                  fused_thermal_2d.jl:251, dsl/bricks_3d.jl:64-229
  *   - unfused step chains:         src/drivers/basic.jl:121-136 (`run_cavity_2d`),
  *                                  src/drivers/thermal.jl:100-133 (`run_rayleigh_benard_2d`),
@@ -174,7 +174,7 @@ typedef struct {
   int32_t thermal;    /* 0 / 1: double-distribution temperature field g (Q populations) */
   int64_t nx, ny, nz; /* nz = 1 for D2Q9 */
   double omega;             /* BGK rate  1/(3 nu + 1/2)                     (basic.jl:30) */
-  double s_plus, s_minus;   /* TRT rates, see kmi_trt_rates              (fused_trt_2d.jl:201) */
+  double s_plus, s_minus;   /* TRT rates, see kmi_trt_rates              (fused_trt_2d.jl:129-133) */
   double s_e, s_eps, s_q, s_nu; /* MRT rates                            (collide_mrt_2d.jl:118) */
   double omega_T;           /* thermal BGK rate 1/(3 alpha + 1/2)           (thermal.jl:65) */
   double beta_g;            /* Boussinesq  F_y = beta_g (T - T_ref), y up (thermal_2d.jl:435,
@@ -240,7 +240,7 @@ int kmi_ctx_destroy(kmi_ctx* ctx);
 int kmi_ctx_synchronize(kmi_ctx* ctx);
 
 /* TRT rates from viscosity and magic parameter: s_plus = 1/(3 nu + 1/2),
- * s_minus = 1/(Lambda/(3 nu) + 1/2)   (src/kernels/fused_trt_2d.jl:201-205). */
+ * s_minus = 1/(Lambda/(3 nu) + 1/2)   (src/kernels/fused_trt_2d.jl:129-133). */
 void kmi_trt_rates(double nu, double Lambda, double* s_plus, double* s_minus);
 /* Fill `desc` with the defaults of a D2Q9/D3Q19 lattice: all faces WALL, BGK from nu. */
 void kmi_desc_init(kmi_desc* desc, int32_t stencil, int64_t nx, int64_t ny, int64_t nz,
@@ -357,7 +357,7 @@ int kmi_lattice_autotune(kmi_lattice* lat, int64_t steps_per_trial, double* best
 int kmi_measure_copy_bandwidth(kmi_ctx* ctx, int64_t bytes, int32_t reps, double* gbps);
 
 /* ---- batched lid-driven cavities (no reference counterpart: the reference runs one
- *      cavity per driver call, e.g. test/test_mrt.jl:57-63 and basic.jl:112-144) -------
+ *      cavity per driver call, e.g. test/test_mrt.jl:19-25 and basic.jl:112-144) -------
  * An ensemble of square D2Q9 cavities of uneven side N_c and viscosity nu_c advanced
  * together: every step is ONE launch over a packed list of tiles (tw × th cells of one
  * cavity) walked by a persistent grid; tiles are ordered so that neighbouring tiles
@@ -473,6 +473,7 @@ int kmi_nn_train_epochs(kmi_nn* nn, int32_t epochs, int32_t batch, float lr, uin
  * A[m*sam + k*sak], B(k, n) = B[k*sbk + n*sbn]; the K range split over `splits` groups. */
 int kmi_gemm_bf16(kmi_ctx* ctx, int32_t M, int32_t N, int32_t K, const float* A, int64_t sam,
                   int64_t sak, const float* B, int64_t sbk, int64_t sbn, int32_t splits, float* C);
+
 
 #ifdef __cplusplus
 }

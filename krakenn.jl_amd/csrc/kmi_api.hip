@@ -93,10 +93,11 @@ int kmi_ctx_create(int device, kmi_ctx** out) {
   if (!c) return fail(nullptr, KMI_ERR_NOMEM, "kmi_ctx_create: out of host memory");
   c->device = device;
   if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
-      (e = hipMalloc(&c->partials, 2048 * 3 * sizeof(double))) != hipSuccess ||
+      (e = hipMalloc(&c->partials, 2048 * 3 * 2 * sizeof(double))) != hipSuccess ||
       (e = hipMalloc(&c->out, 8 * sizeof(double))) != hipSuccess) {
     if (c->stream) (void)hipStreamDestroy(c->stream);
     if (c->partials) (void)hipFree(c->partials);
+    if (c->out) (void)hipFree(c->out);
     delete c;
     return hip_fail(nullptr, e, "kmi_ctx_create");
   }
@@ -124,7 +125,7 @@ int kmi_ctx_synchronize(kmi_ctx* ctx) {
 }
 
 void kmi_trt_rates(double nu, double Lambda, double* s_plus, double* s_minus) {
-  // fused_trt_2d.jl:201-205
+  // fused_trt_2d.jl:129-133
   if (s_plus) *s_plus = 1.0 / (3 * nu + 0.5);
   if (s_minus) *s_minus = 1.0 / (Lambda / (3 * nu) + 0.5);
 }

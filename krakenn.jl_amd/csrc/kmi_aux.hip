@@ -1,9 +1,10 @@
 // kmi_aux.hip — layout conversion, initial states, moments and FP64 reductions.
 //
-// Reductions are deterministic: each work-group folds its cells in FP64 with a
-// wave64 butterfly (__shfl_xor) and an LDS pass across waves, writes one partial per
-// group, and a single work-group combines the partials in fixed order with Neumaier
-// compensation.  No atomics, so repeated calls return identical bits.
+// Reductions are deterministic and compensated at every level: each thread accumulates its
+// cells in FP64 as a Neumaier pair (sum, error), a wave64 butterfly (__shfl_xor) and an LDS
+// pass across waves merge the pairs, each work-group writes its pair, and one 256-thread
+// work-group folds the group pairs (strided per thread, then a fixed LDS tree).  No atomics,
+// so repeated calls return identical bits.
 #include "kmi_common.h"
 
 namespace kmi {
@@ -175,27 +176,37 @@ __global__ void k_moments(const void* f, const void* g, double* rho, double* ux,
 }
 
 // --- block reduction -------------------------------------------------------------------
+// v[n] + c[n] is thread's compensated sum of value n; partials receive (sum, error) per value:
+// partials[2 (group NV + n)], partials[2 (group NV + n) + 1].  is_max: plain maxima, error 0.
 template <int NV>
-KMI_DEV void block_reduce(double (&v)[NV], bool is_max, double* partials) {
-  __shared__ double lds[kRedBlock / 64][kRedMaxVals];
+KMI_DEV void block_reduce(double (&v)[NV], double (&c)[NV], bool is_max, double* partials) {
+  __shared__ double lds[kRedBlock / 64][kRedMaxVals][2];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int n = 0; n < NV; ++n) {
-    double x = v[n];
+    double x = v[n], e = c[n];
     for (int off = 32; off > 0; off >>= 1) {
-      const double y = __shfl_xor(x, off, 64);
-      x = is_max ? fmax(x, y) : x + y;
+      const double y = __shfl_xor(x, off, 64), ye = __shfl_xor(e, off, 64);
+      if (is_max) x = fmax(x, y);
+      else nsum_merge(x, e, y, ye);
     }
-    if (lane == 0) lds[wave][n] = x;
+    if (lane == 0) {
+      lds[wave][n][0] = x;
+      lds[wave][n][1] = e;
+    }
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     const int nw = (blockDim.x + 63) / 64;
 #pragma unroll
     for (int n = 0; n < NV; ++n) {
-      double x = lds[0][n];
-      for (int w = 1; w < nw; ++w) x = is_max ? fmax(x, lds[w][n]) : x + lds[w][n];
-      partials[blockIdx.x * NV + n] = x;
+      double x = lds[0][n][0], e = lds[0][n][1];
+      for (int w = 1; w < nw; ++w) {
+        if (is_max) x = fmax(x, lds[w][n][0]);
+        else nsum_merge(x, e, lds[w][n][0], lds[w][n][1]);
+      }
+      partials[2 * (blockIdx.x * NV + n)] = x;
+      partials[2 * (blockIdx.x * NV + n) + 1] = is_max ? 0.0 : e;
     }
   }
 }
@@ -207,23 +218,27 @@ __global__ void __launch_bounds__(kRedBlock)
     k_reduce_cells(const void* f, const void* g, const uint8_t* mask, double* uprev,
                    long long ncells, long long plane, double T0, MomForce F, int kind,
                    double* partials) {
-  double acc[3] = {0.0, 0.0, 0.0};
+  double acc[3] = {0.0, 0.0, 0.0}, err[3] = {0.0, 0.0, 0.0};
   const bool is_max = kind == KMI_RED_MAX_SPEED;
   for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < ncells;
        c += (long long)gridDim.x * blockDim.x) {
     double r, u, v, w, t;
     cell_moments<Q, ST>(f, g, c, plane, T0, F, r, u, v, w, t);
     switch (kind) {
-      case KMI_RED_MASS: acc[0] += r; break;
-      case KMI_RED_MOMENTUM: acc[0] += r * u; acc[1] += r * v; acc[2] += r * w; break;
-      case KMI_RED_KINETIC_ENERGY: acc[0] += 0.5 * r * (u * u + v * v + w * w); break;
-      case KMI_RED_THERMAL: acc[0] += t; break;
+      case KMI_RED_MASS: nsum_add(acc[0], err[0], r); break;
+      case KMI_RED_MOMENTUM:
+        nsum_add(acc[0], err[0], r * u);
+        nsum_add(acc[1], err[1], r * v);
+        nsum_add(acc[2], err[2], r * w);
+        break;
+      case KMI_RED_KINETIC_ENERGY: nsum_add(acc[0], err[0], 0.5 * r * (u * u + v * v + w * w)); break;
+      case KMI_RED_THERMAL: nsum_add(acc[0], err[0], t); break;
       case KMI_RED_MAX_SPEED: acc[0] = fmax(acc[0], sqrt(u * u + v * v + w * w)); break;
       case 100:
         if (!(mask && mask[c] == kCellSolid)) {
           const double du = u - uprev[c], dv = v - uprev[c + ncells], dw = w - uprev[c + 2 * ncells];
-          acc[0] += du * du + dv * dv + dw * dw;
-          acc[1] += u * u + v * v + w * w;
+          nsum_add(acc[0], err[0], du * du + dv * dv + dw * dw);
+          nsum_add(acc[1], err[1], u * u + v * v + w * w);
         }
         break;
       case 101:
@@ -235,7 +250,7 @@ __global__ void __launch_bounds__(kRedBlock)
     }
   }
   if (kind == 101) return;
-  block_reduce<3>(acc, is_max, partials);
+  block_reduce<3>(acc, err, is_max, partials);
 }
 
 // Hot-wall Nusselt number per wall node (thermal.jl:225-234, 525-533):
@@ -249,7 +264,7 @@ __global__ void __launch_bounds__(kRedBlock)
   using S = Store<ST>;
   using E = typename S::elem;
   const E* g = static_cast<const E*>(g_);
-  double acc[1] = {0.0};
+  double acc[1] = {0.0}, err[1] = {0.0};
   const int n[3] = {nx, ny, nz};
   const int ax = face / 2, t1 = ax == 0 ? 1 : 0, t2 = ax == 2 ? 1 : 2;
   const double H = n[ax];
@@ -276,9 +291,9 @@ __global__ void __launch_bounds__(kRedBlock)
       T1 = T0 + x;
       T2 = T0 + y;
     }
-    acc[0] += -H * (-8 * Th + 9 * T1 - T2) / 3.0 / dT;
+    nsum_add(acc[0], err[0], -H * (-8 * Th + 9 * T1 - T2) / 3.0 / dT);
   }
-  block_reduce<1>(acc, false, partials);
+  block_reduce<1>(acc, err, false, partials);
 }
 
 // Force on the cut-link walls, accumulated in FP64 over the link list.
@@ -306,7 +321,7 @@ __global__ void __launch_bounds__(kRedBlock)
     }
     return v;
   };
-  double acc[3] = {0.0, 0.0, 0.0};
+  double acc[3] = {0.0, 0.0, 0.0}, err[3] = {0.0, 0.0, 0.0};
   for (long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x; n < nlinks;
        n += (long long)gridDim.x * blockDim.x) {
     const long long c = cell[n];
@@ -343,27 +358,45 @@ __global__ void __launch_bounds__(kRedBlock)
       }
       F = fq + arriving;
     }
-    acc[0] += double(cx) * F;
-    acc[1] += double(cy) * F;
-    acc[2] += double(cz) * F;
+    nsum_add(acc[0], err[0], double(cx) * F);
+    nsum_add(acc[1], err[1], double(cy) * F);
+    nsum_add(acc[2], err[2], double(cz) * F);
   }
-  block_reduce<3>(acc, false, partials);
+  block_reduce<3>(acc, err, false, partials);
 }
 
-// Combine `nb` partials of NV values each, in order, with Neumaier compensation.
-__global__ void k_finalize(const double* __restrict__ partials, int nb, int nv, int is_max,
-                           double* __restrict__ out) {
-  const int n = threadIdx.x;
-  if (n >= nv) return;
-  double s = is_max ? partials[n] : 0.0, comp = 0.0;
-  for (int b = is_max ? 1 : 0; b < nb; ++b) {
-    const double x = partials[b * nv + n];
-    if (is_max) { s = fmax(s, x); continue; }
-    const double t = s + x;
-    comp += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
-    s = t;
+// Combine the `nb` group pairs of `nv` values: thread t folds groups t, t + 256, … in order,
+// then a fixed LDS tree merges the 256 thread pairs.  One work-group of 256 threads.
+__global__ void __launch_bounds__(256) k_finalize(const double* __restrict__ partials, int nb, int nv,
+                                                  int is_max, double* __restrict__ out) {
+  __shared__ double ls[256], lc[256];
+  const int t = threadIdx.x;
+  for (int n = 0; n < nv; ++n) {
+    double s = is_max ? -HUGE_VAL : 0.0, c = 0.0;
+    for (int b = t; b < nb; b += 256) {
+      const double x = partials[2 * (b * nv + n)], e = partials[2 * (b * nv + n) + 1];
+      if (is_max) s = fmax(s, x);
+      else nsum_merge(s, c, x, e);
+    }
+    ls[t] = s;
+    lc[t] = c;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (t < w) {
+        if (is_max) {
+          ls[t] = fmax(ls[t], ls[t + w]);
+        } else {
+          double a = ls[t], ac = lc[t];
+          nsum_merge(a, ac, ls[t + w], lc[t + w]);
+          ls[t] = a;
+          lc[t] = ac;
+        }
+      }
+      __syncthreads();
+    }
+    if (t == 0) out[n] = is_max ? ls[0] : ls[0] + lc[0];
+    __syncthreads();
   }
-  out[n] = s + comp;
 }
 
 // Plain device copy (float4 loads, non-temporal float4 stores): the practical 1:1
@@ -446,7 +479,7 @@ hipError_t launch_reduce_cells(int Q, int st, const void* f, const void* g, cons
   else { KMI_ST_SWITCH(st, hipLaunchKernelGGL((k_reduce_cells<19, ST>), dim3(nb), dim3(kRedBlock), 0, s, f, g, mask, uprev, ncells, plane, T0, F, kind, partials)); }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || kind == 101) return e;
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, s, partials, nb, 3,
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, partials, nb, 3,
                      kind == KMI_RED_MAX_SPEED ? 1 : 0, out);
   return hipGetLastError();
 }
@@ -459,7 +492,7 @@ hipError_t launch_drag(int Q, int st, const void* f, long long nlinks, const int
   else { KMI_ST_SWITCH(st, hipLaunchKernelGGL((k_drag<19, ST>), dim3(nb), dim3(kRedBlock), 0, s, f, nlinks, cell, q, qw, uw, nx, ny, nz, plane, mea, partials)); }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, s, partials, nb, 3, 0, out);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, partials, nb, 3, 0, out);
   return hipGetLastError();
 }
 
@@ -471,7 +504,7 @@ hipError_t launch_nusselt(int Q, int st, const void* g, int nx, int ny, int nz, 
   else { KMI_ST_SWITCH(thermal_storage(st), hipLaunchKernelGGL((k_nusselt<19, ST>), dim3(nb), dim3(kRedBlock), 0, s, g, nx, ny, nz, plane, T0, face, Th, dT, lo1, hi1, lo2, hi2, partials)); }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, s, partials, nb, 1, 0, out);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, partials, nb, 1, 0, out);
   return hipGetLastError();
 }
 

@@ -42,7 +42,7 @@ struct BatchArgs {
   double s_e, s_eps, s_q;
   double* uprev;           // 2 × total cells (ux, uy)
   long long total_cells;
-  double* partials;        // 2 per slot
+  double* partials;        // 4 per slot: (sum, error) of d2 and of u2
 };
 
 template <int ST, int COLL, bool NTS>
@@ -266,7 +266,7 @@ KMI_DEV void cell_u(const void* f, const CavRec& cv, int self, double& ux, doubl
 // mode 0: u → uprev;  mode 1: per-slot partials of Σ|u − uprev|² and Σ|u|².
 template <int ST>
 __global__ void __launch_bounds__(256) k_batch_resid(const BatchArgs a, const void* f, int mode) {
-  __shared__ double lds[4][2];
+  __shared__ double lds[4][4];
   const int lx = threadIdx.x % a.tw, ly = threadIdx.x / a.tw;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int t = blockIdx.x; t < a.nslots; t += gridDim.x) {
@@ -290,24 +290,32 @@ __global__ void __launch_bounds__(256) k_batch_resid(const BatchArgs a, const vo
       }
     }
     if (mode == 0) continue;
+    // per-slot compensated pairs (sum, error) of d2 and u2: butterfly, then across waves
+    double d2e = 0.0, u2e = 0.0;
     for (int off = 32; off > 0; off >>= 1) {
-      d2 += __shfl_xor(d2, off, 64);
-      u2 += __shfl_xor(u2, off, 64);
+      const double y = __shfl_xor(d2, off, 64), ye = __shfl_xor(d2e, off, 64);
+      const double z = __shfl_xor(u2, off, 64), ze = __shfl_xor(u2e, off, 64);
+      nsum_merge(d2, d2e, y, ye);
+      nsum_merge(u2, u2e, z, ze);
     }
     __syncthreads();  // previous slot's LDS reads are done
     if (lane == 0) {
       lds[wave][0] = d2;
-      lds[wave][1] = u2;
+      lds[wave][1] = d2e;
+      lds[wave][2] = u2;
+      lds[wave][3] = u2e;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-      double s0 = 0.0, s1 = 0.0;
-      for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) {
-        s0 += lds[w][0];
-        s1 += lds[w][1];
+      double s0 = lds[0][0], e0 = lds[0][1], s1 = lds[0][2], e1 = lds[0][3];
+      for (int w = 1; w < (int)(blockDim.x + 63) / 64; ++w) {
+        nsum_merge(s0, e0, lds[w][0], lds[w][1]);
+        nsum_merge(s1, e1, lds[w][2], lds[w][3]);
       }
-      a.partials[2 * t] = s0;
-      a.partials[2 * t + 1] = s1;
+      a.partials[4 * t] = s0;
+      a.partials[4 * t + 1] = e0;
+      a.partials[4 * t + 2] = s1;
+      a.partials[4 * t + 3] = e1;
     }
   }
 }
@@ -352,27 +360,33 @@ __global__ void __launch_bounds__(256) k_batch_hold(const BatchArgs a, void* f, 
   }
 }
 
-// One thread per cavity: Neumaier sum of its slots' partials (fixed order) → r.
-__global__ void k_batch_final(const double* __restrict__ partials, const int* __restrict__ cav_slots,
-                              const int* __restrict__ cav_range, const int* __restrict__ active,
-                              int ncav, double window, double* __restrict__ out) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncav) return;
-  if (!active[c]) return;
-  double s[2] = {0.0, 0.0}, comp[2] = {0.0, 0.0};
-  for (int k = cav_range[c]; k < cav_range[c + 1]; ++k) {
+// One wave per cavity: lane l folds the cavity's slot pairs l, l + 64, … in order, then a
+// fixed butterfly merges the lane pairs; compensated (Neumaier) throughout → r.
+__global__ void __launch_bounds__(64) k_batch_final(const double* __restrict__ partials,
+                                                    const int* __restrict__ cav_slots,
+                                                    const int* __restrict__ cav_range,
+                                                    const int* __restrict__ active, int ncav, double window,
+                                                    double* __restrict__ out) {
+  const int c = blockIdx.x, lane = threadIdx.x;
+  if (c >= ncav || !active[c]) return;  // uniform over the wave
+  double s0 = 0.0, e0 = 0.0, s1 = 0.0, e1 = 0.0;
+  for (int k = cav_range[c] + lane; k < cav_range[c + 1]; k += 64) {
     const int t = cav_slots[k];
-    for (int v = 0; v < 2; ++v) {
-      const double x = partials[2 * t + v];
-      const double y = s[v] + x;
-      comp[v] += fabs(s[v]) >= fabs(x) ? (s[v] - y) + x : (x - y) + s[v];
-      s[v] = y;
-    }
+    nsum_merge(s0, e0, partials[4 * t], partials[4 * t + 1]);
+    nsum_merge(s1, e1, partials[4 * t + 2], partials[4 * t + 3]);
   }
-  const double d2 = s[0] + comp[0], u2 = s[1] + comp[1];
-  out[3 * c] = d2;
-  out[3 * c + 1] = u2;
-  out[3 * c + 2] = sqrt(d2 / u2) / window;  // window = 1: gnf_periodic_steady.jl:192
+  for (int off = 32; off > 0; off >>= 1) {
+    const double y = __shfl_xor(s0, off, 64), ye = __shfl_xor(e0, off, 64);
+    const double z = __shfl_xor(s1, off, 64), ze = __shfl_xor(e1, off, 64);
+    nsum_merge(s0, e0, y, ye);
+    nsum_merge(s1, e1, z, ze);
+  }
+  if (lane == 0) {
+    const double d2 = s0 + e0, u2 = s1 + e1;
+    out[3 * c] = d2;
+    out[3 * c + 1] = u2;
+    out[3 * c + 2] = sqrt(d2 / u2) / window;  // window = 1: gnf_periodic_steady.jl:192
+  }
 }
 
 #define KMI_BATCH_ST(st, CALL)                                               \
@@ -562,7 +576,7 @@ static int batch_build_tiles(kmi_batch* B) {
   };
   KMI_HIP(c, grow((void**)&B->slots_d, B->slots_cap, slots.size() * sizeof(int)));
   KMI_HIP(c, grow((void**)&B->cav_slots_d, B->cav_slots_cap, cav_slots.size() * sizeof(int)));
-  KMI_HIP(c, grow((void**)&B->partials, B->partials_cap, size_t(2) * B->nslots * sizeof(double)));
+  KMI_HIP(c, grow((void**)&B->partials, B->partials_cap, size_t(4) * B->nslots * sizeof(double)));
   KMI_HIP(c, hipMemcpy(B->slots_d, slots.data(), slots.size() * sizeof(int), hipMemcpyHostToDevice));
   KMI_HIP(c, hipMemcpy(B->cav_slots_d, cav_slots.data(), cav_slots.size() * sizeof(int), hipMemcpyHostToDevice));
   KMI_HIP(c, hipMemcpy(B->cav_range_d, range.data(), range.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -650,7 +664,7 @@ static int batch_record_block(kmi_batch* B, int64_t check, int64_t window, hipSt
   a = batch_args(B);
   e = launch_batch_resid(B->desc.storage, a, B->f[B->cur], 1, batch_groups(B), s);
   if (e != hipSuccess) return hip_fail(B->ctx, e, "batch residual");
-  hipLaunchKernelGGL(k_batch_final, dim3(unsigned((B->ncav + 63) / 64)), dim3(64), 0, s,
+  hipLaunchKernelGGL(k_batch_final, dim3(unsigned(B->ncav)), dim3(64), 0, s,
                      B->partials, B->cav_slots_d, B->cav_range_d, B->active_d, B->ncav,
                      double(window), B->resid_d);
   e = hipGetLastError();

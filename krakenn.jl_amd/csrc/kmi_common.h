@@ -54,6 +54,19 @@ struct Lat<19> {
                                    1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
 };
 
+// ---- compensated FP64 sums (Neumaier): a running sum s with its error term c, value s + c.
+// Built with -ffp-contract=off, so the error terms are exact.
+KMI_HD void nsum_add(double& s, double& c, double x) {
+  const double t = s + x;
+  c += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
+  s = t;
+}
+// Fold the compensated pair (s2, c2) into (s, c).
+KMI_HD void nsum_merge(double& s, double& c, double s2, double c2) {
+  nsum_add(s, c, s2);
+  c += c2;
+}
+
 // 6 w_q of the Ladd moving-wall term, as the reference writes it (T(1/6) for w = 1/36).
 KMI_HD constexpr double six_w(double w) {
   return w == 1.0 / 36.0 ? 1.0 / 6.0 : (w == 1.0 / 18.0 ? 1.0 / 3.0 : (w == 1.0 / 9.0 ? 2.0 / 3.0 : 2.0));

@@ -9,7 +9,7 @@ struct kmi_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
-  double* partials = nullptr;  // 2048 groups × 3 values
+  double* partials = nullptr;  // 2048 groups × 3 values × (sum, error)
   double* out = nullptr;       // 8 values
 };
 

@@ -484,9 +484,13 @@ static int nn_alloc(kmi_nn* N) {
   const long long a3 = (long long)B * N->H1 * N->H1 * d.c1, a4 = (long long)B * N->H2 * N->H2 * d.c2;
   const long long a5 = out_elems(N, B);
   const long long big = std::max({a1, a2, a3, a4, a5});
-  // largest split-K partial: weight gradients (≤ 9·max(c0,c1,c2) × max(c)) × splits, or the
-  // forward products (no split) of up to `big` elements
-  N->P_elems = size_t(std::max<long long>(big, 64LL * 1024 * 1024 / 4));
+  // P holds one split of every product (pick_splits adds splits only while they fit): the
+  // activations above, the data gradients (B·Hi²·Ci per stage, at most `big`), and the
+  // weight gradients of the dense layers (in_dim × hidden, hidden × L) and of the three
+  // convolutions (9 Ci × Co).
+  const long long wmax = std::max({(long long)d.in_dim * d.hidden, (long long)d.hidden * N->L,
+                                   9LL * d.latent_c * d.c1, 9LL * d.c1 * d.c2, 9LL * d.c2 * d.out_c});
+  N->P_elems = size_t(std::max({big, wmax, 64LL * 1024 * 1024 / 4}));
   hipError_t e = hipSuccess;
   auto al = [&](void** p, size_t bytes) { if (e == hipSuccess) e = hipMalloc(p, std::max<size_t>(bytes, 16)); };
   al((void**)&N->w, o * 4);
@@ -881,39 +885,62 @@ int kmi_nn_train_epochs(kmi_nn* N, int32_t epochs, int32_t B, float lr, uint64_t
 
 // C = A·B for host FP32 matrices, products in BF16 on MFMA with FP32 accumulation.
 // A(m, k) = A[m*sam + k*sak], B(k, n) = B[k*sbk + n*sbn], C row-major M×N.
-int kmi_gemm_bf16(kmi_ctx* ctx, int32_t M, int32_t Nn, int32_t K, const float* A, int64_t sam, int64_t sak,
-                  const float* B, int64_t sbk, int64_t sbn, int32_t splits, float* C) {
-  if (!ctx || !A || !B || !C || M < 1 || Nn < 1 || K < 1 || splits < 1)
-    return fail(ctx, KMI_ERR_ARG, "kmi_gemm_bf16: bad arguments");
-  KMI_HIP(ctx, hipSetDevice(ctx->device));
-  const long long na = (long long)(M - 1) * sam + (long long)(K - 1) * sak + 1;
-  const long long nb = (long long)(K - 1) * sbk + (long long)(Nn - 1) * sbn + 1;
-  float *dA = nullptr, *dB = nullptr, *P = nullptr;
-  uint16_t *bA = nullptr, *bB = nullptr;
-  hipError_t e = hipMalloc(&dA, na * 4);
-  if (e == hipSuccess) e = hipMalloc(&dB, nb * 4);
-  if (e == hipSuccess) e = hipMalloc(&bA, na * 2);
-  if (e == hipSuccess) e = hipMalloc(&bB, nb * 2);
-  if (e == hipSuccess) e = hipMalloc(&P, (size_t)splits * M * Nn * 4);
-  if (e == hipSuccess) e = hipMemcpy(dA, A, na * 4, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(dB, B, nb * 4, hipMemcpyHostToDevice);
-  hipStream_t s = ctx->stream;
+// Device buffers: dA, dB (FP32 copies), bA, bB (BF16 copies), P (split-K partials, `sp`
+// splits of M·N; the sum lands in P's first M·N).
+struct GemmBufs {
+  float *dA, *dB, *P;
+  uint16_t *bA, *bB;
+};
+struct GemmShape {
+  long long na, nb;  // elements of A and B spanned by the strides
+  int splits;        // splits asked for
+  int sp;            // splits gemm() runs after rounding the K chunks to whole K-steps
+};
+static bool gemm_shape(int32_t M, int32_t Nn, int32_t K, int64_t sam, int64_t sak, int64_t sbk, int64_t sbn,
+                       int32_t splits, GemmShape& g) {
+  if (M < 1 || Nn < 1 || K < 1 || splits < 1 || sam < 0 || sak < 0 || sbk < 0 || sbn < 0) return false;
+  g.na = (long long)(M - 1) * sam + (long long)(K - 1) * sak + 1;
+  g.nb = (long long)(K - 1) * sbk + (long long)(Nn - 1) * sbn + 1;
+  const int kchunk = ((K + splits - 1) / splits + 15) / 16 * 16;  // as gemm() rounds it
+  g.splits = splits;
+  g.sp = (K + kchunk - 1) / kchunk;
+  return true;
+}
+static hipError_t gemm_bf16_run(hipStream_t s, int32_t M, int32_t Nn, int32_t K, const float* A, int64_t sam,
+                                int64_t sak, const float* B, int64_t sbk, int64_t sbn, const GemmShape& g,
+                                const GemmBufs& d, float* C) {
+  hipError_t e = hipMemcpyAsync(d.dA, A, g.na * 4, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(d.dB, B, g.nb * 4, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_to_bf16, g1(na), dim3(256), 0, s, dA, na, bA);
-    hipLaunchKernelGGL(k_to_bf16, g1(nb), dim3(256), 0, s, dB, nb, bB);
-    e = gemm(M, Nn, K, LoadDense{bA, sam, sak, M, K}, LoadDense{bB, sbk, sbn, K, Nn}, P, splits, s);
+    hipLaunchKernelGGL(k_to_bf16, g1(g.na), dim3(256), 0, s, d.dA, g.na, d.bA);
+    hipLaunchKernelGGL(k_to_bf16, g1(g.nb), dim3(256), 0, s, d.dB, g.nb, d.bB);
+    e = gemm(M, Nn, K, LoadDense{d.bA, sam, sak, M, K}, LoadDense{d.bB, sbk, sbn, K, Nn}, d.P, g.splits, s);
   }
   if (e == hipSuccess) {
-    int kchunk = ((K + splits - 1) / splits + 15) / 16 * 16;
-    const int sp = (K + kchunk - 1) / kchunk;
     const long long mn = (long long)M * Nn;
-    hipLaunchKernelGGL(k_epilogue, g1(mn), dim3(256), 0, s, P, sp, mn, Nn, (const float*)nullptr, 0, P,
+    hipLaunchKernelGGL(k_epilogue, g1(mn), dim3(256), 0, s, d.P, g.sp, mn, Nn, (const float*)nullptr, 0, d.P,
                        (float*)nullptr, (uint16_t*)nullptr);
     e = hipGetLastError();
   }
-  if (e == hipSuccess) e = hipMemcpyAsync(C, P, (size_t)M * Nn * 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(C, d.P, (size_t)M * Nn * 4, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
-  for (void* p : {(void*)dA, (void*)dB, (void*)bA, (void*)bB, (void*)P})
+  return e;
+}
+
+int kmi_gemm_bf16(kmi_ctx* ctx, int32_t M, int32_t Nn, int32_t K, const float* A, int64_t sam, int64_t sak,
+                  const float* B, int64_t sbk, int64_t sbn, int32_t splits, float* C) {
+  GemmShape g;
+  if (!ctx || !A || !B || !C || !gemm_shape(M, Nn, K, sam, sak, sbk, sbn, splits, g))
+    return fail(ctx, KMI_ERR_ARG, "kmi_gemm_bf16: bad arguments");
+  KMI_HIP(ctx, hipSetDevice(ctx->device));
+  GemmBufs d{nullptr, nullptr, nullptr, nullptr, nullptr};
+  hipError_t e = hipMalloc(&d.dA, g.na * 4);
+  if (e == hipSuccess) e = hipMalloc(&d.dB, g.nb * 4);
+  if (e == hipSuccess) e = hipMalloc(&d.bA, g.na * 2);
+  if (e == hipSuccess) e = hipMalloc(&d.bB, g.nb * 2);
+  if (e == hipSuccess) e = hipMalloc(&d.P, (size_t)g.sp * M * Nn * 4);
+  if (e == hipSuccess) e = gemm_bf16_run(ctx->stream, M, Nn, K, A, sam, sak, B, sbk, sbn, g, d, C);
+  for (void* p : {(void*)d.dA, (void*)d.dB, (void*)d.bA, (void*)d.bB, (void*)d.P})
     if (p) (void)hipFree(p);
   if (e != hipSuccess) return hip_fail(ctx, e, "kmi_gemm_bf16");
   return KMI_OK;

@@ -211,7 +211,7 @@ def desc_init(stencil: int, nx: int, ny: int, nz: int, nu: float) -> KmiDesc:
 
 
 def trt_rates(nu: float, Lambda: float = 3.0 / 16.0):
-    """(s_plus, s_minus) of src/kernels/fused_trt_2d.jl:201-205, computed by the library."""
+    """(s_plus, s_minus) of src/kernels/fused_trt_2d.jl:129-133, computed by the library."""
     sp, sm = c_double(), c_double()
     load().kmi_trt_rates(nu, Lambda, ctypes.byref(sp), ctypes.byref(sm))
     return sp.value, sm.value

@@ -48,3 +48,20 @@ def test_more_than_one_gpu_is_refused():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_traffic_record_holds_for_its_launch_shape_only():
+    """A counter record applies to the tile, z-ranges and store policy it was measured with."""
+    with open(os.path.join(ROOT, "profiles", "traffic.json")) as fh:
+        recs = json.load(fh)
+    for key, entry in recs.items():
+        if not key.startswith(("k_step2", "k_step3")):
+            continue
+        kernel, storage = key.rsplit("_trt_", 1)[0], key.rsplit("_", 2)[-2]
+        for rec in entry if isinstance(entry, list) else [entry]:
+            shape = dict(rec["launch"])
+            assert bench.traffic_record(kernel, storage, 256, shape) is not None, key
+            flipped = {**shape, "nontemporal": not shape["nontemporal"]}
+            got = bench.traffic_record(kernel, storage, 256, flipped)
+            others = [r for r in (entry if isinstance(entry, list) else [entry]) if r["launch"] == flipped]
+            assert got == (others[0]["bytes_per_launch"] if others else None), key
