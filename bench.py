@@ -46,11 +46,12 @@ def launch_span(launch):
     return 3 if 6 <= t <= 11 else (2 if t >= 1 else 1)
 
 
-def traffic_record(kernel, storage, N, launch):
-    """Counted HBM bytes per launch for this kernel, storage and launch shape, or None."""
+def traffic_record(kernel, storage, N, launch, tag="trt"):
+    """Counted HBM bytes per launch for this kernel, workload tag, storage and launch shape,
+    or None."""
     try:
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as fh:
-            recs = json.load(fh).get(f"{kernel}_trt_{storage}_{N}")
+            recs = json.load(fh).get(f"{kernel}_{tag}_{storage}_{N}")
     except (OSError, ValueError):
         return None
     recs = recs if isinstance(recs, list) else ([recs] if recs else [])
@@ -93,6 +94,27 @@ def cpu_baseline(N, nu, storage, budget_s=12.0, max_steps=30):
         "sample": f"D3Q19 TRT {'FP64' if storage == 'f64' else 'FP32'} {N}^3 periodic Taylor-Green, "
                   f"{steps} steps ({el:.1f} s) of the C++ oracle fused pull+TRT, OpenMP",
     }
+
+
+def cpu_baseline_chain(desc, N, storage, budget_s=12.0, max_steps=20):
+    """The oracle's descriptor-driven chain (the reference drivers' step order) on the host
+    cores from rest, timed on a bounded sample."""
+    import numpy as np
+
+    sys.path.insert(0, ROOT)
+    from oracle import pyoracle as po
+
+    dt = np.float64 if storage == "f64" else np.float32
+    f = po.rest((N, N, N), dtype=dt)
+    po.run(desc, f, None, 1)  # warm-up (page-in)
+    steps, t0 = 0, time.perf_counter()
+    while steps < max_steps and time.perf_counter() - t0 < budget_s:
+        po.run(desc, f, None, 1)
+        steps += 1
+    el = time.perf_counter() - t0
+    return {"value": round(N ** 3 * steps / el / 1e6, 2), "unit": "MLUPS", "cores": po.threads(), "kind": "port",
+            "sample": f"{steps} steps ({el:.1f} s) of the C++ oracle chain, {'FP64' if dt is np.float64 else 'FP32'}, "
+                      f"same configuration, OpenMP"}
 
 
 def bench_cavities(args):
@@ -327,7 +349,8 @@ def bench_warmstart(args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="tgv", choices=["tgv", "cavities", "warmstart", "cavity", "rb", "nn"])
+    ap.add_argument("--workload", default="tgv",
+                    choices=["tgv", "cavity3d", "cavities", "warmstart", "cavity", "rb", "nn"])
     ap.add_argument("--ws-train", type=int, default=2048)
     ap.add_argument("--ws-test", type=int, default=256)
     ap.add_argument("--ws-epochs", type=int, default=150)
@@ -371,13 +394,22 @@ def main():
 
     import kraken_mi355x as kmi
 
-    N, nu, U0 = args.n, 0.01, 0.05
-    fields = kmi.taylor_green_3d_fields(N, U0)
+    cavity = args.workload == "cavity3d"
+    # the cube at Re = U0 N / nu = 256 (tau = 0.65); the periodic TGV at nu = 0.01
+    N, nu, U0 = args.n, (0.05 if cavity else 0.01), 0.05
+    fields = None if cavity else kmi.taylor_green_3d_fields(N, U0)
+
+    def make(storage):
+        # run_cavity_3d's cube (halfway-BB walls, Zou-He lid u = (U0, 0, 0)) or the periodic TGV
+        if cavity:
+            return kmi.cavity_3d_desc(N, N, N, nu, U0, storage=storage)
+        return kmi.taylor_green_3d_desc(N, nu, storage)
 
     def run(storage, steps, warmup):
-        desc = kmi.taylor_green_3d_desc(N, nu, storage)
+        desc = make(storage)
         with kmi.Lattice(desc) as lat:
-            lat.init_equilibrium(*fields)
+            if fields is not None:
+                lat.init_equilibrium(*fields)
             if args.temporal:
                 tile, zc, *nt = (int(v) for v in args.temporal.split(","))
                 lat.set_launch(256, 1, bool(nt[0]) if nt else False)
@@ -408,15 +440,18 @@ def main():
     # launch shape (tile, z-ranges, non-temporal stores): a record holds for its shape only.
     span = launch_span(launch)
     kernel = {3: "k_step3_d3q19", 2: "k_step2_d3q19", 1: "k_step"}[span]
-    traffic = traffic_record(kernel, args.storage, N, launch)
+    traffic = traffic_record(kernel, args.storage, N, launch, tag="bgk_cavity" if cavity else "trt")
     cells = N ** 3
     mlups = cells * args.steps / wall / 1e6
     launch_s = launch_ms / 1e3
     min_bytes = cells * 19 * 2 * BYTES_PER_POP[args.storage]  # one read + one write per launch
     achieved = min_bytes / launch_s / 1e9
+    name = (f"D3Q19 BGK lid-driven cube {N}^3 (run_cavity_3d: halfway-BB walls, Zou-He lid U0={U0}), "
+            f"nu={nu}, rest start" if cavity else
+            f"D3Q19 TRT Lambda=3/16 triply periodic Taylor-Green {N}^3, U0={U0}, nu={nu}")
     out = {
-        "metric": ("MLUPS (D3Q19 TRT 256^3 Taylor-Green, " + STORAGE_LABEL[args.storage] +
-                   ") and % of HBM roofline"),
+        "metric": (("MLUPS (D3Q19 BGK 256^3 walled cavity, " if cavity else "MLUPS (D3Q19 TRT 256^3 Taylor-Green, ")
+                   + STORAGE_LABEL[args.storage] + ") and % of HBM roofline"),
         "value": round(mlups, 1),
         "unit": "MLUPS",
         "n_gpus": 1,
@@ -428,9 +463,8 @@ def main():
         "vs_baseline": None,
         "dtype": ARITH_DTYPE[args.storage],
         "storage_dtype": STORAGE_DTYPE[args.storage],
-        "data": "synthetic (analytic Taylor-Green initial field, no dataset)",
-        "config": {"workload": f"D3Q19 TRT Lambda=3/16 triply periodic Taylor-Green {N}^3, "
-                               f"U0={U0}, nu={nu}, storage={args.storage}",
+        "data": "synthetic (rest start)" if cavity else "synthetic (analytic Taylor-Green initial field, no dataset)",
+        "config": {"workload": f"{name}, storage={args.storage}",
                    "cells": cells, "storage": args.storage, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
@@ -459,11 +493,13 @@ def main():
                        "dtype": ARITH_DTYPE[st], "storage_dtype": STORAGE_DTYPE[st], "launch": l2}
             log(f"[bench] {st}: {var[st]['MLUPS']} MLUPS, launch {l2}")
         out["variants"] = var
-    if not args.no_warmstart:
+    if not args.no_warmstart and not cavity:
         from kraken_mi355x import warmstart as ws
 
         out["warm_start"] = ws.experiment(log=log)
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and cavity:
+        out["cpu_baseline"] = cpu_baseline_chain(make("f64" if args.storage == "f64" else "f32"), N, args.storage)
+    elif not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(N, nu, args.storage)
         # the FP64 restatement on the same host cores, for the FP64 variant and reference
         out["cpu_baseline_f64"] = cpu_baseline(N, nu, "f64", budget_s=6.0, max_steps=10)

@@ -319,7 +319,9 @@ int kmi_reduce(kmi_lattice* lat, int32_t kind, int32_t arg, double* out);
 int kmi_lattice_steps(const kmi_lattice* lat, int64_t* steps);
 /* Set the step counter (a restored checkpoint resumes its cycle count, state.jl:58-70). */
 int kmi_lattice_set_steps(kmi_lattice* lat, int64_t steps);
-/* Bytes of device memory held by the lattice. */
+/* Bytes of device memory held by the lattice: the population buffers, mask, residual
+ * snapshot, and the staging block of host transfers (uploads, downloads, moments and
+ * initialisations reuse it; it grows to the largest transfer and is kept). */
 int kmi_lattice_device_bytes(const kmi_lattice* lat, int64_t* bytes);
 
 /* ---- launch tuning (no reference counterpart: KA picks its own workgroup size;
@@ -330,8 +332,12 @@ int kmi_lattice_set_launch(kmi_lattice* lat, int32_t bx, int32_t by, int32_t non
 int kmi_lattice_get_launch(const kmi_lattice* lat, int32_t* bx, int32_t* by, int32_t* nontemporal);
 /* Several steps per launch (temporal blocking); the intermediate populations stay in LDS,
  * so HBM sees one read and one write of the field per launch.  tile 0: off.
- * Fully periodic D3Q19 BGK/TRT lattices without mask and with nz >= 4 (nx and ny multiples
- * of the tile; zchunks: z-ranges per tile column, 1..nz):
+ * D3Q19 BGK/TRT lattices without thermal populations, nz >= 4, nx and ny multiples of the
+ * tile (zchunks: z-ranges per tile column, 1..nz).  Fully periodic lattices without mask
+ * take every tile below; lattices with walls, moving walls (Ladd), constant Zou–He velocity
+ * or pressure faces or a solid mask (no forcing, no cut links; not F16S) take tiles 12 and
+ * 14, and 9 except in FP64 — a link bounced at a wall reads the cell's own opposite
+ * population of the previous step from LDS, as the single-step pull reads it from HBM:
  *   two steps, one work-item per interior cell: 1 = 32x8, 2 = 64x4, 3 = 64x8, 4 = 32x16,
  *     5 = 64x16 (not F64);
  *   two steps, one work-item per cell of the tile and its one-cell ring: 12 = 64x8,

@@ -36,6 +36,8 @@ struct kmi_lattice {
   double* link_qw = nullptr;
   double* link_uw = nullptr;
   void* ffield = nullptr;        // per-node body force (KMI_FORCE_GUO_FIELD), Store::real planes
+  void* stage = nullptr;         // device staging of host transfers (grown on demand, kept)
+  size_t stage_bytes = 0;
   int cur = 0;
   int64_t steps = 0;
   StepArgs args{};
@@ -392,8 +394,10 @@ static void default_geometry(kmi_lattice* L) {
   set_geometry(L, bx, 1, field_bytes > 512e6 ? 1 : 0);
 }
 
-// Whether multi-step launches with `tile` apply.  D3Q19: fully periodic BGK/TRT, no mask,
-// the x–y extent a multiple of the tile, and at least 4 planes in z.  D2Q9 (k_step_tb2d):
+// Whether multi-step launches with `tile` apply.  D3Q19: BGK/TRT without thermal populations,
+// the x–y extent a multiple of the tile and at least 4 planes in z; fully periodic lattices
+// without mask on any tile, walled ones (walls, moving walls, constant Zou–He faces, solid
+// mask; no forcing, no cut links) on the walled tiles (12, 14; 9 except FP64; not F16S).  D2Q9 (k_step_tb2d):
 // no cut links, faces periodic / wall / moving wall / constant Zou–He only (thermal lattices
 // included), and a lattice of at least the tile plus its ring on each axis.
 static bool tb_eligible(const kmi_lattice* L, int tile) {
@@ -407,9 +411,17 @@ static bool tb_eligible(const kmi_lattice* L, int tile) {
   }
   int TX, TY;
   if (!step2_tile(tile, L->desc.storage, TX, TY)) return false;
-  return !L->bnd && L->Q == 19 && !L->desc.thermal &&
-         (L->desc.collision == KMI_BGK || L->desc.collision == KMI_TRT) &&
-         L->desc.nx % TX == 0 && L->desc.ny % TY == 0 && L->args.nz >= 4;
+  if (L->Q != 19 || L->desc.thermal || (L->desc.collision != KMI_BGK && L->desc.collision != KMI_TRT) ||
+      L->desc.nx % TX != 0 || L->desc.ny % TY != 0 || L->args.nz < 4)
+    return false;
+  if (!L->bnd) return true;
+  // walled lattices: halfway / moving-wall bounce-back, constant Zou–He faces and solid masks,
+  // without forcing or cut links, on the walled tiles
+  if (!step2_tile_walled(tile, L->desc.storage) || L->desc.forcing != KMI_FORCE_NONE || !L->cut_cells.empty())
+    return false;
+  for (int f = 0; f < 6; ++f)
+    if (L->desc.faces[f].kind > KMI_FACE_ZOUHE_PRESSURE) return false;
+  return true;
 }
 
 // Steps advanced by one launch.
@@ -440,7 +452,7 @@ static int launch_one(kmi_lattice* L, hipStream_t s, bool multi = false) {
       multi ? (L->Q == 9 ? launch_step_tb2d(L->desc.storage, L->desc.collision, L->tb_tile,
                                             L->desc.thermal != 0, a, s)
                          : launch_step2(L->desc.storage, L->desc.collision, L->tb_tile, L->tb_zchunks,
-                                        L->nts != 0, a, s))
+                                        L->nts != 0, L->bnd, a, s))
             : launch_step(L->Q, L->desc.storage, L->desc.collision, L->desc.thermal != 0, L->bnd,
                           L->nts != 0, a, L->grid, L->block, s);
   if (e != hipSuccess) return hip_fail(L->ctx, e, "step kernel launch");
@@ -600,6 +612,7 @@ int kmi_lattice_destroy(kmi_lattice* L) {
                   (void*)L->link_qw, (void*)L->link_uw})
     if (p) (void)hipFree(p);
   if (L->ffield) (void)hipFree(L->ffield);
+  if (L->stage) (void)hipFree(L->stage);
   delete L;
   return KMI_OK;
 }
@@ -794,6 +807,23 @@ static int field_ptrs(kmi_lattice* L, int field, void**& buf, double& shift, int
   return fail(L->ctx, KMI_ERR_ARG, "unknown field or lattice without thermal populations");
 }
 
+// Device staging for host transfers, owned by the lattice: grown when a transfer needs more,
+// then reused, so repeated uploads, downloads, moments and initialisations (a run_simulation
+// callback every few steps) allocate nothing.  The stream is idle when it is replaced.
+static hipError_t stage_reserve(kmi_lattice* L, size_t bytes, double** p) {
+  if (L->stage_bytes < bytes) {
+    hipError_t e = hipStreamSynchronize(L->ctx->stream);
+    if (e != hipSuccess) return e;
+    if (L->stage) (void)hipFree(L->stage);
+    L->stage = nullptr;
+    L->stage_bytes = 0;
+    if ((e = hipMalloc(&L->stage, bytes)) != hipSuccess) return e;
+    L->stage_bytes = bytes;
+  }
+  *p = static_cast<double*>(L->stage);
+  return hipSuccess;
+}
+
 int kmi_upload(kmi_lattice* L, int32_t field, const void* host, int32_t host_dtype) {
   if (!L) return fail(nullptr, KMI_ERR_ARG, "kmi_upload: lattice is NULL");
   kmi_ctx* c = L->ctx;
@@ -821,12 +851,11 @@ int kmi_upload(kmi_lattice* L, int32_t field, const void* host, int32_t host_dty
     src = tmp.data();
   }
   double* stage = nullptr;
-  KMI_HIP(c, hipMalloc(&stage, n * sizeof(double)));
+  KMI_HIP(c, stage_reserve(L, n * sizeof(double), &stage));
   hipError_t e = hipMemcpyAsync(stage, src, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess)
     e = launch_to_storage(L->Q, st, stage, buf[L->cur], L->ncells, L->plane, shift, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  (void)hipFree(stage);
   if (e != hipSuccess) return hip_fail(c, e, "kmi_upload");
   if (field == KMI_FIELD_F) L->steps = 0;
   return KMI_OK;
@@ -850,7 +879,7 @@ int kmi_download(kmi_lattice* L, int32_t field, void* host, int32_t host_dtype) 
     return KMI_OK;
   }
   double* stage = nullptr;
-  KMI_HIP(c, hipMalloc(&stage, n * sizeof(double)));
+  KMI_HIP(c, stage_reserve(L, n * sizeof(double), &stage));
   std::vector<double> tmp;
   double* dst = static_cast<double*>(host);
   if (host_dtype == KMI_HOST_F32) {
@@ -862,7 +891,6 @@ int kmi_download(kmi_lattice* L, int32_t field, void* host, int32_t host_dtype) 
   if (e == hipSuccess)
     e = hipMemcpyAsync(dst, stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  (void)hipFree(stage);
   if (e != hipSuccess) return hip_fail(c, e, "kmi_download");
   if (host_dtype == KMI_HOST_F32) {
     float* h = static_cast<float*>(host);
@@ -871,7 +899,8 @@ int kmi_download(kmi_lattice* L, int32_t field, void* host, int32_t host_dtype) 
   return KMI_OK;
 }
 
-// Copies up to `count` host arrays of ncells doubles to one device block (NULL stays NULL).
+// Copies up to `count` host arrays of ncells doubles to the lattice's staging block (NULL
+// stays NULL).
 static int stage_fields(kmi_lattice* L, const double* const* host, int count, double** dev,
                         double*& block) {
   kmi_ctx* c = L->ctx;
@@ -880,18 +909,14 @@ static int stage_fields(kmi_lattice* L, const double* const* host, int count, do
   block = nullptr;
   for (int k = 0; k < count; ++k) dev[k] = nullptr;
   if (used == 0) return KMI_OK;
-  KMI_HIP(c, hipMalloc(&block, size_t(used) * L->ncells * sizeof(double)));
+  KMI_HIP(c, stage_reserve(L, size_t(used) * L->ncells * sizeof(double), &block));
   int slot = 0;
   for (int k = 0; k < count; ++k) {
     if (!host[k]) continue;
     dev[k] = block + size_t(slot++) * L->ncells;
     hipError_t e = hipMemcpyAsync(dev[k], host[k], size_t(L->ncells) * sizeof(double),
                                   hipMemcpyHostToDevice, c->stream);
-    if (e != hipSuccess) {
-      (void)hipFree(block);
-      block = nullptr;
-      return hip_fail(c, e, "staging host field");
-    }
+    if (e != hipSuccess) return hip_fail(c, e, "staging host field");
   }
   return KMI_OK;
 }
@@ -908,7 +933,6 @@ int kmi_init_equilibrium(kmi_lattice* L, const double* rho, const double* ux, co
   hipError_t e = launch_init_eq(L->Q, L->desc.storage, L->f[L->cur], dev[0], dev[1], dev[2],
                                 dev[3], L->ncells, L->plane, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  if (block) (void)hipFree(block);
   if (e != hipSuccess) return hip_fail(c, e, "kmi_init_equilibrium");
   L->steps = 0;
   return KMI_OK;
@@ -926,7 +950,6 @@ int kmi_init_temperature(kmi_lattice* L, const double* T) {
   hipError_t e = launch_init_temp(L->Q, L->desc.storage, L->g[L->cur], dev[0], L->ncells,
                                   L->plane, L->desc.T_ref, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  if (block) (void)hipFree(block);
   if (e != hipSuccess) return hip_fail(c, e, "kmi_init_temperature");
   return KMI_OK;
 }
@@ -1021,7 +1044,7 @@ int kmi_moments(kmi_lattice* L, double* rho, double* ux, double* uy, double* uz,
     for (long long i = 0; i < L->ncells; ++i) uz[i] = 0.0;
   if (used == 0) return KMI_OK;
   double* block = nullptr;
-  KMI_HIP(c, hipMalloc(&block, size_t(used) * L->ncells * sizeof(double)));
+  KMI_HIP(c, stage_reserve(L, size_t(used) * L->ncells * sizeof(double), &block));
   double* dev[5];
   int slot = 0;
   for (int k = 0; k < 5; ++k) dev[k] = host[k] ? block + size_t(slot++) * L->ncells : nullptr;
@@ -1034,7 +1057,6 @@ int kmi_moments(kmi_lattice* L, double* rho, double* ux, double* uy, double* uz,
       e = hipMemcpyAsync(host[k], dev[k], size_t(L->ncells) * sizeof(double),
                          hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  (void)hipFree(block);
   if (e != hipSuccess) return hip_fail(c, e, "kmi_moments");
   return KMI_OK;
 }
@@ -1161,8 +1183,11 @@ int kmi_lattice_set_temporal(kmi_lattice* L, int32_t tile, int32_t zchunks) {
   if (!L) return fail(nullptr, KMI_ERR_ARG, "kmi_lattice_set_temporal: lattice is NULL");
   if (tile != 0 && !tb_eligible(L, tile))
     return fail(L->ctx, KMI_ERR_UNSUPPORTED,
-                "kmi_lattice_set_temporal: multi-step launches need a fully periodic D3Q19 "
-                "BGK/TRT lattice without mask, nz >= 4, whose nx, ny are multiples of the tile "
+                "kmi_lattice_set_temporal: multi-step launches need a D3Q19 BGK/TRT lattice "
+                "without thermal populations, nz >= 4, whose nx, ny are multiples of the tile, "
+                "fully periodic without mask, or with walls, moving walls, constant Zou-He faces "
+                "or a solid mask (no forcing, no cut links; tiles 12, 14 and 9 (not FP64), not "
+                "F16S) "
                 "(two steps 1: 32x8, 2: 64x4, 3: 64x8, 4: 32x16, 5: 64x16 (not FP64), "
                 "12: 64x8, 13: 32x16, 14: 32x8; three steps 6: 64x8 (not FP64), 7: 32x8, "
                 "8: 32x16 (not FP64), 9: 32x8, 10: 32x16 (not FP64), 11: 64x8 (not FP64)), "
@@ -1318,6 +1343,7 @@ int kmi_lattice_device_bytes(const kmi_lattice* L, int64_t* bytes) {
   if (L->desc.thermal) b += 2 * int64_t(L->plane) * L->Q * int64_t(L->gsize);
   if (L->mask) b += L->ncells;
   if (L->uprev) b += 3 * L->ncells * int64_t(sizeof(double));
+  b += int64_t(L->stage_bytes);
   *bytes = b;
   return KMI_OK;
 }

@@ -27,7 +27,10 @@ hipError_t launch_step(int q, int st, int coll, bool th, bool bnd, bool nts, con
 bool step2_tile(int tile, int st, int& TX, int& TY);
 // Steps one launch of `tile` advances: 2 (tiles 1–5, 12–14), 3 (tiles 6–11), 1 (tile 0).
 int temporal_span(int tile);
-hipError_t launch_step2(int st, int coll, int tile, int zchunks, bool nts, const StepArgs& a,
+// bnd: a lattice with walls, moving walls, Zou–He faces or a solid mask (step2_tile_walled
+// tiles only; non-temporal stores whatever `nts` says).
+bool step2_tile_walled(int tile, int st);
+hipError_t launch_step2(int st, int coll, int tile, int zchunks, bool nts, bool bnd, const StepArgs& a,
                         hipStream_t s);
 // K steps per launch on a D2Q9 lattice (kmi_step_tb2d.hip): `tile` selects tile and K
 // (tb2d_variant).

@@ -4,22 +4,24 @@
 // f_init in src/drivers/gnf_periodic_steady.jl:114-150):
 //   x (in_dim) → dense → SiLU → dense → SiLU → latent hw×hw×c0 (NHWC)
 //   → [nearest ×2, 3×3 conv, SiLU] → [×2, conv, SiLU] → [×2, conv] → (8hw)²×out_c.
-// Every matrix product is one kernel: an implicit GEMM on BF16 MFMA tiles
-// (v_mfma_f32_16x16x16_bf16, FP32 accumulate) whose operand loaders compute the
-// im2col / transposed-conv addresses on the fly (no im2col buffers).  Weights are kept
-// in FP32 (Adam master copy) and mirrored to BF16 for the products; activations and
-// gradients enter the products as BF16, everything else is FP32.
+// Every matrix product is one kernel (k_mm): BF16 operands on gfx950's
+// v_mfma_f32_16x16x32_bf16 with FP32 accumulation, staged through LDS in K-steps of 32 by
+// loaders that compute the im2col / transposed-conv addresses incrementally (no im2col
+// buffers), with the bias / SiLU / activation-gradient epilogues fused where K is not split.
+// Weights are kept in FP32 (Adam master copy) and mirrored to BF16 for the products;
+// activations and gradients enter the products as BF16, everything else is FP32.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "kmi_internal.h"
 
 namespace kmi {
 
-typedef short v4s __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 // FP32 → BF16, round to nearest even (finite inputs).
@@ -33,226 +35,450 @@ KMI_HD float bf2f(uint16_t h) {
   return __builtin_bit_cast(float, uint32_t(h) << 16);
 }
 
-// ---- operand loaders: element (i, j) of a logical matrix, 0 outside ------------------------
-struct LoadDense {  // p[i*s0 + j*s1]
+// ---- operand loaders ------------------------------------------------------------------------
+// A product C[M×N] = Σ_k A(m, k) B(k, n) reads each operand through a loader in units of eight
+// BF16 elements (16 bytes).  A K-major loader returns the eight consecutive k of one row
+// (m for A, n for B); a row-major one returns eight consecutive rows at one k.  Each thread
+// keeps a cursor per unit: the row decode happens once, and every K-step of 32 advances the
+// cursor incrementally (no division per element).  Elements outside the matrix read as 0.
+KMI_DEV bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+KMI_DEV uint4 ld16(const uint16_t* p) { return *reinterpret_cast<const uint4*>(p); }
+KMI_DEV uint4 pack8(const uint16_t (&v)[8]) {
+  return make_uint4(uint32_t(v[0]) | (uint32_t(v[1]) << 16), uint32_t(v[2]) | (uint32_t(v[3]) << 16),
+                    uint32_t(v[4]) | (uint32_t(v[5]) << 16), uint32_t(v[6]) | (uint32_t(v[7]) << 16));
+}
+constexpr uint4 kZero4 = {0u, 0u, 0u, 0u};
+
+// (row, k) = p[row·s0 + k], k contiguous.
+struct DenseK {
+  static constexpr bool kmajor = true;
+  const uint16_t* p;
+  long long s0;
+  int nrow, nk;
+  struct Cur {
+    const uint16_t* q;
+    int k;
+    bool ok;
+  };
+  KMI_DEV void init(Cur& c, int row, int k) const {
+    c.ok = row < nrow;
+    c.k = k;
+    c.q = p + (long long)(c.ok ? row : 0) * s0 + k;
+  }
+  KMI_DEV uint4 load(const Cur& c) const {
+    if (!c.ok || c.k >= nk) return kZero4;
+    if (c.k + 8 <= nk && aligned16(c.q)) return ld16(c.q);
+    uint16_t v[8];
+    for (int e = 0; e < 8; ++e) v[e] = c.k + e < nk ? c.q[e] : uint16_t(0);
+    return pack8(v);
+  }
+  KMI_DEV void next(Cur& c) const {
+    c.q += 32;
+    c.k += 32;
+  }
+};
+
+// (row, k) = p[row + k·s1], rows contiguous (eight consecutive rows per unit).
+struct DenseR {
+  static constexpr bool kmajor = false;
+  const uint16_t* p;
+  long long s1;
+  int nrow, nk;
+  struct Cur {
+    const uint16_t* q;
+    int row, k;
+  };
+  KMI_DEV void init(Cur& c, int row, int k) const {
+    c.row = row;
+    c.k = k;
+    c.q = p + row + (long long)k * s1;
+  }
+  KMI_DEV uint4 load(const Cur& c) const {
+    if (c.k >= nk || c.row >= nrow) return kZero4;
+    if (c.row + 8 <= nrow && aligned16(c.q)) return ld16(c.q);
+    uint16_t v[8];
+    for (int e = 0; e < 8; ++e) v[e] = c.row + e < nrow ? c.q[e] : uint16_t(0);
+    return pack8(v);
+  }
+  KMI_DEV void next(Cur& c) const {
+    c.q += 32 * s1;
+    c.k += 32;
+  }
+};
+
+// (row, k) = p[row·s0 + k·s1], any strides: element loads along k.
+struct DenseS {
+  static constexpr bool kmajor = true;
   const uint16_t* p;
   long long s0, s1;
-  int n0, n1;
-  KMI_DEV uint16_t operator()(int i, int j) const {
-    return (i < n0 && j < n1) ? p[(long long)i * s0 + (long long)j * s1] : uint16_t(0);
+  int nrow, nk;
+  struct Cur {
+    const uint16_t* q;
+    int k;
+    bool ok;
+  };
+  KMI_DEV void init(Cur& c, int row, int k) const {
+    c.ok = row < nrow;
+    c.k = k;
+    c.q = p + (long long)(c.ok ? row : 0) * s0 + (long long)k * s1;
+  }
+  KMI_DEV uint4 load(const Cur& c) const {
+    uint16_t v[8];
+    for (int e = 0; e < 8; ++e) v[e] = (c.ok && c.k + e < nk) ? c.q[(long long)e * s1] : uint16_t(0);
+    return pack8(v);
+  }
+  KMI_DEV void next(Cur& c) const {
+    c.q += 32 * s1;
+    c.k += 32;
   }
 };
 
-// Forward conv after nearest ×2 up-sampling, 3×3, zero padding 1, NHWC input [B, Hi, Hi, C]:
-// A(m = (b, y, x) on the Ho = 2Hi grid, k = (ky, kx, c)).  TRANS swaps (i, j) → (k, m)
-// (the weight-gradient product reads the same matrix transposed).
-template <bool TRANS>
-struct LoadConvUp {
+// Convolution after nearest ×2 up-sampling (3×3, zero padding 1) on an NHWC input
+// [B, Hi, Hi, C] (C a multiple of 8): row m = output pixel (b, y, x) of the Ho = 2 Hi grid,
+// k = (ky, kx, c) = tap·C + c, element in[b, (y+ky−1)/2, (x+kx−1)/2, c].
+struct ConvFwdA {
+  static constexpr bool kmajor = true;
   const uint16_t* in;
   int B, Hi, C;
-  KMI_DEV uint16_t operator()(int i, int j) const {
-    const int m = TRANS ? j : i, k = TRANS ? i : j;
-    const int Ho = 2 * Hi;
-    if (m >= B * Ho * Ho || k >= 9 * C) return 0;
-    const int b = m / (Ho * Ho), r = m % (Ho * Ho), y = r / Ho, x = r % Ho;
-    const int t = k / C, c = k % C, yy = y + t / 3 - 1, xx = x + t % 3 - 1;
-    if (yy < 0 || yy >= Ho || xx < 0 || xx >= Ho) return 0;
-    return in[(((long long)b * Hi + (yy >> 1)) * Hi + (xx >> 1)) * C + c];
-  }
-};
-
-// Data gradient through conv + up-sampling: A(m = (b, yi, xi) on the Hi grid,
-// k = (dy, dx, ky, kx, o)) = dZ[b, 2yi+dy−ky+1, 2xi+dx−kx+1, o] on the Ho grid.
-struct LoadConvUpGradIn {
-  const uint16_t* dz;
-  int B, Hi, Co;
-  KMI_DEV uint16_t operator()(int m, int k) const {
-    const int Ho = 2 * Hi;
-    if (m >= B * Hi * Hi || k >= 36 * Co) return 0;
-    const int b = m / (Hi * Hi), r = m % (Hi * Hi), yi = r / Hi, xi = r % Hi;
-    const int o = k % Co, s = k / Co, t = s % 9, d = s / 9;
-    const int y = 2 * yi + (d >> 1) - t / 3 + 1, x = 2 * xi + (d & 1) - t % 3 + 1;
-    if (y < 0 || y >= Ho || x < 0 || x >= Ho) return 0;
-    return dz[(((long long)b * Ho + y) * Ho + x) * Co + o];
-  }
-};
-// B(k = (dy, dx, ky, kx, o), n = c) = W[(ky, kx, c), o] with W row-major [9 Ci, Co].
-struct LoadConvWeightT {
-  const uint16_t* w;
-  int Ci, Co;
-  KMI_DEV uint16_t operator()(int k, int n) const {
-    if (k >= 36 * Co || n >= Ci) return 0;
-    const int o = k % Co, t = (k / Co) % 9;
-    return w[((long long)t * Ci + n) * Co + o];
-  }
-};
-
-// ---- four consecutive elements (i, j..j+3), 0 at j+e >= jend, packed into a uint2 ---------
-KMI_DEV uint2 pack4(const uint16_t (&v)[4]) {
-  return make_uint2(uint32_t(v[0]) | (uint32_t(v[1]) << 16), uint32_t(v[2]) | (uint32_t(v[3]) << 16));
-}
-KMI_DEV bool aligned8(const uint16_t* p) { return (reinterpret_cast<uintptr_t>(p) & 7u) == 0; }
-KMI_DEV uint2 ld8(const uint16_t* p) { return *reinterpret_cast<const uint2*>(p); }
-
-// Generic: four calls of the loader.
-template <class LD>
-KMI_DEV uint2 load4(const LD& ld, int i, int j, int jend) {
-  uint16_t v[4];
-  for (int e = 0; e < 4; ++e) v[e] = j + e < jend ? ld(i, j + e) : uint16_t(0);
-  return pack4(v);
-}
-// Dense row-major operand: one 8-byte load when the four elements are contiguous.
-KMI_DEV uint2 load4(const LoadDense& ld, int i, int j, int jend) {
-  if (i >= ld.n0) return make_uint2(0u, 0u);
-  const uint16_t* q = ld.p + (long long)i * ld.s0 + (long long)j * ld.s1;
-  if (ld.s1 == 1 && j + 3 < min(jend, ld.n1) && aligned8(q)) return ld8(q);
-  uint16_t v[4];
-  for (int e = 0; e < 4; ++e) v[e] = (j + e < jend && j + e < ld.n1) ? q[(long long)e * ld.s1] : uint16_t(0);
-  return pack4(v);
-}
-// Forward conv operand: the pixel m is decoded once; four consecutive k = (tap, c..c+3) of
-// one tap are four consecutive channels of one input pixel (C a multiple of 4).
-KMI_DEV uint2 load4(const LoadConvUp<false>& ld, int m, int k, int jend) {
-  const int Ho = 2 * ld.Hi, C = ld.C;
-  if (m >= ld.B * Ho * Ho) return make_uint2(0u, 0u);
-  const int b = m / (Ho * Ho), r = m % (Ho * Ho), y = r / Ho, x = r % Ho;
-  const uint16_t* base = ld.in + (long long)b * ld.Hi * ld.Hi * C;
-  if (C % 4 == 0 && k % 4 == 0 && k + 3 < min(jend, 9 * C)) {
-    const int t = k / C, c = k % C, yy = y + t / 3 - 1, xx = x + t % 3 - 1;
-    if (yy < 0 || yy >= Ho || xx < 0 || xx >= Ho) return make_uint2(0u, 0u);
-    const uint16_t* q = base + ((long long)(yy >> 1) * ld.Hi + (xx >> 1)) * C + c;
-    if (aligned8(q)) return ld8(q);
-  }
-  uint16_t v[4];
-  for (int e = 0; e < 4; ++e) {
-    const int kk = k + e;
-    v[e] = 0;
-    if (kk >= jend || kk >= 9 * C) continue;
-    const int t = kk / C, c = kk % C, yy = y + t / 3 - 1, xx = x + t % 3 - 1;
-    if (yy < 0 || yy >= Ho || xx < 0 || xx >= Ho) continue;
-    v[e] = base[((long long)(yy >> 1) * ld.Hi + (xx >> 1)) * C + c];
-  }
-  return pack4(v);
-}
-// Data-gradient operand: four consecutive k = (d, tap, o..o+3) are four consecutive output
-// channels of one output pixel (Co a multiple of 4).
-KMI_DEV uint2 load4(const LoadConvUpGradIn& ld, int m, int k, int jend) {
-  const int Hi = ld.Hi, Ho = 2 * Hi, Co = ld.Co;
-  if (m >= ld.B * Hi * Hi) return make_uint2(0u, 0u);
-  const int b = m / (Hi * Hi), r = m % (Hi * Hi), yi = r / Hi, xi = r % Hi;
-  const uint16_t* base = ld.dz + (long long)b * Ho * Ho * Co;
-  if (Co % 4 == 0 && k % 4 == 0 && k + 3 < min(jend, 36 * Co)) {
-    const int o = k % Co, s = k / Co, t = s % 9, d = s / 9;
-    const int y = 2 * yi + (d >> 1) - t / 3 + 1, x = 2 * xi + (d & 1) - t % 3 + 1;
-    if (y < 0 || y >= Ho || x < 0 || x >= Ho) return make_uint2(0u, 0u);
-    const uint16_t* q = base + ((long long)y * Ho + x) * Co + o;
-    if (aligned8(q)) return ld8(q);
-  }
-  uint16_t v[4];
-  for (int e = 0; e < 4; ++e) {
-    const int kk = k + e;
-    v[e] = 0;
-    if (kk >= jend || kk >= 36 * Co) continue;
-    const int o = kk % Co, s = kk / Co, t = s % 9, d = s / 9;
-    const int y = 2 * yi + (d >> 1) - t / 3 + 1, x = 2 * xi + (d & 1) - t % 3 + 1;
-    if (y < 0 || y >= Ho || x < 0 || x >= Ho) continue;
-    v[e] = base[((long long)y * Ho + x) * Co + o];
-  }
-  return pack4(v);
-}
-
-// Weight-gradient operand A(k, m..m+3): the tap and channel are decoded once; four
-// consecutive pixels (m a multiple of 4, Ho a multiple of 4) lie on one output row.
-KMI_DEV uint2 load4(const LoadConvUp<true>& ld, int k, int m, int jend) {
-  const int Hi = ld.Hi, Ho = 2 * Hi, C = ld.C, HW = Ho * Ho;
-  if (k >= 9 * C) return make_uint2(0u, 0u);
-  const int t = k / C, c = k % C, dy = t / 3 - 1, dx = t % 3 - 1;
-  uint16_t v[4] = {0, 0, 0, 0};
-  if (m % 4 == 0 && Ho % 4 == 0 && m + 3 < min(jend, ld.B * HW)) {
-    const int b = m / HW, r = m % HW, y = r / Ho, x = r % Ho, yy = y + dy;
-    if (yy >= 0 && yy < Ho) {
-      const uint16_t* row = ld.in + ((long long)b * Hi + (yy >> 1)) * Hi * C + c;
-      for (int e = 0; e < 4; ++e) {
-        const int xx = x + e + dx;
-        if (xx >= 0 && xx < Ho) v[e] = row[(long long)(xx >> 1) * C];
-      }
-    }
-    return pack4(v);
-  }
-  for (int e = 0; e < 4; ++e) v[e] = m + e < jend ? ld(k, m + e) : uint16_t(0);
-  return pack4(v);
-}
-// Data-gradient operand B(k, n..n+3) = W[(tap, n), o]: tap and o decoded once.
-KMI_DEV uint2 load4(const LoadConvWeightT& ld, int k, int n, int jend) {
-  if (k >= 36 * ld.Co) return make_uint2(0u, 0u);
-  const int o = k % ld.Co, t = (k / ld.Co) % 9;
-  const uint16_t* q = ld.w + (long long)t * ld.Ci * ld.Co + o;
-  uint16_t v[4];
-  for (int e = 0; e < 4; ++e)
-    v[e] = (n + e < jend && n + e < ld.Ci) ? q[(long long)(n + e) * ld.Co] : uint16_t(0);
-  return pack4(v);
-}
-
-// ---- split-K GEMM: P[z][m][n] = Σ_{k in split z} A(m,k) B(k,n) --------------------------------
-// 256 threads = 4 waves; block tile 64×64; each wave 32×32 = 2×2 MFMA 16×16×16 tiles.
-// MFMA operand layout (lane l): A row l%16, k = 4(l/16)..+3; B col l%16, same k;
-// D rows 4(l/16)..+3 of col l%16.
-template <class LA, class LB>
-__global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, int kchunk, LA la, LB lb,
-                                              float* __restrict__ out) {
-  __shared__ __align__(8) uint16_t As[64][16 + 4];  // rows of 40 B: 8-byte aligned quads
-  __shared__ __align__(8) uint16_t Bs[16][64 + 4];  // rows of 136 B
-  const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
-  const int kbeg = blockIdx.z * kchunk, kend = min(K, kbeg + kchunk);
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
-  v4f acc[2][2];
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
-  // A tile 64×16: thread → row t/4, 4 consecutive k; B tile 16×64: thread → k row t/16,
-  // 4 consecutive n.  The quads of K-step k0+16 are loaded into registers while the MFMAs of
-  // K-step k0 run, so the operand latency overlaps the products.
-  const int ar = t >> 2, ak = (t & 3) * 4, bk = t >> 4, bn = (t & 15) * 4;
-  auto fetch_a = [&](int k0) { return load4(la, m0 + ar, k0 + ak, kend); };
-  auto fetch_b = [&](int k0) {
-    return k0 + bk < kend ? load4(lb, k0 + bk, n0 + bn, 1 << 30) : make_uint2(0u, 0u);
+  struct Cur {
+    const uint16_t* base;  // image b
+    int y, x, ky, kx, c;
+    bool ok;
   };
-  uint2 qa = fetch_a(kbeg), qb = fetch_b(kbeg);
-  for (int k0 = kbeg; k0 < kend; k0 += 16) {
-    *reinterpret_cast<uint2*>(&As[ar][ak]) = qa;
-    *reinterpret_cast<uint2*>(&Bs[bk][bn]) = qb;
-    __syncthreads();
-    if (k0 + 16 < kend) {
-      qa = fetch_a(k0 + 16);
-      qb = fetch_b(k0 + 16);
-    }
-    const int kq = (lane >> 4) * 4;
-    for (int a = 0; a < 2; ++a) {
-      v4s af;
-      const int r = wm + a * 16 + (lane & 15);
-      af[0] = As[r][kq];
-      af[1] = As[r][kq + 1];
-      af[2] = As[r][kq + 2];
-      af[3] = As[r][kq + 3];
-      for (int b = 0; b < 2; ++b) {
-        v4s bfr;
-        const int c = wn + b * 16 + (lane & 15);
-        bfr[0] = Bs[kq][c];
-        bfr[1] = Bs[kq + 1][c];
-        bfr[2] = Bs[kq + 2][c];
-        bfr[3] = Bs[kq + 3][c];
-        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bfr, acc[a][b], 0, 0, 0);
-      }
-    }
-    __syncthreads();
+  KMI_DEV void init(Cur& u, int m, int k) const {
+    const int Ho = 2 * Hi, HW = Ho * Ho;
+    u.ok = m < B * HW;
+    const int mm = u.ok ? m : 0, b = mm / HW, r = mm - b * HW;
+    u.y = r / Ho;
+    u.x = r - u.y * Ho;
+    u.base = in + (long long)b * Hi * Hi * C;
+    const int t = k / C;
+    u.c = k - t * C;
+    u.ky = t / 3;
+    u.kx = t - 3 * u.ky;
   }
-  float* P = out + (long long)blockIdx.z * M * N;
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b)
-      for (int r = 0; r < 4; ++r) {
-        const int m = m0 + wm + a * 16 + (lane >> 4) * 4 + r, n = n0 + wn + b * 16 + (lane & 15);
-        if (m < M && n < N) P[(long long)m * N + n] = acc[a][b][r];
+  KMI_DEV uint4 load(const Cur& u) const {
+    const int Ho = 2 * Hi, yy = u.y + u.ky - 1, xx = u.x + u.kx - 1;
+    if (!u.ok || u.ky > 2 || yy < 0 || yy >= Ho || xx < 0 || xx >= Ho) return kZero4;
+    return ld16(u.base + ((long long)(yy >> 1) * Hi + (xx >> 1)) * C + u.c);
+  }
+  KMI_DEV void next(Cur& u) const {
+    u.c += 32;
+    while (u.c >= C) {
+      u.c -= C;
+      if (++u.kx == 3) {
+        u.kx = 0;
+        ++u.ky;
       }
+    }
+  }
+};
+
+// The same matrix read transposed for the weight gradient: row r = (tap, c) (eight
+// consecutive channels of one tap per unit), k = output pixel (b, y, x).
+struct ConvWgradA {
+  static constexpr bool kmajor = false;
+  const uint16_t* in;
+  int B, Hi, C;
+  struct Cur {
+    int b, y, x, p, ky, kx, c;
+    bool ok;
+  };
+  KMI_DEV void init(Cur& u, int r, int k) const {
+    const int Ho = 2 * Hi, HW = Ho * Ho;
+    u.ok = r < 9 * C;
+    const int t = u.ok ? r / C : 0;
+    u.c = u.ok ? r - t * C : 0;
+    u.ky = t / 3;
+    u.kx = t - 3 * u.ky;
+    u.p = k;
+    u.b = k / HW;
+    const int rr = k - u.b * HW;
+    u.y = rr / Ho;
+    u.x = rr - u.y * Ho;
+  }
+  KMI_DEV uint4 load(const Cur& u) const {
+    const int Ho = 2 * Hi, yy = u.y + u.ky - 1, xx = u.x + u.kx - 1;
+    if (!u.ok || u.p >= B * Ho * Ho || yy < 0 || yy >= Ho || xx < 0 || xx >= Ho) return kZero4;
+    return ld16(in + (((long long)u.b * Hi + (yy >> 1)) * Hi + (xx >> 1)) * C + u.c);
+  }
+  KMI_DEV void next(Cur& u) const {
+    const int Ho = 2 * Hi;
+    u.p += 32;
+    u.x += 32;
+    while (u.x >= Ho) {
+      u.x -= Ho;
+      if (++u.y == Ho) {
+        u.y = 0;
+        ++u.b;
+      }
+    }
+  }
+};
+
+// Data gradient through the convolution and the up-sampling, on the input grid: row
+// m = input pixel (b, yi, xi), k = (d, tap, o) = (4·9 index)·Cp + o with d = (dy, dx) the
+// output pixel's position in its 2×2 block; element dZ[b, 2yi+dy−ky+1, 2xi+dx−kx+1, o] of the
+// output gradient stored [B, Ho, Ho, Cp] (Cp a multiple of 8, channels ≥ Co zero).
+struct ConvDgradA {
+  static constexpr bool kmajor = true;
+  const uint16_t* dz;
+  int B, Hi, Cp;
+  struct Cur {
+    const uint16_t* base;
+    int yi, xi, d, ky, kx, o;
+    bool ok;
+  };
+  KMI_DEV void init(Cur& u, int m, int k) const {
+    const int HW = Hi * Hi;
+    u.ok = m < B * HW;
+    const int mm = u.ok ? m : 0, b = mm / HW, r = mm - b * HW;
+    u.yi = r / Hi;
+    u.xi = r - u.yi * Hi;
+    u.base = dz + (long long)b * 4 * HW * Cp;
+    const int s = k / Cp;
+    u.o = k - s * Cp;
+    u.d = s / 9;
+    const int t = s - 9 * u.d;
+    u.ky = t / 3;
+    u.kx = t - 3 * u.ky;
+  }
+  KMI_DEV uint4 load(const Cur& u) const {
+    const int Ho = 2 * Hi;
+    const int y = 2 * u.yi + (u.d >> 1) - u.ky + 1, x = 2 * u.xi + (u.d & 1) - u.kx + 1;
+    if (!u.ok || u.d > 3 || y < 0 || y >= Ho || x < 0 || x >= Ho) return kZero4;
+    return ld16(u.base + ((long long)y * Ho + x) * Cp + u.o);
+  }
+  KMI_DEV void next(Cur& u) const {
+    u.o += 32;
+    while (u.o >= Cp) {
+      u.o -= Cp;
+      if (++u.kx == 3) {
+        u.kx = 0;
+        if (++u.ky == 3) {
+          u.ky = 0;
+          ++u.d;
+        }
+      }
+    }
+  }
+};
+
+// Its second operand: row n = input channel c, k = (d, tap, o), element W[(tap·Ci + c)·Cp + o]
+// of the weights stored [9 Ci, Cp] (the same for the four d).
+struct ConvDgradB {
+  static constexpr bool kmajor = true;
+  const uint16_t* w;
+  int Ci, Cp;
+  struct Cur {
+    int c, d, t, o;
+    bool ok;
+  };
+  KMI_DEV void init(Cur& u, int n, int k) const {
+    u.ok = n < Ci;
+    u.c = u.ok ? n : 0;
+    const int s = k / Cp;
+    u.o = k - s * Cp;
+    u.d = s / 9;
+    u.t = s - 9 * u.d;
+  }
+  KMI_DEV uint4 load(const Cur& u) const {
+    if (!u.ok || u.d > 3) return kZero4;
+    return ld16(w + ((long long)u.t * Ci + u.c) * Cp + u.o);
+  }
+  KMI_DEV void next(Cur& u) const {
+    u.o += 32;
+    while (u.o >= Cp) {
+      u.o -= Cp;
+      if (++u.t == 9) {
+        u.t = 0;
+        ++u.d;
+      }
+    }
+  }
+};
+
+// ---- epilogues: element (m, n) of the block's result ------------------------------------------
+struct EpiStore {  // P[(z·M + m)·N + n] (split-K partials, or the result itself with one split)
+  float* P;
+  KMI_DEV void operator()(int M, int N, int z, int m, int n, float v) const {
+    P[((long long)z * M + m) * N + n] = v;
+  }
+};
+struct EpiBiasAct {  // z = v + bias; Z = z; h = SiLU(z) or z; H (FP32) and Hb (BF16) = h
+  const float* bias;
+  float* Z;
+  float* H;
+  uint16_t* Hb;
+  int silu;
+  KMI_DEV void operator()(int, int N, int, int m, int n, float v) const {
+    const long long i = (long long)m * N + n;
+    const float z = v + bias[n];
+    if (Z) Z[i] = z;
+    const float h = silu ? z / (1.f + expf(-z)) : z;
+    if (H) H[i] = h;
+    if (Hb) Hb[i] = f2bf(h);
+  }
+};
+struct EpiActGrad {  // g = v ⊙ SiLU'(zin) (or v); FP32 and BF16 copies
+  const float* zin;
+  float* dz;
+  uint16_t* dzb;
+  int silu;
+  KMI_DEV void operator()(int, int N, int, int m, int n, float v) const {
+    const long long i = (long long)m * N + n;
+    float g = v;
+    if (silu) {
+      const float z = zin[i], s = 1.f / (1.f + expf(-z));
+      g = g * (s * (1.f + z * (1.f - s)));
+    }
+    dz[i] = g;
+    dzb[i] = f2bf(g);
+  }
+};
+
+// ---- the product kernel ---------------------------------------------------------------------
+// 256 threads = 4 waves arranged WM × (4/WM) over a BM × BN block tile; each wave owns a
+// (BM/WM) × (BN·WM/4) tile of 16×16 MFMA blocks (v_mfma_f32_16x16x32_bf16: lane l holds A row
+// l%16 and B column l%16 with k = 8(l/16)..+7, and D rows 4(l/16)..+3 of column l%16).  Each
+// K-step of 32 stages the A and B slabs into LDS as [row][32 k] (80-byte rows: the 16-byte
+// fragment reads of a quarter-wave hit 64 distinct banks); K-major units are copied as they
+// come, row-major units are scattered into their eight rows.  Two LDS buffers: the loads of
+// step s+1 are in flight while the MFMAs of step s run, with one barrier per step.  Split-K:
+// blockIdx.z covers K range [z·kchunk, (z+1)·kchunk).
+constexpr int kLdsRow = 40;
+
+template <int BM, int BN, int WM, class LA, class LB, class EPI>
+__global__ void __launch_bounds__(256) k_mm(const int M, const int N, const int K, const int kchunk, const LA la,
+                                            const LB lb, const EPI epi) {
+  constexpr int WN = 4 / WM, TM = BM / WM, TN = BN / WN, RM = TM / 16, RN = TN / 16;
+  static_assert(WM * WN == 4 && TM % 16 == 0 && TN % 16 == 0, "tile shape");
+  constexpr int UA = BM * 4, UB = BN * 4, PA = (UA + 255) / 256, PB = (UB + 255) / 256;
+  __shared__ __align__(16) uint16_t As[2][BM][kLdsRow];
+  __shared__ __align__(16) uint16_t Bs[2][BN][kLdsRow];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, wm = w / WN, wn = w % WN;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  const int kbeg = blockIdx.z * kchunk, kend = min(K, kbeg + kchunk);
+  // unit u: K-major → row u/4, k 8(u%4); row-major → rows 8(u/32)..+7, k u%32
+  auto urow = [](bool km, int u) { return km ? (u >> 2) : ((u >> 5) << 3); };
+  auto uk = [](bool km, int u) { return km ? ((u & 3) << 3) : (u & 31); };
+  typename LA::Cur ca[PA];
+  typename LB::Cur cb[PB];
+#pragma unroll
+  for (int p = 0; p < PA; ++p) {
+    const int u = t + 256 * p;
+    if (u < UA) la.init(ca[p], m0 + urow(LA::kmajor, u), kbeg + uk(LA::kmajor, u));
+  }
+#pragma unroll
+  for (int p = 0; p < PB; ++p) {
+    const int u = t + 256 * p;
+    if (u < UB) lb.init(cb[p], n0 + urow(LB::kmajor, u), kbeg + uk(LB::kmajor, u));
+  }
+  uint4 ra[PA], rb[PB];
+  auto fetch = [&]() {
+#pragma unroll
+    for (int p = 0; p < PA; ++p)
+      if (t + 256 * p < UA) ra[p] = la.load(ca[p]);
+#pragma unroll
+    for (int p = 0; p < PB; ++p)
+      if (t + 256 * p < UB) rb[p] = lb.load(cb[p]);
+  };
+  auto advance = [&]() {
+#pragma unroll
+    for (int p = 0; p < PA; ++p)
+      if (t + 256 * p < UA) la.next(ca[p]);
+#pragma unroll
+    for (int p = 0; p < PB; ++p)
+      if (t + 256 * p < UB) lb.next(cb[p]);
+  };
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int p = 0; p < PA; ++p) {
+      const int u = t + 256 * p;
+      if (u >= UA) continue;
+      const int r = urow(LA::kmajor, u), k = uk(LA::kmajor, u);
+      if constexpr (LA::kmajor) {
+        *reinterpret_cast<uint4*>(&As[buf][r][k]) = ra[p];
+      } else {
+        const uint32_t v[4] = {ra[p].x, ra[p].y, ra[p].z, ra[p].w};
+#pragma unroll
+        for (int e = 0; e < 8; ++e) As[buf][r + e][k] = uint16_t(v[e >> 1] >> (16 * (e & 1)));
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < PB; ++p) {
+      const int u = t + 256 * p;
+      if (u >= UB) continue;
+      const int r = urow(LB::kmajor, u), k = uk(LB::kmajor, u);
+      if constexpr (LB::kmajor) {
+        *reinterpret_cast<uint4*>(&Bs[buf][r][k]) = rb[p];
+      } else {
+        const uint32_t v[4] = {rb[p].x, rb[p].y, rb[p].z, rb[p].w};
+#pragma unroll
+        for (int e = 0; e < 8; ++e) Bs[buf][r + e][k] = uint16_t(v[e >> 1] >> (16 * (e & 1)));
+      }
+    }
+  };
+  v4f acc[RM][RN];
+#pragma unroll
+  for (int a = 0; a < RM; ++a)
+#pragma unroll
+    for (int b = 0; b < RN; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+  if (kbeg < kend) {
+    fetch();
+    stage(0);
+  }
+  __syncthreads();
+  int buf = 0;
+  const int fr = lane & 15, fk = (lane >> 4) << 3;
+  for (int k0 = kbeg; k0 < kend; k0 += 32) {
+    const bool more = k0 + 32 < kend;
+    if (more) {
+      advance();
+      fetch();
+    }
+    v8bf af[RM], bfr[RN];
+#pragma unroll
+    for (int a = 0; a < RM; ++a)
+      af[a] = __builtin_bit_cast(v8bf, *reinterpret_cast<const uint4*>(&As[buf][wm * TM + a * 16 + fr][fk]));
+#pragma unroll
+    for (int b = 0; b < RN; ++b)
+      bfr[b] = __builtin_bit_cast(v8bf, *reinterpret_cast<const uint4*>(&Bs[buf][wn * TN + b * 16 + fr][fk]));
+#pragma unroll
+    for (int a = 0; a < RM; ++a)
+#pragma unroll
+      for (int b = 0; b < RN; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
+    if (more) stage(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+#pragma unroll
+  for (int a = 0; a < RM; ++a)
+#pragma unroll
+    for (int b = 0; b < RN; ++b)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int m = m0 + wm * TM + a * 16 + ((lane >> 4) << 2) + i, n = n0 + wn * TN + b * 16 + fr;
+        if (m < M && n < N) epi(M, N, blockIdx.z, m, n, acc[a][b][i]);
+      }
+}
+
+// K split into `splits` chunks of whole K-steps; the number of chunks that results.
+KMI_HD int split_chunk(int K, int splits, int* nsplit) {
+  int kchunk = (K + splits - 1) / splits;
+  kchunk = (kchunk + 31) / 32 * 32;
+  *nsplit = (K + kchunk - 1) / kchunk;
+  return kchunk;
+}
+
+template <int BM, int BN, int WM, class LA, class LB, class EPI>
+static hipError_t mm(int M, int N, int K, int splits, const LA& la, const LB& lb, const EPI& epi, hipStream_t s) {
+  if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
+  int sp = 1;
+  const int kchunk = split_chunk(K, splits, &sp);
+  const dim3 g(unsigned((M + BM - 1) / BM), unsigned((N + BN - 1) / BN), unsigned(sp));
+  hipLaunchKernelGGL((k_mm<BM, BN, WM, LA, LB, EPI>), g, dim3(256), 0, s, M, N, K, kchunk, la, lb, epi);
+  return hipGetLastError();
 }
 
 // Σ over splits (fixed order) + bias; optional SiLU; writes Z (pre-activation, FP32), the
@@ -292,8 +518,8 @@ __global__ void __launch_bounds__(256) k_split_sum(const float* __restrict__ P, 
   }
 }
 
-// dZ = dH ⊙ SiLU'(Z) (or dH when !silu); FP32 and BF16 copies.  nearest-×2 folding of the
-// up-sampled data gradient happens inside LoadConvUpGradIn, so dH is already per input pixel.
+// dZ = dH ⊙ SiLU'(Z) (or dH when !silu); FP32 and BF16 copies (the dense layers; the
+// convolutions fuse this into their data-gradient products).
 __global__ void k_act_grad(const float* __restrict__ dH, const float* __restrict__ Z, long long n,
                            int silu, float* __restrict__ dZ, uint16_t* __restrict__ dZb) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -369,17 +595,42 @@ __global__ void __launch_bounds__(256) k_mse(const float* __restrict__ Y, const 
   __syncthreads();
   if (threadIdx.x == 0) part[blockIdx.x] = (lds[0] + lds[1] + lds[2] + lds[3]) / double(n);
 }
-// One wave: lane-strided sums in fixed order, then a fixed butterfly.
-__global__ void __launch_bounds__(64) k_sum_parts(const double* __restrict__ part, int n, double* __restrict__ out) {
+// One work-group: thread-strided sums in fixed order, then a fixed LDS tree.
+__global__ void __launch_bounds__(256) k_sum_parts(const double* __restrict__ part, int n, double* __restrict__ out) {
+  __shared__ double lds[256];
+  const int t = threadIdx.x;
   double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += 64) s += part[i];
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-  if (threadIdx.x == 0) out[0] = s;
+  for (int i = t; i < n; i += 256) s += part[i];
+  lds[t] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) lds[t] += lds[t + w];
+    __syncthreads();
+  }
+  if (t == 0) out[0] = lds[0];
 }
 
 __global__ void k_to_bf16(const float* __restrict__ x, long long n, uint16_t* __restrict__ y) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] = f2bf(x[i]);
+}
+
+// FP32 [rows, C] → BF16 [rows, Cp], columns C..Cp−1 zero.
+__global__ void k_pad_bf16(const float* __restrict__ x, long long rows, int C, int Cp, uint16_t* __restrict__ y) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows * Cp) return;
+  const long long r = i / Cp;
+  const int c = int(i - r * Cp);
+  y[i] = c < C ? f2bf(x[r * C + c]) : uint16_t(0);
+}
+// BF16 [rows, C] → BF16 [rows, Cp], columns C..Cp−1 zero.
+__global__ void k_pad_cols_bf16(const uint16_t* __restrict__ x, long long rows, int C, int Cp,
+                                uint16_t* __restrict__ y) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows * Cp) return;
+  const long long r = i / Cp;
+  const int c = int(i - r * Cp);
+  y[i] = c < C ? x[r * C + c] : uint16_t(0);
 }
 
 // Adam (β1 = 0.9, β2 = 0.999, ε = 1e-8) on the FP32 master weights; refreshes the BF16 copy.
@@ -411,17 +662,6 @@ __global__ void k_gather(const float* __restrict__ X, const float* __restrict__ 
   if (t < (long long)B * din) xb[t] = X[(long long)idx[t / din] * din + (t % din)];
 }
 
-template <class LA, class LB>
-static hipError_t gemm(int M, int N, int K, LA la, LB lb, float* P, int splits, hipStream_t s) {
-  if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
-  int kchunk = (K + splits - 1) / splits;
-  kchunk = (kchunk + 15) / 16 * 16;
-  splits = (K + kchunk - 1) / kchunk;
-  dim3 g(unsigned((M + 63) / 64), unsigned((N + 63) / 64), unsigned(splits));
-  hipLaunchKernelGGL((k_gemm<LA, LB>), g, dim3(256), 0, s, M, N, K, kchunk, la, lb, P);
-  return hipGetLastError();
-}
-
 static inline dim3 g1(long long n) { return dim3(unsigned((n + 255) / 256)); }
 
 }  // namespace kmi
@@ -434,19 +674,24 @@ struct kmi_nn {
   kmi_nn_desc d{};
   int L = 0;  // hw·hw·c0
   int H1 = 0, H2 = 0, H3 = 0;  // spatial sizes after conv 1, 2, 3
+  int Cp3 = 0;  // output channels padded to a multiple of 8 (BF16 gradient and weight copies)
   long long nparam = 0;
   // parameter offsets in the flat vector
   long long oW1, ob1, oW2, ob2, oC1, obc1, oC2, obc2, oC3, obc3;
   float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
   uint16_t* wb = nullptr;
+  uint16_t* w3p = nullptr;  // BF16 C3 as [9 c2][Cp3], columns >= out_c zero (null: Cp3 == out_c)
   int64_t adam_t = 0;
   // activations (max_batch)
   float *x = nullptr, *tgt = nullptr;
   uint16_t* xb = nullptr;
   float *z1 = nullptr, *z2 = nullptr, *z3 = nullptr, *z4 = nullptr, *y = nullptr;
   uint16_t *h1b = nullptr, *h2b = nullptr, *h3b = nullptr, *h4b = nullptr;
-  float *dy = nullptr, *dh = nullptr, *dz = nullptr;  // scratch sized for the largest layer
-  uint16_t *dyb = nullptr, *dzb = nullptr;
+  float* dy = nullptr;     // loss gradient [B·H3², out_c]
+  uint16_t* dyb = nullptr; // its BF16 copy [B·H3², Cp3]
+  float* dh = nullptr;     // dense data gradient before the activation
+  float* dz[2] = {nullptr, nullptr};  // ping-pong gradients of the stage inputs (FP32, BF16)
+  uint16_t* dzb[2] = {nullptr, nullptr};
   float* P = nullptr;  // split-K partials
   size_t P_elems = 0;
   float* colpart = nullptr;
@@ -459,6 +704,7 @@ struct kmi_nn {
 };
 
 static long long out_elems(const kmi_nn* N, int B) { return (long long)B * N->H3 * N->H3 * N->d.out_c; }
+static const uint16_t* w3_bf16(const kmi_nn* N) { return N->w3p ? N->w3p : N->wb + N->oC3; }
 
 static int nn_alloc(kmi_nn* N) {
   const kmi_nn_desc& d = N->d;
@@ -467,6 +713,7 @@ static int nn_alloc(kmi_nn* N) {
   N->H1 = 2 * d.latent_hw;
   N->H2 = 4 * d.latent_hw;
   N->H3 = 8 * d.latent_hw;
+  N->Cp3 = (d.out_c + 7) / 8 * 8;
   long long o = 0;
   auto take = [&](long long n) { long long r = o; o += n; return r; };
   N->oW1 = take((long long)d.in_dim * d.hidden);
@@ -482,15 +729,14 @@ static int nn_alloc(kmi_nn* N) {
   N->nparam = o;
   const long long a1 = (long long)B * d.hidden, a2 = (long long)B * N->L;
   const long long a3 = (long long)B * N->H1 * N->H1 * d.c1, a4 = (long long)B * N->H2 * N->H2 * d.c2;
-  const long long a5 = out_elems(N, B);
+  const long long a5 = out_elems(N, B), a5p = (long long)B * N->H3 * N->H3 * N->Cp3;
   const long long big = std::max({a1, a2, a3, a4, a5});
-  // P holds one split of every product (pick_splits adds splits only while they fit): the
-  // activations above, the data gradients (B·Hi²·Ci per stage, at most `big`), and the
-  // weight gradients of the dense layers (in_dim × hidden, hidden × L) and of the three
-  // convolutions (9 Ci × Co).
+  // P holds one split of every product it receives (pick_splits adds splits only while they
+  // fit): the weight gradients of the dense layers (in_dim × hidden, hidden × L) and of the
+  // three convolutions (9 Ci × Co), and dH1 (B × hidden).
   const long long wmax = std::max({(long long)d.in_dim * d.hidden, (long long)d.hidden * N->L,
-                                   9LL * d.latent_c * d.c1, 9LL * d.c1 * d.c2, 9LL * d.c2 * d.out_c});
-  N->P_elems = size_t(std::max({big, wmax, 64LL * 1024 * 1024 / 4}));
+                                   9LL * d.latent_c * d.c1, 9LL * d.c1 * d.c2, 9LL * d.c2 * d.out_c, a1});
+  N->P_elems = size_t(std::max({wmax, 16LL * 1024 * 1024 / 4}));
   hipError_t e = hipSuccess;
   auto al = [&](void** p, size_t bytes) { if (e == hipSuccess) e = hipMalloc(p, std::max<size_t>(bytes, 16)); };
   al((void**)&N->w, o * 4);
@@ -498,6 +744,7 @@ static int nn_alloc(kmi_nn* N) {
   al((void**)&N->m, o * 4);
   al((void**)&N->v, o * 4);
   al((void**)&N->wb, o * 2);
+  if (N->Cp3 != d.out_c) al((void**)&N->w3p, 9LL * d.c2 * N->Cp3 * 2);
   al((void**)&N->x, (size_t)B * d.in_dim * 4);
   al((void**)&N->xb, (size_t)B * d.in_dim * 2);
   al((void**)&N->tgt, a5 * 4);
@@ -511,13 +758,15 @@ static int nn_alloc(kmi_nn* N) {
   al((void**)&N->h4b, a4 * 2);
   al((void**)&N->y, a5 * 4);
   al((void**)&N->dy, a5 * 4);
-  al((void**)&N->dyb, a5 * 2);
-  al((void**)&N->dh, big * 4);
-  al((void**)&N->dz, big * 4);
-  al((void**)&N->dzb, big * 2);
+  al((void**)&N->dyb, a5p * 2);
+  al((void**)&N->dh, a1 * 4);
+  for (int k = 0; k < 2; ++k) {
+    al((void**)&N->dz[k], big * 4);
+    al((void**)&N->dzb[k], big * 2);
+  }
   al((void**)&N->P, N->P_elems * 4);
   al((void**)&N->colpart, 1024LL * std::max(N->L, 256) * 4);
-  al((void**)&N->lpart, 4096 * 8);
+  al((void**)&N->lpart, 1024 * 8);
   al((void**)&N->lout, 64);
   al((void**)&N->idx, (size_t)B * 4);
   if (e != hipSuccess) return hip_fail(N->ctx, e, "kmi_nn allocation");
@@ -526,10 +775,11 @@ static int nn_alloc(kmi_nn* N) {
   return KMI_OK;
 }
 
-// Splits for a product with a long reduction (K) and a small output, capped by P's size.
-static int pick_splits(const kmi_nn* N, long long M, long long Nn, long long K) {
-  long long tiles = ((M + 63) / 64) * ((Nn + 63) / 64);
-  long long s = std::max<long long>(1, std::min<long long>((2048 + tiles - 1) / tiles, K / 256));
+// Splits of a product with a long reduction (K) and a small output on BM × BN tiles: about
+// 512 work-groups, at least 512 of K per split, within P's size.
+static int pick_splits(const kmi_nn* N, int BM, int BN, long long M, long long Nn, long long K) {
+  const long long tiles = ((M + BM - 1) / BM) * ((Nn + BN - 1) / BN);
+  long long s = std::max<long long>(1, std::min<long long>((512 + tiles - 1) / tiles, K / 512));
   s = std::min<long long>(s, (long long)N->P_elems / std::max<long long>(1, M * Nn));
   return int(std::max<long long>(1, s));
 }
@@ -544,11 +794,61 @@ static int colsum(kmi_nn* N, const float* X, long long rows, int n, float* out, 
   return e == hipSuccess ? KMI_OK : hip_fail(N->ctx, e, "bias gradient");
 }
 
+// Σ over the splits of P into out (fixed order).
+static hipError_t split_sum(const float* P, int M, int Nn, int K, int splits, float* out, hipStream_t s) {
+  int sp = 1;
+  split_chunk(K, splits, &sp);
+  const long long mn = (long long)M * Nn;
+  hipLaunchKernelGGL(k_split_sum, dim3(unsigned((mn + 15) / 16)), dim3(256), 0, s, P, sp, mn, out);
+  return hipGetLastError();
+}
+
+// The BF16 copies derived from the weights: C3 padded to Cp3 columns.
+static hipError_t refresh_weight_copies(kmi_nn* N, hipStream_t s) {
+  if (!N->w3p) return hipSuccess;
+  const long long rows = 9LL * N->d.c2;
+  hipLaunchKernelGGL(k_pad_cols_bf16, g1(rows * N->Cp3), dim3(256), 0, s, N->wb + N->oC3, rows, N->d.out_c,
+                     N->Cp3, N->w3p);
+  return hipGetLastError();
+}
+
 #define NN_CHECK(expr)                                            \
   do {                                                            \
     hipError_t _e = (expr);                                       \
     if (_e != hipSuccess) return hip_fail(N->ctx, _e, #expr);     \
   } while (0)
+
+// One convolution stage of the decoder: input [B, Hi, Hi, Ci] (BF16), output channels Co,
+// weights [9 Ci, Cp] in BF16 (Cp = Co rounded up to 8, padded columns zero).
+struct Stage {
+  const uint16_t* in;
+  int Hi, Ci, Co, Cp;
+  const uint16_t* wbf;
+  long long ow, ob;
+};
+
+// Products with N = Co output channels, tiles chosen by the channel count.
+template <class LA, class LB, class EPI>
+static hipError_t mm_by_n(int M, int Nn, int K, int splits, const LA& la, const LB& lb, const EPI& epi, hipStream_t s) {
+  if (Nn > 32) return mm<128, 64, 4>(M, Nn, K, splits, la, lb, epi, s);
+  if (Nn > 16) return mm<128, 32, 4>(M, Nn, K, splits, la, lb, epi, s);
+  return mm<256, 16, 4>(M, Nn, K, splits, la, lb, epi, s);
+}
+// Weight-gradient products (M = 9 Ci rows, N = Co, long K): 64-row tiles.
+template <class LA, class LB, class EPI>
+static hipError_t mm_wgrad(int M, int Nn, int K, int splits, const LA& la, const LB& lb, const EPI& epi,
+                           hipStream_t s) {
+  if (Nn > 32) return mm<64, 64, 2>(M, Nn, K, splits, la, lb, epi, s);
+  if (Nn > 16) return mm<64, 32, 2>(M, Nn, K, splits, la, lb, epi, s);
+  return mm<64, 16, 4>(M, Nn, K, splits, la, lb, epi, s);
+}
+
+static void stages(const kmi_nn* N, Stage (&st)[3]) {
+  const kmi_nn_desc& d = N->d;
+  st[0] = {N->h2b, d.latent_hw, d.latent_c, d.c1, d.c1, N->wb + N->oC1, N->oC1, N->obc1};
+  st[1] = {N->h3b, N->H1, d.c1, d.c2, d.c2, N->wb + N->oC2, N->oC2, N->obc2};
+  st[2] = {N->h4b, N->H2, d.c2, d.out_c, N->Cp3, w3_bf16(N), N->oC3, N->obc3};
+}
 
 // Forward pass for batch B from N->x (FP32) into N->y.
 static int nn_forward(kmi_nn* N, int B) {
@@ -557,31 +857,25 @@ static int nn_forward(kmi_nn* N, int B) {
   const uint16_t* wb = N->wb;
   const float* w = N->w;
   hipLaunchKernelGGL(k_to_bf16, g1((long long)B * d.in_dim), dim3(256), 0, s, N->x, (long long)B * d.in_dim, N->xb);
-  // dense 1
-  NN_CHECK(gemm(B, d.hidden, d.in_dim, LoadDense{N->xb, d.in_dim, 1, B, d.in_dim},
-                LoadDense{wb + N->oW1, d.hidden, 1, d.in_dim, d.hidden}, N->P, 1, s));
-  long long mn = (long long)B * d.hidden;
-  hipLaunchKernelGGL(k_epilogue, g1(mn), dim3(256), 0, s, N->P, 1, mn, d.hidden, w + N->ob1, 1, N->z1, (float*)nullptr, N->h1b);
+  // dense 1: [B, in] · W1[in, hidden], bias, SiLU
+  NN_CHECK((mm<64, 64, 2>(B, d.hidden, d.in_dim, 1, DenseK{N->xb, d.in_dim, B, d.in_dim},
+                          DenseR{wb + N->oW1, d.hidden, d.hidden, d.in_dim},
+                          EpiBiasAct{w + N->ob1, N->z1, nullptr, N->h1b, 1}, s)));
   // dense 2 → latent (NHWC)
-  NN_CHECK(gemm(B, N->L, d.hidden, LoadDense{N->h1b, d.hidden, 1, B, d.hidden},
-                LoadDense{wb + N->oW2, N->L, 1, d.hidden, N->L}, N->P, 1, s));
-  mn = (long long)B * N->L;
-  hipLaunchKernelGGL(k_epilogue, g1(mn), dim3(256), 0, s, N->P, 1, mn, N->L, w + N->ob2, 1, N->z2, (float*)nullptr, N->h2b);
-  // conv 1..3
-  struct Stage { const uint16_t* in; int Hi, Ci, Co; long long ow, ob; float* z; uint16_t* hb; int silu; };
-  const Stage st[3] = {{N->h2b, d.latent_hw, d.latent_c, d.c1, N->oC1, N->obc1, N->z3, N->h3b, 1},
-                       {N->h3b, N->H1, d.c1, d.c2, N->oC2, N->obc2, N->z4, N->h4b, 1},
-                       {N->h4b, N->H2, d.c2, d.out_c, N->oC3, N->obc3, N->y, nullptr, 0}};
-  for (const Stage& S : st) {
-    const int Ho = 2 * S.Hi;
-    const int M = B * Ho * Ho;
-    NN_CHECK(gemm(M, S.Co, 9 * S.Ci, LoadConvUp<false>{S.in, B, S.Hi, S.Ci},
-                  LoadDense{wb + S.ow, S.Co, 1, 9 * S.Ci, S.Co}, N->P, 1, s));
-    mn = (long long)M * S.Co;
-    hipLaunchKernelGGL(k_epilogue, g1(mn), dim3(256), 0, s, N->P, 1, mn, S.Co, w + S.ob, S.silu, S.z,
-                       (float*)nullptr, S.hb);
+  NN_CHECK((mm<64, 64, 2>(B, N->L, d.hidden, 1, DenseK{N->h1b, d.hidden, B, d.hidden},
+                          DenseR{wb + N->oW2, N->L, N->L, d.hidden},
+                          EpiBiasAct{w + N->ob2, N->z2, nullptr, N->h2b, 1}, s)));
+  // conv 1..3 on the ×2 up-sampled inputs; bias and SiLU fused (none after the last)
+  Stage st[3];
+  stages(N, st);
+  float* zout[3] = {N->z3, N->z4, N->y};
+  uint16_t* hout[3] = {N->h3b, N->h4b, nullptr};
+  for (int k = 0; k < 3; ++k) {
+    const Stage& S = st[k];
+    const int Ho = 2 * S.Hi, M = B * Ho * Ho;
+    NN_CHECK(mm_by_n(M, S.Co, 9 * S.Ci, 1, ConvFwdA{S.in, B, S.Hi, S.Ci}, DenseR{S.wbf, S.Cp, S.Cp, 9 * S.Ci},
+                     EpiBiasAct{w + S.ob, zout[k], nullptr, hout[k], k < 2 ? 1 : 0}, s));
   }
-  NN_CHECK(hipGetLastError());
   return KMI_OK;
 }
 
@@ -591,82 +885,61 @@ static int nn_backward(kmi_nn* N, int B) {
   hipStream_t s = N->ctx->stream;
   const uint16_t* wb = N->wb;
   float* g = N->g;
-  struct Stage { const uint16_t* in; int Hi, Ci, Co; long long ow, ob; const float* zin; int silu_in; };
-  // stage k: input activation `in` (BF16, NHWC Hi×Hi×Ci, produced with SiLU from zin)
-  const Stage st[3] = {{N->h2b, d.latent_hw, d.latent_c, d.c1, N->oC1, N->obc1, N->z2, 1},
-                       {N->h3b, N->H1, d.c1, d.c2, N->oC2, N->obc2, N->z3, 1},
-                       {N->h4b, N->H2, d.c2, d.out_c, N->oC3, N->obc3, N->z4, 1}};
-  // gradient w.r.t. the stage output pre-activation, FP32 + BF16
+  Stage st[3];
+  stages(N, st);
+  const float* zin[3] = {N->z2, N->z3, N->z4};  // pre-activations of the stage inputs (SiLU)
+  // gradient w.r.t. the stage output pre-activation: FP32 [M, Co] and BF16 [M, Cp]
   const float* dz = N->dy;
-  long long nout = out_elems(N, B);
-  hipLaunchKernelGGL(k_to_bf16, g1(nout), dim3(256), 0, s, N->dy, nout, N->dyb);
+  const long long rows = (long long)B * N->H3 * N->H3;
+  hipLaunchKernelGGL(k_pad_bf16, g1(rows * N->Cp3), dim3(256), 0, s, N->dy, rows, d.out_c, N->Cp3, N->dyb);
   const uint16_t* dzb = N->dyb;
+  int pp = 0;
   for (int k = 2; k >= 0; --k) {
     const Stage& S = st[k];
     const int Ho = 2 * S.Hi, Mo = B * Ho * Ho;
     // bias gradient
     if (int rc = colsum(N, dz, Mo, S.Co, g + S.ob, s)) return rc;
-    // weight gradient dW[(ky,kx,c), o] = Σ_m A(m, (ky,kx,c)) dZ[m, o]
+    // weight gradient dW[(ky,kx,c), o] = Σ_pixels A(pixel, (ky,kx,c)) dZ[pixel, o]
     const int Mw = 9 * S.Ci;
-    const int sp = pick_splits(N, Mw, S.Co, Mo);
-    NN_CHECK(gemm(Mw, S.Co, Mo, LoadConvUp<true>{S.in, B, S.Hi, S.Ci}, LoadDense{dzb, S.Co, 1, Mo, S.Co}, N->P, sp, s));
-    {
-      int kchunk = (Mo + sp - 1) / sp;
-      kchunk = (kchunk + 15) / 16 * 16;
-      const int splits = (Mo + kchunk - 1) / kchunk;
-      const long long mn = (long long)Mw * S.Co;
-      hipLaunchKernelGGL(k_split_sum, dim3(unsigned((mn + 15) / 16)), dim3(256), 0, s, N->P, splits, mn, g + S.ow);
-    }
-    // data gradient into the stage input (per input pixel, up-sampling folded in)
+    const int sp = pick_splits(N, 64, S.Co > 32 ? 64 : (S.Co > 16 ? 32 : 16), Mw, S.Co, Mo);
+    NN_CHECK(mm_wgrad(Mw, S.Co, Mo, sp, ConvWgradA{S.in, B, S.Hi, S.Ci}, DenseR{dzb, S.Cp, S.Cp, Mo},
+                      EpiStore{N->P}, s));
+    NN_CHECK(split_sum(N->P, Mw, S.Co, Mo, sp, g + S.ow, s));
+    // data gradient into the stage input (per input pixel, up-sampling folded in), times
+    // the SiLU derivative of the input's pre-activation
     const int Mi = B * S.Hi * S.Hi;
-    NN_CHECK(gemm(Mi, S.Ci, 36 * S.Co, LoadConvUpGradIn{dzb, B, S.Hi, S.Co},
-                  LoadConvWeightT{wb + S.ow, S.Ci, S.Co}, N->P, 1, s));
-    const long long ni = (long long)Mi * S.Ci;
-    hipLaunchKernelGGL(k_epilogue, g1(ni), dim3(256), 0, s, N->P, 1, ni, S.Ci, (const float*)nullptr, 0, N->dh,
-                       (float*)nullptr, (uint16_t*)nullptr);
-    hipLaunchKernelGGL(k_act_grad, g1(ni), dim3(256), 0, s, N->dh, S.zin, ni, S.silu_in, N->dz, N->dzb);
-    dz = N->dz;
-    dzb = N->dzb;
+    NN_CHECK(mm_by_n(Mi, S.Ci, 36 * S.Cp, 1, ConvDgradA{dzb, B, S.Hi, S.Cp}, ConvDgradB{S.wbf, S.Ci, S.Cp},
+                     EpiActGrad{zin[k], N->dz[pp], N->dzb[pp], 1}, s));
+    dz = N->dz[pp];
+    dzb = N->dzb[pp];
+    pp ^= 1;
   }
   // dense 2: dZ2 is in dz/dzb as [B, L]
   if (int rc = colsum(N, dz, B, N->L, g + N->ob2, s)) return rc;
-  {
-    const int sp = pick_splits(N, d.hidden, N->L, B);
-    NN_CHECK(gemm(d.hidden, N->L, B, LoadDense{N->h1b, 1, d.hidden, d.hidden, B}, LoadDense{dzb, N->L, 1, B, N->L},
-                  N->P, sp, s));
-    int kchunk = ((B + sp - 1) / sp + 15) / 16 * 16;
-    const int splits = (B + kchunk - 1) / kchunk;
-    const long long mn = (long long)d.hidden * N->L;
-    hipLaunchKernelGGL(k_split_sum, dim3(unsigned((mn + 15) / 16)), dim3(256), 0, s, N->P, splits, mn, g + N->oW2);
-  }
+  NN_CHECK((mm<64, 64, 2>(d.hidden, N->L, B, 1, DenseR{N->h1b, d.hidden, d.hidden, B}, DenseR{dzb, N->L, N->L, B},
+                          EpiStore{g + N->oW2}, s)));
   // dH1 = dZ2 W2ᵀ (long reduction over the latent: split-K)
-  const long long nh = (long long)B * d.hidden;
   {
-    const int sp = pick_splits(N, B, d.hidden, N->L);
-    NN_CHECK(gemm(B, d.hidden, N->L, LoadDense{dzb, N->L, 1, B, N->L},
-                  LoadDense{wb + N->oW2, 1, N->L, N->L, d.hidden}, N->P, sp, s));
-    int kchunk = ((N->L + sp - 1) / sp + 15) / 16 * 16;
-    const int splits = (N->L + kchunk - 1) / kchunk;
-    hipLaunchKernelGGL(k_split_sum, dim3(unsigned((nh + 15) / 16)), dim3(256), 0, s, N->P, splits, nh, N->dh);
+    const int sp = pick_splits(N, 64, 64, B, d.hidden, N->L);
+    NN_CHECK((mm<64, 64, 2>(B, d.hidden, N->L, sp, DenseK{dzb, N->L, B, N->L}, DenseK{wb + N->oW2, N->L, d.hidden, N->L},
+                            EpiStore{N->P}, s)));
+    NN_CHECK(split_sum(N->P, B, d.hidden, N->L, sp, N->dh, s));
   }
-  hipLaunchKernelGGL(k_act_grad, g1(nh), dim3(256), 0, s, N->dh, N->z1, nh, 1, N->dz, N->dzb);
+  const long long nh = (long long)B * d.hidden;
+  hipLaunchKernelGGL(k_act_grad, g1(nh), dim3(256), 0, s, N->dh, N->z1, nh, 1, N->dz[pp], N->dzb[pp]);
   // dense 1
-  if (int rc = colsum(N, N->dz, B, d.hidden, g + N->ob1, s)) return rc;
-  NN_CHECK(gemm(d.in_dim, d.hidden, B, LoadDense{N->xb, 1, d.in_dim, d.in_dim, B}, LoadDense{N->dzb, d.hidden, 1, B, d.hidden},
-                N->P, 1, s));
-  const long long n1 = (long long)d.in_dim * d.hidden;
-  hipLaunchKernelGGL(k_epilogue, g1(n1), dim3(256), 0, s, N->P, 1, n1, d.hidden, (const float*)nullptr, 0, g + N->oW1,
-                     (float*)nullptr, (uint16_t*)nullptr);
-  NN_CHECK(hipGetLastError());
+  if (int rc = colsum(N, N->dz[pp], B, d.hidden, g + N->ob1, s)) return rc;
+  NN_CHECK((mm<64, 64, 2>(d.in_dim, d.hidden, B, 1, DenseR{N->xb, d.in_dim, d.in_dim, B},
+                          DenseR{N->dzb[pp], d.hidden, d.hidden, B}, EpiStore{g + N->oW1}, s)));
   return KMI_OK;
 }
 
 static int nn_loss(kmi_nn* N, int B, double* loss) {
   hipStream_t s = N->ctx->stream;
   const long long n = out_elems(N, B);
-  const int nb = int(std::min<long long>(4096, (n + 255) / 256));
+  const int nb = int(std::min<long long>(1024, (n + 255) / 256));
   hipLaunchKernelGGL(k_mse, dim3(unsigned(nb)), dim3(256), 0, s, N->y, N->tgt, n, N->dy, N->lpart);
-  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, s, N->lpart, nb, N->lout);
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, s, N->lpart, nb, N->lout);
   NN_CHECK(hipGetLastError());
   if (loss) {
     NN_CHECK(hipMemcpyAsync(loss, N->lout, 8, hipMemcpyDeviceToHost, s));
@@ -681,6 +954,7 @@ static int nn_adam(kmi_nn* N, float lr) {
   hipLaunchKernelGGL(k_adam, g1(N->nparam), dim3(256), 0, N->ctx->stream, N->w, N->g, N->m, N->v, N->wb, N->nparam,
                      lr, c1, c2);
   NN_CHECK(hipGetLastError());
+  NN_CHECK(refresh_weight_copies(N, N->ctx->stream));
   return KMI_OK;
 }
 
@@ -720,6 +994,8 @@ int kmi_nn_create(kmi_ctx* ctx, const kmi_nn_desc* d, kmi_nn** out) {
   if (d->in_dim < 1 || d->hidden < 1 || d->latent_hw < 1 || d->latent_c < 1 || d->c1 < 1 || d->c2 < 1 ||
       d->out_c < 1 || d->max_batch < 1 || d->max_batch > 4096 || 8LL * d->latent_hw > 4096)
     return fail(ctx, KMI_ERR_ARG, "kmi_nn_create: sizes out of range");
+  if (d->latent_c % 8 || d->c1 % 8 || d->c2 % 8)
+    return fail(ctx, KMI_ERR_ARG, "kmi_nn_create: latent_c, c1 and c2 must be multiples of 8");
   KMI_HIP(ctx, hipSetDevice(ctx->device));
   kmi_nn* N = new (std::nothrow) kmi_nn();
   if (!N) return fail(ctx, KMI_ERR_NOMEM, "kmi_nn_create: out of host memory");
@@ -753,9 +1029,9 @@ int kmi_nn_destroy(kmi_nn* N) {
   if (!N) return KMI_OK;
   (void)hipSetDevice(N->ctx->device);
   (void)hipStreamSynchronize(N->ctx->stream);
-  void* ptrs[] = {N->w, N->g, N->m, N->v, N->wb, N->x, N->xb, N->tgt, N->z1, N->h1b, N->z2, N->h2b, N->z3,
-                  N->h3b, N->z4, N->h4b, N->y, N->dy, N->dyb, N->dh, N->dz, N->dzb, N->P, N->colpart,
-                  N->lpart, N->lout, N->idx, N->dsX, N->dsY};
+  void* ptrs[] = {N->w, N->g, N->m, N->v, N->wb, N->w3p, N->x, N->xb, N->tgt, N->z1, N->h1b, N->z2, N->h2b,
+                  N->z3, N->h3b, N->z4, N->h4b, N->y, N->dy, N->dyb, N->dh, N->dz[0], N->dz[1], N->dzb[0],
+                  N->dzb[1], N->P, N->colpart, N->lpart, N->lout, N->idx, N->dsX, N->dsY};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete N;
@@ -781,6 +1057,7 @@ int kmi_nn_set_params(kmi_nn* N, const float* host) {
   KMI_HIP(N->ctx, hipMemcpy(N->w, host, N->nparam * 4, hipMemcpyHostToDevice));
   hipLaunchKernelGGL(k_to_bf16, g1(N->nparam), dim3(256), 0, N->ctx->stream, N->w, N->nparam, N->wb);
   KMI_HIP(N->ctx, hipGetLastError());
+  KMI_HIP(N->ctx, refresh_weight_copies(N, N->ctx->stream));
   KMI_HIP(N->ctx, hipStreamSynchronize(N->ctx->stream));
   N->adam_t = 0;
   KMI_HIP(N->ctx, hipMemset(N->m, 0, N->nparam * 4));
@@ -883,6 +1160,8 @@ int kmi_nn_train_epochs(kmi_nn* N, int32_t epochs, int32_t B, float lr, uint64_t
   return KMI_OK;
 }
 
+}  // extern "C"
+
 // C = A·B for host FP32 matrices, products in BF16 on MFMA with FP32 accumulation.
 // A(m, k) = A[m*sam + k*sak], B(k, n) = B[k*sbk + n*sbn], C row-major M×N.
 // Device buffers: dA, dB (FP32 copies), bA, bB (BF16 copies), P (split-K partials, `sp`
@@ -894,18 +1173,34 @@ struct GemmBufs {
 struct GemmShape {
   long long na, nb;  // elements of A and B spanned by the strides
   int splits;        // splits asked for
-  int sp;            // splits gemm() runs after rounding the K chunks to whole K-steps
+  int sp;            // splits mm() runs after rounding the K chunks to whole K-steps
 };
 static bool gemm_shape(int32_t M, int32_t Nn, int32_t K, int64_t sam, int64_t sak, int64_t sbk, int64_t sbn,
                        int32_t splits, GemmShape& g) {
   if (M < 1 || Nn < 1 || K < 1 || splits < 1 || sam < 0 || sak < 0 || sbk < 0 || sbn < 0) return false;
   g.na = (long long)(M - 1) * sam + (long long)(K - 1) * sak + 1;
   g.nb = (long long)(K - 1) * sbk + (long long)(Nn - 1) * sbn + 1;
-  const int kchunk = ((K + splits - 1) / splits + 15) / 16 * 16;  // as gemm() rounds it
   g.splits = splits;
-  g.sp = (K + kchunk - 1) / kchunk;
+  split_chunk(K, splits, &g.sp);  // as mm() rounds the chunks
   return true;
 }
+// A(m, k) = a[m·sam + k·sak] and B(k, n) = b[k·sbk + n·sbn] read through the loader their
+// strides allow: K-major (unit stride along k), row-major (unit stride along the rows), or
+// element by element.
+template <class LA>
+static hipError_t gemm_strided_b(int M, int Nn, int K, const LA& la, const uint16_t* b, int64_t sbk, int64_t sbn,
+                                 int splits, float* P, hipStream_t s) {
+  if (sbk == 1) return mm<64, 64, 2>(M, Nn, K, splits, la, DenseK{b, sbn, Nn, K}, EpiStore{P}, s);
+  if (sbn == 1) return mm<64, 64, 2>(M, Nn, K, splits, la, DenseR{b, sbk, Nn, K}, EpiStore{P}, s);
+  return mm<64, 64, 2>(M, Nn, K, splits, la, DenseS{b, sbn, sbk, Nn, K}, EpiStore{P}, s);
+}
+static hipError_t gemm_strided(int M, int Nn, int K, const uint16_t* a, int64_t sam, int64_t sak, const uint16_t* b,
+                               int64_t sbk, int64_t sbn, int splits, float* P, hipStream_t s) {
+  if (sak == 1) return gemm_strided_b(M, Nn, K, DenseK{a, sam, M, K}, b, sbk, sbn, splits, P, s);
+  if (sam == 1) return gemm_strided_b(M, Nn, K, DenseR{a, sak, M, K}, b, sbk, sbn, splits, P, s);
+  return gemm_strided_b(M, Nn, K, DenseS{a, sam, sak, M, K}, b, sbk, sbn, splits, P, s);
+}
+
 static hipError_t gemm_bf16_run(hipStream_t s, int32_t M, int32_t Nn, int32_t K, const float* A, int64_t sam,
                                 int64_t sak, const float* B, int64_t sbk, int64_t sbn, const GemmShape& g,
                                 const GemmBufs& d, float* C) {
@@ -914,7 +1209,7 @@ static hipError_t gemm_bf16_run(hipStream_t s, int32_t M, int32_t Nn, int32_t K,
   if (e == hipSuccess) {
     hipLaunchKernelGGL(k_to_bf16, g1(g.na), dim3(256), 0, s, d.dA, g.na, d.bA);
     hipLaunchKernelGGL(k_to_bf16, g1(g.nb), dim3(256), 0, s, d.dB, g.nb, d.bB);
-    e = gemm(M, Nn, K, LoadDense{d.bA, sam, sak, M, K}, LoadDense{d.bB, sbk, sbn, K, Nn}, d.P, g.splits, s);
+    e = gemm_strided(M, Nn, K, d.bA, sam, sak, d.bB, sbk, sbn, g.splits, d.P, s);
   }
   if (e == hipSuccess) {
     const long long mn = (long long)M * Nn;
@@ -926,6 +1221,8 @@ static hipError_t gemm_bf16_run(hipStream_t s, int32_t M, int32_t Nn, int32_t K,
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   return e;
 }
+
+extern "C" {
 
 int kmi_gemm_bf16(kmi_ctx* ctx, int32_t M, int32_t Nn, int32_t K, const float* A, int64_t sam, int64_t sak,
                   const float* B, int64_t sbk, int64_t sbn, int32_t splits, float* C) {

@@ -45,6 +45,47 @@ constexpr int group_index(int q) {
 
 KMI_DEV int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
+// Walled lattices (BND): whether a cell's neighbour exists along each axis direction — a
+// periodic axis always wraps; a wall face has none beyond it.
+struct Nbr {
+  bool xm, xp, ym, yp, zm, zp;
+};
+KMI_DEV Nbr nbr(const StepArgs& a, int gx, int gy, int gz) {
+  Nbr n;
+  n.xm = a.per[0] || gx > 0;
+  n.xp = a.per[0] || gx < a.nx - 1;
+  n.ym = a.per[1] || gy > 0;
+  n.yp = a.per[1] || gy < a.ny - 1;
+  n.zm = a.per[2] || gz > 0;
+  n.zp = a.per[2] || gz < a.nz - 1;
+  return n;
+}
+// The source x − c_q of population q exists (else the link bounces back from a wall).
+template <int q>
+KMI_DEV bool src_ok(const Nbr& n) {
+  constexpr int cx = Lat<19>::cx[q], cy = Lat<19>::cy[q], cz = Lat<19>::cz[q];
+  return (cx > 0 ? n.xm : (cx < 0 ? n.xp : true)) && (cy > 0 ? n.ym : (cy < 0 ? n.yp : true)) &&
+         (cz > 0 ? n.zm : (cz < 0 ? n.zp : true));
+}
+// Ladd term of every moving face the bounced link q crosses, x then y then z: the
+// single-step kernel's pull, operation for operation.
+template <int q, class R>
+KMI_DEV R ladd(R v, const StepArgs& a, const Nbr& n) {
+  constexpr int cx = Lat<19>::cx[q], cy = Lat<19>::cy[q], cz = Lat<19>::cz[q];
+  auto add = [&](int fid) {
+    const FaceArgs& F = a.face[fid];
+    if (F.kind == KMI_FACE_MOVING_WALL)
+      v = v + R(six_w(Lat<19>::w[q])) * R(F.rho) * cdotu<cx, cy, cz>(R(F.u[0]), R(F.u[1]), R(F.u[2]));
+  };
+  if (cx > 0 && !n.xm) add(KMI_WEST);
+  if (cx < 0 && !n.xp) add(KMI_EAST);
+  if (cy > 0 && !n.ym) add(KMI_SOUTH);
+  if (cy < 0 && !n.yp) add(KMI_NORTH);
+  if (cz > 0 && !n.zm) add(KMI_BOTTOM);
+  if (cz < 0 && !n.zp) add(KMI_TOP);
+  return v;
+}
+
 // Element at byte offset `off` of a population plane.  Plane bases are uniform (scalar
 // registers) and the offsets 32-bit, so each access is one saddr-form global instruction
 // with no 64-bit address arithmetic.  A field holds fewer than 2^31 / 19 cells, so every
@@ -105,13 +146,77 @@ KMI_DEV void collide19(const R (&fp)[19], R (&fo)[19], const StepArgs& a) {
   }
 }
 
+// After the pull, the single-step kernel's order for a D3Q19 cell without thermal populations,
+// forcing, cut links or rebuilt faces: the Zou–He closures of the faces the cell lies on (in
+// the configured order), then the solid-cell rule or the collision.
+template <int ST, int COLL, bool BND, class R>
+KMI_DEV void update19(R (&fp)[19], R (&fo)[19], const StepArgs& a, int gx, int gy, int gz) {
+  if constexpr (BND) {
+    constexpr bool SH = Store<ST>::shifted;
+    for (int n = 0; n < a.n_zh; ++n) {
+      const int fid = a.zh_face[n];
+      const FaceArgs& F = a.face[fid];
+      switch (fid) {
+        case KMI_WEST: if (gx == 0) zh3_apply<SH, 0>(fp, F); break;
+        case KMI_EAST: if (gx == a.nx - 1) zh3_apply<SH, 1>(fp, F); break;
+        case KMI_SOUTH: if (gy == 0) zh3_apply<SH, 2>(fp, F); break;
+        case KMI_NORTH: if (gy == a.ny - 1) zh3_apply<SH, 3>(fp, F); break;
+        case KMI_BOTTOM: if (gz == 0) zh3_apply<SH, 4>(fp, F); break;
+        default: if (gz == a.nz - 1) zh3_apply<SH, 5>(fp, F); break;
+      }
+    }
+    const long long self = gx + (long long)a.nx * (gy + (long long)a.ny * gz);
+    if (a.mask != nullptr && a.mask[self] == kCellSolid) {
+      if (a.solid_inert) {
+        sfor<19>([&](auto qc) { fo[decltype(qc)::value] = SH ? R(0) : R(Lat<19>::w[decltype(qc)::value]); });
+      } else {
+        sfor<19>([&](auto qc) { fo[decltype(qc)::value] = fp[Lat<19>::opp[decltype(qc)::value]]; });
+      }
+      return;
+    }
+  }
+  collide19<ST, COLL>(fp, fo, a);
+}
+
+// A level's populations of one cell pulled from the previous level's values in LDS (grid
+// width EXp, NEp cells; the cell sits at (ex, ey) of that grid): c_z = 0 from A, c_z = +1 from
+// the plane below (Bs), c_z = −1 from the plane above (Cs).  Walled lattices: a link whose
+// source lies beyond a wall takes the cell's own opposite population of the previous level
+// (A, Bo, Co: the cell's own plane), plus the Ladd term of a moving wall.
+template <bool BND, class R>
+KMI_DEV void pull_lds(R (&fp)[19], const R* A, const R* Bs, const R* Cs, const R* Bo, const R* Co, int NEp,
+                      int EXp, int ex, int ey, const StepArgs& a, const tb::Nbr& nb) {
+  const int own = ey * EXp + ex;
+  sfor<19>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr int cx = Lat<19>::cx[q], cy = Lat<19>::cy[q], cz = Lat<19>::cz[q];
+    constexpr int n = tb::group_index(q), o = Lat<19>::opp[q], no = tb::group_index(o);
+    bool ok = true;
+    if constexpr (BND) ok = tb::src_ok<q>(nb);
+    if (ok) {
+      const int e = (ey - cy) * EXp + (ex - cx);
+      if constexpr (cz == 0) fp[q] = A[n * NEp + e];
+      else if constexpr (cz > 0) fp[q] = Bs[n * NEp + e];
+      else fp[q] = Cs[n * NEp + e];
+    } else if constexpr (BND) {
+      R v;
+      if constexpr (cz == 0) v = A[no * NEp + own];
+      else if constexpr (cz > 0) v = Co[no * NEp + own];  // opp(q) has c_z = −1
+      else v = Bo[no * NEp + own];
+      if constexpr (q != 0)
+        if (a.any_moving) v = tb::ladd<q>(v, a, nb);
+      fp[q] = v;
+    }
+  });
+}
+
 // Grid: one work-group per (column tile, z-range), padded to a multiple of 8.  Work-group
 // b runs on XCD b mod 8; it takes tile (b mod 8)·per_xcd + b/8, so each XCD works through a
 // contiguous band of neighbouring tiles and their shared halo rows meet in its L2.
 // NT threads per work-group, at least TX·TY: one per interior cell (a thread then evaluates
 // up to two extended cells in phase 1), or at least (TX+2)(TY+2), one per extended cell
 // (threads beyond TX·TY idle in phase 2).
-template <int ST, int COLL, int TX, int TY, int NT, bool NTS>
+template <int ST, int COLL, int TX, int TY, int NT, bool NTS, bool BND>
 __global__ void __launch_bounds__(NT)
     k_step2_d3q19(const StepArgs a, int zlen, int ntx, int nty, int ntiles, int per_xcd) {
   using S = Store<ST>;
@@ -121,10 +226,13 @@ __global__ void __launch_bounds__(NT)
   static_assert(NT >= NI && NT % 64 == 0, "one thread per interior cell at least");
   constexpr int EPT = (NE + NT - 1) / NT;  // extended cells per work-item in phase 1
   constexpr unsigned ES = sizeof(E);
-  __shared__ R lds[24 * NE];
+  // walled lattices keep the c_z = −1 values of two planes (the cell's own plane feeds its
+  // bounced links), 29 values per extended cell
+  constexpr int NC = BND ? 2 : 1;
+  __shared__ R lds[(19 + 5 * NC) * NE];
   R* const A = lds;            // c_z = 0:  [9][NE]
   R* const B = lds + 9 * NE;   // c_z = +1: [2 planes][5][NE]
-  R* const C = lds + 19 * NE;  // c_z = −1: [5][NE]
+  R* const C = lds + 19 * NE;  // c_z = −1: [NC planes][5][NE]
 
   const int t = int(blockIdx.x % 8u) * per_xcd + int(blockIdx.x / 8u);
   if (t >= ntiles) return;  // padding groups (uniform per group, before any barrier)
@@ -140,16 +248,24 @@ __global__ void __launch_bounds__(NT)
   // Pull of the t populations of plane s for the extended cells this work-item owns.  With
   // FP32 arithmetic the pull for plane s+1 is issued before plane s is processed, so its HBM
   // latency overlaps the collisions, the barriers and the phase-2 stores of plane s; FP64
-  // and 1024-wide groups have no registers to spare for it.
+  // and 1024-wide groups have no registers to spare for it.  Walled lattices: a link from
+  // beyond a wall reads the cell's own opposite population (halfway bounce-back, with the
+  // Ladd term of a moving wall); extended cells beyond a wall are never read and skipped.
   constexpr bool kPrefetch = sizeof(R) == 4 && (NT <= 512 || EPT == 1);
   R nxt[EPT][19];
+  auto inside = [&](int s, int e) {
+    if constexpr (!BND) return true;
+    const int ux = x0 - 1 + e % EX, uy = y0 - 1 + e / EX;
+    return (a.per[0] || (ux >= 0 && ux < nx)) && (a.per[1] || (uy >= 0 && uy < ny)) &&
+           (a.per[2] || (s >= 0 && s < nz));
+  };
   auto pull_plane = [&](int s) {
     const int zs = tb::wrap(s, nz);
     const int zm = tb::wrap(zs - 1, nz), zp = tb::wrap(zs + 1, nz);
 #pragma unroll
     for (int r = 0; r < EPT; ++r) {
       const int e = tid + r * NT;
-      if (e < NE) {
+      if (e < NE && inside(s, e)) {
         const int ex = e % EX, ey = e / EX;
         const int gx = tb::wrap(x0 - 1 + ex, nx), gy = tb::wrap(y0 - 1 + ey, ny);
         const int xm = tb::wrap(gx - 1, nx), xp = tb::wrap(gx + 1, nx);
@@ -157,11 +273,23 @@ __global__ void __launch_bounds__(NT)
         // byte offsets of the three source columns and of the nine source rows
         const unsigned bx[3] = {unsigned(xm) * ES, unsigned(gx) * ES, unsigned(xp) * ES};
         const int yy[3] = {ym, gy, yp}, zz[3] = {zm, zs, zp};
+        tb::Nbr nb{};
+        if constexpr (BND) nb = tb::nbr(a, gx, gy, zs);
+        const unsigned own = (unsigned(gx) + unsigned(nx) * unsigned(gy + ny * zs)) * ES;
         sfor<19>([&](auto qc) {
           constexpr int q = decltype(qc)::value;
           constexpr int cx = Lat<19>::cx[q], cy = Lat<19>::cy[q], cz = Lat<19>::cz[q];
-          const unsigned row = unsigned(nx) * unsigned(yy[1 - cy] + ny * zz[1 - cz]) * ES;
-          nxt[r][q] = Pops<ST>::ld(&tb::at(fi + q * P, row + bx[1 - cx]));
+          bool ok = true;
+          if constexpr (BND) ok = tb::src_ok<q>(nb);
+          if (ok) {
+            const unsigned row = unsigned(nx) * unsigned(yy[1 - cy] + ny * zz[1 - cz]) * ES;
+            nxt[r][q] = Pops<ST>::ld(&tb::at(fi + q * P, row + bx[1 - cx]));
+          } else if constexpr (BND) {
+            R v = Pops<ST>::ld(&tb::at(fi + Lat<19>::opp[q] * P, own));
+            if constexpr (q != 0)
+              if (a.any_moving) v = tb::ladd<q>(v, a, nb);
+            nxt[r][q] = v;
+          }
         });
       }
     }
@@ -170,6 +298,7 @@ __global__ void __launch_bounds__(NT)
 
   for (int s = z0 - 1; s <= z1; ++s) {
     const int u = s + 3;  // non-negative plane counter for the LDS ring slot
+    R* const Cw = C + (BND ? (u & 1) : 0) * 5 * NE;
     // ---- phase 1: t+1 on plane s, extended tile ----
     if constexpr (!kPrefetch) pull_plane(s);
     R cur[EPT][19];
@@ -183,9 +312,10 @@ __global__ void __launch_bounds__(NT)
 #pragma unroll
     for (int r = 0; r < EPT; ++r) {
       const int e = tid + r * NT;
-      if (e < NE) {
+      if (e < NE && inside(s, e)) {
         R fn[19];
-        collide19<ST, COLL>(cur[r], fn, a);
+        update19<ST, COLL, BND>(cur[r], fn, a, tb::wrap(x0 - 1 + e % EX, nx), tb::wrap(y0 - 1 + e / EX, ny),
+                                tb::wrap(s, nz));
         sfor<19>([&](auto qc) {
           constexpr int q = decltype(qc)::value;
           constexpr int cz = Lat<19>::cz[q];
@@ -193,26 +323,22 @@ __global__ void __launch_bounds__(NT)
           const R v = Pops<ST>::round(fn[q]);
           if constexpr (cz == 0) keep0[r][n] = v;
           else if constexpr (cz > 0) keepp[r][n] = v;
-          else C[n * NE + e] = v;
+          else Cw[n * NE + e] = v;
         });
       }
     }
     __syncthreads();
     // ---- phase 2: t+2 on plane s−1, interior tile ----
     if (s >= z0 + 1 && tid < NI) {
-      const R* const Br = B + ((u - 2) & 1) * 5 * NE;
+      const R* const Br = B + ((u - 2) & 1) * 5 * NE;  // c_z = +1 of plane s−2
+      const R* const Bo = B + ((u - 1) & 1) * 5 * NE;  // c_z = +1 of plane s−1 (own)
+      const R* const Co = C + (BND ? ((u - 1) & 1) : 0) * 5 * NE;  // c_z = −1 of plane s−1 (own)
       const int ix = tid % TX, iy = tid / TX;
+      tb::Nbr nb{};
+      if constexpr (BND) nb = tb::nbr(a, x0 + ix, y0 + iy, s - 1);
       R fp[19], fn[19];
-      sfor<19>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        constexpr int cx = Lat<19>::cx[q], cy = Lat<19>::cy[q], cz = Lat<19>::cz[q];
-        constexpr int n = tb::group_index(q);
-        const int e = (iy + 1 - cy) * EX + (ix + 1 - cx);
-        if constexpr (cz == 0) fp[q] = A[n * NE + e];
-        else if constexpr (cz > 0) fp[q] = Br[n * NE + e];
-        else fp[q] = C[n * NE + e];
-      });
-      collide19<ST, COLL>(fp, fn, a);
+      pull_lds<BND>(fp, A, Br, Cw, Bo, Co, NE, EX, ix + 1, iy + 1, a, nb);
+      update19<ST, COLL, BND>(fp, fn, a, x0 + ix, y0 + iy, s - 1);
       const unsigned cell = (unsigned(x0 + ix) + unsigned(nx) * unsigned(y0 + iy + ny * (s - 1))) * ES;
       sfor<19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
@@ -226,7 +352,7 @@ __global__ void __launch_bounds__(NT)
 #pragma unroll
       for (int r = 0; r < EPT; ++r) {
         const int e = tid + r * NT;
-        if (e < NE) {
+        if (e < NE && inside(s, e)) {
 #pragma unroll
           for (int n = 0; n < 9; ++n) A[n * NE + e] = keep0[r][n];
 #pragma unroll
@@ -258,7 +384,7 @@ __global__ void __launch_bounds__(NT)
 // level-1 and two level-2 cells one after the other; with NT ≥ (TX+4)(TY+4) every cell of a
 // level has its own thread, which halves the registers a thread holds (more resident waves
 // per CU) and runs the halo evaluations side by side; threads beyond TX·TY idle in P3.
-template <int ST, int COLL, int TX, int TY, int NT, bool NTS>
+template <int ST, int COLL, int TX, int TY, int NT, bool NTS, bool BND>
 __global__ void __launch_bounds__(NT)
     k_step3_d3q19(const StepArgs a, int zlen, int ntx, int nty, int ntiles, int per_xcd) {
   using S = Store<ST>;
@@ -270,14 +396,16 @@ __global__ void __launch_bounds__(NT)
   static_assert(NT >= NI && NT % 64 == 0, "one thread per interior cell at least");
   constexpr int EPT1 = (N1 + NT - 1) / NT, EPT2 = (N2 + NT - 1) / NT;
   constexpr unsigned ES = sizeof(E);
-  __shared__ R lds[24 * (N1 + N2)];
+  constexpr int NC = BND ? 2 : 1;  // c_z = −1 planes per level (walled: the own plane too)
+  constexpr int VPC = 19 + 5 * NC;  // values per cell and level
+  __shared__ R lds[VPC * (N1 + N2)];
   R* const A1 = lds;             // c_z = 0:  [9][N1]
   R* const B1 = lds + 9 * N1;    // c_z = +1: [2 planes][5][N1]
-  R* const C1 = lds + 19 * N1;   // c_z = −1: [5][N1]
-  R* const L2 = lds + 24 * N1;
+  R* const C1 = lds + 19 * N1;   // c_z = −1: [NC planes][5][N1]
+  R* const L2 = lds + VPC * N1;
   R* const A2 = L2;              // [9][N2]
   R* const B2 = L2 + 9 * N2;     // [2][5][N2]
-  R* const C2 = L2 + 19 * N2;    // [5][N2]
+  R* const C2 = L2 + 19 * N2;    // [NC][5][N2]
 
   const int t = int(blockIdx.x % 8u) * per_xcd + int(blockIdx.x / 8u);
   if (t >= ntiles) return;  // padding groups (uniform per group, before any barrier)
@@ -290,6 +418,12 @@ __global__ void __launch_bounds__(NT)
   E* __restrict__ fo = static_cast<E*>(a.fo);
   const int tid = threadIdx.x;
 
+  // cell (ux, uy) of plane s lies in the domain (walled lattices; periodic axes always)
+  auto inside = [&](int ux, int uy, int s) {
+    if constexpr (!BND) return true;
+    return (a.per[0] || (ux >= 0 && ux < nx)) && (a.per[1] || (uy >= 0 && uy < ny)) &&
+           (a.per[2] || (s >= 0 && s < nz));
+  };
   constexpr bool kPrefetch = sizeof(R) == 4 && (NT <= 512 || EPT1 == 1);
   R nxt[EPT1][19];
   auto pull_plane = [&](int s) {
@@ -298,18 +432,30 @@ __global__ void __launch_bounds__(NT)
 #pragma unroll
     for (int r = 0; r < EPT1; ++r) {
       const int e = tid + r * NT;
-      if (e < N1) {
+      if (e < N1 && inside(x0 - 2 + e % E1X, y0 - 2 + e / E1X, s)) {
         const int ex = e % E1X, ey = e / E1X;
         const int gx = tb::wrap(x0 - 2 + ex, nx), gy = tb::wrap(y0 - 2 + ey, ny);
         const int xm = tb::wrap(gx - 1, nx), xp = tb::wrap(gx + 1, nx);
         const int ym = tb::wrap(gy - 1, ny), yp = tb::wrap(gy + 1, ny);
         const unsigned bx[3] = {unsigned(xm) * ES, unsigned(gx) * ES, unsigned(xp) * ES};
         const int yy[3] = {ym, gy, yp}, zz[3] = {zm, zs, zp};
+        tb::Nbr nb{};
+        if constexpr (BND) nb = tb::nbr(a, gx, gy, zs);
+        const unsigned own = (unsigned(gx) + unsigned(nx) * unsigned(gy + ny * zs)) * ES;
         sfor<19>([&](auto qc) {
           constexpr int q = decltype(qc)::value;
           constexpr int cx = Lat<19>::cx[q], cy = Lat<19>::cy[q], cz = Lat<19>::cz[q];
-          const unsigned row = unsigned(nx) * unsigned(yy[1 - cy] + ny * zz[1 - cz]) * ES;
-          nxt[r][q] = Pops<ST>::ld(&tb::at(fi + q * P, row + bx[1 - cx]));
+          bool ok = true;
+          if constexpr (BND) ok = tb::src_ok<q>(nb);
+          if (ok) {
+            const unsigned row = unsigned(nx) * unsigned(yy[1 - cy] + ny * zz[1 - cz]) * ES;
+            nxt[r][q] = Pops<ST>::ld(&tb::at(fi + q * P, row + bx[1 - cx]));
+          } else if constexpr (BND) {
+            R v = Pops<ST>::ld(&tb::at(fi + Lat<19>::opp[q] * P, own));
+            if constexpr (q != 0)
+              if (a.any_moving) v = tb::ladd<q>(v, a, nb);
+            nxt[r][q] = v;
+          }
         });
       }
     }
@@ -318,6 +464,8 @@ __global__ void __launch_bounds__(NT)
 
   for (int s = z0 - 2; s <= z1 + 1; ++s) {
     const int u = s + 4;  // non-negative plane counter for the ring slots
+    R* const C1w = C1 + (BND ? (u & 1) : 0) * 5 * N1;        // c_z = −1, level 1, plane s
+    R* const C2w = C2 + (BND ? ((u - 1) & 1) : 0) * 5 * N2;  // c_z = −1, level 2, plane s−1
     // ---- P1: t+1 on plane s, level-1 tile ----
     if constexpr (!kPrefetch) pull_plane(s);
     R cur[EPT1][19];
@@ -331,9 +479,10 @@ __global__ void __launch_bounds__(NT)
 #pragma unroll
     for (int r = 0; r < EPT1; ++r) {
       const int e = tid + r * NT;
-      if (e < N1) {
+      if (e < N1 && inside(x0 - 2 + e % E1X, y0 - 2 + e / E1X, s)) {
         R fn[19];
-        collide19<ST, COLL>(cur[r], fn, a);
+        update19<ST, COLL, BND>(cur[r], fn, a, tb::wrap(x0 - 2 + e % E1X, nx), tb::wrap(y0 - 2 + e / E1X, ny),
+                                tb::wrap(s, nz));
         sfor<19>([&](auto qc) {
           constexpr int q = decltype(qc)::value;
           constexpr int cz = Lat<19>::cz[q];
@@ -341,7 +490,7 @@ __global__ void __launch_bounds__(NT)
           const R v = Pops<ST>::round(fn[q]);
           if constexpr (cz == 0) k1z[r][n] = v;
           else if constexpr (cz > 0) k1p[r][n] = v;
-          else C1[n * N1 + e] = v;
+          else C1w[n * N1 + e] = v;
         });
       }
     }
@@ -349,23 +498,20 @@ __global__ void __launch_bounds__(NT)
     // ---- P2: t+2 on plane s−1, level-2 tile ----
     R k2z[EPT2][9], k2p[EPT2][5];
     if (s >= z0) {
-      const R* const B1r = B1 + ((u - 2) & 1) * 5 * N1;
+      const R* const B1r = B1 + ((u - 2) & 1) * 5 * N1;  // c_z = +1 of plane s−2
+      const R* const B1o = B1 + ((u - 1) & 1) * 5 * N1;  // c_z = +1 of plane s−1 (own)
+      const R* const C1o = C1 + (BND ? ((u - 1) & 1) : 0) * 5 * N1;  // c_z = −1 of plane s−1 (own)
 #pragma unroll
       for (int r = 0; r < EPT2; ++r) {
         const int e = tid + r * NT;
-        if (e < N2) {
+        if (e < N2 && inside(x0 - 1 + e % E2X, y0 - 1 + e / E2X, s - 1)) {
           const int ex = e % E2X, ey = e / E2X;
+          const int gx = tb::wrap(x0 - 1 + ex, nx), gy = tb::wrap(y0 - 1 + ey, ny), gz = tb::wrap(s - 1, nz);
+          tb::Nbr nb{};
+          if constexpr (BND) nb = tb::nbr(a, gx, gy, gz);
           R fp[19], fn[19];
-          sfor<19>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            constexpr int cx = Lat<19>::cx[q], cy = Lat<19>::cy[q], cz = Lat<19>::cz[q];
-            constexpr int n = tb::group_index(q);
-            const int src = (ey + 1 - cy) * E1X + (ex + 1 - cx);
-            if constexpr (cz == 0) fp[q] = A1[n * N1 + src];
-            else if constexpr (cz > 0) fp[q] = B1r[n * N1 + src];
-            else fp[q] = C1[n * N1 + src];
-          });
-          collide19<ST, COLL>(fp, fn, a);
+          pull_lds<BND>(fp, A1, B1r, C1w, B1o, C1o, N1, E1X, ex + 1, ey + 1, a, nb);
+          update19<ST, COLL, BND>(fp, fn, a, gx, gy, gz);
           sfor<19>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             constexpr int cz = Lat<19>::cz[q];
@@ -373,7 +519,7 @@ __global__ void __launch_bounds__(NT)
             const R v = Pops<ST>::round(fn[q]);
             if constexpr (cz == 0) k2z[r][n] = v;
             else if constexpr (cz > 0) k2p[r][n] = v;
-            else C2[n * N2 + e] = v;
+            else C2w[n * N2 + e] = v;
           });
         }
       }
@@ -381,19 +527,15 @@ __global__ void __launch_bounds__(NT)
     __syncthreads();
     // ---- P3: t+3 on plane s−2, interior; then the P1 values into level 1 ----
     if (s >= z0 + 2 && tid < NI) {
-      const R* const B2r = B2 + ((u - 3) & 1) * 5 * N2;
+      const R* const B2r = B2 + ((u - 3) & 1) * 5 * N2;  // c_z = +1 of plane s−3
+      const R* const B2o = B2 + ((u - 2) & 1) * 5 * N2;  // c_z = +1 of plane s−2 (own)
+      const R* const C2o = C2 + (BND ? ((u - 2) & 1) : 0) * 5 * N2;  // c_z = −1 of plane s−2 (own)
       const int ix = tid % TX, iy = tid / TX;
+      tb::Nbr nb{};
+      if constexpr (BND) nb = tb::nbr(a, x0 + ix, y0 + iy, s - 2);
       R fp[19], fn[19];
-      sfor<19>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        constexpr int cx = Lat<19>::cx[q], cy = Lat<19>::cy[q], cz = Lat<19>::cz[q];
-        constexpr int n = tb::group_index(q);
-        const int src = (iy + 1 - cy) * E2X + (ix + 1 - cx);
-        if constexpr (cz == 0) fp[q] = A2[n * N2 + src];
-        else if constexpr (cz > 0) fp[q] = B2r[n * N2 + src];
-        else fp[q] = C2[n * N2 + src];
-      });
-      collide19<ST, COLL>(fp, fn, a);
+      pull_lds<BND>(fp, A2, B2r, C2w, B2o, C2o, N2, E2X, ix + 1, iy + 1, a, nb);
+      update19<ST, COLL, BND>(fp, fn, a, x0 + ix, y0 + iy, s - 2);
       const unsigned cell = (unsigned(x0 + ix) + unsigned(nx) * unsigned(y0 + iy + ny * (s - 2))) * ES;
       sfor<19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
@@ -405,7 +547,7 @@ __global__ void __launch_bounds__(NT)
 #pragma unroll
       for (int r = 0; r < EPT1; ++r) {
         const int e = tid + r * NT;
-        if (e < N1) {
+        if (e < N1 && inside(x0 - 2 + e % E1X, y0 - 2 + e / E1X, s)) {
 #pragma unroll
           for (int n = 0; n < 9; ++n) A1[n * N1 + e] = k1z[r][n];
 #pragma unroll
@@ -420,7 +562,7 @@ __global__ void __launch_bounds__(NT)
 #pragma unroll
       for (int r = 0; r < EPT2; ++r) {
         const int e = tid + r * NT;
-        if (e < N2) {
+        if (e < N2 && inside(x0 - 1 + e % E2X, y0 - 1 + e / E2X, s - 1)) {
 #pragma unroll
           for (int n = 0; n < 9; ++n) A2[n * N2 + e] = k2z[r][n];
 #pragma unroll
@@ -467,77 +609,100 @@ int temporal_span(int tile) {
   return tile >= 1 ? 2 : 1;
 }
 
+// Walled lattices (bnd): tiles 12 and 14 (two steps) and 9 (three steps, not FP64: 29 values
+// per cell of both levels need 179 KB of LDS in FP64), non-temporal stores.
+bool step2_tile_walled(int tile, int st) {
+  if (st == KMI_STORE_F16S) return false;
+  return tile == 12 || tile == 14 || (tile == 9 && st != KMI_STORE_F64);
+}
+
 template <int ST, int COLL, int TX, int TY, int NT = TX * TY>
-static hipError_t launch2(bool nts, const StepArgs& a, int zchunks, hipStream_t s) {
+static hipError_t launch2(bool nts, bool bnd, const StepArgs& a, int zchunks, hipStream_t s) {
   const int ntx = a.nx / TX, nty = a.ny / TY;
   const int zlen = (a.nz + zchunks - 1) / zchunks;
   const int nzc = (a.nz + zlen - 1) / zlen;
   const int ntiles = ntx * nty * nzc;
   const int per_xcd = (ntiles + 7) / 8;
   const dim3 g(static_cast<unsigned>(8 * per_xcd)), b(static_cast<unsigned>(NT));
-  if (nts) hipLaunchKernelGGL((k_step2_d3q19<ST, COLL, TX, TY, NT, true>), g, b, 0, s, a, zlen, ntx, nty, ntiles, per_xcd);
-  else hipLaunchKernelGGL((k_step2_d3q19<ST, COLL, TX, TY, NT, false>), g, b, 0, s, a, zlen, ntx, nty, ntiles, per_xcd);
+  if (bnd) {
+    if constexpr (ST != KMI_STORE_F16S && (NT == 704 || NT == 384))
+      hipLaunchKernelGGL((k_step2_d3q19<ST, COLL, TX, TY, NT, true, true>), g, b, 0, s, a, zlen, ntx, nty, ntiles, per_xcd);
+    else
+      return hipErrorInvalidValue;
+  } else if (nts) {
+    hipLaunchKernelGGL((k_step2_d3q19<ST, COLL, TX, TY, NT, true, false>), g, b, 0, s, a, zlen, ntx, nty, ntiles, per_xcd);
+  } else {
+    hipLaunchKernelGGL((k_step2_d3q19<ST, COLL, TX, TY, NT, false, false>), g, b, 0, s, a, zlen, ntx, nty, ntiles, per_xcd);
+  }
   return hipGetLastError();
 }
 
 template <int ST, int COLL, int TX, int TY, int NT = TX * TY>
-static hipError_t launch3(bool nts, const StepArgs& a, int zchunks, hipStream_t s) {
+static hipError_t launch3(bool nts, bool bnd, const StepArgs& a, int zchunks, hipStream_t s) {
   const int ntx = a.nx / TX, nty = a.ny / TY;
   const int zlen = (a.nz + zchunks - 1) / zchunks;
   const int nzc = (a.nz + zlen - 1) / zlen;
   const int ntiles = ntx * nty * nzc;
   const int per_xcd = (ntiles + 7) / 8;
   const dim3 g(static_cast<unsigned>(8 * per_xcd)), b(static_cast<unsigned>(NT));
-  if (nts) hipLaunchKernelGGL((k_step3_d3q19<ST, COLL, TX, TY, NT, true>), g, b, 0, s, a, zlen, ntx, nty, ntiles, per_xcd);
-  else hipLaunchKernelGGL((k_step3_d3q19<ST, COLL, TX, TY, NT, false>), g, b, 0, s, a, zlen, ntx, nty, ntiles, per_xcd);
+  if (bnd) {
+    if constexpr (ST != KMI_STORE_F16S && ST != KMI_STORE_F64 && NT == 448)
+      hipLaunchKernelGGL((k_step3_d3q19<ST, COLL, TX, TY, NT, true, true>), g, b, 0, s, a, zlen, ntx, nty, ntiles, per_xcd);
+    else
+      return hipErrorInvalidValue;
+  } else if (nts) {
+    hipLaunchKernelGGL((k_step3_d3q19<ST, COLL, TX, TY, NT, true, false>), g, b, 0, s, a, zlen, ntx, nty, ntiles, per_xcd);
+  } else {
+    hipLaunchKernelGGL((k_step3_d3q19<ST, COLL, TX, TY, NT, false, false>), g, b, 0, s, a, zlen, ntx, nty, ntiles, per_xcd);
+  }
   return hipGetLastError();
 }
 
 template <int ST, int COLL>
-static hipError_t launch2_tile(int tile, bool nts, const StepArgs& a, int zchunks, hipStream_t s) {
+static hipError_t launch2_tile(int tile, bool nts, bool bnd, const StepArgs& a, int zchunks, hipStream_t s) {
   switch (tile) {
-    case 7: return launch3<ST, COLL, 32, 8>(nts, a, zchunks, s);
-    case 9: return launch3<ST, COLL, 32, 8, 448>(nts, a, zchunks, s);
+    case 7: return launch3<ST, COLL, 32, 8>(nts, bnd, a, zchunks, s);
+    case 9: return launch3<ST, COLL, 32, 8, 448>(nts, bnd, a, zchunks, s);
     case 6:
     case 8:
     case 10:
     case 11:
       if constexpr (ST != KMI_STORE_F64) {
-        if (tile == 6) return launch3<ST, COLL, 64, 8>(nts, a, zchunks, s);
-        if (tile == 8) return launch3<ST, COLL, 32, 16>(nts, a, zchunks, s);
-        if (tile == 10) return launch3<ST, COLL, 32, 16, 768>(nts, a, zchunks, s);
-        return launch3<ST, COLL, 64, 8, 832>(nts, a, zchunks, s);
+        if (tile == 6) return launch3<ST, COLL, 64, 8>(nts, bnd, a, zchunks, s);
+        if (tile == 8) return launch3<ST, COLL, 32, 16>(nts, bnd, a, zchunks, s);
+        if (tile == 10) return launch3<ST, COLL, 32, 16, 768>(nts, bnd, a, zchunks, s);
+        return launch3<ST, COLL, 64, 8, 832>(nts, bnd, a, zchunks, s);
       } else {
         return hipErrorInvalidValue;
       }
-    case 1: return launch2<ST, COLL, 32, 8>(nts, a, zchunks, s);
-    case 2: return launch2<ST, COLL, 64, 4>(nts, a, zchunks, s);
-    case 3: return launch2<ST, COLL, 64, 8>(nts, a, zchunks, s);
-    case 4: return launch2<ST, COLL, 32, 16>(nts, a, zchunks, s);
-    case 12: return launch2<ST, COLL, 64, 8, 704>(nts, a, zchunks, s);
-    case 13: return launch2<ST, COLL, 32, 16, 640>(nts, a, zchunks, s);
-    case 14: return launch2<ST, COLL, 32, 8, 384>(nts, a, zchunks, s);
+    case 1: return launch2<ST, COLL, 32, 8>(nts, bnd, a, zchunks, s);
+    case 2: return launch2<ST, COLL, 64, 4>(nts, bnd, a, zchunks, s);
+    case 3: return launch2<ST, COLL, 64, 8>(nts, bnd, a, zchunks, s);
+    case 4: return launch2<ST, COLL, 32, 16>(nts, bnd, a, zchunks, s);
+    case 12: return launch2<ST, COLL, 64, 8, 704>(nts, bnd, a, zchunks, s);
+    case 13: return launch2<ST, COLL, 32, 16, 640>(nts, bnd, a, zchunks, s);
+    case 14: return launch2<ST, COLL, 32, 8, 384>(nts, bnd, a, zchunks, s);
     case 5:
-      if constexpr (ST != KMI_STORE_F64) return launch2<ST, COLL, 64, 16>(nts, a, zchunks, s);
+      if constexpr (ST != KMI_STORE_F64) return launch2<ST, COLL, 64, 16>(nts, bnd, a, zchunks, s);
       else return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
   }
 }
 
 template <int ST>
-static hipError_t launch2_coll(int coll, int tile, bool nts, const StepArgs& a, int zchunks,
+static hipError_t launch2_coll(int coll, int tile, bool nts, bool bnd, const StepArgs& a, int zchunks,
                                hipStream_t s) {
-  return coll == KMI_TRT ? launch2_tile<ST, KMI_TRT>(tile, nts, a, zchunks, s)
-                         : launch2_tile<ST, KMI_BGK>(tile, nts, a, zchunks, s);
+  return coll == KMI_TRT ? launch2_tile<ST, KMI_TRT>(tile, nts, bnd, a, zchunks, s)
+                         : launch2_tile<ST, KMI_BGK>(tile, nts, bnd, a, zchunks, s);
 }
 
-hipError_t launch_step2(int st, int coll, int tile, int zchunks, bool nts, const StepArgs& a,
+hipError_t launch_step2(int st, int coll, int tile, int zchunks, bool nts, bool bnd, const StepArgs& a,
                         hipStream_t s) {
   switch (st) {
-    case KMI_STORE_F64: return launch2_coll<KMI_STORE_F64>(coll, tile, nts, a, zchunks, s);
-    case KMI_STORE_F32: return launch2_coll<KMI_STORE_F32>(coll, tile, nts, a, zchunks, s);
-    case KMI_STORE_F32S: return launch2_coll<KMI_STORE_F32S>(coll, tile, nts, a, zchunks, s);
-    default: return launch2_coll<KMI_STORE_F16S>(coll, tile, nts, a, zchunks, s);
+    case KMI_STORE_F64: return launch2_coll<KMI_STORE_F64>(coll, tile, nts, bnd, a, zchunks, s);
+    case KMI_STORE_F32: return launch2_coll<KMI_STORE_F32>(coll, tile, nts, bnd, a, zchunks, s);
+    case KMI_STORE_F32S: return launch2_coll<KMI_STORE_F32S>(coll, tile, nts, bnd, a, zchunks, s);
+    default: return launch2_coll<KMI_STORE_F16S>(coll, tile, nts, bnd, a, zchunks, s);
   }
 }
 

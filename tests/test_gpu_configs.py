@@ -1,13 +1,15 @@
 """The benchmark configurations at their real sizes.
 
 Parity at full size: FP64 populations bit-identical to the oracle (cavity 128² for the
-full 20000 steps; TGV 256³ and RB 512×256 for as many steps as the CPU oracle finishes in
-seconds).  Reduced-precision storage is judged against the FP64 run; every test prints
-the max / L2 errors it measured and asserts the bound written in it.
+full 20000 steps; TGV 256³ for the configuration's 1000 steps; RB 512×256 for as many steps
+as the CPU oracle finishes in seconds).  Reduced-precision storage is judged against the
+FP64 oracle field (TGV) or the FP64 run (RB, 150k steps); every test prints the max / L2
+errors it measured and asserts the bound written in it.
 """
 import json
 import math
 import os
+import threading
 
 import numpy as np
 import pytest
@@ -23,6 +25,14 @@ def _record(name, data):
     cur = json.load(open(path)) if os.path.exists(path) else {}
     cur[name] = data
     json.dump(cur, open(path, "w"), indent=1)
+
+
+def _progress(msg):
+    """Print and append to gpurun_out/progress.log (long tests report as they go)."""
+    print(msg, flush=True)
+    os.makedirs(OUT, exist_ok=True)
+    with open(os.path.join(OUT, "progress.log"), "a") as fh:
+        fh.write(msg + "\n")
 
 
 def _err(a, b):
@@ -97,32 +107,58 @@ def test_config2_tgv_256_fp64_bitwise_and_reduced_precision(kmi, po, backend):
                 out.append(lat.download_raw())  # the stored elements verbatim
         assert np.array_equal(out[0], out[1]), (st, tile)
         del out
+    # 1000 steps: the FP64 oracle on the host cores (a thread; ctypes releases the GIL) while
+    # the device runs the four storages; FP64 is compared bit for bit with the oracle, the
+    # reduced storages state their errors against the oracle's field
     steps = 1000
+    f_ref = po.equilibrium(*fields)
+    oracle_done = []
+
+    def oracle_run():
+        for k in range(0, steps, 100):
+            po.run(d64, f_ref, None, 100)
+            _progress(f"[config2] oracle FP64 {k + 100}/{steps} steps")
+        oracle_done.append(True)
+
+    th = threading.Thread(target=oracle_run)
+    th.start()
     res = {}
-    for st in ("f64", "f32", "f32s", "f16s"):
-        print(f"[config2] TGV 256^3 {st} {steps} steps", flush=True)
-        with kmi.Lattice(kmi.taylor_green_3d_desc(N, nu, st), backend) as lat:
-            lat.init_equilibrium(*fields)
-            m0 = lat.reduce("mass")
-            ke0 = lat.reduce("kinetic_energy")
-            lat.step(steps)
-            res[st] = lat.moments()
-            res[st]["mass_drift"] = abs(lat.reduce("mass") - m0) / m0
-            res[st]["ke_ratio"] = lat.reduce("kinetic_energy") / ke0
-    ref = res["f64"]
+    try:
+        for st in ("f64", "f32", "f32s", "f16s"):
+            _progress(f"[config2] TGV 256^3 {st} {steps} steps on the device")
+            with kmi.Lattice(kmi.taylor_green_3d_desc(N, nu, st), backend) as lat:
+                lat.init_equilibrium(*fields)
+                m0 = lat.reduce("mass")
+                ke0 = lat.reduce("kinetic_energy")
+                lat.step(steps)
+                res[st] = lat.moments()
+                res[st]["mass_drift"] = abs(lat.reduce("mass") - m0) / m0
+                res[st]["ke_ratio"] = lat.reduce("kinetic_energy") / ke0
+                if st == "f64":
+                    f64_dev = lat.download()
+    finally:
+        th.join()
+    assert oracle_done
+    assert np.array_equal(f64_dev, f_ref), "FP64 populations differ from the oracle after 1000 steps"
+    del f64_dev
+    ref = po.moments(f_ref)
+    del f_ref
     k = 2 * np.pi / N
     decay = math.exp(-2 * 3 * nu * k * k * steps)
-    report = {"ke_ratio_f64": ref["ke_ratio"], "ke_linear_theory": decay, "mass_drift_f64": ref["mass_drift"]}
-    assert ref["mass_drift"] < 1e-12
-    assert abs(ref["ke_ratio"] - decay) < 0.01 * decay
-    bounds = {"f32": 1e-4, "f32s": 1e-5, "f16s": 3e-2}
-    for st in ("f32", "f32s", "f16s"):
+    report = {"steps": steps, "against": "FP64 oracle (C++ restatement, host cores)",
+              "ke_ratio_f64": res["f64"]["ke_ratio"], "ke_linear_theory": decay,
+              "mass_drift_f64": res["f64"]["mass_drift"]}
+    assert res["f64"]["mass_drift"] < 1e-12
+    assert abs(res["f64"]["ke_ratio"] - decay) < 0.01 * decay
+    # written bounds (L2-relative u, max |ρ error|) against the FP64 oracle field
+    bounds = {"f64": 1e-12, "f32": 1e-4, "f32s": 1e-5, "f16s": 3e-2}
+    for st in ("f64", "f32", "f32s", "f16s"):
         e = {q: _err(res[st][q], ref[q]) for q in ("ux", "uy", "uz", "rho")}
         e["mass_drift"] = res[st]["mass_drift"]
         report[st] = e
-        print(f"[config2] {st}: " + ", ".join(f"{q} max {e[q]['max_abs']:.2e} L2rel {e[q]['l2_rel']:.2e}"
-                                               for q in ("ux", "uy", "rho")), flush=True)
-        assert e["ux"]["l2_rel"] < bounds[st] and e["uy"]["l2_rel"] < bounds[st]
+        _progress(f"[config2] {st} vs oracle: " + ", ".join(f"{q} max {e[q]['max_abs']:.2e} L2rel {e[q]['l2_rel']:.2e}"
+                                                             for q in ("ux", "uy", "uz", "rho")))
+        assert e["ux"]["l2_rel"] < bounds[st] and e["uy"]["l2_rel"] < bounds[st] and e["uz"]["l2_rel"] < 2 * bounds[st]
         assert e["rho"]["max_abs"] < bounds[st]
     _record("config2_tgv256", report)
 
@@ -141,27 +177,100 @@ def test_config3_rayleigh_benard_512x256(kmi, po, backend):
         fg, gg = lat.download(), lat.download("g")
     po.run(d64, f0, g0, 200)
     assert np.array_equal(fg, f0) and np.array_equal(gg, g0)
-    steps = 150_000
+    # Ra = 1e5 in this box is unsteady: after 150k steps runs that differ by rounding have
+    # decorrelated pointwise, so the storages are compared on the Nusselt number averaged over
+    # the last 20k steps (20 samples) and, for T and u, on the fields at the end
+    steps, window, every = 150_000, 20_000, 1_000
     res = {}
     for st in ("f64", "f32", "f32s", "f16s"):
-        print(f"[config3] RB {st} {steps} steps", flush=True)
+        _progress(f"[config3] RB {st} {steps} steps")
         d, _ = kmi.rayleigh_benard_desc(Nx, Ny, Ra, Pr, storage=st)
         with kmi.Lattice(d, backend) as lat:
             lat.init_temperature(T0)
-            lat.step(steps)
+            lat.step(steps - window)
+            nus = []
+            for _ in range(window // every):
+                lat.step(every)
+                nus.append(lat.reduce("nusselt", "south"))
             m = lat.moments(temperature=True)
-            m["Nu"] = lat.reduce("nusselt", "south")
+            m["Nu"] = float(np.mean(nus))
+            m["Nu_end"] = nus[-1]
             res[st] = m
     ref = res["f64"]
-    report = {"steps": steps, "Nu_f64": ref["Nu"]}
-    print(f"[config3] Nu f64 = {ref['Nu']:.6f}", flush=True)
-    assert 1.0 < ref["Nu"] < 8.0
+    report = {"steps": steps, "Nu_window_steps": window, "Nu_f64": ref["Nu"], "Nu_end_f64": ref["Nu_end"],
+              "against": "FP64 storage on the device (bit-identical to the oracle over the 200 steps above)"}
+    _progress(f"[config3] Nu f64 = {ref['Nu']:.6f} (mean over the last {window} steps)")
+    assert 4.0 < ref["Nu"] < 6.0
+    # Written bounds: relative error of the mean Nu, L2-relative error of T at the end.  150k
+    # steps are 0.16 thermal diffusion times (H²/α = 930k steps): the plumes are still
+    # developing, so the trajectories of different roundings drift apart.  Plain FP32 storage
+    # keeps g ≈ w_q T in absolute FP32 and loses the per-step temperature increments (its T
+    # field decorrelates); the shifted storages keep them.
+    nu_bound = {"f32": 0.15, "f32s": 2e-3, "f16s": 1e-1}
+    t_bound = {"f32": 1.0, "f32s": 1e-2, "f16s": 1.0}
     for st in ("f32", "f32s", "f16s"):
         e = {q: _err(res[st][q], ref[q]) for q in ("ux", "uy", "rho", "Temp")}
         e["Nu"] = res[st]["Nu"]
         e["Nu_rel_err"] = abs(res[st]["Nu"] - ref["Nu"]) / ref["Nu"]
+        e["Nu_end"] = res[st]["Nu_end"]
         report[st] = e
-        print(f"[config3] {st}: Nu {e['Nu']:.6f} (rel err {e['Nu_rel_err']:.2e}), T max {e['Temp']['max_abs']:.2e}, "
-              f"ux L2rel {e['ux']['l2_rel']:.2e}", flush=True)
-        assert np.all(np.isfinite(res[st]["Temp"]))
+        _progress(f"[config3] {st}: mean Nu {e['Nu']:.6f} (rel err {e['Nu_rel_err']:.2e}), T max {e['Temp']['max_abs']:.2e} "
+                  f"L2rel {e['Temp']['l2_rel']:.2e}, ux L2rel {e['ux']['l2_rel']:.2e}")
     _record("config3_rb512x256", report)
+    for st in ("f32", "f32s", "f16s"):
+        e = report[st]
+        assert np.all(np.isfinite(res[st]["Temp"]))
+        assert e["Nu_rel_err"] < nu_bound[st], (st, e["Nu_rel_err"])
+        assert e["Temp"]["l2_rel"] < t_bound[st], (st, e["Temp"]["l2_rel"])
+
+
+def test_config4_ensemble_of_256_cavities(kmi, backend):
+    """256 D2Q9 MRT lid-driven cavities (side N uniform in {64, 96, …, 512}, Re log-uniform in
+    [100, 3200], seed 2024, u_lid = 0.1), one launch per step over all running cavities, each
+    run to r < 1e-7 (windowed, m = 100, checked every 100 steps).
+
+    - FP64 ensemble, 1000 steps: one cavity of every side, bit for bit against single-lattice
+      runs of the same cavity (themselves bit-identical to the oracle chain).
+    - F32S ensemble (the configuration) to the residual, and the FP64 runs of every cavity
+      with tau < 0.53 for 100k steps: the same cavities diverge in both, all of them at
+      tau < 0.522, where the reference's own scheme has no stable solution
+      (tests/test_oracle_mrt.py)."""
+    n, nu, Re = kmi.ensemble_parameters(256, 2024)
+    tau = 3 * nu + 0.5
+    sample = [int(np.where(n == s)[0][0]) for s in sorted(set(n.tolist()))]
+    _progress(f"[config4] FP64 ensemble, 1000 steps; sample of {len(sample)} cavities (one per side)")
+    with kmi.Batch(n, nu, storage="f64", collision="mrt", backend=backend) as b:
+        b.step(1000)
+        got = {c: b.download(c) for c in sample}
+    for c in sample:
+        with kmi.Lattice(kmi.cavity_desc(int(n[c]), float(nu[c]), 0.1, storage="f64", collision="mrt"), backend) as lat:
+            lat.step(1000)
+            assert np.array_equal(lat.download(), got[c], equal_nan=True), f"cavity {c} (N={n[c]})"
+    del got
+    # the reference's scheme in FP64 on every cavity near the stability margin (tau < 0.53):
+    # 100k steps, no convergence stop, so each one ends diverged or at the cap
+    marginal = np.where(tau < 0.53)[0]
+    _progress(f"[config4] FP64, {len(marginal)} cavities with tau < 0.53, 100k steps")
+    with kmi.Batch(n[marginal], nu[marginal], storage="f64", collision="mrt", backend=backend) as b:
+        b.run(1e-30, 100, 100_000, window=1)
+        st64 = b.status()
+    _progress("[config4] F32S ensemble to r < 1e-7 (cap 600k steps)")
+    with kmi.Batch(n, nu, storage="f32s", collision="mrt", backend=backend) as b:
+        b.autotune(4)
+        b.run(1e-7, 100, 600_000, window=100)
+        st = b.status()
+    counts = {kmi.STATUS_NAMES[k]: int((st["status"] == k).sum()) for k in range(4)}
+    div32 = set(np.where(st["status"] == kmi._lib.CAV_DIVERGED)[0].tolist())
+    div64 = set(marginal[st64["status"] == kmi._lib.CAV_DIVERGED].tolist())
+    conv = st["status"] == kmi._lib.CAV_CONVERGED
+    _progress(f"[config4] F32S status {counts}; diverged {sorted(div32)} (FP64: {sorted(div64)}); "
+              f"converged steps median {np.median(st['steps'][conv]):.0f} max {st['steps'][conv].max()}")
+    _record("config4_cavities256", {"status_f32s": counts, "diverged_f32s": sorted(div32),
+                                     "diverged_f64_marginal_100k_steps": sorted(div64),
+                                     "tau_max_diverged": float(tau[sorted(div32)].max()) if div32 else None,
+                                     "tau_min_converged": float(tau[conv].min()),
+                                     "median_steps_converged": float(np.median(st["steps"][conv]))})
+    assert div32 == div64
+    assert all(tau[c] < 0.522 for c in div32)
+    assert counts["converged"] >= 230
+    assert np.all(st["residual"][conv] < 1e-7)

@@ -340,6 +340,89 @@ def test_three_step_launches_bit_identical(kmi, backend, storage, collision):
                     lat.set_temporal(tile, 1)  # FP64 LDS budget
 
 
+def _walled_3d_cases(kmi, storage, collision, shape):
+    """D3Q19 lattices with walls in the walled multi-step kernels' scope: (name, desc, mask)."""
+    nx, ny, nz = shape
+    walls = {f: {"kind": "wall"} for f in ("west", "east", "south", "north", "bottom")}
+    cav = dict(walls, top={"kind": "zouhe_velocity", "u": (0.08, 0.0, 0.0)})  # run_cavity_3d
+    yield "cavity_zh_lid", kmi.make_desc("d3q19", *shape, 0.02, storage=storage, collision=collision, faces=cav), None
+    ladd = dict(walls, top={"kind": "moving_wall", "u": (0.05, 0.02, 0.0), "rho": 1.0})
+    x, y, z = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    ball = ((x - nx / 3) ** 2 + (y - ny / 2) ** 2 + (z - nz / 2) ** 2 < (min(ny, nz) / 3) ** 2).astype(np.uint8)
+    yield "ladd_lid_ball", kmi.make_desc("d3q19", *shape, 0.03, storage=storage, collision=collision, faces=ladd), ball
+    mixed = {"west": {"kind": "periodic"}, "east": {"kind": "periodic"}, "south": {"kind": "wall"},
+             "north": {"kind": "moving_wall", "u": (0.04, 0.0, 0.01), "rho": 1.0},
+             "bottom": {"kind": "moving_wall", "u": (-0.02, 0.03, 0.0), "rho": 1.0}, "top": {"kind": "wall"}}
+    yield "periodic_x_ladd", kmi.make_desc("d3q19", *shape, 0.02, storage=storage, collision=collision,
+                                           faces=mixed), None
+    chan = {"west": {"kind": "zouhe_velocity", "u": (0.03, 0.0, 0.0)}, "east": {"kind": "zouhe_pressure", "rho": 1.0},
+            "south": {"kind": "wall"}, "north": {"kind": "wall"}, "bottom": {"kind": "periodic"},
+            "top": {"kind": "periodic"}}
+    yield "zh_channel_inert_ball", kmi.make_desc("d3q19", *shape, 0.04, storage=storage, collision=collision,
+                                                 faces=chan, solid_mode="inert"), ball
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32", "f32s"])
+@pytest.mark.parametrize("collision", ["trt", "bgk"])
+def test_walled_multi_step_launches_bit_identical(kmi, backend, storage, collision):
+    """Two- and three-step launches on walled D3Q19 lattices (halfway and moving-wall
+    bounce-back, Zou–He velocity and pressure faces, bounce-back and inert solid masks, mixed
+    with periodic axes) reproduce single-step launches bit for bit: the bounced links of the
+    LDS levels take the cell's own opposite population of the previous level."""
+    for shape in ((64, 16, 12), (32, 8, 5)):
+        for name, desc, mask in _walled_3d_cases(kmi, storage, collision, shape):
+            f = rng_pops(shape, 19, seed=8, amp=0.05)
+            for steps in (37, 65):
+                print(f"[walled] {name} {storage} {collision} shape={shape} steps={steps}", flush=True)
+                with kmi.Lattice(desc, backend) as lat:
+                    if mask is not None:
+                        lat.set_mask(mask)
+                    lat.upload(f)
+                    lat.step(steps)
+                    ref = lat.download()
+                modes = [(14, 1), (14, 3)] + ([(12, 2)] if shape[0] % 64 == 0 else [])
+                if storage != "f64":
+                    modes += [(9, 1), (9, 2), (9, shape[2])]
+                for tile, zc in modes:
+                    with kmi.Lattice(desc, backend) as lat:
+                        if mask is not None:
+                            lat.set_mask(mask)
+                        lat.upload(f)
+                        lat.set_temporal(tile, zc)
+                        lat.step(steps)
+                        got = lat.download()
+                    assert np.array_equal(got, ref), (name, shape, steps, tile, zc,
+                                                      float(np.nanmax(np.abs(got - ref))))
+    # out of the walled scope: the other tiles, F16S, forcing
+    walls = kmi.make_desc("d3q19", 64, 16, 8, 0.02, storage=storage, collision=collision)
+    with kmi.Lattice(walls, backend) as lat:
+        for tile in (1, 3, 7, 11):
+            with pytest.raises(kmi.KrakenError):
+                lat.set_temporal(tile, 1)
+    with kmi.Lattice(kmi.make_desc("d3q19", 64, 16, 8, 0.02, storage="f16s", collision=collision), backend) as lat:
+        with pytest.raises(kmi.KrakenError):
+            lat.set_temporal(12, 1)
+    forced = kmi.make_desc("d3q19", 64, 16, 8, 0.02, storage=storage, forcing="guo", force=(1e-5, 0.0, 0.0))
+    with kmi.Lattice(forced, backend) as lat:
+        with pytest.raises(kmi.KrakenError):
+            lat.set_temporal(14, 1)
+
+
+def test_walled_cavity_autotune_picks_a_multi_step_launch(kmi, backend):
+    """run_cavity_3d's lattice at 128³ (FP32 shifted): the autotuner now offers the walled
+    multi-step launches, and its choice reproduces single-step launches bit for bit."""
+    N = 128
+    desc = kmi.cavity_3d_desc(N, N, N, 0.05, 0.1, storage="f32s")
+    with kmi.Lattice(desc, backend) as lat:
+        lat.step(30)
+        ref = lat.download_raw()
+    with kmi.Lattice(desc, backend) as lat:
+        lat.autotune(32)
+        print(f"[walled] cavity 128^3 tuned launch {lat.launch} temporal {lat.temporal}", flush=True)
+        lat.step(30)
+        assert np.array_equal(lat.download_raw(), ref)
+
+
 def test_two_step_tuned_and_rejected_where_it_does_not_apply(kmi, backend):
     N = 64
     desc = kmi.taylor_green_3d_desc(N, 0.01, "f32")
