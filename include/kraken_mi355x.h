@@ -480,6 +480,20 @@ int kmi_nn_train_epochs(kmi_nn* nn, int32_t epochs, int32_t batch, float lr, uin
 int kmi_gemm_bf16(kmi_ctx* ctx, int32_t M, int32_t N, int32_t K, const float* A, int64_t sam,
                   int64_t sak, const float* B, int64_t sbk, int64_t sbn, int32_t splits, float* C);
 
+/* Bytes of the static device workspace of the allocation-free entry point below (per GPU). */
+#define KMI_NOALLOC_WORKSPACE_BYTES (32u * 1024u * 1024u)
+/* kmi_gemm_bf16 without any device allocation: no counterpart in the reference.  For callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The same kernels and result bits as kmi_gemm_bf16; the
+ * device copies of A and B (FP32 and BF16) and the split-K partials are carved from a static
+ * workspace of KMI_NOALLOC_WORKSPACE_BYTES per GPU, held until the process ends, so the call
+ * allocates and frees nothing.  Problems that need more than the workspace are rejected with
+ * KMI_ERR_ARG (each buffer rounded up to 256 bytes: 6 (na + nb) + 4 splits M N bytes, na and nb
+ * the elements the strides of A and B span).  Every call on one GPU uses the same workspace and
+ * leaves its contents behind; calls that overlap on one GPU (from contexts on different
+ * streams or from different threads) race. */
+int kmi_gemm_bf16_noalloc(kmi_ctx* ctx, int32_t M, int32_t N, int32_t K, const float* A, int64_t sam,
+                          int64_t sak, const float* B, int64_t sbk, int64_t sbn, int32_t splits, float* C);
 
 #ifdef __cplusplus
 }

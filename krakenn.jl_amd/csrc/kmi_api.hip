@@ -96,7 +96,8 @@ int kmi_ctx_create(int device, kmi_ctx** out) {
   c->device = device;
   if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
       (e = hipMalloc(&c->partials, 2048 * 3 * 2 * sizeof(double))) != hipSuccess ||
-      (e = hipMalloc(&c->out, 8 * sizeof(double))) != hipSuccess) {
+      (e = hipMalloc(&c->out, 8 * sizeof(double))) != hipSuccess ||
+      (e = resolve_noalloc_workspace(device)) != hipSuccess) {
     if (c->stream) (void)hipStreamDestroy(c->stream);
     if (c->partials) (void)hipFree(c->partials);
     if (c->out) (void)hipFree(c->out);

@@ -57,6 +57,11 @@ hipError_t launch_reduce_cells(int Q, int st, const void* f, const void* g, cons
 hipError_t launch_drag(int Q, int st, const void* f, long long nlinks, const int* cell,
                        const int* q, const double* qw, const double* uw, int nx, int ny, int nz,
                        long long plane, int mea, double* partials, double* out, hipStream_t s);
+// Static device workspace of kmi_gemm_bf16_noalloc (kmi_nn.hip): its address on `device`,
+// taken with hipGetSymbolAddress while that device is current (once per device, at context
+// creation); nullptr until then.
+hipError_t resolve_noalloc_workspace(int device);
+void* noalloc_workspace(int device);
 hipError_t launch_nusselt(int Q, int st, const void* g, int nx, int ny, int nz, long long plane,
                           double T0, int face, double Th, double dT, int lo1, int hi1, int lo2,
                           int hi2, double* partials, double* out, hipStream_t s);

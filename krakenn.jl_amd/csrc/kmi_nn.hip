@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -664,6 +665,35 @@ __global__ void k_gather(const float* __restrict__ X, const float* __restrict__ 
 
 static inline dim3 g1(long long n) { return dim3(unsigned((n + 255) / 256)); }
 
+// ---- static workspace of the allocation-free entry point ----------------------------------
+// kmi_gemm_bf16_noalloc stages its operands and split-K partials here; each GPU holds its own
+// copy of the array until the process ends.  In host code `scratch` names a host-side stand-in,
+// not a device address: its address on each device comes from hipGetSymbolAddress, taken once
+// per device (with that device current) when a context is created, and only that address is
+// ever handed to kernels or copies.
+static __device__ __align__(256) unsigned char scratch[KMI_NOALLOC_WORKSPACE_BYTES];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_dev[kMaxDevices] = {};
+
+hipError_t resolve_noalloc_workspace(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[device]) return hipSuccess;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch_dev[device] = ptr;
+  return hipSuccess;
+}
+
+void* noalloc_workspace(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_dev[device];
+}
+
 }  // namespace kmi
 
 using namespace kmi;
@@ -1240,6 +1270,32 @@ int kmi_gemm_bf16(kmi_ctx* ctx, int32_t M, int32_t Nn, int32_t K, const float* A
   for (void* p : {(void*)d.dA, (void*)d.dB, (void*)d.bA, (void*)d.bB, (void*)d.P})
     if (p) (void)hipFree(p);
   if (e != hipSuccess) return hip_fail(ctx, e, "kmi_gemm_bf16");
+  return KMI_OK;
+}
+
+// The same product with every device buffer carved from the static workspace of this GPU:
+// no allocation.  Calls on one GPU share the workspace, so calls that overlap on one GPU race.
+int kmi_gemm_bf16_noalloc(kmi_ctx* ctx, int32_t M, int32_t Nn, int32_t K, const float* A, int64_t sam,
+                          int64_t sak, const float* B, int64_t sbk, int64_t sbn, int32_t splits, float* C) {
+  GemmShape g;
+  if (!ctx || !A || !B || !C || !gemm_shape(M, Nn, K, sam, sak, sbk, sbn, splits, g))
+    return fail(ctx, KMI_ERR_ARG, "kmi_gemm_bf16_noalloc: bad arguments");
+  auto up = [](unsigned long long b) { return (b + 255ull) / 256ull * 256ull; };
+  const unsigned long long oA = 0, oB = oA + up(g.na * 4ull), obA = oB + up(g.nb * 4ull),
+                           obB = obA + up(g.na * 2ull), oP = obB + up(g.nb * 2ull),
+                           end = oP + up((unsigned long long)g.sp * M * Nn * 4ull);
+  if (end > KMI_NOALLOC_WORKSPACE_BYTES)
+    return fail(ctx, KMI_ERR_ARG, "kmi_gemm_bf16_noalloc: operands and partials need " + std::to_string(end) +
+                                      " bytes, more than the " + std::to_string(KMI_NOALLOC_WORKSPACE_BYTES) +
+                                      "-byte workspace; use kmi_gemm_bf16");
+  unsigned char* base = static_cast<unsigned char*>(noalloc_workspace(ctx->device));
+  if (!base) return fail(ctx, KMI_ERR_STATE, "kmi_gemm_bf16_noalloc: no workspace address for this device");
+  KMI_HIP(ctx, hipSetDevice(ctx->device));
+  const GemmBufs d{reinterpret_cast<float*>(base + oA), reinterpret_cast<float*>(base + oB),
+                   reinterpret_cast<float*>(base + oP), reinterpret_cast<uint16_t*>(base + obA),
+                   reinterpret_cast<uint16_t*>(base + obB)};
+  hipError_t e = gemm_bf16_run(ctx->stream, M, Nn, K, A, sam, sak, B, sbk, sbn, g, d, C);
+  if (e != hipSuccess) return hip_fail(ctx, e, "kmi_gemm_bf16_noalloc");
   return KMI_OK;
 }
 

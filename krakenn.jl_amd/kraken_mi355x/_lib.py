@@ -164,6 +164,8 @@ SIGNATURES = {
     "kmi_nn_train_epochs": (c_int, [c_void_p, c_int32, c_int32, c_float, c_uint64, POINTER(c_double)]),
     "kmi_gemm_bf16": (c_int, [c_void_p, c_int32, c_int32, c_int32, POINTER(c_float), c_int64, c_int64,
                               POINTER(c_float), c_int64, c_int64, c_int32, POINTER(c_float)]),
+    "kmi_gemm_bf16_noalloc": (c_int, [c_void_p, c_int32, c_int32, c_int32, POINTER(c_float), c_int64, c_int64,
+                              POINTER(c_float), c_int64, c_int64, c_int32, POINTER(c_float)]),
 }
 
 _lib = None

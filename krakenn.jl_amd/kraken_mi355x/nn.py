@@ -110,8 +110,10 @@ class WarmStartNet:
         return loss.value
 
 
-def gemm_bf16(A, B, splits=1, backend=None):
-    """C = A·B (BF16 products on MFMA, FP32 accumulation); A, B are 2-D arrays (any strides)."""
+def gemm_bf16(A, B, splits=1, backend=None, noalloc=False):
+    """C = A·B (BF16 products on MFMA, FP32 accumulation); A, B are 2-D arrays (any strides).
+    noalloc: kmi_gemm_bf16_noalloc, which stages everything in the library's static device
+    workspace (KMI_NOALLOC_WORKSPACE_BYTES per GPU) and allocates nothing."""
     backend = backend or MI355XBackend()
     A = np.asarray(A, dtype=np.float32)
     B = np.asarray(B, dtype=np.float32)
@@ -128,6 +130,6 @@ def gemm_bf16(A, B, splits=1, backend=None):
     C = np.empty((M, N), dtype=np.float32)
     pa = A.ctypes.data_as(POINTER(c_float))
     pb = B.ctypes.data_as(POINTER(c_float))
-    L.check(L.load().kmi_gemm_bf16(backend.ctx, M, N, K, pa, sa[0], sa[1], pb, sb[0], sb[1], int(splits), _fp(C)),
-            backend.ctx)
+    fn = L.load().kmi_gemm_bf16_noalloc if noalloc else L.load().kmi_gemm_bf16
+    L.check(fn(backend.ctx, M, N, K, pa, sa[0], sa[1], pb, sb[0], sb[1], int(splits), _fp(C)), backend.ctx)
     return C
