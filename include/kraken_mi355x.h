@@ -311,7 +311,11 @@ int kmi_step_residual_window(kmi_lattice* lat, int64_t nsteps, int64_t window, d
 int kmi_step_timed(kmi_lattice* lat, int64_t nsteps, double* ms);
 /* Moments of the stored (post-collision) populations, as compute_macroscopic_2d/3d and
  * compute_temperature_2d (macroscopic.jl:5-80, thermal_2d.jl:113-138).  Any pointer may
- * be NULL.  Host arrays of nx*ny*nz doubles. */
+ * be NULL.  Host arrays of nx*ny*nz doubles.  The reference's fused kernels
+ * (fused_bgk_step!, fused_trt_step!, fused_bgk_2d.jl:26-56, fused_trt_2d.jl:53-66) write the
+ * pre-collision moments instead, with rho = 1, u = 0 at solid cells: collisions conserve
+ * rho and rho u, so at fluid cells the two differ by rounding only; the solid-cell values
+ * are the caller's to set (the host runner does, for its TRT path). */
 int kmi_moments(kmi_lattice* lat, double* rho, double* ux, double* uy, double* uz,
                 double* T);
 int kmi_reduce(kmi_lattice* lat, int32_t kind, int32_t arg, double* out);

@@ -134,6 +134,39 @@ struct DenseS {
   }
 };
 
+// The row-major matrix of LA with row M0 replaced by ones (BF16 1.0) for k < nk (LA reads
+// zeros there): a weight-gradient product Σ_k A(m, k) B(k, n) then also yields
+// Σ_k B(k, n) — the bias gradient — in its row M0.
+template <class LA>
+struct WithOnes {
+  static_assert(!LA::kmajor, "rows of eight");
+  static constexpr bool kmajor = false;
+  LA la;
+  int M0, nk;
+  struct Cur {
+    typename LA::Cur c;
+    int row, k;
+  };
+  KMI_DEV void init(Cur& u, int row, int k) const {
+    u.row = row;
+    u.k = k;
+    la.init(u.c, row, k);
+  }
+  KMI_DEV uint4 load(const Cur& u) const {
+    uint4 v = la.load(u.c);
+    const int e = M0 - u.row;
+    if (e >= 0 && e < 8 && u.k < nk) {
+      uint32_t* w = reinterpret_cast<uint32_t*>(&v);
+      w[e >> 1] |= 0x3F80u << (16 * (e & 1));
+    }
+    return v;
+  }
+  KMI_DEV void next(Cur& u) const {
+    la.next(u.c);
+    u.k += 32;
+  }
+};
+
 // Convolution after nearest ×2 up-sampling (3×3, zero padding 1) on an NHWC input
 // [B, Hi, Hi, C] (C a multiple of 8): row m = output pixel (b, y, x) of the Ho = 2 Hi grid,
 // k = (ky, kx, c) = tap·C + c, element in[b, (y+ky−1)/2, (x+kx−1)/2, c].
@@ -534,6 +567,21 @@ __global__ void k_act_grad(const float* __restrict__ dH, const float* __restrict
   if (dZb) dZb[i] = f2bf(g);
 }
 
+// Σ over `splits` partials (fixed order), then dZ = Σ ⊙ SiLU'(Z) (or Σ); FP32 and BF16 copies.
+__global__ void k_split_act(const float* __restrict__ P, int splits, long long n, const float* __restrict__ Z,
+                            int silu, float* __restrict__ dZ, uint16_t* __restrict__ dZb) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float g = 0.f;
+  for (int s = 0; s < splits; ++s) g += P[s * n + i];
+  if (silu) {
+    const float z = Z[i], sg = 1.f / (1.f + expf(-z));
+    g = g * (sg * (1.f + z * (1.f - sg)));
+  }
+  dZ[i] = g;
+  dZb[i] = f2bf(g);
+}
+
 // Column sums of X[rows, N] (bias gradients): partial over row chunks, then fixed-order sum.
 // Wide rows (N > 128): one thread per column walks the chunk.  Narrow rows: the 256 threads
 // of a group split into G = 256 / N row lanes per column (consecutive threads read
@@ -727,10 +775,12 @@ struct kmi_nn {
   float* colpart = nullptr;
   double* lpart = nullptr;
   double* lout = nullptr;
-  // device dataset
+  // device dataset, the epoch's permutation and the per-step losses of an epoch
   float *dsX = nullptr, *dsY = nullptr;
   int dsN = 0;
   int* idx = nullptr;
+  int* perm_d = nullptr;
+  double* lhist = nullptr;
 };
 
 static long long out_elems(const kmi_nn* N, int B) { return (long long)B * N->H3 * N->H3 * N->d.out_c; }
@@ -762,11 +812,12 @@ static int nn_alloc(kmi_nn* N) {
   const long long a5 = out_elems(N, B), a5p = (long long)B * N->H3 * N->H3 * N->Cp3;
   const long long big = std::max({a1, a2, a3, a4, a5});
   // P holds one split of every product it receives (pick_splits adds splits only while they
-  // fit): the weight gradients of the dense layers (in_dim × hidden, hidden × L) and of the
-  // three convolutions (9 Ci × Co), and dH1 (B × hidden).
-  const long long wmax = std::max({(long long)d.in_dim * d.hidden, (long long)d.hidden * N->L,
-                                   9LL * d.latent_c * d.c1, 9LL * d.c1 * d.c2, 9LL * d.c2 * d.out_c, a1});
-  N->P_elems = size_t(std::max({wmax, 16LL * 1024 * 1024 / 4}));
+  // fit): the weight gradients with their bias row ((9 Ci + 1) × Co, (hidden + 1) × L,
+  // in_dim × hidden), the data gradients (B·Hi²·Ci, at most `big`) and dH1 (B × hidden).
+  const long long wmax = std::max({(long long)d.in_dim * d.hidden, (long long)(d.hidden + 1) * N->L,
+                                   (9LL * d.latent_c + 1) * d.c1, (9LL * d.c1 + 1) * d.c2,
+                                   (9LL * d.c2 + 1) * d.out_c, a1, big});
+  N->P_elems = size_t(std::max({wmax, 16LL * 1024 * 1024}));
   hipError_t e = hipSuccess;
   auto al = [&](void** p, size_t bytes) { if (e == hipSuccess) e = hipMalloc(p, std::max<size_t>(bytes, 16)); };
   al((void**)&N->w, o * 4);
@@ -805,11 +856,13 @@ static int nn_alloc(kmi_nn* N) {
   return KMI_OK;
 }
 
-// Splits of a product with a long reduction (K) and a small output on BM × BN tiles: about
-// 512 work-groups, at least 512 of K per split, within P's size.
-static int pick_splits(const kmi_nn* N, int BM, int BN, long long M, long long Nn, long long K) {
+// Splits of a product on BM × BN tiles: about `groups` work-groups (enough resident groups
+// per CU to hide the operand latency of the K loop), at least `kmin` of K per split, within
+// P's size.
+static int pick_splits(const kmi_nn* N, int BM, int BN, long long M, long long Nn, long long K, long long groups,
+                       long long kmin) {
   const long long tiles = ((M + BM - 1) / BM) * ((Nn + BN - 1) / BN);
-  long long s = std::max<long long>(1, std::min<long long>((512 + tiles - 1) / tiles, K / 512));
+  long long s = std::max<long long>(1, std::min<long long>((groups + tiles - 1) / tiles, K / kmin));
   s = std::min<long long>(s, (long long)N->P_elems / std::max<long long>(1, M * Nn));
   return int(std::max<long long>(1, s));
 }
@@ -918,8 +971,7 @@ static int nn_backward(kmi_nn* N, int B) {
   Stage st[3];
   stages(N, st);
   const float* zin[3] = {N->z2, N->z3, N->z4};  // pre-activations of the stage inputs (SiLU)
-  // gradient w.r.t. the stage output pre-activation: FP32 [M, Co] and BF16 [M, Cp]
-  const float* dz = N->dy;
+  // gradient w.r.t. the stage output pre-activation in BF16 [M, Cp]
   const long long rows = (long long)B * N->H3 * N->H3;
   hipLaunchKernelGGL(k_pad_bf16, g1(rows * N->Cp3), dim3(256), 0, s, N->dy, rows, d.out_c, N->Cp3, N->dyb);
   const uint16_t* dzb = N->dyb;
@@ -927,36 +979,51 @@ static int nn_backward(kmi_nn* N, int B) {
   for (int k = 2; k >= 0; --k) {
     const Stage& S = st[k];
     const int Ho = 2 * S.Hi, Mo = B * Ho * Ho;
-    // bias gradient
-    if (int rc = colsum(N, dz, Mo, S.Co, g + S.ob, s)) return rc;
-    // weight gradient dW[(ky,kx,c), o] = Σ_pixels A(pixel, (ky,kx,c)) dZ[pixel, o]
-    const int Mw = 9 * S.Ci;
-    const int sp = pick_splits(N, 64, S.Co > 32 ? 64 : (S.Co > 16 ? 32 : 16), Mw, S.Co, Mo);
-    NN_CHECK(mm_wgrad(Mw, S.Co, Mo, sp, ConvWgradA{S.in, B, S.Hi, S.Ci}, DenseR{dzb, S.Cp, S.Cp, Mo},
-                      EpiStore{N->P}, s));
+    // weight gradient dW[(ky,kx,c), o] = Σ_pixels A(pixel, (ky,kx,c)) dZ[pixel, o], with a row
+    // of ones below A: its result row is the bias gradient Σ_pixels dZ[pixel, o], which the
+    // flat layout stores right after the weights (g + ob = g + ow + 9 Ci Co)
+    const int Mw = 9 * S.Ci + 1;
+    const int sp = pick_splits(N, 64, S.Co > 32 ? 64 : (S.Co > 16 ? 32 : 16), Mw, S.Co, Mo, 2048, 512);
+    NN_CHECK(mm_wgrad(Mw, S.Co, Mo, sp, WithOnes<ConvWgradA>{ConvWgradA{S.in, B, S.Hi, S.Ci}, 9 * S.Ci, Mo},
+                      DenseR{dzb, S.Cp, S.Cp, Mo}, EpiStore{N->P}, s));
     NN_CHECK(split_sum(N->P, Mw, S.Co, Mo, sp, g + S.ow, s));
     // data gradient into the stage input (per input pixel, up-sampling folded in), times
-    // the SiLU derivative of the input's pre-activation
-    const int Mi = B * S.Hi * S.Hi;
-    NN_CHECK(mm_by_n(Mi, S.Ci, 36 * S.Cp, 1, ConvDgradA{dzb, B, S.Hi, S.Cp}, ConvDgradB{S.wbf, S.Ci, S.Cp},
-                     EpiActGrad{zin[k], N->dz[pp], N->dzb[pp], 1}, s));
-    dz = N->dz[pp];
+    // the SiLU derivative of the input's pre-activation; split over K when the tiles alone
+    // leave the device idle
+    const int Mi = B * S.Hi * S.Hi, Kd = 36 * S.Cp;
+    const int spd = pick_splits(N, S.Ci > 16 ? 128 : 256, S.Ci > 32 ? 64 : (S.Ci > 16 ? 32 : 16), Mi, S.Ci, Kd,
+                                1024, 256);
+    if (spd == 1) {
+      NN_CHECK(mm_by_n(Mi, S.Ci, Kd, 1, ConvDgradA{dzb, B, S.Hi, S.Cp}, ConvDgradB{S.wbf, S.Ci, S.Cp},
+                       EpiActGrad{zin[k], N->dz[pp], N->dzb[pp], 1}, s));
+    } else {
+      NN_CHECK(mm_by_n(Mi, S.Ci, Kd, spd, ConvDgradA{dzb, B, S.Hi, S.Cp}, ConvDgradB{S.wbf, S.Ci, S.Cp},
+                       EpiStore{N->P}, s));
+      int sp2 = 1;
+      split_chunk(Kd, spd, &sp2);
+      const long long ni = (long long)Mi * S.Ci;
+      hipLaunchKernelGGL(k_split_act, g1(ni), dim3(256), 0, s, N->P, sp2, ni, zin[k], 1, N->dz[pp], N->dzb[pp]);
+      NN_CHECK(hipGetLastError());
+    }
     dzb = N->dzb[pp];
     pp ^= 1;
   }
-  // dense 2: dZ2 is in dz/dzb as [B, L]
-  if (int rc = colsum(N, dz, B, N->L, g + N->ob2, s)) return rc;
-  NN_CHECK((mm<64, 64, 2>(d.hidden, N->L, B, 1, DenseR{N->h1b, d.hidden, d.hidden, B}, DenseR{dzb, N->L, N->L, B},
-                          EpiStore{g + N->oW2}, s)));
+  // dense 2: dZ2 is in dzb as [B, L]; weight and bias gradients (a row of ones below
+  // h1ᵀ; b2 follows W2 in the flat layout)
+  NN_CHECK((mm<64, 64, 2>(d.hidden + 1, N->L, B, 1,
+                          WithOnes<DenseR>{DenseR{N->h1b, d.hidden, d.hidden, B}, d.hidden, B},
+                          DenseR{dzb, N->L, N->L, B}, EpiStore{g + N->oW2}, s)));
   // dH1 = dZ2 W2ᵀ (long reduction over the latent: split-K)
   {
-    const int sp = pick_splits(N, 64, 64, B, d.hidden, N->L);
+    const int sp = pick_splits(N, 64, 64, B, d.hidden, N->L, 512, 512);
     NN_CHECK((mm<64, 64, 2>(B, d.hidden, N->L, sp, DenseK{dzb, N->L, B, N->L}, DenseK{wb + N->oW2, N->L, d.hidden, N->L},
                             EpiStore{N->P}, s)));
-    NN_CHECK(split_sum(N->P, B, d.hidden, N->L, sp, N->dh, s));
+    int sp2 = 1;
+    split_chunk(N->L, sp, &sp2);
+    const long long nh = (long long)B * d.hidden;
+    hipLaunchKernelGGL(k_split_act, g1(nh), dim3(256), 0, s, N->P, sp2, nh, N->z1, 1, N->dz[pp], N->dzb[pp]);
+    NN_CHECK(hipGetLastError());
   }
-  const long long nh = (long long)B * d.hidden;
-  hipLaunchKernelGGL(k_act_grad, g1(nh), dim3(256), 0, s, N->dh, N->z1, nh, 1, N->dz[pp], N->dzb[pp]);
   // dense 1
   if (int rc = colsum(N, N->dz[pp], B, d.hidden, g + N->ob1, s)) return rc;
   NN_CHECK((mm<64, 64, 2>(d.in_dim, d.hidden, B, 1, DenseR{N->xb, d.in_dim, d.in_dim, B},
@@ -964,15 +1031,18 @@ static int nn_backward(kmi_nn* N, int B) {
   return KMI_OK;
 }
 
-static int nn_loss(kmi_nn* N, int B, double* loss) {
+// MSE loss and its gradient dY; the loss lands in `slot` (device; default lout) and, when
+// `loss` is given, is read back.
+static int nn_loss(kmi_nn* N, int B, double* loss, double* slot = nullptr) {
   hipStream_t s = N->ctx->stream;
   const long long n = out_elems(N, B);
   const int nb = int(std::min<long long>(1024, (n + 255) / 256));
+  if (!slot) slot = N->lout;
   hipLaunchKernelGGL(k_mse, dim3(unsigned(nb)), dim3(256), 0, s, N->y, N->tgt, n, N->dy, N->lpart);
-  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, s, N->lpart, nb, N->lout);
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, s, N->lpart, nb, slot);
   NN_CHECK(hipGetLastError());
   if (loss) {
-    NN_CHECK(hipMemcpyAsync(loss, N->lout, 8, hipMemcpyDeviceToHost, s));
+    NN_CHECK(hipMemcpyAsync(loss, slot, 8, hipMemcpyDeviceToHost, s));
     NN_CHECK(hipStreamSynchronize(s));
   }
   return KMI_OK;
@@ -1061,7 +1131,8 @@ int kmi_nn_destroy(kmi_nn* N) {
   (void)hipStreamSynchronize(N->ctx->stream);
   void* ptrs[] = {N->w, N->g, N->m, N->v, N->wb, N->w3p, N->x, N->xb, N->tgt, N->z1, N->h1b, N->z2, N->h2b,
                   N->z3, N->h3b, N->z4, N->h4b, N->y, N->dy, N->dyb, N->dh, N->dz[0], N->dz[1], N->dzb[0],
-                  N->dzb[1], N->P, N->colpart, N->lpart, N->lout, N->idx, N->dsX, N->dsY};
+                  N->dzb[1], N->P, N->colpart, N->lpart, N->lout, N->idx, N->dsX, N->dsY, N->perm_d,
+                  N->lhist};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete N;
@@ -1138,13 +1209,17 @@ int kmi_nn_set_dataset(kmi_nn* N, int32_t n, const float* x, const float* y) {
   if (!N || !x || !y || n < 1) return fail(nullptr, KMI_ERR_ARG, "kmi_nn_set_dataset: bad arguments");
   kmi_ctx* c = N->ctx;
   KMI_HIP(c, hipSetDevice(c->device));
-  if (N->dsX) (void)hipFree(N->dsX);
-  if (N->dsY) (void)hipFree(N->dsY);
-  N->dsX = N->dsY = nullptr;
+  KMI_HIP(c, hipStreamSynchronize(c->stream));
+  for (void** p : {(void**)&N->dsX, (void**)&N->dsY, (void**)&N->perm_d, (void**)&N->lhist}) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
   N->dsN = 0;
   const long long dout = out_elems(N, 1);
   KMI_HIP(c, hipMalloc(&N->dsX, (size_t)n * N->d.in_dim * 4));
   KMI_HIP(c, hipMalloc(&N->dsY, (size_t)n * dout * 4));
+  KMI_HIP(c, hipMalloc(&N->perm_d, (size_t)n * 4));
+  KMI_HIP(c, hipMalloc(&N->lhist, (size_t)n * 8));
   KMI_HIP(c, hipMemcpy(N->dsX, x, (size_t)n * N->d.in_dim * 4, hipMemcpyHostToDevice));
   KMI_HIP(c, hipMemcpy(N->dsY, y, (size_t)n * dout * 4, hipMemcpyHostToDevice));
   N->dsN = n;
@@ -1153,7 +1228,9 @@ int kmi_nn_set_dataset(kmi_nn* N, int32_t n, const float* x, const float* y) {
 
 // Epochs of Adam over the device dataset in mini-batches of B (a Fisher–Yates shuffle per
 // epoch from `seed`; the last partial batch is dropped).  Returns the mean loss of the
-// final epoch.
+// final epoch.  An epoch's permutation is uploaded once and the training steps queue on the
+// stream without host round trips; each step's loss stays on the device until the epoch's
+// end.
 int kmi_nn_train_epochs(kmi_nn* N, int32_t epochs, int32_t B, float lr, uint64_t seed, double* mean_loss) {
   if (!N) return fail(nullptr, KMI_ERR_ARG, "kmi_nn_train_epochs: NULL argument");
   kmi_ctx* c = N->ctx;
@@ -1164,29 +1241,29 @@ int kmi_nn_train_epochs(kmi_nn* N, int32_t epochs, int32_t B, float lr, uint64_t
   const long long dout = out_elems(N, 1);
   std::vector<int> perm(N->dsN);
   uint64_t s = seed;
-  double last = 0.0;
+  const int nb = N->dsN / B;
   for (int ep = 0; ep < epochs; ++ep) {
     for (int i = 0; i < N->dsN; ++i) perm[i] = i;
     for (int i = N->dsN - 1; i > 0; --i) std::swap(perm[i], perm[int(splitmix(s) % uint64_t(i + 1))]);
-    const int nb = N->dsN / B;
-    double acc = 0.0;
+    // the previous epoch's gathers have read the permutation before it is replaced
+    KMI_HIP(c, hipStreamSynchronize(c->stream));
+    KMI_HIP(c, hipMemcpyAsync(N->perm_d, perm.data(), (size_t)N->dsN * 4, hipMemcpyHostToDevice, c->stream));
     for (int b = 0; b < nb; ++b) {
-      KMI_HIP(c, hipMemcpyAsync(N->idx, perm.data() + (size_t)b * B, (size_t)B * 4, hipMemcpyHostToDevice, c->stream));
-      hipLaunchKernelGGL(k_gather, g1((long long)B * dout), dim3(256), 0, c->stream, N->dsX, N->dsY, N->idx, B,
-                         N->d.in_dim, dout, N->x, N->tgt);
+      hipLaunchKernelGGL(k_gather, g1((long long)B * dout), dim3(256), 0, c->stream, N->dsX, N->dsY,
+                         N->perm_d + (size_t)b * B, B, N->d.in_dim, dout, N->x, N->tgt);
       KMI_HIP(c, hipGetLastError());
       if (int rc = nn_forward(N, B)) return rc;
-      double l = 0.0;
-      if (int rc = nn_loss(N, B, ep == epochs - 1 ? &l : nullptr)) return rc;
+      if (int rc = nn_loss(N, B, nullptr, N->lhist + b)) return rc;
       if (int rc = nn_backward(N, B)) return rc;
       if (int rc = nn_adam(N, lr)) return rc;
-      acc += l;
-      if (ep != epochs - 1) KMI_HIP(c, hipStreamSynchronize(c->stream));  // perm buffer reuse
     }
-    last = acc / nb;
   }
+  std::vector<double> lh(nb);
+  KMI_HIP(c, hipMemcpyAsync(lh.data(), N->lhist, (size_t)nb * 8, hipMemcpyDeviceToHost, c->stream));
   KMI_HIP(c, hipStreamSynchronize(c->stream));
-  if (mean_loss) *mean_loss = last;
+  double acc = 0.0;
+  for (int b = 0; b < nb; ++b) acc += lh[b];
+  if (mean_loss) *mean_loss = epochs > 0 ? acc / nb : 0.0;
   return KMI_OK;
 }
 

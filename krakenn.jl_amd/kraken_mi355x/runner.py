@@ -268,6 +268,20 @@ def _apply(lat, updates):
             lat.set_face(face, v)
 
 
+def _fields(lat, plan, setup):
+    """The moment planes the reference's loop hands back.  The default path computes them
+    from the post-collision populations (compute_macroscopic_2d!, simulation_runner.jl:404-408),
+    which is kmi_moments.  With collision = trt the fused kernel writes its pre-collision
+    moments instead, and ρ = 1, u = 0 at solid cells (fused_trt_2d.jl:53-66); a collision
+    conserves ρ and ρu, so at fluid cells the two differ by rounding only, and the solid
+    cells are set here."""
+    m = lat.moments()
+    if str(setup.collision).lstrip(":").lower() == "trt" and plan.mask is not None:
+        solid = np.asarray(plan.mask, dtype=bool)
+        m["rho"][solid], m["ux"][solid], m["uy"][solid] = 1.0, 0.0, 0.0
+    return m
+
+
 def run_simulation(setup: SimulationSetup, *, backend=None, T=np.float64,
                    callback: Callable | None = None, callback_every: int = 100):
     """Default single-phase D2Q9 path of run_simulation (simulation_runner.jl:279-443).
@@ -294,7 +308,7 @@ def run_simulation(setup: SimulationSetup, *, backend=None, T=np.float64,
             lat.step(n)
             step += n
             if callback is not None and step % callback_every == 0:
-                m = lat.moments()
+                m = _fields(lat, plan, setup)
                 callback(step, {"rho": m["rho"], "ux": m["ux"], "uy": m["uy"]})
-        m = lat.moments()
+        m = _fields(lat, plan, setup)
     return {"ρ": m["rho"], "ux": m["ux"], "uy": m["uy"], "setup": setup}

@@ -121,14 +121,14 @@ def loss_grad(arch, flat, x, t, bf=True):
     dz = dy
     for name, bname, cols, Hi, Ci, Co, zin in reversed(stages):
         dzb = r(dz)
-        g[bname] = dz.sum(axis=0)
+        g[bname] = dzb.sum(axis=0)  # the GPU sums the BF16 copy (a row of ones in the product)
         g[name] = c[cols].T @ dzb
         dcols = dzb @ r(p[name]).T
         dh = col2im_up(dcols, B, Hi, Ci).reshape(-1, Ci) if name != "C1" else col2im_up(dcols, B, Hi, Ci).reshape(B, -1)
         dz = dh * dsilu(zin.reshape(dh.shape))
     # dense 2 (dz = dZ2, [B, L])
     dzb = r(dz)
-    g["b2"] = dz.sum(axis=0)
+    g["b2"] = dzb.sum(axis=0)
     g["W2"] = c["h1b"].T @ dzb
     dh1 = dzb @ r(p["W2"]).T
     dz1 = dh1 * dsilu(c["z1"])

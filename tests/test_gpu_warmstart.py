@@ -1,42 +1,42 @@
-"""The learned warm start gains over rest initialisation and leaves the converged field
-unchanged (reduced size: 64² cavities).
+"""The learned warm start at its configured size: 2048 steady 128² MRT cavities from the
+batched solver train the decoder on the GPU; 256 held-out cases run to the residual 1e-7
+warm-started in one batch, against the same batch from rest and against rest-initialised
+plain single-lattice runs; FP64 runs of both starts to r < 1e-12 compare the converged
+fields.  Re log-uniform on [100, 1000], lid amplitude uniform on [0.06, 0.12] (tau >= 0.523,
+inside the reference scheme's stability range, tests/test_oracle_mrt.py)."""
+import json
+import os
 
-The full-size experiment (128² cavities, 2048 training states, 256 held-out cases) is
-`bench.py --workload warmstart`; this test runs the same pipeline small enough for the
-suite and asserts that the trained network cuts the steps to the residual."""
-import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
+OUT = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpurun_out")
 
-def test_warm_start_beats_rest_init(kmi, backend):
+
+def test_config5_warm_start_at_full_size(kmi, backend):
     from kraken_mi355x import warmstart as ws
 
-    N = 64
-    Re_tr, A_tr = ws.draw_cases(96, seed=1, Re_range=(100, 800))
-    data = ws.generate_steady_states(Re_tr, A_tr, N=N, tol=1e-6, max_steps=400_000, backend=backend,
-                                     log=lambda s: print(s, flush=True))
-    assert len(data["index"]) >= 80, data["status"]
-    net, loss = ws.train(data["x"], data["y"], epochs=200, batch=16, lr=3e-3, backend=backend,
-                         log=lambda s: print(s, flush=True))
-    assert loss < 2e-2, loss
-    Re_te, A_te = ws.draw_cases(16, seed=99, Re_range=(100, 800))
-    rest = ws.time_to_residual(Re_te, A_te, N=N, tol=1e-7, backend=backend)
-    warm = ws.time_to_residual(Re_te, A_te, N=N, net=net, tol=1e-7, backend=backend)
-    ok = (rest["status"] == 1) & (warm["status"] == 1)
-    assert ok.sum() >= 14
-    ratio = np.median(rest["steps"][ok] / warm["steps"][ok])
-    du = max(float(np.max(np.abs(warm["fields"][c]["ux"] - rest["fields"][c]["ux"])) / A_te[c]) for c in np.where(ok)[0])
-    print(f"[warm] median step ratio {ratio:.2f}, wall rest {rest['wall_s']:.2f}s warm {warm['wall_s']:.2f}s, "
-          f"max |Δu|/A between converged fields {du:.2e}", flush=True)
-    assert ratio > 1.5
-    assert du < 1e-2
-    # Driven in FP64 to r < 1e-11, the warm and rest starts reach the same field: the
-    # residual stop rule, not the start, sets the 1e-7 runs' disagreement above.
-    pick = np.where(ok)[0][[0, -1]]
-    agree = ws.converged_agreement(Re_te[pick], A_te[pick], net, N=N, tol=1e-11, backend=backend)
-    print(f"[warm] FP64 r<1e-11: max |Δu|/A {agree['dU_over_A']}, steps rest {agree['rest_steps']} "
-          f"warm {agree['warm_steps']}", flush=True)
-    assert len(agree["index"]) == 2
-    assert agree["dU_over_A"].max() < 1e-6
+    rec = ws.experiment(n_train=2048, n_test=256, epochs=150, target_tol=1e-8, tol=1e-7, plain_sample=16,
+                        agree_sample=8, agree_tol=1e-12, backend=backend, log=lambda m: print(m, flush=True))
+    os.makedirs(OUT, exist_ok=True)
+    with open(os.path.join(OUT, "warmstart_test.json"), "w") as fh:
+        json.dump(rec, fh, indent=1)
+    print(json.dumps({k: rec[k] for k in ("value", "batched_rest_over_warm", "median_step_ratio_rest_over_warm",
+                                          "prediction_rel_L2_median", "fp64_agreement")}), flush=True)
+    # every training case has a steady state and every held-out case converges from both starts
+    assert rec["train"]["steady_states"] == 2048
+    assert rec["status_rest"][1] == 256 and rec["status_warm"][1] == 256
+    # the configuration's target: warm-started batched runs reach r < 1e-7 in at least 3x less
+    # wall time than rest-initialised plain runs (measured 25.6x)
+    assert rec["value"] >= 3.0
+    # what the network itself gains over the same batch started at rest (measured 1.51x)
+    assert rec["batched_rest_over_warm"] > 1.2
+    assert rec["prediction_rel_L2_median"] < 0.01
+    # the warm start leaves the answer unchanged: driven in FP64 to r < 1e-12 the two starts'
+    # fields agree to the stop rule's own spread (r·tau from the steady state; measured
+    # median 1.4e-7, max 1.7e-6 of the lid amplitude)
+    agree = rec["fp64_agreement"]
+    assert agree["converged_both"] >= agree["cases"] - 1
+    assert agree["median_dU_over_A"] < 1e-6
+    assert agree["max_dU_over_A"] < 5e-6
