@@ -333,9 +333,15 @@ struct ConvDgradB {
 };
 
 // ---- epilogues: element (m, n) of the block's result ------------------------------------------
+// In two parts: pre(N, m, n) loads what the element needs from memory (a bias, a
+// pre-activation), and operator() combines it with the product value and stores.  The kernels
+// call pre() for all of a thread's elements before any store: the stores may alias the inputs
+// as far as the compiler can tell, so interleaving the two would serialise one global-memory
+// round trip per element.
 struct EpiStore {  // P[(z·M + m)·N + n] (split-K partials, or the result itself with one split)
   float* P;
-  KMI_DEV void operator()(int M, int N, int z, int m, int n, float v) const {
+  KMI_DEV float pre(int, int, int) const { return 0.f; }
+  KMI_DEV void operator()(int M, int N, int z, int m, int n, float v, float) const {
     P[((long long)z * M + m) * N + n] = v;
   }
 };
@@ -345,9 +351,10 @@ struct EpiBiasAct {  // z = v + bias; Z = z; h = SiLU(z) or z; H (FP32) and Hb (
   float* H;
   uint16_t* Hb;
   int silu;
-  KMI_DEV void operator()(int, int N, int, int m, int n, float v) const {
+  KMI_DEV float pre(int, int, int n) const { return bias[n]; }
+  KMI_DEV void operator()(int, int N, int, int m, int n, float v, float b) const {
     const long long i = (long long)m * N + n;
-    const float z = v + bias[n];
+    const float z = v + b;
     if (Z) Z[i] = z;
     const float h = silu ? z / (1.f + expf(-z)) : z;
     if (H) H[i] = h;
@@ -359,11 +366,12 @@ struct EpiActGrad {  // g = v ⊙ SiLU'(zin) (or v); FP32 and BF16 copies
   float* dz;
   uint16_t* dzb;
   int silu;
-  KMI_DEV void operator()(int, int N, int, int m, int n, float v) const {
+  KMI_DEV float pre(int N, int m, int n) const { return silu ? zin[(long long)m * N + n] : 0.f; }
+  KMI_DEV void operator()(int, int N, int, int m, int n, float v, float z) const {
     const long long i = (long long)m * N + n;
     float g = v;
     if (silu) {
-      const float z = zin[i], s = 1.f / (1.f + expf(-z));
+      const float s = 1.f / (1.f + expf(-z));
       g = g * (s * (1.f + z * (1.f - s)));
     }
     dz[i] = g;
@@ -486,6 +494,7 @@ __global__ void __launch_bounds__(256) k_mm(const int M, const int N, const int 
     __syncthreads();
     buf ^= 1;
   }
+  float pre[RM][RN][4];
 #pragma unroll
   for (int a = 0; a < RM; ++a)
 #pragma unroll
@@ -493,7 +502,16 @@ __global__ void __launch_bounds__(256) k_mm(const int M, const int N, const int 
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int m = m0 + wm * TM + a * 16 + ((lane >> 4) << 2) + i, n = n0 + wn * TN + b * 16 + fr;
-        if (m < M && n < N) epi(M, N, blockIdx.z, m, n, acc[a][b][i]);
+        pre[a][b][i] = m < M && n < N ? epi.pre(N, m, n) : 0.f;
+      }
+#pragma unroll
+  for (int a = 0; a < RM; ++a)
+#pragma unroll
+    for (int b = 0; b < RN; ++b)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int m = m0 + wm * TM + a * 16 + ((lane >> 4) << 2) + i, n = n0 + wn * TN + b * 16 + fr;
+        if (m < M && n < N) epi(M, N, blockIdx.z, m, n, acc[a][b][i], pre[a][b][i]);
       }
 }
 
@@ -513,6 +531,344 @@ static hipError_t mm(int M, int N, int K, int splits, const LA& la, const LB& lb
   const dim3 g(unsigned((M + BM - 1) / BM), unsigned((N + BN - 1) / BN), unsigned(sp));
   hipLaunchKernelGGL((k_mm<BM, BN, WM, LA, LB, EPI>), g, dim3(256), 0, s, M, N, K, kchunk, la, lb, epi);
   return hipGetLastError();
+}
+
+// ---- direct convolutions ----------------------------------------------------------------------
+// The three products of a decoder stage (forward, data gradient, weight gradient) as direct
+// convolutions: a work-group stages the input patch its tile reads, zero halo included, and
+// the weights in LDS once, so each input element crosses L2 about once per tile rather than
+// once per tap (the im2col loaders of k_mm), and the K loop runs from LDS with no global loads
+// and no barriers.  They cover 32 or 64 input channels and up to 64 output channels (the
+// weight gradient also needs a 16-, 32- or 64-pixel input grid); other shapes take k_mm.
+
+// Row stride (BF16 elements) for n elements per row such that 16 lanes reading 16-byte (or
+// 4-byte) fragments from 16 consecutive rows hit 16 distinct groups of four LDS banks: a
+// multiple of 8 elements whose count of 8-element units is odd.
+constexpr int lds_stride(int n) {
+  const int s = (n + 7) / 8 * 8;
+  return (s / 8) % 2 == 1 ? s : s + 8;
+}
+
+// Stages n units of 16 bytes into LDS: unit u is loaded by src(u) and stored by dst(u, v).
+// Each thread issues the loads of up to BATCH of its units before their stores, so a batch's
+// global-memory latency is paid once rather than once per unit.
+template <int NT, int BATCH, class Src, class Dst>
+KMI_DEV void stage_units(int n, const Src& src, const Dst& dst) {
+  for (int u0 = int(threadIdx.x); u0 < n; u0 += BATCH * NT) {
+    uint4 v[BATCH];
+#pragma unroll
+    for (int j = 0; j < BATCH; ++j) {
+      const int u = u0 + j * NT;
+      v[j] = u < n ? src(u) : kZero4;
+    }
+#pragma unroll
+    for (int j = 0; j < BATCH; ++j) {
+      const int u = u0 + j * NT;
+      if (u < n) dst(u, v[j]);
+    }
+  }
+}
+
+// Forward: one work-group per TY × TX tile of output pixels of one image (TX a multiple of 16,
+// TY even), all COP ≥ Co output channels (a multiple of 16).  Up-sampled pixel (Y, X) of the
+// tile reads source pixel ((Y + ky − 1) >> 1, (X + kx − 1) >> 1), all inside the staged patch of
+// (TY/2 + 2) × (TX/2 + 2) source pixels (zero outside the image: the up-grid's zero padding).
+// The 4 waves own TY·TX/64 blocks of 16 pixels along x each.  A fragment (16 pixels × 8
+// channels of one tap) is one 16-byte LDS read per lane at its pixel's source position; a B
+// fragment (8 consecutive k of one output channel) one read of the transposed weights
+// [COP][9 CI] (k = tap·CI + c).
+template <int CI, int COP, int TY, int TX, class EPI>
+__global__ void __launch_bounds__(256) k_conv_fwd(const uint16_t* __restrict__ in, const uint16_t* __restrict__ wt,
+                                                  const int Hi, const int Co, const EPI epi) {
+  constexpr int PY = TY / 2 + 2, PX = TX / 2 + 2, PS = lds_stride(CI), KW = lds_stride(9 * CI);
+  constexpr int XB = TX / 16, MB = TY * XB, MBW = MB / 4, NB = COP / 16, UC = CI / 8, UW = 9 * CI / 8;
+  static_assert(CI % 32 == 0 && COP % 16 == 0 && TX % 16 == 0 && TY % 2 == 0 && MB % 4 == 0, "conv tile");
+  __shared__ __align__(16) uint16_t patch[PY * PX * PS];
+  __shared__ __align__(16) uint16_t wl[COP * KW];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int Ho = 2 * Hi, x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, b = blockIdx.z;
+  const uint16_t* img = in + (long long)b * Hi * Hi * CI;
+  // the patch's units, then the weights'
+  constexpr int NP = PY * PX * UC;
+  stage_units<256, 12>(
+      NP + COP * UW,
+      [&](int u) {
+        if (u >= NP) return ld16(wt + (long long)(u - NP) * 8);  // [COP][9 CI] is contiguous
+        const int cu = u % UC, p = u / UC, px = p % PX, py = p / PX;
+        const int sy = y0 / 2 - 1 + py, sx = x0 / 2 - 1 + px;
+        return (sy >= 0 && sy < Hi && sx >= 0 && sx < Hi) ? ld16(img + ((long long)sy * Hi + sx) * CI + cu * 8) : kZero4;
+      },
+      [&](int u, uint4 v) {
+        if (u >= NP) {
+          const int o = (u - NP) / UW, ku = (u - NP) - o * UW;
+          *reinterpret_cast<uint4*>(&wl[o * KW + ku * 8]) = v;
+        } else {
+          const int cu = u % UC, p = u / UC;
+          *reinterpret_cast<uint4*>(&patch[p * PS + cu * 8]) = v;
+        }
+      });
+  __syncthreads();
+  const int fr = lane & 15, fk = (lane >> 4) << 3;
+  v4f acc[MBW][NB];
+#pragma unroll
+  for (int j = 0; j < MBW; ++j)
+#pragma unroll
+    for (int n = 0; n < NB; ++n) acc[j][n] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int tap = 0; tap < 9; ++tap) {
+    const int ky = tap / 3, kx = tap % 3;
+    int pa[MBW];
+#pragma unroll
+    for (int j = 0; j < MBW; ++j) {
+      const int blk = w * MBW + j, r = blk / XB, cx = (blk % XB) * 16;
+      pa[j] = ((((r + ky - 1) >> 1) + 1) * PX + ((cx + fr + kx - 1) >> 1) + 1) * PS + fk;
+    }
+#pragma unroll
+    for (int cs = 0; cs < CI; cs += 32) {
+      v8bf bfr[NB];
+#pragma unroll
+      for (int n = 0; n < NB; ++n)
+        bfr[n] = __builtin_bit_cast(v8bf, *reinterpret_cast<const uint4*>(&wl[(n * 16 + fr) * KW + tap * CI + cs + fk]));
+#pragma unroll
+      for (int j = 0; j < MBW; ++j) {
+        const v8bf af = __builtin_bit_cast(v8bf, *reinterpret_cast<const uint4*>(&patch[pa[j] + cs]));
+#pragma unroll
+        for (int n = 0; n < NB; ++n) acc[j][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfr[n], acc[j][n], 0, 0, 0);
+      }
+    }
+  }
+  float pre[MBW][NB][4];
+#pragma unroll
+  for (int j = 0; j < MBW; ++j) {
+    const int blk = w * MBW + j, y = y0 + blk / XB, xb = x0 + (blk % XB) * 16 + ((lane >> 4) << 2);
+#pragma unroll
+    for (int n = 0; n < NB; ++n)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int x = xb + i, o = n * 16 + fr;
+        pre[j][n][i] = y < Ho && x < Ho && o < Co ? epi.pre(Co, (b * Ho + y) * Ho + x, o) : 0.f;
+      }
+  }
+#pragma unroll
+  for (int j = 0; j < MBW; ++j) {
+    const int blk = w * MBW + j, y = y0 + blk / XB, xb = x0 + (blk % XB) * 16 + ((lane >> 4) << 2);
+#pragma unroll
+    for (int n = 0; n < NB; ++n)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int x = xb + i, o = n * 16 + fr;
+        if (y < Ho && x < Ho && o < Co) epi(0, Co, 0, (b * Ho + y) * Ho + x, o, acc[j][n][i], pre[j][n][i]);
+      }
+  }
+}
+
+// Data gradient on the input grid: one work-group per TY × TX tile of input pixels of one image
+// (TX a multiple of 16), all CI input channels.  dIn[s, c] = Σ_{d, tap, o} dZ[2s + d − k + 1, o]
+// · W[tap, c, o], d = (dy, dx) the output pixel's place in its 2 × 2 block and k = (ky, kx) the
+// tap; K = 36·CP in the order (d, tap, o).  The dZ patch (up-grid rows 2y0 − 1 … 2y0 + 2TY,
+// columns 2x0 − 2 … 2x0 + 2TX + 1, zero outside the image) is stored split by column parity, so
+// the 16 pixels of a fragment read 16 consecutive LDS pixels; the weights as [tap·CI + c][CP].
+template <int CI, int CP, int TY, int TX, class EPI>
+__global__ void __launch_bounds__(256) k_conv_dgrad(const uint16_t* __restrict__ dz, const uint16_t* __restrict__ wsrc,
+                                                    const int Hi, const EPI epi) {
+  constexpr int QY = 2 * TY + 2, HX = TX + 2, PS = lds_stride(CP), UC = CP / 8;
+  constexpr int XB = TX / 16, MB = TY * XB, MBW = MB / 4, NB = CI / 16, KSTEPS = 36 * CP / 32;
+  static_assert(CI % 16 == 0 && CP % 8 == 0 && TX % 16 == 0 && MB % 4 == 0, "conv tile");
+  __shared__ __align__(16) uint16_t dzs[QY * 2 * HX * PS];
+  __shared__ __align__(16) uint16_t wl[9 * CI * PS];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int Ho = 2 * Hi, x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, b = blockIdx.z;
+  const uint16_t* img = dz + (long long)b * Ho * Ho * CP;
+  // slot (q, par, h) holds up-grid pixel (2y0 − 1 + q, 2x0 − 2 + 2h + par); then the weights
+  constexpr int NP = QY * HX * 2 * UC;
+  stage_units<256, 12>(
+      NP + 9 * CI * UC,
+      [&](int u) {
+        if (u >= NP) return ld16(wsrc + (long long)(u - NP) * 8);  // [9 CI][CP] is contiguous
+        const int cu = u % UC, p = u / UC, par = p & 1, h = (p >> 1) % HX, q = (p >> 1) / HX;
+        const int Y = 2 * y0 - 1 + q, X = 2 * x0 - 2 + 2 * h + par;
+        return (Y >= 0 && Y < Ho && X >= 0 && X < Ho) ? ld16(img + ((long long)Y * Ho + X) * CP + cu * 8) : kZero4;
+      },
+      [&](int u, uint4 v) {
+        if (u >= NP) {
+          const int cu = (u - NP) % UC, row = (u - NP) / UC;
+          *reinterpret_cast<uint4*>(&wl[row * PS + cu * 8]) = v;
+        } else {
+          const int cu = u % UC, p = u / UC, par = p & 1, h = (p >> 1) % HX, q = (p >> 1) / HX;
+          *reinterpret_cast<uint4*>(&dzs[((q * 2 + par) * HX + h) * PS + cu * 8]) = v;
+        }
+      });
+  __syncthreads();
+  const int fr = lane & 15, fk = (lane >> 4) << 3;
+  v4f acc[MBW][NB];
+#pragma unroll
+  for (int j = 0; j < MBW; ++j)
+#pragma unroll
+    for (int n = 0; n < NB; ++n) acc[j][n] = v4f{0.f, 0.f, 0.f, 0.f};
+  for (int ks = 0; ks < KSTEPS; ++ks) {
+    // this lane's 8 consecutive k: (d, tap) = s, channels o … o + 7
+    const int k = ks * 32 + fk, s = k / CP, o = k - s * CP, d = s / 9, tap = s - 9 * d;
+    const int ky = tap / 3, kx = tap - 3 * ky, e = (d & 1) - kx + 1;
+    const int par = e & 1, hoff = (e + 2) >> 1, qoff = (d >> 1) - ky + 2;
+    v8bf bfr[NB];
+#pragma unroll
+    for (int n = 0; n < NB; ++n)
+      bfr[n] = __builtin_bit_cast(v8bf, *reinterpret_cast<const uint4*>(&wl[(tap * CI + n * 16 + fr) * PS + o]));
+#pragma unroll
+    for (int j = 0; j < MBW; ++j) {
+      const int blk = w * MBW + j, r = blk / XB, cx = (blk % XB) * 16;
+      const v8bf af = __builtin_bit_cast(
+          v8bf, *reinterpret_cast<const uint4*>(&dzs[(((2 * r + qoff) * 2 + par) * HX + cx + fr + hoff) * PS + o]));
+#pragma unroll
+      for (int n = 0; n < NB; ++n) acc[j][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfr[n], acc[j][n], 0, 0, 0);
+    }
+  }
+  float pre[MBW][NB][4];
+#pragma unroll
+  for (int j = 0; j < MBW; ++j) {
+    const int blk = w * MBW + j, ys = y0 + blk / XB, xb = x0 + (blk % XB) * 16 + ((lane >> 4) << 2);
+#pragma unroll
+    for (int n = 0; n < NB; ++n)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int xs = xb + i;
+        pre[j][n][i] = ys < Hi && xs < Hi ? epi.pre(CI, (b * Hi + ys) * Hi + xs, n * 16 + fr) : 0.f;
+      }
+  }
+#pragma unroll
+  for (int j = 0; j < MBW; ++j) {
+    const int blk = w * MBW + j, ys = y0 + blk / XB, xb = x0 + (blk % XB) * 16 + ((lane >> 4) << 2);
+#pragma unroll
+    for (int n = 0; n < NB; ++n)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int xs = xb + i;
+        if (ys < Hi && xs < Hi) epi(0, CI, 0, (b * Hi + ys) * Hi + xs, n * 16 + fr, acc[j][n][i], pre[j][n][i]);
+      }
+  }
+}
+
+// Weight and bias gradients over a chunk of input rows, for one block of 16 output channels:
+// dW[tap, c, o] = Σ in_up[Y + ky − 1, X + kx − 1, c] dZ[Y, X, o] and Σ dZ[Y, X, o] over the
+// chunk's output pixels, into the split partials P[z] laid out [9 CI + 1][Co] (bias row last).
+// On the input grid, output pixel (Y, X) = (2s + dy, 2x + dx) under tap (ky, kx) reads input
+// pixel (s + oy, x + ox) with oy = (dy + ky − 1) >> 1 ∈ {−1, 0, 1} (likewise ox): the 36
+// (tap, d) products are 16 distinct products Q[d, oy, ox] (oy − dy + 1 and ox − dx + 1 in
+// {0, 1}), accumulated once each and summed per tap at the end.  MFMA rows are channels c,
+// columns o, k input pixels along a row.  Each step stages TS = 128/HI input rows: the input
+// transposed to [c][row][8 + x] with zero margins (rows s0 − 1 … s0 + TS), the ±1 column
+// shifts assembled from the aligned fragment and the dwords either side of it; dZ transposed
+// per parity d to [d][o][row·HI + x].  One wave per 16 input channels (CI/16 waves); the
+// first also forms the bias row from a fragment of ones.
+template <int CI, int HI>
+__global__ void __launch_bounds__(64 * (CI / 16)) k_conv_wgrad(const uint16_t* __restrict__ in, const uint16_t* __restrict__ dz,
+                                                            const int CP, const int Co, const int steps,
+                                                            const int steps_per_chunk, float* __restrict__ P) {
+  constexpr int NT = 64 * (CI / 16), TS = 128 / HI, RL = HI + 16, CS = lds_stride((TS + 2) * RL);
+  constexpr int DS = lds_stride(TS * HI), UC = CI / 8, HO = 2 * HI;
+  static_assert(CI % 16 == 0 && 128 % HI == 0 && HI % 8 == 0 && TS <= HI, "wgrad tile");
+  __shared__ __align__(16) uint16_t xt[CI * CS];
+  __shared__ __align__(16) uint16_t dt[4 * 16 * DS];
+  const int t = threadIdx.x, lane = t & 63, cb = t >> 6, ob = blockIdx.y;
+  const int fr = lane & 15, g = lane >> 4;
+  for (int u = t; u < CI * CS / 2; u += NT) reinterpret_cast<uint32_t*>(xt)[u] = 0u;  // margins stay zero
+  v4f q[16], qb = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int i = 0; i < 16; ++i) q[i] = v4f{0.f, 0.f, 0.f, 0.f};
+  const v8bf ones = __builtin_bit_cast(v8bf, make_uint4(0x3F803F80u, 0x3F803F80u, 0x3F803F80u, 0x3F803F80u));
+  const int st0 = blockIdx.x * steps_per_chunk, st1 = min(steps, st0 + steps_per_chunk);
+  for (int st = st0; st < st1; ++st) {
+    const int row0 = st * TS, b = row0 / HI, s0 = row0 - b * HI;
+    __syncthreads();  // the previous step's reads are done
+    // input rows s0 − 1 … s0 + TS (channels scattered to their transposed rows), then dZ of
+    // the 2TS output rows, 16 channels of block ob (zero beyond CP)
+    const uint16_t* img = in + (long long)b * HI * HI * CI;
+    const uint16_t* dimg = dz + (long long)b * HO * HO * CP;
+    constexpr int NX = (TS + 2) * HI * UC;
+    stage_units<NT, 8>(
+        NX + 2 * TS * HO * 2,
+        [&](int u) {
+          if (u < NX) {
+            const int cu = u % UC, p = u / UC, x = p % HI, s = s0 - 1 + p / HI;
+            return (s >= 0 && s < HI) ? ld16(img + ((long long)s * HI + x) * CI + cu * 8) : kZero4;
+          }
+          const int w = u - NX, cu = w & 1, X = (w >> 1) % HO, Yr = (w >> 1) / HO, o0 = ob * 16 + cu * 8;
+          return o0 < CP ? ld16(dimg + ((long long)(2 * s0 + Yr) * HO + X) * CP + o0) : kZero4;
+        },
+        [&](int u, uint4 v) {
+          const uint32_t vv[4] = {v.x, v.y, v.z, v.w};
+          if (u < NX) {
+            const int cu = u % UC, p = u / UC, x = p % HI, qr = p / HI;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) xt[(cu * 8 + e) * CS + qr * RL + 8 + x] = uint16_t(vv[e >> 1] >> (16 * (e & 1)));
+          } else {
+            const int w = u - NX, cu = w & 1, X = (w >> 1) % HO, Yr = (w >> 1) / HO;
+            const int d = ((Yr & 1) << 1) | (X & 1), r = Yr >> 1, x = X >> 1;
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+              dt[(d * 16 + cu * 8 + e) * DS + r * HI + x] = uint16_t(vv[e >> 1] >> (16 * (e & 1)));
+          }
+        });
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < TS * HI / 32; ++kk) {
+      const int p = kk * 32 + g * 8, r = p / HI, x = p - r * HI;
+      v8bf bd[4];
+#pragma unroll
+      for (int d = 0; d < 4; ++d)
+        bd[d] = __builtin_bit_cast(v8bf, *reinterpret_cast<const uint4*>(&dt[(d * 16 + fr) * DS + p]));
+      if (cb == 0)
+#pragma unroll
+        for (int d = 0; d < 4; ++d) qb = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ones, bd[d], qb, 0, 0, 0);
+#pragma unroll
+      for (int oy = -1; oy <= 1; ++oy) {
+        const uint16_t* base = &xt[(cb * 16 + fr) * CS + (r + oy + 1) * RL + 8 + x];
+        const uint4 c0 = *reinterpret_cast<const uint4*>(base);
+        const uint32_t lf = *reinterpret_cast<const uint32_t*>(base - 2), rt = *reinterpret_cast<const uint32_t*>(base + 8);
+        v8bf f[3];
+        f[0] = __builtin_bit_cast(v8bf, make_uint4(__builtin_amdgcn_alignbyte(c0.x, lf, 2), __builtin_amdgcn_alignbyte(c0.y, c0.x, 2),
+                                                   __builtin_amdgcn_alignbyte(c0.z, c0.y, 2), __builtin_amdgcn_alignbyte(c0.w, c0.z, 2)));
+        f[1] = __builtin_bit_cast(v8bf, c0);
+        f[2] = __builtin_bit_cast(v8bf, make_uint4(__builtin_amdgcn_alignbyte(c0.y, c0.x, 2), __builtin_amdgcn_alignbyte(c0.z, c0.y, 2),
+                                                   __builtin_amdgcn_alignbyte(c0.w, c0.z, 2), __builtin_amdgcn_alignbyte(rt, c0.w, 2)));
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy) {
+          const int oyi = oy - dy + 1;
+          if (oyi < 0 || oyi > 1) continue;
+#pragma unroll
+          for (int ox = -1; ox <= 1; ++ox)
+#pragma unroll
+            for (int dx = 0; dx < 2; ++dx) {
+              const int oxi = ox - dx + 1;
+              if (oxi < 0 || oxi > 1) continue;
+              const int qi = ((dy * 2 + oyi) * 2 + dx) * 2 + oxi;
+              q[qi] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(f[ox + 1], bd[dy * 2 + dx], q[qi], 0, 0, 0);
+            }
+        }
+      }
+    }
+  }
+  // dW[tap] = Σ_d Q[d, oy(dy, ky), ox(dx, kx)] in the order d = 0 … 3
+  const int Mw = 9 * CI + 1, o = ob * 16 + fr;
+  float* Pz = P + (long long)blockIdx.x * Mw * Co;
+  if (o < Co) {
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+      const int ky = tap / 3, kx = tap % 3;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        float v = 0.f;
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+          const int dy = d >> 1, dx = d & 1;
+          const int oyi = ((dy + ky - 1) >> 1) - dy + 1, oxi = ((dx + kx - 1) >> 1) - dx + 1;
+          v += q[((dy * 2 + oyi) * 2 + dx) * 2 + oxi][i];
+        }
+        Pz[(long long)(tap * CI + cb * 16 + g * 4 + i) * Co + o] = v;
+      }
+    }
+    if (cb == 0 && g == 0) Pz[(long long)(9 * CI) * Co + o] = qb[0];
+  }
 }
 
 // Σ over splits (fixed order) + bias; optional SiLU; writes Z (pre-activation, FP32), the
@@ -626,9 +982,10 @@ __global__ void __launch_bounds__(256) k_colsum_final(const float* __restrict__ 
   if (t == 0) out[n] = lds[0];
 }
 
-// MSE loss partials and dY = 2 (Y − T) / count.
+// MSE loss partials and dY = 2 (Y − T) / count, stored in BF16 as [rows][Cp] (the padding
+// columns C … Cp − 1 are zero from allocation on).
 __global__ void __launch_bounds__(256) k_mse(const float* __restrict__ Y, const float* __restrict__ T,
-                                             long long n, float* __restrict__ dY,
+                                             long long n, int C, int Cp, uint16_t* __restrict__ dYb,
                                              double* __restrict__ part) {
   __shared__ double lds[4];
   double acc = 0.0;
@@ -637,7 +994,8 @@ __global__ void __launch_bounds__(256) k_mse(const float* __restrict__ Y, const 
        i += (long long)gridDim.x * blockDim.x) {
     const float d = Y[i] - T[i];
     acc += double(d) * double(d);
-    dY[i] = scale * d;
+    const long long r = i / C;
+    dYb[r * Cp + (i - r * C)] = f2bf(scale * d);
   }
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
   if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = acc;
@@ -664,22 +1022,35 @@ __global__ void k_to_bf16(const float* __restrict__ x, long long n, uint16_t* __
   if (i < n) y[i] = f2bf(x[i]);
 }
 
-// FP32 [rows, C] → BF16 [rows, Cp], columns C..Cp−1 zero.
-__global__ void k_pad_bf16(const float* __restrict__ x, long long rows, int C, int Cp, uint16_t* __restrict__ y) {
+// The BF16 weight copies the products read, in one launch: job k writes n elements of dst
+// from a BF16 [rows][cols] source — transposed to [outc][rows] (transpose = 1), or with its
+// columns padded to outc (transpose = 0); elements beyond cols are zero.
+struct WCopy {
+  const uint16_t* src;
+  uint16_t* dst;
+  int rows, cols, outc, transpose;
+};
+struct WCopies {
+  WCopy job[4];
+  long long end[4];  // cumulative element counts
+  int n;
+};
+__global__ void k_weight_copies(const WCopies c) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= rows * Cp) return;
-  const long long r = i / Cp;
-  const int c = int(i - r * Cp);
-  y[i] = c < C ? f2bf(x[r * C + c]) : uint16_t(0);
-}
-// BF16 [rows, C] → BF16 [rows, Cp], columns C..Cp−1 zero.
-__global__ void k_pad_cols_bf16(const uint16_t* __restrict__ x, long long rows, int C, int Cp,
-                                uint16_t* __restrict__ y) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= rows * Cp) return;
-  const long long r = i / Cp;
-  const int c = int(i - r * Cp);
-  y[i] = c < C ? x[r * C + c] : uint16_t(0);
+  for (int k = 0; k < c.n; ++k) {
+    if (i >= c.end[k]) continue;
+    const WCopy& J = c.job[k];
+    const long long e = i - (k ? c.end[k - 1] : 0);
+    if (J.transpose) {
+      const int o = int(e / J.rows), r = int(e - (long long)o * J.rows);
+      J.dst[e] = o < J.cols ? J.src[(long long)r * J.cols + o] : uint16_t(0);
+    } else {
+      const long long r = e / J.outc;
+      const int col = int(e - r * J.outc);
+      J.dst[e] = col < J.cols ? J.src[r * J.cols + col] : uint16_t(0);
+    }
+    return;
+  }
 }
 
 // Adam (β1 = 0.9, β2 = 0.999, ε = 1e-8) on the FP32 master weights; refreshes the BF16 copy.
@@ -759,14 +1130,14 @@ struct kmi_nn {
   float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
   uint16_t* wb = nullptr;
   uint16_t* w3p = nullptr;  // BF16 C3 as [9 c2][Cp3], columns >= out_c zero (null: Cp3 == out_c)
+  uint16_t* wt[3] = {nullptr, nullptr, nullptr};  // BF16 conv weights transposed to [COP][9 Ci] (direct forward)
   int64_t adam_t = 0;
   // activations (max_batch)
   float *x = nullptr, *tgt = nullptr;
   uint16_t* xb = nullptr;
   float *z1 = nullptr, *z2 = nullptr, *z3 = nullptr, *z4 = nullptr, *y = nullptr;
   uint16_t *h1b = nullptr, *h2b = nullptr, *h3b = nullptr, *h4b = nullptr;
-  float* dy = nullptr;     // loss gradient [B·H3², out_c]
-  uint16_t* dyb = nullptr; // its BF16 copy [B·H3², Cp3]
+  uint16_t* dyb = nullptr; // BF16 loss gradient [B·H3², Cp3] (columns ≥ out_c zero)
   float* dh = nullptr;     // dense data gradient before the activation
   float* dz[2] = {nullptr, nullptr};  // ping-pong gradients of the stage inputs (FP32, BF16)
   uint16_t* dzb[2] = {nullptr, nullptr};
@@ -782,6 +1153,13 @@ struct kmi_nn {
   int* perm_d = nullptr;
   double* lhist = nullptr;
 };
+
+// Output channels of the direct kernels' N dimension (0: the shape takes k_mm).
+static int direct_cop(int Co) { return Co <= 16 ? 16 : Co <= 32 ? 32 : Co <= 64 ? 64 : 0; }
+static bool direct_ci(int Ci) { return Ci == 32 || Ci == 64; }
+static bool fwd_direct(int Ci, int Co) { return direct_ci(Ci) && direct_cop(Co) != 0; }
+static bool dgrad_direct(int Ci, int Cp) { return direct_ci(Ci) && (Cp == 8 || Cp == 16 || Cp == 32 || Cp == 64); }
+static bool wgrad_direct(int Ci, int Hi) { return direct_ci(Ci) && (Hi == 16 || Hi == 32 || Hi == 64); }
 
 static long long out_elems(const kmi_nn* N, int B) { return (long long)B * N->H3 * N->H3 * N->d.out_c; }
 static const uint16_t* w3_bf16(const kmi_nn* N) { return N->w3p ? N->w3p : N->wb + N->oC3; }
@@ -826,6 +1204,11 @@ static int nn_alloc(kmi_nn* N) {
   al((void**)&N->v, o * 4);
   al((void**)&N->wb, o * 2);
   if (N->Cp3 != d.out_c) al((void**)&N->w3p, 9LL * d.c2 * N->Cp3 * 2);
+  {
+    const int ci[3] = {d.latent_c, d.c1, d.c2}, co[3] = {d.c1, d.c2, d.out_c};
+    for (int k = 0; k < 3; ++k)
+      if (fwd_direct(ci[k], co[k])) al((void**)&N->wt[k], (size_t)direct_cop(co[k]) * 9 * ci[k] * 2);
+  }
   al((void**)&N->x, (size_t)B * d.in_dim * 4);
   al((void**)&N->xb, (size_t)B * d.in_dim * 2);
   al((void**)&N->tgt, a5 * 4);
@@ -838,7 +1221,6 @@ static int nn_alloc(kmi_nn* N) {
   al((void**)&N->z4, a4 * 4);
   al((void**)&N->h4b, a4 * 2);
   al((void**)&N->y, a5 * 4);
-  al((void**)&N->dy, a5 * 4);
   al((void**)&N->dyb, a5p * 2);
   al((void**)&N->dh, a1 * 4);
   for (int k = 0; k < 2; ++k) {
@@ -851,7 +1233,8 @@ static int nn_alloc(kmi_nn* N) {
   al((void**)&N->lout, 64);
   al((void**)&N->idx, (size_t)B * 4);
   if (e != hipSuccess) return hip_fail(N->ctx, e, "kmi_nn allocation");
-  if ((e = hipMemset(N->m, 0, o * 4)) != hipSuccess || (e = hipMemset(N->v, 0, o * 4)) != hipSuccess)
+  if ((e = hipMemset(N->m, 0, o * 4)) != hipSuccess || (e = hipMemset(N->v, 0, o * 4)) != hipSuccess ||
+      (e = hipMemset(N->dyb, 0, a5p * 2)) != hipSuccess)
     return hip_fail(N->ctx, e, "kmi_nn init");
   return KMI_OK;
 }
@@ -888,10 +1271,21 @@ static hipError_t split_sum(const float* P, int M, int Nn, int K, int splits, fl
 
 // The BF16 copies derived from the weights: C3 padded to Cp3 columns.
 static hipError_t refresh_weight_copies(kmi_nn* N, hipStream_t s) {
-  if (!N->w3p) return hipSuccess;
-  const long long rows = 9LL * N->d.c2;
-  hipLaunchKernelGGL(k_pad_cols_bf16, g1(rows * N->Cp3), dim3(256), 0, s, N->wb + N->oC3, rows, N->d.out_c,
-                     N->Cp3, N->w3p);
+  const kmi_nn_desc& d = N->d;
+  WCopies c{};
+  long long total = 0;
+  auto add = [&](const uint16_t* src, uint16_t* dst, int rows, int cols, int outc, int transpose) {
+    c.job[c.n] = WCopy{src, dst, rows, cols, outc, transpose};
+    total += (long long)rows * outc;
+    c.end[c.n++] = total;
+  };
+  if (N->w3p) add(N->wb + N->oC3, N->w3p, 9 * d.c2, d.out_c, N->Cp3, 0);
+  const int ci[3] = {d.latent_c, d.c1, d.c2}, co[3] = {d.c1, d.c2, d.out_c};
+  const long long off[3] = {N->oC1, N->oC2, N->oC3};
+  for (int k = 0; k < 3; ++k)
+    if (N->wt[k]) add(N->wb + off[k], N->wt[k], 9 * ci[k], co[k], direct_cop(co[k]), 1);
+  if (!c.n) return hipSuccess;
+  hipLaunchKernelGGL(k_weight_copies, g1(total), dim3(256), 0, s, c);
   return hipGetLastError();
 }
 
@@ -926,6 +1320,76 @@ static hipError_t mm_wgrad(int M, int Nn, int K, int splits, const LA& la, const
   return mm<64, 16, 4>(M, Nn, K, splits, la, lb, epi, s);
 }
 
+// Direct-convolution launches (shapes checked by fwd_direct / dgrad_direct / wgrad_direct).
+template <int CI, int COP, int TY, int TX, class EPI>
+static hipError_t launch_fwd(const Stage& S, int B, const uint16_t* wt, const EPI& epi, hipStream_t s) {
+  const int Ho = 2 * S.Hi;
+  const dim3 g(unsigned((Ho + TX - 1) / TX), unsigned((Ho + TY - 1) / TY), unsigned(B));
+  hipLaunchKernelGGL((k_conv_fwd<CI, COP, TY, TX, EPI>), g, dim3(256), 0, s, S.in, wt, S.Hi, S.Co, epi);
+  return hipGetLastError();
+}
+template <int CI, class EPI>
+static hipError_t conv_fwd_ci(const Stage& S, int B, const uint16_t* wt, const EPI& epi, hipStream_t s) {
+  switch (direct_cop(S.Co)) {
+    case 64: return launch_fwd<CI, 64, 4, 32>(S, B, wt, epi, s);
+    case 32: return launch_fwd<CI, 32, 16, 16>(S, B, wt, epi, s);
+    default: return launch_fwd<CI, 16, 16, 32>(S, B, wt, epi, s);
+  }
+}
+template <class EPI>
+static hipError_t conv_fwd(const Stage& S, int B, const uint16_t* wt, const EPI& epi, hipStream_t s) {
+  return S.Ci == 64 ? conv_fwd_ci<64>(S, B, wt, epi, s) : conv_fwd_ci<32>(S, B, wt, epi, s);
+}
+
+template <int CI, int CP, class EPI>
+static hipError_t launch_dgrad(const Stage& S, int B, const uint16_t* dzb, const EPI& epi, hipStream_t s) {
+  constexpr int TY = CP >= 64 ? 4 : 8, TX = CI == 32 ? 32 : 16;
+  const dim3 g(unsigned((S.Hi + TX - 1) / TX), unsigned((S.Hi + TY - 1) / TY), unsigned(B));
+  hipLaunchKernelGGL((k_conv_dgrad<CI, CP, TY, TX, EPI>), g, dim3(256), 0, s, dzb, S.wbf, S.Hi, epi);
+  return hipGetLastError();
+}
+template <int CI, class EPI>
+static hipError_t conv_dgrad_ci(const Stage& S, int B, const uint16_t* dzb, const EPI& epi, hipStream_t s) {
+  switch (S.Cp) {
+    case 8: return launch_dgrad<CI, 8>(S, B, dzb, epi, s);
+    case 16: return launch_dgrad<CI, 16>(S, B, dzb, epi, s);
+    case 32: return launch_dgrad<CI, 32>(S, B, dzb, epi, s);
+    default: return launch_dgrad<CI, 64>(S, B, dzb, epi, s);
+  }
+}
+template <class EPI>
+static hipError_t conv_dgrad(const Stage& S, int B, const uint16_t* dzb, const EPI& epi, hipStream_t s) {
+  return S.Ci == 64 ? conv_dgrad_ci<64>(S, B, dzb, epi, s) : conv_dgrad_ci<32>(S, B, dzb, epi, s);
+}
+
+// Weight and bias gradients of a stage into out = g + ow ([9 Ci][Co], then the Co biases):
+// chunks of input rows per work-group (about 256 work-groups, within P), partials summed in
+// chunk order.
+template <int CI, int HI>
+static hipError_t launch_wgrad(kmi_nn* N, const Stage& S, int B, const uint16_t* dzb, float* out, hipStream_t s) {
+  const int nob = (S.Co + 15) / 16, steps = B * HI * HI / 128;
+  const long long mn = (9LL * CI + 1) * S.Co;
+  long long nz = std::min<long long>(steps, std::max(1, 256 / nob));
+  nz = std::max<long long>(1, std::min<long long>(nz, (long long)N->P_elems / mn));
+  const int spc = int((steps + nz - 1) / nz);
+  nz = (steps + spc - 1) / spc;
+  hipLaunchKernelGGL((k_conv_wgrad<CI, HI>), dim3(unsigned(nz), unsigned(nob)), dim3(64 * (CI / 16)), 0, s, S.in, dzb,
+                     S.Cp, S.Co, steps, spc, N->P);
+  hipLaunchKernelGGL(k_split_sum, dim3(unsigned((mn + 15) / 16)), dim3(256), 0, s, N->P, int(nz), mn, out);
+  return hipGetLastError();
+}
+template <int CI>
+static hipError_t conv_wgrad_ci(kmi_nn* N, const Stage& S, int B, const uint16_t* dzb, float* out, hipStream_t s) {
+  switch (S.Hi) {
+    case 16: return launch_wgrad<CI, 16>(N, S, B, dzb, out, s);
+    case 32: return launch_wgrad<CI, 32>(N, S, B, dzb, out, s);
+    default: return launch_wgrad<CI, 64>(N, S, B, dzb, out, s);
+  }
+}
+static hipError_t conv_wgrad(kmi_nn* N, const Stage& S, int B, const uint16_t* dzb, float* out, hipStream_t s) {
+  return S.Ci == 64 ? conv_wgrad_ci<64>(N, S, B, dzb, out, s) : conv_wgrad_ci<32>(N, S, B, dzb, out, s);
+}
+
 static void stages(const kmi_nn* N, Stage (&st)[3]) {
   const kmi_nn_desc& d = N->d;
   st[0] = {N->h2b, d.latent_hw, d.latent_c, d.c1, d.c1, N->wb + N->oC1, N->oC1, N->obc1};
@@ -956,13 +1420,17 @@ static int nn_forward(kmi_nn* N, int B) {
   for (int k = 0; k < 3; ++k) {
     const Stage& S = st[k];
     const int Ho = 2 * S.Hi, M = B * Ho * Ho;
-    NN_CHECK(mm_by_n(M, S.Co, 9 * S.Ci, 1, ConvFwdA{S.in, B, S.Hi, S.Ci}, DenseR{S.wbf, S.Cp, S.Cp, 9 * S.Ci},
-                     EpiBiasAct{w + S.ob, zout[k], nullptr, hout[k], k < 2 ? 1 : 0}, s));
+    const EpiBiasAct epi{w + S.ob, zout[k], nullptr, hout[k], k < 2 ? 1 : 0};
+    if (N->wt[k])
+      NN_CHECK(conv_fwd(S, B, N->wt[k], epi, s));
+    else
+      NN_CHECK(mm_by_n(M, S.Co, 9 * S.Ci, 1, ConvFwdA{S.in, B, S.Hi, S.Ci}, DenseR{S.wbf, S.Cp, S.Cp, 9 * S.Ci}, epi,
+                       s));
   }
   return KMI_OK;
 }
 
-// Backward pass from N->dy (FP32 loss gradient) into N->g (all parameter gradients).
+// Backward pass from N->dyb (BF16 loss gradient, [B·H3²][Cp3]) into N->g (all parameter gradients).
 static int nn_backward(kmi_nn* N, int B) {
   const kmi_nn_desc& d = N->d;
   hipStream_t s = N->ctx->stream;
@@ -972,8 +1440,6 @@ static int nn_backward(kmi_nn* N, int B) {
   stages(N, st);
   const float* zin[3] = {N->z2, N->z3, N->z4};  // pre-activations of the stage inputs (SiLU)
   // gradient w.r.t. the stage output pre-activation in BF16 [M, Cp]
-  const long long rows = (long long)B * N->H3 * N->H3;
-  hipLaunchKernelGGL(k_pad_bf16, g1(rows * N->Cp3), dim3(256), 0, s, N->dy, rows, d.out_c, N->Cp3, N->dyb);
   const uint16_t* dzb = N->dyb;
   int pp = 0;
   for (int k = 2; k >= 0; --k) {
@@ -982,18 +1448,24 @@ static int nn_backward(kmi_nn* N, int B) {
     // weight gradient dW[(ky,kx,c), o] = Σ_pixels A(pixel, (ky,kx,c)) dZ[pixel, o], with a row
     // of ones below A: its result row is the bias gradient Σ_pixels dZ[pixel, o], which the
     // flat layout stores right after the weights (g + ob = g + ow + 9 Ci Co)
-    const int Mw = 9 * S.Ci + 1;
-    const int sp = pick_splits(N, 64, S.Co > 32 ? 64 : (S.Co > 16 ? 32 : 16), Mw, S.Co, Mo, 2048, 512);
-    NN_CHECK(mm_wgrad(Mw, S.Co, Mo, sp, WithOnes<ConvWgradA>{ConvWgradA{S.in, B, S.Hi, S.Ci}, 9 * S.Ci, Mo},
-                      DenseR{dzb, S.Cp, S.Cp, Mo}, EpiStore{N->P}, s));
-    NN_CHECK(split_sum(N->P, Mw, S.Co, Mo, sp, g + S.ow, s));
+    if (wgrad_direct(S.Ci, S.Hi)) {
+      NN_CHECK(conv_wgrad(N, S, B, dzb, g + S.ow, s));
+    } else {
+      const int Mw = 9 * S.Ci + 1;
+      const int sp = pick_splits(N, 64, S.Co > 32 ? 64 : (S.Co > 16 ? 32 : 16), Mw, S.Co, Mo, 2048, 512);
+      NN_CHECK(mm_wgrad(Mw, S.Co, Mo, sp, WithOnes<ConvWgradA>{ConvWgradA{S.in, B, S.Hi, S.Ci}, 9 * S.Ci, Mo},
+                        DenseR{dzb, S.Cp, S.Cp, Mo}, EpiStore{N->P}, s));
+      NN_CHECK(split_sum(N->P, Mw, S.Co, Mo, sp, g + S.ow, s));
+    }
     // data gradient into the stage input (per input pixel, up-sampling folded in), times
     // the SiLU derivative of the input's pre-activation; split over K when the tiles alone
     // leave the device idle
     const int Mi = B * S.Hi * S.Hi, Kd = 36 * S.Cp;
     const int spd = pick_splits(N, S.Ci > 16 ? 128 : 256, S.Ci > 32 ? 64 : (S.Ci > 16 ? 32 : 16), Mi, S.Ci, Kd,
                                 1024, 256);
-    if (spd == 1) {
+    if (dgrad_direct(S.Ci, S.Cp)) {
+      NN_CHECK(conv_dgrad(S, B, dzb, EpiActGrad{zin[k], N->dz[pp], N->dzb[pp], 1}, s));
+    } else if (spd == 1) {
       NN_CHECK(mm_by_n(Mi, S.Ci, Kd, 1, ConvDgradA{dzb, B, S.Hi, S.Cp}, ConvDgradB{S.wbf, S.Ci, S.Cp},
                        EpiActGrad{zin[k], N->dz[pp], N->dzb[pp], 1}, s));
     } else {
@@ -1038,7 +1510,8 @@ static int nn_loss(kmi_nn* N, int B, double* loss, double* slot = nullptr) {
   const long long n = out_elems(N, B);
   const int nb = int(std::min<long long>(1024, (n + 255) / 256));
   if (!slot) slot = N->lout;
-  hipLaunchKernelGGL(k_mse, dim3(unsigned(nb)), dim3(256), 0, s, N->y, N->tgt, n, N->dy, N->lpart);
+  hipLaunchKernelGGL(k_mse, dim3(unsigned(nb)), dim3(256), 0, s, N->y, N->tgt, n, N->d.out_c, N->Cp3, N->dyb,
+                     N->lpart);
   hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, s, N->lpart, nb, slot);
   NN_CHECK(hipGetLastError());
   if (loss) {
@@ -1096,6 +1569,13 @@ int kmi_nn_create(kmi_ctx* ctx, const kmi_nn_desc* d, kmi_nn** out) {
     return fail(ctx, KMI_ERR_ARG, "kmi_nn_create: sizes out of range");
   if (d->latent_c % 8 || d->c1 % 8 || d->c2 % 8)
     return fail(ctx, KMI_ERR_ARG, "kmi_nn_create: latent_c, c1 and c2 must be multiples of 8");
+  {
+    // the kernels index pixels and activation elements in 32 bits
+    const long long h3 = 8LL * d->latent_hw, cmax = std::max({d->latent_c, d->c1, d->c2, d->out_c + 7, d->hidden});
+    if ((long long)d->max_batch * h3 * h3 * cmax >= (1LL << 31) ||
+        (long long)d->hidden * d->latent_hw * d->latent_hw * d->latent_c >= (1LL << 31))
+      return fail(ctx, KMI_ERR_ARG, "kmi_nn_create: max_batch·(8·latent_hw)²·channels must stay below 2^31");
+  }
   KMI_HIP(ctx, hipSetDevice(ctx->device));
   kmi_nn* N = new (std::nothrow) kmi_nn();
   if (!N) return fail(ctx, KMI_ERR_NOMEM, "kmi_nn_create: out of host memory");
@@ -1130,9 +1610,9 @@ int kmi_nn_destroy(kmi_nn* N) {
   (void)hipSetDevice(N->ctx->device);
   (void)hipStreamSynchronize(N->ctx->stream);
   void* ptrs[] = {N->w, N->g, N->m, N->v, N->wb, N->w3p, N->x, N->xb, N->tgt, N->z1, N->h1b, N->z2, N->h2b,
-                  N->z3, N->h3b, N->z4, N->h4b, N->y, N->dy, N->dyb, N->dh, N->dz[0], N->dz[1], N->dzb[0],
+                  N->z3, N->h3b, N->z4, N->h4b, N->y, N->dyb, N->dh, N->dz[0], N->dz[1], N->dzb[0],
                   N->dzb[1], N->P, N->colpart, N->lpart, N->lout, N->idx, N->dsX, N->dsY, N->perm_d,
-                  N->lhist};
+                  N->lhist, N->wt[0], N->wt[1], N->wt[2]};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete N;

@@ -45,21 +45,31 @@ constexpr int group_index(int q) {
 
 KMI_DEV int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
-// Walled lattices (BND): whether a cell's neighbour exists along each axis direction — a
-// periodic axis always wraps; a wall face has none beyond it.
+// Walled lattices: the kernels' template flag BND is 0 for a fully periodic lattice, 1 for
+// the tiles of a walled lattice whose cells at every level stay at least one cell away from
+// every x and y wall (only z walls remain, and those are uniform across a plane), 2 for the
+// other tiles (any wall, Zou–He face or solid cell).
+// Whether a cell's neighbour exists along each axis direction — a periodic axis always wraps;
+// a wall face has none beyond it.
 struct Nbr {
   bool xm, xp, ym, yp, zm, zp;
 };
+template <int BND>
 KMI_DEV Nbr nbr(const StepArgs& a, int gx, int gy, int gz) {
   Nbr n;
-  n.xm = a.per[0] || gx > 0;
-  n.xp = a.per[0] || gx < a.nx - 1;
-  n.ym = a.per[1] || gy > 0;
-  n.yp = a.per[1] || gy < a.ny - 1;
+  if constexpr (BND == 1) {
+    n.xm = n.xp = n.ym = n.yp = true;
+  } else {
+    n.xm = a.per[0] || gx > 0;
+    n.xp = a.per[0] || gx < a.nx - 1;
+    n.ym = a.per[1] || gy > 0;
+    n.yp = a.per[1] || gy < a.ny - 1;
+  }
   n.zm = a.per[2] || gz > 0;
   n.zp = a.per[2] || gz < a.nz - 1;
   return n;
 }
+
 // The source x − c_q of population q exists (else the link bounces back from a wall).
 template <int q>
 KMI_DEV bool src_ok(const Nbr& n) {
@@ -149,9 +159,17 @@ KMI_DEV void collide19(const R (&fp)[19], R (&fo)[19], const StepArgs& a) {
 // After the pull, the single-step kernel's order for a D3Q19 cell without thermal populations,
 // forcing, cut links or rebuilt faces: the Zou–He closures of the faces the cell lies on (in
 // the configured order), then the solid-cell rule or the collision.
-template <int ST, int COLL, bool BND, class R>
+template <int ST, int COLL, int BND, class R>
 KMI_DEV void update19(R (&fp)[19], R (&fo)[19], const StepArgs& a, int gx, int gy, int gz) {
-  if constexpr (BND) {
+  if constexpr (BND == 1) {
+    // interior tiles: only the z faces can hold the cell, and no solid mask exists
+    constexpr bool SH = Store<ST>::shifted;
+    for (int n = 0; n < a.n_zh; ++n) {
+      const int fid = a.zh_face[n];
+      if (fid == KMI_BOTTOM && gz == 0) zh3_apply<SH, 4>(fp, a.face[fid]);
+      if (fid == KMI_TOP && gz == a.nz - 1) zh3_apply<SH, 5>(fp, a.face[fid]);
+    }
+  } else if constexpr (BND == 2) {
     constexpr bool SH = Store<ST>::shifted;
     for (int n = 0; n < a.n_zh; ++n) {
       const int fid = a.zh_face[n];
@@ -183,7 +201,7 @@ KMI_DEV void update19(R (&fp)[19], R (&fo)[19], const StepArgs& a, int gx, int g
 // the plane below (Bs), c_z = −1 from the plane above (Cs).  Walled lattices: a link whose
 // source lies beyond a wall takes the cell's own opposite population of the previous level
 // (A, Bo, Co: the cell's own plane), plus the Ladd term of a moving wall.
-template <bool BND, class R>
+template <int BND, class R>
 KMI_DEV void pull_lds(R (&fp)[19], const R* A, const R* Bs, const R* Cs, const R* Bo, const R* Co, int NEp,
                       int EXp, int ex, int ey, const StepArgs& a, const tb::Nbr& nb) {
   const int own = ey * EXp + ex;
@@ -210,15 +228,12 @@ KMI_DEV void pull_lds(R (&fp)[19], const R* A, const R* Bs, const R* Cs, const R
   });
 }
 
-// Grid: one work-group per (column tile, z-range), padded to a multiple of 8.  Work-group
-// b runs on XCD b mod 8; it takes tile (b mod 8)·per_xcd + b/8, so each XCD works through a
-// contiguous band of neighbouring tiles and their shared halo rows meet in its L2.
+// The two-step march of one column tile (tx, ty) over z-range zc, LDS at `lds`.
 // NT threads per work-group, at least TX·TY: one per interior cell (a thread then evaluates
 // up to two extended cells in phase 1), or at least (TX+2)(TY+2), one per extended cell
 // (threads beyond TX·TY idle in phase 2).
-template <int ST, int COLL, int TX, int TY, int NT, bool NTS, bool BND>
-__global__ void __launch_bounds__(NT)
-    k_step2_d3q19(const StepArgs a, int zlen, int ntx, int nty, int ntiles, int per_xcd) {
+template <int ST, int COLL, int TX, int TY, int NT, bool NTS, int BND>
+KMI_DEV void march2(const StepArgs& a, int zlen, int tx, int ty, int zc, typename Store<ST>::real* lds) {
   using S = Store<ST>;
   using E = typename S::elem;
   using R = typename S::real;
@@ -229,16 +244,12 @@ __global__ void __launch_bounds__(NT)
   // walled lattices keep the c_z = −1 values of two planes (the cell's own plane feeds its
   // bounced links), 29 values per extended cell
   constexpr int NC = BND ? 2 : 1;
-  __shared__ R lds[(19 + 5 * NC) * NE];
   R* const A = lds;            // c_z = 0:  [9][NE]
   R* const B = lds + 9 * NE;   // c_z = +1: [2 planes][5][NE]
   R* const C = lds + 19 * NE;  // c_z = −1: [NC planes][5][NE]
 
-  const int t = int(blockIdx.x % 8u) * per_xcd + int(blockIdx.x / 8u);
-  if (t >= ntiles) return;  // padding groups (uniform per group, before any barrier)
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const long long P = a.plane;
-  const int tx = t % ntx, ty = (t / ntx) % nty, zc = t / (ntx * nty);
   const int x0 = tx * TX, y0 = ty * TY;
   const int z0 = zc * zlen, z1 = min(z0 + zlen, nz);
   const E* __restrict__ fi = static_cast<const E*>(a.fi);
@@ -254,7 +265,8 @@ __global__ void __launch_bounds__(NT)
   constexpr bool kPrefetch = sizeof(R) == 4 && (NT <= 512 || EPT == 1);
   R nxt[EPT][19];
   auto inside = [&](int s, int e) {
-    if constexpr (!BND) return true;
+    if constexpr (BND == 0) return true;
+    if constexpr (BND == 1) return a.per[2] || (s >= 0 && s < nz);
     const int ux = x0 - 1 + e % EX, uy = y0 - 1 + e / EX;
     return (a.per[0] || (ux >= 0 && ux < nx)) && (a.per[1] || (uy >= 0 && uy < ny)) &&
            (a.per[2] || (s >= 0 && s < nz));
@@ -274,7 +286,7 @@ __global__ void __launch_bounds__(NT)
         const unsigned bx[3] = {unsigned(xm) * ES, unsigned(gx) * ES, unsigned(xp) * ES};
         const int yy[3] = {ym, gy, yp}, zz[3] = {zm, zs, zp};
         tb::Nbr nb{};
-        if constexpr (BND) nb = tb::nbr(a, gx, gy, zs);
+        if constexpr (BND) nb = tb::nbr<BND>(a, gx, gy, zs);
         const unsigned own = (unsigned(gx) + unsigned(nx) * unsigned(gy + ny * zs)) * ES;
         sfor<19>([&](auto qc) {
           constexpr int q = decltype(qc)::value;
@@ -335,7 +347,7 @@ __global__ void __launch_bounds__(NT)
       const R* const Co = C + (BND ? ((u - 1) & 1) : 0) * 5 * NE;  // c_z = −1 of plane s−1 (own)
       const int ix = tid % TX, iy = tid / TX;
       tb::Nbr nb{};
-      if constexpr (BND) nb = tb::nbr(a, x0 + ix, y0 + iy, s - 1);
+      if constexpr (BND) nb = tb::nbr<BND>(a, x0 + ix, y0 + iy, s - 1);
       R fp[19], fn[19];
       pull_lds<BND>(fp, A, Br, Cw, Bo, Co, NE, EX, ix + 1, iy + 1, a, nb);
       update19<ST, COLL, BND>(fp, fn, a, x0 + ix, y0 + iy, s - 1);
@@ -363,6 +375,28 @@ __global__ void __launch_bounds__(NT)
   }
 }
 
+// Grid: one work-group per (column tile, z-range), padded to a multiple of 8.  Work-group
+// b runs on XCD b mod 8; it takes tile (b mod 8)·per_xcd + b/8, so each XCD works through a
+// contiguous band of neighbouring tiles and their shared halo rows meet in its L2.  Walled
+// lattices (BND = 2): the tiles of the block `inner` ([x, y) × [z, w) in tile units) march on
+// the interior variant (BND = 1), the others on the full one.
+template <int ST, int COLL, int TX, int TY, int NT, bool NTS, int BND>
+__global__ void __launch_bounds__(NT)
+    k_step2_d3q19(const StepArgs a, int zlen, int ntx, int nty, int ntiles, int per_xcd, const int4 inner) {
+  using R = typename Store<ST>::real;
+  __shared__ R lds[(19 + 5 * (BND ? 2 : 1)) * (TX + 2) * (TY + 2)];
+  const int t = int(blockIdx.x % 8u) * per_xcd + int(blockIdx.x / 8u);
+  if (t >= ntiles) return;  // padding groups (uniform per group, before any barrier)
+  const int tx = t % ntx, ty = (t / ntx) % nty, zc = t / (ntx * nty);
+  if constexpr (BND == 2) {
+    if (tx >= inner.x && tx < inner.y && ty >= inner.z && ty < inner.w) {
+      march2<ST, COLL, TX, TY, NT, NTS, 1>(a, zlen, tx, ty, zc, lds);
+      return;
+    }
+  }
+  march2<ST, COLL, TX, TY, NT, NTS, BND>(a, zlen, tx, ty, zc, lds);
+}
+
 // ---- three steps per launch ------------------------------------------------------------
 // The same march with two LDS levels: the t+1 populations on a two-cell x–y halo
 // ((TX+4)(TY+4) cells, level 1) and the t+2 populations on a one-cell halo ((TX+2)(TY+2),
@@ -384,9 +418,8 @@ __global__ void __launch_bounds__(NT)
 // level-1 and two level-2 cells one after the other; with NT ≥ (TX+4)(TY+4) every cell of a
 // level has its own thread, which halves the registers a thread holds (more resident waves
 // per CU) and runs the halo evaluations side by side; threads beyond TX·TY idle in P3.
-template <int ST, int COLL, int TX, int TY, int NT, bool NTS, bool BND>
-__global__ void __launch_bounds__(NT)
-    k_step3_d3q19(const StepArgs a, int zlen, int ntx, int nty, int ntiles, int per_xcd) {
+template <int ST, int COLL, int TX, int TY, int NT, bool NTS, int BND>
+KMI_DEV void march3(const StepArgs& a, int zlen, int tx, int ty, int zc, typename Store<ST>::real* lds) {
   using S = Store<ST>;
   using E = typename S::elem;
   using R = typename S::real;
@@ -398,7 +431,6 @@ __global__ void __launch_bounds__(NT)
   constexpr unsigned ES = sizeof(E);
   constexpr int NC = BND ? 2 : 1;  // c_z = −1 planes per level (walled: the own plane too)
   constexpr int VPC = 19 + 5 * NC;  // values per cell and level
-  __shared__ R lds[VPC * (N1 + N2)];
   R* const A1 = lds;             // c_z = 0:  [9][N1]
   R* const B1 = lds + 9 * N1;    // c_z = +1: [2 planes][5][N1]
   R* const C1 = lds + 19 * N1;   // c_z = −1: [NC planes][5][N1]
@@ -407,11 +439,8 @@ __global__ void __launch_bounds__(NT)
   R* const B2 = L2 + 9 * N2;     // [2][5][N2]
   R* const C2 = L2 + 19 * N2;    // [NC][5][N2]
 
-  const int t = int(blockIdx.x % 8u) * per_xcd + int(blockIdx.x / 8u);
-  if (t >= ntiles) return;  // padding groups (uniform per group, before any barrier)
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const long long P = a.plane;
-  const int tx = t % ntx, ty = (t / ntx) % nty, zc = t / (ntx * nty);
   const int x0 = tx * TX, y0 = ty * TY;
   const int z0 = zc * zlen, z1 = min(z0 + zlen, nz);
   const E* __restrict__ fi = static_cast<const E*>(a.fi);
@@ -420,7 +449,8 @@ __global__ void __launch_bounds__(NT)
 
   // cell (ux, uy) of plane s lies in the domain (walled lattices; periodic axes always)
   auto inside = [&](int ux, int uy, int s) {
-    if constexpr (!BND) return true;
+    if constexpr (BND == 0) return true;
+    if constexpr (BND == 1) return a.per[2] || (s >= 0 && s < nz);
     return (a.per[0] || (ux >= 0 && ux < nx)) && (a.per[1] || (uy >= 0 && uy < ny)) &&
            (a.per[2] || (s >= 0 && s < nz));
   };
@@ -440,7 +470,7 @@ __global__ void __launch_bounds__(NT)
         const unsigned bx[3] = {unsigned(xm) * ES, unsigned(gx) * ES, unsigned(xp) * ES};
         const int yy[3] = {ym, gy, yp}, zz[3] = {zm, zs, zp};
         tb::Nbr nb{};
-        if constexpr (BND) nb = tb::nbr(a, gx, gy, zs);
+        if constexpr (BND) nb = tb::nbr<BND>(a, gx, gy, zs);
         const unsigned own = (unsigned(gx) + unsigned(nx) * unsigned(gy + ny * zs)) * ES;
         sfor<19>([&](auto qc) {
           constexpr int q = decltype(qc)::value;
@@ -508,7 +538,7 @@ __global__ void __launch_bounds__(NT)
           const int ex = e % E2X, ey = e / E2X;
           const int gx = tb::wrap(x0 - 1 + ex, nx), gy = tb::wrap(y0 - 1 + ey, ny), gz = tb::wrap(s - 1, nz);
           tb::Nbr nb{};
-          if constexpr (BND) nb = tb::nbr(a, gx, gy, gz);
+          if constexpr (BND) nb = tb::nbr<BND>(a, gx, gy, gz);
           R fp[19], fn[19];
           pull_lds<BND>(fp, A1, B1r, C1w, B1o, C1o, N1, E1X, ex + 1, ey + 1, a, nb);
           update19<ST, COLL, BND>(fp, fn, a, gx, gy, gz);
@@ -532,7 +562,7 @@ __global__ void __launch_bounds__(NT)
       const R* const C2o = C2 + (BND ? ((u - 2) & 1) : 0) * 5 * N2;  // c_z = −1 of plane s−2 (own)
       const int ix = tid % TX, iy = tid / TX;
       tb::Nbr nb{};
-      if constexpr (BND) nb = tb::nbr(a, x0 + ix, y0 + iy, s - 2);
+      if constexpr (BND) nb = tb::nbr<BND>(a, x0 + ix, y0 + iy, s - 2);
       R fp[19], fn[19];
       pull_lds<BND>(fp, A2, B2r, C2w, B2o, C2o, N2, E2X, ix + 1, iy + 1, a, nb);
       update19<ST, COLL, BND>(fp, fn, a, x0 + ix, y0 + iy, s - 2);
@@ -571,6 +601,24 @@ __global__ void __launch_bounds__(NT)
       }
     }
   }
+}
+
+// Grid and walled tiles as for k_step2_d3q19.
+template <int ST, int COLL, int TX, int TY, int NT, bool NTS, int BND>
+__global__ void __launch_bounds__(NT)
+    k_step3_d3q19(const StepArgs a, int zlen, int ntx, int nty, int ntiles, int per_xcd, const int4 inner) {
+  using R = typename Store<ST>::real;
+  __shared__ R lds[(19 + 5 * (BND ? 2 : 1)) * ((TX + 4) * (TY + 4) + (TX + 2) * (TY + 2))];
+  const int t = int(blockIdx.x % 8u) * per_xcd + int(blockIdx.x / 8u);
+  if (t >= ntiles) return;  // padding groups (uniform per group, before any barrier)
+  const int tx = t % ntx, ty = (t / ntx) % nty, zc = t / (ntx * nty);
+  if constexpr (BND == 2) {
+    if (tx >= inner.x && tx < inner.y && ty >= inner.z && ty < inner.w) {
+      march3<ST, COLL, TX, TY, NT, NTS, 1>(a, zlen, tx, ty, zc, lds);
+      return;
+    }
+  }
+  march3<ST, COLL, TX, TY, NT, NTS, BND>(a, zlen, tx, ty, zc, lds);
 }
 
 // ---- dispatch ---------------------------------------------------------------------------
@@ -616,46 +664,77 @@ bool step2_tile_walled(int tile, int st) {
   return tile == 12 || tile == 14 || (tile == 9 && st != KMI_STORE_F64);
 }
 
-template <int ST, int COLL, int TX, int TY, int NT = TX * TY>
-static hipError_t launch2(bool nts, bool bnd, const StepArgs& a, int zchunks, hipStream_t s) {
+// Along one axis, the tiles whose cells at every level (H halo cells either side of the tile)
+// lie at least one cell from both walls: [lo, hi), the whole axis when it is periodic.
+static void interior_tiles(bool per, int n, int T, int H, int ntiles, int& lo, int& hi) {
+  lo = 0;
+  hi = ntiles;
+  if (per) return;
+  while (lo < ntiles && lo * T - H < 1) ++lo;
+  while (hi > lo && (hi - 1) * T + T - 1 + H > n - 2) --hi;
+  if (hi <= lo) lo = hi = 0;
+}
+
+// One multi-step pass: every column tile × z-range in one launch, on the periodic kernel
+// (BND = 0) or, for a walled lattice, the walled one (BND = 2) with the interior block of
+// tiles (none when the lattice has a solid mask) marching on the interior variant.
+template <class Go>
+static hipError_t launch_tiles(bool bnd, int span, int TX, int TY, const StepArgs& a, int zchunks, Go&& go) {
   const int ntx = a.nx / TX, nty = a.ny / TY;
   const int zlen = (a.nz + zchunks - 1) / zchunks;
   const int nzc = (a.nz + zlen - 1) / zlen;
   const int ntiles = ntx * nty * nzc;
   const int per_xcd = (ntiles + 7) / 8;
-  const dim3 g(static_cast<unsigned>(8 * per_xcd)), b(static_cast<unsigned>(NT));
-  if (bnd) {
-    if constexpr (ST != KMI_STORE_F16S && (NT == 704 || NT == 384))
-      hipLaunchKernelGGL((k_step2_d3q19<ST, COLL, TX, TY, NT, true, true>), g, b, 0, s, a, zlen, ntx, nty, ntiles, per_xcd);
-    else
-      return hipErrorInvalidValue;
-  } else if (nts) {
-    hipLaunchKernelGGL((k_step2_d3q19<ST, COLL, TX, TY, NT, true, false>), g, b, 0, s, a, zlen, ntx, nty, ntiles, per_xcd);
-  } else {
-    hipLaunchKernelGGL((k_step2_d3q19<ST, COLL, TX, TY, NT, false, false>), g, b, 0, s, a, zlen, ntx, nty, ntiles, per_xcd);
+  int4 inner = make_int4(0, 0, 0, 0);
+  if (bnd && a.mask == nullptr) {
+    interior_tiles(a.per[0] != 0, a.nx, TX, span - 1, ntx, inner.x, inner.y);
+    interior_tiles(a.per[1] != 0, a.ny, TY, span - 1, nty, inner.z, inner.w);
   }
-  return hipGetLastError();
+  return go(dim3(static_cast<unsigned>(8 * per_xcd)), zlen, ntx, nty, ntiles, per_xcd, inner);
+}
+
+template <int ST, int COLL, int TX, int TY, int NT = TX * TY>
+static hipError_t launch2(bool nts, bool bnd, const StepArgs& a, int zchunks, hipStream_t s) {
+  return launch_tiles(bnd, 2, TX, TY, a, zchunks,
+                      [&](dim3 g, int zlen, int ntx, int nty, int ntiles, int per_xcd, int4 inner) -> hipError_t {
+                        const dim3 b(static_cast<unsigned>(NT));
+                        if (bnd) {
+                          if constexpr (ST != KMI_STORE_F16S && (NT == 704 || NT == 384))
+                            hipLaunchKernelGGL((k_step2_d3q19<ST, COLL, TX, TY, NT, true, 2>), g, b, 0, s, a, zlen,
+                                               ntx, nty, ntiles, per_xcd, inner);
+                          else
+                            return hipErrorInvalidValue;
+                        } else if (nts) {
+                          hipLaunchKernelGGL((k_step2_d3q19<ST, COLL, TX, TY, NT, true, 0>), g, b, 0, s, a, zlen, ntx,
+                                             nty, ntiles, per_xcd, inner);
+                        } else {
+                          hipLaunchKernelGGL((k_step2_d3q19<ST, COLL, TX, TY, NT, false, 0>), g, b, 0, s, a, zlen,
+                                             ntx, nty, ntiles, per_xcd, inner);
+                        }
+                        return hipGetLastError();
+                      });
 }
 
 template <int ST, int COLL, int TX, int TY, int NT = TX * TY>
 static hipError_t launch3(bool nts, bool bnd, const StepArgs& a, int zchunks, hipStream_t s) {
-  const int ntx = a.nx / TX, nty = a.ny / TY;
-  const int zlen = (a.nz + zchunks - 1) / zchunks;
-  const int nzc = (a.nz + zlen - 1) / zlen;
-  const int ntiles = ntx * nty * nzc;
-  const int per_xcd = (ntiles + 7) / 8;
-  const dim3 g(static_cast<unsigned>(8 * per_xcd)), b(static_cast<unsigned>(NT));
-  if (bnd) {
-    if constexpr (ST != KMI_STORE_F16S && ST != KMI_STORE_F64 && NT == 448)
-      hipLaunchKernelGGL((k_step3_d3q19<ST, COLL, TX, TY, NT, true, true>), g, b, 0, s, a, zlen, ntx, nty, ntiles, per_xcd);
-    else
-      return hipErrorInvalidValue;
-  } else if (nts) {
-    hipLaunchKernelGGL((k_step3_d3q19<ST, COLL, TX, TY, NT, true, false>), g, b, 0, s, a, zlen, ntx, nty, ntiles, per_xcd);
-  } else {
-    hipLaunchKernelGGL((k_step3_d3q19<ST, COLL, TX, TY, NT, false, false>), g, b, 0, s, a, zlen, ntx, nty, ntiles, per_xcd);
-  }
-  return hipGetLastError();
+  return launch_tiles(bnd, 3, TX, TY, a, zchunks,
+                      [&](dim3 g, int zlen, int ntx, int nty, int ntiles, int per_xcd, int4 inner) -> hipError_t {
+                        const dim3 b(static_cast<unsigned>(NT));
+                        if (bnd) {
+                          if constexpr (ST != KMI_STORE_F16S && ST != KMI_STORE_F64 && NT == 448)
+                            hipLaunchKernelGGL((k_step3_d3q19<ST, COLL, TX, TY, NT, true, 2>), g, b, 0, s, a, zlen,
+                                               ntx, nty, ntiles, per_xcd, inner);
+                          else
+                            return hipErrorInvalidValue;
+                        } else if (nts) {
+                          hipLaunchKernelGGL((k_step3_d3q19<ST, COLL, TX, TY, NT, true, 0>), g, b, 0, s, a, zlen, ntx,
+                                             nty, ntiles, per_xcd, inner);
+                        } else {
+                          hipLaunchKernelGGL((k_step3_d3q19<ST, COLL, TX, TY, NT, false, 0>), g, b, 0, s, a, zlen,
+                                             ntx, nty, ntiles, per_xcd, inner);
+                        }
+                        return hipGetLastError();
+                      });
 }
 
 template <int ST, int COLL>

@@ -176,3 +176,17 @@ def test_check_compatible():
     less = {k: v for k, v in idn.items() if k != "Re"}
     with pytest.raises(ck.CheckpointError, match="identity/Re: the snapshot has .*a key this run does not declare"):
         ck.check_compatible(snap, solver="test_client", schema_version=3, identity=less)
+
+
+def test_container_bytes_follow_the_format_specification(tmp_path):
+    """The reference creates its checkpoints with libver_bounds = (1.10, 1.10), which selects
+    superblock v3 (checkpoint_hdf5.jl:8-13, 116): read from the file's bytes as the HDF5 file
+    format specification lays them out (signature at offset 0, superblock version at byte 8,
+    then the size of offsets and lengths), independent of the library."""
+    _libhdf5_or_skip()
+    p = str(tmp_path / "c.h5")
+    ck.write_checkpoint(p, _snap())
+    head = open(p, "rb").read(16)
+    assert head[:8] == b"\x89HDF\r\n\x1a\n"
+    assert head[8] == 3  # superblock version
+    assert head[9] == 8 and head[10] == 8  # 8-byte offsets and lengths

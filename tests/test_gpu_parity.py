@@ -360,6 +360,7 @@ def _walled_3d_cases(kmi, storage, collision, shape):
             "top": {"kind": "periodic"}}
     yield "zh_channel_inert_ball", kmi.make_desc("d3q19", *shape, 0.04, storage=storage, collision=collision,
                                                  faces=chan, solid_mode="inert"), ball
+    yield "zh_channel", kmi.make_desc("d3q19", *shape, 0.04, storage=storage, collision=collision, faces=chan), None
 
 
 @pytest.mark.parametrize("storage", ["f64", "f32", "f32s"])
@@ -368,8 +369,10 @@ def test_walled_multi_step_launches_bit_identical(kmi, backend, storage, collisi
     """Two- and three-step launches on walled D3Q19 lattices (halfway and moving-wall
     bounce-back, Zou–He velocity and pressure faces, bounce-back and inert solid masks, mixed
     with periodic axes) reproduce single-step launches bit for bit: the bounced links of the
-    LDS levels take the cell's own opposite population of the previous level."""
-    for shape in ((64, 16, 12), (32, 8, 5)):
+    LDS levels take the cell's own opposite population of the previous level.  On the
+    192 × 32 lattice the tiles clear of the x and y walls run the interior variant (z walls
+    only) and the rest the full one; lattices with a mask run the full one throughout."""
+    for shape in ((64, 16, 12), (32, 8, 5), (192, 32, 10)):
         for name, desc, mask in _walled_3d_cases(kmi, storage, collision, shape):
             f = rng_pops(shape, 19, seed=8, amp=0.05)
             for steps in (37, 65):
