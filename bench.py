@@ -356,8 +356,8 @@ def main():
     ap.add_argument("--ws-epochs", type=int, default=150)
     ap.add_argument("--ws-plain", type=int, default=32, help="plain rest runs timed (sample of the converging set)")
     ap.add_argument("--ws-agree", type=int, default=16, help="cases run in FP64 to a tight residual from both starts")
-    ap.add_argument("--ws-agree-tol", type=float, default=1e-12)
-    ap.add_argument("--ws-agree-max-steps", type=int, default=4_000_000)
+    ap.add_argument("--ws-agree-tol", type=float, default=2e-13)
+    ap.add_argument("--ws-agree-max-steps", type=int, default=8_000_000)
     ap.add_argument("--ws-target-tol", type=float, default=1e-8, help="residual of the training states")
     ap.add_argument("--ws-agree-all", action="store_true", help="FP64 agreement over every held-out case")
     ap.add_argument("--cavity-max-steps", type=int, default=600_000)

@@ -25,13 +25,22 @@ namespace kmi {
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-// FP32 → BF16, round to nearest even (finite inputs).
+// FP32 → BF16, round to nearest even: on the device gfx950's v_cvt_pk_bf16_f32, on the host
+// the same rounding in integer arithmetic (inf and NaN kept).
 KMI_HD uint16_t f2bf(float f) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_bit_cast(uint16_t, static_cast<__bf16>(f));
+#else
   uint32_t u = __builtin_bit_cast(uint32_t, f);
   if ((u & 0x7f800000u) == 0x7f800000u) return uint16_t(u >> 16);  // inf / nan
   u += 0x7fffu + ((u >> 16) & 1u);
   return uint16_t(u >> 16);
+#endif
 }
+// The logistic function 1 / (1 + e^−z) with the hardware exponential and reciprocal (a few
+// ulp; the network's products round to BF16 anyway): SiLU(z) = z·σ(z), SiLU'(z) =
+// σ(z)(1 + z(1 − σ(z))).
+KMI_DEV float sigmoid_fast(float z) { return __builtin_amdgcn_rcpf(1.f + __expf(-z)); }
 KMI_HD float bf2f(uint16_t h) {
   return __builtin_bit_cast(float, uint32_t(h) << 16);
 }
@@ -356,12 +365,12 @@ struct EpiBiasAct {  // z = v + bias; Z = z; h = SiLU(z) or z; H (FP32) and Hb (
     const long long i = (long long)m * N + n;
     const float z = v + b;
     if (Z) Z[i] = z;
-    const float h = silu ? z / (1.f + expf(-z)) : z;
+    const float h = silu ? z * sigmoid_fast(z) : z;
     if (H) H[i] = h;
     if (Hb) Hb[i] = f2bf(h);
   }
 };
-struct EpiActGrad {  // g = v ⊙ SiLU'(zin) (or v); FP32 and BF16 copies
+struct EpiActGrad {  // g = v ⊙ SiLU'(zin) (or v); FP32 (when dz is set) and BF16 copies
   const float* zin;
   float* dz;
   uint16_t* dzb;
@@ -371,10 +380,10 @@ struct EpiActGrad {  // g = v ⊙ SiLU'(zin) (or v); FP32 and BF16 copies
     const long long i = (long long)m * N + n;
     float g = v;
     if (silu) {
-      const float s = 1.f / (1.f + expf(-z));
+      const float s = sigmoid_fast(z);
       g = g * (s * (1.f + z * (1.f - s)));
     }
-    dz[i] = g;
+    if (dz) dz[i] = g;
     dzb[i] = f2bf(g);
   }
 };
@@ -579,14 +588,17 @@ KMI_DEV void stage_units(int n, const Src& src, const Dst& dst) {
 // [COP][9 CI] (k = tap·CI + c).
 template <int CI, int COP, int TY, int TX, class EPI>
 __global__ void __launch_bounds__(256) k_conv_fwd(const uint16_t* __restrict__ in, const uint16_t* __restrict__ wt,
-                                                  const int Hi, const int Co, const EPI epi) {
+                                                  const int Hi, const int Co, const int groups, const EPI epi) {
   constexpr int PY = TY / 2 + 2, PX = TX / 2 + 2, PS = lds_stride(CI), KW = lds_stride(9 * CI);
   constexpr int XB = TX / 16, MB = TY * XB, MBW = MB / 4, NB = COP / 16, UC = CI / 8, UW = 9 * CI / 8;
   static_assert(CI % 32 == 0 && COP % 16 == 0 && TX % 16 == 0 && TY % 2 == 0 && MB % 4 == 0, "conv tile");
   __shared__ __align__(16) uint16_t patch[PY * PX * PS];
   __shared__ __align__(16) uint16_t wl[COP * KW];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int Ho = 2 * Hi, x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, b = blockIdx.z;
+  // blockIdx.z = image · groups + channel group (COP output channels each)
+  const int Ho = 2 * Hi, x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, b = blockIdx.z / groups;
+  const int o0 = (blockIdx.z - b * groups) * COP;
+  wt += (long long)o0 * 9 * CI;
   const uint16_t* img = in + (long long)b * Hi * Hi * CI;
   // the patch's units, then the weights'
   constexpr int NP = PY * PX * UC;
@@ -645,7 +657,7 @@ __global__ void __launch_bounds__(256) k_conv_fwd(const uint16_t* __restrict__ i
     for (int n = 0; n < NB; ++n)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int x = xb + i, o = n * 16 + fr;
+        const int x = xb + i, o = o0 + n * 16 + fr;
         pre[j][n][i] = y < Ho && x < Ho && o < Co ? epi.pre(Co, (b * Ho + y) * Ho + x, o) : 0.f;
       }
   }
@@ -656,7 +668,7 @@ __global__ void __launch_bounds__(256) k_conv_fwd(const uint16_t* __restrict__ i
     for (int n = 0; n < NB; ++n)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int x = xb + i, o = n * 16 + fr;
+        const int x = xb + i, o = o0 + n * 16 + fr;
         if (y < Ho && x < Ho && o < Co) epi(0, Co, 0, (b * Ho + y) * Ho + x, o, acc[j][n][i], pre[j][n][i]);
       }
   }
@@ -668,23 +680,28 @@ __global__ void __launch_bounds__(256) k_conv_fwd(const uint16_t* __restrict__ i
 // tap; K = 36·CP in the order (d, tap, o).  The dZ patch (up-grid rows 2y0 − 1 … 2y0 + 2TY,
 // columns 2x0 − 2 … 2x0 + 2TX + 1, zero outside the image) is stored split by column parity, so
 // the 16 pixels of a fragment read 16 consecutive LDS pixels; the weights as [tap·CI + c][CP].
-template <int CI, int CP, int TY, int TX, class EPI>
+template <int CI, int CG, int CP, int TY, int TX, class EPI>
 __global__ void __launch_bounds__(256) k_conv_dgrad(const uint16_t* __restrict__ dz, const uint16_t* __restrict__ wsrc,
                                                     const int Hi, const EPI epi) {
-  constexpr int QY = 2 * TY + 2, HX = TX + 2, PS = lds_stride(CP), UC = CP / 8;
-  constexpr int XB = TX / 16, MB = TY * XB, MBW = MB / 4, NB = CI / 16, KSTEPS = 36 * CP / 32;
-  static_assert(CI % 16 == 0 && CP % 8 == 0 && TX % 16 == 0 && MB % 4 == 0, "conv tile");
+  // CG input channels per work-group (CI / CG groups: blockIdx.z = image · groups + group)
+  constexpr int QY = 2 * TY + 2, HX = TX + 2, PS = lds_stride(CP), UC = CP / 8, GROUPS = CI / CG;
+  constexpr int XB = TX / 16, MB = TY * XB, MBW = MB / 4, NB = CG / 16, KSTEPS = 36 * CP / 32;
+  static_assert(CG % 16 == 0 && CI % CG == 0 && CP % 8 == 0 && TX % 16 == 0 && MB % 4 == 0, "conv tile");
   __shared__ __align__(16) uint16_t dzs[QY * 2 * HX * PS];
-  __shared__ __align__(16) uint16_t wl[9 * CI * PS];
+  __shared__ __align__(16) uint16_t wl[9 * CG * PS];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int Ho = 2 * Hi, x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, b = blockIdx.z;
+  const int Ho = 2 * Hi, x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, b = blockIdx.z / GROUPS;
+  const int c0 = (blockIdx.z - b * GROUPS) * CG;
   const uint16_t* img = dz + (long long)b * Ho * Ho * CP;
   // slot (q, par, h) holds up-grid pixel (2y0 − 1 + q, 2x0 − 2 + 2h + par); then the weights
   constexpr int NP = QY * HX * 2 * UC;
   stage_units<256, 12>(
-      NP + 9 * CI * UC,
+      NP + 9 * CG * UC,
       [&](int u) {
-        if (u >= NP) return ld16(wsrc + (long long)(u - NP) * 8);  // [9 CI][CP] is contiguous
+        if (u >= NP) {  // rows tap·CI + c0 … + CG − 1 of the weights [9 CI][CP]
+          const int cu = (u - NP) % UC, row = (u - NP) / UC, tap = row / CG;
+          return ld16(wsrc + ((long long)tap * CI + c0 + (row - tap * CG)) * CP + cu * 8);
+        }
         const int cu = u % UC, p = u / UC, par = p & 1, h = (p >> 1) % HX, q = (p >> 1) / HX;
         const int Y = 2 * y0 - 1 + q, X = 2 * x0 - 2 + 2 * h + par;
         return (Y >= 0 && Y < Ho && X >= 0 && X < Ho) ? ld16(img + ((long long)Y * Ho + X) * CP + cu * 8) : kZero4;
@@ -705,6 +722,38 @@ __global__ void __launch_bounds__(256) k_conv_dgrad(const uint16_t* __restrict__
   for (int j = 0; j < MBW; ++j)
 #pragma unroll
     for (int n = 0; n < NB; ++n) acc[j][n] = v4f{0.f, 0.f, 0.f, 0.f};
+  if constexpr (CP % 32 == 0) {
+    // a K-step of 32 channels lies in one (d, tap): with both loops unrolled every offset
+    // beyond the lane's base is a compile-time constant
+    int abase[MBW];
+#pragma unroll
+    for (int j = 0; j < MBW; ++j) {
+      const int blk = w * MBW + j, r = blk / XB, cx = (blk % XB) * 16;
+      abase[j] = (4 * r * HX + cx + fr) * PS + fk;
+    }
+    const int bbase = fr * PS + fk;
+#pragma unroll
+    for (int d = 0; d < 4; ++d)
+#pragma unroll
+      for (int tap = 0; tap < 9; ++tap) {
+        const int ky = tap / 3, kx = tap % 3, e = (d & 1) - kx + 1;
+        const int aoff = ((2 * ((d >> 1) - ky + 2) + (e & 1)) * HX + ((e + 2) >> 1)) * PS;
+#pragma unroll
+        for (int cs = 0; cs < CP; cs += 32) {
+          v8bf bfr[NB];
+#pragma unroll
+          for (int n = 0; n < NB; ++n)
+            bfr[n] = __builtin_bit_cast(v8bf, *reinterpret_cast<const uint4*>(&wl[bbase + (tap * CG + n * 16) * PS + cs]));
+#pragma unroll
+          for (int j = 0; j < MBW; ++j) {
+            const v8bf af = __builtin_bit_cast(v8bf, *reinterpret_cast<const uint4*>(&dzs[abase[j] + aoff + cs]));
+#pragma unroll
+            for (int n = 0; n < NB; ++n)
+              acc[j][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfr[n], acc[j][n], 0, 0, 0);
+          }
+        }
+      }
+  } else {
   for (int ks = 0; ks < KSTEPS; ++ks) {
     // this lane's 8 consecutive k: (d, tap) = s, channels o … o + 7
     const int k = ks * 32 + fk, s = k / CP, o = k - s * CP, d = s / 9, tap = s - 9 * d;
@@ -713,7 +762,7 @@ __global__ void __launch_bounds__(256) k_conv_dgrad(const uint16_t* __restrict__
     v8bf bfr[NB];
 #pragma unroll
     for (int n = 0; n < NB; ++n)
-      bfr[n] = __builtin_bit_cast(v8bf, *reinterpret_cast<const uint4*>(&wl[(tap * CI + n * 16 + fr) * PS + o]));
+      bfr[n] = __builtin_bit_cast(v8bf, *reinterpret_cast<const uint4*>(&wl[(tap * CG + n * 16 + fr) * PS + o]));
 #pragma unroll
     for (int j = 0; j < MBW; ++j) {
       const int blk = w * MBW + j, r = blk / XB, cx = (blk % XB) * 16;
@@ -722,6 +771,7 @@ __global__ void __launch_bounds__(256) k_conv_dgrad(const uint16_t* __restrict__
 #pragma unroll
       for (int n = 0; n < NB; ++n) acc[j][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfr[n], acc[j][n], 0, 0, 0);
     }
+  }
   }
   float pre[MBW][NB][4];
 #pragma unroll
@@ -732,7 +782,7 @@ __global__ void __launch_bounds__(256) k_conv_dgrad(const uint16_t* __restrict__
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int xs = xb + i;
-        pre[j][n][i] = ys < Hi && xs < Hi ? epi.pre(CI, (b * Hi + ys) * Hi + xs, n * 16 + fr) : 0.f;
+        pre[j][n][i] = ys < Hi && xs < Hi ? epi.pre(CI, (b * Hi + ys) * Hi + xs, c0 + n * 16 + fr) : 0.f;
       }
   }
 #pragma unroll
@@ -743,7 +793,7 @@ __global__ void __launch_bounds__(256) k_conv_dgrad(const uint16_t* __restrict__
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int xs = xb + i;
-        if (ys < Hi && xs < Hi) epi(0, CI, 0, (b * Hi + ys) * Hi + xs, n * 16 + fr, acc[j][n][i], pre[j][n][i]);
+        if (ys < Hi && xs < Hi) epi(0, CI, 0, (b * Hi + ys) * Hi + xs, c0 + n * 16 + fr, acc[j][n][i], pre[j][n][i]);
       }
   }
 }
@@ -883,103 +933,63 @@ __global__ void k_epilogue(const float* __restrict__ P, int splits, long long MN
   if (bias) z += bias[i % N];
   if (Z) Z[i] = z;
   float h = z;
-  if (silu) h = z / (1.f + expf(-z));
+  if (silu) h = z * sigmoid_fast(z);
   if (H) H[i] = h;
   if (Hb) Hb[i] = f2bf(h);
 }
 
 // Σ over split-K partials of products with a long reduction and a small output (weight
-// gradients): 16 outputs × 16 split lanes per work-group, lane g summing splits g, g+16, …,
-// the lane sums added in lane order (a fixed order: the result is deterministic).
+// gradients): 256/SL outputs × SL split lanes per work-group, lane g summing splits g, g + SL,
+// …, the lane sums added in lane order (a fixed order: the result is deterministic).
+template <int SL>
 __global__ void __launch_bounds__(256) k_split_sum(const float* __restrict__ P, int splits, long long MN,
                                                     float* __restrict__ out) {
-  __shared__ float part[16][17];
-  const int o = threadIdx.x & 15, g = threadIdx.x >> 4;
-  const long long i = (long long)blockIdx.x * 16 + o;
+  constexpr int NO = 256 / SL;
+  __shared__ float part[SL][NO + 1];
+  const int o = threadIdx.x % NO, g = threadIdx.x / NO;
+  const long long i = (long long)blockIdx.x * NO + o;
   float z = 0.f;
   if (i < MN)
-    for (int sp = g; sp < splits; sp += 16) z += P[(long long)sp * MN + i];
+    for (int sp = g; sp < splits; sp += SL) z += P[(long long)sp * MN + i];
   part[g][o] = z;
   __syncthreads();
   if (g == 0 && i < MN) {
     float t = 0.f;
-    for (int k = 0; k < 16; ++k) t += part[k][o];
+    for (int k = 0; k < SL; ++k) t += part[k][o];
     out[i] = t;
   }
 }
-
-// dZ = dH ⊙ SiLU'(Z) (or dH when !silu); FP32 and BF16 copies (the dense layers; the
-// convolutions fuse this into their data-gradient products).
-__global__ void k_act_grad(const float* __restrict__ dH, const float* __restrict__ Z, long long n,
-                           int silu, float* __restrict__ dZ, uint16_t* __restrict__ dZb) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float g = dH[i];
-  if (silu) {
-    const float z = Z[i], s = 1.f / (1.f + expf(-z));
-    g = g * (s * (1.f + z * (1.f - s)));
-  }
-  if (dZ) dZ[i] = g;
-  if (dZb) dZb[i] = f2bf(g);
+static hipError_t launch_split_sum(const float* P, int splits, long long mn, float* out, hipStream_t s) {
+  if (splits > 256)
+    hipLaunchKernelGGL(k_split_sum<64>, dim3(unsigned((mn + 3) / 4)), dim3(256), 0, s, P, splits, mn, out);
+  else
+    hipLaunchKernelGGL(k_split_sum<16>, dim3(unsigned((mn + 15) / 16)), dim3(256), 0, s, P, splits, mn, out);
+  return hipGetLastError();
 }
 
-// Σ over `splits` partials (fixed order), then dZ = Σ ⊙ SiLU'(Z) (or Σ); FP32 and BF16 copies.
-__global__ void k_split_act(const float* __restrict__ P, int splits, long long n, const float* __restrict__ Z,
-                            int silu, float* __restrict__ dZ, uint16_t* __restrict__ dZb) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float g = 0.f;
-  for (int s = 0; s < splits; ++s) g += P[s * n + i];
-  if (silu) {
-    const float z = Z[i], sg = 1.f / (1.f + expf(-z));
-    g = g * (sg * (1.f + z * (1.f - sg)));
-  }
-  dZ[i] = g;
-  dZb[i] = f2bf(g);
-}
-
-// Column sums of X[rows, N] (bias gradients): partial over row chunks, then fixed-order sum.
-// Wide rows (N > 128): one thread per column walks the chunk.  Narrow rows: the 256 threads
-// of a group split into G = 256 / N row lanes per column (consecutive threads read
-// consecutive elements), and the G lane sums of a column are added in lane order.
-__global__ void __launch_bounds__(256) k_colsum_partial(const float* __restrict__ X, long long rows,
-                                                        int N, int chunk, float* __restrict__ part) {
-  const long long r0 = (long long)blockIdx.y * chunk, r1 = min(rows, r0 + chunk);
-  if (N > 128) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
-    float s = 0.f;
-    for (long long r = r0; r < r1; ++r) s += X[r * N + n];
-    part[(long long)blockIdx.y * N + n] = s;
-    return;
-  }
-  __shared__ float lds[256];
-  const int G = 256 / N, t = threadIdx.x, n = t % N, g = t / N;
-  float s = 0.f;
-  if (g < G)
-    for (long long r = r0 + g; r < r1; r += G) s += X[r * N + n];
-  lds[t] = s;
+// Σ over `splits` partials, then dZ = Σ ⊙ SiLU'(Z) (or Σ); FP32 (when dZ is set) and BF16
+// copies.  16 outputs × 16 split lanes per work-group, lane g summing splits g, g + 16, …,
+// the lane sums added in lane order (fixed order).
+__global__ void __launch_bounds__(256) k_split_act(const float* __restrict__ P, int splits, long long n,
+                                                    const float* __restrict__ Z, int silu, float* __restrict__ dZ,
+                                                    uint16_t* __restrict__ dZb) {
+  __shared__ float part[16][17];
+  const int o = threadIdx.x & 15, g = threadIdx.x >> 4;
+  const long long i = (long long)blockIdx.x * 16 + o;
+  float z = 0.f;
+  if (i < n)
+    for (int sp = g; sp < splits; sp += 16) z += P[(long long)sp * n + i];
+  part[g][o] = z;
   __syncthreads();
-  if (t < N) {
-    float acc = 0.f;
-    for (int k = 0; k < G; ++k) acc += lds[k * N + t];
-    part[(long long)blockIdx.y * N + t] = acc;
+  if (g != 0 || i >= n) return;
+  float v = 0.f;
+  for (int k = 0; k < 16; ++k) v += part[k][o];
+  if (silu) {
+    const float zz = Z[i], sg = sigmoid_fast(zz);
+    v = v * (sg * (1.f + zz * (1.f - sg)));
   }
-}
-// One group per column: strided partial sums in fixed order, then a fixed-order tree.
-__global__ void __launch_bounds__(256) k_colsum_final(const float* __restrict__ part, int nchunk, int N,
-                                                      float* __restrict__ out) {
-  __shared__ float lds[256];
-  const int n = blockIdx.x, t = threadIdx.x;
-  float s = 0.f;
-  for (int c = t; c < nchunk; c += 256) s += part[(long long)c * N + n];
-  lds[t] = s;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (t < w) lds[t] += lds[t + w];
-    __syncthreads();
-  }
-  if (t == 0) out[n] = lds[0];
+  if (dZ) dZ[i] = v;
+  dZb[i] = f2bf(v);
 }
 
 // MSE loss partials and dY = 2 (Y − T) / count, stored in BF16 as [rows][Cp] (the padding
@@ -1069,17 +1079,16 @@ __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float
   wb[i] = f2bf(wi);
 }
 
-// Gather rows idx[0..B) of the device dataset into the batch buffers.
-__global__ void k_gather(const float* __restrict__ X, const float* __restrict__ Y,
-                         const int* __restrict__ idx, int B, int din, long long dout,
-                         float* __restrict__ xb, float* __restrict__ yb) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long total = (long long)B * dout;
-  if (t < total) {
-    const int b = int(t / dout);
-    yb[t] = Y[(long long)idx[b] * dout + (t % dout)];
-  }
-  if (t < (long long)B * din) xb[t] = X[(long long)idx[t / din] * din + (t % din)];
+// Gather rows idx[0..B) of the device dataset into the batch buffers: grid (chunks, B), one
+// float4 of a target row per thread (a row holds (8·latent_hw)²·out_c floats, a multiple of
+// 4), and the first group of each row copies its input features.
+__global__ void k_gather(const float* __restrict__ X, const float* __restrict__ Y, const int* __restrict__ idx,
+                         int din, int dout4, float* __restrict__ xb, float* __restrict__ yb) {
+  const int b = blockIdx.y, src = idx[b], t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < dout4)
+    reinterpret_cast<float4*>(yb)[(long long)b * dout4 + t] = reinterpret_cast<const float4*>(Y)[(long long)src * dout4 + t];
+  if (blockIdx.x == 0)
+    for (int i = threadIdx.x; i < din; i += blockDim.x) xb[(long long)b * din + i] = X[(long long)src * din + i];
 }
 
 static inline dim3 g1(long long n) { return dim3(unsigned((n + 255) / 256)); }
@@ -1138,12 +1147,9 @@ struct kmi_nn {
   float *z1 = nullptr, *z2 = nullptr, *z3 = nullptr, *z4 = nullptr, *y = nullptr;
   uint16_t *h1b = nullptr, *h2b = nullptr, *h3b = nullptr, *h4b = nullptr;
   uint16_t* dyb = nullptr; // BF16 loss gradient [B·H3², Cp3] (columns ≥ out_c zero)
-  float* dh = nullptr;     // dense data gradient before the activation
-  float* dz[2] = {nullptr, nullptr};  // ping-pong gradients of the stage inputs (FP32, BF16)
-  uint16_t* dzb[2] = {nullptr, nullptr};
+  uint16_t* dzb[2] = {nullptr, nullptr};  // ping-pong gradients of the stage inputs (BF16)
   float* P = nullptr;  // split-K partials
   size_t P_elems = 0;
-  float* colpart = nullptr;
   double* lpart = nullptr;
   double* lout = nullptr;
   // device dataset, the epoch's permutation and the per-step losses of an epoch
@@ -1222,13 +1228,8 @@ static int nn_alloc(kmi_nn* N) {
   al((void**)&N->h4b, a4 * 2);
   al((void**)&N->y, a5 * 4);
   al((void**)&N->dyb, a5p * 2);
-  al((void**)&N->dh, a1 * 4);
-  for (int k = 0; k < 2; ++k) {
-    al((void**)&N->dz[k], big * 4);
-    al((void**)&N->dzb[k], big * 2);
-  }
+  for (int k = 0; k < 2; ++k) al((void**)&N->dzb[k], big * 2);
   al((void**)&N->P, N->P_elems * 4);
-  al((void**)&N->colpart, 1024LL * std::max(N->L, 256) * 4);
   al((void**)&N->lpart, 1024 * 8);
   al((void**)&N->lout, 64);
   al((void**)&N->idx, (size_t)B * 4);
@@ -1250,23 +1251,11 @@ static int pick_splits(const kmi_nn* N, int BM, int BN, long long M, long long N
   return int(std::max<long long>(1, s));
 }
 
-static int colsum(kmi_nn* N, const float* X, long long rows, int n, float* out, hipStream_t s) {
-  const int chunk = int(std::max<long long>(256, (rows + 1023) / 1024));
-  const int nchunk = int((rows + chunk - 1) / chunk);
-  hipLaunchKernelGGL(k_colsum_partial, dim3(n > 128 ? unsigned((n + 255) / 256) : 1u, unsigned(nchunk)),
-                     dim3(256), 0, s, X, rows, n, chunk, N->colpart);
-  hipLaunchKernelGGL(k_colsum_final, dim3(unsigned(n)), dim3(256), 0, s, N->colpart, nchunk, n, out);
-  hipError_t e = hipGetLastError();
-  return e == hipSuccess ? KMI_OK : hip_fail(N->ctx, e, "bias gradient");
-}
-
 // Σ over the splits of P into out (fixed order).
 static hipError_t split_sum(const float* P, int M, int Nn, int K, int splits, float* out, hipStream_t s) {
   int sp = 1;
   split_chunk(K, splits, &sp);
-  const long long mn = (long long)M * Nn;
-  hipLaunchKernelGGL(k_split_sum, dim3(unsigned((mn + 15) / 16)), dim3(256), 0, s, P, sp, mn, out);
-  return hipGetLastError();
+  return launch_split_sum(P, sp, (long long)M * Nn, out, s);
 }
 
 // The BF16 copies derived from the weights: C3 padded to Cp3 columns.
@@ -1321,19 +1310,22 @@ static hipError_t mm_wgrad(int M, int Nn, int K, int splits, const LA& la, const
 }
 
 // Direct-convolution launches (shapes checked by fwd_direct / dgrad_direct / wgrad_direct).
+// COP output channels per work-group, `groups` groups (the transposed weights hold
+// groups · COP rows).
 template <int CI, int COP, int TY, int TX, class EPI>
-static hipError_t launch_fwd(const Stage& S, int B, const uint16_t* wt, const EPI& epi, hipStream_t s) {
+static hipError_t launch_fwd(const Stage& S, int B, const uint16_t* wt, const EPI& epi, hipStream_t s,
+                             int groups = 1) {
   const int Ho = 2 * S.Hi;
-  const dim3 g(unsigned((Ho + TX - 1) / TX), unsigned((Ho + TY - 1) / TY), unsigned(B));
-  hipLaunchKernelGGL((k_conv_fwd<CI, COP, TY, TX, EPI>), g, dim3(256), 0, s, S.in, wt, S.Hi, S.Co, epi);
+  const dim3 g(unsigned((Ho + TX - 1) / TX), unsigned((Ho + TY - 1) / TY), unsigned(B * groups));
+  hipLaunchKernelGGL((k_conv_fwd<CI, COP, TY, TX, EPI>), g, dim3(256), 0, s, S.in, wt, S.Hi, S.Co, groups, epi);
   return hipGetLastError();
 }
 template <int CI, class EPI>
 static hipError_t conv_fwd_ci(const Stage& S, int B, const uint16_t* wt, const EPI& epi, hipStream_t s) {
   switch (direct_cop(S.Co)) {
-    case 64: return launch_fwd<CI, 64, 4, 32>(S, B, wt, epi, s);
-    case 32: return launch_fwd<CI, 32, 16, 16>(S, B, wt, epi, s);
-    default: return launch_fwd<CI, 16, 16, 32>(S, B, wt, epi, s);
+    case 64: return launch_fwd<CI, 32, 8, 16>(S, B, wt, epi, s, 2);  // two groups of 32 channels
+    case 32: return launch_fwd<CI, 32, 8, 16>(S, B, wt, epi, s);
+    default: return launch_fwd<CI, 16, 8, 32>(S, B, wt, epi, s);
   }
 }
 template <class EPI>
@@ -1341,11 +1333,13 @@ static hipError_t conv_fwd(const Stage& S, int B, const uint16_t* wt, const EPI&
   return S.Ci == 64 ? conv_fwd_ci<64>(S, B, wt, epi, s) : conv_fwd_ci<32>(S, B, wt, epi, s);
 }
 
+// Tiles: 64 input channels in groups of 32 (16 with 64 output channels), so the weight slab
+// a group stages stays small and two or three groups fit per CU.
 template <int CI, int CP, class EPI>
 static hipError_t launch_dgrad(const Stage& S, int B, const uint16_t* dzb, const EPI& epi, hipStream_t s) {
-  constexpr int TY = CP >= 64 ? 4 : 8, TX = CI == 32 ? 32 : 16;
-  const dim3 g(unsigned((S.Hi + TX - 1) / TX), unsigned((S.Hi + TY - 1) / TY), unsigned(B));
-  hipLaunchKernelGGL((k_conv_dgrad<CI, CP, TY, TX, EPI>), g, dim3(256), 0, s, dzb, S.wbf, S.Hi, epi);
+  constexpr int CG = CI == 64 ? (CP >= 64 ? 16 : 32) : CI, TY = CP >= 32 ? 4 : 8, TX = CI == 32 ? 32 : 16;
+  const dim3 g(unsigned((S.Hi + TX - 1) / TX), unsigned((S.Hi + TY - 1) / TY), unsigned(B * (CI / CG)));
+  hipLaunchKernelGGL((k_conv_dgrad<CI, CG, CP, TY, TX, EPI>), g, dim3(256), 0, s, dzb, S.wbf, S.Hi, epi);
   return hipGetLastError();
 }
 template <int CI, class EPI>
@@ -1369,14 +1363,16 @@ template <int CI, int HI>
 static hipError_t launch_wgrad(kmi_nn* N, const Stage& S, int B, const uint16_t* dzb, float* out, hipStream_t s) {
   const int nob = (S.Co + 15) / 16, steps = B * HI * HI / 128;
   const long long mn = (9LL * CI + 1) * S.Co;
-  long long nz = std::min<long long>(steps, std::max(1, 256 / nob));
+  // about 1024 work-groups, with at most 8 MB of partials
+  long long nz = std::min<long long>(steps, std::max(1, 1024 / nob));
+  nz = std::min<long long>(nz, std::max<long long>(2 * 1024 * 1024 / mn, 256 / nob));
   nz = std::max<long long>(1, std::min<long long>(nz, (long long)N->P_elems / mn));
   const int spc = int((steps + nz - 1) / nz);
   nz = (steps + spc - 1) / spc;
   hipLaunchKernelGGL((k_conv_wgrad<CI, HI>), dim3(unsigned(nz), unsigned(nob)), dim3(64 * (CI / 16)), 0, s, S.in, dzb,
                      S.Cp, S.Co, steps, spc, N->P);
-  hipLaunchKernelGGL(k_split_sum, dim3(unsigned((mn + 15) / 16)), dim3(256), 0, s, N->P, int(nz), mn, out);
-  return hipGetLastError();
+  if (hipError_t e = hipGetLastError(); e != hipSuccess) return e;
+  return launch_split_sum(N->P, int(nz), mn, out, s);
 }
 template <int CI>
 static hipError_t conv_wgrad_ci(kmi_nn* N, const Stage& S, int B, const uint16_t* dzb, float* out, hipStream_t s) {
@@ -1464,17 +1460,18 @@ static int nn_backward(kmi_nn* N, int B) {
     const int spd = pick_splits(N, S.Ci > 16 ? 128 : 256, S.Ci > 32 ? 64 : (S.Ci > 16 ? 32 : 16), Mi, S.Ci, Kd,
                                 1024, 256);
     if (dgrad_direct(S.Ci, S.Cp)) {
-      NN_CHECK(conv_dgrad(S, B, dzb, EpiActGrad{zin[k], N->dz[pp], N->dzb[pp], 1}, s));
+      NN_CHECK(conv_dgrad(S, B, dzb, EpiActGrad{zin[k], nullptr, N->dzb[pp], 1}, s));
     } else if (spd == 1) {
       NN_CHECK(mm_by_n(Mi, S.Ci, Kd, 1, ConvDgradA{dzb, B, S.Hi, S.Cp}, ConvDgradB{S.wbf, S.Ci, S.Cp},
-                       EpiActGrad{zin[k], N->dz[pp], N->dzb[pp], 1}, s));
+                       EpiActGrad{zin[k], nullptr, N->dzb[pp], 1}, s));
     } else {
       NN_CHECK(mm_by_n(Mi, S.Ci, Kd, spd, ConvDgradA{dzb, B, S.Hi, S.Cp}, ConvDgradB{S.wbf, S.Ci, S.Cp},
                        EpiStore{N->P}, s));
       int sp2 = 1;
       split_chunk(Kd, spd, &sp2);
       const long long ni = (long long)Mi * S.Ci;
-      hipLaunchKernelGGL(k_split_act, g1(ni), dim3(256), 0, s, N->P, sp2, ni, zin[k], 1, N->dz[pp], N->dzb[pp]);
+      hipLaunchKernelGGL(k_split_act, dim3(unsigned((ni + 15) / 16)), dim3(256), 0, s, N->P, sp2, ni, zin[k], 1,
+                         nullptr, N->dzb[pp]);
       NN_CHECK(hipGetLastError());
     }
     dzb = N->dzb[pp];
@@ -1493,12 +1490,14 @@ static int nn_backward(kmi_nn* N, int B) {
     int sp2 = 1;
     split_chunk(N->L, sp, &sp2);
     const long long nh = (long long)B * d.hidden;
-    hipLaunchKernelGGL(k_split_act, g1(nh), dim3(256), 0, s, N->P, sp2, nh, N->z1, 1, N->dz[pp], N->dzb[pp]);
+    hipLaunchKernelGGL(k_split_act, dim3(unsigned((nh + 15) / 16)), dim3(256), 0, s, N->P, sp2, nh, N->z1, 1,
+                       nullptr, N->dzb[pp]);
     NN_CHECK(hipGetLastError());
   }
-  // dense 1
-  if (int rc = colsum(N, N->dz[pp], B, d.hidden, g + N->ob1, s)) return rc;
-  NN_CHECK((mm<64, 64, 2>(d.in_dim, d.hidden, B, 1, DenseR{N->xb, d.in_dim, d.in_dim, B},
+  // dense 1: weight and bias gradients (a row of ones below xᵀ; b1 follows W1 in the flat
+  // layout)
+  NN_CHECK((mm<64, 64, 2>(d.in_dim + 1, d.hidden, B, 1,
+                          WithOnes<DenseR>{DenseR{N->xb, d.in_dim, d.in_dim, B}, d.in_dim, B},
                           DenseR{N->dzb[pp], d.hidden, d.hidden, B}, EpiStore{g + N->oW1}, s)));
   return KMI_OK;
 }
@@ -1610,8 +1609,8 @@ int kmi_nn_destroy(kmi_nn* N) {
   (void)hipSetDevice(N->ctx->device);
   (void)hipStreamSynchronize(N->ctx->stream);
   void* ptrs[] = {N->w, N->g, N->m, N->v, N->wb, N->w3p, N->x, N->xb, N->tgt, N->z1, N->h1b, N->z2, N->h2b,
-                  N->z3, N->h3b, N->z4, N->h4b, N->y, N->dyb, N->dh, N->dz[0], N->dz[1], N->dzb[0],
-                  N->dzb[1], N->P, N->colpart, N->lpart, N->lout, N->idx, N->dsX, N->dsY, N->perm_d,
+                  N->z3, N->h3b, N->z4, N->h4b, N->y, N->dyb, N->dzb[0],
+                  N->dzb[1], N->P, N->lpart, N->lout, N->idx, N->dsX, N->dsY, N->perm_d,
                   N->lhist, N->wt[0], N->wt[1], N->wt[2]};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -1729,8 +1728,8 @@ int kmi_nn_train_epochs(kmi_nn* N, int32_t epochs, int32_t B, float lr, uint64_t
     KMI_HIP(c, hipStreamSynchronize(c->stream));
     KMI_HIP(c, hipMemcpyAsync(N->perm_d, perm.data(), (size_t)N->dsN * 4, hipMemcpyHostToDevice, c->stream));
     for (int b = 0; b < nb; ++b) {
-      hipLaunchKernelGGL(k_gather, g1((long long)B * dout), dim3(256), 0, c->stream, N->dsX, N->dsY,
-                         N->perm_d + (size_t)b * B, B, N->d.in_dim, dout, N->x, N->tgt);
+      hipLaunchKernelGGL(k_gather, dim3(unsigned((dout / 4 + 255) / 256), unsigned(B)), dim3(256), 0, c->stream,
+                         N->dsX, N->dsY, N->perm_d + (size_t)b * B, N->d.in_dim, int(dout / 4), N->x, N->tgt);
       KMI_HIP(c, hipGetLastError());
       if (int rc = nn_forward(N, B)) return rc;
       if (int rc = nn_loss(N, B, nullptr, N->lhist + b)) return rc;

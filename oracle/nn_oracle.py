@@ -132,6 +132,6 @@ def loss_grad(arch, flat, x, t, bf=True):
     g["W2"] = c["h1b"].T @ dzb
     dh1 = dzb @ r(p["W2"]).T
     dz1 = dh1 * dsilu(c["z1"])
-    g["b1"] = dz1.sum(axis=0)
+    g["b1"] = r(dz1).sum(axis=0)  # a row of ones in the BF16 product, as for b2
     g["W1"] = c["xb"].T @ r(dz1)
     return loss, arch.pack(g), y

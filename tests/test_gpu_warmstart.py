@@ -1,7 +1,7 @@
 """The learned warm start at its configured size: 2048 steady 128² MRT cavities from the
 batched solver train the decoder on the GPU; 256 held-out cases run to the residual 1e-7
 warm-started in one batch, against the same batch from rest and against rest-initialised
-plain single-lattice runs; FP64 runs of both starts to r < 1e-12 compare the converged
+plain single-lattice runs; FP64 runs of both starts to r < 2e-13 compare the converged
 fields.  Re log-uniform on [100, 1000], lid amplitude uniform on [0.06, 0.12] (tau >= 0.523,
 inside the reference scheme's stability range, tests/test_oracle_mrt.py)."""
 import json
@@ -18,7 +18,7 @@ def test_config5_warm_start_at_full_size(kmi, backend):
     from kraken_mi355x import warmstart as ws
 
     rec = ws.experiment(n_train=2048, n_test=256, epochs=150, target_tol=1e-8, tol=1e-7, plain_sample=16,
-                        agree_sample=8, agree_tol=1e-12, backend=backend, log=lambda m: print(m, flush=True))
+                        agree_sample=8, agree_tol=2e-13, backend=backend, log=lambda m: print(m, flush=True))
     os.makedirs(OUT, exist_ok=True)
     with open(os.path.join(OUT, "warmstart_test.json"), "w") as fh:
         json.dump(rec, fh, indent=1)
@@ -33,10 +33,10 @@ def test_config5_warm_start_at_full_size(kmi, backend):
     # what the network itself gains over the same batch started at rest (measured 1.51x)
     assert rec["batched_rest_over_warm"] > 1.2
     assert rec["prediction_rel_L2_median"] < 0.01
-    # the warm start leaves the answer unchanged: driven in FP64 to r < 1e-12 the two starts'
-    # fields agree to the stop rule's own spread (r·tau from the steady state; measured
-    # median 1.4e-7, max 1.7e-6 of the lid amplitude)
+    # the warm start leaves the answer unchanged: driven in FP64 to r < 2e-13 the two starts'
+    # fields agree within the target 1e-6 of the lid amplitude (the spread left is the stop
+    # rule's, about r·τ from the steady state; at r < 1e-12 it was up to 1.7e-6); the slowest
+    # cases may not reach the residual within the 8M-step cap
     agree = rec["fp64_agreement"]
-    assert agree["converged_both"] >= agree["cases"] - 1
-    assert agree["median_dU_over_A"] < 1e-6
-    assert agree["max_dU_over_A"] < 5e-6
+    assert agree["converged_both"] >= agree["cases"] - 2
+    assert agree["max_dU_over_A"] < 1e-6
