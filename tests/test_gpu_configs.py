@@ -123,10 +123,15 @@ def test_config2_tgv_256_fp64_bitwise_and_reduced_precision(kmi, po, backend):
     th = threading.Thread(target=oracle_run)
     th.start()
     res = {}
+    # each storage on the launch the benchmark tunes to (FP64 on two-step 64x8 tiles with 704
+    # threads, FP32 three steps on 32x8 with 448, F32S three steps on 64x8 with 832, F16S two
+    # steps on 64x16), so the errors below are those of the measured launches
+    launches = {"f64": (12, 2), "f32": (9, 2), "f32s": (11, 2), "f16s": (5, 4)}
     try:
         for st in ("f64", "f32", "f32s", "f16s"):
-            _progress(f"[config2] TGV 256^3 {st} {steps} steps on the device")
+            _progress(f"[config2] TGV 256^3 {st} {steps} steps on the device, launch {launches[st]}")
             with kmi.Lattice(kmi.taylor_green_3d_desc(N, nu, st), backend) as lat:
+                lat.set_temporal(*launches[st])
                 lat.init_equilibrium(*fields)
                 m0 = lat.reduce("mass")
                 ke0 = lat.reduce("kinetic_energy")
