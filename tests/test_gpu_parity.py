@@ -361,6 +361,11 @@ def _walled_3d_cases(kmi, storage, collision, shape):
     yield "zh_channel_inert_ball", kmi.make_desc("d3q19", *shape, 0.04, storage=storage, collision=collision,
                                                  faces=chan, solid_mode="inert"), ball
     yield "zh_channel", kmi.make_desc("d3q19", *shape, 0.04, storage=storage, collision=collision, faces=chan), None
+    # walls only in z (a Couette gap): every tile takes the interior variant
+    couette = {"west": {"kind": "periodic"}, "east": {"kind": "periodic"}, "south": {"kind": "periodic"},
+               "north": {"kind": "periodic"}, "bottom": {"kind": "wall"},
+               "top": {"kind": "moving_wall", "u": (0.05, 0.01, 0.0), "rho": 1.0}}
+    yield "couette_z", kmi.make_desc("d3q19", *shape, 0.03, storage=storage, collision=collision, faces=couette), None
 
 
 @pytest.mark.parametrize("storage", ["f64", "f32", "f32s"])
