@@ -14,7 +14,8 @@ steps the launch advances — divided by the launch's average duration, timed wi
 the lattice stream over whole launches only.  `traffic` is the rocprofv3 counter figure for
 the same kernel and launch shape (FETCH_SIZE and WRITE_SIZE, calibrated on a copy kernel),
 `frac_counter_bytes` the counted bytes over the same time.  The per-step figure (152 B per
-cell update × steps per launch) is reported apart as `per_step_GBps`.
+cell update × steps per launch) is reported apart as `per_step_GBps` and, over 8 TB/s, as
+`per_step_frac`.
 
 The same line carries the learned warm-start record (`warm_start`): 2048 steady 128² MRT
 cavities from the batched solver train the decoder, 256 held-out cases run to the residual
@@ -474,6 +475,9 @@ def main():
                      "frac_counter_bytes": (round(traffic / launch_s / 1e9 / HBM_PEAK_GBPS, 4)
                                             if traffic else None),
                      "per_step_GBps": round(span * achieved, 1),
+                     # the same per cell update (152 B in FP32 D3Q19): 0.8 of the 8 TB/s
+                     # roofline per update is 42 GLUPS
+                     "per_step_frac": round(span * achieved / HBM_PEAK_GBPS, 4),
                      "copy_measured": round(copy_gbps, 1),
                      "frac_of_copy": round(achieved / copy_gbps, 4)},
         "launch": launch,

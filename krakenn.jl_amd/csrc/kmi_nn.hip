@@ -1004,8 +1004,8 @@ __global__ void __launch_bounds__(256) k_mse(const float* __restrict__ Y, const 
        i += (long long)gridDim.x * blockDim.x) {
     const float d = Y[i] - T[i];
     acc += double(d) * double(d);
-    const long long r = i / C;
-    dYb[r * Cp + (i - r * C)] = f2bf(scale * d);
+    const int ii = int(i), r = ii / C;  // element counts stay below 2^31 (kmi_nn_create)
+    dYb[(long long)r * Cp + (ii - r * C)] = f2bf(scale * d);
   }
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
   if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = acc;

@@ -11,6 +11,9 @@ for p in (ROOT, os.path.join(ROOT, "krakenn.jl_amd")):
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an AMD GPU (MI355X) and the built HIP library")
+    from tests import _records
+
+    _records.set_writer(config)
 
 
 @pytest.fixture(scope="session")

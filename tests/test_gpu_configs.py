@@ -2,37 +2,27 @@
 
 Parity at full size: FP64 populations bit-identical to the oracle (cavity 128² for the
 full 20000 steps; TGV 256³ for the configuration's 1000 steps; RB 512×256 for as many steps
-as the CPU oracle finishes in seconds).  Reduced-precision storage is judged against the
+as the CPU oracle finishes in seconds).  Reduced-precision storage is compared with the
 FP64 oracle field (TGV) or the FP64 run (RB, 150k steps); every test prints the max / L2
 errors it measured and asserts the bound written in it.
 """
 import json
 import math
-import os
 import threading
 
 import numpy as np
 import pytest
 
+from tests import _records
+
 pytestmark = pytest.mark.gpu
 
-OUT = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpurun_out")
-
-
 def _record(name, data):
-    os.makedirs(OUT, exist_ok=True)
-    path = os.path.join(OUT, "config_errors.json")
-    cur = json.load(open(path)) if os.path.exists(path) else {}
-    cur[name] = data
-    json.dump(cur, open(path, "w"), indent=1)
+    _records.record("config_errors.json", name, data)
 
 
 def _progress(msg):
-    """Print and append to gpurun_out/progress.log (long tests report as they go)."""
-    print(msg, flush=True)
-    os.makedirs(OUT, exist_ok=True)
-    with open(os.path.join(OUT, "progress.log"), "a") as fh:
-        fh.write(msg + "\n")
+    _records.progress(msg)
 
 
 def _err(a, b):

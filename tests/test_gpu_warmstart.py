@@ -5,23 +5,20 @@ plain single-lattice runs; FP64 runs of both starts to r < 2e-13 compare the con
 fields.  Re log-uniform on [100, 1000], lid amplitude uniform on [0.06, 0.12] (tau >= 0.523,
 inside the reference scheme's stability range, tests/test_oracle_mrt.py)."""
 import json
-import os
 
 import pytest
 
-pytestmark = pytest.mark.gpu
+from tests import _records
 
-OUT = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpurun_out")
+pytestmark = pytest.mark.gpu
 
 
 def test_config5_warm_start_at_full_size(kmi, backend):
     from kraken_mi355x import warmstart as ws
 
     rec = ws.experiment(n_train=2048, n_test=256, epochs=150, target_tol=1e-8, tol=1e-7, plain_sample=16,
-                        agree_sample=8, agree_tol=2e-13, backend=backend, log=lambda m: print(m, flush=True))
-    os.makedirs(OUT, exist_ok=True)
-    with open(os.path.join(OUT, "warmstart_test.json"), "w") as fh:
-        json.dump(rec, fh, indent=1)
+                        agree_sample=8, agree_tol=2e-13, backend=backend, log=_records.progress)
+    _records.record("warmstart_test.json", "config5", rec)
     print(json.dumps({k: rec[k] for k in ("value", "batched_rest_over_warm", "median_step_ratio_rest_over_warm",
                                           "prediction_rel_L2_median", "fp64_agreement")}), flush=True)
     # every training case has a steady state and every held-out case converges from both starts
