@@ -7,18 +7,21 @@ variable KMI_TEST_RECORDS, when it is set."""
 import json
 import os
 
-_writer = None
+_config = None
 
 
 def set_writer(config):
-    global _writer
-    _writer = config.pluginmanager.get_plugin("terminalreporter")
+    """Keep pytest's config; its terminal reporter is looked up when a line is written (it may
+    register after the conftest that calls this)."""
+    global _config
+    _config = config
 
 
 def progress(msg):
     print(msg, flush=True)
-    if _writer is not None:
-        _writer.write_line(msg)
+    writer = _config.pluginmanager.get_plugin("terminalreporter") if _config is not None else None
+    if writer is not None:
+        writer.write_line(msg)
     d = os.environ.get("KMI_TEST_RECORDS")
     if d:
         os.makedirs(d, exist_ok=True)
