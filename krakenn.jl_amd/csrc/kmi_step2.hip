@@ -400,9 +400,9 @@ __global__ void __launch_bounds__(NT)
 // ---- three steps per launch ------------------------------------------------------------
 // The same march with two LDS levels: the t+1 populations on a two-cell x–y halo
 // ((TX+4)(TY+4) cells, level 1) and the t+2 populations on a one-cell halo ((TX+2)(TY+2),
-// level 2), each held as the 24 values per cell of the two-step kernel.  HBM sees one read
-// and one write per population per three steps; the extra evaluations are the two halos and
-// the two planes below and above the z-range at each level.  Per marched plane s:
+// level 2), grouped by c_z as in the two-step kernel.  HBM sees one read and one write per
+// population per three steps; the extra evaluations are the two halos and the two planes
+// below and above the z-range at each level.  Per marched plane s (walled lattices):
 //   P1  t+1 on plane s, level-1 tile, pulled from the t field in HBM: c_z = −1 → L1.C at
 //       once, c_z = 0 and +1 kept in registers;                                    barrier
 //   P2  t+2 on plane s−1, level-2 tile, pulled from L1 (c_z = +1 of s−2, 0 of s−1, −1 of s):
@@ -411,13 +411,27 @@ __global__ void __launch_bounds__(NT)
 //       stored to HBM; the values kept in P1 replace the L1 slots P2 consumed;     barrier
 //   P4  the values kept in P2 replace the L2 slots P3 consumed.
 // P1 writes only L1.C, whose readers (P2) finished before the second barrier; P4's L2
-// writes are read after the next iteration's second barrier.  Every t+1 and t+2 value
-// passes through the storage rounding, so the result is bit-identical to three single-step
-// launches in every storage.
+// writes are read after the next iteration's second barrier.
+// Periodic lattices drop the third barrier: level 2 keeps its c_z = 0 values of planes s−1
+// and s−2 in two slots, and each level writes its c_z = +1 values one plane late (held in
+// registers for a plane), so the single level-1 c_z = +1 slot holds plane s−2 when P2
+// reads it and two level-2 slots hold planes s−3 (read by P3) and s−2 (written beside it).
+// All L1 and L2 writes then follow P3 in the same barrier interval, and the threads with
+// no P3 cell (and the waves that finish P3 first) run into the next plane's P1 while P3 is
+// still being evaluated.  Every t+1 and t+2 value passes through the storage
+// rounding, so the result is bit-identical to three single-step launches in every storage.
 // NT threads per work-group (at least TX·TY): with NT = TX·TY a thread evaluates up to two
 // level-1 and two level-2 cells one after the other; with NT ≥ (TX+4)(TY+4) every cell of a
 // level has its own thread, which halves the registers a thread holds (more resident waves
 // per CU) and runs the halo evaluations side by side; threads beyond TX·TY idle in P3.
+// LDS values of one three-step work-group: periodic, 19 per level-1 cell and 33 per level-2
+// cell (c_z = 0 and +1 of level 2 in two slots each); walled, 29 per cell of both levels.
+template <int BND>
+constexpr int march3_lds_values(int TX, int TY) {
+  const int n1 = (TX + 4) * (TY + 4), n2 = (TX + 2) * (TY + 2);
+  return BND ? 29 * (n1 + n2) : 19 * n1 + 33 * n2;
+}
+
 template <int ST, int COLL, int TX, int TY, int NT, bool NTS, int BND>
 KMI_DEV void march3(const StepArgs& a, int zlen, int tx, int ty, int zc, typename Store<ST>::real* lds) {
   using S = Store<ST>;
@@ -429,15 +443,20 @@ KMI_DEV void march3(const StepArgs& a, int zlen, int tx, int ty, int zc, typenam
   static_assert(NT >= NI && NT % 64 == 0, "one thread per interior cell at least");
   constexpr int EPT1 = (N1 + NT - 1) / NT, EPT2 = (N2 + NT - 1) / NT;
   constexpr unsigned ES = sizeof(E);
-  constexpr int NC = BND ? 2 : 1;  // c_z = −1 planes per level (walled: the own plane too)
-  constexpr int VPC = 19 + 5 * NC;  // values per cell and level
-  R* const A1 = lds;             // c_z = 0:  [9][N1]
-  R* const B1 = lds + 9 * N1;    // c_z = +1: [2 planes][5][N1]
-  R* const C1 = lds + 19 * N1;   // c_z = −1: [NC planes][5][N1]
-  R* const L2 = lds + VPC * N1;
-  R* const A2 = L2;              // [9][N2]
-  R* const B2 = L2 + 9 * N2;     // [2][5][N2]
-  R* const C2 = L2 + 19 * N2;    // [NC][5][N2]
+  // Periodic lattices march with two barriers per plane (layout march3_lds_values<0>):
+  // level 1 keeps one slot per set (its c_z = +1 values are written one plane late, see
+  // below), level 2 two c_z = 0 slots and two c_z = +1 slots.  Walled lattices keep the
+  // three-barrier schedule and 24 values per cell and level, plus a second c_z = −1 plane
+  // (the cell's own plane feeds its bounced links).
+  constexpr bool TWO = BND == 0;
+  constexpr int NC = BND ? 2 : 1;  // c_z = −1 planes per level
+  R* const A1 = lds;                         // c_z = 0:  [9][N1]
+  R* const B1 = lds + 9 * N1;                // c_z = +1: [TWO ? 1 : 2 planes][5][N1]
+  R* const C1 = B1 + (TWO ? 5 : 10) * N1;    // c_z = −1: [NC planes][5][N1]
+  R* const L2 = C1 + 5 * NC * N1;
+  R* const A2 = L2;                          // [TWO ? 2 : 1 planes][9][N2]
+  R* const B2 = A2 + (TWO ? 18 : 9) * N2;    // [2][5][N2]
+  R* const C2 = B2 + 10 * N2;                // [NC][5][N2]
 
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const long long P = a.plane;
@@ -491,6 +510,8 @@ KMI_DEV void march3(const StepArgs& a, int zlen, int tx, int ty, int zc, typenam
     }
   };
   if constexpr (kPrefetch) pull_plane(z0 - 2);
+  // two-barrier schedule: the c_z = +1 values of the previous plane, held one plane longer
+  R k1q[TWO ? EPT1 : 1][5] = {}, k2q[TWO ? EPT2 : 1][5] = {};
 
   for (int s = z0 - 2; s <= z1 + 1; ++s) {
     const int u = s + 4;  // non-negative plane counter for the ring slots
@@ -528,8 +549,8 @@ KMI_DEV void march3(const StepArgs& a, int zlen, int tx, int ty, int zc, typenam
     // ---- P2: t+2 on plane s−1, level-2 tile ----
     R k2z[EPT2][9], k2p[EPT2][5];
     if (s >= z0) {
-      const R* const B1r = B1 + ((u - 2) & 1) * 5 * N1;  // c_z = +1 of plane s−2
-      const R* const B1o = B1 + ((u - 1) & 1) * 5 * N1;  // c_z = +1 of plane s−1 (own)
+      const R* const B1r = B1 + (TWO ? 0 : ((u - 2) & 1) * 5 * N1);  // c_z = +1 of plane s−2
+      const R* const B1o = B1 + ((u - 1) & 1) * 5 * N1;  // c_z = +1 of plane s−1 (own; walled only)
       const R* const C1o = C1 + (BND ? ((u - 1) & 1) : 0) * 5 * N1;  // c_z = −1 of plane s−1 (own)
 #pragma unroll
       for (int r = 0; r < EPT2; ++r) {
@@ -555,8 +576,9 @@ KMI_DEV void march3(const StepArgs& a, int zlen, int tx, int ty, int zc, typenam
       }
     }
     __syncthreads();
-    // ---- P3: t+3 on plane s−2, interior; then the P1 values into level 1 ----
+    // ---- P3: t+3 on plane s−2, interior (walled: then the P1 values into level 1) ----
     if (s >= z0 + 2 && tid < NI) {
+      const R* const A2r = A2 + (TWO ? ((u - 2) & 1) * 9 * N2 : 0);  // c_z = 0 of plane s−2
       const R* const B2r = B2 + ((u - 3) & 1) * 5 * N2;  // c_z = +1 of plane s−3
       const R* const B2o = B2 + ((u - 2) & 1) * 5 * N2;  // c_z = +1 of plane s−2 (own)
       const R* const C2o = C2 + (BND ? ((u - 2) & 1) : 0) * 5 * N2;  // c_z = −1 of plane s−2 (own)
@@ -564,7 +586,7 @@ KMI_DEV void march3(const StepArgs& a, int zlen, int tx, int ty, int zc, typenam
       tb::Nbr nb{};
       if constexpr (BND) nb = tb::nbr<BND>(a, x0 + ix, y0 + iy, s - 2);
       R fp[19], fn[19];
-      pull_lds<BND>(fp, A2, B2r, C2w, B2o, C2o, N2, E2X, ix + 1, iy + 1, a, nb);
+      pull_lds<BND>(fp, A2r, B2r, C2w, B2o, C2o, N2, E2X, ix + 1, iy + 1, a, nb);
       update19<ST, COLL, BND>(fp, fn, a, x0 + ix, y0 + iy, s - 2);
       const unsigned cell = (unsigned(x0 + ix) + unsigned(nx) * unsigned(y0 + iy + ny * (s - 2))) * ES;
       sfor<19>([&](auto qc) {
@@ -572,31 +594,73 @@ KMI_DEV void march3(const StepArgs& a, int zlen, int tx, int ty, int zc, typenam
         Pops<ST>::template st<NTS>(tb::at(fo + q * P, cell), fn[q]);
       });
     }
-    {
-      R* const B1w = B1 + (u & 1) * 5 * N1;
+    if constexpr (TWO) {
+      // The values kept in P1 and P2 go to LDS after P3 (so P3, which the next barrier
+      // waits for, starts at once), into slots nobody reads until the next plane's P2
+      // (level 1) and P3 (level 2): level 1 takes plane s's c_z = 0 values and plane s−1's
+      // c_z = +1 values (P2 has read plane s−2's from that slot); level 2 takes plane s−1's
+      // c_z = 0 values and plane s−2's c_z = +1 values, in the slots P3 does not read.  The
+      // next plane's P1 writes only L1.C, whose reader (P2) finished before the last
+      // barrier, so no third barrier is needed: threads without a P3 cell go straight on to
+      // the next plane's P1.
 #pragma unroll
       for (int r = 0; r < EPT1; ++r) {
         const int e = tid + r * NT;
-        if (e < N1 && inside(x0 - 2 + e % E1X, y0 - 2 + e / E1X, s)) {
+        if (e < N1) {
 #pragma unroll
           for (int n = 0; n < 9; ++n) A1[n * N1 + e] = k1z[r][n];
 #pragma unroll
-          for (int n = 0; n < 5; ++n) B1w[n * N1 + e] = k1p[r][n];
+          for (int n = 0; n < 5; ++n) {
+            B1[n * N1 + e] = k1q[r][n];
+            k1q[r][n] = k1p[r][n];
+          }
+        }
+      }
+      if (s >= z0) {
+        R* const A2w = A2 + ((u - 1) & 1) * 9 * N2;  // plane s−1
+        R* const B2w = B2 + ((u - 2) & 1) * 5 * N2;  // plane s−2
+#pragma unroll
+        for (int r = 0; r < EPT2; ++r) {
+          const int e = tid + r * NT;
+          if (e < N2) {
+#pragma unroll
+            for (int n = 0; n < 9; ++n) A2w[n * N2 + e] = k2z[r][n];
+#pragma unroll
+            for (int n = 0; n < 5; ++n) {
+              B2w[n * N2 + e] = k2q[r][n];
+              k2q[r][n] = k2p[r][n];
+            }
+          }
         }
       }
     }
-    __syncthreads();
-    // ---- P4: the P2 values (plane s−1) into level 2 ----
-    if (s >= z0) {
-      R* const B2w = B2 + ((u - 1) & 1) * 5 * N2;
+    if constexpr (!TWO) {
+      {
+        R* const B1w = B1 + (u & 1) * 5 * N1;
 #pragma unroll
-      for (int r = 0; r < EPT2; ++r) {
-        const int e = tid + r * NT;
-        if (e < N2 && inside(x0 - 1 + e % E2X, y0 - 1 + e / E2X, s - 1)) {
+        for (int r = 0; r < EPT1; ++r) {
+          const int e = tid + r * NT;
+          if (e < N1 && inside(x0 - 2 + e % E1X, y0 - 2 + e / E1X, s)) {
 #pragma unroll
-          for (int n = 0; n < 9; ++n) A2[n * N2 + e] = k2z[r][n];
+            for (int n = 0; n < 9; ++n) A1[n * N1 + e] = k1z[r][n];
 #pragma unroll
-          for (int n = 0; n < 5; ++n) B2w[n * N2 + e] = k2p[r][n];
+            for (int n = 0; n < 5; ++n) B1w[n * N1 + e] = k1p[r][n];
+          }
+        }
+      }
+      __syncthreads();
+      // ---- P4: the P2 values (plane s−1) into level 2 ----
+      if (s >= z0) {
+        R* const B2w = B2 + ((u - 1) & 1) * 5 * N2;
+#pragma unroll
+        for (int r = 0; r < EPT2; ++r) {
+          const int e = tid + r * NT;
+          if (e < N2 && inside(x0 - 1 + e % E2X, y0 - 1 + e / E2X, s - 1)) {
+#pragma unroll
+            for (int n = 0; n < 9; ++n) A2[n * N2 + e] = k2z[r][n];
+#pragma unroll
+            for (int n = 0; n < 5; ++n) B2w[n * N2 + e] = k2p[r][n];
+          }
         }
       }
     }
@@ -608,7 +672,7 @@ template <int ST, int COLL, int TX, int TY, int NT, bool NTS, int BND>
 __global__ void __launch_bounds__(NT)
     k_step3_d3q19(const StepArgs a, int zlen, int ntx, int nty, int ntiles, int per_xcd, const int4 inner) {
   using R = typename Store<ST>::real;
-  __shared__ R lds[(19 + 5 * (BND ? 2 : 1)) * ((TX + 4) * (TY + 4) + (TX + 2) * (TY + 2))];
+  __shared__ R lds[march3_lds_values<BND>(TX, TY)];
   const int t = int(blockIdx.x % 8u) * per_xcd + int(blockIdx.x / 8u);
   if (t >= ntiles) return;  // padding groups (uniform per group, before any barrier)
   const int tx = t % ntx, ty = (t / ntx) % nty, zc = t / (ntx * nty);
@@ -629,8 +693,8 @@ __global__ void __launch_bounds__(NT)
 // (384) with one thread per extended cell.
 // Three steps per launch: 6 = 64×8, 7 = 32×8, 8 = 32×16 with one thread per interior cell;
 // 9 = 32×8 (448 threads), 10 = 32×16 (768), 11 = 64×8 (832) with one thread per level-1
-// cell; FP64 only 7 and 9 (LDS: 24 values per cell of both levels, 141 / 74 / 128 KB in
-// FP32; evaluations per update ((TX+4)(TY+4) + (TX+2)(TY+2) + TX·TY) / (3·TX·TY) = 1.29,
+// cell; FP64 only 7 and 9 (LDS of the periodic march, march3_lds_values<0>: 149 / 78 / 136 KB
+// for 64×8 / 32×8 / 32×16 in FP32, 155 KB for 32×8 in FP64; evaluations per update ((TX+4)(TY+4) + (TX+2)(TY+2) + TX·TY) / (3·TX·TY) = 1.29,
 // 1.34, 1.20).
 bool step2_tile(int tile, int st, int& TX, int& TY) {
   switch (tile) {
