@@ -578,6 +578,10 @@ KMI_DEV void march3(const StepArgs& a, int zlen, int tx, int ty, int zc, typenam
     __syncthreads();
     // ---- P3: t+3 on plane s−2, interior (walled: then the P1 values into level 1) ----
     if (s >= z0 + 2 && tid < NI) {
+      // Two-barrier schedule: these waves also evaluate the next plane's P1 before the next
+      // barrier, so their P3 issues ahead of the waves that already moved on to it (wave
+      // priority only; 0.8 % shorter launches for 64×8 / 832 threads, measured same-box).
+      if constexpr (TWO) __builtin_amdgcn_s_setprio(2);
       const R* const A2r = A2 + (TWO ? ((u - 2) & 1) * 9 * N2 : 0);  // c_z = 0 of plane s−2
       const R* const B2r = B2 + ((u - 3) & 1) * 5 * N2;  // c_z = +1 of plane s−3
       const R* const B2o = B2 + ((u - 2) & 1) * 5 * N2;  // c_z = +1 of plane s−2 (own)
@@ -593,6 +597,7 @@ KMI_DEV void march3(const StepArgs& a, int zlen, int tx, int ty, int zc, typenam
         constexpr int q = decltype(qc)::value;
         Pops<ST>::template st<NTS>(tb::at(fo + q * P, cell), fn[q]);
       });
+      if constexpr (TWO) __builtin_amdgcn_s_setprio(0);
     }
     if constexpr (TWO) {
       // The values kept in P1 and P2 go to LDS after P3 (so P3, which the next barrier
