@@ -98,12 +98,14 @@ def train(x, y, *, epochs=300, batch=32, lr=2e-3, lr_final=1e-4, seed=0, latent_
 
 
 def time_to_residual(Re, A, *, N=128, net=None, tol=1e-7, check=100, window=100, max_steps=1_000_000,
-                     storage="f32s", collision="mrt", consistent=0, backend=None):
+                     storage="f32s", collision="mrt", consistent=0, backend=None, log=None, chunk=500_000):
     """All cavities in one batch, from rest (net=None) or from the network's prediction.
     With `consistent` > 0 the predicted velocity is held for that many steps so that density
     and non-equilibrium populations settle to it (Batch.consistent_init) before the run;
-    those steps are counted.  Returns wall seconds (including inference and initialisation),
-    per-cavity steps/status and the final velocity fields."""
+    those steps are counted.  The batch advances in spans of `chunk` steps (a cavity stopped
+    at a span's end resumes in the next; `log` gets one line per span).  Returns wall seconds
+    (including inference and initialisation), per-cavity steps/status and the final velocity
+    fields."""
     backend = backend or MI355XBackend()
     nu = A * N / Re
     with Batch(np.full(len(Re), N), nu, storage=storage, collision=collision, backend=backend) as b:
@@ -118,16 +120,26 @@ def time_to_residual(Re, A, *, N=128, net=None, tol=1e-7, check=100, window=100,
             if consistent > 0:
                 b.consistent_init(consistent)
         t_init = time.perf_counter() - t0
-        ms = b.run(tol, check, max_steps, window=window)
+        ms, cap = 0.0, 0
+        while True:
+            cap = min(cap + chunk, max_steps)
+            ms += b.run(tol, check, cap, window=window)
+            st = b.status()
+            if log:
+                log(f"[batch] {'warm' if net is not None else 'rest'} {storage} r<{tol:g} {cap} steps: "
+                    f"converged {(st['status'] == 1).sum()}/{len(Re)} diverged {(st['status'] == 2).sum()} "
+                    f"t {time.perf_counter() - t0:.1f}s")
+            if cap >= max_steps or not np.any(st["status"] == 3):
+                break
+            b.resume()
         wall = time.perf_counter() - t0
-        st = b.status()
         fields = [b.moments(c) for c in range(len(Re))]
     return {"wall_s": wall, "init_s": t_init, "device_ms": ms, "steps": st["steps"], "status": st["status"],
             "residual": st["residual"], "fields": fields}
 
 
 def converged_agreement(Re, A, net, *, N=128, tol=1e-11, check=100, window=100, max_steps=3_000_000,
-                        storage="f64", consistent=0, backend=None):
+                        storage="f64", consistent=0, backend=None, log=None):
     """Whether the warm start changes the answer: the same cavities run from rest and from the
     prediction, in FP64, to a tight residual.  A field stopped at r < tol still sits about
     r·τ from the steady state (τ the slowest mode's decay time in steps), so two starts agree
@@ -135,9 +147,9 @@ def converged_agreement(Re, A, net, *, N=128, tol=1e-11, check=100, window=100, 
     Returns per-case max |Δu|/A and relative L2 difference of u over the cases converged from
     both starts, with steps and status of both runs."""
     rest = time_to_residual(Re, A, N=N, tol=tol, check=check, window=window, max_steps=max_steps,
-                            storage=storage, backend=backend)
+                            storage=storage, backend=backend, log=log)
     warm = time_to_residual(Re, A, N=N, net=net, tol=tol, check=check, window=window,
-                            max_steps=max_steps, storage=storage, consistent=consistent, backend=backend)
+                            max_steps=max_steps, storage=storage, consistent=consistent, backend=backend, log=log)
     ok = np.where((rest["status"] == 1) & (warm["status"] == 1))[0]
     du = np.array([max(float(np.max(np.abs(warm["fields"][c][k] - rest["fields"][c][k]))) for k in ("ux", "uy"))
                    / float(A[c]) for c in ok])
@@ -224,7 +236,7 @@ def experiment(*, n_train=2048, n_test=256, N=128, epochs=150, target_tol=1e-8, 
     pa = np.arange(n_test) if agree_all else (
         conv[np.linspace(0, len(conv) - 1, min(agree_sample, len(conv))).astype(int)] if len(conv) else conv)
     agree = converged_agreement(Re_te[pa], A_te[pa], net, N=N, tol=agree_tol, max_steps=agree_max_steps,
-                                backend=backend)
+                                backend=backend, log=say)
     du = agree["dU_over_A"]
     say(f"[warmstart] FP64 r<{agree_tol:g} on {len(pa)}: converged both {len(agree['index'])}, "
         f"max dU/A {du.max() if len(du) else float('nan'):.2e}")
