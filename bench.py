@@ -374,6 +374,9 @@ def main():
     ap.add_argument("--temporal", default=None,
                     help="TILE,ZCHUNKS[,NT]: fixed two-step launch (implies --no-autotune)")
     ap.add_argument("--no-variants", action="store_true", help="skip timing the other storages")
+    ap.add_argument("--walls", default="cube", choices=["cube", "z", "y"],
+                    help="cavity3d workload: run_cavity_3d's cube, or the same lattice with walls only in z "
+                         "(resting bottom, moving top wall) or only in y (other axes periodic)")
     ap.add_argument("--no-warmstart", action="store_true", help="skip the learned warm-start record")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -402,8 +405,15 @@ def main():
 
     def make(storage):
         # run_cavity_3d's cube (halfway-BB walls, Zou-He lid u = (U0, 0, 0)) or the periodic TGV
-        if cavity:
+        if cavity and args.walls == "cube":
             return kmi.cavity_3d_desc(N, N, N, nu, U0, storage=storage)
+        if cavity:  # one walled axis, the others periodic: how the tiles' wall cost splits
+            faces = {f: {"kind": "periodic"} for f in ("west", "east", "south", "north", "bottom", "top")}
+            if args.walls == "z":
+                faces.update(bottom={"kind": "wall"}, top={"kind": "moving_wall", "u": (U0, 0.0, 0.0), "rho": 1.0})
+            else:
+                faces.update(south={"kind": "wall"}, north={"kind": "wall"})
+            return kmi.make_desc("d3q19", N, N, N, nu, storage=storage, collision="bgk", faces=faces)
         return kmi.taylor_green_3d_desc(N, nu, storage)
 
     def run(storage, steps, warmup):
@@ -441,14 +451,17 @@ def main():
     # launch shape (tile, z-ranges, non-temporal stores): a record holds for its shape only.
     span = launch_span(launch)
     kernel = {3: "k_step3_d3q19", 2: "k_step2_d3q19", 1: "k_step"}[span]
-    traffic = traffic_record(kernel, args.storage, N, launch, tag="bgk_cavity" if cavity else "trt")
+    tag = "trt" if not cavity else ("bgk_cavity" if args.walls == "cube" else f"bgk_walls_{args.walls}")
+    traffic = traffic_record(kernel, args.storage, N, launch, tag=tag)
     cells = N ** 3
     mlups = cells * args.steps / wall / 1e6
     launch_s = launch_ms / 1e3
     min_bytes = cells * 19 * 2 * BYTES_PER_POP[args.storage]  # one read + one write per launch
     achieved = min_bytes / launch_s / 1e9
-    name = (f"D3Q19 BGK lid-driven cube {N}^3 (run_cavity_3d: halfway-BB walls, Zou-He lid U0={U0}), "
-            f"nu={nu}, rest start" if cavity else
+    layout = {"cube": f"lid-driven cube {N}^3 (run_cavity_3d: halfway-BB walls, Zou-He lid U0={U0})",
+              "z": f"{N}^3 walls in z only (resting bottom, top wall moving at U0={U0}; x, y periodic)",
+              "y": f"{N}^3 walls in y only (x, z periodic)"}[args.walls]
+    name = (f"D3Q19 BGK {layout}, nu={nu}, rest start" if cavity else
             f"D3Q19 TRT Lambda=3/16 triply periodic Taylor-Green {N}^3, U0={U0}, nu={nu}")
     out = {
         "metric": (("MLUPS (D3Q19 BGK 256^3 walled cavity, " if cavity else "MLUPS (D3Q19 TRT 256^3 Taylor-Green, ")

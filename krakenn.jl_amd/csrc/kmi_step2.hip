@@ -48,7 +48,9 @@ KMI_DEV int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 // Walled lattices: the kernels' template flag BND is 0 for a fully periodic lattice, 1 for
 // the tiles of a walled lattice whose cells at every level stay at least one cell away from
 // every x and y wall (only z walls remain, and those are uniform across a plane), 2 for the
-// other tiles (any wall, Zou–He face or solid cell).
+// other tiles (any wall, Zou–He face or solid cell), 3 for those tiles when the lattice has
+// no solid mask and its x and y faces are periodic or resting walls (x–y walls bounce links
+// back with no Ladd term and carry no Zou–He closure; the z faces are handled as in mode 1).
 // Whether a cell's neighbour exists along each axis direction — a periodic axis always wraps;
 // a wall face has none beyond it.
 struct Nbr {
@@ -79,18 +81,19 @@ KMI_DEV bool src_ok(const Nbr& n) {
 }
 // Ladd term of every moving face the bounced link q crosses, x then y then z: the
 // single-step kernel's pull, operation for operation.
-template <int q, class R>
+template <int q, int BND, class R>
 KMI_DEV R ladd(R v, const StepArgs& a, const Nbr& n) {
   constexpr int cx = Lat<19>::cx[q], cy = Lat<19>::cy[q], cz = Lat<19>::cz[q];
+  constexpr bool XY = BND == 2;  // mode 3: resting x–y walls, mode 1: none in reach
   auto add = [&](int fid) {
     const FaceArgs& F = a.face[fid];
     if (F.kind == KMI_FACE_MOVING_WALL)
       v = v + R(six_w(Lat<19>::w[q])) * R(F.rho) * cdotu<cx, cy, cz>(R(F.u[0]), R(F.u[1]), R(F.u[2]));
   };
-  if (cx > 0 && !n.xm) add(KMI_WEST);
-  if (cx < 0 && !n.xp) add(KMI_EAST);
-  if (cy > 0 && !n.ym) add(KMI_SOUTH);
-  if (cy < 0 && !n.yp) add(KMI_NORTH);
+  if (XY && cx > 0 && !n.xm) add(KMI_WEST);
+  if (XY && cx < 0 && !n.xp) add(KMI_EAST);
+  if (XY && cy > 0 && !n.ym) add(KMI_SOUTH);
+  if (XY && cy < 0 && !n.yp) add(KMI_NORTH);
   if (cz > 0 && !n.zm) add(KMI_BOTTOM);
   if (cz < 0 && !n.zp) add(KMI_TOP);
   return v;
@@ -161,8 +164,9 @@ KMI_DEV void collide19(const R (&fp)[19], R (&fo)[19], const StepArgs& a) {
 // the configured order), then the solid-cell rule or the collision.
 template <int ST, int COLL, int BND, class R>
 KMI_DEV void update19(R (&fp)[19], R (&fo)[19], const StepArgs& a, int gx, int gy, int gz) {
-  if constexpr (BND == 1) {
-    // interior tiles: only the z faces can hold the cell, and no solid mask exists
+  if constexpr (BND == 1 || BND == 3) {
+    // interior tiles, or a lattice whose x–y faces are periodic or resting walls: only the
+    // z faces can carry a Zou–He closure, and no solid mask exists
     constexpr bool SH = Store<ST>::shifted;
     for (int n = 0; n < a.n_zh; ++n) {
       const int fid = a.zh_face[n];
@@ -222,7 +226,7 @@ KMI_DEV void pull_lds(R (&fp)[19], const R* A, const R* Bs, const R* Cs, const R
       else if constexpr (cz > 0) v = Co[no * NEp + own];  // opp(q) has c_z = −1
       else v = Bo[no * NEp + own];
       if constexpr (q != 0)
-        if (a.any_moving) v = tb::ladd<q>(v, a, nb);
+        if (a.any_moving) v = tb::ladd<q, BND>(v, a, nb);
       fp[q] = v;
     }
   });
@@ -299,7 +303,7 @@ KMI_DEV void march2(const StepArgs& a, int zlen, int tx, int ty, int zc, typenam
           } else if constexpr (BND) {
             R v = Pops<ST>::ld(&tb::at(fi + Lat<19>::opp[q] * P, own));
             if constexpr (q != 0)
-              if (a.any_moving) v = tb::ladd<q>(v, a, nb);
+              if (a.any_moving) v = tb::ladd<q, BND>(v, a, nb);
             nxt[r][q] = v;
           }
         });
@@ -378,8 +382,9 @@ KMI_DEV void march2(const StepArgs& a, int zlen, int tx, int ty, int zc, typenam
 // Grid: one work-group per (column tile, z-range), padded to a multiple of 8.  Work-group
 // b runs on XCD b mod 8; it takes tile (b mod 8)·per_xcd + b/8, so each XCD works through a
 // contiguous band of neighbouring tiles and their shared halo rows meet in its L2.  Walled
-// lattices (BND = 2): the tiles of the block `inner` ([x, y) × [z, w) in tile units) march on
-// the interior variant (BND = 1), the others on the full one.
+// lattices (BND = 2, or 3 when the x–y faces are periodic or resting walls and no mask
+// exists): the tiles of the block `inner` ([x, y) × [z, w) in tile units) march on the
+// interior variant (BND = 1), the others on the walled one.
 template <int ST, int COLL, int TX, int TY, int NT, bool NTS, int BND>
 __global__ void __launch_bounds__(NT)
     k_step2_d3q19(const StepArgs a, int zlen, int ntx, int nty, int ntiles, int per_xcd, const int4 inner) {
@@ -388,7 +393,7 @@ __global__ void __launch_bounds__(NT)
   const int t = int(blockIdx.x % 8u) * per_xcd + int(blockIdx.x / 8u);
   if (t >= ntiles) return;  // padding groups (uniform per group, before any barrier)
   const int tx = t % ntx, ty = (t / ntx) % nty, zc = t / (ntx * nty);
-  if constexpr (BND == 2) {
+  if constexpr (BND >= 2) {
     if (tx >= inner.x && tx < inner.y && ty >= inner.z && ty < inner.w) {
       march2<ST, COLL, TX, TY, NT, NTS, 1>(a, zlen, tx, ty, zc, lds);
       return;
@@ -502,7 +507,7 @@ KMI_DEV void march3(const StepArgs& a, int zlen, int tx, int ty, int zc, typenam
           } else if constexpr (BND) {
             R v = Pops<ST>::ld(&tb::at(fi + Lat<19>::opp[q] * P, own));
             if constexpr (q != 0)
-              if (a.any_moving) v = tb::ladd<q>(v, a, nb);
+              if (a.any_moving) v = tb::ladd<q, BND>(v, a, nb);
             nxt[r][q] = v;
           }
         });
@@ -681,7 +686,7 @@ __global__ void __launch_bounds__(NT)
   const int t = int(blockIdx.x % 8u) * per_xcd + int(blockIdx.x / 8u);
   if (t >= ntiles) return;  // padding groups (uniform per group, before any barrier)
   const int tx = t % ntx, ty = (t / ntx) % nty, zc = t / (ntx * nty);
-  if constexpr (BND == 2) {
+  if constexpr (BND >= 2) {
     if (tx >= inner.x && tx < inner.y && ty >= inner.z && ty < inner.w) {
       march3<ST, COLL, TX, TY, NT, NTS, 1>(a, zlen, tx, ty, zc, lds);
       return;
@@ -744,6 +749,15 @@ static void interior_tiles(bool per, int n, int T, int H, int ntiles, int& lo, i
   if (hi <= lo) lo = hi = 0;
 }
 
+// The walled kernels' lean ring variant (BND = 3) applies: no solid mask, and every x and y
+// face periodic or a resting wall.
+static bool lean_walls(const StepArgs& a) {
+  if (a.mask != nullptr) return false;
+  for (int f : {KMI_WEST, KMI_EAST, KMI_SOUTH, KMI_NORTH})
+    if (a.face[f].kind != KMI_FACE_PERIODIC && a.face[f].kind != KMI_FACE_WALL) return false;
+  return true;
+}
+
 // One multi-step pass: every column tile × z-range in one launch, on the periodic kernel
 // (BND = 0) or, for a walled lattice, the walled one (BND = 2) with the interior block of
 // tiles (none when the lattice has a solid mask) marching on the interior variant.
@@ -768,11 +782,16 @@ static hipError_t launch2(bool nts, bool bnd, const StepArgs& a, int zchunks, hi
                       [&](dim3 g, int zlen, int ntx, int nty, int ntiles, int per_xcd, int4 inner) -> hipError_t {
                         const dim3 b(static_cast<unsigned>(NT));
                         if (bnd) {
-                          if constexpr (ST != KMI_STORE_F16S && (NT == 704 || NT == 384))
-                            hipLaunchKernelGGL((k_step2_d3q19<ST, COLL, TX, TY, NT, true, 2>), g, b, 0, s, a, zlen,
-                                               ntx, nty, ntiles, per_xcd, inner);
-                          else
+                          if constexpr (ST != KMI_STORE_F16S && (NT == 704 || NT == 384)) {
+                            if (lean_walls(a))
+                              hipLaunchKernelGGL((k_step2_d3q19<ST, COLL, TX, TY, NT, true, 3>), g, b, 0, s, a,
+                                                 zlen, ntx, nty, ntiles, per_xcd, inner);
+                            else
+                              hipLaunchKernelGGL((k_step2_d3q19<ST, COLL, TX, TY, NT, true, 2>), g, b, 0, s, a,
+                                                 zlen, ntx, nty, ntiles, per_xcd, inner);
+                          } else {
                             return hipErrorInvalidValue;
+                          }
                         } else if (nts) {
                           hipLaunchKernelGGL((k_step2_d3q19<ST, COLL, TX, TY, NT, true, 0>), g, b, 0, s, a, zlen, ntx,
                                              nty, ntiles, per_xcd, inner);
@@ -790,11 +809,16 @@ static hipError_t launch3(bool nts, bool bnd, const StepArgs& a, int zchunks, hi
                       [&](dim3 g, int zlen, int ntx, int nty, int ntiles, int per_xcd, int4 inner) -> hipError_t {
                         const dim3 b(static_cast<unsigned>(NT));
                         if (bnd) {
-                          if constexpr (ST != KMI_STORE_F16S && ST != KMI_STORE_F64 && NT == 448)
-                            hipLaunchKernelGGL((k_step3_d3q19<ST, COLL, TX, TY, NT, true, 2>), g, b, 0, s, a, zlen,
-                                               ntx, nty, ntiles, per_xcd, inner);
-                          else
+                          if constexpr (ST != KMI_STORE_F16S && ST != KMI_STORE_F64 && NT == 448) {
+                            if (lean_walls(a))
+                              hipLaunchKernelGGL((k_step3_d3q19<ST, COLL, TX, TY, NT, true, 3>), g, b, 0, s, a,
+                                                 zlen, ntx, nty, ntiles, per_xcd, inner);
+                            else
+                              hipLaunchKernelGGL((k_step3_d3q19<ST, COLL, TX, TY, NT, true, 2>), g, b, 0, s, a,
+                                                 zlen, ntx, nty, ntiles, per_xcd, inner);
+                          } else {
                             return hipErrorInvalidValue;
+                          }
                         } else if (nts) {
                           hipLaunchKernelGGL((k_step3_d3q19<ST, COLL, TX, TY, NT, true, 0>), g, b, 0, s, a, zlen, ntx,
                                              nty, ntiles, per_xcd, inner);

@@ -1,6 +1,7 @@
 """The benchmark script's own bookkeeping (no GPU): steps per launch of a launch record, the
 decoder's FLOP count, the traffic records it reads, and its refusal of several GPUs."""
 import json
+import re
 import os
 import subprocess
 import sys
@@ -57,11 +58,13 @@ def test_traffic_record_holds_for_its_launch_shape_only():
     for key, entry in recs.items():
         if not key.startswith(("k_step2", "k_step3")):
             continue
-        kernel, storage = key.rsplit("_trt_", 1)[0], key.rsplit("_", 2)[-2]
+        m = re.match(r"^(k_step[23]_d3q19)_(.+)_(f64|f32s|f32|f16s)_(\d+)$", key)
+        assert m, key
+        kernel, tag, storage, n = m.group(1), m.group(2), m.group(3), int(m.group(4))
         for rec in entry if isinstance(entry, list) else [entry]:
             shape = dict(rec["launch"])
-            assert bench.traffic_record(kernel, storage, 256, shape) is not None, key
+            assert bench.traffic_record(kernel, storage, n, shape, tag=tag) is not None, key
             flipped = {**shape, "nontemporal": not shape["nontemporal"]}
-            got = bench.traffic_record(kernel, storage, 256, flipped)
+            got = bench.traffic_record(kernel, storage, n, flipped, tag=tag)
             others = [r for r in (entry if isinstance(entry, list) else [entry]) if r["launch"] == flipped]
             assert got == (others[0]["bytes_per_launch"] if others else None), key

@@ -361,6 +361,13 @@ def _walled_3d_cases(kmi, storage, collision, shape):
     yield "zh_channel_inert_ball", kmi.make_desc("d3q19", *shape, 0.04, storage=storage, collision=collision,
                                                  faces=chan, solid_mode="inert"), ball
     yield "zh_channel", kmi.make_desc("d3q19", *shape, 0.04, storage=storage, collision=collision, faces=chan), None
+    # resting x walls, periodic y, a moving z wall: the ring tiles take the lean variant
+    # (no mask, x–y faces periodic or resting) with the z wall's Ladd term
+    box = {"west": {"kind": "wall"}, "east": {"kind": "wall"}, "south": {"kind": "periodic"},
+           "north": {"kind": "periodic"}, "bottom": {"kind": "wall"},
+           "top": {"kind": "moving_wall", "u": (0.04, 0.02, 0.0), "rho": 1.0}}
+    yield "x_walls_ladd_z", kmi.make_desc("d3q19", *shape, 0.03, storage=storage, collision=collision,
+                                          faces=box), None
     # walls only in z (a Couette gap): every tile takes the interior variant
     couette = {"west": {"kind": "periodic"}, "east": {"kind": "periodic"}, "south": {"kind": "periodic"},
                "north": {"kind": "periodic"}, "bottom": {"kind": "wall"},
@@ -376,7 +383,9 @@ def test_walled_multi_step_launches_bit_identical(kmi, backend, storage, collisi
     with periodic axes) reproduce single-step launches bit for bit: the bounced links of the
     LDS levels take the cell's own opposite population of the previous level.  On the
     192 × 32 lattice the tiles clear of the x and y walls run the interior variant (z walls
-    only) and the rest the full one; lattices with a mask run the full one throughout."""
+    only) and the rest the full one, or its lean form when the lattice has no mask and its x
+    and y faces are periodic or resting walls (run_cavity_3d's cube, the x-walled box);
+    lattices with a mask run the full one throughout."""
     for shape in ((64, 16, 12), (32, 8, 5), (192, 32, 10)):
         for name, desc, mask in _walled_3d_cases(kmi, storage, collision, shape):
             f = rng_pops(shape, 19, seed=8, amp=0.05)
