@@ -44,6 +44,10 @@ struct kmi_lattice {
   bool bnd = true;
   int bx = 0, by = 1, nts = 0;  // launch shape and store policy (0: not chosen yet)
   int tb_tile = 0, tb_zchunks = 1;  // two-step launches (0: off), see kmi_step2.hip
+  // work-group → tile table of the walled multi-step launch (device), built for order_key
+  int* order = nullptr;
+  size_t order_cap = 0;
+  int order_key[2] = {0, 0};
   dim3 grid, block;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   hipEvent_t ev[2] = {nullptr, nullptr};
@@ -449,6 +453,12 @@ static int launch_one(kmi_lattice* L, hipStream_t s, bool multi = false) {
   a.gi = L->g[L->cur];
   a.go = L->g[L->cur ^ 1];
   a.mask = L->mask;
+  // the table only for the tile and z-split it was built for (any other launch keeps the
+  // default order, which covers every tile as well)
+  a.tile_order = multi && L->bnd && L->Q == 19 && L->order && L->order_key[0] == L->tb_tile &&
+                         L->order_key[1] == L->tb_zchunks
+                     ? L->order
+                     : nullptr;
   hipError_t e =
       multi ? (L->Q == 9 ? launch_step_tb2d(L->desc.storage, L->desc.collision, L->tb_tile,
                                             L->desc.thermal != 0, a, s)
@@ -470,9 +480,37 @@ static void drop_graphs(kmi_lattice* L) {
 }
 
 // graph_steps(L) steps captured once per buffer parity and replayed.
+// The walled D3Q19 multi-step launch's work-group order (walled_tile_order) for the current
+// tile and z-split, rebuilt when those change — before any graph capture, since the upload
+// is an ordinary copy on the lattice stream.
+static int refresh_tile_order(kmi_lattice* L) {
+  if (!(L->bnd && L->Q == 19 && L->tb_tile)) return KMI_OK;
+  if (L->order && L->order_key[0] == L->tb_tile && L->order_key[1] == L->tb_zchunks) return KMI_OK;
+  std::vector<int> h;
+  walled_tile_order(L->args, L->tb_tile, L->tb_zchunks, h);
+  if (h.empty()) return fail(L->ctx, KMI_ERR_ARG, "walled multi-step launch: unknown tile");
+  const size_t bytes = h.size() * sizeof(int);
+  if (bytes > L->order_cap) {
+    if (L->order) {
+      KMI_HIP(L->ctx, hipStreamSynchronize(L->ctx->stream));
+      (void)hipFree(L->order);
+      L->order = nullptr;
+      L->order_cap = 0;
+    }
+    KMI_HIP(L->ctx, hipMalloc(&L->order, bytes));
+    L->order_cap = bytes;
+  }
+  KMI_HIP(L->ctx, hipMemcpyAsync(L->order, h.data(), bytes, hipMemcpyHostToDevice, L->ctx->stream));
+  KMI_HIP(L->ctx, hipStreamSynchronize(L->ctx->stream));  // h leaves scope
+  L->order_key[0] = L->tb_tile;
+  L->order_key[1] = L->tb_zchunks;
+  return KMI_OK;
+}
+
 static int run_steps(kmi_lattice* L, int64_t n) {
   kmi_ctx* c = L->ctx;
   hipStream_t s = c->stream;
+  if (int rc = refresh_tile_order(L); rc != KMI_OK) return rc;
   const int span = launch_span(L);
   const int gsteps = graph_steps(L);
   while (n >= gsteps) {
@@ -614,6 +652,7 @@ int kmi_lattice_destroy(kmi_lattice* L) {
     if (p) (void)hipFree(p);
   if (L->ffield) (void)hipFree(L->ffield);
   if (L->stage) (void)hipFree(L->stage);
+  if (L->order) (void)hipFree(L->order);
   delete L;
   return KMI_OK;
 }

@@ -239,6 +239,9 @@ struct StepArgs {
   // record in cut_rec: Q wall fractions (0: link not cut), then Q × 3 wall velocities.
   const int* cut_slot;
   const double* cut_rec;
+  // Walled multi-step launches: work-group b marches tile tile_order[b] (-1: padding);
+  // nullptr: the XCD-banded order of the periodic launches.
+  const int* tile_order;
   FaceArgs face[6];
 };
 

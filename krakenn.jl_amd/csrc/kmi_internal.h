@@ -2,6 +2,7 @@
 #pragma once
 
 #include <string>
+#include <vector>
 
 #include "kmi_common.h"
 
@@ -30,6 +31,8 @@ int temporal_span(int tile);
 // bnd: a lattice with walls, moving walls, Zou–He faces or a solid mask (step2_tile_walled
 // tiles only; non-temporal stores whatever `nts` says).
 bool step2_tile_walled(int tile, int st);
+// Work-group → tile table of a walled multi-step launch (ring tiles first in each XCD band).
+void walled_tile_order(const StepArgs& a, int tile, int zchunks, std::vector<int>& order);
 hipError_t launch_step2(int st, int coll, int tile, int zchunks, bool nts, bool bnd, const StepArgs& a,
                         hipStream_t s);
 // K steps per launch on a D2Q9 lattice (kmi_step_tb2d.hip): `tile` selects tile and K

@@ -24,6 +24,9 @@
 // The arithmetic per cell and step is the single-step kernel's (same collide functions,
 // -ffp-contract=off), and the t+1 values pass through the storage rounding before they
 // enter LDS, so the result is bit-identical to two single-step launches in every storage.
+#include <algorithm>
+#include <vector>
+
 #include "kmi_cell.h"
 
 namespace kmi {
@@ -405,8 +408,10 @@ __global__ void __launch_bounds__(NT)
     k_step2_d3q19(const StepArgs a, int zlen, int ntx, int nty, int ntiles, int per_xcd, const int4 inner) {
   using R = typename Store<ST>::real;
   __shared__ R lds[(19 + 5 * (BND ? 2 : 1)) * (TX + 2) * (TY + 2)];
-  const int t = int(blockIdx.x % 8u) * per_xcd + int(blockIdx.x / 8u);
-  if (t >= ntiles) return;  // padding groups (uniform per group, before any barrier)
+  int t = int(blockIdx.x % 8u) * per_xcd + int(blockIdx.x / 8u);
+  if constexpr (BND >= 2)
+    if (a.tile_order) t = a.tile_order[blockIdx.x];
+  if (t < 0 || t >= ntiles) return;  // padding groups (uniform per group, before any barrier)
   const int tx = t % ntx, ty = (t / ntx) % nty, zc = t / (ntx * nty);
   if constexpr (BND >= 2) {
     if (tx >= inner.x && tx < inner.y && ty >= inner.z && ty < inner.w) {
@@ -698,8 +703,10 @@ __global__ void __launch_bounds__(NT)
     k_step3_d3q19(const StepArgs a, int zlen, int ntx, int nty, int ntiles, int per_xcd, const int4 inner) {
   using R = typename Store<ST>::real;
   __shared__ R lds[march3_lds_values<BND>(TX, TY)];
-  const int t = int(blockIdx.x % 8u) * per_xcd + int(blockIdx.x / 8u);
-  if (t >= ntiles) return;  // padding groups (uniform per group, before any barrier)
+  int t = int(blockIdx.x % 8u) * per_xcd + int(blockIdx.x / 8u);
+  if constexpr (BND >= 2)
+    if (a.tile_order) t = a.tile_order[blockIdx.x];
+  if (t < 0 || t >= ntiles) return;  // padding groups (uniform per group, before any barrier)
   const int tx = t % ntx, ty = (t / ntx) % nty, zc = t / (ntx * nty);
   if constexpr (BND >= 2) {
     if (tx >= inner.x && tx < inner.y && ty >= inner.z && ty < inner.w) {
@@ -789,6 +796,39 @@ static hipError_t launch_tiles(bool bnd, int span, int TX, int TY, const StepArg
     interior_tiles(a.per[1] != 0, a.ny, TY, span - 1, nty, inner.z, inner.w);
   }
   return go(dim3(static_cast<unsigned>(8 * per_xcd)), zlen, ntx, nty, ntiles, per_xcd, inner);
+}
+
+// Work-group order of a walled multi-step launch: each XCD's band of tiles (the same bands
+// as the periodic launch) with its ring tiles — the slower walled variant — first, so that
+// when the groups outnumber the CUs the interior groups fill the rounds the ring groups
+// leave (longest first).  order[b] = tile of work-group b, -1 for the padding groups.
+void walled_tile_order(const StepArgs& a, int tile, int zchunks, std::vector<int>& order) {
+  int TX = 0, TY = 0;
+  order.clear();
+  if (!step2_tile(tile, KMI_STORE_F32, TX, TY)) return;
+  const int span = temporal_span(tile);
+  const int ntx = a.nx / TX, nty = a.ny / TY;
+  const int zlen = (a.nz + zchunks - 1) / zchunks;
+  const int ntiles = ntx * nty * ((a.nz + zlen - 1) / zlen);
+  const int per_xcd = (ntiles + 7) / 8;
+  int ix0 = 0, ix1 = 0, iy0 = 0, iy1 = 0;
+  if (a.mask == nullptr) {
+    interior_tiles(a.per[0] != 0, a.nx, TX, span - 1, ntx, ix0, ix1);
+    interior_tiles(a.per[1] != 0, a.ny, TY, span - 1, nty, iy0, iy1);
+  }
+  order.assign(size_t(8) * per_xcd, -1);
+  std::vector<int> band;
+  for (int x = 0; x < 8; ++x) {
+    band.clear();
+    const int t0 = x * per_xcd, t1 = std::min(t0 + per_xcd, ntiles);
+    for (int pass = 0; pass < 2; ++pass)  // ring tiles, then interior tiles
+      for (int t = t0; t < t1; ++t) {
+        const int tx = t % ntx, ty = (t / ntx) % nty;
+        const bool inner = tx >= ix0 && tx < ix1 && ty >= iy0 && ty < iy1;
+        if (inner == (pass == 1)) band.push_back(t);
+      }
+    for (size_t i = 0; i < band.size(); ++i) order[size_t(x) + 8 * i] = band[i];
+  }
 }
 
 template <int ST, int COLL, int TX, int TY, int NT = TX * TY>
