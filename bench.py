@@ -354,7 +354,7 @@ def main():
                     choices=["tgv", "cavity3d", "cavities", "warmstart", "cavity", "rb", "nn"])
     ap.add_argument("--ws-train", type=int, default=2048)
     ap.add_argument("--ws-test", type=int, default=256)
-    ap.add_argument("--ws-epochs", type=int, default=150)
+    ap.add_argument("--ws-epochs", type=int, default=1000)
     ap.add_argument("--ws-plain", type=int, default=32, help="plain rest runs timed (sample of the converging set)")
     ap.add_argument("--ws-agree", type=int, default=16, help="cases run in FP64 to a tight residual from both starts")
     ap.add_argument("--ws-agree-tol", type=float, default=2e-13)

@@ -199,7 +199,7 @@ def plain_time_to_residual(Re, A, *, N=128, tol=1e-7, check=100, window=100, max
     return {"wall_s": np.array(walls), "steps": np.array(steps), "status": np.array(status)}
 
 
-def experiment(*, n_train=2048, n_test=256, N=128, epochs=150, target_tol=1e-8, tol=1e-7,
+def experiment(*, n_train=2048, n_test=256, N=128, epochs=1000, target_tol=1e-8, tol=1e-7,
                plain_sample=32, agree_sample=16, agree_tol=2e-13, agree_max_steps=8_000_000,
                agree_all=False, backend=None, log=None):
     """The learned warm start end to end: steady states of `n_train` cavities from the batched
