@@ -409,9 +409,14 @@ __global__ void __launch_bounds__(NT)
   using R = typename Store<ST>::real;
   __shared__ R lds[(19 + 5 * (BND ? 2 : 1)) * (TX + 2) * (TY + 2)];
   int t = int(blockIdx.x % 8u) * per_xcd + int(blockIdx.x / 8u);
-  if constexpr (BND >= 2)
+  if constexpr (BND >= 2) {
     if (a.tile_order) t = a.tile_order[blockIdx.x];
-  if (t < 0 || t >= ntiles) return;  // padding groups (uniform per group, before any barrier)
+    if (t < 0) return;  // padding groups of the table
+  }
+  // padding groups (uniform per group, before any barrier); the periodic kernels' prologue
+  // is kept to this one test: a longer one shifts the plane loop's code alignment, which
+  // cost the 64×8 / 832-thread three-step launch 3 %
+  if (t >= ntiles) return;
   const int tx = t % ntx, ty = (t / ntx) % nty, zc = t / (ntx * nty);
   if constexpr (BND >= 2) {
     if (tx >= inner.x && tx < inner.y && ty >= inner.z && ty < inner.w) {
@@ -704,9 +709,14 @@ __global__ void __launch_bounds__(NT)
   using R = typename Store<ST>::real;
   __shared__ R lds[march3_lds_values<BND>(TX, TY)];
   int t = int(blockIdx.x % 8u) * per_xcd + int(blockIdx.x / 8u);
-  if constexpr (BND >= 2)
+  if constexpr (BND >= 2) {
     if (a.tile_order) t = a.tile_order[blockIdx.x];
-  if (t < 0 || t >= ntiles) return;  // padding groups (uniform per group, before any barrier)
+    if (t < 0) return;  // padding groups of the table
+  }
+  // padding groups (uniform per group, before any barrier); the periodic kernels' prologue
+  // is kept to this one test: a longer one shifts the plane loop's code alignment, which
+  // cost the 64×8 / 832-thread three-step launch 3 %
+  if (t >= ntiles) return;
   const int tx = t % ntx, ty = (t / ntx) % nty, zc = t / (ntx * nty);
   if constexpr (BND >= 2) {
     if (tx >= inner.x && tx < inner.y && ty >= inner.z && ty < inner.w) {
