@@ -1,4 +1,6 @@
-// kmi_step2.hip — two or three time steps per launch for fully periodic D3Q19 lattices.
+// kmi_step2.hip — two or three time steps per launch for D3Q19 BGK/TRT lattices: fully
+// periodic ones (below), and walled ones (halfway and Ladd walls, Zou–He faces, solid masks;
+// the BND variants further down), with the three-step march after the two-step one.
 //
 // A single-step kernel reads and writes every population once per step: 2·19·sizeof(elem)
 // bytes of HBM traffic per cell update, and no kernel of that shape can beat the device's
