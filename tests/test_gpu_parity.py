@@ -425,6 +425,25 @@ def test_walled_multi_step_launches_bit_identical(kmi, backend, storage, collisi
             lat.set_temporal(14, 1)
 
 
+def test_walled_cube_256_bench_launches_bit_identical(kmi, backend):
+    """The walled benchmark lattice itself: run_cavity_3d's cube at 256³ (F32S, BGK, Zou–He
+    lid, rest start) on the lean ring variant with ring tiles handed out first (64×8 with 6
+    z-ranges, 768 groups in three rounds; 32×8 with 3 z-ranges) and on the three-step 32×8
+    tile equals single-step launches bit for bit after 30 steps."""
+    N = 256
+    desc = kmi.cavity_3d_desc(N, N, N, 0.05, 0.05, storage="f32s")
+    with kmi.Lattice(desc, backend) as lat:
+        lat.step(30)
+        ref = lat.download_raw()
+    for tile, zc in ((12, 6), (14, 3), (9, 2)):
+        print(f"[walled] cube 256^3 f32s tile {tile} z-ranges {zc}", flush=True)
+        with kmi.Lattice(desc, backend) as lat:
+            lat.set_temporal(tile, zc)
+            lat.step(30)
+            got = lat.download_raw()
+        assert np.array_equal(got, ref), (tile, zc)
+
+
 def test_walled_cavity_autotune_picks_a_multi_step_launch(kmi, backend):
     """run_cavity_3d's lattice at 128³ (FP32 shifted): the autotuner now offers the walled
     multi-step launches, and its choice reproduces single-step launches bit for bit."""
