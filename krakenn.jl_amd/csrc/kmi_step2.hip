@@ -416,8 +416,8 @@ __global__ void __launch_bounds__(NT)
     if (t < 0) return;  // padding groups of the table
   }
   // padding groups (uniform per group, before any barrier); the periodic kernels' prologue
-  // is kept to this one test: a longer one shifts the plane loop's code alignment, which
-  // cost the 64×8 / 832-thread three-step launch 3 %
+  // is kept to this one test: a longer one moved the plane loop and cost the 64×8 /
+  // 832-thread three-step launch 3 % (measured before loop heads were 64-byte aligned)
   if (t >= ntiles) return;
   const int tx = t % ntx, ty = (t / ntx) % nty, zc = t / (ntx * nty);
   if constexpr (BND >= 2) {
@@ -716,8 +716,8 @@ __global__ void __launch_bounds__(NT)
     if (t < 0) return;  // padding groups of the table
   }
   // padding groups (uniform per group, before any barrier); the periodic kernels' prologue
-  // is kept to this one test: a longer one shifts the plane loop's code alignment, which
-  // cost the 64×8 / 832-thread three-step launch 3 %
+  // is kept to this one test: a longer one moved the plane loop and cost the 64×8 /
+  // 832-thread three-step launch 3 % (measured before loop heads were 64-byte aligned)
   if (t >= ntiles) return;
   const int tx = t % ntx, ty = (t / ntx) % nty, zc = t / (ntx * nty);
   if constexpr (BND >= 2) {
