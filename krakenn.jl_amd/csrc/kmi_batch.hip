@@ -244,7 +244,7 @@ __global__ void __launch_bounds__(1024) k_batch_stepK(const BatchArgs a) {
 }
 
 // Work-items of a K-step region for tile width tw (0: the region exceeds 1024).
-KMI_HD inline int stepK_region(int tw, int K) {
+KMI_HD int stepK_region(int tw, int K) {
   const int r = (tw + 2 * K) * (256 / tw + 2 * K);
   return r <= 1024 ? r : 0;
 }
