@@ -54,7 +54,7 @@ def test_config1_cavity_fp64_full_run_bitwise(kmi, po, backend):
             lat.set_temporal(tile, 1)
             lat.step(steps)
             assert np.array_equal(lat.download(), fr), tile
-    # the Ladd-lid variant (BASELINE's "moving-lid bounce-back") against its oracle chain
+    # the Ladd-lid variant (moving-lid bounce-back) against its oracle chain
     d2 = kmi.cavity_desc(N, nu, ul, lid="ladd")
     with kmi.Lattice(d2, backend) as lat:
         lat.step(2000)
